@@ -1,0 +1,163 @@
+/*
+ * lucent_hip.h — C ABI of the MI355X (gfx950) implementation of Lucent's
+ * path-tracing hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns a lucent_status: 0 on success, a negative code on failure with a
+ * message in lucent_last_error().  Host arrays are borrowed for the duration
+ * of the call only.  Entry points suffixed _device take device pointers and
+ * enqueue on the given HIP stream (NULL = the context's own stream) without
+ * synchronising.  A context is not re-entrant: calls on one context must not
+ * overlap.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * <file>:<line> of clark719664/lucent.
+ */
+#ifndef LUCENT_HIP_H
+#define LUCENT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUCENT_ABI_VERSION 1
+
+typedef int32_t lucent_status;
+enum {
+  LUCENT_OK = 0,
+  LUCENT_E_INVALID = -1,     /* bad argument; mirrors the reference's thrown Error */
+  LUCENT_E_HIP = -2,         /* a HIP runtime call failed */
+  LUCENT_E_NO_DEVICE = -3,   /* no usable gfx950 device */
+  LUCENT_E_OUT_OF_MEMORY = -4,
+  LUCENT_E_UNSUPPORTED = -5, /* input outside what the kernels handle (e.g. tree too deep) */
+  LUCENT_E_OVERFLOW = -6     /* a device-side bound was hit; results are not trustworthy */
+};
+
+typedef struct lucent_ctx lucent_ctx;
+typedef struct lucent_scene lucent_scene;
+
+/* Version string of the library build. */
+const char *lucent_version(void);
+
+/* Message of the last failed call on the calling thread ("" if none).
+ * ctx may be NULL. */
+const char *lucent_last_error(const lucent_ctx *ctx);
+
+/* Opens device `device` and creates the context's stream.
+ * Replaces the WebGL2 context set-up of Renderer's constructor
+ * (src/renderer.ts:226; src/gl/context.ts:18-29, which throws GLError). */
+lucent_status lucent_create(int device, lucent_ctx **out);
+void lucent_destroy(lucent_ctx *ctx);
+
+/* Waits for all work enqueued on the context's stream. */
+lucent_status lucent_synchronize(lucent_ctx *ctx);
+
+/* Number of compute units of the context's device (for launch sizing). */
+lucent_status lucent_device_info(lucent_ctx *ctx, int32_t *compute_units, int64_t *global_mem_bytes);
+
+/* ------------------------------------------------------------------------
+ * BVH build — buildBVH(positions, {maxLeafSize, bins}) (src/math/bvh.ts:52).
+ * positions: 9 float32 per triangle.  nodes_out must hold
+ * max(1, 2*tri_count-1)*8 floats in the layout of src/math/bvh.ts:6-16
+ * [min.xyz, leftFirst, max.xyz, count]; tri_indices_out tri_count entries;
+ * bounds6_out (min.xyz, max.xyz) may be NULL.  Host-only: needs no device.
+ * ---------------------------------------------------------------------- */
+lucent_status lucent_build_bvh(const float *positions, uint32_t tri_count, uint32_t max_leaf, uint32_t bins,
+                               float *nodes_out, uint32_t *node_count_out, uint32_t *tri_indices_out,
+                               double *bounds6_out);
+
+/* ------------------------------------------------------------------------
+ * Scene upload — PackedScene (src/scene/scene.ts:106-134) as uploaded by
+ * Renderer.loadScene (src/renderer.ts:279-312).  Arrays use the reference's
+ * texel layouts verbatim:
+ *   tri_geometry   12 f32/tri: (v0, material) (v1, light) (v2, alpha flag)  scene.ts:213-235
+ *   tri_attributes 16 f32/tri: (n0,u0)(n1,u1)(n2,u2)(v0,v1,v2,0); may be NULL
+ *   bvh_nodes       8 f32/node, bvh.ts:6-16
+ *   tri_indices    slot -> original triangle id (BVH.triIndices); NULL = identity
+ *   materials      40 f32/material, material.ts:117-131
+ *   spheres        4 f32/sphere (centre, radius); sphere_materials, sphere_lights int32
+ *   lights         16 f32/light, scene.ts:88-95
+ * Integer-valued float fields are converted to integer fields on the device.
+ * ---------------------------------------------------------------------- */
+typedef struct lucent_packed_scene {
+  const float *tri_geometry;
+  const float *tri_attributes;
+  uint32_t tri_count;
+  const float *bvh_nodes;
+  uint32_t bvh_node_count;
+  const uint32_t *tri_indices;
+  const float *materials;
+  uint32_t material_count;
+  const float *spheres;
+  const int32_t *sphere_materials;
+  const int32_t *sphere_lights;
+  uint32_t sphere_count;
+  const float *lights;
+  uint32_t light_count;
+  int32_t ground_material; /* -1 = no ground plane (uGroundMaterial) */
+  float ground_height;
+} lucent_packed_scene;
+
+lucent_status lucent_scene_upload(lucent_ctx *ctx, const lucent_packed_scene *packed, lucent_scene **out);
+
+/* Convenience form for traversal only: the BVH of buildBVH plus the positions
+ * it was built over (the arguments of intersectBVH, src/math/bvh.ts:265-271);
+ * triangles are gathered into leaf order on upload, as packScene does. */
+lucent_status lucent_scene_from_bvh(lucent_ctx *ctx, const float *positions, uint32_t tri_count,
+                                    const float *bvh_nodes, uint32_t bvh_node_count,
+                                    const uint32_t *tri_indices, lucent_scene **out);
+
+void lucent_scene_destroy(lucent_scene *scene);
+
+typedef struct lucent_scene_stats {
+  uint32_t tri_count;
+  uint32_t node_count;  /* reference nodes */
+  uint32_t pair_count;  /* 64-byte child-pair records on the device */
+  uint32_t depth;       /* deepest leaf, root = 0 */
+  uint64_t device_bytes;
+} lucent_scene_stats;
+lucent_status lucent_scene_stats_get(const lucent_scene *scene, lucent_scene_stats *out);
+
+/* ------------------------------------------------------------------------
+ * Batched ray queries — intersectBVH (src/math/bvh.ts:265-320), called per
+ * ray by pickDistance (src/scene/scene.ts:327-341).
+ *
+ * flags: LUCENT_PREC_EXACT_FP64 reproduces the CPU traversal's hit records
+ * bit for bit (FP64 Möller–Trumbore in the op order of src/math/ray.ts:22-45,
+ * ties resolved to the lowest leaf slot as its left-first traversal does);
+ * LUCENT_PREC_GLSL_FP32 follows the shader's float32 traversal
+ * (src/shaders/scene.glsl:221-238 hitTriangle, EPS = 1e-6 replaces tmin).
+ * LUCENT_QUERY_ANY returns any accepted hit (occlusion, scene.glsl:444-485)
+ * with t/u/v of that hit.
+ * A miss writes tri_id = 0xFFFFFFFF, t = +inf, u = v = 0 (the reference's null).
+ * ---------------------------------------------------------------------- */
+#define LUCENT_QUERY_CLOSEST 0u
+#define LUCENT_QUERY_ANY 1u
+#define LUCENT_PREC_EXACT_FP64 0u
+#define LUCENT_PREC_GLSL_FP32 2u
+#define LUCENT_MISS 0xFFFFFFFFu
+/* Testing aid: keeps only 2 stack entries per lane in LDS so that the global
+ * spill path of the traversal stack runs on any tree (results unchanged). */
+#define LUCENT_FLAG_TINY_LDS_STACK 0x100u
+
+lucent_status lucent_intersect(lucent_ctx *ctx, const lucent_scene *scene, const float *org3, const float *dir3,
+                               uint32_t n, double tmin, double tmax, uint32_t flags, double *t_out,
+                               uint32_t *tri_id_out, double *u_out, double *v_out);
+
+/* Same with double-precision rays (JavaScript numbers, as pickFocus builds them). */
+lucent_status lucent_intersect_f64(lucent_ctx *ctx, const lucent_scene *scene, const double *org3,
+                                   const double *dir3, uint32_t n, double tmin, double tmax, uint32_t flags,
+                                   double *t_out, uint32_t *tri_id_out, double *u_out, double *v_out);
+
+/* Device-pointer form of lucent_intersect; asynchronous on `stream`. */
+lucent_status lucent_intersect_device(lucent_ctx *ctx, const lucent_scene *scene, const float *d_org3,
+                                      const float *d_dir3, uint32_t n, double tmin, double tmax, uint32_t flags,
+                                      double *d_t, uint32_t *d_tri_id, double *d_u, double *d_v, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LUCENT_HIP_H */

@@ -1,0 +1,114 @@
+"""ctypes binding of ``liblucent_hip.so`` (the C ABI in include/lucent_hip.h).
+
+The library must have been built (``python -m lucent_amd.build``); there is
+no fallback of any kind — importing a compute entry point without the
+library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "liblucent_hip.so"
+
+LUCENT_OK = 0
+LUCENT_QUERY_CLOSEST = 0
+LUCENT_QUERY_ANY = 1
+LUCENT_PREC_EXACT_FP64 = 0
+LUCENT_PREC_GLSL_FP32 = 2
+LUCENT_MISS = 0xFFFFFFFF
+
+STATUS_NAMES = {
+    -1: "LUCENT_E_INVALID",
+    -2: "LUCENT_E_HIP",
+    -3: "LUCENT_E_NO_DEVICE",
+    -4: "LUCENT_E_OUT_OF_MEMORY",
+    -5: "LUCENT_E_UNSUPPORTED",
+    -6: "LUCENT_E_OVERFLOW",
+}
+
+
+class LucentError(RuntimeError):
+    """A failed C-ABI call; carries the status code and lucent_last_error()."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class PackedSceneStruct(C.Structure):
+    _fields_ = [
+        ("tri_geometry", C.c_void_p),
+        ("tri_attributes", C.c_void_p),
+        ("tri_count", C.c_uint32),
+        ("bvh_nodes", C.c_void_p),
+        ("bvh_node_count", C.c_uint32),
+        ("tri_indices", C.c_void_p),
+        ("materials", C.c_void_p),
+        ("material_count", C.c_uint32),
+        ("spheres", C.c_void_p),
+        ("sphere_materials", C.c_void_p),
+        ("sphere_lights", C.c_void_p),
+        ("sphere_count", C.c_uint32),
+        ("lights", C.c_void_p),
+        ("light_count", C.c_uint32),
+        ("ground_material", C.c_int32),
+        ("ground_height", C.c_float),
+    ]
+
+
+class SceneStatsStruct(C.Structure):
+    _fields_ = [
+        ("tri_count", C.c_uint32),
+        ("node_count", C.c_uint32),
+        ("pair_count", C.c_uint32),
+        ("depth", C.c_uint32),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+# name -> (restype, argtypes)
+_P = C.c_void_p
+_U32 = C.c_uint32
+_I32 = C.c_int32
+_D = C.c_double
+SIGNATURES: dict[str, tuple] = {
+    "lucent_version": (C.c_char_p, []),
+    "lucent_last_error": (C.c_char_p, [_P]),
+    "lucent_create": (_I32, [C.c_int, C.POINTER(_P)]),
+    "lucent_destroy": (None, [_P]),
+    "lucent_synchronize": (_I32, [_P]),
+    "lucent_device_info": (_I32, [_P, C.POINTER(_I32), C.POINTER(C.c_int64)]),
+    "lucent_build_bvh": (_I32, [_P, _U32, _U32, _U32, _P, C.POINTER(_U32), _P, _P]),
+    "lucent_scene_upload": (_I32, [_P, C.POINTER(PackedSceneStruct), C.POINTER(_P)]),
+    "lucent_scene_from_bvh": (_I32, [_P, _P, _U32, _P, _U32, _P, C.POINTER(_P)]),
+    "lucent_scene_destroy": (None, [_P]),
+    "lucent_scene_stats_get": (_I32, [_P, C.POINTER(SceneStatsStruct)]),
+    "lucent_intersect": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P]),
+    "lucent_intersect_f64": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P]),
+    "lucent_intersect_device": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads the library once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise LucentError(-3, f"{LIB_PATH} is missing: run `python -m lucent_amd.build` first")
+        l = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | C.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != LUCENT_OK:
+        msg = lib().lucent_last_error(None)
+        raise LucentError(status, msg.decode() if msg else "")

@@ -1,0 +1,110 @@
+// Internal declarations shared by the host C++ and the HIP translation units.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lucent_hip.h"
+
+namespace lucent {
+
+using Status = lucent_status;
+
+// Records `msg` as the calling thread's last error and returns `code`.
+Status fail(Status code, const std::string& msg);
+// Same for a failed HIP call; `what` names the call.
+Status fail_hip(int hip_err, const char* what);
+void clear_error();
+
+Status build_bvh(const float* positions, uint32_t tri_count, uint32_t max_leaf, uint32_t bins, float* nodes_out,
+                 uint32_t* node_count, uint32_t* tri_indices_out, double bounds[6]);
+
+// ---------------------------------------------------------------------------
+// Device scene layout (all device pointers; plain types so that host C++
+// compiled without HIP can fill it in).
+//
+//   pairs    : one 64-byte record per interior node, describing its two
+//              children: three float4 of bounds, one per axis,
+//              (lo_child0, lo_child1, hi_child0, hi_child1), then a uint4
+//              (link0, count0, link1, count1).  count > 0: leaf holding
+//              triangles [link, link+count) in leaf order; count == 0:
+//              interior, link = the child's own pair record.
+//   tris     : three float4 per leaf slot: (v0, material) (v1, light)
+//              (v2, alpha flag), the w fields as int32 bits.
+//   attrs    : four float4 per slot, the reference ATTR texels verbatim.
+//   tri_ids  : slot -> original triangle id.
+// ---------------------------------------------------------------------------
+struct DevBvh {
+  const float* pairs = nullptr;   // float4[4 * pair_count]
+  const float* tris = nullptr;    // float4[3 * tri_count]
+  const float* attrs = nullptr;   // float4[4 * tri_count] or null
+  const uint32_t* tri_ids = nullptr;
+  uint32_t tri_count = 0;
+  uint32_t pair_count = 0;
+  uint32_t root_link = 0;   // pair index, or first slot when the root is a leaf
+  uint32_t root_count = 0;  // > 0 when the root itself is a leaf
+  uint32_t stack_depth = 0; // entries a traversal stack may need
+  uint32_t pad = 0;
+};
+
+struct Context {
+  int device = 0;
+  void* stream = nullptr;  // hipStream_t
+  int compute_units = 0;
+  int64_t global_mem = 0;
+  // Scratch for host-pointer entry points, grown on demand.
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // Spill area for traversal stacks that outgrow their LDS part.
+  void* spill = nullptr;
+  size_t spill_bytes = 0;
+  uint32_t* d_flags = nullptr;  // device status word(s)
+};
+
+struct Scene {
+  Context* ctx = nullptr;
+  DevBvh bvh;
+  std::vector<void*> allocations;
+  lucent_scene_stats stats{};
+  // Host copies of the small per-scene tables.
+  std::vector<float> materials, spheres, lights;
+  std::vector<int32_t> sphere_materials, sphere_lights;
+  int32_t ground_material = -1;
+  float ground_height = 0;
+};
+
+// Host-side conversion of the reference layout into the device layout.
+struct HostBvhImage {
+  std::vector<float> pairs;  // 16 floats per pair (links as uint32 bits)
+  uint32_t pair_count = 0;
+  uint32_t root_link = 0, root_count = 0;
+  uint32_t depth = 0;
+};
+Status convert_bvh(const float* nodes, uint32_t node_count, uint32_t tri_count, HostBvhImage* out);
+
+// Device memory helpers (implemented over the HIP runtime).
+Status dev_alloc(void** p, size_t bytes);
+Status dev_upload(void* dst, const void* src, size_t bytes, void* stream);
+Status dev_download(void* dst, const void* src, size_t bytes, void* stream);
+Status stream_sync(void* stream);
+Status ensure_scratch(Context* ctx, size_t bytes);
+Status ensure_spill(Context* ctx, size_t bytes);
+
+// Kernel launchers (traverse.hip).
+struct RayQuery {
+  const void* org;  // float or double xyz per ray
+  const void* dir;
+  bool f64_rays;
+  uint32_t n;
+  double tmin, tmax;
+  uint32_t flags;
+  double* t;
+  uint32_t* id;
+  double* u;
+  double* v;
+};
+Status launch_intersect(Context* ctx, const DevBvh& bvh, const RayQuery& q, void* stream);
+
+}  // namespace lucent

@@ -1,0 +1,75 @@
+"""The product's C++ SAH builder against the oracle's independent restatement
+of buildBVH (src/math/bvh.ts:52-252): bit-identical nodes and leaf order."""
+import numpy as np
+import pytest
+
+import oracle as O
+from lucent_amd import workloads
+from lucent_amd.bvh import build_bvh
+
+
+def _same(a, b):
+    assert a.node_count == b.node_count
+    assert np.array_equal(a.nodes.view(np.uint32), b.nodes.view(np.uint32))
+    assert np.array_equal(a.tri_indices, b.tri_indices)
+
+
+@pytest.mark.parametrize("count,seed", [(0, 1), (1, 1), (5, 2), (300, 3), (500, 7), (2000, 42), (5000, 11)])
+def test_builder_bit_identical_on_soups(count, seed):
+    soup = O.random_soup(count, O.Lcg(seed))
+    _same(build_bvh(soup), O.build_bvh(soup))
+
+
+@pytest.mark.parametrize("leaf,bins", [(1, 2), (2, 8), (4, 12), (8, 32)])
+def test_builder_options(leaf, bins):
+    soup = O.random_soup(1500, O.Lcg(9))
+    _same(build_bvh(soup, leaf, bins), O.build_bvh(soup, leaf, bins))
+
+
+def test_builder_bit_identical_torus_knot_and_degenerate():
+    pos, _ = O.torus_knot()
+    _same(build_bvh(pos), O.build_bvh(pos))
+    one = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0] * 64, np.float32)
+    _same(build_bvh(one), O.build_bvh(one))
+    # Negative zeros and flat (axis-aligned) triangles.
+    flat = np.array([[-0.0, 0, 0, 1, 0, 0, 0, 0, 1], [0, -0.0, 0, 0, 0, -1, -1, 0, 0]] * 40, np.float32).reshape(-1)
+    _same(build_bvh(flat), O.build_bvh(flat))
+
+
+def test_builder_bit_identical_icosphere():
+    pos = workloads.icosphere_fbm(4, 1.0, 0.1, 1)
+    _same(build_bvh(pos), O.build_bvh(pos))
+
+
+def test_builder_rejects_bad_length():
+    with pytest.raises(ValueError):
+        build_bvh(np.zeros(10, np.float32))
+
+
+def test_icosphere_counts_and_closure():
+    v, f = workloads.icosphere(3)
+    assert f.shape[0] == 20 * 4 ** 3
+    # Closed: every undirected edge used by exactly two faces; Euler V - E + F = 2.
+    e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), axis=1)
+    uniq, cnt = np.unique(e, axis=0, return_counts=True)
+    assert (cnt == 2).all()
+    assert v.shape[0] - uniq.shape[0] + f.shape[0] == 2
+    assert np.allclose(np.linalg.norm(v, axis=1), 1)
+
+
+def test_fbm_range_and_determinism():
+    p = np.random.default_rng(0).standard_normal((5000, 3))
+    a = workloads.fbm3(p, 1)
+    assert np.array_equal(a, workloads.fbm3(p, 1))
+    assert np.abs(a).max() <= 1.0 and a.std() > 0.05
+    assert not np.array_equal(a, workloads.fbm3(p, 2))
+
+
+def test_sphere_rays_geometry():
+    o, d = workloads.sphere_rays(4096, 5)
+    assert np.allclose(np.linalg.norm(o.astype(np.float64), axis=1), 3, atol=1e-5)
+    assert np.allclose(np.linalg.norm(d.astype(np.float64), axis=1), 1, atol=1e-6)
+    # Every ray passes within distance 1 of the origin (aimed into the unit ball).
+    oo, dd = o.astype(np.float64), d.astype(np.float64)
+    closest = oo - (oo * dd).sum(1, keepdims=True) * dd
+    assert (np.linalg.norm(closest, axis=1) <= 1 + 1e-5).all()
