@@ -157,6 +157,106 @@ lucent_status lucent_intersect_device(lucent_ctx *ctx, const lucent_scene *scene
                                       const float *d_dir3, uint32_t n, double tmin, double tmax, uint32_t flags,
                                       double *d_t, uint32_t *d_tri_id, double *d_u, double *d_v, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Lighting — the Scene's sun, sky and environment (src/scene/scene.ts:21-38,
+ * src/scene/environment.ts:12-22) as uploaded by Renderer.uploadLighting
+ * (src/renderer.ts:355-413): the analytic sun is enabled only without an
+ * environment map; without a map the procedural sky is baked into a
+ * 1024x512 map (environment.ts:106-132); the map's importance sampler is
+ * buildEnvSampler (environment.ts:52-85).  Angles in degrees.
+ * ---------------------------------------------------------------------- */
+typedef struct lucent_sun {
+  double elevation, azimuth;
+  double color[3];
+  double intensity;
+  double angular_radius;
+} lucent_sun;
+
+typedef struct lucent_sky {
+  int32_t enabled;
+  double zenith[3], horizon[3], ground[3];
+  double intensity;
+} lucent_sky;
+
+typedef struct lucent_environment {
+  const float *map_rgb; /* NULL = procedural sky; else width*height*3 float32, rows top to bottom */
+  uint32_t width, height;
+  double rotation; /* about +Y, degrees */
+  double intensity;
+  int32_t background; /* camera rays that miss see the map (1) or the backdrop (0) */
+  double backdrop[3];
+} lucent_environment;
+
+typedef struct lucent_lighting {
+  lucent_sun sun;
+  lucent_sky sky;
+  lucent_environment env;
+} lucent_lighting;
+
+lucent_status lucent_scene_set_lighting(lucent_ctx *ctx, lucent_scene *scene, const lucent_lighting *lighting);
+
+/* buildEnvSampler(img, maxWidth) (environment.ts:52-85), host-only.  table_out
+ * must hold 4 floats per grid cell (at most min(width,max_width) x height). */
+lucent_status lucent_env_sampler(const float *rgb, uint32_t width, uint32_t height, uint32_t max_width,
+                                 float *table_out, uint32_t *grid_w, uint32_t *grid_h, double *total);
+
+/* buildAliasTable (src/math/alias.ts:17-64), host-only. */
+lucent_status lucent_alias_table(const double *weights, uint32_t n, float *prob_out, uint32_t *alias_out,
+                                 double *pmf_out, double *total_out);
+
+/* sheenAlbedoLUT(size, steps) (src/scene/sheen.ts:52-61), host-only, float64
+ * values (the device copy is rounded to half, as the R16F upload is). */
+lucent_status lucent_sheen_lut(uint32_t size, uint32_t steps, double *out);
+
+/* ------------------------------------------------------------------------
+ * Film — the accumulation targets of Renderer.resize (src/renderer.ts:476-496,
+ * formats at :490): colour RGBA32F (mean radiance, mean luminance^2), albedo
+ * RGBA16F (guide albedo, coverage), normal RGBA16F (guide normal, primary
+ * depth), history R32F (samples behind each pixel).  Rows bottom to top.
+ * The RNG stream index `seed` advances every sample and survives resets
+ * (renderer.ts:201-202, 728).
+ * ---------------------------------------------------------------------- */
+typedef struct lucent_film lucent_film;
+
+lucent_status lucent_film_create(lucent_ctx *ctx, uint32_t width, uint32_t height, lucent_film **out);
+void lucent_film_destroy(lucent_film *film);
+/* Renderer.reset (renderer.ts:463-474): zero the averages and the history. */
+lucent_status lucent_film_reset(lucent_ctx *ctx, lucent_film *film);
+lucent_status lucent_film_set_seed(lucent_film *film, uint32_t seed);
+lucent_status lucent_film_info(const lucent_film *film, uint32_t *width, uint32_t *height, uint32_t *samples,
+                               uint32_t *seed);
+/* Copies the targets to host memory; the half targets are widened to float.
+ * Any pointer may be NULL. */
+lucent_status lucent_film_read(lucent_ctx *ctx, const lucent_film *film, float *color4, float *albedo4,
+                               float *normal4, float *history);
+/* Device addresses of the four targets (for zero-copy consumers). */
+lucent_status lucent_film_device_ptrs(const lucent_film *film, void **color4, void **albedo_half4,
+                                      void **normal_half4, void **history);
+
+/* ------------------------------------------------------------------------
+ * Progressive tracing — Renderer.traceSamples (src/renderer.ts:702-737): each
+ * of `count` samples is one launch over the film with uSeed = seed++, adding
+ * one path per pixel to the running means (trace.frag:324-355).
+ * camera: Camera (scene.ts:10-19), basis as cameraBasis (renderer.ts:156-161).
+ * settings: the integrator part of RenderSettings (renderer.ts:42-66).
+ * ---------------------------------------------------------------------- */
+typedef struct lucent_camera {
+  double position[3];
+  double target[3];
+  double fov; /* vertical, degrees */
+  double aperture;
+  double focus_distance;
+} lucent_camera;
+
+typedef struct lucent_render_settings {
+  uint32_t max_bounces;
+  float firefly_clamp; /* 0 disables (uClamp) */
+} lucent_render_settings;
+
+lucent_status lucent_trace_samples(lucent_ctx *ctx, lucent_scene *scene, lucent_film *film,
+                                   const lucent_camera *camera, const lucent_render_settings *settings,
+                                   uint32_t count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,34 @@ class SceneStatsStruct(C.Structure):
     ]
 
 
+class SunStruct(C.Structure):
+    _fields_ = [("elevation", C.c_double), ("azimuth", C.c_double), ("color", C.c_double * 3),
+                ("intensity", C.c_double), ("angular_radius", C.c_double)]
+
+
+class SkyStruct(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("zenith", C.c_double * 3), ("horizon", C.c_double * 3),
+                ("ground", C.c_double * 3), ("intensity", C.c_double)]
+
+
+class EnvironmentStruct(C.Structure):
+    _fields_ = [("map_rgb", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("rotation", C.c_double),
+                ("intensity", C.c_double), ("background", C.c_int32), ("backdrop", C.c_double * 3)]
+
+
+class LightingStruct(C.Structure):
+    _fields_ = [("sun", SunStruct), ("sky", SkyStruct), ("env", EnvironmentStruct)]
+
+
+class CameraStruct(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("target", C.c_double * 3), ("fov", C.c_double),
+                ("aperture", C.c_double), ("focus_distance", C.c_double)]
+
+
+class RenderSettingsStruct(C.Structure):
+    _fields_ = [("max_bounces", C.c_uint32), ("firefly_clamp", C.c_float)]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _U32 = C.c_uint32
@@ -88,6 +116,18 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_intersect": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P]),
     "lucent_intersect_f64": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P]),
     "lucent_intersect_device": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P, _P]),
+    "lucent_scene_set_lighting": (_I32, [_P, _P, C.POINTER(LightingStruct)]),
+    "lucent_env_sampler": (_I32, [_P, _U32, _U32, _U32, _P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_D)]),
+    "lucent_alias_table": (_I32, [_P, _U32, _P, _P, _P, C.POINTER(_D)]),
+    "lucent_sheen_lut": (_I32, [_U32, _U32, _P]),
+    "lucent_film_create": (_I32, [_P, _U32, _U32, C.POINTER(_P)]),
+    "lucent_film_destroy": (None, [_P]),
+    "lucent_film_reset": (_I32, [_P, _P]),
+    "lucent_film_set_seed": (_I32, [_P, _U32]),
+    "lucent_film_info": (_I32, [_P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
+    "lucent_film_read": (_I32, [_P, _P, _P, _P, _P, _P]),
+    "lucent_film_device_ptrs": (_I32, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "lucent_trace_samples": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _P]),
 }
 
 _lib = None

@@ -33,7 +33,7 @@ HIPCC_FLAGS = [
 ]
 CXX_FLAGS = [
     "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-    "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", "-Wall",
+    "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", "-Wall", "-Wno-unknown-pragmas",
 ]
 
 

@@ -9,6 +9,8 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "render_host.hpp"
+#include "scene.hpp"
 
 namespace lucent {
 
@@ -96,6 +98,7 @@ void destroy_scene(Scene* sc) {
   if (!sc) return;
   if (sc->ctx) hipStreamSynchronize(static_cast<hipStream_t>(sc->ctx->stream));
   for (void* p : sc->allocations) hipFree(p);
+  for (void* p : sc->env_allocations) hipFree(p);
   delete sc;
 }
 
@@ -132,6 +135,7 @@ Status finish_scene(Context* ctx, Scene* sc, const float* nodes, uint32_t node_c
   const float* idp = nullptr;
   if ((s = upload_owned(sc, ids.data(), size_t(n) * sizeof(uint32_t), &idp)) != LUCENT_OK) return s;
   sc->bvh.tri_ids = reinterpret_cast<const uint32_t*>(idp);
+  sc->pt.bvh = sc->bvh;
   return stream_sync(ctx->stream);
 }
 
@@ -142,8 +146,6 @@ int32_t float_to_index(float f) { return int32_t(f); }
 
 using namespace lucent;
 
-struct lucent_ctx : Context {};
-struct lucent_scene : Scene {};
 
 extern "C" {
 
@@ -288,16 +290,50 @@ lucent_status lucent_scene_upload(lucent_ctx* ctx, const lucent_packed_scene* ps
       destroy_scene(sc);
       return fail(LUCENT_E_INVALID, "triangle geometry contains NaN or infinity");
     }
-  sc->materials.assign(ps->materials, ps->materials + size_t(ps->material_count) * 40);
-  if (ps->sphere_count) {
-    sc->spheres.assign(ps->spheres, ps->spheres + size_t(ps->sphere_count) * 4);
-    sc->sphere_materials.assign(ps->sphere_materials, ps->sphere_materials + ps->sphere_count);
-    sc->sphere_lights.assign(ps->sphere_lights, ps->sphere_lights + ps->sphere_count);
+  // Integrator tables: materials, spheres, lights, ground (packScene outputs).
+  PtScene& pt = sc->pt;
+  Status s = LUCENT_OK;
+  if (ps->material_count) {
+    const float* m = nullptr;
+    if ((s = upload_owned(sc, ps->materials, size_t(ps->material_count) * 40 * sizeof(float), &m)) != LUCENT_OK) {
+      destroy_scene(sc);
+      return s;
+    }
+    pt.materials = m;
   }
-  if (ps->light_count) sc->lights.assign(ps->lights, ps->lights + size_t(ps->light_count) * 16);
-  sc->ground_material = ps->ground_material;
-  sc->ground_height = ps->ground_height;
-  Status s = finish_scene(ctx, sc, ps->bvh_nodes, ps->bvh_node_count, ps->tri_indices, tris, ps->tri_attributes);
+  pt.material_count = ps->material_count;
+  if (ps->sphere_count) {
+    if (!ps->spheres || !ps->sphere_materials || !ps->sphere_lights) {
+      destroy_scene(sc);
+      return fail(LUCENT_E_INVALID, "sphere arrays are null");
+    }
+    for (uint32_t i = 0; i < ps->sphere_count; ++i)
+      if (ps->sphere_materials[i] < 0 || uint32_t(ps->sphere_materials[i]) >= ps->material_count ||
+          ps->sphere_lights[i] >= int32_t(ps->light_count)) {
+        destroy_scene(sc);
+        return fail(LUCENT_E_INVALID, "sphere material or light index out of range");
+      }
+    const float *sp = nullptr, *sm = nullptr, *sl = nullptr;
+    if ((s = upload_owned(sc, ps->spheres, size_t(ps->sphere_count) * 16, &sp)) != LUCENT_OK ||
+        (s = upload_owned(sc, ps->sphere_materials, size_t(ps->sphere_count) * 4, &sm)) != LUCENT_OK ||
+        (s = upload_owned(sc, ps->sphere_lights, size_t(ps->sphere_count) * 4, &sl)) != LUCENT_OK) {
+      destroy_scene(sc);
+      return s;
+    }
+    pt.spheres = sp;
+    pt.sphere_material = reinterpret_cast<const int32_t*>(sm);
+    pt.sphere_light = reinterpret_cast<const int32_t*>(sl);
+  }
+  pt.sphere_count = ps->sphere_count;
+  if (ps->light_count) std::memcpy(pt.lights, ps->lights, size_t(ps->light_count) * 16 * sizeof(float));
+  pt.light_count = ps->light_count;
+  if (ps->ground_material >= 0 && uint32_t(ps->ground_material) >= ps->material_count) {
+    destroy_scene(sc);
+    return fail(LUCENT_E_INVALID, "ground material index out of range");
+  }
+  pt.ground_material = ps->ground_material;
+  pt.ground_height = ps->ground_height;
+  s = finish_scene(ctx, sc, ps->bvh_nodes, ps->bvh_node_count, ps->tri_indices, tris, ps->tri_attributes);
   if (s != LUCENT_OK) {
     destroy_scene(sc);
     return s;

@@ -1,0 +1,297 @@
+// C-ABI entry points of the rendering path: lighting upload, lighting tables,
+// the accumulation film and progressive tracing.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "render_host.hpp"
+#include "scene.hpp"
+
+using namespace lucent;
+
+struct lucent_film {
+  Context* ctx = nullptr;
+  uint32_t width = 0, height = 0;
+  uint32_t samples = 0;  // Renderer.frame
+  uint32_t seed = 0;     // Renderer.seed
+  void* base = nullptr;
+  PtFilm f;
+};
+
+namespace {
+
+Status ensure_sheen(Context* ctx) {
+  if (ctx->sheen_lut) return LUCENT_OK;
+  std::vector<float> lut(kSheenLutSize * kSheenLutSize);
+  sheen_lut(kSheenLutSize, 96, lut.data());
+  for (float& v : lut) v = round_to_half(v);
+  void* p = nullptr;
+  Status s = dev_alloc(&p, lut.size() * sizeof(float));
+  if (s != LUCENT_OK) return s;
+  if ((s = dev_upload(p, lut.data(), lut.size() * sizeof(float), ctx->stream)) != LUCENT_OK) return s;
+  if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+  ctx->sheen_lut = static_cast<float*>(p);
+  return LUCENT_OK;
+}
+
+Status upload_env(Scene* sc, const void* src, size_t bytes, const float** dst) {
+  void* p = nullptr;
+  Status s = dev_alloc(&p, bytes);
+  if (s != LUCENT_OK) return s;
+  sc->env_allocations.push_back(p);
+  if ((s = dev_upload(p, src, bytes, sc->ctx->stream)) != LUCENT_OK) return s;
+  *dst = static_cast<const float*>(p);
+  return LUCENT_OK;
+}
+
+double deg(double d) { return d * M_PI / 180; }
+
+}  // namespace
+
+extern "C" {
+
+lucent_status lucent_alias_table(const double* weights, uint32_t n, float* prob_out, uint32_t* alias_out,
+                                 double* pmf_out, double* total_out) {
+  clear_error();
+  if (n > 0 && (!weights || !prob_out || !alias_out || !pmf_out)) return fail(LUCENT_E_INVALID, "null array");
+  return build_alias_table(weights, n, prob_out, alias_out, pmf_out, total_out);
+}
+
+lucent_status lucent_env_sampler(const float* rgb, uint32_t width, uint32_t height, uint32_t max_width,
+                                 float* table_out, uint32_t* grid_w, uint32_t* grid_h, double* total) {
+  clear_error();
+  EnvSampler es;
+  Status s = build_env_sampler(rgb, width, height, max_width, &es);
+  if (s != LUCENT_OK) return s;
+  if (table_out) std::memcpy(table_out, es.table.data(), es.table.size() * sizeof(float));
+  if (grid_w) *grid_w = es.grid_w;
+  if (grid_h) *grid_h = es.grid_h;
+  if (total) *total = es.total;
+  return LUCENT_OK;
+}
+
+lucent_status lucent_sheen_lut(uint32_t size, uint32_t steps, double* out) {
+  clear_error();
+  if (size < 2 || steps < 2 || !out) return fail(LUCENT_E_INVALID, "size and steps must be >= 2");
+  for (uint32_t y = 0; y < size; ++y)
+    for (uint32_t x = 0; x < size; ++x)
+      out[y * size + x] = sheen_albedo(std::max(1e-3, double(x) / (size - 1)), double(y) / (size - 1), steps);
+  return LUCENT_OK;
+}
+
+lucent_status lucent_scene_set_lighting(lucent_ctx* ctx, lucent_scene* scene, const lucent_lighting* L) {
+  clear_error();
+  if (!ctx || !scene || !L) return fail(LUCENT_E_INVALID, "null ctx/scene/lighting");
+  Status s = ensure_sheen(ctx);
+  if (s != LUCENT_OK) return s;
+  PtScene& pt = scene->pt;
+  const lucent_sun& sun = L->sun;
+  const lucent_environment& env = L->env;
+  // renderer.ts:355-370: sun uniforms; the analytic sun only without a map.
+  const double el = deg(sun.elevation), az = deg(sun.azimuth);
+  const double c = std::cos(el);
+  const double dir[3] = {c * std::sin(az), std::sin(el), c * std::cos(az)};
+  const double cos_angle = std::cos(deg(sun.angular_radius));
+  const double solid = 2 * M_PI * (1 - cos_angle);
+  const bool has_map = env.map_rgb != nullptr;
+  pt.sun_enabled = (!has_map && sun.intensity > 0 && sun.elevation > -sun.angular_radius) ? 1 : 0;
+  for (int i = 0; i < 3; ++i) {
+    pt.sun_dir[i] = float(dir[i]);
+    pt.sun_radiance[i] = float(sun.color[i] * (sun.intensity / solid));
+  }
+  pt.sun_cos = float(cos_angle);
+  pt.sun_pdf = float(1 / solid);
+  // Environment map: the HDRI, or the procedural sky baked at 1024x512.
+  std::vector<float> baked;
+  const float* rgb = env.map_rgb;
+  uint32_t w = env.width, h = env.height;
+  if (!has_map) {
+    bake_sky(L->sky, sun, 1024, &baked);
+    rgb = baked.data();
+    w = 1024;
+    h = 512;
+  } else if (w == 0 || h == 0) {
+    return fail(LUCENT_E_INVALID, "environment map has zero size");
+  }
+  EnvSampler es;
+  if ((s = build_env_sampler(rgb, w, h, 1024, &es)) != LUCENT_OK) return s;
+  hipStreamSynchronize(static_cast<hipStream_t>(ctx->stream));
+  for (void* p : scene->env_allocations) hipFree(p);
+  scene->env_allocations.clear();
+  std::vector<float> map4(size_t(w) * h * 4);
+  for (size_t i = 0; i < size_t(w) * h; ++i) {
+    map4[i * 4] = rgb[i * 3];
+    map4[i * 4 + 1] = rgb[i * 3 + 1];
+    map4[i * 4 + 2] = rgb[i * 3 + 2];
+    map4[i * 4 + 3] = 0.0f;
+  }
+  const float *dm = nullptr, *dt = nullptr;
+  if ((s = upload_env(scene, map4.data(), map4.size() * sizeof(float), &dm)) != LUCENT_OK) return s;
+  if ((s = upload_env(scene, es.table.data(), es.table.size() * sizeof(float), &dt)) != LUCENT_OK) return s;
+  if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+  pt.env_map = dm;
+  pt.env_w = int32_t(w);
+  pt.env_h = int32_t(h);
+  pt.env_table = dt;
+  pt.grid_w = int32_t(es.grid_w);
+  pt.grid_h = int32_t(es.grid_h);
+  pt.env_sampling = es.total > 0 ? 1 : 0;
+  // renderer.ts:372-379: the baked sky is in world orientation at unit intensity.
+  const double rot = has_map ? deg(env.rotation) : 0.0;
+  pt.env_cos = float(std::cos(rot));
+  pt.env_sin = float(std::sin(rot));
+  pt.env_intensity = float(has_map ? env.intensity : 1.0);
+  pt.env_background = env.background ? 1 : 0;
+  for (int i = 0; i < 3; ++i) pt.backdrop[i] = float(env.backdrop[i]);
+  pt.sheen_lut = ctx->sheen_lut;
+  scene->lighting_set = true;
+  return LUCENT_OK;
+}
+
+lucent_status lucent_film_create(lucent_ctx* ctx, uint32_t width, uint32_t height, lucent_film** out) {
+  clear_error();
+  if (!ctx || !out) return fail(LUCENT_E_INVALID, "null ctx/out");
+  *out = nullptr;
+  if (width == 0 || height == 0 || uint64_t(width) * height > (1ull << 28))
+    return fail(LUCENT_E_INVALID, "film size must be positive and at most 2^28 pixels");
+  const size_t px = size_t(width) * height;
+  const size_t bytes = px * (16 + 8 + 8 + 4);
+  void* base = nullptr;
+  Status s = dev_alloc(&base, bytes);
+  if (s != LUCENT_OK) return s;
+  auto* f = new lucent_film();
+  f->ctx = ctx;
+  f->width = width;
+  f->height = height;
+  f->base = base;
+  char* b = static_cast<char*>(base);
+  f->f.color = reinterpret_cast<float*>(b);
+  f->f.albedo = reinterpret_cast<uint16_t*>(b + px * 16);
+  f->f.normal = reinterpret_cast<uint16_t*>(b + px * 24);
+  f->f.history = reinterpret_cast<float*>(b + px * 32);
+  if ((s = lucent_film_reset(ctx, f)) != LUCENT_OK) {
+    lucent_film_destroy(f);
+    return s;
+  }
+  *out = f;
+  return LUCENT_OK;
+}
+
+void lucent_film_destroy(lucent_film* film) {
+  if (!film) return;
+  hipStreamSynchronize(static_cast<hipStream_t>(film->ctx->stream));
+  hipFree(film->base);
+  delete film;
+}
+
+lucent_status lucent_film_reset(lucent_ctx* ctx, lucent_film* film) {
+  clear_error();
+  if (!ctx || !film) return fail(LUCENT_E_INVALID, "null ctx/film");
+  const size_t bytes = size_t(film->width) * film->height * 36;
+  const hipError_t e = hipMemsetAsync(film->base, 0, bytes, static_cast<hipStream_t>(ctx->stream));
+  if (e != hipSuccess) return fail_hip(e, "hipMemsetAsync(film)");
+  film->samples = 0;
+  return stream_sync(ctx->stream);
+}
+
+lucent_status lucent_film_set_seed(lucent_film* film, uint32_t seed) {
+  if (!film) return fail(LUCENT_E_INVALID, "null film");
+  film->seed = seed;
+  return LUCENT_OK;
+}
+
+lucent_status lucent_film_info(const lucent_film* film, uint32_t* w, uint32_t* h, uint32_t* samples, uint32_t* seed) {
+  if (!film) return fail(LUCENT_E_INVALID, "null film");
+  if (w) *w = film->width;
+  if (h) *h = film->height;
+  if (samples) *samples = film->samples;
+  if (seed) *seed = film->seed;
+  return LUCENT_OK;
+}
+
+static float half_to_float(uint16_t h) {
+  const uint32_t sign = uint32_t(h & 0x8000u) << 16;
+  const uint32_t exp = (h >> 10) & 0x1Fu, man = h & 0x3FFu;
+  float out;
+  if (exp == 0) {
+    out = std::ldexp(float(man), -24);
+    if (sign) out = -out;
+    return out;
+  }
+  uint32_t bits;
+  if (exp == 31)
+    bits = sign | 0x7F800000u | (man << 13);
+  else
+    bits = sign | ((exp + 112) << 23) | (man << 13);
+  std::memcpy(&out, &bits, 4);
+  return out;
+}
+
+lucent_status lucent_film_read(lucent_ctx* ctx, const lucent_film* film, float* color4, float* albedo4, float* normal4,
+                               float* history) {
+  clear_error();
+  if (!ctx || !film) return fail(LUCENT_E_INVALID, "null ctx/film");
+  const size_t px = size_t(film->width) * film->height;
+  std::vector<uint16_t> tmp;
+  Status s;
+  if (color4 && (s = dev_download(color4, film->f.color, px * 16, ctx->stream)) != LUCENT_OK) return s;
+  if (history && (s = dev_download(history, film->f.history, px * 4, ctx->stream)) != LUCENT_OK) return s;
+  for (int k = 0; k < 2; ++k) {
+    float* dst = k == 0 ? albedo4 : normal4;
+    if (!dst) continue;
+    tmp.resize(px * 4);
+    if ((s = dev_download(tmp.data(), k == 0 ? film->f.albedo : film->f.normal, px * 8, ctx->stream)) != LUCENT_OK)
+      return s;
+    if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+    for (size_t i = 0; i < px * 4; ++i) dst[i] = half_to_float(tmp[i]);
+  }
+  return stream_sync(ctx->stream);
+}
+
+lucent_status lucent_film_device_ptrs(const lucent_film* film, void** color4, void** albedo, void** normal,
+                                      void** history) {
+  if (!film) return fail(LUCENT_E_INVALID, "null film");
+  if (color4) *color4 = film->f.color;
+  if (albedo) *albedo = film->f.albedo;
+  if (normal) *normal = film->f.normal;
+  if (history) *history = film->f.history;
+  return LUCENT_OK;
+}
+
+lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_film* film, const lucent_camera* cam,
+                                   const lucent_render_settings* settings, uint32_t count, void* stream) {
+  clear_error();
+  if (!ctx || !scene || !film || !cam || !settings) return fail(LUCENT_E_INVALID, "null argument");
+  if (!scene->lighting_set) return fail(LUCENT_E_INVALID, "call lucent_scene_set_lighting before tracing");
+  if (scene->pt.material_count == 0 && scene->bvh.tri_count > 0)
+    return fail(LUCENT_E_INVALID, "scene has triangles but no materials (use lucent_scene_upload)");
+  PtFrame f;
+  f.width = film->width;
+  f.height = film->height;
+  f.max_bounces = settings->max_bounces;
+  f.clamp = settings->firefly_clamp;
+  double fwd[3], right[3], up[3], tan_half;
+  camera_basis(*cam, fwd, right, up, &tan_half);
+  for (int i = 0; i < 3; ++i) {
+    f.cam_pos[i] = float(cam->position[i]);
+    f.cam_fwd[i] = float(fwd[i]);
+    f.cam_right[i] = float(right[i]);
+    f.cam_up[i] = float(up[i]);
+  }
+  f.tan_half_fov = float(tan_half);
+  f.aperture = float(cam->aperture);
+  f.focus_dist = float(cam->focus_distance);
+  void* st = stream ? stream : ctx->stream;
+  for (uint32_t i = 0; i < count; ++i) {
+    f.seed = film->seed++;
+    Status s = launch_trace_sample(ctx, scene->pt, f, film->f, st);
+    if (s != LUCENT_OK) return s;
+    film->samples++;
+  }
+  return LUCENT_OK;
+}
+
+}  // extern "C"

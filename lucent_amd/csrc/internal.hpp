@@ -61,18 +61,7 @@ struct Context {
   void* spill = nullptr;
   size_t spill_bytes = 0;
   uint32_t* d_flags = nullptr;  // device status word(s)
-};
-
-struct Scene {
-  Context* ctx = nullptr;
-  DevBvh bvh;
-  std::vector<void*> allocations;
-  lucent_scene_stats stats{};
-  // Host copies of the small per-scene tables.
-  std::vector<float> materials, spheres, lights;
-  std::vector<int32_t> sphere_materials, sphere_lights;
-  int32_t ground_material = -1;
-  float ground_height = 0;
+  float* sheen_lut = nullptr;   // 16x16 half-rounded sheen albedo table
 };
 
 // Host-side conversion of the reference layout into the device layout.

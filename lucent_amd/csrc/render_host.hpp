@@ -1,0 +1,28 @@
+// Host-side rendering helpers (lighting tables, camera) and launchers.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "internal.hpp"
+#include "pt_types.hpp"
+
+namespace lucent {
+
+struct EnvSampler {
+  uint32_t grid_w = 0, grid_h = 0;
+  double total = 0;
+  std::vector<float> table;  // 4 floats per cell
+};
+
+Status build_alias_table(const double* w, uint32_t n, float* prob, uint32_t* alias, double* pmf, double* total);
+Status build_env_sampler(const float* rgb, uint32_t width, uint32_t height, uint32_t max_width, EnvSampler* out);
+void bake_sky(const lucent_sky& sky, const lucent_sun& sun, uint32_t width, std::vector<float>* out);
+double sheen_albedo(double cos_v, double roughness, uint32_t steps);
+void sheen_lut(uint32_t size, uint32_t steps, float* out);
+float round_to_half(float f);
+void camera_basis(const lucent_camera& cam, double fwd[3], double right[3], double up[3], double* tan_half_fov);
+
+Status launch_trace_sample(Context* ctx, const PtScene& sc, const PtFrame& f, const PtFilm& film, void* stream);
+
+}  // namespace lucent

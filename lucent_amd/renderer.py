@@ -1,0 +1,170 @@
+"""Progressive renderer — the host mirror of Renderer (src/renderer.ts:171-871)
+for the tracing path: load_scene / upload_lighting / resize (film) / reset /
+trace_samples / benchmark / read-back.  Display, preview scaling and the GPU
+budget scheduler are not part of this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+from .runtime import Context, Scene as DeviceScene
+from .scene import PackedScene, Scene, pack_scene
+
+
+@dataclass
+class RenderSettings:
+    """Integrator part of RenderSettings (renderer.ts:42-84); the reference's
+    default firefly clamp is 8, the benchmark configurations use 0 (unbiased)."""
+    max_bounces: int = 8
+    firefly_clamp: float = 8.0
+
+
+def lighting_struct(scene: Scene, keep: list) -> _native.LightingStruct:
+    L = _native.LightingStruct()
+    s = scene.sun
+    L.sun.elevation, L.sun.azimuth = s.elevation, s.azimuth
+    L.sun.color[:] = list(s.color)
+    L.sun.intensity, L.sun.angular_radius = s.intensity, s.angular_radius
+    k = scene.sky
+    L.sky.enabled = 1 if k.enabled else 0
+    L.sky.zenith[:], L.sky.horizon[:], L.sky.ground[:] = list(k.zenith), list(k.horizon), list(k.ground)
+    L.sky.intensity = k.intensity
+    e = scene.env
+    if e.map is not None:
+        w, h, data = e.map
+        arr = np.ascontiguousarray(np.asarray(data, np.float32).reshape(-1))
+        if arr.size != w * h * 3:
+            raise ValueError("environment map size mismatch")
+        keep.append(arr)
+        L.env.map_rgb, L.env.width, L.env.height = arr.ctypes.data, w, h
+    else:
+        L.env.map_rgb, L.env.width, L.env.height = None, 0, 0
+    L.env.rotation, L.env.intensity = e.rotation, e.intensity
+    L.env.background = 1 if e.background else 0
+    L.env.backdrop[:] = list(e.backdrop)
+    return L
+
+
+def camera_struct(cam) -> _native.CameraStruct:
+    c = _native.CameraStruct()
+    c.position[:] = list(cam.position)
+    c.target[:] = list(cam.target)
+    c.fov, c.aperture, c.focus_distance = cam.fov, cam.aperture, cam.focus_distance
+    return c
+
+
+class Film:
+    """The accumulation targets (renderer.ts:476-496); rows bottom to top."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _native.check(_native.lib().lucent_film_create(ctx.handle, width, height, C.byref(h)))
+        self._h = h
+        self.width, self.height = width, height
+
+    @property
+    def handle(self):
+        return self._h
+
+    def reset(self) -> None:
+        _native.check(_native.lib().lucent_film_reset(self.ctx.handle, self._h))
+
+    def set_seed(self, seed: int) -> None:
+        _native.check(_native.lib().lucent_film_set_seed(self._h, seed))
+
+    def info(self) -> dict:
+        w, h, n, s = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _native.check(_native.lib().lucent_film_info(self._h, C.byref(w), C.byref(h), C.byref(n), C.byref(s)))
+        return {"width": w.value, "height": h.value, "samples": n.value, "seed": s.value}
+
+    def read(self) -> dict:
+        n = self.width * self.height
+        color = np.empty((self.height, self.width, 4), np.float32)
+        albedo = np.empty((self.height, self.width, 4), np.float32)
+        normal = np.empty((self.height, self.width, 4), np.float32)
+        hist = np.empty((self.height, self.width), np.float32)
+        _native.check(_native.lib().lucent_film_read(self.ctx.handle, self._h, color.ctypes.data, albedo.ctypes.data,
+                                                     normal.ctypes.data, hist.ctypes.data))
+        assert color.size == n * 4
+        return {"color": color, "albedo": albedo, "normal": normal, "history": hist}
+
+    def device_ptrs(self) -> dict:
+        p = [C.c_void_p() for _ in range(4)]
+        _native.check(_native.lib().lucent_film_device_ptrs(self._h, *[C.byref(x) for x in p]))
+        return dict(zip(["color", "albedo", "normal", "history"], [x.value for x in p]))
+
+    def destroy(self) -> None:
+        if self._h is not None:
+            _native.lib().lucent_film_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Renderer:
+    def __init__(self, ctx: Context, width: int, height: int, settings: Optional[RenderSettings] = None):
+        self.ctx = ctx
+        self.settings = settings or RenderSettings()
+        self.film = Film(ctx, width, height)
+        self.scene: Optional[Scene] = None
+        self.packed: Optional[PackedScene] = None
+        self.device_scene: Optional[DeviceScene] = None
+
+    @property
+    def frame(self) -> int:
+        return self.film.info()["samples"]
+
+    def load_scene(self, scene: Scene, bvh=None) -> None:
+        """Renderer.loadScene (renderer.ts:279-312)."""
+        self.packed = pack_scene(scene, bvh)
+        if self.device_scene is not None:
+            self.device_scene.destroy()
+        self.device_scene = self.ctx.upload_scene(self.packed)
+        self.scene = scene
+        self.upload_lighting(scene)
+
+    def upload_lighting(self, scene: Scene) -> None:
+        """Renderer.uploadLighting (renderer.ts:355-413)."""
+        keep: list = []
+        L = lighting_struct(scene, keep)
+        _native.check(_native.lib().lucent_scene_set_lighting(self.ctx.handle, self.device_scene.handle, C.byref(L)))
+        self.reset()
+
+    def reset(self) -> None:
+        self.film.reset()
+
+    def trace_samples(self, count: int, stream: Optional[int] = None) -> None:
+        """Renderer.traceSamples (renderer.ts:702-737)."""
+        cam = camera_struct(self.scene.camera)
+        st = _native.RenderSettingsStruct(self.settings.max_bounces, self.settings.firefly_clamp)
+        _native.check(_native.lib().lucent_trace_samples(self.ctx.handle, self.device_scene.handle, self.film.handle,
+                                                         C.byref(cam), C.byref(st), count, stream))
+
+    def benchmark(self, samples: int) -> dict:
+        """Renderer.benchmark (renderer.ts:745-777): 1 warm-up sample, then
+        `samples` timed samples; Mpaths/s = samples * W * H / time."""
+        self.reset()
+        self.trace_samples(1)
+        self.ctx.synchronize()
+        t0 = time.perf_counter()
+        self.trace_samples(samples)
+        self.ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        w, h = self.film.width, self.film.height
+        return {"traceMs": ms, "mpathsPerSec": samples * w * h / (ms / 1e3) / 1e6, "width": w, "height": h}
+
+    def close(self) -> None:
+        self.film.destroy()
+        if self.device_scene is not None:
+            self.device_scene.destroy()

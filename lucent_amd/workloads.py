@@ -3,8 +3,9 @@
 Procedural definitions fixed here (the configurations name them but leave
 their functional forms open):
 
-* ``pcg_hash``: the PCG output hash of the reference shader (common.glsl:13-18)
-  applied to a uint32 key; the value-noise lattice hash.
+* ``pcg_hash``: the standard PCG hash of a uint32 key (Jarzynski & Olano
+  2020: one LCG step, then the RXS-M-XS output function that the reference
+  shader's pcg() also uses, common.glsl:13-18); the value-noise lattice hash.
 * ``value_noise3``: trilinear interpolation, with smoothstep fade, of lattice
   values in [-1, 1) drawn from ``pcg_hash`` of the lattice coordinates, the
   octave and the seed.
@@ -130,3 +131,103 @@ def config1(n_rays: int = CFG1_RAYS, seed: int = 1) -> dict:
     pos = icosphere_fbm(CFG1_SUBDIV, 1.0, 0.1, seed)
     org, d = sphere_rays(n_rays, seed + 1)
     return {"positions": pos, "origins": org, "dirs": d}
+
+
+# ---------------------------------------------------------------------------
+# Progressive path-trace workload (1440x900, 64 spp, depth 8)
+# ---------------------------------------------------------------------------
+def direction_from_uv(u, v):
+    """environment.ts:88-93 (v = 0 is straight up)."""
+    phi = (u - 0.5) * 2 * np.pi
+    theta = v * np.pi
+    s = np.sin(theta)
+    return np.stack([s * np.cos(phi), np.cos(theta), s * np.sin(phi)], -1)
+
+
+def synthetic_hdri(width: int = 1024, height: int = 512, sun_elevation: float = 35.0, sun_azimuth: float = 40.0,
+                   sigma_deg: float = 0.5, peak: float = 5e4) -> np.ndarray:
+    """Sky gradient plus a Gaussian sun lobe, float32 rgb rows top to bottom.
+
+    Sky: L = 0.2 + 1.8 * (elevation + 90) / 180 (linear in elevation angle,
+    0.2 straight down to 2.0 at the zenith), tinted (0.85, 0.92, 1.0).
+    Sun: peak * exp(-a^2 / (2 sigma^2)) with a the angle to the sun direction
+    (elevation 35 deg, azimuth 40 deg), sigma = 0.5 deg, tinted (1, 0.96, 0.9).
+    Texels are evaluated at their centres.
+    """
+    xs = (np.arange(width) + 0.5) / width
+    ys = (np.arange(height) + 0.5) / height
+    u, v = np.meshgrid(xs, ys)
+    d = direction_from_uv(u, v)
+    elev = np.degrees(np.arcsin(np.clip(d[..., 1], -1, 1)))
+    sky = 0.2 + 1.8 * (elev + 90.0) / 180.0
+    el, az = np.radians(sun_elevation), np.radians(sun_azimuth)
+    sd = np.array([np.cos(el) * np.sin(az), np.sin(el), np.cos(el) * np.cos(az)])
+    ang = np.degrees(np.arccos(np.clip(d @ sd, -1, 1)))
+    sun = peak * np.exp(-(ang ** 2) / (2 * sigma_deg ** 2))
+    rgb = sky[..., None] * np.array([0.85, 0.92, 1.0]) + sun[..., None] * np.array([1.0, 0.96, 0.9])
+    return rgb.astype(np.float32).reshape(-1)
+
+
+def displace_along_normals(mesh, amplitude: float, seed: int, freq_scale: float = 1.0):
+    """Moves each vertex along its (ring) normal by amplitude * fbm3(position)."""
+    from .scene import Mesh
+    p = mesh.positions.reshape(-1, 3).astype(np.float64)
+    n = mesh.normals.reshape(-1, 3).astype(np.float64)
+    d = amplitude * fbm3(p * freq_scale, seed)
+    q = p + n * d[:, None]
+    return Mesh(q.reshape(-1).astype(np.float32), mesh.normals.copy(), mesh.uvs)
+
+
+KNOT_PQ = [(2, 3), (3, 4), (2, 5), (3, 5), (3, 7), (2, 7)]
+
+
+def random_material(rng: np.random.Generator):
+    """Per-mesh layered BSDF draw: albedo U[0.05,0.95]^3, roughness U[0.02,1],
+    metallic Bernoulli(0.3), clearcoat Bernoulli(0.5) with coat roughness
+    U[0,0.3], sheen s ~ U[0,1] as sheenColor (s,s,s) with sheenRoughness 0.5,
+    transmission Bernoulli(0.2) with IOR 1.5 and Beer-Lambert absorption
+    (attenuationColor U[0.5,1]^3 over distance 1)."""
+    from .scene import Material
+    albedo = tuple(float(x) for x in rng.uniform(0.05, 0.95, 3))
+    rough = float(rng.uniform(0.02, 1.0))
+    metallic = 1.0 if rng.random() < 0.3 else 0.0
+    coat = rng.random() < 0.5
+    coat_r = float(rng.uniform(0.0, 0.3))
+    sheen = float(rng.uniform(0.0, 1.0))
+    trans = rng.random() < 0.2
+    att = tuple(float(x) for x in rng.uniform(0.5, 1.0, 3))
+    return Material(albedo=albedo, roughness=rough, metallic=metallic, clearcoat=1.0 if coat else 0.0,
+                    clearcoat_roughness=coat_r if coat else 0.0, sheen_color=(sheen, sheen, sheen),
+                    sheen_roughness=0.5, transmission=1.0 if trans else 0.0, ior=1.5,
+                    attenuation_color=att if trans else (1.0, 1.0, 1.0),
+                    attenuation_distance=1.0 if trans else float("inf"))
+
+
+def knot_scene(seed: int = 7, knots: int = 6, tri_range=(30000, 40000), radial: int = 40,
+               hdri_size=(1024, 512)):
+    """Seeded displaced torus knots (30-40k triangles each) on a 2-triangle
+    ground quad, one emissive quad light, and the synthetic HDRI."""
+    from . import scene as S
+    rng = np.random.Generator(np.random.PCG64(seed))
+    meshes = []
+    for k in range(knots):
+        target = int(rng.integers(tri_range[0], tri_range[1] + 1))
+        tubular = max(8, target // (2 * radial))
+        p, q = KNOT_PQ[k % len(KNOT_PQ)]
+        m = S.torus_knot(p=p, q=q, radius=1.0, tube=0.3, tubular_segments=tubular, radial_segments=radial)
+        m = displace_along_normals(m, 0.06, seed * 100 + k, 3.0)
+        col, row = k % 3, k // 3
+        m = S.transform(m, translate=(-3.6 + 3.6 * col, 1.15, -1.8 + 3.6 * row), scale=1.0,
+                        rotate_y=float(rng.uniform(0, 2 * np.pi)))
+        meshes.append(S.MeshInstance(m, random_material(rng)))
+    meshes.append(S.MeshInstance(S.floor((0.0, 0.0, 0.0), 12.0), S.matte((0.5, 0.5, 0.5), 1.0)))
+    # Winding chosen so the lamp's normal points down (-Y).
+    lamp = S.quad((-1.0, 5.5, -1.0), (1.0, 5.5, -1.0), (1.0, 5.5, 1.0), (-1.0, 5.5, 1.0))
+    meshes.append(S.MeshInstance(lamp, S.light((1.0, 0.9, 0.75), 12.0)))
+    w, h = hdri_size
+    env = S.Environment(map=(w, h, synthetic_hdri(w, h)), rotation=0.0, intensity=1.0, background=True)
+    cam = S.Camera(position=(0.0, 5.0, 11.5), target=(0.0, 0.9, 0.0), fov=40.0, aperture=0.0, focus_distance=11.0)
+    return S.Scene(camera=cam, meshes=meshes, env=env, name="knots")
+
+
+PT_WIDTH, PT_HEIGHT, PT_SPP, PT_DEPTH = 1440, 900, 64, 8

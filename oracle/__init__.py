@@ -7,6 +7,7 @@ cpu_baseline leg may import this package; the product never does.
 from __future__ import annotations
 
 import concurrent.futures as cf
+import math
 import ctypes as C
 import os
 import subprocess
@@ -202,3 +203,255 @@ def shader_order_counts(bvh, positions, org, dirs) -> dict:
 def bvh_depth(bvh) -> int:
     nodes = np.ascontiguousarray(bvh.nodes, np.float32)
     return int(lib().or_bvh_depth(_ptr(nodes), bvh.node_count))
+
+
+# ---------------------------------------------------------------------------
+# Integrator oracle (pt_oracle.c)
+# ---------------------------------------------------------------------------
+_F = C.c_float
+_I = C.c_int32
+
+
+class OrScene(C.Structure):
+    _fields_ = [
+        ("geo", _P), ("attr", _P), ("tri_count", _U32), ("nodes", _P), ("node_count", _U32),
+        ("materials", _P), ("material_count", _U32), ("spheres", _P), ("sphere_mat", _P), ("sphere_light", _P),
+        ("sphere_count", _U32), ("lights", _P), ("light_count", _U32), ("ground_material", _I),
+        ("ground_height", _F), ("env_rgb", _P), ("env_w", _I), ("env_h", _I), ("env_table", _P), ("grid_w", _I),
+        ("grid_h", _I), ("env_cos", _F), ("env_sin", _F), ("env_intensity", _F), ("env_sampling", _I),
+        ("env_background", _I), ("backdrop", _F * 3), ("sun_enabled", _I), ("sun_dir", _F * 3),
+        ("sun_radiance", _F * 3), ("sun_cos", _F), ("sun_pdf", _F), ("sheen_lut", _P), ("color_maps", _P),
+        ("data_maps", _P), ("tex_size", _I), ("color_layers", _I), ("data_layers", _I),
+    ]
+
+
+class OrFrame(C.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("max_bounces", _U32), ("clamp", _F), ("cam_pos", _F * 3),
+                ("cam_right", _F * 3), ("cam_up", _F * 3), ("cam_fwd", _F * 3), ("tan_half_fov", _F),
+                ("aperture", _F), ("focus_dist", _F)]
+
+
+def _pt_lib():
+    l = lib()
+    if not getattr(l, "_pt_ready", False):
+        l.or_trace_pixels.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), _U32, _U32, _P, _U32, _P, _P]
+        l.or_trace_pixels.restype = C.c_int
+        l.or_alias_table.argtypes = [_P, _U32, _P, _P, _P, C.POINTER(_D)]
+        l.or_alias_table.restype = C.c_int
+        l.or_env_sampler.argtypes = [_P, _U32, _U32, _U32, _P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_D)]
+        l.or_env_sampler.restype = C.c_int
+        l.or_sheen_albedo.argtypes = [_D, _D, _U32]
+        l.or_sheen_albedo.restype = _D
+        l.or_round_half.argtypes = [_F]
+        l.or_round_half.restype = _F
+        l.or_sheen_lut_half.argtypes = [_P]
+        l.or_sheen_lut_half.restype = None
+        l.or_eval_opaque.argtypes = [_P, _P, _P, _P, _P, C.POINTER(_D)]
+        l.or_eval_opaque.restype = None
+        l.or_sample_opaque.argtypes = [_P, _P, _P, _U32, _P, _P, C.POINTER(_D)]
+        l.or_sample_opaque.restype = None
+        l.or_sample_dielectric.argtypes = [_P, C.c_int, _P, _U32, _P, C.POINTER(C.c_int)]
+        l.or_sample_dielectric.restype = C.c_int
+        l.or_pcg_stream.argtypes = [_U32, _U32, _U32, _P]
+        l.or_pcg_stream.restype = _U32
+        l.or_d_ggx.argtypes = [_D, _D]
+        l.or_d_ggx.restype = _D
+        l.or_g1_ggx.argtypes = [_D, _D]
+        l.or_g1_ggx.restype = _D
+        l.or_sample_vndf.argtypes = [_P, _D, _D, _D, _P]
+        l.or_sample_vndf.restype = None
+        l._pt_ready = True
+    return l
+
+
+def alias_table(weights):
+    w = np.ascontiguousarray(weights, np.float64)
+    n = w.size
+    prob = np.zeros(n, np.float32)
+    alias = np.zeros(n, np.uint32)
+    pmf = np.zeros(n, np.float64)
+    total = C.c_double()
+    if _pt_lib().or_alias_table(_ptr(w), n, _ptr(prob), _ptr(alias), _ptr(pmf), C.byref(total)) != 0:
+        raise ValueError("invalid alias weights")
+    return prob, alias, pmf, total.value
+
+
+def sample_alias(prob, alias, u0, u1):
+    """alias.ts:68-72."""
+    n = prob.size
+    i = min(n - 1, int(math.floor(u0 * n)))
+    return i if u1 < prob[i] else int(alias[i])
+
+
+def env_sampler(rgb, width, height, max_width=1024):
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1)
+    gw0 = max(1, min(width, max_width))
+    table = np.zeros(gw0 * height * 4 + 4, np.float32)
+    gw, gh, total = C.c_uint32(), C.c_uint32(), C.c_double()
+    rc = _pt_lib().or_env_sampler(_ptr(rgb), width, height, max_width, _ptr(table), C.byref(gw), C.byref(gh),
+                                  C.byref(total))
+    if rc != 0:
+        raise ValueError("invalid environment map")
+    return table[: gw.value * gh.value * 4].copy(), gw.value, gh.value, total.value
+
+
+def sheen_albedo(cos_v, roughness, steps=128):
+    return _pt_lib().or_sheen_albedo(cos_v, roughness, steps)
+
+
+def sheen_lut_half():
+    out = np.zeros(256, np.float32)
+    _pt_lib().or_sheen_lut_half(_ptr(out))
+    return out
+
+
+def round_half(x: float) -> float:
+    return _pt_lib().or_round_half(x)
+
+
+def pcg_stream(pixel_index: int, seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    _pt_lib().or_pcg_stream(pixel_index, seed, n, _ptr(out))
+    return out
+
+
+def eval_opaque(mat40, wo, wi, lut=None):
+    m = np.ascontiguousarray(mat40, np.float32)
+    lut = sheen_lut_half() if lut is None else np.ascontiguousarray(lut, np.float32)
+    wo = np.ascontiguousarray(wo, np.float64)
+    wi = np.ascontiguousarray(wi, np.float64)
+    f = np.zeros(3)
+    pdf = C.c_double()
+    _pt_lib().or_eval_opaque(_ptr(m), _ptr(lut), _ptr(wo), _ptr(wi), _ptr(f), C.byref(pdf))
+    return f, pdf.value
+
+
+def sample_opaque(mat40, wo, state, lut=None):
+    m = np.ascontiguousarray(mat40, np.float32)
+    lut = sheen_lut_half() if lut is None else np.ascontiguousarray(lut, np.float32)
+    wo = np.ascontiguousarray(wo, np.float64)
+    d = np.zeros(3)
+    w = np.zeros(3)
+    pdf = C.c_double()
+    _pt_lib().or_sample_opaque(_ptr(m), _ptr(lut), _ptr(wo), state & 0xFFFFFFFF, _ptr(d), _ptr(w), C.byref(pdf))
+    return d, w, pdf.value
+
+
+def sample_dielectric(mat40, front, wo, state):
+    m = np.ascontiguousarray(mat40, np.float32)
+    wo = np.ascontiguousarray(wo, np.float64)
+    wi = np.zeros(3)
+    refl = C.c_int()
+    ok = _pt_lib().or_sample_dielectric(_ptr(m), 1 if front else 0, _ptr(wo), state & 0xFFFFFFFF, _ptr(wi),
+                                        C.byref(refl))
+    return bool(ok), wi, bool(refl.value)
+
+
+def camera_basis(position, target, fov):
+    """renderer.ts:156-161 in float64; returns (forward, right, up, tanHalfFov)."""
+    p = np.asarray(position, np.float64)
+    f = np.asarray(target, np.float64) - p
+    f = f / math.sqrt(float(f @ f))
+    up0 = np.array([0.0, 0.0, 1.0]) if abs(f[1]) > 0.999 else np.array([0.0, 1.0, 0.0])
+    r = np.cross(f, up0)
+    r = r / math.sqrt(float(r @ r))
+    u = np.cross(r, f)
+    return f, r, u, math.tan(fov * math.pi / 360)
+
+
+class PtOracle:
+    """Holds the arrays an OrScene points to, built from a packed scene and its lighting."""
+
+    def __init__(self, packed, scene, textures=None):
+        keep = []
+
+        def arr(a, dt):
+            x = np.ascontiguousarray(np.asarray(a, dt).reshape(-1))
+            keep.append(x)
+            return x.ctypes.data
+
+        s = OrScene()
+        s.geo = arr(packed.tri_geometry, np.float32)
+        s.attr = arr(packed.tri_attributes, np.float32)
+        s.tri_count = packed.tri_count
+        s.nodes = arr(packed.bvh_nodes, np.float32)
+        s.node_count = packed.bvh_node_count
+        s.materials = arr(packed.materials, np.float32)
+        s.material_count = packed.material_count
+        s.spheres = arr(packed.spheres, np.float32)
+        s.sphere_mat = arr(packed.sphere_materials, np.int32)
+        s.sphere_light = arr(packed.sphere_lights, np.int32)
+        s.sphere_count = packed.sphere_count
+        s.lights = arr(packed.lights, np.float32)
+        s.light_count = packed.light_count
+        s.ground_material = packed.ground_material_active
+        s.ground_height = packed.ground_height
+        # Lighting as Renderer.uploadLighting sets it (renderer.ts:355-413).
+        sun, env = scene.sun, scene.env
+        el, az = math.radians(sun.elevation), math.radians(sun.azimuth)
+        sd = [math.cos(el) * math.sin(az), math.sin(el), math.cos(el) * math.cos(az)]
+        ca = math.cos(math.radians(sun.angular_radius))
+        solid = 2 * math.pi * (1 - ca)
+        s.sun_enabled = 1 if (env.map is None and sun.intensity > 0 and sun.elevation > -sun.angular_radius) else 0
+        s.sun_dir[:] = sd
+        s.sun_radiance[:] = [c * sun.intensity / solid for c in sun.color]
+        s.sun_cos = ca
+        s.sun_pdf = 1 / solid
+        if env.map is None:
+            raise NotImplementedError("oracle scenes use an explicit environment map")
+        w, h, rgb = env.map
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        table, gw, gh, total = env_sampler(rgb, w, h, 1024)
+        s.env_rgb = arr(rgb, np.float32)
+        s.env_w, s.env_h = w, h
+        s.env_table = arr(table, np.float32)
+        s.grid_w, s.grid_h = gw, gh
+        rot = math.radians(env.rotation)
+        s.env_cos, s.env_sin = math.cos(rot), math.sin(rot)
+        s.env_intensity = env.intensity
+        s.env_sampling = 1 if total > 0 else 0
+        s.env_background = 1 if env.background else 0
+        s.backdrop[:] = list(env.backdrop)
+        s.sheen_lut = arr(sheen_lut_half(), np.float32)
+        if textures is not None:
+            size, color, data = textures
+            s.color_maps = arr(color, np.uint32)
+            s.data_maps = arr(data, np.uint32)
+            s.tex_size = size
+            s.color_layers = color.size // (size * size)
+            s.data_layers = data.size // (size * size)
+        self.scene = s
+        self.keep = keep
+        self.env_table = table
+
+    def frame(self, camera, width, height, max_bounces=8, clamp=0.0) -> OrFrame:
+        f = OrFrame()
+        f.width, f.height, f.max_bounces, f.clamp = width, height, max_bounces, clamp
+        fw, r, u, t = camera_basis(camera.position, camera.target, camera.fov)
+        # The reference uploads these as float32 uniforms.
+        f.cam_pos[:] = list(np.float32(camera.position))
+        f.cam_fwd[:] = list(np.float32(fw))
+        f.cam_right[:] = list(np.float32(r))
+        f.cam_up[:] = list(np.float32(u))
+        f.tan_half_fov = t
+        f.aperture = camera.aperture
+        f.focus_dist = camera.focus_distance
+        return f
+
+    def trace_pixels(self, frame: OrFrame, seed0: int, samples: int, xy: np.ndarray, per_sample: bool = False,
+                     threads: int | None = None):
+        xy = np.ascontiguousarray(xy, np.uint32).reshape(-1, 2)
+        n = xy.shape[0]
+        out = np.zeros((n, 12))
+        ps = np.zeros((n, samples, 3)) if per_sample else None
+        threads = threads or min(16, os.cpu_count() or 1)
+
+        def run(rng):
+            a, b = rng
+            return _pt_lib().or_trace_pixels(C.byref(self.scene), C.byref(frame), seed0, samples,
+                                             _ptr(xy) + a * 8, b - a, _ptr(out) + a * 96,
+                                             (_ptr(ps) + a * samples * 24) if per_sample else None)
+
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(run, _split(n, threads * 4)))
+        return out, ps

@@ -1,0 +1,273 @@
+"""Pins the integrator oracle's tables and BSDF against the reference's own
+tests (test/environment.test.ts) and against closed forms, and checks the
+product's host-side table producers bit for bit against it."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lucent_amd import _native, workloads
+from lucent_amd.scene import Material, pack_materials
+
+
+def lcg(seed):
+    s = seed & 0xFFFFFFFF
+    while True:
+        s = (s * 1664525 + 1013904223) & 0xFFFFFFFF
+        yield s / 4294967296.0
+
+
+# ---- alias table (environment.test.ts:16-44) -------------------------------
+def test_alias_reproduces_distribution():
+    w = [1, 0, 3, 6, 0.5, 9.5]
+    prob, alias, pmf, total = O.alias_table(w)
+    n = len(w)
+    p = np.zeros(n)
+    for j in range(n):
+        p[j] += prob[j] / n
+        p[alias[j]] += (1 - prob[j]) / n
+    assert np.allclose(p, np.array(w) / sum(w), atol=1e-6)
+    assert total == sum(w)
+
+
+def test_alias_never_draws_zero_weight():
+    prob, alias, _, _ = O.alias_table([0, 5, 0, 1])
+    r = lcg(7)
+    for _ in range(2000):
+        assert O.sample_alias(prob, alias, next(r), next(r)) in (1, 3)
+
+
+def test_alias_uniform_fallback_and_rejects_bad_input():
+    _, _, pmf, total = O.alias_table([0, 0, 0, 0])
+    assert list(pmf) == [0.25] * 4 and total == 0
+    for bad in ([1, -1], [1, float("nan")], []):
+        with pytest.raises(ValueError):
+            O.alias_table(bad)
+
+
+def _product_alias(w):
+    w = np.ascontiguousarray(w, np.float64)
+    n = w.size
+    prob = np.zeros(n, np.float32)
+    alias = np.zeros(n, np.uint32)
+    pmf = np.zeros(n)
+    total = C.c_double()
+    _native.check(_native.lib().lucent_alias_table(w.ctypes.data, n, prob.ctypes.data, alias.ctypes.data,
+                                                   pmf.ctypes.data, C.byref(total)))
+    return prob, alias, pmf, total.value
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_product_alias_table_bit_identical(seed):
+    rng = np.random.default_rng(seed)
+    w = rng.random(5000) ** 4
+    w[rng.random(5000) < 0.1] = 0
+    a, b = O.alias_table(w), _product_alias(w)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    assert a[3] == b[3]
+    with pytest.raises(_native.LucentError):
+        _product_alias([1.0, -1.0])
+
+
+# ---- environment sampler (environment.test.ts:101-136) ---------------------
+def _bright_map():
+    w, h = 64, 32
+    data = np.full(w * h * 3, 0.1, np.float32)
+    b = (10 * w + 40) * 3
+    data[b:b + 3] = 500
+    return data, w, h
+
+
+def test_env_pdf_integrates_to_one_and_favours_bright_cell():
+    data, w, h = _bright_map()
+    table, gw, gh, total = O.env_sampler(data, w, h, 32)
+    assert (gw, gh) == (32, 16)
+    t = table.reshape(-1, 4)
+    assert t[:, 2].sum() / (gw * gh) == pytest.approx(1, abs=1e-6)
+    assert int(np.argmax(t[:, 2])) == 5 * 32 + 20
+    for i in range(20):
+        assert t[i, 3] == pytest.approx(t[int(t[i, 1]), 2], abs=1e-6)
+
+
+def _product_env(data, w, h, max_w):
+    table = np.zeros(min(w, max_w) * h * 4 + 4, np.float32)
+    gw, gh, total = C.c_uint32(), C.c_uint32(), C.c_double()
+    _native.check(_native.lib().lucent_env_sampler(np.ascontiguousarray(data, np.float32).ctypes.data, w, h, max_w,
+                                                   table.ctypes.data, C.byref(gw), C.byref(gh), C.byref(total)))
+    return table[: gw.value * gh.value * 4], gw.value, gh.value, total.value
+
+
+@pytest.mark.parametrize("which", ["bright", "hdri", "odd"])
+def test_product_env_sampler_bit_identical(which):
+    if which == "bright":
+        data, w, h = _bright_map()
+        mw = 32
+    elif which == "hdri":
+        w, h = 256, 128
+        data = workloads.synthetic_hdri(w, h, sigma_deg=3.0)
+        mw = 1024
+    else:
+        w, h = 50, 31
+        data = np.random.default_rng(4).random(w * h * 3).astype(np.float32)
+        mw = 20
+    a, b = O.env_sampler(data, w, h, mw), _product_env(data, w, h, mw)
+    assert a[1:3] == b[1:3] and a[3] == b[3]
+    assert np.array_equal(a[0], b[0])
+
+
+def test_uv_direction_round_trip():
+    r = lcg(3)
+    for _ in range(100):
+        u, v = next(r), 0.01 + next(r) * 0.98
+        d = workloads.direction_from_uv(np.array(u), np.array(v))
+        u2 = math.atan2(d[2], d[0]) / (2 * math.pi) + 0.5
+        v2 = math.acos(max(-1, min(1, d[1]))) / math.pi
+        assert u2 == pytest.approx(u, abs=1e-9) and v2 == pytest.approx(v, abs=1e-9)
+
+
+# ---- sheen (environment.test.ts:151-183) ------------------------------------
+def test_sheen_bounds_and_growth():
+    lut = O.sheen_lut_half()
+    assert (lut >= 0).all() and (lut <= 1).all()
+    assert O.sheen_albedo(0.7, 1, 96) > O.sheen_albedo(0.7, 0.2, 96)
+
+
+def test_sheen_matches_monte_carlo():
+    cos_v, r = 0.5, 0.6
+    alpha = r * r
+    v = np.array([math.sqrt(1 - cos_v * cos_v), 0, cos_v])
+    g = lcg(11)
+    n = 200000
+    u = np.array([next(g) for _ in range(2 * n)]).reshape(n, 2)
+    z = u[:, 0]
+    phi = 2 * math.pi * u[:, 1]
+    s = np.sqrt(1 - z * z)
+    l = np.stack([s * np.cos(phi), s * np.sin(phi), z], -1)
+    hv = l + v
+    cos_h = hv[:, 2] / np.linalg.norm(hv, axis=1)
+    inv = 1 / alpha
+    d = (2 + inv) * np.maximum(0, 1 - cos_h * cos_h) ** (inv / 2) / (2 * math.pi)
+    vis = 1 / (4 * (z + cos_v - z * cos_v))
+    mc = 2 * math.pi * (d * vis * z).sum() / n
+    assert abs(O.sheen_albedo(cos_v, r, 256) - mc) < 0.005
+
+
+def test_product_sheen_lut_identical():
+    out = np.zeros(256)
+    _native.check(_native.lib().lucent_sheen_lut(16, 96, out.ctypes.data))
+    ref = np.array([O.sheen_albedo(max(1e-3, x / 15), y / 15, 96) for y in range(16) for x in range(16)])
+    assert np.array_equal(out, ref)
+    assert np.array_equal(O.sheen_lut_half(), np.array([O.round_half(float(x)) for x in ref], np.float32))
+
+
+def test_round_half_matches_numpy():
+    x = np.random.default_rng(1).standard_normal(20000).astype(np.float32) * 100
+    x = np.concatenate([x, np.float32([1e-6, 6.1e-5, 65504, 65519, 0.5, 1 / 3])])
+    ours = np.array([O.round_half(float(v)) for v in x], np.float32)
+    assert np.array_equal(ours, x.astype(np.float16).astype(np.float32))
+
+
+# ---- BSDF closed forms ------------------------------------------------------
+@pytest.mark.parametrize("a", [0.05, 0.3, 0.8])
+def test_ggx_normalised(a):
+    # ∫ D(h) (n·h) dω_h = 1 over the hemisphere (midpoint rule in cos θ).
+    n = 200000
+    c = (np.arange(n) + 0.5) / n
+    fn = O._pt_lib().or_d_ggx
+    d = np.array([fn(float(x), a) for x in c])
+    integral = (d * c).mean() * 2 * math.pi
+    assert integral == pytest.approx(1, rel=2e-3)
+    assert O._pt_lib().or_g1_ggx(1.0, a) == pytest.approx(1.0)
+
+
+def _mat(**kw):
+    return pack_materials([Material(**kw)])
+
+
+@pytest.mark.parametrize("kw", [
+    dict(albedo=(0.8, 0.6, 0.4), roughness=1.0),
+    dict(albedo=(0.9, 0.7, 0.3), metallic=1.0, roughness=0.4),
+    dict(albedo=(0.5, 0.5, 0.5), roughness=0.3, clearcoat=1.0, clearcoat_roughness=0.1),
+    dict(albedo=(0.3, 0.4, 0.5), roughness=0.7, sheen_color=(0.6, 0.6, 0.6), sheen_roughness=0.5),
+])
+def test_bsdf_sampler_consistent_with_evaluation(kw):
+    """E[f cos / pdf] over the sampler equals ∫ f cos dω by quadrature: a pdf
+    that does not match the sampler's density fails this."""
+    m = _mat(**kw)
+    lut = O.sheen_lut_half()
+    wo = np.array([math.sin(0.6), 0.0, math.cos(0.6)])
+    # Quadrature of ∫ f(wo, wi) cos θi dωi.
+    nt, nphi = 160, 160
+    acc = np.zeros(3)
+    for i in range(nt):
+        ct = (i + 0.5) / nt
+        st = math.sqrt(1 - ct * ct)
+        for j in range(nphi):
+            ph = (j + 0.5) / nphi * 2 * math.pi
+            wi = np.array([st * math.cos(ph), st * math.sin(ph), ct])
+            f, _ = O.eval_opaque(m, wo, wi, lut)
+            acc += f * ct
+    quad = acc * (1.0 / nt) * (2 * math.pi / nphi)
+    rng = np.random.default_rng(5)
+    est = np.zeros(3)
+    N = 40000
+    states = rng.integers(0, 2 ** 32, N, dtype=np.uint64)
+    for s in states:
+        d, w, pdf = O.sample_opaque(m, wo, int(s), lut)
+        est += w
+    est /= N
+    assert np.allclose(est, quad, rtol=0.04, atol=2e-3), (est, quad)
+
+
+def test_lambert_white_furnace_closed_form():
+    # A pure diffuse lobe (F0 = 0 via ior = 1) with albedo 1: ∫ f cos = 1 - ∫ F...; with
+    # ior 1 the Schlick F0 is 0, so the albedo is 1 minus a Fresnel tail; at normal
+    # incidence the specular lobe vanishes with F0 = 0 except the (1-cos)^5 term.
+    m = _mat(albedo=(1.0, 1.0, 1.0), roughness=1.0, ior=1.0)
+    wo = np.array([0.0, 0.0, 1.0])
+    f, pdf = O.eval_opaque(m, wo, np.array([0.0, 0.0, 1.0]))
+    # At wi = wo = n: H = n, F = 0, diffuse = 1/pi exactly; specular D*G*F = 0.
+    assert f == pytest.approx(np.full(3, 1 / math.pi), rel=1e-12)
+
+
+def test_dielectric_reflection_probability_is_schlick():
+    m = _mat(albedo=(1, 1, 1), transmission=1.0, roughness=0.0, ior=1.5)
+    th = 0.9
+    wo = np.array([math.sin(th), 0.0, math.cos(th)])
+    rng = np.random.default_rng(2)
+    N = 40000
+    refl = 0
+    for s in rng.integers(0, 2 ** 32, N, dtype=np.uint64):
+        ok, wi, r = O.sample_dielectric(m, True, wo, int(s))
+        refl += r
+        if r:
+            assert np.allclose(wi, [-wo[0], -wo[1], wo[2]], atol=1e-12)  # mirror about n
+    f0 = ((1.5 - 1) / (1.5 + 1)) ** 2
+    fr = f0 + (1 - f0) * (1 - math.cos(th)) ** 5
+    assert refl / N == pytest.approx(fr, abs=4 * math.sqrt(fr * (1 - fr) / N))
+
+
+def test_pcg_stream_seeding():
+    # trace.frag:326-328: state = pixel*1973 + seed*9277 + 26699, one warm-up draw.
+    a = O.pcg_stream(10, 3, 4)
+    b = O.pcg_stream(0, 0, 1)
+    assert a.dtype == np.uint32 and len(set(a.tolist())) == 4
+    # Independent restatement of the published hash (Jarzynski & Olano 2020, "pcg").
+    def pcg(state):
+        s = state & 0xFFFFFFFF
+        nxt = (s * 747796405 + 2891336453) & 0xFFFFFFFF
+        word = (((s >> ((s >> 28) + 4)) ^ s) * 277803737) & 0xFFFFFFFF
+        return nxt, ((word >> 22) ^ word) & 0xFFFFFFFF
+    st = (10 * 1973 + 3 * 9277 + 26699) & 0xFFFFFFFF
+    st, _ = pcg(st)
+    ref = []
+    for _ in range(4):
+        st, out = pcg(st)
+        ref.append(out)
+    assert a.tolist() == ref
+    # The noise hash of the workload generators is the standard pcg_hash: the same
+    # output function applied to the advanced state.
+    assert int(workloads.pcg_hash(np.array([12345]))[0]) == pcg(pcg(12345)[0])[1]
