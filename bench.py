@@ -2,8 +2,12 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload closest-hit]
 
-closest-hit: one step = one batched exact closest-hit launch of 2^20 rays
-against the 81,920-triangle fBm icosphere (inputs resident in HBM).
+Headline (default): the plain progressive path trace at 1440x900 — one step
+is a full 64-spp frame (64 launches, one per sample, as Renderer.traceSamples)
+of the 222k-triangle knot scene; value = camera paths (pixel samples) per
+second.  The closest-hit benchmark is reported beside it: one step = one
+batched exact closest-hit launch of 2^20 rays against the 81,920-triangle
+fBm icosphere (inputs resident in HBM).
 
 Timing: W untimed steps, then K steps bracketed by torch.cuda.synchronize();
 each launch is also bracketed by HIP events on the stream it runs on (torch's
@@ -28,6 +32,19 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s closest-hit; samples/s @1440x900 batched vs plain; adjoint grad/s"
 HBM_PEAK_GBS = 8000.0
+
+
+def profiled_traffic(key: str):
+    """HBM bytes per launch for `key` from the committed rocprofv3 --pmc summary, or None."""
+    path = ROOT / "profiles/hbm_traffic.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith(key + " "):
+            for tok in line.split():
+                if tok.startswith("hbm_bytes_per_launch="):
+                    return float(tok.split("=", 1)[1])
+    return None
 
 
 def log(msg: str) -> None:
@@ -113,10 +130,7 @@ def bench_closest_hit(args) -> dict:
     hit_bytes = 8 + 4 + 8 + 8
     b_ray = 24 + hit_bytes + 32 + 64 * P + 48 * T
     achieved = b_ray * n / avg_launch_s / 1e9
-    traffic = None
-    tr_path = ROOT / "profiles/closest_hit_icosphere81920_2e20rays_traffic.json"
-    if tr_path.exists():
-        traffic = json.loads(tr_path.read_text()).get("hbm_bytes_per_launch")
+    traffic = profiled_traffic("closest_hit_icosphere81920_2e20rays")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_ray": round(b_ray, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
@@ -193,6 +207,17 @@ def bench_path_trace(args) -> dict:
     launch_s = float(np.mean(step_ms)) / 1e3 / SPP
     img = r.film.read()["color"]
     mean_rgb = [float(x) for x in img[..., :3].reshape(-1, 3).mean(0)]
+    counts = json.loads((ROOT / "tests/golden/pathtrace_counts.json").read_text())["per_path"]
+    b_path = path_bytes(counts)
+    achieved = b_path * W * H / launch_s / 1e9
+    traffic = profiled_traffic("path_trace_1440x900")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_path": round(b_path, 1),
+                "kernel": "k_trace_sample", "avg_launch_ms": round(launch_s * 1e3, 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_path_trace(r.packed, scene, W, H)
+    rays_per_path = counts["closest_rays"] + counts["shadow_rays"]
     r.close()
     ctx.close()
     return {"value": round(value, 1), "unit": "samples/s", "ms_per_step": round(wall / args.steps * 1e3, 3),
@@ -202,8 +227,113 @@ def bench_path_trace(args) -> dict:
                                    f"({stats['tri_count']} tris), quad light, 1024x512 HDRI",
                        "width": W, "height": H, "spp": SPP, "tris": stats["tri_count"],
                        "scene_load_s": round(load_s, 3)},
+            "roofline": roofline, "cpu_baseline": cpu,
             "avg_sample_launch_ms": round(launch_s * 1e3, 4), "image_mean_rgb": mean_rgb,
-            "mpaths_per_s": round(value / 1e6, 2)}
+            "mpaths_per_s": round(value / 1e6, 2), "mrays_per_s_all_rays": round(value * rays_per_path / 1e6, 1)}
+
+
+def path_bytes(c: dict) -> float:
+    """Algorithmic bytes of one camera path: accumulate read+write (72), per
+    closest-hit ray 24 in + 16 out + 32 root + 64 per child pair + 48 per
+    triangle, per shadow ray 24 in + 32 per node + 48 per triangle, per surface
+    vertex 48 GEO + 64 ATTR + 160 material, per environment NEE 16 alias + 48
+    bilinear, per escaped ray 48 bilinear."""
+    return (72 + c["closest_rays"] * 72 + 64 * c["closest_interior_visits"] + 48 * c["closest_tri_tests"]
+            + c["shadow_rays"] * 24 + 32 * c["shadow_node_visits"] + 48 * c["shadow_tri_tests"]
+            + c["vertices"] * 272 + c["env_nee"] * 64 + c["escaped"] * 48)
+
+
+def cpu_baseline_path_trace(packed, scene, W, H) -> dict:
+    """The FP64 oracle's path tracer (a C restatement of trace.frag) on the
+    host cores: a 256x256 crop at the frame centre, 16 spp, one thread per core."""
+    import oracle as O
+
+    o = O.PtOracle(packed, scene)
+    f = o.frame(scene.camera, W, H, 8, 0.0)
+    yy, xx = np.mgrid[H // 2 - 128:H // 2 + 128, W // 2 - 128:W // 2 + 128]
+    xy = np.stack([xx.reshape(-1), yy.reshape(-1)], -1)
+    threads = min(16, os.cpu_count() or 1)
+    log(f"cpu baseline: FP64 oracle path tracer, 256x256 crop x 16 spp on {threads} threads")
+    t0 = time.perf_counter()
+    o.trace_pixels(f, 0, 16, xy, threads=threads)
+    dt = time.perf_counter() - t0
+    n = xy.shape[0] * 16
+    return {"value": round(n / dt, 1), "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"256x256 centre crop of the 1440x900 frame, 16 spp ({n} paths), FP64 C restatement of "
+                      f"trace.frag on {threads} threads, {dt:.2f} s wall"}
+
+
+def bench_denoise(args) -> dict:
+    """Mixed-precision denoise: a 1920x1080 film at 8 spp of the knot scene
+    (FP32 radiance, FP16 guides), variance pass + 5 à-trous iterations with
+    sigma_n 128, sigma_p 1 (for sigma_z), sigma_l 4; FP32 taps."""
+    import torch
+
+    from lucent_amd import workloads
+    from lucent_amd.renderer import Renderer, RenderSettings
+    from lucent_amd.runtime import Context
+
+    W, H = 1920, 1080
+    ctx = Context(0)
+    r = Renderer(ctx, W, H, RenderSettings(8, 0.0))
+    r.load_scene(workloads.knot_scene())
+    r.trace_samples(8)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        r.run_denoiser(4.0, 128.0, 1.0, 5, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        r.run_denoiser(4.0, 128.0, 1.0, 5, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    px = W * H
+    b_px = 44 + 5 * 48  # variance pass + 5 iterations, compulsory bytes per pixel
+    achieved = b_px * px / (ms / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+        film = r.film.read()
+        cam = O.camera13(r.scene.camera)
+        t0 = time.perf_counter()
+        O.denoise_window(film["color"], film["albedo"], film["normal"], film["history"], cam, 4.0, 128.0, 1.0, 5,
+                         720, 405, 480, 270)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(dt * 1e3 * 16, 1), "unit": "ms per 1920x1080 denoise (extrapolated)", "cores": 1,
+               "kind": "port", "sample": f"FP64 oracle of the same filter on a 480x270 window (1/16 of the frame): "
+                                         f"{dt * 1e3:.1f} ms, x16"}
+    r.close()
+    ctx.close()
+    return {"metric": "denoise ms @1920x1080, 8 spp, 5 a-trous iterations", "value": round(ms, 4), "unit": "ms",
+            "higher_is_better": False,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_pixel": b_px},
+            "cpu_baseline": cpu}
+
+
+def bench_copy(args) -> dict:
+    """Calibration: a plain device copy of 1 GiB per step (for scaling the
+    FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+    n = 1 << 30
+    src = torch.empty(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"value": round(2 * n / dt / 1e9, 1), "unit": "GB/s (read+write)", "ms_per_step": round(dt * 1e3, 4),
+            "dtype": "u8", "config": {"workload": "device copy of 1 GiB (counter calibration)", "bytes": n}}
 
 
 def main() -> int:
@@ -211,7 +341,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="closest-hit", choices=["closest-hit", "path-trace"])
+    ap.add_argument("--workload", default="all", choices=["all", "closest-hit", "path-trace", "denoise", "copy"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -220,7 +350,22 @@ def main() -> int:
               "rays/pixels/scenes with no exchange, see DESIGN.md, but no multi-GPU figure is produced)",
               file=sys.stderr)
         return 2
-    res = bench_closest_hit(args) if args.workload == "closest-hit" else bench_path_trace(args)
+    extra = {}
+    if args.workload == "closest-hit":
+        res = bench_closest_hit(args)
+    elif args.workload == "copy":
+        res = bench_copy(args)
+    elif args.workload == "denoise":
+        d = bench_denoise(args)
+        res = {"value": d["value"], "unit": d["unit"], "ms_per_step": d["value"], "dtype": "f32/f16",
+               "config": {"workload": d["metric"]}, "roofline": d["roofline"], "cpu_baseline": d["cpu_baseline"]}
+    else:
+        res = bench_path_trace(args)
+        if args.workload == "all":
+            ch = bench_closest_hit(args)
+            extra["closest_hit"] = {"metric": "Mrays/s closest-hit", "value": ch.pop("value"),
+                                    "unit": ch.pop("unit"), **ch}
+            extra["denoise"] = bench_denoise(args)
     line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
@@ -229,6 +374,7 @@ def main() -> int:
     if "roofline" not in line or line["roofline"] is None:
         line["roofline"] = res.pop("roofline", None)
     line.update(res)
+    line.update(extra)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -257,6 +257,26 @@ lucent_status lucent_trace_samples(lucent_ctx *ctx, lucent_scene *scene, lucent_
                                    const lucent_camera *camera, const lucent_render_settings *settings,
                                    uint32_t count, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Denoiser — Renderer.runDenoiser (src/renderer.ts:788-830): variance.frag,
+ * then `iterations` à-trous passes (atrous.frag) with steps 1, 2, 4, ...
+ * iterations = 0 picks atrousLevels(samples) (renderer.ts:140-142).  The
+ * reference's parameters are sigma_l = 4, sigma_n = 64, sigma_p = 1.5.
+ * The result (demodulated illumination, variance) stays in the film's filter
+ * target; lucent_film_read_denoised copies it out, together with the
+ * remodulated radiance illumination * max(albedo, 1e-3) (display.frag:32-40).
+ * ---------------------------------------------------------------------- */
+typedef struct lucent_denoise_params {
+  float sigma_l, sigma_n, sigma_p;
+  uint32_t iterations;
+} lucent_denoise_params;
+
+lucent_status lucent_denoise(lucent_ctx *ctx, lucent_film *film, const lucent_camera *camera,
+                             const lucent_denoise_params *params, void *stream);
+/* Either pointer may be NULL; each receives 4 floats per pixel. */
+lucent_status lucent_film_read_denoised(lucent_ctx *ctx, const lucent_film *film, float *illum_var4,
+                                        float *radiance4);
+
 #ifdef __cplusplus
 }
 #endif

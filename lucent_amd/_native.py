@@ -96,6 +96,10 @@ class RenderSettingsStruct(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("firefly_clamp", C.c_float)]
 
 
+class DenoiseParamsStruct(C.Structure):
+    _fields_ = [("sigma_l", C.c_float), ("sigma_n", C.c_float), ("sigma_p", C.c_float), ("iterations", C.c_uint32)]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _U32 = C.c_uint32
@@ -128,6 +132,8 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_film_read": (_I32, [_P, _P, _P, _P, _P, _P]),
     "lucent_film_device_ptrs": (_I32, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "lucent_trace_samples": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _P]),
+    "lucent_denoise": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(DenoiseParamsStruct), _P]),
+    "lucent_film_read_denoised": (_I32, [_P, _P, _P, _P]),
 }
 
 _lib = None

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "denoise.hpp"
 #include "render_host.hpp"
 #include "scene.hpp"
 
@@ -19,6 +20,9 @@ struct lucent_film {
   uint32_t seed = 0;     // Renderer.seed
   void* base = nullptr;
   PtFilm f;
+  void* filter = nullptr;       // two RGBA32F filter targets + one remodulated RGBA32F target
+  float* denoised = nullptr;    // the target holding the last denoise result
+  float* remodulated = nullptr;
 };
 
 namespace {
@@ -184,6 +188,7 @@ void lucent_film_destroy(lucent_film* film) {
   if (!film) return;
   hipStreamSynchronize(static_cast<hipStream_t>(film->ctx->stream));
   hipFree(film->base);
+  if (film->filter) hipFree(film->filter);
   delete film;
 }
 
@@ -292,6 +297,61 @@ lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_
     film->samples++;
   }
   return LUCENT_OK;
+}
+
+static uint32_t atrous_levels(uint32_t spp) {
+  return spp <= 4 ? 5 : spp <= 16 ? 4 : spp <= 64 ? 3 : spp <= 256 ? 2 : 1;
+}
+
+lucent_status lucent_denoise(lucent_ctx* ctx, lucent_film* film, const lucent_camera* cam,
+                             const lucent_denoise_params* p, void* stream) {
+  clear_error();
+  if (!ctx || !film || !cam || !p) return fail(LUCENT_E_INVALID, "null argument");
+  if (p->iterations > 12) return fail(LUCENT_E_INVALID, "at most 12 a-trous iterations");
+  const size_t px = size_t(film->width) * film->height;
+  if (!film->filter) {
+    Status s = dev_alloc(&film->filter, px * 16 * 3);
+    if (s != LUCENT_OK) return s;
+  }
+  DenoiseArgs a;
+  a.color = film->f.color;
+  a.albedo = film->f.albedo;
+  a.normal = film->f.normal;
+  a.history = film->f.history;
+  a.ping = static_cast<float*>(film->filter);
+  a.pong = a.ping + px * 4;
+  a.remodulated = a.pong + px * 4;
+  a.result = &film->denoised;
+  a.width = int(film->width);
+  a.height = int(film->height);
+  a.iterations = p->iterations ? p->iterations : atrous_levels(film->samples);
+  a.sigma_l = p->sigma_l;
+  a.sigma_n = p->sigma_n;
+  a.sigma_p = p->sigma_p;
+  double fwd[3], right[3], up[3], tan_half;
+  camera_basis(*cam, fwd, right, up, &tan_half);
+  for (int i = 0; i < 3; ++i) {
+    a.cam_pos[i] = float(cam->position[i]);
+    a.cam_fwd[i] = float(fwd[i]);
+    a.cam_right[i] = float(right[i]);
+    a.cam_up[i] = float(up[i]);
+  }
+  a.tan_half_fov = float(tan_half);
+  film->remodulated = a.remodulated;
+  return launch_denoise(a, stream ? stream : ctx->stream);
+}
+
+lucent_status lucent_film_read_denoised(lucent_ctx* ctx, const lucent_film* film, float* illum_var4,
+                                        float* radiance4) {
+  clear_error();
+  if (!ctx || !film) return fail(LUCENT_E_INVALID, "null ctx/film");
+  if (!film->denoised) return fail(LUCENT_E_INVALID, "film has not been denoised");
+  const size_t px = size_t(film->width) * film->height;
+  hipDeviceSynchronize();
+  Status s;
+  if (illum_var4 && (s = dev_download(illum_var4, film->denoised, px * 16, ctx->stream)) != LUCENT_OK) return s;
+  if (radiance4 && (s = dev_download(radiance4, film->remodulated, px * 16, ctx->stream)) != LUCENT_OK) return s;
+  return stream_sync(ctx->stream);
 }
 
 }  // extern "C"

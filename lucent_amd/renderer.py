@@ -164,6 +164,21 @@ class Renderer:
         w, h = self.film.width, self.film.height
         return {"traceMs": ms, "mpathsPerSec": samples * w * h / (ms / 1e3) / 1e6, "width": w, "height": h}
 
+    def run_denoiser(self, sigma_l: float = 4.0, sigma_n: float = 64.0, sigma_p: float = 1.5, iterations: int = 0,
+                     stream: Optional[int] = None) -> None:
+        """Renderer.runDenoiser (renderer.ts:788-830); iterations 0 = atrousLevels(frame)."""
+        cam = camera_struct(self.scene.camera)
+        p = _native.DenoiseParamsStruct(sigma_l, sigma_n, sigma_p, iterations)
+        _native.check(_native.lib().lucent_denoise(self.ctx.handle, self.film.handle, C.byref(cam), C.byref(p), stream))
+
+    def read_denoised(self) -> dict:
+        w, h = self.film.width, self.film.height
+        iv = np.empty((h, w, 4), np.float32)
+        rad = np.empty((h, w, 4), np.float32)
+        _native.check(_native.lib().lucent_film_read_denoised(self.ctx.handle, self.film.handle, iv.ctypes.data,
+                                                              rad.ctypes.data))
+        return {"illum_var": iv, "radiance": rad}
+
     def close(self) -> None:
         self.film.destroy()
         if self.device_scene is not None:

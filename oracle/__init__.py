@@ -234,7 +234,7 @@ class OrFrame(C.Structure):
 def _pt_lib():
     l = lib()
     if not getattr(l, "_pt_ready", False):
-        l.or_trace_pixels.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), _U32, _U32, _P, _U32, _P, _P]
+        l.or_trace_pixels.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), _U32, _U32, _P, _U32, _P, _P, _P]
         l.or_trace_pixels.restype = C.c_int
         l.or_alias_table.argtypes = [_P, _U32, _P, _P, _P, C.POINTER(_D)]
         l.or_alias_table.restype = C.c_int
@@ -438,20 +438,72 @@ class PtOracle:
         f.focus_dist = camera.focus_distance
         return f
 
+    COUNT_NAMES = ["paths", "closest_rays", "closest_interior_visits", "closest_tri_tests", "shadow_rays",
+                   "shadow_node_visits", "shadow_tri_tests", "vertices", "env_nee", "escaped", "light_nee"]
+
     def trace_pixels(self, frame: OrFrame, seed0: int, samples: int, xy: np.ndarray, per_sample: bool = False,
-                     threads: int | None = None):
+                     threads: int | None = None, counts: bool = False):
         xy = np.ascontiguousarray(xy, np.uint32).reshape(-1, 2)
         n = xy.shape[0]
         out = np.zeros((n, 12))
         ps = np.zeros((n, samples, 3)) if per_sample else None
         threads = threads or min(16, os.cpu_count() or 1)
+        chunks = _split(n, threads * 4)
+        cnt = np.zeros((max(1, len(chunks)), 16))
 
-        def run(rng):
-            a, b = rng
+        def run(arg):
+            i, (a, b) = arg
             return _pt_lib().or_trace_pixels(C.byref(self.scene), C.byref(frame), seed0, samples,
                                              _ptr(xy) + a * 8, b - a, _ptr(out) + a * 96,
-                                             (_ptr(ps) + a * samples * 24) if per_sample else None)
+                                             (_ptr(ps) + a * samples * 24) if per_sample else None,
+                                             (_ptr(cnt) + i * 128) if counts else None)
 
         with cf.ThreadPoolExecutor(threads) as ex:
-            list(ex.map(run, _split(n, threads * 4)))
+            list(ex.map(run, enumerate(chunks)))
+        if counts:
+            tot = cnt.sum(0)
+            return out, ps, dict(zip(self.COUNT_NAMES, tot[: len(self.COUNT_NAMES)].tolist()))
         return out, ps
+
+
+# ---------------------------------------------------------------------------
+# Denoiser oracle (denoise_oracle.c)
+# ---------------------------------------------------------------------------
+def denoise(color, albedo, normal, history, cam13, sigma_l=4.0, sigma_n=64.0, sigma_p=1.5, iterations=5):
+    """FP64 variance + à-trous over a film read back as float arrays (H, W, 4)."""
+    l = lib()
+    if not getattr(l, "_dn_ready", False):
+        l.or_denoise.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, _P, _D, _D, _D, C.c_int, _P, _P]
+        l.or_denoise.restype = C.c_int
+        l._dn_ready = True
+    h, w = color.shape[:2]
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (color, albedo, normal, history)]
+    cam = np.ascontiguousarray(cam13, np.float32)
+    out = np.zeros((h, w, 4))
+    remod = np.zeros((h, w, 3))
+    l.or_denoise(*map(_ptr, arrs), w, h, _ptr(cam), sigma_l, sigma_n, sigma_p, iterations, _ptr(out), _ptr(remod))
+    return out, remod
+
+
+def denoise_window(color, albedo, normal, history, cam13, sigma_l, sigma_n, sigma_p, iterations, x0, y0, w, h):
+    """The oracle filter over the window [x0, x0+w) x [y0, y0+h) of a full film
+    (H, W, 4 arrays, rows bottom-up); pixels outside the window are treated as
+    outside the image, camera rays use full-frame coordinates."""
+    l = lib()
+    l.or_denoise_window.argtypes = [_P, _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P, _D, _D,
+                                    _D, C.c_int, _P, _P]
+    l.or_denoise_window.restype = C.c_int
+    H, W = color.shape[:2]
+    arrs = [np.ascontiguousarray(x[y0:y0 + h, x0:x0 + w], np.float32) for x in (color, albedo, normal, history)]
+    cam = np.ascontiguousarray(cam13, np.float32)
+    out = np.zeros((h, w, 4))
+    remod = np.zeros((h, w, 3))
+    l.or_denoise_window(*map(_ptr, arrs), w, h, x0, y0, W, H, _ptr(cam), sigma_l, sigma_n, sigma_p, iterations,
+                        _ptr(out), _ptr(remod))
+    return out, remod
+
+
+def camera13(camera) -> np.ndarray:
+    fw, r, u, t = camera_basis(camera.position, camera.target, camera.fov)
+    return np.concatenate([np.float32(camera.position), np.float32(r), np.float32(u), np.float32(fw),
+                           [np.float32(t)]]).astype(np.float32)

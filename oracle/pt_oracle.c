@@ -291,22 +291,33 @@ static double slab(const float *n, d3 o, d3 inv, double tmax) {
   /* Slightly inflated: a rounding in the slab must never hide a hit. */
   return b * (1 + 1e-9) + 1e-12 >= a ? a : INF_D;
 }
-/* Closest accepted triangle (t in (EPS, closest)), any visiting order: the
- * box test only prunes what cannot be nearer. Alpha-tested triangles are
- * resolved in slot order of the left-first walk, as the RNG draw order in
- * blend mode depends on it only in the reference's own traversal. */
+/* Fetch counters for the algorithmic-bytes model (per calling thread):
+ * 0 paths, 1 closest rays, 2 closest interior visits (64-byte child pairs),
+ * 3 closest triangle tests, 4 shadow rays, 5 shadow node visits (32 bytes),
+ * 6 shadow triangle tests, 7 surface vertices, 8 environment NEE samples,
+ * 9 escaped rays, 10 light NEE samples. */
+static _Thread_local double *g_counts = NULL;
+#define COUNT(i, v) \
+  do {                \
+    if (g_counts) g_counts[i] += (v); \
+  } while (0)
+
+/* Closest accepted triangle (t in (EPS, closest)) in the reference shader's
+ * visiting order (scene.glsl:321-376: root box untested, near child first);
+ * the FP64 slab is inflated so rounding never hides a hit, and the result is
+ * the lowest t found, whatever the order. */
 static int bvh_closest(const or_scene *s, d3 ro, d3 rd, double *closest, double *bu, double *bv, rng_t *r) {
   if (s->tri_count == 0) return -1;
+  COUNT(1, 1);
   d3 inv = V(1 / rd.x, 1 / rd.y, 1 / rd.z);
   uint32_t stack[128];
   int sp = 0, best = -1;
-  stack[sp++] = 0;
-  while (sp > 0) {
-    uint32_t n = stack[--sp];
-    const float *nd = s->nodes + (size_t)n * 8;
-    if (slab(nd, ro, inv, *closest) == INF_D) continue;
+  uint32_t node = 0;
+  for (;;) {
+    const float *nd = s->nodes + (size_t)node * 8;
     uint32_t first = (uint32_t)nd[3], count = (uint32_t)nd[7];
     if (count > 0 || s->node_count == 1) {
+      COUNT(3, count);
       for (uint32_t k = first; k < first + count; k++) {
         const float *g = s->geo + (size_t)k * 12;
         double t, u, v;
@@ -318,10 +329,31 @@ static int bvh_closest(const or_scene *s, d3 ro, d3 rd, double *closest, double 
           *bv = v;
         }
       }
+      if (sp == 0) break;
+      node = stack[--sp];
+      continue;
+    }
+    COUNT(2, 1);
+    uint32_t left = first, right = first + 1;
+    double tl = slab(s->nodes + (size_t)left * 8, ro, inv, *closest);
+    double tr = slab(s->nodes + (size_t)right * 8, ro, inv, *closest);
+    if (tl > tr) {
+      double tt = tl;
+      tl = tr;
+      tr = tt;
+      uint32_t ti = left;
+      left = right;
+      right = ti;
+    }
+    if (tl == INF_D) {
+      if (sp == 0) break;
+      node = stack[--sp];
     } else {
-      if (sp + 2 > 128) return -2;
-      stack[sp++] = first + 1;
-      stack[sp++] = first;
+      node = left;
+      if (tr != INF_D) {
+        if (sp >= 128) return -2;
+        stack[sp++] = right;
+      }
     }
   }
   return best;
@@ -331,6 +363,7 @@ static int occluded(const or_scene *s, d3 ro, d3 rd, double tmax, rng_t *r) {
     if (hit_sphere(ro, rd, s->spheres + 4 * i, tmax) < tmax) return 1;
   if (hit_ground(s, ro, rd, tmax) < tmax) return 1;
   if (s->tri_count == 0) return 0;
+  COUNT(4, 1);
   d3 inv = V(1 / rd.x, 1 / rd.y, 1 / rd.z);
   uint32_t stack[128];
   int sp = 0;
@@ -338,10 +371,12 @@ static int occluded(const or_scene *s, d3 ro, d3 rd, double tmax, rng_t *r) {
   while (sp > 0) {
     uint32_t n = stack[--sp];
     const float *nd = s->nodes + (size_t)n * 8;
+    COUNT(5, 1);
     if (slab(nd, ro, inv, tmax) == INF_D) continue;
     uint32_t first = (uint32_t)nd[3], count = (uint32_t)nd[7];
     if (count > 0 || s->node_count == 1) {
       for (uint32_t k = first; k < first + count; k++) {
+        COUNT(6, 1);
         const float *g = s->geo + (size_t)k * 12;
         double t, u, v;
         if (hit_tri(ro, rd, Vf(g), Vf(g + 4), Vf(g + 8), tmax, &t, &u, &v) &&
@@ -830,6 +865,7 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
   for (int bounce = 0; bounce <= (int)f->max_bounces; bounce++) {
     hit_t h;
     if (!trace_scene(sc, ro, rd, &h, r)) {
+      COUNT(9, 1);
       d3 env = (bounce == 0 && sc->env_background == 0) ? Vf(sc->backdrop) : environment(sc, rd, spec, bpdf);
       rad = add(rad, clamp_c(mul(thr, env), bounce, f->clamp));
       if (!g->done) {
@@ -839,6 +875,7 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
       }
       break;
     }
+    COUNT(7, 1);
     if (bounce == 0) {
       g->depth = h.t;
       g->coverage = 1;
@@ -895,6 +932,7 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
         }
       }
       if (sc->env_sampling) {
+        COUNT(8, 1);
         lsample ls = sample_env(sc, r);
         double ndl = dot(n, ls.dir);
         if (ls.pdf > 0 && ndl > 0 && dot(ng, ls.dir) > 0 && !occluded(sc, so, ls.dir, INF_D, r)) {
@@ -905,6 +943,7 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
         }
       }
       if (sc->light_count > 0) {
+        COUNT(10, 1);
         int li = (int)((float)rnd(r) * (float)sc->light_count);
         if (li > (int)sc->light_count - 1) li = (int)sc->light_count - 1;
         double u0 = rnd(r), u1 = rnd(r);
@@ -937,9 +976,12 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
 /* Traces `samples` samples (seeds seed0, seed0+1, ...) at each listed pixel
  * (x, y with rows bottom-up) and writes per pixel: mean rgb, mean lum^2,
  * mean albedo rgb, mean coverage, mean normal xyz, mean depth (12 doubles),
- * plus the per-sample radiance (3 doubles per sample) when per_sample != NULL. */
+ * plus the per-sample radiance (3 doubles per sample) when per_sample != NULL
+ * and the fetch counters (16 doubles, accumulated) when counts16 != NULL. */
 int or_trace_pixels(const or_scene *sc, const or_frame *f, uint32_t seed0, uint32_t samples, const uint32_t *xy,
-                    uint32_t npix, double *out12, double *per_sample) {
+                    uint32_t npix, double *out12, double *per_sample, double *counts16) {
+  g_counts = counts16;
+  if (counts16) counts16[0] += (double)npix * samples;
   for (uint32_t p = 0; p < npix; p++) {
     uint32_t x = xy[2 * p], y = xy[2 * p + 1];
     double acc[12] = {0};
@@ -973,6 +1015,7 @@ int or_trace_pixels(const or_scene *sc, const or_frame *f, uint32_t seed0, uint3
     }
     for (int i = 0; i < 12; i++) out12[(size_t)p * 12 + i] = acc[i] / samples;
   }
+  g_counts = NULL;
   return 0;
 }
 
