@@ -67,9 +67,31 @@ def build(verbose: bool = True) -> Path:
     _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
           f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, OUT)
+    build_node_addon()
     if verbose:
-        print(f"built {OUT} from {len(sources)} sources", file=sys.stderr)
+        print(f"built {OUT} from {len(sources)} sources (+ node/build/lucent_napi.node)", file=sys.stderr)
     return OUT
+
+
+NODE_DIR = PKG.parent / "node"
+NODE_INCLUDE = Path("/usr/include/node")
+
+
+def build_node_addon() -> Path | None:
+    """The N-API addon (node/build/lucent_napi.node), linked against the
+    library with a relative rpath so the tree can move as a whole."""
+    if not (NODE_INCLUDE / "node_api.h").exists():
+        print("node headers not found: skipping the N-API addon", file=sys.stderr)
+        return None
+    out_dir = NODE_DIR / "build"
+    out_dir.mkdir(exist_ok=True)
+    out = out_dir / "lucent_napi.node"
+    tmp = out.with_suffix(".node.tmp")
+    _run(["gcc", "-O2", "-shared", "-fPIC", "-std=c11", "-Wall", f"-I{NODE_INCLUDE}",
+          "-DNODE_GYP_MODULE_NAME=lucent_napi", str(NODE_DIR / "lucent_napi.c"), "-o", str(tmp),
+          f"-L{PKG}", "-llucent_hip", "-Wl,-rpath,$ORIGIN/../../lucent_amd"])
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

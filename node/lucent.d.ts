@@ -1,0 +1,38 @@
+// Types of node/lucent.js (the Node host facade over the N-API addon).
+export interface BVH {
+  nodes: Float32Array;
+  nodeCount: number;
+  triIndices: Uint32Array;
+  bounds: { min: [number, number, number]; max: [number, number, number] };
+}
+export interface BVHOptions { maxLeafSize?: number; bins?: number; }
+export interface BVHHit { t: number; triIndex: number; u: number; v: number; }
+export interface HitBatch { t: Float64Array; triIndex: Uint32Array; u: Float64Array; v: Float64Array; }
+export interface Ray { origin: number[]; dir: number[]; }
+export declare const MISS: number;
+export declare function version(): string;
+export declare function buildBVH(positions: Float32Array, options?: BVHOptions): BVH;
+export declare class GpuTraversal {
+  constructor(bvh: BVH, positions: Float32Array, device?: number);
+  intersectBatch(origins: Float32Array | Float64Array, dirs: Float32Array | Float64Array, tMin?: number,
+    tMax?: number, opts?: { anyHit?: boolean; glsl?: boolean }): HitBatch;
+  intersect(ray: Ray, tMin?: number, tMax?: number): BVHHit | null;
+}
+export interface Camera { position: number[]; target: number[]; fov: number; aperture: number; focusDistance: number; }
+export interface FilmTargets { color: Float32Array; albedo: Float32Array; normal: Float32Array; history: Float32Array; }
+export declare class LucentRenderer {
+  constructor(opts: { width: number; height: number; device?: number });
+  settings: { maxBounces: number; fireflyClamp: number };
+  loadScene(packed: object, scene: object): void;
+  uploadLighting(scene: object): void;
+  uploadCamera(cam: Camera): void;
+  reset(): void;
+  traceSamples(count: number): void;
+  benchmark(samples: number): { traceMs: number; mpathsPerSec: number; denoiseMs: number; width: number; height: number };
+  runDenoiser(params?: { sigmaL: number; sigmaN: number; sigmaP: number; iterations: number }):
+    { illumVar: Float32Array; radiance: Float32Array };
+  read(): FilmTargets;
+  stats(): { samples: number; width: number; height: number; seed: number };
+  pickFocus(traversal: GpuTraversal, u: number, v: number): number | null;
+  dispose(): void;
+}

@@ -1,0 +1,191 @@
+// @ts-check
+'use strict';
+/**
+ * Node host facade of the MI355X path-tracing library (CommonJS, runs on
+ * Node 12).  It mirrors the reference's TypeScript entry points on this
+ * path so that a caller swaps implementations, not call sites:
+ *   buildBVH(positions, options)          src/math/bvh.ts:52
+ *   GpuTraversal#intersect / pickDistance src/math/bvh.ts:265, src/scene/scene.ts:327
+ *   LucentRenderer                        src/renderer.ts:171 (loadScene, uploadLighting,
+ *                                         uploadCamera, reset, traceSamples, benchmark,
+ *                                         runDenoiser, stats, dispose)
+ * Types are in lucent.d.ts.
+ */
+const path = require('path');
+
+/** @type {any} */
+const addon = require(path.join(__dirname, 'build', 'lucent_napi.node'));
+
+const QUERY_ANY = 1;
+const PREC_GLSL_FP32 = 2;
+const MISS = 0xffffffff;
+
+/**
+ * buildBVH (src/math/bvh.ts:52): binned SAH, same node layout and leaf order.
+ * @param {Float32Array} positions 9 floats per triangle
+ * @param {{maxLeafSize?: number, bins?: number}} [options]
+ */
+function buildBVH(positions, options) {
+  return addon.buildBVH(positions, options || {});
+}
+
+/** Batched closest-hit queries against one uploaded BVH (intersectBVH on the GPU). */
+class GpuTraversal {
+  /**
+   * @param {{nodes: Float32Array, triIndices: Uint32Array}} bvh
+   * @param {Float32Array} positions
+   * @param {number} [device]
+   */
+  constructor(bvh, positions, device) {
+    this.ctx = addon.createContext(device || 0);
+    this.scene = addon.sceneFromBVH(this.ctx, positions, bvh);
+  }
+
+  /**
+   * Exact FP64 hit records for many rays (intersectBVH semantics, bit for bit).
+   * @param {Float32Array|Float64Array} origins xyz per ray
+   * @param {Float32Array|Float64Array} dirs xyz per ray
+   * @param {number} [tMin]
+   * @param {number} [tMax]
+   * @param {{anyHit?: boolean, glsl?: boolean}} [opts]
+   */
+  intersectBatch(origins, dirs, tMin, tMax, opts) {
+    const o = opts || {};
+    const flags = (o.anyHit ? QUERY_ANY : 0) | (o.glsl ? PREC_GLSL_FP32 : 0);
+    return addon.intersect(this.ctx, this.scene, origins, dirs, tMin === undefined ? 1e-5 : tMin,
+      tMax === undefined ? Infinity : tMax, flags);
+  }
+
+  /**
+   * intersectBVH(bvh, positions, ray, tMin, tMax) for one ray: a BVHHit or null.
+   * @param {{origin: number[], dir: number[]}} ray
+   * @param {number} [tMin]
+   * @param {number} [tMax]
+   */
+  intersect(ray, tMin, tMax) {
+    const r = this.intersectBatch(Float64Array.from(ray.origin), Float64Array.from(ray.dir), tMin, tMax);
+    if (r.triIndex[0] === MISS) return null;
+    return { t: r.t[0], triIndex: r.triIndex[0], u: r.u[0], v: r.v[0] };
+  }
+}
+
+/** Camera basis of Renderer (src/renderer.ts:156-161), for pickFocus. */
+function cameraBasis(cam) {
+  const sub = (a, b) => [a[0] - b[0], a[1] - b[1], a[2] - b[2]];
+  const cross = (a, b) => [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]];
+  const norm = (a) => { const l = Math.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); return l === 0 ? [0, 0, 0] : [a[0] / l, a[1] / l, a[2] / l]; };
+  const forward = norm(sub(cam.target, cam.position));
+  const worldUp = Math.abs(forward[1]) > 0.999 ? [0, 0, 1] : [0, 1, 0];
+  const right = norm(cross(forward, worldUp));
+  return { forward, right, up: cross(right, forward) };
+}
+
+/**
+ * The progressive renderer (Renderer, src/renderer.ts:171) on the GPU:
+ * accumulation film, one launch per sample, SVGF-style denoiser.
+ */
+class LucentRenderer {
+  /** @param {{width: number, height: number, device?: number}} opts */
+  constructor(opts) {
+    this.ctx = addon.createContext(opts.device || 0);
+    this.width = opts.width;
+    this.height = opts.height;
+    this.film = addon.createFilm(this.ctx, opts.width, opts.height);
+    /** RenderSettings (renderer.ts:42-84), integrator part. */
+    this.settings = { maxBounces: 8, fireflyClamp: 8 };
+    this.scene = null;
+    this.camera = null;
+    this.seed = 0;
+  }
+
+  /**
+   * loadScene (renderer.ts:279): `packed` is the reference's PackedScene
+   * (packScene output), `scene` the Scene it came from (camera, lighting, ground).
+   */
+  loadScene(packed, scene) {
+    const p = Object.assign({}, packed, {
+      groundMaterial: scene.ground && scene.ground.enabled ? packed.groundMaterial : -1,
+      groundHeight: scene.ground ? scene.ground.height : 0,
+    });
+    this.scene = addon.uploadScene(this.ctx, p);
+    this.uploadLighting(scene);
+    this.uploadCamera(scene.camera);
+  }
+
+  /** uploadLighting (renderer.ts:355): {sun, sky, env}; env.map = {width, height, data} or null. */
+  uploadLighting(scene) {
+    addon.setLighting(this.ctx, this.scene, { sun: scene.sun, sky: scene.sky, env: scene.env });
+    this.reset();
+  }
+
+  /** uploadCamera (renderer.ts:424). */
+  uploadCamera(cam) {
+    this.camera = cam;
+    this.reset();
+  }
+
+  /** reset (renderer.ts:463): the RNG stream keeps advancing. */
+  reset() {
+    addon.resetFilm(this.ctx, this.film);
+  }
+
+  /** traceSamples (renderer.ts:702): `count` progressive samples. */
+  traceSamples(count) {
+    addon.traceSamples(this.ctx, this.scene, this.film, this.camera, this.settings, count);
+  }
+
+  /** benchmark (renderer.ts:745): warm-up sample, then `samples` timed. */
+  benchmark(samples) {
+    this.reset();
+    this.traceSamples(1);
+    let t0 = process.hrtime.bigint();
+    this.traceSamples(samples);
+    const traceMs = Number(process.hrtime.bigint() - t0) / 1e6;
+    t0 = process.hrtime.bigint();
+    this.runDenoiser();
+    const denoiseMs = Number(process.hrtime.bigint() - t0) / 1e6;
+    return { traceMs, mpathsPerSec: (samples * this.width * this.height) / (traceMs / 1000) / 1e6, denoiseMs,
+      width: this.width, height: this.height };
+  }
+
+  /** runDenoiser (renderer.ts:788): sigmaL 4, sigmaN 64, sigmaP 1.5 by default. */
+  runDenoiser(params) {
+    return addon.denoise(this.ctx, this.film, this.camera, params || { sigmaL: 4, sigmaN: 64, sigmaP: 1.5, iterations: 0 });
+  }
+
+  /** The accumulation targets (rows bottom to top, like readPixels). */
+  read() {
+    return addon.readFilm(this.ctx, this.film);
+  }
+
+  /** stats (renderer.ts:832), the fields this path knows. */
+  stats() {
+    const f = addon.filmInfo(this.film);
+    return { samples: f.samples, width: f.width, height: f.height, seed: f.seed };
+  }
+
+  /**
+   * pickFocus (renderer.ts:499) for a pixel of the film: the focus distance
+   * along the view axis of the nearest triangle hit, or null.
+   */
+  pickFocus(traversal, u, v) {
+    const cam = this.camera;
+    const { forward, right, up } = cameraBasis(cam);
+    const tanHalf = Math.tan((cam.fov * Math.PI) / 360);
+    const aspect = this.width / this.height;
+    const d = [0, 1, 2].map((i) => forward[i] + right[i] * u * aspect * tanHalf + up[i] * v * tanHalf);
+    const l = Math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const dir = [d[0] / l, d[1] / l, d[2] / l];
+    const hit = traversal.intersect({ origin: cam.position, dir });
+    if (!hit) return null;
+    return hit.t * (dir[0] * forward[0] + dir[1] * forward[1] + dir[2] * forward[2]);
+  }
+
+  dispose() {
+    this.scene = null;
+    this.film = null;
+    this.ctx = null;
+  }
+}
+
+module.exports = { buildBVH, GpuTraversal, LucentRenderer, cameraBasis, version: addon.version, addon, MISS };

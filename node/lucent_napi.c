@@ -1,0 +1,545 @@
+/*
+ * N-API addon over the C ABI of include/lucent_hip.h: the binding a Node
+ * host (Lucent's TypeScript, run as JavaScript) uses to drive the gfx950
+ * kernels.  TypedArray arguments are passed to the library zero-copy
+ * (napi_get_typedarray_info); results come back as new TypedArrays.  A
+ * failed call throws a JavaScript Error carrying lucent_last_error(), as the
+ * reference throws Error/GLError.  Handles are externals whose finalisers
+ * release the device objects; a scene or film keeps its context alive.
+ */
+#include <node_api.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/lucent_hip.h"
+
+#define CHECK_NAPI(call)                                  \
+  do {                                                    \
+    if ((call) != napi_ok) {                              \
+      napi_throw_error(env, NULL, "N-API call failed: " #call); \
+      return NULL;                                        \
+    }                                                     \
+  } while (0)
+
+static napi_value throw_status(napi_env env, lucent_status st) {
+  char code[32];
+  snprintf(code, sizeof code, "LUCENT_%d", (int)st);
+  napi_throw_error(env, code, lucent_last_error(NULL));
+  return NULL;
+}
+#define CHECK_LUCENT(call)                      \
+  do {                                          \
+    lucent_status st_ = (call);                 \
+    if (st_ != LUCENT_OK) return throw_status(env, st_); \
+  } while (0)
+
+/* ---- handles ------------------------------------------------------------ */
+typedef struct {
+  lucent_ctx *ctx;
+} ctx_holder;
+typedef struct {
+  lucent_scene *scene;
+  napi_ref ctx_ref;
+} scene_holder;
+typedef struct {
+  lucent_film *film;
+  napi_ref ctx_ref;
+} film_holder;
+
+static void ctx_finalize(napi_env env, void *data, void *hint) {
+  ctx_holder *h = (ctx_holder *)data;
+  if (h->ctx) lucent_destroy(h->ctx);
+  free(h);
+}
+static void scene_finalize(napi_env env, void *data, void *hint) {
+  scene_holder *h = (scene_holder *)data;
+  if (h->scene) lucent_scene_destroy(h->scene);
+  napi_delete_reference(env, h->ctx_ref);
+  free(h);
+}
+static void film_finalize(napi_env env, void *data, void *hint) {
+  film_holder *h = (film_holder *)data;
+  if (h->film) lucent_film_destroy(h->film);
+  napi_delete_reference(env, h->ctx_ref);
+  free(h);
+}
+
+static void *get_external(napi_env env, napi_value v) {
+  void *p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return NULL;
+  return p;
+}
+static lucent_ctx *get_ctx(napi_env env, napi_value v) {
+  ctx_holder *h = (ctx_holder *)get_external(env, v);
+  return h ? h->ctx : NULL;
+}
+
+/* ---- argument helpers --------------------------------------------------- */
+static int typed(napi_env env, napi_value v, napi_typedarray_type want, void **data, size_t *len) {
+  bool is = false;
+  if (napi_is_typedarray(env, v, &is) != napi_ok || !is) return 0;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  if (napi_get_typedarray_info(env, v, &t, len, data, &ab, &off) != napi_ok) return 0;
+  return t == want;
+}
+static napi_value prop(napi_env env, napi_value obj, const char *name) {
+  napi_value v = NULL;
+  napi_get_named_property(env, obj, name, &v);
+  return v;
+}
+static double num(napi_env env, napi_value v, double dflt) {
+  napi_valuetype t;
+  if (!v || napi_typeof(env, v, &t) != napi_ok) return dflt;
+  if (t == napi_boolean) {
+    bool b;
+    napi_get_value_bool(env, v, &b);
+    return b ? 1 : 0;
+  }
+  if (t != napi_number) return dflt;
+  double d;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+static void vec3(napi_env env, napi_value arr, double *out, double dflt) {
+  for (int i = 0; i < 3; i++) {
+    napi_value e = NULL;
+    out[i] = dflt;
+    if (arr && napi_get_element(env, arr, i, &e) == napi_ok) out[i] = num(env, e, dflt);
+  }
+}
+static napi_value new_typed(napi_env env, napi_typedarray_type t, size_t n, size_t esz, void **data) {
+  napi_value ab, ta;
+  if (napi_create_arraybuffer(env, n * esz, data, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, t, n, ab, 0, &ta) != napi_ok) return NULL;
+  return ta;
+}
+static napi_value make_number(napi_env env, double d) {
+  napi_value v;
+  napi_create_double(env, d, &v);
+  return v;
+}
+
+/* ---- exports ------------------------------------------------------------ */
+static napi_value js_version(napi_env env, napi_callback_info info) {
+  napi_value v;
+  CHECK_NAPI(napi_create_string_utf8(env, lucent_version(), NAPI_AUTO_LENGTH, &v));
+  return v;
+}
+
+/* buildBVH(positions: Float32Array, {maxLeafSize, bins}) — src/math/bvh.ts:52. */
+static napi_value js_build_bvh(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  void *pos;
+  size_t len;
+  if (argc < 1 || !typed(env, argv[0], napi_float32_array, &pos, &len)) {
+    napi_throw_type_error(env, NULL, "buildBVH: positions must be a Float32Array");
+    return NULL;
+  }
+  if (len % 9 != 0) {
+    char msg[96];
+    snprintf(msg, sizeof msg, "positions length %zu is not a multiple of 9", len);
+    napi_throw_error(env, NULL, msg);
+    return NULL;
+  }
+  uint32_t max_leaf = 4, bins = 12;
+  if (argc > 1) {
+    max_leaf = (uint32_t)num(env, prop(env, argv[1], "maxLeafSize"), 4);
+    bins = (uint32_t)num(env, prop(env, argv[1], "bins"), 12);
+  }
+  uint32_t tris = (uint32_t)(len / 9);
+  size_t cap = (tris > 0 ? 2 * (size_t)tris - 1 : 1) * 8;
+  float *nodes = (float *)malloc(cap * sizeof(float));
+  void *idx_data;
+  napi_value idx = new_typed(env, napi_uint32_array, tris, 4, &idx_data);
+  uint32_t count = 0;
+  double bounds[6];
+  lucent_status st = lucent_build_bvh((const float *)pos, tris, max_leaf, bins, nodes, &count, (uint32_t *)idx_data,
+                                      bounds);
+  if (st != LUCENT_OK) {
+    free(nodes);
+    return throw_status(env, st);
+  }
+  void *nd;
+  napi_value nodes_ta = new_typed(env, napi_float32_array, (size_t)count * 8, 4, &nd);
+  memcpy(nd, nodes, (size_t)count * 8 * sizeof(float));
+  free(nodes);
+  napi_value out, bmin, bmax, b;
+  CHECK_NAPI(napi_create_object(env, &out));
+  CHECK_NAPI(napi_set_named_property(env, out, "nodes", nodes_ta));
+  CHECK_NAPI(napi_set_named_property(env, out, "nodeCount", make_number(env, count)));
+  CHECK_NAPI(napi_set_named_property(env, out, "triIndices", idx));
+  CHECK_NAPI(napi_create_array_with_length(env, 3, &bmin));
+  CHECK_NAPI(napi_create_array_with_length(env, 3, &bmax));
+  for (int i = 0; i < 3; i++) {
+    napi_set_element(env, bmin, i, make_number(env, bounds[i]));
+    napi_set_element(env, bmax, i, make_number(env, bounds[3 + i]));
+  }
+  CHECK_NAPI(napi_create_object(env, &b));
+  CHECK_NAPI(napi_set_named_property(env, b, "min", bmin));
+  CHECK_NAPI(napi_set_named_property(env, b, "max", bmax));
+  CHECK_NAPI(napi_set_named_property(env, out, "bounds", b));
+  return out;
+}
+
+static napi_value js_create_context(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  int device = argc > 0 ? (int)num(env, argv[0], 0) : 0;
+  lucent_ctx *ctx = NULL;
+  CHECK_LUCENT(lucent_create(device, &ctx));
+  ctx_holder *h = (ctx_holder *)calloc(1, sizeof *h);
+  h->ctx = ctx;
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, h, ctx_finalize, NULL, &ext));
+  return ext;
+}
+
+static napi_value wrap_scene(napi_env env, napi_value ctx_v, lucent_scene *s) {
+  scene_holder *h = (scene_holder *)calloc(1, sizeof *h);
+  h->scene = s;
+  napi_create_reference(env, ctx_v, 1, &h->ctx_ref);
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, h, scene_finalize, NULL, &ext));
+  return ext;
+}
+
+/* sceneFromBVH(ctx, positions, bvh) — the arguments of intersectBVH. */
+static napi_value js_scene_from_bvh(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  void *pos, *nodes, *idx;
+  size_t plen, nlen, ilen;
+  if (!ctx || argc < 3 || !typed(env, argv[1], napi_float32_array, &pos, &plen) ||
+      !typed(env, prop(env, argv[2], "nodes"), napi_float32_array, &nodes, &nlen) ||
+      !typed(env, prop(env, argv[2], "triIndices"), napi_uint32_array, &idx, &ilen)) {
+    napi_throw_type_error(env, NULL, "sceneFromBVH(ctx, positions: Float32Array, bvh: BVH)");
+    return NULL;
+  }
+  lucent_scene *s = NULL;
+  CHECK_LUCENT(lucent_scene_from_bvh(ctx, (const float *)pos, (uint32_t)(plen / 9), (const float *)nodes,
+                                     (uint32_t)(nlen / 8), (const uint32_t *)idx, &s));
+  return wrap_scene(env, argv[0], s);
+}
+
+/* uploadScene(ctx, packed: PackedScene) — Renderer.loadScene (renderer.ts:279-312). */
+static napi_value js_upload_scene(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  if (!ctx || argc < 2) {
+    napi_throw_type_error(env, NULL, "uploadScene(ctx, packed)");
+    return NULL;
+  }
+  napi_value p = argv[1];
+  lucent_packed_scene ps;
+  memset(&ps, 0, sizeof ps);
+  void *d;
+  size_t n;
+  if (!typed(env, prop(env, p, "triGeometry"), napi_float32_array, &d, &n)) goto bad;
+  ps.tri_geometry = (const float *)d;
+  ps.tri_count = (uint32_t)num(env, prop(env, p, "triCount"), (double)(n / 12));
+  if (typed(env, prop(env, p, "triAttributes"), napi_float32_array, &d, &n)) ps.tri_attributes = (const float *)d;
+  if (!typed(env, prop(env, p, "bvhNodes"), napi_float32_array, &d, &n)) goto bad;
+  ps.bvh_nodes = (const float *)d;
+  ps.bvh_node_count = (uint32_t)num(env, prop(env, p, "bvhNodeCount"), (double)(n / 8));
+  {
+    napi_value cpu = prop(env, p, "cpu"), bvh = cpu ? prop(env, cpu, "bvh") : NULL;
+    if (bvh && typed(env, prop(env, bvh, "triIndices"), napi_uint32_array, &d, &n)) ps.tri_indices = (const uint32_t *)d;
+  }
+  if (!typed(env, prop(env, p, "materials"), napi_float32_array, &d, &n)) goto bad;
+  ps.materials = (const float *)d;
+  ps.material_count = (uint32_t)num(env, prop(env, p, "materialCount"), (double)(n / 40));
+  if (typed(env, prop(env, p, "spheres"), napi_float32_array, &d, &n)) ps.spheres = (const float *)d;
+  if (typed(env, prop(env, p, "sphereMaterials"), napi_int32_array, &d, &n)) ps.sphere_materials = (const int32_t *)d;
+  if (typed(env, prop(env, p, "sphereLights"), napi_int32_array, &d, &n)) ps.sphere_lights = (const int32_t *)d;
+  ps.sphere_count = (uint32_t)num(env, prop(env, p, "sphereCount"), 0);
+  if (typed(env, prop(env, p, "lights"), napi_float32_array, &d, &n)) ps.lights = (const float *)d;
+  ps.light_count = (uint32_t)num(env, prop(env, p, "lightCount"), 0);
+  ps.ground_material = (int32_t)num(env, prop(env, p, "groundMaterial"), -1);
+  ps.ground_height = (float)num(env, prop(env, p, "groundHeight"), 0);
+  {
+    lucent_scene *s = NULL;
+    CHECK_LUCENT(lucent_scene_upload(ctx, &ps, &s));
+    return wrap_scene(env, argv[0], s);
+  }
+bad:
+  napi_throw_type_error(env, NULL, "uploadScene: packed scene arrays missing or of the wrong type");
+  return NULL;
+}
+
+/* setLighting(ctx, scene, {sun, sky, env}) — Renderer.uploadLighting (renderer.ts:355-413). */
+static napi_value js_set_lighting(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  scene_holder *sh = argc > 1 ? (scene_holder *)get_external(env, argv[1]) : NULL;
+  if (!ctx || !sh || argc < 3) {
+    napi_throw_type_error(env, NULL, "setLighting(ctx, scene, {sun, sky, env})");
+    return NULL;
+  }
+  lucent_lighting L;
+  memset(&L, 0, sizeof L);
+  napi_value sun = prop(env, argv[2], "sun"), sky = prop(env, argv[2], "sky"), e = prop(env, argv[2], "env");
+  if (sun) {
+    L.sun.elevation = num(env, prop(env, sun, "elevation"), 45);
+    L.sun.azimuth = num(env, prop(env, sun, "azimuth"), 0);
+    vec3(env, prop(env, sun, "color"), L.sun.color, 1);
+    L.sun.intensity = num(env, prop(env, sun, "intensity"), 0);
+    L.sun.angular_radius = num(env, prop(env, sun, "angularRadius"), 0.27);
+  }
+  if (sky) {
+    L.sky.enabled = (int32_t)num(env, prop(env, sky, "enabled"), 0);
+    vec3(env, prop(env, sky, "zenith"), L.sky.zenith, 0);
+    vec3(env, prop(env, sky, "horizon"), L.sky.horizon, 0);
+    vec3(env, prop(env, sky, "ground"), L.sky.ground, 0);
+    L.sky.intensity = num(env, prop(env, sky, "intensity"), 0);
+  }
+  if (e) {
+    napi_value map = prop(env, e, "map");
+    napi_valuetype t = napi_null;
+    if (map) napi_typeof(env, map, &t);
+    if (t == napi_object) {
+      void *d;
+      size_t n;
+      if (!typed(env, prop(env, map, "data"), napi_float32_array, &d, &n)) {
+        napi_throw_type_error(env, NULL, "env.map.data must be a Float32Array");
+        return NULL;
+      }
+      L.env.map_rgb = (const float *)d;
+      L.env.width = (uint32_t)num(env, prop(env, map, "width"), 0);
+      L.env.height = (uint32_t)num(env, prop(env, map, "height"), 0);
+      if ((size_t)L.env.width * L.env.height * 3 != n) {
+        napi_throw_error(env, NULL, "env.map: width * height * 3 != data.length");
+        return NULL;
+      }
+    }
+    L.env.rotation = num(env, prop(env, e, "rotation"), 0);
+    L.env.intensity = num(env, prop(env, e, "intensity"), 1);
+    L.env.background = (int32_t)num(env, prop(env, e, "background"), 1);
+    vec3(env, prop(env, e, "backdrop"), L.env.backdrop, 0);
+  }
+  CHECK_LUCENT(lucent_scene_set_lighting(ctx, sh->scene, &L));
+  return NULL;
+}
+
+/* intersect(ctx, scene, origins, dirs, tMin, tMax, flags) — intersectBVH over a batch. */
+static napi_value js_intersect(napi_env env, napi_callback_info info) {
+  size_t argc = 7;
+  napi_value argv[7];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  scene_holder *sh = argc > 1 ? (scene_holder *)get_external(env, argv[1]) : NULL;
+  void *o, *d;
+  size_t on, dn;
+  int f64 = 0;
+  if (!ctx || !sh || argc < 4) goto bad;
+  if (typed(env, argv[2], napi_float64_array, &o, &on) && typed(env, argv[3], napi_float64_array, &d, &dn))
+    f64 = 1;
+  else if (!(typed(env, argv[2], napi_float32_array, &o, &on) && typed(env, argv[3], napi_float32_array, &d, &dn)))
+    goto bad;
+  if (on != dn || on % 3) goto bad;
+  {
+    double tmin = argc > 4 ? num(env, argv[4], 1e-5) : 1e-5;
+    double tmax = argc > 5 ? num(env, argv[5], 1.0 / 0.0) : 1.0 / 0.0;
+    uint32_t flags = argc > 6 ? (uint32_t)num(env, argv[6], 0) : 0;
+    size_t n = on / 3;
+    void *t, *u, *v, *id;
+    napi_value ta = new_typed(env, napi_float64_array, n, 8, &t), ua = new_typed(env, napi_float64_array, n, 8, &u),
+               va = new_typed(env, napi_float64_array, n, 8, &v), ia = new_typed(env, napi_uint32_array, n, 4, &id);
+    if (f64)
+      CHECK_LUCENT(lucent_intersect_f64(ctx, sh->scene, (const double *)o, (const double *)d, (uint32_t)n, tmin, tmax,
+                                        flags, (double *)t, (uint32_t *)id, (double *)u, (double *)v));
+    else
+      CHECK_LUCENT(lucent_intersect(ctx, sh->scene, (const float *)o, (const float *)d, (uint32_t)n, tmin, tmax, flags,
+                                    (double *)t, (uint32_t *)id, (double *)u, (double *)v));
+    napi_value out;
+    CHECK_NAPI(napi_create_object(env, &out));
+    napi_set_named_property(env, out, "t", ta);
+    napi_set_named_property(env, out, "triIndex", ia);
+    napi_set_named_property(env, out, "u", ua);
+    napi_set_named_property(env, out, "v", va);
+    return out;
+  }
+bad:
+  napi_throw_type_error(env, NULL,
+                        "intersect(ctx, scene, origins, dirs, tMin, tMax, flags): origins/dirs are Float32Array or "
+                        "Float64Array of equal length 3n");
+  return NULL;
+}
+
+static napi_value js_create_film(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  if (!ctx || argc < 3) {
+    napi_throw_type_error(env, NULL, "createFilm(ctx, width, height)");
+    return NULL;
+  }
+  lucent_film *f = NULL;
+  CHECK_LUCENT(lucent_film_create(ctx, (uint32_t)num(env, argv[1], 0), (uint32_t)num(env, argv[2], 0), &f));
+  film_holder *h = (film_holder *)calloc(1, sizeof *h);
+  h->film = f;
+  napi_create_reference(env, argv[0], 1, &h->ctx_ref);
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, h, film_finalize, NULL, &ext));
+  return ext;
+}
+
+static napi_value js_reset_film(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  film_holder *fh = argc > 1 ? (film_holder *)get_external(env, argv[1]) : NULL;
+  if (!ctx || !fh) {
+    napi_throw_type_error(env, NULL, "resetFilm(ctx, film[, seed])");
+    return NULL;
+  }
+  CHECK_LUCENT(lucent_film_reset(ctx, fh->film));
+  if (argc > 2) CHECK_LUCENT(lucent_film_set_seed(fh->film, (uint32_t)num(env, argv[2], 0)));
+  return NULL;
+}
+
+static napi_value js_film_info(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  film_holder *fh = argc > 0 ? (film_holder *)get_external(env, argv[0]) : NULL;
+  if (!fh) {
+    napi_throw_type_error(env, NULL, "filmInfo(film)");
+    return NULL;
+  }
+  uint32_t w, h, n, s;
+  CHECK_LUCENT(lucent_film_info(fh->film, &w, &h, &n, &s));
+  napi_value out;
+  CHECK_NAPI(napi_create_object(env, &out));
+  napi_set_named_property(env, out, "width", make_number(env, w));
+  napi_set_named_property(env, out, "height", make_number(env, h));
+  napi_set_named_property(env, out, "samples", make_number(env, n));
+  napi_set_named_property(env, out, "seed", make_number(env, s));
+  return out;
+}
+
+static int read_camera(napi_env env, napi_value c, lucent_camera *cam) {
+  if (!c) return 0;
+  vec3(env, prop(env, c, "position"), cam->position, 0);
+  vec3(env, prop(env, c, "target"), cam->target, 0);
+  cam->fov = num(env, prop(env, c, "fov"), 45);
+  cam->aperture = num(env, prop(env, c, "aperture"), 0);
+  cam->focus_distance = num(env, prop(env, c, "focusDistance"), 1);
+  return 1;
+}
+
+/* traceSamples(ctx, scene, film, camera, {maxBounces, fireflyClamp}, count) — traceSamples (renderer.ts:702). */
+static napi_value js_trace_samples(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  scene_holder *sh = argc > 1 ? (scene_holder *)get_external(env, argv[1]) : NULL;
+  film_holder *fh = argc > 2 ? (film_holder *)get_external(env, argv[2]) : NULL;
+  lucent_camera cam;
+  if (!ctx || !sh || !fh || argc < 6 || !read_camera(env, argv[3], &cam)) {
+    napi_throw_type_error(env, NULL, "traceSamples(ctx, scene, film, camera, settings, count)");
+    return NULL;
+  }
+  lucent_render_settings rs;
+  rs.max_bounces = (uint32_t)num(env, prop(env, argv[4], "maxBounces"), 8);
+  rs.firefly_clamp = (float)num(env, prop(env, argv[4], "fireflyClamp"), 8);
+  CHECK_LUCENT(lucent_trace_samples(ctx, sh->scene, fh->film, &cam, &rs, (uint32_t)num(env, argv[5], 1), NULL));
+  CHECK_LUCENT(lucent_synchronize(ctx));
+  return NULL;
+}
+
+/* readFilm(ctx, film) -> {color, albedo, normal, history} (rows bottom to top). */
+static napi_value js_read_film(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  film_holder *fh = argc > 1 ? (film_holder *)get_external(env, argv[1]) : NULL;
+  if (!ctx || !fh) {
+    napi_throw_type_error(env, NULL, "readFilm(ctx, film)");
+    return NULL;
+  }
+  uint32_t w, h, n, s;
+  CHECK_LUCENT(lucent_film_info(fh->film, &w, &h, &n, &s));
+  size_t px = (size_t)w * h;
+  void *c, *a, *nm, *hi;
+  napi_value ca = new_typed(env, napi_float32_array, px * 4, 4, &c), aa = new_typed(env, napi_float32_array, px * 4, 4, &a),
+             na = new_typed(env, napi_float32_array, px * 4, 4, &nm), ha = new_typed(env, napi_float32_array, px, 4, &hi);
+  CHECK_LUCENT(lucent_film_read(ctx, fh->film, (float *)c, (float *)a, (float *)nm, (float *)hi));
+  napi_value out;
+  CHECK_NAPI(napi_create_object(env, &out));
+  napi_set_named_property(env, out, "color", ca);
+  napi_set_named_property(env, out, "albedo", aa);
+  napi_set_named_property(env, out, "normal", na);
+  napi_set_named_property(env, out, "history", ha);
+  return out;
+}
+
+/* denoise(ctx, film, camera, {sigmaL, sigmaN, sigmaP, iterations}) -> {illumVar, radiance} (runDenoiser). */
+static napi_value js_denoise(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  film_holder *fh = argc > 1 ? (film_holder *)get_external(env, argv[1]) : NULL;
+  lucent_camera cam;
+  if (!ctx || !fh || argc < 3 || !read_camera(env, argv[2], &cam)) {
+    napi_throw_type_error(env, NULL, "denoise(ctx, film, camera, params)");
+    return NULL;
+  }
+  lucent_denoise_params p = {4.0f, 64.0f, 1.5f, 0};
+  if (argc > 3) {
+    p.sigma_l = (float)num(env, prop(env, argv[3], "sigmaL"), 4);
+    p.sigma_n = (float)num(env, prop(env, argv[3], "sigmaN"), 64);
+    p.sigma_p = (float)num(env, prop(env, argv[3], "sigmaP"), 1.5);
+    p.iterations = (uint32_t)num(env, prop(env, argv[3], "iterations"), 0);
+  }
+  CHECK_LUCENT(lucent_denoise(ctx, fh->film, &cam, &p, NULL));
+  uint32_t w, h, n, s;
+  CHECK_LUCENT(lucent_film_info(fh->film, &w, &h, &n, &s));
+  size_t px = (size_t)w * h;
+  void *iv, *rad;
+  napi_value ia = new_typed(env, napi_float32_array, px * 4, 4, &iv), ra = new_typed(env, napi_float32_array, px * 4, 4, &rad);
+  CHECK_LUCENT(lucent_film_read_denoised(ctx, fh->film, (float *)iv, (float *)rad));
+  napi_value out;
+  CHECK_NAPI(napi_create_object(env, &out));
+  napi_set_named_property(env, out, "illumVar", ia);
+  napi_set_named_property(env, out, "radiance", ra);
+  return out;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  napi_property_descriptor props[] = {
+      {"version", 0, js_version, 0, 0, 0, napi_default, 0},
+      {"buildBVH", 0, js_build_bvh, 0, 0, 0, napi_default, 0},
+      {"createContext", 0, js_create_context, 0, 0, 0, napi_default, 0},
+      {"sceneFromBVH", 0, js_scene_from_bvh, 0, 0, 0, napi_default, 0},
+      {"uploadScene", 0, js_upload_scene, 0, 0, 0, napi_default, 0},
+      {"setLighting", 0, js_set_lighting, 0, 0, 0, napi_default, 0},
+      {"intersect", 0, js_intersect, 0, 0, 0, napi_default, 0},
+      {"createFilm", 0, js_create_film, 0, 0, 0, napi_default, 0},
+      {"resetFilm", 0, js_reset_film, 0, 0, 0, napi_default, 0},
+      {"filmInfo", 0, js_film_info, 0, 0, 0, napi_default, 0},
+      {"traceSamples", 0, js_trace_samples, 0, 0, 0, napi_default, 0},
+      {"readFilm", 0, js_read_film, 0, 0, 0, napi_default, 0},
+      {"denoise", 0, js_denoise, 0, 0, 0, napi_default, 0},
+  };
+  napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

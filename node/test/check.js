@@ -1,0 +1,53 @@
+// Test driver for the N-API facade (run by tests/test_node_addon.py).
+//   node check.js bvh <dir>     buildBVH over <dir>/positions.f32 -> nodes.f32, tri.u32
+//   node check.js hits <dir>    GpuTraversal over positions + bvh, rays org.f64/dir.f64 -> t/u/v.f64, tri.u32
+//   node check.js render <dir>  LucentRenderer over the packed scene in <dir> -> color.f32, albedo.f32
+'use strict';
+const fs = require('fs');
+const path = require('path');
+const lucent = require('../lucent.js');
+
+const [mode, dir] = process.argv.slice(2);
+const f32 = (n) => { const b = fs.readFileSync(path.join(dir, n)); return new Float32Array(b.buffer, b.byteOffset, b.length / 4); };
+const f64 = (n) => { const b = fs.readFileSync(path.join(dir, n)); return new Float64Array(b.buffer, b.byteOffset, b.length / 8); };
+const u32 = (n) => { const b = fs.readFileSync(path.join(dir, n)); return new Uint32Array(b.buffer, b.byteOffset, b.length / 4); };
+const i32 = (n) => { const b = fs.readFileSync(path.join(dir, n)); return new Int32Array(b.buffer, b.byteOffset, b.length / 4); };
+const write = (n, ta) => fs.writeFileSync(path.join(dir, n), Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength));
+
+if (mode === 'bvh') {
+  const bvh = lucent.buildBVH(f32('positions.f32'), { maxLeafSize: 4, bins: 12 });
+  write('nodes.f32', bvh.nodes);
+  write('tri.u32', bvh.triIndices);
+  let err = '';
+  try { lucent.buildBVH(new Float32Array(10)); } catch (e) { err = e.message; }
+  process.stdout.write(JSON.stringify({ nodeCount: bvh.nodeCount, bounds: bvh.bounds, error: err }) + '\n');
+} else if (mode === 'hits') {
+  const bvh = { nodes: f32('nodes.f32'), triIndices: u32('tri.u32') };
+  const tr = new lucent.GpuTraversal(bvh, f32('positions.f32'));
+  const r = tr.intersectBatch(f64('org.f64'), f64('dir.f64'));
+  write('t.f64', r.t); write('u.f64', r.u); write('v.f64', r.v); write('hit.u32', r.triIndex);
+  const one = tr.intersect({ origin: Array.from(f64('org.f64').subarray(0, 3)), dir: Array.from(f64('dir.f64').subarray(0, 3)) });
+  process.stdout.write(JSON.stringify({ first: one }) + '\n');
+} else if (mode === 'render') {
+  const meta = JSON.parse(fs.readFileSync(path.join(dir, 'meta.json'), 'utf8'));
+  const packed = {
+    triGeometry: f32('geo.f32'), triAttributes: f32('attr.f32'), triCount: meta.triCount,
+    bvhNodes: f32('nodes.f32'), bvhNodeCount: meta.nodeCount, materials: f32('mat.f32'),
+    materialCount: meta.materialCount, spheres: f32('sph.f32'), sphereMaterials: i32('sphm.i32'),
+    sphereLights: i32('sphl.i32'), sphereCount: meta.sphereCount, lights: f32('lights.f32'),
+    lightCount: meta.lightCount, groundMaterial: meta.groundMaterial, cpu: { bvh: { triIndices: u32('tri.u32') } },
+  };
+  const scene = Object.assign({}, meta.scene);
+  scene.env = Object.assign({}, meta.scene.env, { map: { width: meta.envW, height: meta.envH, data: f32('env.f32') } });
+  const r = new lucent.LucentRenderer({ width: meta.width, height: meta.height });
+  r.settings = { maxBounces: 8, fireflyClamp: 0 };
+  r.loadScene(packed, scene);
+  lucent.addon.resetFilm(r.ctx, r.film, meta.seed);
+  r.traceSamples(meta.spp);
+  const out = r.read();
+  write('color.f32', out.color);
+  write('albedo.f32', out.albedo);
+  const d = r.runDenoiser({ sigmaL: 4, sigmaN: 64, sigmaP: 1.5, iterations: 3 });
+  write('den.f32', d.illumVar);
+  process.stdout.write(JSON.stringify(r.stats()) + '\n');
+}

@@ -1,0 +1,114 @@
+"""The Node host path: the N-API addon and node/lucent.js facade over the same
+C ABI.  CPU: buildBVH through Node equals the product and oracle builders and
+throws the reference's Error on a bad length.  GPU: intersect through Node is
+bit-identical to the oracle, and a LucentRenderer render equals the Python
+binding's render of the same packed scene."""
+import json
+import math
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lucent_amd.bvh import build_bvh
+
+ROOT = Path(__file__).resolve().parents[1]
+CHECK = ROOT / "node" / "test" / "check.js"
+pytestmark = pytest.mark.skipif(shutil.which("node") is None or not (ROOT / "node/build/lucent_napi.node").exists(),
+                                reason="node or the built addon is missing")
+
+
+def node(mode, d):
+    r = subprocess.run(["node", str(CHECK), mode, str(d)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_build_bvh_through_node(tmp_path):
+    soup = O.random_soup(3000, O.Lcg(21))
+    soup.tofile(tmp_path / "positions.f32")
+    meta = node("bvh", tmp_path)
+    nodes = np.fromfile(tmp_path / "nodes.f32", np.float32)
+    tri = np.fromfile(tmp_path / "tri.u32", np.uint32)
+    ref = O.build_bvh(soup)
+    assert meta["nodeCount"] == ref.node_count
+    assert np.array_equal(nodes.view(np.uint32), ref.nodes.view(np.uint32))
+    assert np.array_equal(tri, ref.tri_indices)
+    assert np.array_equal(tri, build_bvh(soup).tri_indices)
+    assert meta["error"] == "positions length 10 is not a multiple of 9"
+
+
+@pytest.mark.gpu
+def test_intersect_through_node_bit_identical(tmp_path):
+    rand = O.Lcg(42)
+    soup = O.random_soup(2000, rand)
+    org = np.zeros((400, 3))
+    dirs = np.zeros((400, 3))
+    for i in range(400):
+        org[i] = [rand() * 16 - 8, rand() * 16 - 8, rand() * 16 - 8]
+        d = np.array([rand() - 0.5, rand() - 0.5, rand() - 0.5])
+        dirs[i] = d / math.sqrt(float(d @ d))
+    b = build_bvh(soup)
+    soup.tofile(tmp_path / "positions.f32")
+    b.nodes.tofile(tmp_path / "nodes.f32")
+    b.tri_indices.tofile(tmp_path / "tri.u32")
+    org.tofile(tmp_path / "org.f64")
+    dirs.tofile(tmp_path / "dir.f64")
+    meta = node("hits", tmp_path)
+    t, tri, u, v = O.intersect_bvh_batch(b, soup, org, dirs)
+    assert np.array_equal(np.fromfile(tmp_path / "hit.u32", np.uint32), tri)
+    for name, ref in (("t", t), ("u", u), ("v", v)):
+        got = np.fromfile(tmp_path / f"{name}.f64", np.float64)
+        assert np.array_equal(got, ref), name
+    if tri[0] != 0xFFFFFFFF:
+        assert meta["first"]["triIndex"] == int(tri[0]) and meta["first"]["t"] == t[0]
+
+
+@pytest.mark.gpu
+def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
+    import sys
+    sys.path.insert(0, str(Path(__file__).parent))
+    from test_pathtrace_gpu import feature_scene
+    from lucent_amd.renderer import Renderer, RenderSettings
+    from lucent_amd.scene import pack_scene
+    sc = feature_scene()
+    W, H, SPP, SEED = 48, 32, 3, 9
+    p = pack_scene(sc)
+    for name, arr in (("geo.f32", p.tri_geometry), ("attr.f32", p.tri_attributes), ("nodes.f32", p.bvh_nodes),
+                      ("mat.f32", p.materials), ("sph.f32", p.spheres), ("sphm.i32", p.sphere_materials),
+                      ("sphl.i32", p.sphere_lights), ("lights.f32", p.lights), ("tri.u32", p.tri_indices)):
+        np.ascontiguousarray(arr).tofile(tmp_path / name)
+    w, h, env = sc.env.map
+    np.asarray(env, np.float32).tofile(tmp_path / "env.f32")
+    scene_json = {
+        "camera": {"position": list(sc.camera.position), "target": list(sc.camera.target), "fov": sc.camera.fov,
+                   "aperture": sc.camera.aperture, "focusDistance": sc.camera.focus_distance},
+        "sun": {"elevation": sc.sun.elevation, "azimuth": sc.sun.azimuth, "color": list(sc.sun.color),
+                "intensity": sc.sun.intensity, "angularRadius": sc.sun.angular_radius},
+        "sky": {"enabled": sc.sky.enabled, "zenith": list(sc.sky.zenith), "horizon": list(sc.sky.horizon),
+                "ground": list(sc.sky.ground), "intensity": sc.sky.intensity},
+        "env": {"rotation": sc.env.rotation, "intensity": sc.env.intensity, "background": sc.env.background,
+                "backdrop": list(sc.env.backdrop)},
+        "ground": {"enabled": sc.ground.enabled, "height": sc.ground.height},
+    }
+    meta = {"triCount": p.tri_count, "nodeCount": p.bvh_node_count, "materialCount": p.material_count,
+            "sphereCount": p.sphere_count, "lightCount": p.light_count, "groundMaterial": p.ground_material,
+            "envW": w, "envH": h, "width": W, "height": H, "spp": SPP, "seed": SEED, "scene": scene_json}
+    (tmp_path / "meta.json").write_text(json.dumps(meta))
+    stats = node("render", tmp_path)
+    assert stats["samples"] == SPP and stats["seed"] == SEED + SPP
+    r = Renderer(ctx, W, H, RenderSettings(8, 0.0))
+    r.load_scene(sc)
+    r.film.set_seed(SEED)
+    r.reset()
+    r.trace_samples(SPP)
+    ref = r.film.read()
+    r.run_denoiser(4.0, 64.0, 1.5, 3)
+    den = r.read_denoised()
+    r.close()
+    assert np.array_equal(np.fromfile(tmp_path / "color.f32", np.float32).reshape(H, W, 4), ref["color"])
+    assert np.array_equal(np.fromfile(tmp_path / "albedo.f32", np.float32).reshape(H, W, 4), ref["albedo"])
+    assert np.array_equal(np.fromfile(tmp_path / "den.f32", np.float32).reshape(H, W, 4), den["illum_var"])
