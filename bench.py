@@ -316,6 +316,54 @@ def bench_denoise(args) -> dict:
             "cpu_baseline": cpu}
 
 
+def bench_adjoint(args) -> dict:
+    """Differentiable rendering: 512x512, 32 spp, depth 6, 16 displaced spheres
+    (20,480 tris), 16 materials, 256x128 env map.  One gradient evaluation =
+    forward render (32 launches) + path-replay backward (32 launches + FP64
+    reductions) of the L2 loss against a target rendered with ±20% perturbed
+    parameters; grad/s = 1 / (forward + backward time)."""
+    import torch
+
+    from lucent_amd import workloads
+    from lucent_amd.diff import DiffRenderer
+    from lucent_amd.runtime import Context
+
+    W, H, SPP = workloads.DIFF_WIDTH, workloads.DIFF_HEIGHT, workloads.DIFF_SPP
+    ctx = Context(0)
+    sc = workloads.spheres_scene()
+    d = DiffRenderer(ctx, sc, W, H, workloads.DIFF_DEPTH)
+    base_mats, base_env = list(d.materials), d.env.copy()
+    pm, penv = workloads.perturb_params(base_mats, base_env, seed=5)
+    d.set_materials(pm)
+    d.set_env_texels(penv)
+    target = d.forward(128, 1000)
+    d.set_materials(base_mats)
+    d.set_env_texels(base_env)
+    img = d.forward(SPP, 0)
+    dl = 2 * (img - target)
+    d.backward(dl, SPP, 0)  # warm-up
+    reps = 3
+    t_fwd = t_bwd = 0.0
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        img = d.forward(SPP, 0)
+        t1 = time.perf_counter()
+        gm, ge = d.backward(2 * (img - target), SPP, 0)
+        t2 = time.perf_counter()
+        t_fwd += t1 - t0
+        t_bwd += t2 - t1
+    t_fwd /= reps
+    t_bwd /= reps
+    d.close()
+    ctx.close()
+    return {"metric": "adjoint grad/s (forward + backward, 512x512, 32 spp, depth 6)",
+            "value": round(1.0 / (t_fwd + t_bwd), 3), "unit": "grad/s", "forward_ms": round(t_fwd * 1e3, 2),
+            "backward_ms": round(t_bwd * 1e3, 2), "params": {"material_scalars": int(gm[:16].size),
+                                                               "env_channels": int(ge.size)},
+            "loss": float(((img - target) ** 2).sum())}
+
+
 def bench_copy(args) -> dict:
     """Calibration: a plain device copy of 1 GiB per step (for scaling the
     FETCH_SIZE / WRITE_SIZE counters)."""
@@ -341,7 +389,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="all", choices=["all", "closest-hit", "path-trace", "denoise", "copy"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "closest-hit", "path-trace", "denoise", "adjoint", "copy"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -355,6 +404,10 @@ def main() -> int:
         res = bench_closest_hit(args)
     elif args.workload == "copy":
         res = bench_copy(args)
+    elif args.workload == "adjoint":
+        d = bench_adjoint(args)
+        res = {"value": d.pop("value"), "unit": d.pop("unit"), "ms_per_step": d["forward_ms"] + d["backward_ms"],
+               "dtype": "f32/f64", "config": {"workload": d.pop("metric")}, **d}
     elif args.workload == "denoise":
         d = bench_denoise(args)
         res = {"value": d["value"], "unit": d["unit"], "ms_per_step": d["value"], "dtype": "f32/f16",
@@ -366,6 +419,7 @@ def main() -> int:
             extra["closest_hit"] = {"metric": "Mrays/s closest-hit", "value": ch.pop("value"),
                                     "unit": ch.pop("unit"), **ch}
             extra["denoise"] = bench_denoise(args)
+            extra["adjoint"] = bench_adjoint(args)
     line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),

@@ -277,6 +277,54 @@ lucent_status lucent_denoise(lucent_ctx *ctx, lucent_film *film, const lucent_ca
 lucent_status lucent_film_read_denoised(lucent_ctx *ctx, const lucent_film *film, float *illum_var4,
                                         float *radiance4);
 
+/* ------------------------------------------------------------------------
+ * Differentiable rendering.  The forward pass is lucent_trace_samples; the
+ * backward pass replays each of the `spp` paths per pixel (seeds seed0 ..
+ * seed0+spp-1, the seeds the forward used) from its RNG seed and returns the
+ * gradient of sum_pixels dL_dI · mean_radiance with respect to each
+ * material's base colour (material texel 0 rgb), roughness (texel 1 x) and
+ * metallic (texel 0 w) and to every environment-map texel channel.
+ * Sampling probabilities, MIS weights, Russian roulette and the environment
+ * sampler are held constant (the estimator is unbiased for any of them).
+ * No counterpart in the reference (which has no differentiation); the
+ * parameters are those of material.ts:117-131 and lights.glsl:45-47.
+ * All pointers are device pointers; grads are overwritten (not accumulated).
+ * ---------------------------------------------------------------------- */
+lucent_status lucent_backward(lucent_ctx *ctx, lucent_scene *scene, const lucent_camera *camera,
+                              const lucent_render_settings *settings, uint32_t width, uint32_t height,
+                              uint32_t spp, uint32_t seed0, const float *d_dl_di, double *d_grad_materials,
+                              double *d_grad_env, void *stream);
+
+/* The estimator lucent_backward differentiates, evaluated at other material
+ * parameters: the samples of seeds seed0 .. seed0+spp-1 take every random
+ * decision, direction, pdf, MIS weight and roulette probability from the
+ * scene's current materials, and BSDF values, refraction tints, absorption
+ * and emission from `value_materials` (host, 40 floats per material, `count`
+ * = the scene's material count).  With value_materials equal to the scene's
+ * table this is the lucent_trace_samples image of those seeds (clamp 0) up to
+ * rounding; its derivative with respect to value_materials at that point is
+ * what lucent_backward returns, so central differences of it check the
+ * adjoint sample by sample.  d_out_rgba: device, float4 per pixel (rows
+ * bottom-up), receives the mean over the spp samples.  No counterpart in the
+ * reference. */
+lucent_status lucent_detached_forward(lucent_ctx *ctx, lucent_scene *scene, const lucent_camera *camera,
+                                      const lucent_render_settings *settings, uint32_t width, uint32_t height,
+                                      uint32_t spp, uint32_t seed0, const float *value_materials, uint32_t count,
+                                      float *d_out_rgba, void *stream);
+
+/* Replaces the scene's material table (40 floats per material, the layout
+ * of material.ts:117-131; the count must not change). */
+lucent_status lucent_scene_set_materials(lucent_ctx *ctx, lucent_scene *scene, const float *materials,
+                                         uint32_t material_count);
+
+/* Replaces the environment map's texels (same size), keeping the importance
+ * sampler built from the map given to lucent_scene_set_lighting. */
+lucent_status lucent_scene_update_env_texels(lucent_ctx *ctx, lucent_scene *scene, const float *rgb,
+                                             uint32_t width, uint32_t height);
+
+/* Environment map size of the scene's lighting (0 before lighting is set). */
+lucent_status lucent_scene_env_size(const lucent_scene *scene, uint32_t *width, uint32_t *height);
+
 #ifdef __cplusplus
 }
 #endif

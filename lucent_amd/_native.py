@@ -134,6 +134,13 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_trace_samples": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _P]),
     "lucent_denoise": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(DenoiseParamsStruct), _P]),
     "lucent_film_read_denoised": (_I32, [_P, _P, _P, _P]),
+    "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,
+                                _U32, _P, _P, _P, _P]),
+    "lucent_detached_forward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32,
+                                       _U32, _U32, _P, _U32, _P, _P]),
+    "lucent_scene_set_materials": (_I32, [_P, _P, _P, _U32]),
+    "lucent_scene_update_env_texels": (_I32, [_P, _P, _P, _U32, _U32]),
+    "lucent_scene_env_size": (_I32, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
 }
 
 _lib = None

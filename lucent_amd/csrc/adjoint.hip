@@ -1,0 +1,634 @@
+// Path-replay adjoint of the progressive path tracer on gfx950.
+//
+// For each pixel sample the kernel first re-runs the primal path from its
+// RNG seed (trace.frag semantics, pt_device.hpp) to obtain the path radiance
+// L, then replays the same path (same random numbers, so the same vertices
+// and decisions) and differentiates each contribution:
+//   * NEE terms  beta * f(θ) * Li * k     -> d/dθ through f and through the
+//                                            environment texels behind Li;
+//   * escapes    beta * Lenv * w          -> d/d(texels) of the bilinear lookup;
+//   * the BSDF continuation weight f(θ) cos / pdf scales every later term,
+//     whose sum is the remaining radiance Lr = L - (terms so far), so it
+//     contributes Lr * (df/dθ) / f.
+// Sampling probabilities, MIS weights, Russian-roulette probabilities and the
+// environment sampler's pdf are held constant (detached): the estimator is
+// unbiased for any choice of them, so its expectation's gradient is the
+// expectation of these terms.  df/dθ comes from forward-mode dual numbers
+// over the opaque BSDF (bsdf.glsl:272-324) with θ = (albedo rgb, roughness,
+// metallic) of the hit material.
+//
+// Reductions: material gradients are summed per block in LDS and written as
+// per-block partials, then reduced in FP64 by a second kernel (deterministic);
+// environment-texel gradients go through float atomics into 8 replicas to
+// spread contention and are reduced into FP64 by a final kernel.
+#include <hip/hip_runtime.h>
+
+#include "adjoint.hpp"
+#include "pt_device.hpp"
+
+namespace lucent {
+namespace {
+
+using namespace pt;
+
+constexpr int kAdjBlock = 64;
+constexpr int kAdjLds = 16;
+constexpr int kTile = 8;
+constexpr int kNumParams = 5;        // albedo r, g, b, roughness, metallic
+constexpr int kLdsMaterials = 64;    // materials whose gradients are pre-reduced in LDS
+constexpr int kEnvReplicas = 8;
+
+// Forward-mode dual number with the five material-parameter tangents.
+struct D5 {
+  float v;
+  float d[kNumParams];
+};
+__device__ inline D5 dc(float v) {
+  D5 r;
+  r.v = v;
+  for (int i = 0; i < kNumParams; ++i) r.d[i] = 0.0f;
+  return r;
+}
+__device__ inline D5 dvar(float v, int i) {
+  D5 r = dc(v);
+  r.d[i] = 1.0f;
+  return r;
+}
+__device__ inline D5 operator+(D5 a, D5 b) {
+  D5 r;
+  r.v = a.v + b.v;
+  for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] + b.d[i];
+  return r;
+}
+__device__ inline D5 operator-(D5 a, D5 b) {
+  D5 r;
+  r.v = a.v - b.v;
+  for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] - b.d[i];
+  return r;
+}
+__device__ inline D5 operator*(D5 a, D5 b) {
+  D5 r;
+  r.v = a.v * b.v;
+  for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+__device__ inline D5 operator*(D5 a, float s) {
+  D5 r;
+  r.v = a.v * s;
+  for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] * s;
+  return r;
+}
+__device__ inline D5 operator+(D5 a, float s) {
+  a.v += s;
+  return a;
+}
+__device__ inline D5 operator/(D5 a, D5 b) {
+  D5 r;
+  const float inv = 1.0f / b.v;
+  r.v = a.v * inv;
+  for (int i = 0; i < kNumParams; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * inv;
+  return r;
+}
+__device__ inline D5 dsqrt(D5 a) {
+  D5 r;
+  r.v = sqrtf(a.v);
+  const float k = r.v > 0.0f ? 0.5f / r.v : 0.0f;
+  for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] * k;
+  return r;
+}
+__device__ inline D5 one_minus(D5 a) { return dc(1.0f) - a; }
+
+// f of evalOpaque (bsdf.glsl:272-324) and its derivatives with respect to
+// the hit material's albedo, roughness and metallic.  The sheen energy scale
+// and the coat layering do not depend on these parameters.
+__device__ inline bool eval_opaque_grad(const PtScene& sc, const Surface& s, v3 wo, v3 wi, float f[3],
+                                        float df[3][kNumParams]) {
+  const v3 n = s.n;
+  const float ndl = dot(n, wi), ndv = dot(n, wo);
+  if (ndl <= 0.0f || ndv <= 0.0f) return false;
+  const D5 alb[3] = {dvar(s.albedo.x, 0), dvar(s.albedo.y, 1), dvar(s.albedo.z, 2)};
+  const D5 rough = dvar(s.roughness, 3);
+  const D5 metal = dvar(s.metallic, 4);
+  const D5 a = (rough.v * rough.v > 1e-3f) ? rough * rough : dc(1e-3f);
+  const v3 H = normalize(wo + wi);
+  const float ndh = fmaxf(dot(n, H), 0.0f);
+  const float vdh = fmaxf(dot(wo, H), 0.0f);
+  const float fres = powf(1.0f - clampf(vdh, 0.0f, 1.0f), 5.0f);
+  const float d0 = dielectric_f0(s.ior);
+  const D5 a2 = a * a;
+  const D5 dd = (a2 + (-1.0f)) * (ndh * ndh) + 1.0f;
+  const D5 D = a2 / (dd * dd * PI);
+  auto g1 = [&](float x) { return dc(2.0f * x) / (dsqrt(a2 + one_minus(a2) * (x * x)) + x); };
+  const D5 G = g1(ndv) * g1(ndl);
+  const float denom = 1.0f / fmaxf(4.0f * ndv * ndl, 1e-6f);
+  const D5 DG = D * G * denom;
+  const D5 one_m = one_minus(metal);
+  float scale = 1.0f, sheen[3] = {0.0f, 0.0f, 0.0f};
+  const float sheen_max = max3(s.sheen_color);
+  if (sheen_max > 0.0f) {
+    const float sr = fmaxf(s.sheen_roughness, MIN_SHEEN_ROUGHNESS);
+    const float k = d_charlie(sr * sr, ndh) * v_neubelt(ndl, ndv);
+    sheen[0] = s.sheen_color.x * k;
+    sheen[1] = s.sheen_color.y * k;
+    sheen[2] = s.sheen_color.z * k;
+    scale = 1.0f - sheen_max * fmaxf(sheen_albedo(sc, ndv, sr), sheen_albedo(sc, ndl, sr));
+  }
+  float coat_mul = 1.0f, coat_add = 0.0f;
+  if (coat_probability(s, wo) > 0.0f) {
+    const float cv = dot(s.nc, wo), cl = dot(s.nc, wi);
+    float coat = 0.0f;
+    if (cv > 0.0f && cl > 0.0f) {
+      const float ac = alpha_from_roughness(s.clearcoat_roughness);
+      const float ch = fmaxf(dot(s.nc, H), 0.0f);
+      coat = d_ggx(ch, ac) * g1_ggx(cv, ac) * g1_ggx(cl, ac) * fresnel_schlick1(vdh, 0.04f) / (4.0f * cv * cl);
+    }
+    coat_mul = 1.0f - s.clearcoat * fresnel_schlick1(fmaxf(cv, 0.0f), 0.04f);
+    coat_add = s.clearcoat * coat;
+  }
+  for (int c = 0; c < 3; ++c) {
+    const D5 f0 = dc(d0) * one_m + alb[c] * metal;
+    const D5 F = f0 + one_minus(f0) * fres;
+    const D5 spec = DG * F;
+    const D5 diff = one_minus(F) * alb[c] * one_m * INV_PI;
+    D5 fc = (diff + spec) * scale + sheen[c];
+    fc = fc * coat_mul + coat_add;
+    f[c] = fc.v;
+    for (int i = 0; i < kNumParams; ++i) df[c][i] = fc.d[i];
+  }
+  return true;
+}
+
+// Scales of the surface parameters with respect to the material's own
+// (surfaceAt, trace.frag:72-91: checker colour, base/metal-rough maps).
+struct ParamScale {
+  float albedo[3];
+  float rough, metal;
+};
+
+__device__ inline ParamScale param_scale(const PtScene& sc, const Hit& hit) {
+  const Material m = fetch_material(sc, hit.material);
+  ParamScale p{{1.0f, 1.0f, 1.0f}, 1.0f, 1.0f};
+  float tu, tv;
+  transform_uv(hit.uv_u, hit.uv_v, m.uv0, m.uv1, tu, tv);
+  if (m.checker_scale > 0.0f) {
+    const float sum = floorf(hit.position.x * m.checker_scale + 0.5f) + floorf(hit.position.y * m.checker_scale + 0.5f) +
+                      floorf(hit.position.z * m.checker_scale + 0.5f);
+    if (sum - 2.0f * floorf(sum / 2.0f) >= 0.5f) p.albedo[0] = p.albedo[1] = p.albedo[2] = 0.0f;
+  }
+  const bool tex = sc.tex_size > 0;
+  if (m.map_base >= 0 && tex) {
+    const float4 t = sample_layer(sc.color_maps, sc.tex_size, m.map_base, tu, tv, true);
+    p.albedo[0] *= t.x;
+    p.albedo[1] *= t.y;
+    p.albedo[2] *= t.z;
+  }
+  if (m.map_mr >= 0 && tex) {
+    const float4 mr = sample_layer(sc.data_maps, sc.tex_size, m.map_mr, tu, tv, false);
+    p.rough = mr.y;
+    p.metal = mr.z;
+  }
+  return p;
+}
+
+struct GradSink {
+  float* lds;  // kLdsMaterials * 5 floats (block-shared) or null
+  float* env;  // kEnvReplicas replicas of env_w * env_h * 3 floats
+  size_t env_stride;
+  uint32_t replica;
+  int material_count;
+  float* global_mat;  // fallback for materials beyond the LDS table (float atomics)
+};
+
+__device__ inline void add_material(GradSink& g, int mat, const float v[kNumParams]) {
+  if (mat < kLdsMaterials) {
+    for (int i = 0; i < kNumParams; ++i)
+      if (v[i] != 0.0f) atomicAdd(&g.lds[mat * kNumParams + i], v[i]);
+  } else {
+    for (int i = 0; i < kNumParams; ++i)
+      if (v[i] != 0.0f) unsafeAtomicAdd(&g.global_mat[mat * kNumParams + i], v[i]);
+  }
+}
+
+__device__ inline void add_env(const PtScene& sc, GradSink& g, float u, float v, v3 weight) {
+  int idx[4];
+  float w[4];
+  env_taps(sc, u, v, idx, w);
+  float* base = g.env + g.replica * g.env_stride;
+  for (int k = 0; k < 4; ++k) {
+    if (w[k] == 0.0f) continue;
+    const float s = w[k] * sc.env_intensity;
+    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 0], weight.x * s);
+    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 1], weight.y * s);
+    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 2], weight.z * s);
+  }
+}
+
+// Gradient of δL · (beta * f * Li * k) through f.
+__device__ inline void add_f_term(GradSink& g, int mat, const ParamScale& ps, v3 coeff, const float df[3][kNumParams]) {
+  const float c[3] = {coeff.x, coeff.y, coeff.z};
+  float out[kNumParams] = {0, 0, 0, 0, 0};
+  for (int ch = 0; ch < 3; ++ch)
+    for (int i = 0; i < kNumParams; ++i) out[i] += c[ch] * df[ch][i];
+  out[0] *= ps.albedo[0];
+  out[1] *= ps.albedo[1];
+  out[2] *= ps.albedo[2];
+  out[3] *= ps.rough;
+  out[4] *= ps.metal;
+  add_material(g, mat, out);
+}
+
+// The replay of tracePath (trace.frag:136-272) accumulating d(dL · L)/dθ.
+template <class StackT>
+__device__ inline void replay(const PtScene& sc, const PtFrame& f, v3 ro, v3 rd, Rng& rng, StackT& st, v3 L, v3 dl,
+                              GradSink& g) {
+  v3 lr = L;
+  v3 beta = mk(1.f, 1.f, 1.f);
+  bool specular = true;
+  float bsdf_pdf = 0.0f;
+  for (int bounce = 0; bounce <= int(f.max_bounces); ++bounce) {
+    Hit hit;
+    if (!trace_scene(sc, ro, rd, hit, rng, st)) {
+      if (!(bounce == 0 && sc.env_background == 0)) {
+        float w = 1.0f;
+        if (!specular && sc.env_sampling != 0) w = mis_weight(bsdf_pdf, env_pdf(sc, rd));
+        float u, v;
+        equirect_uv(env_to_local(sc, rd), u, v);
+        add_env(sc, g, u, v, dl * beta * w);
+      }
+      return;
+    }
+    const v3 wo = -rd;
+    v3 emission;
+    const Surface s = surface_at(sc, hit, wo, emission);
+    const v3 ng = hit.geom_normal;
+    if (!hit.front_face && !s.thin_walled && s.transmission > 0.0f) beta = beta * vexp(-s.absorption * hit.t);
+    if (emission.x > 0.0f || emission.y > 0.0f || emission.z > 0.0f) {
+      float w = 1.0f;
+      if (!specular && hit.light >= 0) w = mis_weight(bsdf_pdf, light_pdf(sc, hit.light, ro, rd, hit.t) / float(sc.light_count));
+      lr = lr - beta * emission * w;
+      if (hit.light >= 0) return;
+    }
+    const bool glass = s.transmission > 0.0f && rng.rand() < s.transmission;
+    const ParamScale ps = param_scale(sc, hit);
+    if (glass) {
+      v3 wi, weight;
+      bool reflected;
+      if (!sample_dielectric(s, hit.front_face, ng, wo, rng, wi, weight, reflected)) return;
+      if (!reflected) {
+        // Refraction weight = albedo: later terms scale with it.
+        float out[kNumParams] = {0, 0, 0, 0, 0};
+        if (s.albedo.x > 0.0f) out[0] = dl.x * lr.x / s.albedo.x * ps.albedo[0];
+        if (s.albedo.y > 0.0f) out[1] = dl.y * lr.y / s.albedo.y * ps.albedo[1];
+        if (s.albedo.z > 0.0f) out[2] = dl.z * lr.z / s.albedo.z * ps.albedo[2];
+        add_material(g, hit.material, out);
+      }
+      beta = beta * weight;
+      ro = offset_ray(hit.position, reflected ? ng : -ng);
+      rd = wi;
+      specular = true;
+    } else {
+      const v3 shadow_origin = offset_ray(hit.position, ng);
+      const v3 n = s.n;
+      float fv[3], dfv[3][kNumParams];
+      if (sc.sun_enabled != 0) {
+        const float u0 = rng.rand(), u1 = rng.rand();
+        const v3 wi = sample_sun_direction(sc, u0, u1);
+        const float ndl = dot(n, wi);
+        if (ndl > 0.0f && dot(ng, wi) > 0.0f && !occluded(sc, shadow_origin, wi, INF, rng, st)) {
+          float pb;
+          eval_opaque(sc, s, wo, wi, pb);
+          const float k = ndl * mis_weight(sc.sun_pdf, pb) / sc.sun_pdf;
+          if (eval_opaque_grad(sc, s, wo, wi, fv, dfv)) {
+            const v3 li = mk(sc.sun_radiance) * k;
+            lr = lr - beta * mk(fv[0], fv[1], fv[2]) * li;
+            add_f_term(g, hit.material, ps, dl * beta * li, dfv);
+          }
+        }
+      }
+      if (sc.env_sampling != 0) {
+        const LightSample ls = sample_environment(sc, rng);
+        const float ndl = dot(n, ls.dir);
+        if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f && !occluded(sc, shadow_origin, ls.dir, INF, rng, st)) {
+          float pb;
+          eval_opaque(sc, s, wo, ls.dir, pb);
+          const float k = ndl * mis_weight(ls.pdf, pb) / ls.pdf;
+          if (eval_opaque_grad(sc, s, wo, ls.dir, fv, dfv)) {
+            const v3 fvec = mk(fv[0], fv[1], fv[2]);
+            lr = lr - beta * fvec * ls.radiance * k;
+            add_f_term(g, hit.material, ps, dl * beta * ls.radiance * k, dfv);
+            add_env(sc, g, ls.u, ls.v, dl * beta * fvec * k);
+          }
+        }
+      }
+      if (sc.light_count > 0) {
+        const int li = min(int(rng.rand() * float(sc.light_count)), int(sc.light_count) - 1);
+        const float u0 = rng.rand(), u1 = rng.rand();
+        const LightSample ls = sample_light(sc, li, hit.position, u0, u1);
+        const float ndl = dot(n, ls.dir);
+        if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f &&
+            !occluded(sc, shadow_origin, ls.dir, ls.dist * (1.0f - 1e-4f), rng, st)) {
+          const float pl = ls.pdf / float(sc.light_count);
+          float pb;
+          eval_opaque(sc, s, wo, ls.dir, pb);
+          const float k = ndl * mis_weight(pl, pb) / pl;
+          if (eval_opaque_grad(sc, s, wo, ls.dir, fv, dfv)) {
+            lr = lr - beta * mk(fv[0], fv[1], fv[2]) * ls.radiance * k;
+            add_f_term(g, hit.material, ps, dl * beta * ls.radiance * k, dfv);
+          }
+        }
+      }
+      const BsdfSample bs = sample_opaque(sc, s, wo, rng);
+      if (bs.pdf <= 0.0f || dot(bs.dir, ng) <= 0.0f) return;
+      // Every later term carries this vertex's f: δL · Lr * (df/dθ) / f.
+      if (eval_opaque_grad(sc, s, wo, bs.dir, fv, dfv)) {
+        v3 coeff = mk(0.f, 0.f, 0.f);
+        if (fv[0] > 1e-30f) coeff.x = dl.x * lr.x / fv[0];
+        if (fv[1] > 1e-30f) coeff.y = dl.y * lr.y / fv[1];
+        if (fv[2] > 1e-30f) coeff.z = dl.z * lr.z / fv[2];
+        add_f_term(g, hit.material, ps, coeff, dfv);
+      }
+      beta = beta * bs.weight;
+      bsdf_pdf = bs.pdf;
+      specular = false;
+      ro = shadow_origin;
+      rd = bs.dir;
+    }
+    if (bounce >= 2) {
+      const float p = fminf(0.95f, max3(beta));
+      if (rng.rand() >= p) return;
+      beta = beta / p;
+    }
+  }
+}
+
+// The detached estimator as a function of the material parameters: every
+// random decision, direction, pdf, MIS weight and roulette probability comes
+// from the scene's materials `sc` (θ0), the BSDF values, refraction tints,
+// absorption and emission from `sv` (θ, same geometry and lighting).  At θ = θ0
+// this is tracePath (trace.frag:136-272, clamp 0) up to rounding; its
+// derivative in θ at θ0 is the sum replay() accumulates.  Two throughputs run
+// side by side: beta0 drives roulette, beta carries the θ values.
+template <class StackT>
+__device__ inline v3 detached_path(const PtScene& sc, const PtScene& sv, const PtFrame& f, v3 ro, v3 rd, Rng& rng,
+                                   StackT& st) {
+  v3 radiance = mk(0.f, 0.f, 0.f);
+  v3 beta0 = mk(1.f, 1.f, 1.f), beta = mk(1.f, 1.f, 1.f);
+  bool specular = true;
+  float bsdf_pdf = 0.0f;
+  for (int bounce = 0; bounce <= int(f.max_bounces); ++bounce) {
+    Hit hit;
+    if (!trace_scene(sc, ro, rd, hit, rng, st)) {
+      const v3 env = (bounce == 0 && sc.env_background == 0) ? mk(sc.backdrop) : environment(sc, rd, specular, bsdf_pdf);
+      radiance = radiance + beta * env;
+      break;
+    }
+    const v3 wo = -rd;
+    v3 emission, emission_v;
+    const Surface s = surface_at(sc, hit, wo, emission);
+    const Surface s_v = surface_at(sv, hit, wo, emission_v);
+    const v3 ng = hit.geom_normal;
+    if (!hit.front_face && !s.thin_walled && s.transmission > 0.0f) {
+      beta0 = beta0 * vexp(-s.absorption * hit.t);
+      beta = beta * vexp(-s_v.absorption * hit.t);
+    }
+    if (emission.x > 0.0f || emission.y > 0.0f || emission.z > 0.0f) {
+      float w = 1.0f;
+      if (!specular && hit.light >= 0) w = mis_weight(bsdf_pdf, light_pdf(sc, hit.light, ro, rd, hit.t) / float(sc.light_count));
+      radiance = radiance + beta * emission_v * w;
+      if (hit.light >= 0) break;
+    }
+    const bool glass = s.transmission > 0.0f && rng.rand() < s.transmission;
+    if (glass) {
+      v3 wi, weight;
+      bool reflected;
+      if (!sample_dielectric(s, hit.front_face, ng, wo, rng, wi, weight, reflected)) break;
+      beta0 = beta0 * weight;
+      beta = beta * (reflected ? weight : s_v.albedo);
+      ro = offset_ray(hit.position, reflected ? ng : -ng);
+      rd = wi;
+      specular = true;
+    } else {
+      const v3 shadow_origin = offset_ray(hit.position, ng);
+      const v3 n = s.n;
+      if (sc.sun_enabled != 0) {
+        const float u0 = rng.rand(), u1 = rng.rand();
+        const v3 wi = sample_sun_direction(sc, u0, u1);
+        const float ndl = dot(n, wi);
+        if (ndl > 0.0f && dot(ng, wi) > 0.0f && !occluded(sc, shadow_origin, wi, INF, rng, st)) {
+          float pb, unused;
+          eval_opaque(sc, s, wo, wi, pb);
+          const v3 fv = eval_opaque(sv, s_v, wo, wi, unused);
+          radiance = radiance + beta * fv * ndl * mk(sc.sun_radiance) * (mis_weight(sc.sun_pdf, pb) / sc.sun_pdf);
+        }
+      }
+      if (sc.env_sampling != 0) {
+        const LightSample ls = sample_environment(sc, rng);
+        const float ndl = dot(n, ls.dir);
+        if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f && !occluded(sc, shadow_origin, ls.dir, INF, rng, st)) {
+          float pb, unused;
+          eval_opaque(sc, s, wo, ls.dir, pb);
+          const v3 fv = eval_opaque(sv, s_v, wo, ls.dir, unused);
+          radiance = radiance + beta * fv * ndl * ls.radiance * (mis_weight(ls.pdf, pb) / ls.pdf);
+        }
+      }
+      if (sc.light_count > 0) {
+        const int li = min(int(rng.rand() * float(sc.light_count)), int(sc.light_count) - 1);
+        const float u0 = rng.rand(), u1 = rng.rand();
+        const LightSample ls = sample_light(sc, li, hit.position, u0, u1);
+        const float ndl = dot(n, ls.dir);
+        if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f &&
+            !occluded(sc, shadow_origin, ls.dir, ls.dist * (1.0f - 1e-4f), rng, st)) {
+          const float pl = ls.pdf / float(sc.light_count);
+          float pb, unused;
+          eval_opaque(sc, s, wo, ls.dir, pb);
+          const v3 fv = eval_opaque(sv, s_v, wo, ls.dir, unused);
+          radiance = radiance + beta * fv * ndl * ls.radiance * (mis_weight(pl, pb) / pl);
+        }
+      }
+      const BsdfSample bs = sample_opaque(sc, s, wo, rng);
+      if (bs.pdf <= 0.0f || dot(bs.dir, ng) <= 0.0f) break;
+      float unused;
+      const v3 fv = eval_opaque(sv, s_v, wo, bs.dir, unused);
+      beta0 = beta0 * bs.weight;
+      beta = beta * fv * (dot(n, bs.dir) / bs.pdf);
+      bsdf_pdf = bs.pdf;
+      specular = false;
+      ro = shadow_origin;
+      rd = bs.dir;
+    }
+    if (bounce >= 2) {
+      const float p = fminf(0.95f, max3(beta0));
+      if (rng.rand() >= p) break;
+      beta0 = beta0 / p;
+      beta = beta / p;
+    }
+  }
+  return radiance;
+}
+
+// One sample per pixel of detached_path: out += L / spp (the first launch
+// overwrites), so after spp launches out holds the mean.
+__global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, PtFrame f, float inv_spp, int first,
+                                                        float* __restrict__ out, uint32_t* spill, uint32_t spill_depth,
+                                                        uint32_t tiles_x) {
+  __shared__ uint32_t lds[kAdjLds * kAdjBlock];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t x = (tile % tiles_x) * kTile + (threadIdx.x & (kTile - 1));
+  const uint32_t y = (tile / tiles_x) * kTile + (threadIdx.x / kTile);
+  if (x >= f.width || y >= f.height) return;
+  const uint32_t idx = y * f.width + x;
+  Rng rng = seed_rng(idx, f.seed);
+  v3 ro, rd;
+  camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
+  Stack<kAdjLds, kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0,
+                               0};
+  v3 L = detached_path(sc, sv, f, ro, rd, rng, st);
+  if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z)) L = mk(0.f, 0.f, 0.f);
+  float4* o = reinterpret_cast<float4*>(out) + idx;
+  const float4 prev = first ? make_float4(0.f, 0.f, 0.f, 0.f) : *o;
+  *o = make_float4(prev.x + L.x * inv_spp, prev.y + L.y * inv_spp, prev.z + L.z * inv_spp, 0.0f);
+}
+
+__global__ __launch_bounds__(kAdjBlock) void k_adjoint(PtScene sc, PtFrame f, const float* __restrict__ dl_di,
+                                                       float inv_spp, float* __restrict__ mat_partials,
+                                                       float* __restrict__ env_acc, size_t env_stride,
+                                                       float* __restrict__ mat_overflow, uint32_t* spill,
+                                                       uint32_t spill_depth, uint32_t tiles_x) {
+  __shared__ uint32_t lds[kAdjLds * kAdjBlock];
+  __shared__ float lds_grad[kLdsMaterials * kNumParams];
+  for (int i = threadIdx.x; i < kLdsMaterials * kNumParams; i += kAdjBlock) lds_grad[i] = 0.0f;
+  __syncthreads();
+  const uint32_t tile = blockIdx.x;
+  const uint32_t x = (tile % tiles_x) * kTile + (threadIdx.x & (kTile - 1));
+  const uint32_t y = (tile / tiles_x) * kTile + (threadIdx.x / kTile);
+  if (x < f.width && y < f.height) {
+    const uint32_t idx = y * f.width + x;
+    const float4 d = reinterpret_cast<const float4*>(dl_di)[idx];
+    const v3 dl = mk(d.x * inv_spp, d.y * inv_spp, d.z * inv_spp);
+    if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
+      Rng rng = seed_rng(idx, f.seed);
+      v3 ro, rd;
+      camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
+      Stack<kAdjLds, kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth,
+                                   0, 0};
+      Rng replay_rng = rng;
+      Guide gd;
+      const v3 L = trace_path(sc, f, ro, rd, rng, st, gd);
+      if (!(isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))) {
+        GradSink g{lds_grad, env_acc, env_stride, blockIdx.x % kEnvReplicas, int(sc.material_count), mat_overflow};
+        replay(sc, f, ro, rd, replay_rng, st, L, dl, g);
+      }
+    }
+  }
+  __syncthreads();
+  const int nm = min(int(sc.material_count), kLdsMaterials);
+  for (int i = threadIdx.x; i < nm * kNumParams; i += kAdjBlock)
+    mat_partials[size_t(blockIdx.x) * kLdsMaterials * kNumParams + i] = lds_grad[i];
+}
+
+// FP64 reductions: per-block material partials, then the env replicas.
+__global__ void k_reduce_materials(const float* __restrict__ partials, uint32_t blocks, uint32_t entries,
+                                   const float* __restrict__ overflow, uint32_t overflow_entries,
+                                   double* __restrict__ out) {
+  const uint32_t e = blockIdx.x;
+  __shared__ double part[256];
+  double acc = 0.0;
+  if (e < entries)
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) acc += double(partials[size_t(b) * kLdsMaterials * kNumParams + e]);
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double v = e < entries ? part[0] : 0.0;
+    if (e < overflow_entries) v += double(overflow[e]);
+    out[e] += v;
+  }
+}
+
+__global__ void k_reduce_env(const float* __restrict__ acc, size_t stride, size_t n, double* __restrict__ out) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int r = 0; r < kEnvReplicas; ++r) s += double(acc[r * stride + i]);
+  out[i] += s;
+}
+
+__global__ void k_zero(float* p, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+}  // namespace
+
+size_t adjoint_scratch_bytes(uint32_t width, uint32_t height, uint32_t material_count, uint32_t env_texels) {
+  const size_t blocks = size_t((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
+  const size_t partials = blocks * kLdsMaterials * kNumParams;
+  const size_t overflow = size_t(material_count) * kNumParams;
+  const size_t env = size_t(kEnvReplicas) * env_texels * 3;
+  return (partials + overflow + env) * sizeof(float) + 1024;
+}
+
+Status launch_detached(Context* ctx, const DetachedArgs& a, void* stream_) {
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  const uint32_t tiles_x = (a.frame.width + kTile - 1) / kTile;
+  const uint32_t tiles_y = (a.frame.height + kTile - 1) / kTile;
+  const uint32_t blocks = tiles_x * tiles_y;
+  const uint32_t depth = a.scene->bvh.stack_depth + 1;
+  Status s = ensure_spill(ctx, size_t(blocks) * kAdjBlock * depth * sizeof(uint32_t));
+  if (s != LUCENT_OK) return s;
+  PtScene sv = *a.scene;
+  sv.materials = a.value_materials;
+  for (uint32_t k = 0; k < a.spp; ++k) {
+    PtFrame f = a.frame;
+    f.seed = a.seed0 + k;
+    hipLaunchKernelGGL(k_detached, dim3(blocks), dim3(kAdjBlock), 0, stream, *a.scene, sv, f, 1.0f / float(a.spp),
+                       k == 0 ? 1 : 0, a.out, static_cast<uint32_t*>(ctx->spill), depth, tiles_x);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(e, "detached forward launch");
+  return LUCENT_OK;
+}
+
+Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  const PtScene& sc = *a.scene;
+  const uint32_t tiles_x = (a.frame.width + kTile - 1) / kTile;
+  const uint32_t tiles_y = (a.frame.height + kTile - 1) / kTile;
+  const uint32_t blocks = tiles_x * tiles_y;
+  const size_t env_texels = size_t(sc.env_w) * sc.env_h;
+  const size_t partials = size_t(blocks) * kLdsMaterials * kNumParams;
+  const size_t overflow = size_t(sc.material_count) * kNumParams;
+  const size_t env_stride = env_texels * 3;
+  float* base = static_cast<float*>(a.scratch);
+  float* mat_partials = base;
+  float* mat_overflow = base + partials;
+  float* env_acc = mat_overflow + overflow;
+  const uint32_t depth = sc.bvh.stack_depth + 1;
+  Status s = ensure_spill(ctx, size_t(blocks) * kAdjBlock * depth * sizeof(uint32_t));
+  if (s != LUCENT_OK) return s;
+  const size_t zero_n = overflow + kEnvReplicas * env_stride;
+  for (uint32_t k = 0; k < a.spp; ++k) {
+    hipLaunchKernelGGL(k_zero, dim3(uint32_t((zero_n + 255) / 256)), dim3(256), 0, stream, mat_overflow, zero_n);
+    PtFrame f = a.frame;
+    f.seed = a.seed0 + k;
+    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(kAdjBlock), 0, stream, sc, f, a.dl_di, 1.0f / float(a.spp),
+                       mat_partials, env_acc, env_stride, mat_overflow, static_cast<uint32_t*>(ctx->spill), depth,
+                       tiles_x);
+    const uint32_t entries = uint32_t(min(int(sc.material_count), kLdsMaterials)) * kNumParams;
+    const uint32_t all = uint32_t(sc.material_count) * kNumParams;
+    if (all > 0)
+      hipLaunchKernelGGL(k_reduce_materials, dim3(all), dim3(256), 0, stream, mat_partials, blocks, entries, mat_overflow,
+                         all, a.grad_materials);
+    if (env_stride > 0 && a.grad_env)
+      hipLaunchKernelGGL(k_reduce_env, dim3(uint32_t((env_stride + 255) / 256)), dim3(256), 0, stream, env_acc, env_stride,
+                         env_stride, a.grad_env);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(e, "adjoint launch");
+  return LUCENT_OK;
+}
+
+}  // namespace lucent

@@ -1,0 +1,35 @@
+// Path-replay adjoint launch arguments.
+#pragma once
+
+#include <cstdint>
+
+#include "internal.hpp"
+#include "pt_types.hpp"
+
+namespace lucent {
+
+struct AdjointArgs {
+  const PtScene* scene = nullptr;
+  PtFrame frame;
+  uint32_t spp = 1;
+  uint32_t seed0 = 0;
+  const float* dl_di = nullptr;     // dLoss/dImage, float4 per pixel (rows bottom-up)
+  double* grad_materials = nullptr; // material_count * 5 (albedo rgb, roughness, metallic)
+  double* grad_env = nullptr;       // env_w * env_h * 3, or null
+  void* scratch = nullptr;          // adjoint_scratch_bytes(...)
+};
+
+struct DetachedArgs {
+  const PtScene* scene = nullptr;          // sampling parameters θ0
+  const float* value_materials = nullptr;  // device, 40 floats per material: BSDF values θ
+  PtFrame frame;
+  uint32_t spp = 1;
+  uint32_t seed0 = 0;
+  float* out = nullptr;                    // float4 per pixel (rows bottom-up), mean over spp
+};
+
+Status launch_detached(Context* ctx, const DetachedArgs& a, void* stream);
+size_t adjoint_scratch_bytes(uint32_t width, uint32_t height, uint32_t material_count, uint32_t env_texels);
+Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream);
+
+}  // namespace lucent

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "denoise.hpp"
 #include "render_host.hpp"
 #include "scene.hpp"
@@ -296,6 +297,125 @@ lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_
     if (s != LUCENT_OK) return s;
     film->samples++;
   }
+  return LUCENT_OK;
+}
+
+static Status fill_frame(const lucent_camera* cam, const lucent_render_settings* settings, uint32_t w, uint32_t h,
+                         PtFrame* f) {
+  f->width = w;
+  f->height = h;
+  f->max_bounces = settings->max_bounces;
+  f->clamp = settings->firefly_clamp;
+  double fwd[3], right[3], up[3], tan_half;
+  camera_basis(*cam, fwd, right, up, &tan_half);
+  for (int i = 0; i < 3; ++i) {
+    f->cam_pos[i] = float(cam->position[i]);
+    f->cam_fwd[i] = float(fwd[i]);
+    f->cam_right[i] = float(right[i]);
+    f->cam_up[i] = float(up[i]);
+  }
+  f->tan_half_fov = float(tan_half);
+  f->aperture = float(cam->aperture);
+  f->focus_dist = float(cam->focus_distance);
+  return LUCENT_OK;
+}
+
+lucent_status lucent_backward(lucent_ctx* ctx, lucent_scene* scene, const lucent_camera* cam,
+                              const lucent_render_settings* settings, uint32_t width, uint32_t height, uint32_t spp,
+                              uint32_t seed0, const float* d_dl_di, double* d_grad_materials, double* d_grad_env,
+                              void* stream) {
+  clear_error();
+  if (!ctx || !scene || !cam || !settings || !d_dl_di || !d_grad_materials) return fail(LUCENT_E_INVALID, "null argument");
+  if (!scene->lighting_set) return fail(LUCENT_E_INVALID, "call lucent_scene_set_lighting before lucent_backward");
+  if (width == 0 || height == 0 || spp == 0) return fail(LUCENT_E_INVALID, "empty image or zero samples");
+  if (settings->firefly_clamp > 0.0f)
+    return fail(LUCENT_E_UNSUPPORTED, "the firefly clamp is not differentiable; use firefly_clamp = 0");
+  const PtScene& pt = scene->pt;
+  const uint32_t env_texels = uint32_t(pt.env_w) * uint32_t(pt.env_h);
+  const size_t need = adjoint_scratch_bytes(width, height, pt.material_count, env_texels);
+  Status s = ensure_scratch(ctx, need);
+  if (s != LUCENT_OK) return s;
+  hipStream_t st = static_cast<hipStream_t>(stream ? stream : ctx->stream);
+  hipError_t e = hipMemsetAsync(d_grad_materials, 0, size_t(pt.material_count) * 5 * sizeof(double), st);
+  if (e == hipSuccess && d_grad_env) e = hipMemsetAsync(d_grad_env, 0, size_t(env_texels) * 3 * sizeof(double), st);
+  if (e != hipSuccess) return fail_hip(e, "hipMemsetAsync(gradients)");
+  AdjointArgs a;
+  a.scene = &pt;
+  fill_frame(cam, settings, width, height, &a.frame);
+  a.spp = spp;
+  a.seed0 = seed0;
+  a.dl_di = d_dl_di;
+  a.grad_materials = d_grad_materials;
+  a.grad_env = d_grad_env;
+  a.scratch = ctx->scratch;
+  return launch_adjoint(ctx, a, st);
+}
+
+lucent_status lucent_detached_forward(lucent_ctx* ctx, lucent_scene* scene, const lucent_camera* cam,
+                                      const lucent_render_settings* settings, uint32_t width, uint32_t height,
+                                      uint32_t spp, uint32_t seed0, const float* value_materials, uint32_t count,
+                                      float* d_out_rgba, void* stream) {
+  clear_error();
+  if (!ctx || !scene || !cam || !settings || !value_materials || !d_out_rgba) return fail(LUCENT_E_INVALID, "null argument");
+  if (!scene->lighting_set) return fail(LUCENT_E_INVALID, "call lucent_scene_set_lighting before lucent_detached_forward");
+  if (width == 0 || height == 0 || spp == 0) return fail(LUCENT_E_INVALID, "empty image or zero samples");
+  if (count != scene->pt.material_count) return fail(LUCENT_E_INVALID, "material count differs from the uploaded scene");
+  if (settings->firefly_clamp > 0.0f)
+    return fail(LUCENT_E_UNSUPPORTED, "the detached estimator has no firefly clamp; use firefly_clamp = 0");
+  const size_t bytes = size_t(count) * kMaterialTexels * 4 * sizeof(float);
+  Status s = ensure_scratch(ctx, bytes);
+  if (s != LUCENT_OK) return s;
+  hipStream_t st = static_cast<hipStream_t>(stream ? stream : ctx->stream);
+  if ((s = dev_upload(ctx->scratch, value_materials, bytes, st)) != LUCENT_OK) return s;
+  DetachedArgs a;
+  a.scene = &scene->pt;
+  a.value_materials = static_cast<const float*>(ctx->scratch);
+  fill_frame(cam, settings, width, height, &a.frame);
+  a.spp = spp;
+  a.seed0 = seed0;
+  a.out = d_out_rgba;
+  if ((s = launch_detached(ctx, a, st)) != LUCENT_OK) return s;
+  // The host table must outlive the copy.
+  return stream_sync(st);
+}
+
+lucent_status lucent_scene_set_materials(lucent_ctx* ctx, lucent_scene* scene, const float* materials, uint32_t n) {
+  clear_error();
+  if (!ctx || !scene || !materials) return fail(LUCENT_E_INVALID, "null argument");
+  if (n != scene->pt.material_count) return fail(LUCENT_E_INVALID, "material count differs from the uploaded scene");
+  Status s = stream_sync(ctx->stream);
+  if (s != LUCENT_OK) return s;
+  if ((s = dev_upload(const_cast<float*>(scene->pt.materials), materials, size_t(n) * 40 * sizeof(float), ctx->stream)) !=
+      LUCENT_OK)
+    return s;
+  return stream_sync(ctx->stream);
+}
+
+lucent_status lucent_scene_update_env_texels(lucent_ctx* ctx, lucent_scene* scene, const float* rgb, uint32_t w,
+                                             uint32_t h) {
+  clear_error();
+  if (!ctx || !scene || !rgb) return fail(LUCENT_E_INVALID, "null argument");
+  if (!scene->lighting_set || int32_t(w) != scene->pt.env_w || int32_t(h) != scene->pt.env_h)
+    return fail(LUCENT_E_INVALID, "environment size differs from the scene's map");
+  std::vector<float> map4(size_t(w) * h * 4);
+  for (size_t i = 0; i < size_t(w) * h; ++i) {
+    map4[i * 4] = rgb[i * 3];
+    map4[i * 4 + 1] = rgb[i * 3 + 1];
+    map4[i * 4 + 2] = rgb[i * 3 + 2];
+    map4[i * 4 + 3] = 0.0f;
+  }
+  Status s = stream_sync(ctx->stream);
+  if (s != LUCENT_OK) return s;
+  if ((s = dev_upload(const_cast<float*>(scene->pt.env_map), map4.data(), map4.size() * sizeof(float), ctx->stream)) !=
+      LUCENT_OK)
+    return s;
+  return stream_sync(ctx->stream);
+}
+
+lucent_status lucent_scene_env_size(const lucent_scene* scene, uint32_t* w, uint32_t* h) {
+  if (!scene) return fail(LUCENT_E_INVALID, "null scene");
+  if (w) *w = uint32_t(scene->pt.env_w);
+  if (h) *h = uint32_t(scene->pt.env_h);
   return LUCENT_OK;
 }
 

@@ -565,6 +565,7 @@ struct LightSample {
   v3 dir;
   float dist, pdf;
   v3 radiance;
+  float u, v;  // environment samples: the map coordinates looked up
 };
 
 __device__ inline v3 env_to_local(const PtScene& s, v3 d) {
@@ -577,7 +578,26 @@ __device__ inline void equirect_uv(v3 l, float& u, float& v) {
   u = atan2f(l.z, l.x) * (0.5f * INV_PI) + 0.5f;
   v = acosf(clampf(l.y, -1.0f, 1.0f)) * INV_PI;
 }
-// LINEAR filter, REPEAT in u, CLAMP_TO_EDGE in v (renderer.ts:397-403).
+// Texels and weights of the bilinear lookup at (u, v): LINEAR filter,
+// REPEAT in u, CLAMP_TO_EDGE in v (renderer.ts:397-403).  Order: (x0,y0),
+// (x1,y0), (x0,y1), (x1,y1).
+__device__ inline void env_taps(const PtScene& s, float u, float v, int idx[4], float w[4]) {
+  const float x = u * float(s.env_w) - 0.5f, y = v * float(s.env_h) - 0.5f;
+  const float fx = floorf(x), fy = floorf(y);
+  const float tx = x - fx, ty = y - fy;
+  int x0 = int(fx) % s.env_w;
+  if (x0 < 0) x0 += s.env_w;
+  const int x1 = x0 + 1 == s.env_w ? 0 : x0 + 1;
+  const int y0 = min(max(int(fy), 0), s.env_h - 1), y1 = min(max(int(fy) + 1, 0), s.env_h - 1);
+  idx[0] = y0 * s.env_w + x0;
+  idx[1] = y0 * s.env_w + x1;
+  idx[2] = y1 * s.env_w + x0;
+  idx[3] = y1 * s.env_w + x1;
+  w[0] = (1.0f - tx) * (1.0f - ty);
+  w[1] = tx * (1.0f - ty);
+  w[2] = (1.0f - tx) * ty;
+  w[3] = tx * ty;
+}
 __device__ inline v3 env_lookup(const PtScene& s, float u, float v) {
   const float x = u * float(s.env_w) - 0.5f, y = v * float(s.env_h) - 0.5f;
   const float fx = floorf(x), fy = floorf(y);
@@ -630,6 +650,8 @@ __device__ inline LightSample sample_environment(const PtScene& s, Rng& rng) {
   ls.dist = INF;
   ls.pdf = st > 1e-6f ? pdf_uv / (2.0f * PI * PI * st) : 0.0f;
   ls.radiance = env_lookup(s, u, v);
+  ls.u = u;
+  ls.v = v;
   return ls;
 }
 __device__ inline v3 environment(const PtScene& s, v3 d, bool specular, float bsdf_pdf) {

@@ -1,0 +1,107 @@
+"""Differentiable rendering: forward/backward pair over the C ABI.
+
+forward() is Renderer.traceSamples into a fresh film (the image is the
+running mean of `spp` samples from seeds seed0..seed0+spp-1); backward()
+replays those same samples (lucent_backward) and returns the gradient of
+sum_pixels dL_dI · image with respect to every material's (albedo rgb,
+roughness, metallic) and every environment texel channel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+from .renderer import Renderer, RenderSettings, camera_struct
+from .runtime import Context
+from .scene import Scene, pack_materials
+
+PARAM_NAMES = ("albedo_r", "albedo_g", "albedo_b", "roughness", "metallic")
+
+
+class DiffRenderer:
+    def __init__(self, ctx: Context, scene: Scene, width: int, height: int, max_bounces: int = 6):
+        self.ctx = ctx
+        self.scene = scene
+        self.width, self.height = width, height
+        self.renderer = Renderer(ctx, width, height, RenderSettings(max_bounces=max_bounces, firefly_clamp=0.0))
+        self.renderer.load_scene(scene)
+        self.materials = list(self.renderer.packed.material_list)
+        ew, eh, env = scene.env.map
+        self.env_size = (ew, eh)
+        self.env = np.asarray(env, np.float32).copy()
+
+    @property
+    def settings(self):
+        return self.renderer.settings
+
+    def set_materials(self, materials) -> None:
+        packed = pack_materials(materials)
+        _native.check(_native.lib().lucent_scene_set_materials(self.ctx.handle, self.renderer.device_scene.handle,
+                                                               packed.ctypes.data, len(materials)))
+        self.materials = list(materials)
+
+    def set_env_texels(self, rgb) -> None:
+        arr = np.ascontiguousarray(np.asarray(rgb, np.float32).reshape(-1))
+        _native.check(_native.lib().lucent_scene_update_env_texels(self.ctx.handle, self.renderer.device_scene.handle,
+                                                                   arr.ctypes.data, *self.env_size))
+        self.env = arr.copy()
+
+    def forward(self, spp: int, seed0: int = 0, stream: Optional[int] = None) -> np.ndarray:
+        """Image (H, W, 3) as float64, rows bottom-up."""
+        r = self.renderer
+        r.film.set_seed(seed0)
+        r.reset()
+        r.trace_samples(spp, stream=stream)
+        return r.film.read()["color"][..., :3].astype(np.float64)
+
+    def detached_forward(self, value_materials, spp: int, seed0: int = 0) -> np.ndarray:
+        """Image (H, W, 3) of the detached estimator (lucent_detached_forward):
+        sampling from the uploaded materials, BSDF values from
+        `value_materials`."""
+        import torch
+        packed = pack_materials(value_materials)
+        if len(value_materials) != len(self.materials):
+            raise ValueError("material count differs from the uploaded scene")
+        h, w = self.height, self.width
+        out = torch.empty(h * w * 4, dtype=torch.float32, device=torch.device("cuda:0"))
+        cam = camera_struct(self.scene.camera)
+        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0)
+        torch.cuda.synchronize()
+        _native.check(_native.lib().lucent_detached_forward(
+            self.ctx.handle, self.renderer.device_scene.handle, C.byref(cam), C.byref(st), w, h, spp, seed0,
+            packed.ctypes.data, len(value_materials), out.data_ptr(), None))
+        self.ctx.synchronize()
+        return out.cpu().numpy().reshape(h, w, 4)[..., :3].astype(np.float64)
+
+    def backward(self, dl_di: np.ndarray, spp: int, seed0: int = 0, env: bool = True):
+        """Gradients (material_count x 5, env (h, w, 3)) of sum dl_di · image."""
+        import torch
+        dev = torch.device("cuda:0")
+        h, w = self.height, self.width
+        d = np.zeros((h, w, 4), np.float32)
+        d[..., :3] = dl_di
+        d_dl = torch.from_numpy(d).to(dev)
+        nm = len(self.materials)
+        g_mat = torch.zeros(nm * 5, dtype=torch.float64, device=dev)
+        ew, eh = self.env_size
+        g_env = torch.zeros(ew * eh * 3, dtype=torch.float64, device=dev) if env else None
+        cam = camera_struct(self.scene.camera)
+        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0)
+        torch.cuda.synchronize()
+        _native.check(_native.lib().lucent_backward(
+            self.ctx.handle, self.renderer.device_scene.handle, C.byref(cam), C.byref(st), w, h, spp, seed0,
+            d_dl.data_ptr(), g_mat.data_ptr(), g_env.data_ptr() if env else None, None))
+        self.ctx.synchronize()
+        gm = g_mat.cpu().numpy().reshape(nm, 5)
+        ge = g_env.cpu().numpy().reshape(eh, ew, 3) if env else None
+        return gm, ge
+
+    def close(self) -> None:
+        self.renderer.close()
+
+
+def l2_loss(image: np.ndarray, target: np.ndarray) -> float:
+    return float(((image - target) ** 2).sum())

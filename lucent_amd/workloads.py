@@ -231,3 +231,65 @@ def knot_scene(seed: int = 7, knots: int = 6, tri_range=(30000, 40000), radial: 
 
 
 PT_WIDTH, PT_HEIGHT, PT_SPP, PT_DEPTH = 1440, 900, 64, 8
+
+
+# ---------------------------------------------------------------------------
+# Differentiable-rendering workload (512x512, 32 spp, depth 6)
+# ---------------------------------------------------------------------------
+DIFF_WIDTH, DIFF_HEIGHT, DIFF_SPP, DIFF_DEPTH = 512, 512, 32, 6
+DIFF_ENV = (256, 128)
+
+
+def diff_material(rng: np.random.Generator):
+    """Opaque layered BSDF with continuous parameters: albedo U[0.05,0.95]^3,
+    roughness U[0.15,0.9], metallic U[0,1], clearcoat Bernoulli(0.5) with coat
+    roughness U[0,0.3], sheen s ~ U[0,1] as (s,s,s) with roughness 0.5.  No
+    transmission: the reference's opaque/glass choice (trace.frag:186) is a
+    stochastic blend, not an importance-sampled lobe, and needs a boundary
+    term that this adjoint does not model."""
+    from .scene import Material
+    coat = rng.random() < 0.5
+    sheen = float(rng.uniform(0, 1))
+    return Material(albedo=tuple(float(x) for x in rng.uniform(0.05, 0.95, 3)), roughness=float(rng.uniform(0.15, 0.9)),
+                    metallic=float(rng.uniform(0, 1)), clearcoat=1.0 if coat else 0.0,
+                    clearcoat_roughness=float(rng.uniform(0, 0.3)) if coat else 0.0,
+                    sheen_color=(sheen, sheen, sheen), sheen_roughness=0.5)
+
+
+def spheres_scene(seed: int = 11):
+    """16 fBm-displaced icospheres (subdivision 3, 1,280 triangles each: 20,480
+    in all) on a 4x4 grid, 16 materials, lit by a 256x128 synthetic HDRI (sky
+    gradient plus a sun lobe with sigma 3 deg, peak 500)."""
+    from . import scene as S
+    rng = np.random.Generator(np.random.PCG64(seed))
+    meshes = []
+    v, f = icosphere(3)
+    for k in range(16):
+        r = 0.55 * (1.0 + 0.12 * fbm3(v * 1.5, seed * 31 + k))
+        pos = (v * r[:, None])[f]
+        nrm = v[f]
+        m = S.Mesh(pos.reshape(-1).astype(np.float32), nrm.reshape(-1).astype(np.float32))
+        col, row = k % 4, k // 4
+        m = S.transform(m, translate=(-1.8 + 1.2 * col, -1.8 + 1.2 * row, 0.0))
+        meshes.append(S.MeshInstance(m, diff_material(rng)))
+    w, h = DIFF_ENV
+    env = S.Environment(map=(w, h, synthetic_hdri(w, h, sun_elevation=40.0, sun_azimuth=-30.0, sigma_deg=3.0,
+                                                  peak=500.0)), rotation=0.0, intensity=1.0, background=True)
+    cam = S.Camera(position=(0.0, 0.0, 6.5), target=(0.0, 0.0, 0.0), fov=45.0, aperture=0.0, focus_distance=6.5)
+    return S.Scene(camera=cam, meshes=meshes, env=env, name="spheres")
+
+
+def perturb_params(materials, env_rgb, seed: int = 5, amount: float = 0.2):
+    """Seeded ±20% perturbation of every material's albedo, roughness and
+    metallic (clamped to their valid ranges) and of every env texel channel."""
+    from dataclasses import replace
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for m in materials:
+        a = np.clip(np.array(m.albedo) * rng.uniform(1 - amount, 1 + amount, 3), 0.0, 1.0)
+        out.append(replace(m, albedo=tuple(float(x) for x in a),
+                           roughness=float(np.clip(m.roughness * rng.uniform(1 - amount, 1 + amount), 0.05, 1.0)),
+                           metallic=float(np.clip(m.metallic * rng.uniform(1 - amount, 1 + amount), 0.0, 1.0))))
+    env = (np.asarray(env_rgb, np.float32) * rng.uniform(1 - amount, 1 + amount, np.asarray(env_rgb).size)
+           ).astype(np.float32)
+    return out, env

@@ -465,6 +465,43 @@ class PtOracle:
             return out, ps, dict(zip(self.COUNT_NAMES, tot[: len(self.COUNT_NAMES)].tolist()))
         return out, ps
 
+    def set_materials(self, packed_materials) -> None:
+        """Replaces the material texels the oracle scene points to (same count)."""
+        m = np.ascontiguousarray(packed_materials, np.float32).reshape(-1)
+        self.keep.append(m)
+        self.scene.materials = m.ctypes.data
+
+    def set_env_texels(self, rgb) -> None:
+        e = np.ascontiguousarray(rgb, np.float32).reshape(-1)
+        self.keep.append(e)
+        self.scene.env_rgb = e.ctypes.data
+
+    def adjoint_pixels(self, frame: OrFrame, seed0: int, samples: int, xy, dl3, material_count: int,
+                       threads: int | None = None):
+        """FP64 path-replay gradient of sum_p dl3[p] · mean radiance[p] over the listed pixels:
+        (material_count x 5 array, env (env_h, env_w, 3) array)."""
+        l = _pt_lib()
+        l.or_adjoint_pixels.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), _U32, _U32, _P, _P, _U32, _P, _P]
+        l.or_adjoint_pixels.restype = C.c_int
+        xy = np.ascontiguousarray(xy, np.uint32).reshape(-1, 2)
+        dl = np.ascontiguousarray(dl3, np.float64).reshape(-1, 3)
+        n = xy.shape[0]
+        threads = threads or min(16, os.cpu_count() or 1)
+        chunks = _split(n, threads)
+        envn = self.scene.env_w * self.scene.env_h * 3
+        gm = np.zeros((len(chunks), material_count * 5))
+        ge = np.zeros((len(chunks), envn))
+
+        def run(arg):
+            i, (a, b) = arg
+            return l.or_adjoint_pixels(C.byref(self.scene), C.byref(frame), seed0, samples, _ptr(xy) + a * 8,
+                                       _ptr(dl) + a * 24, b - a, _ptr(gm) + i * gm.shape[1] * 8,
+                                       _ptr(ge) + i * envn * 8)
+
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(run, enumerate(chunks)))
+        return gm.sum(0).reshape(material_count, 5), ge.sum(0).reshape(self.scene.env_h, self.scene.env_w, 3)
+
 
 # ---------------------------------------------------------------------------
 # Denoiser oracle (denoise_oracle.c)

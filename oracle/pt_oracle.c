@@ -1230,3 +1230,227 @@ int or_sample_dielectric(const float *mat40, int front, const double *wo, uint32
   wi[2] = o.z;
   return ok;
 }
+
+/* ---- FP64 path-replay adjoint ------------------------------------------ */
+/* Derivatives of the opaque BSDF value f with respect to (albedo r, g, b,
+ * roughness, metallic) by central differences in double precision: an
+ * evaluation independent of the product's forward-mode dual numbers. */
+static int eval_grad_fd(const or_scene *sc, const surf_t *s, d3 wo, d3 wi, double f[3], double df[3][5]) {
+  double pdf;
+  d3 f0 = eval_opaque(sc, s, wo, wi, &pdf);
+  if (pdf <= 0 && f0.x == 0 && f0.y == 0 && f0.z == 0) {
+    if (dot(s->n, wi) <= 0 || dot(s->n, wo) <= 0) return 0;
+  }
+  f[0] = f0.x;
+  f[1] = f0.y;
+  f[2] = f0.z;
+  for (int i = 0; i < 5; i++) {
+    surf_t p = *s, m = *s;
+    double *vp, *vm, base;
+    switch (i) {
+      case 0: vp = &p.albedo.x; vm = &m.albedo.x; base = s->albedo.x; break;
+      case 1: vp = &p.albedo.y; vm = &m.albedo.y; base = s->albedo.y; break;
+      case 2: vp = &p.albedo.z; vm = &m.albedo.z; base = s->albedo.z; break;
+      case 3: vp = &p.roughness; vm = &m.roughness; base = s->roughness; break;
+      default: vp = &p.metallic; vm = &m.metallic; base = s->metallic; break;
+    }
+    double h = 1e-6;
+    *vp = base + h;
+    *vm = base - h;
+    double pp, pm;
+    d3 fp = eval_opaque(sc, &p, wo, wi, &pp), fm = eval_opaque(sc, &m, wo, wi, &pm);
+    df[0][i] = (fp.x - fm.x) / (2 * h);
+    df[1][i] = (fp.y - fm.y) / (2 * h);
+    df[2][i] = (fp.z - fm.z) / (2 * h);
+  }
+  return 1;
+}
+
+typedef struct {
+  double *mat;  /* material_count * 5 */
+  double *env;  /* env_w * env_h * 3 or NULL */
+} grads_t;
+
+static void env_taps_d(const or_scene *s, double u, double v, int idx[4], double w[4]) {
+  double x = u * s->env_w - 0.5, y = v * s->env_h - 0.5, fx = floor(x), fy = floor(y), tx = x - fx, ty = y - fy;
+  int x0 = ((int)fx % s->env_w + s->env_w) % s->env_w, x1 = (x0 + 1) % s->env_w;
+  int y0 = (int)fy, y1 = (int)fy + 1;
+  y0 = y0 < 0 ? 0 : (y0 >= s->env_h ? s->env_h - 1 : y0);
+  y1 = y1 < 0 ? 0 : (y1 >= s->env_h ? s->env_h - 1 : y1);
+  idx[0] = y0 * s->env_w + x0;
+  idx[1] = y0 * s->env_w + x1;
+  idx[2] = y1 * s->env_w + x0;
+  idx[3] = y1 * s->env_w + x1;
+  w[0] = (1 - tx) * (1 - ty);
+  w[1] = tx * (1 - ty);
+  w[2] = (1 - tx) * ty;
+  w[3] = tx * ty;
+}
+static void add_env_grad(const or_scene *s, grads_t *g, double u, double v, d3 wgt) {
+  if (!g->env) return;
+  int idx[4];
+  double w[4];
+  env_taps_d(s, u, v, idx, w);
+  for (int k = 0; k < 4; k++) {
+    double sc = w[k] * s->env_intensity;
+    g->env[(size_t)idx[k] * 3] += wgt.x * sc;
+    g->env[(size_t)idx[k] * 3 + 1] += wgt.y * sc;
+    g->env[(size_t)idx[k] * 3 + 2] += wgt.z * sc;
+  }
+}
+static void add_f_grad(grads_t *g, int mat, d3 coeff, double df[3][5]) {
+  double c[3] = {coeff.x, coeff.y, coeff.z};
+  for (int i = 0; i < 5; i++) g->mat[mat * 5 + i] += c[0] * df[0][i] + c[1] * df[1][i] + c[2] * df[2][i];
+}
+
+/* Replay of trace_path accumulating d(dl · L)/dθ (material parameters through
+ * f, environment texels through every lookup); sampling decisions, pdfs, MIS
+ * weights and Russian roulette held constant. */
+static void replay_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t *r, d3 L, d3 dl, grads_t *g) {
+  d3 lr = L, beta = V(1, 1, 1);
+  int spec = 1;
+  double bpdf = 0;
+  for (int bounce = 0; bounce <= (int)f->max_bounces; bounce++) {
+    hit_t h;
+    if (!trace_scene(sc, ro, rd, &h, r)) {
+      if (!(bounce == 0 && sc->env_background == 0)) {
+        double w = (!spec && sc->env_sampling) ? mis(bpdf, env_pdf(sc, rd)) : 1;
+        double u, v;
+        eq_uv(env_local(sc, rd), &u, &v);
+        add_env_grad(sc, g, u, v, scl(mul(dl, beta), w));
+      }
+      return;
+    }
+    d3 wo = neg(rd), em;
+    surf_t s = surface_at(sc, &h, wo, &em);
+    d3 ng = h.ng;
+    if (!h.front && !s.thin && s.transmission > 0) {
+      d3 a = scl(s.absorption, -h.t);
+      beta = mul(beta, V(exp(a.x), exp(a.y), exp(a.z)));
+    }
+    if (em.x > 0 || em.y > 0 || em.z > 0) {
+      double w = 1;
+      if (!spec && h.light >= 0) w = mis(bpdf, light_pdf(sc, h.light, ro, rd, h.t) / sc->light_count);
+      lr = sub(lr, scl(mul(beta, em), w));
+      if (h.light >= 0) return;
+    }
+    int glass = s.transmission > 0 && rnd(r) < s.transmission;
+    if (glass) {
+      d3 wi, w;
+      int refl;
+      if (!sample_dielectric(&s, h.front, ng, wo, r, &wi, &w, &refl)) return;
+      if (!refl) {
+        if (s.albedo.x > 0) g->mat[h.material * 5 + 0] += dl.x * lr.x / s.albedo.x;
+        if (s.albedo.y > 0) g->mat[h.material * 5 + 1] += dl.y * lr.y / s.albedo.y;
+        if (s.albedo.z > 0) g->mat[h.material * 5 + 2] += dl.z * lr.z / s.albedo.z;
+      }
+      beta = mul(beta, w);
+      ro = offset_ray(h.pos, refl ? ng : neg(ng));
+      rd = wi;
+      spec = 1;
+    } else {
+      d3 so = offset_ray(h.pos, ng), n = s.n;
+      double fv[3], dfv[3][5];
+      if (sc->sun_enabled) {
+        double u0 = rnd(r), u1 = rnd(r);
+        d3 t, b, sd = Vf(sc->sun_dir);
+        basis(sd, &t, &b);
+        d3 wi = to_world(uni_cone(u0, u1, sc->sun_cos), t, b, sd);
+        double ndl = dot(n, wi);
+        if (ndl > 0 && dot(ng, wi) > 0 && !occluded(sc, so, wi, INF_D, r)) {
+          double pb;
+          eval_opaque(sc, &s, wo, wi, &pb);
+          double k = ndl * mis(sc->sun_pdf, pb) / sc->sun_pdf;
+          if (eval_grad_fd(sc, &s, wo, wi, fv, dfv)) {
+            d3 li = scl(Vf(sc->sun_radiance), k);
+            lr = sub(lr, mul(mul(beta, V(fv[0], fv[1], fv[2])), li));
+            add_f_grad(g, h.material, mul(mul(dl, beta), li), dfv);
+          }
+        }
+      }
+      if (sc->env_sampling) {
+        lsample ls = sample_env(sc, r);
+        double ndl = dot(n, ls.dir);
+        if (ls.pdf > 0 && ndl > 0 && dot(ng, ls.dir) > 0 && !occluded(sc, so, ls.dir, INF_D, r)) {
+          double pb;
+          eval_opaque(sc, &s, wo, ls.dir, &pb);
+          double k = ndl * mis(ls.pdf, pb) / ls.pdf;
+          if (eval_grad_fd(sc, &s, wo, ls.dir, fv, dfv)) {
+            d3 fvec = V(fv[0], fv[1], fv[2]);
+            lr = sub(lr, scl(mul(mul(beta, fvec), ls.rad), k));
+            add_f_grad(g, h.material, scl(mul(mul(dl, beta), ls.rad), k), dfv);
+            /* The sampled map coordinates, recovered from the direction. */
+            double u, v;
+            eq_uv(env_local(sc, ls.dir), &u, &v);
+            add_env_grad(sc, g, u, v, scl(mul(mul(dl, beta), fvec), k));
+          }
+        }
+      }
+      if (sc->light_count > 0) {
+        int li = (int)((float)rnd(r) * (float)sc->light_count);
+        if (li > (int)sc->light_count - 1) li = (int)sc->light_count - 1;
+        double u0 = rnd(r), u1 = rnd(r);
+        lsample ls = sample_light(sc, li, h.pos, u0, u1);
+        double ndl = dot(n, ls.dir);
+        if (ls.pdf > 0 && ndl > 0 && dot(ng, ls.dir) > 0 && !occluded(sc, so, ls.dir, ls.dist * (1 - 1e-4), r)) {
+          double pl = ls.pdf / sc->light_count, pb;
+          eval_opaque(sc, &s, wo, ls.dir, &pb);
+          double k = ndl * mis(pl, pb) / pl;
+          if (eval_grad_fd(sc, &s, wo, ls.dir, fv, dfv)) {
+            lr = sub(lr, scl(mul(mul(beta, V(fv[0], fv[1], fv[2])), ls.rad), k));
+            add_f_grad(g, h.material, scl(mul(mul(dl, beta), ls.rad), k), dfv);
+          }
+        }
+      }
+      bsample bs = sample_opaque(sc, &s, wo, r);
+      if (bs.pdf <= 0 || dot(bs.dir, ng) <= 0) return;
+      if (eval_grad_fd(sc, &s, wo, bs.dir, fv, dfv)) {
+        d3 coeff = V(fv[0] > 1e-300 ? dl.x * lr.x / fv[0] : 0, fv[1] > 1e-300 ? dl.y * lr.y / fv[1] : 0,
+                     fv[2] > 1e-300 ? dl.z * lr.z / fv[2] : 0);
+        add_f_grad(g, h.material, coeff, dfv);
+      }
+      beta = mul(beta, bs.w);
+      bpdf = bs.pdf;
+      spec = 0;
+      ro = so;
+      rd = bs.dir;
+    }
+    if (bounce >= 2) {
+      double p = mind(0.95, max3(beta));
+      if (rnd(r) >= p) return;
+      beta = scl(beta, 1 / p);
+    }
+  }
+}
+
+/* Gradient of sum_p dl[p] · mean_radiance[p] over the listed pixels (rows
+ * bottom-up), `samples` samples from seed0; accumulates into grad_mat
+ * (material_count * 5) and grad_env (env texels * 3, may be NULL). */
+int or_adjoint_pixels(const or_scene *sc, const or_frame *f, uint32_t seed0, uint32_t samples, const uint32_t *xy,
+                      const double *dl3, uint32_t npix, double *grad_mat, double *grad_env) {
+  grads_t g = {grad_mat, grad_env};
+  for (uint32_t p = 0; p < npix; p++) {
+    uint32_t x = xy[2 * p], y = xy[2 * p + 1];
+    d3 dl = scl(V(dl3[p * 3], dl3[p * 3 + 1], dl3[p * 3 + 2]), 1.0 / samples);
+    for (uint32_t k = 0; k < samples; k++) {
+      rng_t r = {(x + y * f->width) * 1973u + (seed0 + k) * 9277u + 26699u};
+      pcg(&r);
+      double r0 = rnd(&r), r1 = rnd(&r);
+      double ux = (x + 0.5 + r0) / f->width * 2 - 1, uy = (y + 0.5 + r1) / f->height * 2 - 1;
+      double aspect = (double)f->width / f->height;
+      d3 F = Vf(f->cam_fwd), R = Vf(f->cam_right), U = Vf(f->cam_up), P = Vf(f->cam_pos);
+      d3 dir = nrm(add(add(F, scl(R, ux * aspect * f->tan_half_fov)), scl(U, uy * f->tan_half_fov)));
+      d3 focus = add(P, scl(dir, f->focus_dist / dot(dir, F)));
+      double l0 = rnd(&r), l1 = rnd(&r), lx, ly;
+      concentric(l0, l1, &lx, &ly);
+      d3 ro = add(add(P, scl(R, lx * f->aperture)), scl(U, ly * f->aperture));
+      d3 rd = nrm(sub(focus, ro));
+      rng_t replay_rng = r;
+      guide_t gd;
+      d3 L = trace_path(sc, f, ro, rd, &r, &gd);
+      if (!isfinite(L.x) || !isfinite(L.y) || !isfinite(L.z)) continue;
+      replay_path(sc, f, ro, rd, &replay_rng, L, dl, &g);
+    }
+  }
+  return 0;
+}

@@ -1,0 +1,216 @@
+"""GPU path-replay adjoint: against the FP64 oracle adjoint on 64 sampled
+pixels; against exact central differences where the estimator is smooth in
+the parameter; against central differences (h = 1e-3, common random numbers)
+of the detached estimator's L2 loss on the differentiable-rendering workload
+(sample by sample); and, in expectation, against central differences of the
+renderer itself, whose sampling moves with the parameters."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lucent_amd import scene as S
+from lucent_amd import workloads
+from lucent_amd.diff import DiffRenderer
+
+pytestmark = pytest.mark.gpu
+
+W, H, SPP, DEPTH = workloads.DIFF_WIDTH, workloads.DIFF_HEIGHT, workloads.DIFF_SPP, workloads.DIFF_DEPTH
+
+
+@pytest.fixture(scope="module")
+def diff(ctx):
+    sc = workloads.spheres_scene()
+    d = DiffRenderer(ctx, sc, W, H, DEPTH)
+    yield d
+    d.close()
+
+
+def test_matches_fp64_adjoint_on_64_pixels(diff):
+    rng = np.random.default_rng(3)
+    xy = np.stack([rng.integers(96, W - 96, 64), rng.integers(96, H - 96, 64)], -1).astype(np.uint32)
+    dl = np.zeros((H, W, 3))
+    vals = rng.standard_normal((64, 3))
+    dl[xy[:, 1], xy[:, 0]] = vals
+    gm, ge = diff.backward(dl, SPP, 0)
+    packed = diff.renderer.packed
+    o = O.PtOracle(packed, diff.scene)
+    f = o.frame(diff.scene.camera, W, H, DEPTH, 0.0)
+    om, oe = o.adjoint_pixels(f, 0, SPP, xy, vals, packed.material_count)
+    scale = np.abs(om).max()
+    err = np.abs(gm - om) / scale
+    print(f"material grads vs FP64 adjoint: max |diff| / max|g| = {err.max():.2e}")
+    assert err.max() < 2e-2
+    # Entry-wise agreement for the dominant entries.
+    big = np.abs(om) > 0.05 * scale
+    rel = np.abs(gm[big] - om[big]) / np.abs(om[big])
+    print(f"dominant entries: median rel {np.median(rel):.2e}, max rel {rel.max():.2e}")
+    assert np.median(rel) < 1e-2
+    es = np.abs(oe).max()
+    eerr = np.abs(ge - oe).max() / es
+    print(f"env grads vs FP64 adjoint: max |diff| / max|g| = {eerr:.2e}; sums {ge.sum():.6g} vs {oe.sum():.6g}")
+    assert eerr < 5e-2
+    assert abs(ge.sum() - oe.sum()) <= 2e-2 * np.abs(oe).sum()
+
+
+def test_exact_where_smooth(ctx):
+    """Depth 2, dielectric coat-free materials: the image is a smooth function of
+    base colour and env texels under common random numbers, so the adjoint must
+    match central differences to float32 accuracy."""
+    import sys
+    import pathlib
+    sys.path.insert(0, str(pathlib.Path(__file__).parent))
+    from test_adjoint_oracle import small_scene
+    sc = small_scene()
+    w, h, spp = 96, 64, 4
+    d = DiffRenderer(ctx, sc, w, h, 2)
+    rng = np.random.default_rng(1)
+    dl = rng.standard_normal((h, w, 3))
+    gm, ge = d.backward(dl, spp, 0)
+
+    def obj():
+        return float((d.forward(spp, 0) * dl).sum())
+
+    mats = list(d.materials)
+    for mi in range(3):
+        for c in range(3):
+            hh = 2e-3
+            up = [replace(m, albedo=tuple(a + (hh if (j == mi and k == c) else 0) for k, a in enumerate(m.albedo)))
+                  for j, m in enumerate(mats)]
+            dn = [replace(m, albedo=tuple(a - (hh if (j == mi and k == c) else 0) for k, a in enumerate(m.albedo)))
+                  for j, m in enumerate(mats)]
+            d.set_materials(up)
+            fp = obj()
+            d.set_materials(dn)
+            fm = obj()
+            d.set_materials(mats)
+            fd = (fp - fm) / (2 * hh)
+            assert fd == pytest.approx(gm[mi, c], rel=5e-3, abs=1e-3 * np.abs(gm).max()), (mi, c)
+    env = d.env.reshape(d.env_size[1], d.env_size[0], 3).copy()
+    for idx in np.argsort(np.abs(ge).reshape(-1))[::-1][:4]:
+        y, x, c = np.unravel_index(idx, ge.shape)
+        hh = 1e-2 * max(1.0, float(env[y, x, c]))
+        ep, em = env.copy(), env.copy()
+        ep[y, x, c] += hh
+        em[y, x, c] -= hh
+        d.set_env_texels(ep)
+        fp = obj()
+        d.set_env_texels(em)
+        fm = obj()
+        d.set_env_texels(env)
+        fd = (fp - fm) / (2 * hh)
+        assert fd == pytest.approx(ge[y, x, c], rel=5e-3, abs=1e-3 * np.abs(ge).max()), (y, x, c)
+    d.close()
+
+
+def test_detached_forward_is_the_image(diff):
+    """At the scene's own parameters the detached estimator is the rendered
+    image of the same seeds (running mean vs sum / spp rounding only)."""
+    img = diff.forward(SPP, 0)
+    det = diff.detached_forward(diff.materials, SPP, 0)
+    err = np.abs(det - img)
+    print(f"detached vs rendered: max |diff| {err.max():.3e}, image max {img.max():.3e}")
+    assert np.all(err <= 1e-4 * np.abs(img) + 1e-6 * max(1.0, float(np.abs(img).mean())))
+
+
+def _bump(m, p, s):
+    if p < 3:
+        a = list(m.albedo)
+        a[p] += s
+        return replace(m, albedo=tuple(a))
+    if p == 3:
+        return replace(m, roughness=m.roughness + s)
+    return replace(m, metallic=m.metallic + s)
+
+
+def test_loss_gradient_matches_central_differences(diff):
+    """The differentiable-rendering configuration: L2 loss against a target
+    rendered with ±20% perturbed parameters; all 16x5 material gradients
+    against central differences (h = 1e-3, the same 32 seeds) of the detached
+    estimator, and the env-map gradient's aggregate d/ds (env <- env (1 + s))
+    against central differences of the renderer itself (the env importance
+    sampler stays the one built at set-up, so the env is detached already)."""
+    base_mats = list(diff.materials)
+    base_env = diff.env.copy()
+    pm, penv = workloads.perturb_params(base_mats, base_env, seed=5)
+    diff.set_materials(pm)
+    diff.set_env_texels(penv)
+    target = diff.forward(128, 1000)
+    diff.set_materials(base_mats)
+    diff.set_env_texels(base_env)
+    img = diff.forward(SPP, 0)
+    dl = 2 * (img - target)
+    gm, ge = diff.backward(dl, SPP, 0)
+
+    def loss(mats):
+        return float(((diff.detached_forward(mats, SPP, 0) - target) ** 2).sum())
+
+    h = 1e-3
+    fd = np.zeros((16, 5))
+    for mi in range(16):
+        for p in range(5):
+            lp = loss([_bump(m, p, h) if j == mi else m for j, m in enumerate(base_mats)])
+            lm = loss([_bump(m, p, -h) if j == mi else m for j, m in enumerate(base_mats)])
+            fd[mi, p] = (lp - lm) / (2 * h)
+    g = gm[:16]
+    scale = np.abs(fd).max()
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-12)
+    sig = np.abs(fd) > 0.02 * scale
+    np.set_printoptions(precision=4, suppress=True)
+    print(f"material loss gradients: {sig.sum()} of 80 significant; rel error median {np.median(rel[sig]):.2e}, "
+          f"p90 {np.quantile(rel[sig], 0.9):.2e}, max {rel[sig].max():.2e}; max |diff|/max|g| "
+          f"{np.abs(g - fd).max() / scale:.2e}")
+    print("adjoint:\n", g)
+    print("central differences:\n", fd)
+    assert np.median(rel[sig]) < 1e-2
+    assert np.quantile(rel[sig], 0.9) < 1e-2
+    assert np.abs(g - fd).max() / scale < 1e-2
+    diff.set_env_texels(base_env * (1 + h))
+    lp = float(((diff.forward(SPP, 0) - target) ** 2).sum())
+    diff.set_env_texels(base_env * (1 - h))
+    lm = float(((diff.forward(SPP, 0) - target) ** 2).sum())
+    diff.set_env_texels(base_env)
+    fd_scale = (lp - lm) / (2 * h)
+    ad_scale = float((ge.reshape(-1) * base_env.astype(np.float64)).sum())
+    print(f"env scale gradient: adjoint {ad_scale:.6g}, central differences {fd_scale:.6g}")
+    assert ad_scale == pytest.approx(fd_scale, rel=1e-2)
+
+
+def test_detached_estimator_is_unbiased(ctx):
+    """Holding sampling constant must not bias the gradient: over 16 batches
+    of 64 seeds, the adjoint of a fixed linear image functional agrees in
+    expectation with central differences (h = 0.02) of the renderer itself,
+    whose sampling does move with the parameters.  Tolerance: 4 standard
+    errors of the per-batch difference plus 3% of the gradient."""
+    sc = workloads.spheres_scene()
+    w = h_ = 128
+    d = DiffRenderer(ctx, sc, w, h_, DEPTH)
+    try:
+        rng = np.random.default_rng(11)
+        weights = rng.uniform(0.0, 1.0, (h_, w, 3))
+        base = list(d.materials)
+        batches, spp, hh = 16, 64, 0.02
+        gs = np.stack([d.backward(weights, spp, b * spp, env=False)[0][:16] for b in range(batches)])
+        mean_g = gs.mean(0)
+        order = np.argsort(np.abs(mean_g).reshape(-1))[::-1][:8]
+        worst = 0.0
+        for idx in order:
+            mi, p = np.unravel_index(idx, (16, 5))
+            fds = []
+            for b in range(batches):
+                d.set_materials([_bump(m, p, hh) if j == mi else m for j, m in enumerate(base)])
+                jp = float((d.forward(spp, b * spp) * weights).sum())
+                d.set_materials([_bump(m, p, -hh) if j == mi else m for j, m in enumerate(base)])
+                jm = float((d.forward(spp, b * spp) * weights).sum())
+                fds.append((jp - jm) / (2 * hh))
+            d.set_materials(base)
+            delta = gs[:, mi, p] - np.array(fds)
+            se = delta.std(ddof=1) / np.sqrt(batches)
+            tol = 4 * se + 0.03 * abs(mean_g[mi, p])
+            print(f"material {mi} param {p}: adjoint {mean_g[mi, p]:.5g}, renderer FD {np.mean(fds):.5g}, "
+                  f"mean diff {delta.mean():.3g} (SE {se:.3g})")
+            worst = max(worst, abs(delta.mean()) / tol)
+        assert worst < 1.0
+    finally:
+        d.close()

@@ -1,0 +1,100 @@
+"""Pins the FP64 adjoint oracle against central finite differences of the
+FP64 path tracer itself, with common random numbers, where the estimator is
+a smooth function of the parameter (no sampling decision depends on it):
+  * environment texels (the sampler table is held fixed, so every decision
+    is independent of the texel values and the image is linear in them);
+  * base colour of dielectric (metallic 0), coat-free materials at depth 2
+    (lobe choice uses the dielectric F0; Russian roulette first acts after
+    the last bounce).
+There the path-replay gradient must equal the derivative of the realisation
+to rounding, which pins the replay logic (remaining-radiance bookkeeping,
+NEE and escape terms, bilinear texel weights)."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+import oracle as O
+from lucent_amd import scene as S
+from lucent_amd import workloads
+
+
+def small_scene():
+    rng = np.random.Generator(np.random.PCG64(2))
+    v, f = workloads.icosphere(2)
+    meshes = []
+    for k in range(3):
+        pos = (v * 0.6)[f] + np.array([-1.3 + 1.3 * k, 0.0, 0.0])
+        m = S.Mesh(pos.reshape(-1).astype(np.float32), v[f].reshape(-1).astype(np.float32))
+        mat = S.Material(albedo=tuple(rng.uniform(0.2, 0.8, 3)), roughness=0.4 + 0.2 * k, metallic=0.0)
+        meshes.append(S.MeshInstance(m, mat))
+    w, h = 32, 16
+    env = S.Environment(map=(w, h, workloads.synthetic_hdri(w, h, sigma_deg=12.0, peak=20.0)), background=True)
+    cam = S.Camera((0.0, 0.5, 4.0), (0.0, 0.0, 0.0), 50.0, 0.0, 4.0)
+    return S.Scene(camera=cam, meshes=meshes, env=env, ground=S.Ground(True, -0.7, S.matte((0.5, 0.5, 0.5))))
+
+
+def setup(depth=2):
+    sc = small_scene()
+    packed = S.pack_scene(sc)
+    o = O.PtOracle(packed, sc)
+    W, H = 24, 16
+    f = o.frame(sc.camera, W, H, depth, 0.0)
+    yy, xx = np.mgrid[0:H, 0:W]
+    xy = np.stack([xx.reshape(-1), yy.reshape(-1)], -1)
+    dl = np.random.default_rng(0).standard_normal((xy.shape[0], 3))
+    return sc, packed, o, f, xy, dl
+
+
+def objective(o, f, xy, dl, spp):
+    out, _ = o.trace_pixels(f, 0, spp, xy)
+    return float((out[:, :3] * dl).sum())
+
+
+def test_env_texel_gradient_equals_finite_difference():
+    sc, packed, o, f, xy, dl = setup()
+    spp = 4
+    gm, ge = o.adjoint_pixels(f, 0, spp, xy, dl, packed.material_count)
+    env = np.asarray(sc.env.map[2], np.float32).reshape(16, 32, 3).copy()
+    flat = np.argsort(np.abs(ge).reshape(-1))[::-1][:6]
+    for idx in flat:
+        y, x, c = np.unravel_index(idx, ge.shape)
+        h = 1e-3 * max(1.0, float(env[y, x, c]))
+        ep, em = env.copy(), env.copy()
+        ep[y, x, c] += h
+        em[y, x, c] -= h
+        o.set_env_texels(ep)
+        fp = objective(o, f, xy, dl, spp)
+        o.set_env_texels(em)
+        fm = objective(o, f, xy, dl, spp)
+        o.set_env_texels(env)
+        # float32 texels: the perturbation actually applied is the rounded one.
+        hp = float(np.float32(env[y, x, c] + h) - env[y, x, c])
+        hm = float(env[y, x, c] - np.float32(env[y, x, c] - h))
+        fd = (fp - fm) / (hp + hm)
+        assert fd == pytest.approx(ge[y, x, c], rel=1e-6, abs=1e-9), (y, x, c)
+
+
+def test_albedo_gradient_equals_finite_difference_at_depth_2():
+    sc, packed, o, f, xy, dl = setup(depth=2)
+    spp = 4
+    gm, _ = o.adjoint_pixels(f, 0, spp, xy, dl, packed.material_count)
+    mats = packed.material_list
+    for mi in range(3):
+        for c in range(3):
+            h = 1e-4
+            mp = [replace(m, albedo=tuple(a + (h if (j == mi and k == c) else 0) for k, a in enumerate(m.albedo)))
+                  for j, m in enumerate(mats)]
+            mm = [replace(m, albedo=tuple(a - (h if (j == mi and k == c) else 0) for k, a in enumerate(m.albedo)))
+                  for j, m in enumerate(mats)]
+            P = S.pack_materials(mp)
+            M = S.pack_materials(mm)
+            hp = float(P[mi * 40 + c]) - float(S.pack_materials(mats)[mi * 40 + c])
+            hm = float(S.pack_materials(mats)[mi * 40 + c]) - float(M[mi * 40 + c])
+            o.set_materials(P)
+            fp = objective(o, f, xy, dl, spp)
+            o.set_materials(M)
+            fm = objective(o, f, xy, dl, spp)
+            o.set_materials(S.pack_materials(mats))
+            fd = (fp - fm) / (hp + hm)
+            assert fd == pytest.approx(gm[mi, c], rel=1e-4, abs=1e-7), (mi, c, fd, gm[mi, c])
