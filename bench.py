@@ -162,8 +162,11 @@ def bench_closest_hit(args) -> dict:
 
 
 def bench_path_trace(args) -> dict:
-    """Plain progressive path trace: 1440x900, 64 spp per step (one launch per
-    sample, as Renderer.traceSamples), depth 8, RR from bounce 2, clamp 0."""
+    """Progressive path trace: 1440x900, 64 spp per step, depth 8, RR from
+    bounce 2, clamp 0.  The step is Renderer.traceSamples(64) after a reset,
+    run by the persistent tile kernel (one launch per step); the reference's
+    launch structure (one launch per sample) is timed beside it on the same
+    film and seeds (the two films are bit-identical)."""
     import torch
 
     from lucent_amd import workloads
@@ -182,54 +185,70 @@ def bench_path_trace(args) -> dict:
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
 
-    def step():
-        r.reset()
-        r.trace_samples(SPP, stream=stream.cuda_stream)
+    def timed_steps(n, per_sample):
+        r.settings.per_sample_launch = per_sample
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        for i in range(n):
+            # The reset is ordered on the library's own stream: drain this one first.
+            stream.synchronize()
+            r.film.set_seed(0)
+            r.reset()
+            starts[i].record(stream)
+            r.trace_samples(SPP, stream=stream.cuda_stream)
+            ends[i].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
+        return wall, [s_.elapsed_time(e) for s_, e in zip(starts, ends)]
 
     log(f"warmup {args.warmup}")
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    if args.warmup:
+        timed_steps(args.warmup, False)
     log(f"timing {args.steps} steps")
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        step()
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    step_ms = [s_.elapsed_time(e) for s_, e in zip(starts, ends)]
+    wall, trace_ms = timed_steps(args.steps, False)
     paths = W * H * SPP
     value = paths * args.steps / wall
-    launch_s = float(np.mean(step_ms)) / 1e3 / SPP
-    img = r.film.read()["color"]
-    mean_rgb = [float(x) for x in img[..., :3].reshape(-1, 3).mean(0)]
+    launch_s = float(np.mean(trace_ms)) / 1e3
+    img = r.film.read()
+    mean_rgb = [float(x) for x in img["color"][..., :3].reshape(-1, 3).mean(0)]
+    log("timing the per-sample launch structure (1 step)")
+    wall_ps, trace_ms_ps = timed_steps(1, True)
+    img_ps = r.film.read()
+    identical = all(np.array_equal(img[k], img_ps[k]) for k in img)
+    r.settings.per_sample_launch = False
     counts = json.loads((ROOT / "tests/golden/pathtrace_counts.json").read_text())["per_path"]
     b_path = path_bytes(counts)
-    achieved = b_path * W * H / launch_s / 1e9
+    # One launch accumulates SPP samples per pixel: the film (72 B per pixel)
+    # is read and written once, everything else is per path.
+    b_launch = (b_path - 72) * paths + 72 * W * H
+    achieved = b_launch / launch_s / 1e9
     traffic = profiled_traffic("path_trace_1440x900")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_path": round(b_path, 1),
-                "kernel": "k_trace_sample", "avg_launch_ms": round(launch_s * 1e3, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_path": round(b_path - 72, 1),
+                "film_bytes_per_pixel": 72, "kernel": "k_trace_tiles", "avg_launch_ms": round(launch_s * 1e3, 3)}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_path_trace(r.packed, scene, W, H)
     rays_per_path = counts["closest_rays"] + counts["shadow_rays"]
     r.close()
     ctx.close()
+    per_sample_value = paths / (float(np.mean(trace_ms_ps)) / 1e3)
     return {"value": round(value, 1), "unit": "samples/s", "ms_per_step": round(wall / args.steps * 1e3, 3),
             "dtype": "f32",
-            "config": {"workload": f"plain progressive path trace {W}x{H}, {SPP} spp, depth {workloads.PT_DEPTH}, "
+            "config": {"workload": f"progressive path trace {W}x{H}, {SPP} spp, depth {workloads.PT_DEPTH}, "
                                    f"RR from bounce 2, clamp 0; 6 fBm torus knots + ground quad "
                                    f"({stats['tri_count']} tris), quad light, 1024x512 HDRI",
                        "width": W, "height": H, "spp": SPP, "tris": stats["tri_count"],
                        "scene_load_s": round(load_s, 3)},
-            "roofline": roofline, "cpu_baseline": cpu,
-            "avg_sample_launch_ms": round(launch_s * 1e3, 4), "image_mean_rgb": mean_rgb,
-            "mpaths_per_s": round(value / 1e6, 2), "mrays_per_s_all_rays": round(value * rays_per_path / 1e6, 1)}
+            "roofline": roofline, "cpu_baseline": cpu, "image_mean_rgb": mean_rgb,
+            "mpaths_per_s": round(value / 1e6, 2), "mrays_per_s_all_rays": round(value * rays_per_path / 1e6, 1),
+            "per_sample_launches": {"samples_per_s": round(per_sample_value, 1),
+                                    "trace_ms": round(float(np.mean(trace_ms_ps)), 3),
+                                    "avg_sample_launch_ms": round(float(np.mean(trace_ms_ps)) / SPP, 4),
+                                    "film_bit_identical": identical,
+                                    "speedup_of_persistent_tiles": round(float(np.mean(trace_ms_ps)) / (launch_s * 1e3), 3)}}
 
 
 def path_bytes(c: dict) -> float:
@@ -364,6 +383,88 @@ def bench_adjoint(args) -> dict:
             "loss": float(((img - target) ** 2).sum())}
 
 
+def bench_batch(args) -> dict:
+    """Batched multi-scene tracing: 64 seeded scenes (1k..1.5M triangles,
+    11.8M in all, 6% of meshes Worley alpha-masked), resolution uniform over
+    256^2 / 640x480 / 1280x720 / 1920x1080, spp uniform in 1..64, depth 8.
+    Plain: lucent_trace_samples per scene with one launch per sample (the
+    reference's structure).  Batched: ONE lucent_trace_batch over all 64
+    films.  Same seeds, same estimator: the films are compared bit for bit,
+    so the per-pixel error is equal by construction."""
+    import torch
+
+    from lucent_amd import workloads
+    from lucent_amd.renderer import Renderer, RenderSettings, trace_batch
+    from lucent_amd.runtime import Context
+
+    log(f"building {workloads.BATCH_SCENES} batch scenes")
+    t0 = time.time()
+    ctx = Context(0)
+    jobs, tris = [], 0
+    for i in range(workloads.BATCH_SCENES):
+        sc, w, h, spp = workloads.batch_scene(i)
+        r = Renderer(ctx, w, h, RenderSettings(max_bounces=workloads.BATCH_DEPTH, firefly_clamp=0.0))
+        r.load_scene(sc)
+        tris += r.packed.tri_count
+        r.packed = None  # host copies are not needed any more
+        jobs.append((r, spp))
+    setup_s = time.time() - t0
+    paths = sum(r.film.width * r.film.height * n for r, n in jobs)
+    log(f"{tris} triangles, {paths} paths; set-up {setup_s:.1f} s")
+
+    def reset_all():
+        for r, _ in jobs:
+            r.film.set_seed(0)
+            r.reset()
+        ctx.synchronize()
+
+    def run_plain():
+        for r, n in jobs:
+            r.settings.per_sample_launch = True
+            r.trace_samples(n)
+        ctx.synchronize()
+
+    settings = RenderSettings(max_bounces=workloads.BATCH_DEPTH, firefly_clamp=0.0)
+
+    def run_batched():
+        trace_batch(ctx, jobs, settings)
+        ctx.synchronize()
+
+    def films_digest():
+        out = []
+        for r, _ in jobs:
+            f = r.film.read()
+            out.append(hash(f["color"].tobytes()) ^ hash(f["albedo"].tobytes()) ^ hash(f["normal"].tobytes()))
+        return out
+
+    def timed(fn, n):
+        best = []
+        for _ in range(n):
+            reset_all()
+            t = time.perf_counter()
+            fn()
+            best.append(time.perf_counter() - t)
+        return best
+
+    log("plain: one launch per sample per scene")
+    plain_s = timed(run_plain, 1)
+    plain_digest = films_digest()
+    log(f"plain {plain_s[0]:.2f} s; batched warm-up")
+    timed(run_batched, 1)
+    log("batched: timing")
+    batch_s = timed(run_batched, max(1, min(args.steps, 3)))
+    same = films_digest() == plain_digest
+    for r, _ in jobs:
+        r.close()
+    ctx.close()
+    tb, tp = float(np.median(batch_s)), plain_s[0]
+    return {"metric": "samples/s, 64-scene batch: one persistent launch vs per-sample launches per scene",
+            "value": round(paths / tb, 1), "unit": "samples/s", "plain_samples_per_s": round(paths / tp, 1),
+            "speedup_vs_plain": round(tp / tb, 3), "batched_s": round(tb, 3), "plain_s": round(tp, 3),
+            "films_bit_identical": same, "scenes": len(jobs), "triangles": tris, "paths": paths,
+            "setup_s": round(setup_s, 1)}
+
+
 def bench_copy(args) -> dict:
     """Calibration: a plain device copy of 1 GiB per step (for scaling the
     FETCH_SIZE / WRITE_SIZE counters)."""
@@ -390,7 +491,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "closest-hit", "path-trace", "denoise", "adjoint", "copy"])
+                    choices=["all", "closest-hit", "path-trace", "batch", "denoise", "adjoint", "copy"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -408,6 +509,10 @@ def main() -> int:
         d = bench_adjoint(args)
         res = {"value": d.pop("value"), "unit": d.pop("unit"), "ms_per_step": d["forward_ms"] + d["backward_ms"],
                "dtype": "f32/f64", "config": {"workload": d.pop("metric")}, **d}
+    elif args.workload == "batch":
+        d = bench_batch(args)
+        res = {"value": d.pop("value"), "unit": d.pop("unit"), "ms_per_step": d["batched_s"] * 1e3, "dtype": "f32",
+               "config": {"workload": d.pop("metric")}, **d}
     elif args.workload == "denoise":
         d = bench_denoise(args)
         res = {"value": d["value"], "unit": d["unit"], "ms_per_step": d["value"], "dtype": "f32/f16",
@@ -418,6 +523,7 @@ def main() -> int:
             ch = bench_closest_hit(args)
             extra["closest_hit"] = {"metric": "Mrays/s closest-hit", "value": ch.pop("value"),
                                     "unit": ch.pop("unit"), **ch}
+            extra["batch"] = bench_batch(args)
             extra["denoise"] = bench_denoise(args)
             extra["adjoint"] = bench_adjoint(args)
     line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,

@@ -195,6 +195,18 @@ typedef struct lucent_lighting {
 
 lucent_status lucent_scene_set_lighting(lucent_ctx *ctx, lucent_scene *scene, const lucent_lighting *lighting);
 
+/* Texture arrays of Renderer.buildTextureArrays (src/renderer.ts:315-335,
+ * createTextureArray src/gl/context.ts:224-250): `color` holds the sRGB
+ * layers (base colour, emissive; decoded to linear per texel), `data` the
+ * linear ones (metal-rough, normal), each layer `size` x `size` RGBA8 texels
+ * (R in the low byte), rows in v order, already resampled to the shared
+ * size.  Sampling is bilinear with REPEAT wrap at level 0.  Materials refer
+ * to layers by index (packScene's layerMap, src/scene/scene.ts:136-153).
+ * Replaces the scene's previous arrays; 0 layers clears them. */
+lucent_status lucent_scene_set_textures(lucent_ctx *ctx, lucent_scene *scene, uint32_t size,
+                                        const uint32_t *color_rgba8, uint32_t color_layers,
+                                        const uint32_t *data_rgba8, uint32_t data_layers);
+
 /* buildEnvSampler(img, maxWidth) (environment.ts:52-85), host-only.  table_out
  * must hold 4 floats per grid cell (at most min(width,max_width) x height). */
 lucent_status lucent_env_sampler(const float *rgb, uint32_t width, uint32_t height, uint32_t max_width,
@@ -220,7 +232,9 @@ typedef struct lucent_film lucent_film;
 
 lucent_status lucent_film_create(lucent_ctx *ctx, uint32_t width, uint32_t height, lucent_film **out);
 void lucent_film_destroy(lucent_film *film);
-/* Renderer.reset (renderer.ts:463-474): zero the averages and the history. */
+/* Renderer.reset (renderer.ts:463-474): zero the averages and the history.
+ * Ordered on the context's stream and synchronous; work the caller enqueued
+ * on another stream that touches the film must have completed. */
 lucent_status lucent_film_reset(lucent_ctx *ctx, lucent_film *film);
 lucent_status lucent_film_set_seed(lucent_film *film, uint32_t seed);
 lucent_status lucent_film_info(const lucent_film *film, uint32_t *width, uint32_t *height, uint32_t *samples,
@@ -248,14 +262,43 @@ typedef struct lucent_camera {
   double focus_distance;
 } lucent_camera;
 
+/* Launch structure of lucent_trace_samples / lucent_trace_batch.  Both give
+ * bit-identical films.  Default: persistent waves run all samples of an 8x8
+ * tile, starting a pixel's next sample as soon as its path ends.
+ * LUCENT_TRACE_PER_SAMPLE: one launch over the film per sample, as the
+ * reference's traceSamples draws one full-screen pass per sample. */
+#define LUCENT_TRACE_PER_SAMPLE 0x1u
+
 typedef struct lucent_render_settings {
   uint32_t max_bounces;
   float firefly_clamp; /* 0 disables (uClamp) */
+  uint32_t flags;      /* LUCENT_TRACE_* */
+  uint32_t waves_per_cu; /* persistent launch width; 0 = default (8) */
 } lucent_render_settings;
 
 lucent_status lucent_trace_samples(lucent_ctx *ctx, lucent_scene *scene, lucent_film *film,
                                    const lucent_camera *camera, const lucent_render_settings *settings,
                                    uint32_t count, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Batched multi-scene tracing: for every item, `samples` samples of every
+ * pixel of its film (seeds continue from the film's seed, history advances by
+ * `samples`), all items in ONE persistent launch whose waves pull 8x8 tile
+ * jobs from a shared queue, longest jobs first.  The result for each film is
+ * bit-identical to lucent_trace_samples(scene, film, camera, settings,
+ * samples).  A film may appear once per batch.  No counterpart in the
+ * reference, which renders one scene per Renderer with one draw per sample
+ * (renderer.ts:279, 702-737).
+ * ---------------------------------------------------------------------- */
+typedef struct lucent_batch_item {
+  lucent_scene *scene;
+  lucent_film *film;
+  lucent_camera camera;
+  uint32_t samples;
+} lucent_batch_item;
+
+lucent_status lucent_trace_batch(lucent_ctx *ctx, const lucent_batch_item *items, uint32_t count,
+                                 const lucent_render_settings *settings, void *stream);
 
 /* ------------------------------------------------------------------------
  * Denoiser — Renderer.runDenoiser (src/renderer.ts:788-830): variance.frag,

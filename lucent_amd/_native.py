@@ -92,8 +92,16 @@ class CameraStruct(C.Structure):
                 ("aperture", C.c_double), ("focus_distance", C.c_double)]
 
 
+TRACE_PER_SAMPLE = 0x1
+
+
 class RenderSettingsStruct(C.Structure):
-    _fields_ = [("max_bounces", C.c_uint32), ("firefly_clamp", C.c_float)]
+    _fields_ = [("max_bounces", C.c_uint32), ("firefly_clamp", C.c_float), ("flags", C.c_uint32),
+                ("waves_per_cu", C.c_uint32)]
+
+
+class BatchItemStruct(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("film", C.c_void_p), ("camera", CameraStruct), ("samples", C.c_uint32)]
 
 
 class DenoiseParamsStruct(C.Structure):
@@ -132,6 +140,7 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_film_read": (_I32, [_P, _P, _P, _P, _P, _P]),
     "lucent_film_device_ptrs": (_I32, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "lucent_trace_samples": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _P]),
+    "lucent_trace_batch": (_I32, [_P, _P, _U32, C.POINTER(RenderSettingsStruct), _P]),
     "lucent_denoise": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(DenoiseParamsStruct), _P]),
     "lucent_film_read_denoised": (_I32, [_P, _P, _P, _P]),
     "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,
@@ -139,6 +148,7 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_detached_forward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32,
                                        _U32, _U32, _P, _U32, _P, _P]),
     "lucent_scene_set_materials": (_I32, [_P, _P, _P, _U32]),
+    "lucent_scene_set_textures": (_I32, [_P, _P, _U32, _P, _U32, _P, _U32]),
     "lucent_scene_update_env_texels": (_I32, [_P, _P, _P, _U32, _U32]),
     "lucent_scene_env_size": (_I32, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
 }

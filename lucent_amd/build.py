@@ -27,6 +27,10 @@ HIPCC_FLAGS = [
     "-O3",
     "-fPIC",
     "-std=c++17",
+    # Multiply-adds are fused only within one source expression, never across
+    # statements: the same path computed by different kernels (per-sample
+    # launch, persistent tiles, adjoint replay) then rounds identically.
+    "-ffp-contract=on",
     # Packed FP32 (v_pk_*_f32) stays off on this hardware pool.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -176,13 +176,13 @@ __device__ inline ParamScale param_scale(const PtScene& sc, const Hit& hit) {
     if (sum - 2.0f * floorf(sum / 2.0f) >= 0.5f) p.albedo[0] = p.albedo[1] = p.albedo[2] = 0.0f;
   }
   const bool tex = sc.tex_size > 0;
-  if (m.map_base >= 0 && tex) {
+  if (m.map_base >= 0 && m.map_base < sc.color_layers && tex) {
     const float4 t = sample_layer(sc.color_maps, sc.tex_size, m.map_base, tu, tv, true);
     p.albedo[0] *= t.x;
     p.albedo[1] *= t.y;
     p.albedo[2] *= t.z;
   }
-  if (m.map_mr >= 0 && tex) {
+  if (m.map_mr >= 0 && m.map_mr < sc.data_layers && tex) {
     const float4 mr = sample_layer(sc.data_maps, sc.tex_size, m.map_mr, tu, tv, false);
     p.rough = mr.y;
     p.metal = mr.z;

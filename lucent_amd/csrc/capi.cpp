@@ -79,6 +79,9 @@ static Status grow(void** buf, size_t* have, size_t bytes, void* stream) {
 
 Status ensure_scratch(Context* ctx, size_t bytes) { return grow(&ctx->scratch, &ctx->scratch_bytes, bytes, ctx->stream); }
 Status ensure_spill(Context* ctx, size_t bytes) { return grow(&ctx->spill, &ctx->spill_bytes, bytes, ctx->stream); }
+Status ensure_tile_table(Context* ctx, size_t bytes) {
+  return grow(&ctx->tile_table, &ctx->tile_table_bytes, bytes, ctx->stream);
+}
 
 namespace {
 
@@ -99,6 +102,7 @@ void destroy_scene(Scene* sc) {
   if (sc->ctx) hipStreamSynchronize(static_cast<hipStream_t>(sc->ctx->stream));
   for (void* p : sc->allocations) hipFree(p);
   for (void* p : sc->env_allocations) hipFree(p);
+  for (void* p : sc->tex_allocations) hipFree(p);
   delete sc;
 }
 
@@ -184,6 +188,14 @@ lucent_status lucent_create(int device, lucent_ctx** out) {
   }
   hipMemset(flags, 0, 256);
   ctx->d_flags = static_cast<uint32_t*>(flags);
+  hipEvent_t ev = nullptr;
+  if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+    hipFree(flags);
+    hipStreamDestroy(s);
+    delete ctx;
+    return fail_hip(e, "hipEventCreate");
+  }
+  ctx->tile_event = ev;
   *out = ctx;
   return LUCENT_OK;
 }
@@ -194,6 +206,8 @@ void lucent_destroy(lucent_ctx* ctx) {
   if (ctx->scratch) hipFree(ctx->scratch);
   if (ctx->spill) hipFree(ctx->spill);
   if (ctx->d_flags) hipFree(ctx->d_flags);
+  if (ctx->tile_table) hipFree(ctx->tile_table);
+  if (ctx->tile_event) hipEventDestroy(static_cast<hipEvent_t>(ctx->tile_event));
   hipStreamDestroy(static_cast<hipStream_t>(ctx->stream));
   delete ctx;
 }

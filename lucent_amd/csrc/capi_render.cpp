@@ -2,6 +2,7 @@
 // the accumulation film and progressive tracing.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -156,6 +157,47 @@ lucent_status lucent_scene_set_lighting(lucent_ctx* ctx, lucent_scene* scene, co
   return LUCENT_OK;
 }
 
+lucent_status lucent_scene_set_textures(lucent_ctx* ctx, lucent_scene* scene, uint32_t size,
+                                       const uint32_t* color_rgba8, uint32_t color_layers, const uint32_t* data_rgba8,
+                                       uint32_t data_layers) {
+  clear_error();
+  if (!ctx || !scene) return fail(LUCENT_E_INVALID, "null ctx/scene");
+  if ((color_layers && !color_rgba8) || (data_layers && !data_rgba8)) return fail(LUCENT_E_INVALID, "null texel array");
+  if ((color_layers || data_layers) && (size == 0 || size > 16384))
+    return fail(LUCENT_E_INVALID, "texture size must be in 1..16384");
+  const size_t layer = size_t(size) * size * sizeof(uint32_t);
+  if ((size_t(color_layers) + data_layers) * layer > (size_t(1) << 34))
+    return fail(LUCENT_E_INVALID, "texture arrays larger than 16 GiB");
+  Status s = stream_sync(ctx->stream);
+  if (s != LUCENT_OK) return s;
+  for (void* p : scene->tex_allocations) hipFree(p);
+  scene->tex_allocations.clear();
+  PtScene& pt = scene->pt;
+  pt.color_maps = pt.data_maps = nullptr;
+  pt.tex_size = 0;
+  pt.color_layers = pt.data_layers = 0;
+  auto upload = [&](const uint32_t* src, uint32_t layers, const uint32_t** dst) -> Status {
+    if (layers == 0) return LUCENT_OK;
+    void* p = nullptr;
+    Status st = dev_alloc(&p, layer * layers);
+    if (st != LUCENT_OK) return st;
+    scene->tex_allocations.push_back(p);
+    if ((st = dev_upload(p, src, layer * layers, ctx->stream)) != LUCENT_OK) return st;
+    *dst = static_cast<const uint32_t*>(p);
+    return LUCENT_OK;
+  };
+  const uint32_t *dc = nullptr, *dd = nullptr;
+  if ((s = upload(color_rgba8, color_layers, &dc)) != LUCENT_OK) return s;
+  if ((s = upload(data_rgba8, data_layers, &dd)) != LUCENT_OK) return s;
+  if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+  pt.color_maps = dc;
+  pt.data_maps = dd;
+  pt.color_layers = int32_t(color_layers);
+  pt.data_layers = int32_t(data_layers);
+  pt.tex_size = (color_layers || data_layers) ? int32_t(size) : 0;
+  return LUCENT_OK;
+}
+
 lucent_status lucent_film_create(lucent_ctx* ctx, uint32_t width, uint32_t height, lucent_film** out) {
   clear_error();
   if (!ctx || !out) return fail(LUCENT_E_INVALID, "null ctx/out");
@@ -267,35 +309,98 @@ lucent_status lucent_film_device_ptrs(const lucent_film* film, void** color4, vo
   return LUCENT_OK;
 }
 
+static Status fill_frame(const lucent_camera* cam, const lucent_render_settings* settings, uint32_t w, uint32_t h,
+                         PtFrame* f);
+
+static Status check_traceable(const lucent_scene* scene) {
+  if (!scene->lighting_set) return fail(LUCENT_E_INVALID, "call lucent_scene_set_lighting before tracing");
+  if (scene->pt.material_count == 0 && scene->bvh.tri_count > 0)
+    return fail(LUCENT_E_INVALID, "scene has triangles but no materials (use lucent_scene_upload)");
+  return LUCENT_OK;
+}
+
 lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_film* film, const lucent_camera* cam,
                                    const lucent_render_settings* settings, uint32_t count, void* stream) {
   clear_error();
   if (!ctx || !scene || !film || !cam || !settings) return fail(LUCENT_E_INVALID, "null argument");
-  if (!scene->lighting_set) return fail(LUCENT_E_INVALID, "call lucent_scene_set_lighting before tracing");
-  if (scene->pt.material_count == 0 && scene->bvh.tri_count > 0)
-    return fail(LUCENT_E_INVALID, "scene has triangles but no materials (use lucent_scene_upload)");
+  Status s = check_traceable(scene);
+  if (s != LUCENT_OK) return s;
   PtFrame f;
-  f.width = film->width;
-  f.height = film->height;
-  f.max_bounces = settings->max_bounces;
-  f.clamp = settings->firefly_clamp;
-  double fwd[3], right[3], up[3], tan_half;
-  camera_basis(*cam, fwd, right, up, &tan_half);
-  for (int i = 0; i < 3; ++i) {
-    f.cam_pos[i] = float(cam->position[i]);
-    f.cam_fwd[i] = float(fwd[i]);
-    f.cam_right[i] = float(right[i]);
-    f.cam_up[i] = float(up[i]);
-  }
-  f.tan_half_fov = float(tan_half);
-  f.aperture = float(cam->aperture);
-  f.focus_dist = float(cam->focus_distance);
+  fill_frame(cam, settings, film->width, film->height, &f);
   void* st = stream ? stream : ctx->stream;
+  if (count == 0) return LUCENT_OK;
+  if (settings->flags & LUCENT_TRACE_PER_SAMPLE) {
+    for (uint32_t i = 0; i < count; ++i) {
+      f.seed = film->seed++;
+      if ((s = launch_trace_sample(ctx, scene->pt, f, film->f, st)) != LUCENT_OK) return s;
+      film->samples++;
+    }
+    return LUCENT_OK;
+  }
+  TileItem it;
+  it.scene = scene->pt;
+  it.frame = f;
+  it.frame.seed = film->seed;
+  it.film = film->f;
+  it.spp = count;
+  TileLaunch cfg;
+  cfg.waves_per_cu = settings->waves_per_cu;
+  if ((s = launch_trace_tiles(ctx, &it, 1, cfg, st)) != LUCENT_OK) return s;
+  film->seed += count;
+  film->samples += count;
+  return LUCENT_OK;
+}
+
+lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items, uint32_t count,
+                                 const lucent_render_settings* settings, void* stream) {
+  clear_error();
+  if (!ctx || (!items && count) || !settings) return fail(LUCENT_E_INVALID, "null argument");
+  std::vector<TileItem> tiles;
+  std::vector<uint64_t> cost;
+  tiles.reserve(count);
   for (uint32_t i = 0; i < count; ++i) {
-    f.seed = film->seed++;
-    Status s = launch_trace_sample(ctx, scene->pt, f, film->f, st);
+    const lucent_batch_item& b = items[i];
+    if (!b.scene || !b.film) return fail(LUCENT_E_INVALID, "batch item without scene or film");
+    Status s = check_traceable(b.scene);
     if (s != LUCENT_OK) return s;
-    film->samples++;
+    for (uint32_t j = 0; j < i; ++j)
+      if (items[j].film == b.film) return fail(LUCENT_E_INVALID, "a film appears twice in one batch");
+  }
+  void* st = stream ? stream : ctx->stream;
+  if (settings->flags & LUCENT_TRACE_PER_SAMPLE) {
+    for (uint32_t i = 0; i < count; ++i) {
+      lucent_status s = lucent_trace_samples(ctx, items[i].scene, items[i].film, &items[i].camera, settings,
+                                             items[i].samples, st);
+      if (s != LUCENT_OK) return s;
+    }
+    return LUCENT_OK;
+  }
+  // Longest jobs first: a tile job costs about samples x (log of the scene size).
+  std::vector<uint32_t> order(count);
+  for (uint32_t i = 0; i < count; ++i) order[i] = i;
+  auto job_cost = [&](uint32_t i) {
+    const double tris = double(items[i].scene->bvh.tri_count) + 2.0;
+    return double(items[i].samples) * std::log2(tris);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return job_cost(a) > job_cost(b); });
+  for (uint32_t i : order) {
+    const lucent_batch_item& b = items[i];
+    if (b.samples == 0) continue;
+    TileItem it;
+    it.scene = b.scene->pt;
+    fill_frame(&b.camera, settings, b.film->width, b.film->height, &it.frame);
+    it.frame.seed = b.film->seed;
+    it.film = b.film->f;
+    it.spp = b.samples;
+    tiles.push_back(it);
+  }
+  TileLaunch cfg;
+  cfg.waves_per_cu = settings->waves_per_cu;
+  Status s = launch_trace_tiles(ctx, tiles.data(), uint32_t(tiles.size()), cfg, st);
+  if (s != LUCENT_OK) return s;
+  for (uint32_t i = 0; i < count; ++i) {
+    items[i].film->seed += items[i].samples;
+    items[i].film->samples += items[i].samples;
   }
   return LUCENT_OK;
 }

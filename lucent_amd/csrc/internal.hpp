@@ -60,6 +60,10 @@ struct Context {
   // Spill area for traversal stacks that outgrow their LDS part.
   void* spill = nullptr;
   size_t spill_bytes = 0;
+  // Persistent tile launches: job counter, per-film job ranges and films.
+  void* tile_table = nullptr;
+  size_t tile_table_bytes = 0;
+  void* tile_event = nullptr;   // hipEvent_t: the table upload has completed
   uint32_t* d_flags = nullptr;  // device status word(s)
   float* sheen_lut = nullptr;   // 16x16 half-rounded sheen albedo table
 };
@@ -80,6 +84,7 @@ Status dev_download(void* dst, const void* src, size_t bytes, void* stream);
 Status stream_sync(void* stream);
 Status ensure_scratch(Context* ctx, size_t bytes);
 Status ensure_spill(Context* ctx, size_t bytes);
+Status ensure_tile_table(Context* ctx, size_t bytes);
 
 // Kernel launchers (traverse.hip).
 struct RayQuery {

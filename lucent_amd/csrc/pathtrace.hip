@@ -1,11 +1,29 @@
-// Progressive path tracing on gfx950: one launch per sample over the whole
-// film (Renderer.traceSamples, reference src/renderer.ts:702-737), one lane
-// per pixel, each wave64 covering an 8x8 pixel tile so that neighbouring
-// primary rays share BVH nodes in the caches.  The integrator itself is
-// pt_device.hpp (trace.frag semantics in float32).
+// Progressive path tracing on gfx950.  The integrator is pt_device.hpp
+// (trace.frag semantics in float32); two launch structures run it:
+//
+// * k_trace_sample — the reference's structure (Renderer.traceSamples,
+//   src/renderer.ts:702-737): one launch per sample over the whole film, one
+//   lane per pixel, each wave64 an 8x8 pixel tile so that neighbouring rays
+//   share BVH nodes in the caches.  A wave lasts as long as its longest path.
+//
+// * k_trace_tiles — persistent waves pull 8x8 tile jobs from an atomic queue
+//   (one or many films / scenes per launch) and run ALL of a job's samples:
+//   each lane owns one pixel, advances its path one bounce at a time and, as
+//   soon as the path ends, folds it into the pixel's running mean (kept in
+//   registers) and starts the pixel's next sample.  Lanes are refilled
+//   instead of idling behind the longest path of the wave.  Per pixel the
+//   samples run in seed order with the film's exact running-mean and FP16
+//   rounding steps, so the film is bit-identical to spp k_trace_sample
+//   launches; the film is read and written once per job instead of once per
+//   sample.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "pt_device.hpp"
+#include "render_host.hpp"
 
 namespace lucent {
 namespace {
@@ -32,6 +50,74 @@ __global__ __launch_bounds__(kPtBlock) void k_trace_sample(PtScene sc, PtFrame f
   pt::accumulate(film, idx, c, g);
 }
 
+template <class StackT>
+__device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t x, uint32_t y, StackT& st) {
+  const PtScene& sc = it.scene;
+  const PtFrame& f = it.frame;
+  const uint32_t idx = y * f.width + x;
+  pt::PixelAccum acc = pt::load_accum(it.film, idx);
+  uint32_t k = 0;
+  pt::Rng rng = pt::seed_rng(idx, f.seed);
+  pt::v3 ro, rd;
+  pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
+  pt::PathState p;
+  pt::path_begin(p, ro, rd);
+  for (;;) {
+    if (!pt::path_step(sc, f, p, rng, st)) {
+      pt::fold_sample(acc, p.radiance, p.guide);
+      if (++k >= it.spp) break;
+      rng = pt::seed_rng(idx, f.seed + k);
+      pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
+      pt::path_begin(p, ro, rd);
+    }
+  }
+  pt::store_accum(it.film, idx, acc);
+}
+
+template <class StackT>
+__device__ __forceinline__ void run_tile(const TileItem& it, uint32_t tile, StackT& st) {
+  const uint32_t x = (tile % it.tiles_x) * kTile + (threadIdx.x & (kTile - 1));
+  const uint32_t y = (tile / it.tiles_x) * kTile + (threadIdx.x / kTile);
+  if (x < it.frame.width && y < it.frame.height) trace_pixel_samples(it, x, y, st);
+}
+
+// Jobs are numbered film by film: film i owns jobs [job_begin[i], job_begin[i+1]).
+// kSingle: one film, passed by value (its fields live in the kernel arguments).
+template <bool kSingle>
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_tiles(TileItem single, const TileItem* __restrict__ items,
+                                                         const uint32_t* __restrict__ job_begin, uint32_t n_items,
+                                                         uint32_t n_jobs, uint32_t* __restrict__ counter,
+                                                         uint32_t* spill, uint32_t spill_depth) {
+  __shared__ uint32_t lds[kPtLds * kPtBlock];
+  __shared__ uint32_t next_job;
+  pt::Stack<kPtLds, kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
+                                 0, 0};
+  for (;;) {
+    if (threadIdx.x == 0) next_job = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t job = __builtin_amdgcn_readfirstlane(next_job);
+    __syncthreads();
+    if (job >= n_jobs) break;
+    uint32_t item = 0;
+    if (!kSingle) {
+      uint32_t hi = n_items;
+      while (hi - item > 1) {
+        const uint32_t mid = (item + hi) / 2;
+        if (job_begin[mid] <= job)
+          item = mid;
+        else
+          hi = mid;
+      }
+    }
+    if constexpr (kSingle)
+      run_tile(single, job, st);
+    else
+      run_tile(items[item], job - job_begin[item], st);
+  }
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
 }  // namespace
 
 Status launch_trace_sample(Context* ctx, const PtScene& sc, const PtFrame& f, const PtFilm& film, void* stream) {
@@ -46,6 +132,57 @@ Status launch_trace_sample(Context* ctx, const PtScene& sc, const PtFrame& f, co
                      film, static_cast<uint32_t*>(ctx->spill), depth, tiles_x);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "k_trace_sample launch");
+  return LUCENT_OK;
+}
+
+uint32_t tile_jobs(uint32_t width, uint32_t height) {
+  return ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
+}
+
+Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const TileLaunch& cfg, void* stream_) {
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  if (n_items == 0) return LUCENT_OK;
+  std::vector<uint32_t> job_begin(n_items + 1, 0);
+  uint32_t depth = 1;
+  for (uint32_t i = 0; i < n_items; ++i) {
+    items[i].tiles_x = (items[i].frame.width + kTile - 1) / kTile;
+    const uint32_t jobs = items[i].spp ? tile_jobs(items[i].frame.width, items[i].frame.height) : 0;
+    job_begin[i + 1] = job_begin[i] + jobs;
+    depth = std::max(depth, items[i].scene.bvh.stack_depth + 1);
+  }
+  const uint32_t n_jobs = job_begin[n_items];
+  if (n_jobs == 0) return LUCENT_OK;
+  const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 8;
+  const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
+  Status s = ensure_spill(ctx, size_t(grid) * kPtBlock * depth * sizeof(uint32_t));
+  if (s != LUCENT_OK) return s;
+  // Device table: [counter | job_begin (n+1) | items], 256-byte aligned parts.
+  const size_t begin_off = 256;
+  const size_t items_off = (begin_off + job_begin.size() * sizeof(uint32_t) + 255) / 256 * 256;
+  const size_t bytes = items_off + (n_items > 1 ? size_t(n_items) * sizeof(TileItem) : 0);
+  if ((s = ensure_tile_table(ctx, bytes)) != LUCENT_OK) return s;
+  char* table = static_cast<char*>(ctx->tile_table);
+  uint32_t* counter = reinterpret_cast<uint32_t*>(table);
+  if (n_items > 1) {
+    std::vector<char> host(bytes - begin_off);
+    std::memcpy(host.data(), job_begin.data(), job_begin.size() * sizeof(uint32_t));
+    std::memcpy(host.data() + (items_off - begin_off), items, size_t(n_items) * sizeof(TileItem));
+    hipError_t e = hipMemcpyAsync(table + begin_off, host.data(), host.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipEventRecord(static_cast<hipEvent_t>(ctx->tile_event), stream);
+    if (e == hipSuccess) e = hipEventSynchronize(static_cast<hipEvent_t>(ctx->tile_event));  // `host` must outlive the copy
+    if (e != hipSuccess) return fail_hip(e, "tile table upload");
+  }
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
+  const TileItem* d_items = reinterpret_cast<const TileItem*>(table + items_off);
+  const uint32_t* d_begin = reinterpret_cast<const uint32_t*>(table + begin_off);
+  if (n_items == 1)
+    hipLaunchKernelGGL(k_trace_tiles<true>, dim3(grid), dim3(kPtBlock), 0, stream, items[0], nullptr, nullptr, 1u,
+                       n_jobs, counter, static_cast<uint32_t*>(ctx->spill), depth);
+  else
+    hipLaunchKernelGGL(k_trace_tiles<false>, dim3(grid), dim3(kPtBlock), 0, stream, TileItem{}, d_items, d_begin,
+                       n_items, n_jobs, counter, static_cast<uint32_t*>(ctx->spill), depth);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(e, "k_trace_tiles launch");
   return LUCENT_OK;
 }
 

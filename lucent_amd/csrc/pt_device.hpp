@@ -957,19 +957,19 @@ __device__ inline Surface surface_at(const PtScene& sc, const Hit& hit, v3 wo, v
     if (sum - 2.0f * floorf(sum / 2.0f) >= 0.5f) s.albedo = m.albedo2;
   }
   const bool tex = sc.tex_size > 0;
-  if (m.map_base >= 0 && tex) s.albedo = s.albedo * mk(sample_layer(sc.color_maps, sc.tex_size, m.map_base, tu, tv, true));
+  if (m.map_base >= 0 && m.map_base < sc.color_layers && tex) s.albedo = s.albedo * mk(sample_layer(sc.color_maps, sc.tex_size, m.map_base, tu, tv, true));
   s.metallic = m.metallic;
   s.roughness = m.roughness;
-  if (m.map_mr >= 0 && tex) {
+  if (m.map_mr >= 0 && m.map_mr < sc.data_layers && tex) {
     const float4 mr = sample_layer(sc.data_maps, sc.tex_size, m.map_mr, tu, tv, false);
     s.roughness *= mr.y;
     s.metallic *= mr.z;
   }
   emission = m.emission;
-  if (m.map_emissive >= 0 && tex) emission = emission * mk(sample_layer(sc.color_maps, sc.tex_size, m.map_emissive, tu, tv, true));
+  if (m.map_emissive >= 0 && m.map_emissive < sc.color_layers && tex) emission = emission * mk(sample_layer(sc.color_maps, sc.tex_size, m.map_emissive, tu, tv, true));
   s.nc = hit.normal;
   s.n = hit.normal;
-  if (m.map_normal >= 0 && tex && dot(hit.dpdu, hit.dpdu) > 0.0f) {
+  if (m.map_normal >= 0 && m.map_normal < sc.data_layers && tex && dot(hit.dpdu, hit.dpdu) > 0.0f) {
     const v3 N = hit.front_face ? hit.normal : -hit.normal;
     const v3 T = normalize(hit.dpdu - N * dot(N, hit.dpdu));
     v3 B = cross(N, T);
@@ -1023,147 +1023,217 @@ __device__ inline void camera_ray(const PtFrame& f, float fx, float fy, Rng& rng
   rd = normalize(focus - ro);
 }
 
-// tracePath (trace.frag:136-272).
-template <class StackT>
-__device__ inline v3 trace_path(const PtScene& sc, const PtFrame& f, v3 ro, v3 rd, Rng& rng, StackT& st, Guide& guide) {
-  v3 radiance = mk(0.f, 0.f, 0.f);
-  v3 throughput = mk(1.f, 1.f, 1.f);
-  bool specular = true;
-  float bsdf_pdf = 0.0f;
-  v3 tint = mk(1.f, 1.f, 1.f);
-  guide.albedo = mk(1.f, 1.f, 1.f);
-  guide.normal = -rd;
-  guide.depth = FAR;
-  guide.coverage = 0.0f;
-  guide.done = false;
-  for (int bounce = 0; bounce <= int(f.max_bounces); ++bounce) {
-    Hit hit;
-    if (!trace_scene(sc, ro, rd, hit, rng, st)) {
-      const v3 env = (bounce == 0 && sc.env_background == 0) ? mk(sc.backdrop) : environment(sc, rd, specular, bsdf_pdf);
-      radiance = radiance + clamp_contribution(throughput * env, bounce, f.clamp);
-      if (!guide.done) {
-        guide.albedo = tint;
-        guide.normal = -rd;
-        guide.done = true;
-      }
-      break;
-    }
-    if (bounce == 0) {
-      guide.depth = hit.t;
-      guide.coverage = 1.0f;
-    }
-    const v3 wo = -rd;
-    v3 emission;
-    const Surface s = surface_at(sc, hit, wo, emission);
-    const v3 ng = hit.geom_normal;
-    if (!hit.front_face && !s.thin_walled && s.transmission > 0.0f) throughput = throughput * vexp(-s.absorption * hit.t);
-    if (emission.x > 0.0f || emission.y > 0.0f || emission.z > 0.0f) {
-      float w = 1.0f;
-      if (!specular && hit.light >= 0) {
-        const float pl = light_pdf(sc, hit.light, ro, rd, hit.t) / float(sc.light_count);
-        w = mis_weight(bsdf_pdf, pl);
-      }
-      radiance = radiance + clamp_contribution(throughput * emission * w, bounce, f.clamp);
-      if (!guide.done) {
-        guide.albedo = tint;
-        guide.normal = s.n;
-        guide.done = true;
-      }
-      if (hit.light >= 0) break;
-    }
-    const bool glass = s.transmission > 0.0f && rng.rand() < s.transmission;
-    const bool mirror = !glass && s.metallic > 0.9f && s.roughness < 0.1f;
-    if (!guide.done && !((glass || mirror) && bounce < 3)) {
-      guide.albedo = tint * s.albedo;
-      guide.normal = s.n;
-      guide.done = true;
-    }
-    if (glass) {
-      v3 wi, weight;
-      bool reflected;
-      if (!sample_dielectric(s, hit.front_face, ng, wo, rng, wi, weight, reflected)) break;
-      throughput = throughput * weight;
-      if (!guide.done) tint = tint * weight;
-      ro = offset_ray(hit.position, reflected ? ng : -ng);
-      rd = wi;
-      specular = true;
-    } else {
-      if (!guide.done) tint = tint * s.albedo;
-      const v3 shadow_origin = offset_ray(hit.position, ng);
-      const v3 n = s.n;
-      if (sc.sun_enabled != 0) {
-        const float u0 = rng.rand(), u1 = rng.rand();
-        const v3 wi = sample_sun_direction(sc, u0, u1);
-        const float ndl = dot(n, wi);
-        if (ndl > 0.0f && dot(ng, wi) > 0.0f && !occluded(sc, shadow_origin, wi, INF, rng, st)) {
-          float pb;
-          const v3 fv = eval_opaque(sc, s, wo, wi, pb);
-          const float w = mis_weight(sc.sun_pdf, pb);
-          radiance = radiance + clamp_contribution(throughput * fv * ndl * mk(sc.sun_radiance) * (w / sc.sun_pdf), bounce, f.clamp);
-        }
-      }
-      if (sc.env_sampling != 0) {
-        const LightSample ls = sample_environment(sc, rng);
-        const float ndl = dot(n, ls.dir);
-        if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f && !occluded(sc, shadow_origin, ls.dir, INF, rng, st)) {
-          float pb;
-          const v3 fv = eval_opaque(sc, s, wo, ls.dir, pb);
-          const float w = mis_weight(ls.pdf, pb);
-          radiance = radiance + clamp_contribution(throughput * fv * ndl * ls.radiance * (w / ls.pdf), bounce, f.clamp);
-        }
-      }
-      if (sc.light_count > 0) {
-        const int li = min(int(rng.rand() * float(sc.light_count)), int(sc.light_count) - 1);
-        const float u0 = rng.rand(), u1 = rng.rand();
-        const LightSample ls = sample_light(sc, li, hit.position, u0, u1);
-        const float ndl = dot(n, ls.dir);
-        if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f &&
-            !occluded(sc, shadow_origin, ls.dir, ls.dist * (1.0f - 1e-4f), rng, st)) {
-          const float pl = ls.pdf / float(sc.light_count);
-          float pb;
-          const v3 fv = eval_opaque(sc, s, wo, ls.dir, pb);
-          const float w = mis_weight(pl, pb);
-          radiance = radiance + clamp_contribution(throughput * fv * ndl * ls.radiance * (w / pl), bounce, f.clamp);
-        }
-      }
-      const BsdfSample bs = sample_opaque(sc, s, wo, rng);
-      if (bs.pdf <= 0.0f || dot(bs.dir, ng) <= 0.0f) break;
-      throughput = throughput * bs.weight;
-      bsdf_pdf = bs.pdf;
-      specular = false;
-      ro = shadow_origin;
-      rd = bs.dir;
-    }
-    if (bounce >= 2) {
-      const float p = fminf(0.95f, max3(throughput));
-      if (rng.rand() >= p) break;
-      throughput = throughput / p;
-    }
-  }
-  return radiance;
+// The state of one camera path between bounces of tracePath
+// (trace.frag:136-272).  trace_path() runs a path to the end; the persistent
+// tile kernel advances each lane's path one bounce at a time and starts the
+// lane's next sample as soon as a path ends.
+struct PathState {
+  v3 ro, rd;
+  v3 radiance, throughput, tint;
+  float bsdf_pdf;
+  int bounce;
+  bool specular;
+  Guide guide;
+};
+
+__device__ __forceinline__ void path_begin(PathState& p, v3 ro, v3 rd) {
+  p.ro = ro;
+  p.rd = rd;
+  p.radiance = mk(0.f, 0.f, 0.f);
+  p.throughput = mk(1.f, 1.f, 1.f);
+  p.tint = mk(1.f, 1.f, 1.f);
+  p.bsdf_pdf = 0.0f;
+  p.bounce = 0;
+  p.specular = true;
+  p.guide.albedo = mk(1.f, 1.f, 1.f);
+  p.guide.normal = -rd;
+  p.guide.depth = FAR;
+  p.guide.coverage = 0.0f;
+  p.guide.done = false;
 }
 
-// Running-mean update of trace.frag:335-354 (uReproject = 0).
-__device__ inline void accumulate(const PtFilm& film, uint32_t idx, v3 color, const Guide& g) {
+// One iteration of tracePath's bounce loop; false once the path has ended
+// (p.radiance and p.guide are then final).
+template <class StackT>
+__device__ __forceinline__ bool path_step(const PtScene& sc, const PtFrame& f, PathState& p, Rng& rng, StackT& st) {
+  const int bounce = p.bounce;
+  Hit hit;
+  if (!trace_scene(sc, p.ro, p.rd, hit, rng, st)) {
+    const v3 env = (bounce == 0 && sc.env_background == 0) ? mk(sc.backdrop) : environment(sc, p.rd, p.specular, p.bsdf_pdf);
+    p.radiance = p.radiance + clamp_contribution(p.throughput * env, bounce, f.clamp);
+    if (!p.guide.done) {
+      p.guide.albedo = p.tint;
+      p.guide.normal = -p.rd;
+      p.guide.done = true;
+    }
+    return false;
+  }
+  if (bounce == 0) {
+    p.guide.depth = hit.t;
+    p.guide.coverage = 1.0f;
+  }
+  const v3 wo = -p.rd;
+  v3 emission;
+  const Surface s = surface_at(sc, hit, wo, emission);
+  const v3 ng = hit.geom_normal;
+  if (!hit.front_face && !s.thin_walled && s.transmission > 0.0f) p.throughput = p.throughput * vexp(-s.absorption * hit.t);
+  if (emission.x > 0.0f || emission.y > 0.0f || emission.z > 0.0f) {
+    float w = 1.0f;
+    if (!p.specular && hit.light >= 0) {
+      const float pl = light_pdf(sc, hit.light, p.ro, p.rd, hit.t) / float(sc.light_count);
+      w = mis_weight(p.bsdf_pdf, pl);
+    }
+    p.radiance = p.radiance + clamp_contribution(p.throughput * emission * w, bounce, f.clamp);
+    if (!p.guide.done) {
+      p.guide.albedo = p.tint;
+      p.guide.normal = s.n;
+      p.guide.done = true;
+    }
+    if (hit.light >= 0) return false;
+  }
+  const bool glass = s.transmission > 0.0f && rng.rand() < s.transmission;
+  const bool mirror = !glass && s.metallic > 0.9f && s.roughness < 0.1f;
+  if (!p.guide.done && !((glass || mirror) && bounce < 3)) {
+    p.guide.albedo = p.tint * s.albedo;
+    p.guide.normal = s.n;
+    p.guide.done = true;
+  }
+  if (glass) {
+    v3 wi, weight;
+    bool reflected;
+    if (!sample_dielectric(s, hit.front_face, ng, wo, rng, wi, weight, reflected)) return false;
+    p.throughput = p.throughput * weight;
+    if (!p.guide.done) p.tint = p.tint * weight;
+    p.ro = offset_ray(hit.position, reflected ? ng : -ng);
+    p.rd = wi;
+    p.specular = true;
+  } else {
+    if (!p.guide.done) p.tint = p.tint * s.albedo;
+    const v3 shadow_origin = offset_ray(hit.position, ng);
+    const v3 n = s.n;
+    if (sc.sun_enabled != 0) {
+      const float u0 = rng.rand(), u1 = rng.rand();
+      const v3 wi = sample_sun_direction(sc, u0, u1);
+      const float ndl = dot(n, wi);
+      if (ndl > 0.0f && dot(ng, wi) > 0.0f && !occluded(sc, shadow_origin, wi, INF, rng, st)) {
+        float pb;
+        const v3 fv = eval_opaque(sc, s, wo, wi, pb);
+        const float w = mis_weight(sc.sun_pdf, pb);
+        p.radiance = p.radiance + clamp_contribution(p.throughput * fv * ndl * mk(sc.sun_radiance) * (w / sc.sun_pdf), bounce, f.clamp);
+      }
+    }
+    if (sc.env_sampling != 0) {
+      const LightSample ls = sample_environment(sc, rng);
+      const float ndl = dot(n, ls.dir);
+      if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f && !occluded(sc, shadow_origin, ls.dir, INF, rng, st)) {
+        float pb;
+        const v3 fv = eval_opaque(sc, s, wo, ls.dir, pb);
+        const float w = mis_weight(ls.pdf, pb);
+        p.radiance = p.radiance + clamp_contribution(p.throughput * fv * ndl * ls.radiance * (w / ls.pdf), bounce, f.clamp);
+      }
+    }
+    if (sc.light_count > 0) {
+      const int li = min(int(rng.rand() * float(sc.light_count)), int(sc.light_count) - 1);
+      const float u0 = rng.rand(), u1 = rng.rand();
+      const LightSample ls = sample_light(sc, li, hit.position, u0, u1);
+      const float ndl = dot(n, ls.dir);
+      if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f &&
+          !occluded(sc, shadow_origin, ls.dir, ls.dist * (1.0f - 1e-4f), rng, st)) {
+        const float pl = ls.pdf / float(sc.light_count);
+        float pb;
+        const v3 fv = eval_opaque(sc, s, wo, ls.dir, pb);
+        const float w = mis_weight(pl, pb);
+        p.radiance = p.radiance + clamp_contribution(p.throughput * fv * ndl * ls.radiance * (w / pl), bounce, f.clamp);
+      }
+    }
+    const BsdfSample bs = sample_opaque(sc, s, wo, rng);
+    if (bs.pdf <= 0.0f || dot(bs.dir, ng) <= 0.0f) return false;
+    p.throughput = p.throughput * bs.weight;
+    p.bsdf_pdf = bs.pdf;
+    p.specular = false;
+    p.ro = shadow_origin;
+    p.rd = bs.dir;
+  }
+  if (bounce >= 2) {
+    const float q = fminf(0.95f, max3(p.throughput));
+    if (rng.rand() >= q) return false;
+    p.throughput = p.throughput / q;
+  }
+  p.bounce = bounce + 1;
+  return p.bounce <= int(f.max_bounces);
+}
+
+template <class StackT>
+__device__ inline v3 trace_path(const PtScene& sc, const PtFrame& f, v3 ro, v3 rd, Rng& rng, StackT& st, Guide& guide) {
+  PathState p;
+  path_begin(p, ro, rd);
+  if (int(f.max_bounces) >= 0)
+    while (path_step(sc, f, p, rng, st)) {
+    }
+  guide = p.guide;
+  return p.radiance;
+}
+
+// A pixel's accumulation targets (renderer.ts:490 formats) as float values;
+// the FP16 guides hold exactly representable half values.
+struct PixelAccum {
+  float4 color, albedo, normal;
+  float n;
+};
+
+__device__ inline PixelAccum load_accum(const PtFilm& film, uint32_t idx) {
+  auto h2f = [](unsigned short h) { return __half2float(__ushort_as_half(h)); };
+  PixelAccum a;
+  a.color = reinterpret_cast<const float4*>(film.color)[idx];
+  const ushort4 pa = reinterpret_cast<const ushort4*>(film.albedo)[idx];
+  const ushort4 pn = reinterpret_cast<const ushort4*>(film.normal)[idx];
+  a.albedo = make_float4(h2f(pa.x), h2f(pa.y), h2f(pa.z), h2f(pa.w));
+  a.normal = make_float4(h2f(pn.x), h2f(pn.y), h2f(pn.z), h2f(pn.w));
+  a.n = film.history[idx];
+  return a;
+}
+
+__device__ inline float half_round(float x) { return __half2float(__float2half_rn(x)); }
+
+// One sample into the running means (trace.frag:330-354): NaN/Inf -> 0,
+// k = 1/(n+1), GLSL mix = x(1-k) + yk, guides rounded to FP16 as the RGBA16F
+// targets store them.  Contraction is off so that every kernel that folds
+// samples (one per launch, or many in registers) rounds identically.
+__device__ inline float mix_nc(float x, float y, float j, float k) {
+#pragma clang fp contract(off)
+  return x * j + y * k;
+}
+
+__device__ inline void fold_sample(PixelAccum& a, v3 color, const Guide& g) {
+#pragma clang fp contract(off)
   if (isnan(color.x) || isnan(color.y) || isnan(color.z) || isinf(color.x) || isinf(color.y) || isinf(color.z))
     color = mk(0.f, 0.f, 0.f);
-  const float lum = luminance(color);
-  const float n = film.history[idx];
-  const float k = 1.0f / (n + 1.0f);
-  float4 pc = reinterpret_cast<float4*>(film.color)[idx];
-  pc = make_float4(mixf(pc.x, color.x, k), mixf(pc.y, color.y, k), mixf(pc.z, color.z, k), mixf(pc.w, lum * lum, k));
-  reinterpret_cast<float4*>(film.color)[idx] = pc;
-  ushort4 pa = reinterpret_cast<ushort4*>(film.albedo)[idx];
-  ushort4 pn = reinterpret_cast<ushort4*>(film.normal)[idx];
-  auto h2f = [](unsigned short h) { return __half2float(__ushort_as_half(h)); };
+  const float lum = 0.2126f * color.x + 0.7152f * color.y + 0.0722f * color.z;
+  const float k = 1.0f / (a.n + 1.0f);
+  const float j = 1.0f - k;
+  a.color = make_float4(mix_nc(a.color.x, color.x, j, k), mix_nc(a.color.y, color.y, j, k),
+                        mix_nc(a.color.z, color.z, j, k), mix_nc(a.color.w, lum * lum, j, k));
+  a.albedo = make_float4(half_round(mix_nc(a.albedo.x, g.albedo.x, j, k)), half_round(mix_nc(a.albedo.y, g.albedo.y, j, k)),
+                         half_round(mix_nc(a.albedo.z, g.albedo.z, j, k)), half_round(mix_nc(a.albedo.w, g.coverage, j, k)));
+  a.normal = make_float4(half_round(mix_nc(a.normal.x, g.normal.x, j, k)), half_round(mix_nc(a.normal.y, g.normal.y, j, k)),
+                         half_round(mix_nc(a.normal.z, g.normal.z, j, k)), half_round(mix_nc(a.normal.w, g.depth, j, k)));
+  a.n = a.n + 1.0f;
+}
+
+__device__ inline void store_accum(const PtFilm& film, uint32_t idx, const PixelAccum& a) {
   auto f2h = [](float x) { return __half_as_ushort(__float2half_rn(x)); };
-  pa = make_ushort4(f2h(mixf(h2f(pa.x), g.albedo.x, k)), f2h(mixf(h2f(pa.y), g.albedo.y, k)),
-                    f2h(mixf(h2f(pa.z), g.albedo.z, k)), f2h(mixf(h2f(pa.w), g.coverage, k)));
-  pn = make_ushort4(f2h(mixf(h2f(pn.x), g.normal.x, k)), f2h(mixf(h2f(pn.y), g.normal.y, k)),
-                    f2h(mixf(h2f(pn.z), g.normal.z, k)), f2h(mixf(h2f(pn.w), g.depth, k)));
-  reinterpret_cast<ushort4*>(film.albedo)[idx] = pa;
-  reinterpret_cast<ushort4*>(film.normal)[idx] = pn;
-  film.history[idx] = n + 1.0f;
+  reinterpret_cast<float4*>(film.color)[idx] = a.color;
+  reinterpret_cast<ushort4*>(film.albedo)[idx] =
+      make_ushort4(f2h(a.albedo.x), f2h(a.albedo.y), f2h(a.albedo.z), f2h(a.albedo.w));
+  reinterpret_cast<ushort4*>(film.normal)[idx] =
+      make_ushort4(f2h(a.normal.x), f2h(a.normal.y), f2h(a.normal.z), f2h(a.normal.w));
+  film.history[idx] = a.n;
+}
+
+__device__ inline void accumulate(const PtFilm& film, uint32_t idx, v3 color, const Guide& g) {
+  PixelAccum a = load_accum(film, idx);
+  fold_sample(a, color, g);
+  store_accum(film, idx, a);
 }
 
 }  // namespace pt

@@ -78,4 +78,14 @@ struct PtFilm {
   float* history = nullptr;     // float per pixel
 };
 
+// One film's share of a persistent tile launch: `spp` samples of every pixel
+// with seeds frame.seed .. frame.seed + spp - 1, in that order per pixel.
+struct TileItem {
+  PtScene scene;
+  PtFrame frame;
+  PtFilm film;
+  uint32_t spp = 0;
+  uint32_t tiles_x = 0;
+};
+
 }  // namespace lucent

@@ -25,4 +25,13 @@ void camera_basis(const lucent_camera& cam, double fwd[3], double right[3], doub
 
 Status launch_trace_sample(Context* ctx, const PtScene& sc, const PtFrame& f, const PtFilm& film, void* stream);
 
+// Launch shape of the persistent tile kernel (0 = default).
+struct TileLaunch {
+  uint32_t waves_per_cu = 0;
+};
+uint32_t tile_jobs(uint32_t width, uint32_t height);
+// Runs items[i].spp samples of every pixel of each item's film, seeds from
+// items[i].frame.seed; fills items[i].tiles_x.
+Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const TileLaunch& cfg, void* stream);
+
 }  // namespace lucent

@@ -16,6 +16,7 @@ struct Scene {
   PtScene pt;                    // bvh, materials, lights, environment as the kernels read them
   bool lighting_set = false;
   std::vector<void*> env_allocations;  // replaced by each lucent_scene_set_lighting
+  std::vector<void*> tex_allocations;  // replaced by each lucent_scene_set_textures
 };
 
 }  // namespace lucent

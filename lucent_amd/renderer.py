@@ -14,15 +14,25 @@ import numpy as np
 
 from . import _native
 from .runtime import Context, Scene as DeviceScene
-from .scene import PackedScene, Scene, pack_scene
+from .scene import PackedScene, Scene, pack_scene, texture_arrays
 
 
 @dataclass
 class RenderSettings:
     """Integrator part of RenderSettings (renderer.ts:42-84); the reference's
-    default firefly clamp is 8, the benchmark configurations use 0 (unbiased)."""
+    default firefly clamp is 8, the benchmark configurations use 0 (unbiased).
+    per_sample_launch selects the reference's launch structure (one pass over
+    the film per sample) instead of the persistent tile kernel; both give
+    bit-identical films."""
     max_bounces: int = 8
     firefly_clamp: float = 8.0
+    per_sample_launch: bool = False
+    waves_per_cu: int = 0
+
+    def struct(self) -> "_native.RenderSettingsStruct":
+        return _native.RenderSettingsStruct(self.max_bounces, self.firefly_clamp,
+                                            _native.TRACE_PER_SAMPLE if self.per_sample_launch else 0,
+                                            self.waves_per_cu)
 
 
 def lighting_struct(scene: Scene, keep: list) -> _native.LightingStruct:
@@ -132,7 +142,17 @@ class Renderer:
             self.device_scene.destroy()
         self.device_scene = self.ctx.upload_scene(self.packed)
         self.scene = scene
+        self.upload_textures(scene)
         self.upload_lighting(scene)
+
+    def upload_textures(self, scene: Scene) -> None:
+        """The texture arrays of loadScene (renderer.ts:295, 315-335)."""
+        size, color, data = texture_arrays(scene, self.packed)
+        self.texture_size = size
+        _native.check(_native.lib().lucent_scene_set_textures(
+            self.ctx.handle, self.device_scene.handle, size, color.ctypes.data if color.size else None,
+            color.size // (size * size) if size else 0, data.ctypes.data if data.size else None,
+            data.size // (size * size) if size else 0))
 
     def upload_lighting(self, scene: Scene) -> None:
         """Renderer.uploadLighting (renderer.ts:355-413)."""
@@ -147,7 +167,7 @@ class Renderer:
     def trace_samples(self, count: int, stream: Optional[int] = None) -> None:
         """Renderer.traceSamples (renderer.ts:702-737)."""
         cam = camera_struct(self.scene.camera)
-        st = _native.RenderSettingsStruct(self.settings.max_bounces, self.settings.firefly_clamp)
+        st = self.settings.struct()
         _native.check(_native.lib().lucent_trace_samples(self.ctx.handle, self.device_scene.handle, self.film.handle,
                                                          C.byref(cam), C.byref(st), count, stream))
 
@@ -183,3 +203,20 @@ class Renderer:
         self.film.destroy()
         if self.device_scene is not None:
             self.device_scene.destroy()
+
+
+def trace_batch(ctx: Context, jobs, settings: RenderSettings, stream: Optional[int] = None) -> None:
+    """lucent_trace_batch: `jobs` is a list of (Renderer, samples); every
+    renderer's film gets `samples` more samples, all in one persistent launch.
+    The renderers must share `ctx`; their own settings are ignored in favour
+    of `settings`."""
+    arr = (_native.BatchItemStruct * max(1, len(jobs)))()
+    for i, (r, n) in enumerate(jobs):
+        if r.ctx is not ctx:
+            raise ValueError("every renderer of a batch must use the batch's context")
+        arr[i].scene = r.device_scene.handle
+        arr[i].film = r.film.handle
+        arr[i].camera = camera_struct(r.scene.camera)
+        arr[i].samples = int(n)
+    st = settings.struct()
+    _native.check(_native.lib().lucent_trace_batch(ctx.handle, arr, len(jobs), C.byref(st), stream))

@@ -162,8 +162,12 @@ def box(mn, mx, inward=False) -> Mesh:
     return merge_meshes([quad(d, c, b, a) if inward else quad(a, b, c, d) for a, b, c, d in faces])
 
 
-def torus_knot(p=2, q=3, radius=1.0, tube=0.32, tubular_segments=200, radial_segments=20) -> Mesh:
-    """mesh.ts:288-349: (p, q) knot, ring normals, 2*tubular*radial triangles."""
+def torus_knot(p=2, q=3, radius=1.0, tube=0.32, tubular_segments=200, radial_segments=20, uvs=False,
+               displace=None) -> Mesh:
+    """mesh.ts:288-349: (p, q) knot, ring normals, 2*tubular*radial triangles.
+    uvs=True adds (i / tubular, j / radial) texture coordinates per vertex (the
+    reference's knot has none).  displace(points, normals) -> offsets, if
+    given, moves the ring vertices along their normals before triangulation."""
     def curve(t):
         cu, su = np.cos(t), np.sin(t)
         qp = (q / p) * t
@@ -184,6 +188,10 @@ def torus_knot(p=2, q=3, radius=1.0, tube=0.32, tubular_segments=200, radial_seg
     off = N[:, None, :] * cx[None, :, None] + B[:, None, :] * cy[None, :, None]
     ring = P1[:, None, :] + off
     ringn = _normalize_rows(off)
+    if displace is not None:
+        flat = ring.reshape(-1, 3)
+        d = np.asarray(displace(flat, ringn.reshape(-1, 3)), np.float64).reshape(ring.shape[:2])
+        ring = ring + ringn * d[..., None]
     ii, jj = np.meshgrid(np.arange(tubular_segments), np.arange(radial_segments), indexing="ij")
     ii, jj = ii.reshape(-1), jj.reshape(-1)
     i1 = (ii + 1) % tubular_segments
@@ -193,7 +201,39 @@ def torus_knot(p=2, q=3, radius=1.0, tube=0.32, tubular_segments=200, radial_seg
     tris = np.stack([tri_a, tri_b], 1).reshape(-1, 3, 2)
     pos = ring[tris[..., 0], tris[..., 1]]
     nrm = ringn[tris[..., 0], tris[..., 1]]
-    return Mesh(pos.reshape(-1).astype(np.float32), nrm.reshape(-1).astype(np.float32))
+    uv = None
+    if uvs:
+        # Seam-free: the wrapped index (i + 1) % n keeps its unwrapped coordinate.
+        ui = np.stack([np.stack([ii, ii + 1, ii + 1], -1), np.stack([ii, ii + 1, ii], -1)], 1).reshape(-1, 3)
+        vi = np.stack([np.stack([jj, jj, jj + 1], -1), np.stack([jj, jj + 1, jj + 1], -1)], 1).reshape(-1, 3)
+        uv = np.stack([ui / tubular_segments, vi / radial_segments], -1).reshape(-1).astype(np.float32)
+    return Mesh(pos.reshape(-1).astype(np.float32), nrm.reshape(-1).astype(np.float32), uv)
+
+
+def uv_sphere(rings: int, segments: int, radius: float = 1.0, displace=None) -> Mesh:
+    """Latitude-longitude sphere, 2 * rings * segments triangles (the pole rows
+    degenerate), with (longitude, latitude) texture coordinates; displace as
+    for torus_knot, along the radial normal."""
+    th = np.linspace(0.0, math.pi, rings + 1)
+    ph = np.linspace(0.0, 2 * math.pi, segments + 1)
+    st, ct = np.sin(th)[:, None], np.cos(th)[:, None]
+    n = np.stack([st * np.cos(ph)[None, :], np.broadcast_to(ct, (rings + 1, segments + 1)),
+                  st * np.sin(ph)[None, :]], -1)
+    r = np.full((rings + 1, segments + 1), radius)
+    if displace is not None:
+        r = r + np.asarray(displace((n * radius).reshape(-1, 3), n.reshape(-1, 3)), np.float64).reshape(r.shape)
+        r[:, -1] = r[:, 0]  # the seam column is the same points
+    pts = n * r[..., None]
+    ii, jj = np.meshgrid(np.arange(rings), np.arange(segments), indexing="ij")
+    ii, jj = ii.reshape(-1), jj.reshape(-1)
+    a = np.stack([np.stack([ii, jj], -1), np.stack([ii + 1, jj], -1), np.stack([ii + 1, jj + 1], -1)], 1)
+    b = np.stack([np.stack([ii, jj], -1), np.stack([ii + 1, jj + 1], -1), np.stack([ii, jj + 1], -1)], 1)
+    tris = np.stack([a, b], 1).reshape(-1, 3, 2)
+    pos = pts[tris[..., 0], tris[..., 1]]
+    nrm = n[tris[..., 0], tris[..., 1]]
+    uv = np.stack([tris[..., 1] / segments, tris[..., 0] / rings], -1)
+    return Mesh(pos.reshape(-1).astype(np.float32), nrm.reshape(-1).astype(np.float32),
+                uv.reshape(-1).astype(np.float32))
 
 
 def transform(mesh: Mesh, translate=(0.0, 0.0, 0.0), scale=1.0, rotate_y=0.0) -> Mesh:
@@ -296,6 +336,8 @@ class Scene:
     env: Environment = field(default_factory=Environment)
     ground: Ground = field(default_factory=Ground)
     name: str = ""
+    # SceneTexture images (scene.ts:59-62): uint8 (H, W, 4) RGBA arrays, rows in v order.
+    textures: list = field(default_factory=list)
 
 
 @dataclass
@@ -320,6 +362,8 @@ class PackedScene:
     bvh: BVH
     positions: np.ndarray
     material_list: list
+    color_layers: list = field(default_factory=list)  # scene texture index per sRGB layer
+    data_layers: list = field(default_factory=list)   # scene texture index per linear layer
 
 
 def pack_scene(scene: Scene, prebuilt: Optional[BVH] = None) -> PackedScene:
@@ -391,10 +435,84 @@ def pack_scene(scene: Scene, prebuilt: Optional[BVH] = None) -> PackedScene:
     for i, (kind, p, r, e1, e2, rad) in enumerate(lights):
         o = i * 16
         lt[o:o + 16] = [*p, 0 if kind == "sphere" else 1, *e1, r, *e2, 0, *rad, 0]
+    # layerMap (scene.ts:136-153, 272-283): each referenced texture gets one
+    # layer of its array, in first-use order; materials store layer indices.
+    color_layers: list = []
+    data_layers: list = []
+
+    def layer(table: list, tex: int) -> int:
+        if tex < 0:
+            return -1
+        if tex >= len(scene.textures):
+            raise ValueError(f"material references texture {tex}, scene has {len(scene.textures)}")
+        if tex not in table:
+            table.append(tex)
+        return table.index(tex)
+
+    materials = [replace(m, base_color_map=layer(color_layers, m.base_color_map),
+                         emissive_map=layer(color_layers, m.emissive_map),
+                         metal_rough_map=layer(data_layers, m.metal_rough_map),
+                         normal_map=layer(data_layers, m.normal_map)) for m in materials]
     return PackedScene(
         tri_geometry=geo.reshape(-1), tri_attributes=attr.reshape(-1), tri_count=n, bvh_nodes=bvh.nodes,
         bvh_node_count=bvh.node_count, tri_indices=bvh.tri_indices, materials=pack_materials(materials),
         material_count=len(materials), spheres=spheres, sphere_materials=sphere_mats, sphere_lights=sphere_lights,
         sphere_count=len(scene.spheres), lights=lt, light_count=len(lights), ground_material=ground_mat,
         ground_material_active=ground_mat if scene.ground.enabled else -1, ground_height=scene.ground.height,
-        bvh=bvh, positions=merged.positions, material_list=materials)
+        bvh=bvh, positions=merged.positions, material_list=materials, color_layers=color_layers,
+        data_layers=data_layers)
+
+
+TEXTURE_BUDGET_BYTES = 256 * 1024 * 1024
+
+
+def _next_pow2(n: int) -> int:
+    p = 1
+    while p < n:
+        p *= 2
+    return p
+
+
+def resample_rgba8(img: np.ndarray, size: int) -> np.ndarray:
+    """Bilinear resample of an RGBA8 image to size x size (the role of the
+    resample pass in createTextureArray, context.ts:243-250)."""
+    img = np.asarray(img, np.uint8)
+    h, w = img.shape[:2]
+    if h == size and w == size:
+        return img
+    ys = (np.arange(size) + 0.5) * h / size - 0.5
+    xs = (np.arange(size) + 0.5) * w / size - 0.5
+    y0 = np.floor(ys).astype(np.int64)
+    x0 = np.floor(xs).astype(np.int64)
+    ty = (ys - y0)[:, None, None]
+    tx = (xs - x0)[None, :, None]
+    f = img.astype(np.float64)
+
+    def at(yy, xx):
+        return f[np.mod(yy, h)][:, np.mod(xx, w)]
+
+    top = at(y0, x0) * (1 - tx) + at(y0, x0 + 1) * tx
+    bot = at(y0 + 1, x0) * (1 - tx) + at(y0 + 1, x0 + 1) * tx
+    return np.clip(np.rint(top * (1 - ty) + bot * ty), 0, 255).astype(np.uint8)
+
+
+def texture_arrays(scene: Scene, packed: PackedScene):
+    """Renderer.buildTextureArrays (renderer.ts:315-335): the shared layer size
+    (next power of two of the largest used image, at most 2048, halved down to
+    256 while the arrays exceed the 256 MiB budget) and the two RGBA8 arrays
+    as uint32 texels (R in the low byte).  Returns (size, color, data)."""
+    used = list(packed.color_layers) + list(packed.data_layers)
+    if not used:
+        return 0, np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+    largest = max(max(np.asarray(scene.textures[i]).shape[:2]) for i in used)
+    size = min(2048, _next_pow2(largest))
+    while size > 256 and len(used) * size * size * 4 > TEXTURE_BUDGET_BYTES:
+        size //= 2
+
+    def array(layers):
+        if not layers:
+            return np.zeros(0, np.uint32)
+        out = np.stack([resample_rgba8(scene.textures[i], size) for i in layers])
+        return np.ascontiguousarray(out).view(np.uint32).reshape(-1)
+
+    return size, array(packed.color_layers), array(packed.data_layers)

@@ -22,6 +22,9 @@ their functional forms open):
 """
 from __future__ import annotations
 
+import math
+from dataclasses import replace
+
 import numpy as np
 
 _M32 = np.uint64(0xFFFFFFFF)
@@ -293,3 +296,98 @@ def perturb_params(materials, env_rgb, seed: int = 5, amount: float = 0.2):
     env = (np.asarray(env_rgb, np.float32) * rng.uniform(1 - amount, 1 + amount, np.asarray(env_rgb).size)
            ).astype(np.float32)
     return out, env
+
+
+# ---------------------------------------------------------------------------
+# Batched multi-scene workload: 64 seeded scenes of 1k..1.5M triangles
+# ---------------------------------------------------------------------------
+BATCH_SCENES = 64
+BATCH_RESOLUTIONS = ((256, 256), (640, 480), (1280, 720), (1920, 1080))
+BATCH_TRI_RANGE = (1000, 1500000)
+BATCH_DEPTH = 8
+WORLEY_SIZE = 256
+
+
+def worley_alpha(size: int = WORLEY_SIZE, cells: int = 8, seed: int = 3, radius: float = 0.35) -> np.ndarray:
+    """Tileable 0/1 Worley mask as an RGBA8 image (size x size x 4): one
+    jittered feature point per cell of a cells x cells grid; alpha = 0 where
+    the distance to the nearest point (F1, in cell units, with wrap-around) is
+    below `radius`, 255 elsewhere; RGB white."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pts = (np.stack(np.meshgrid(np.arange(cells), np.arange(cells), indexing="ij"), -1)
+           + rng.random((cells, cells, 2)))  # [cx, cy] -> point in cell units
+    c = (np.arange(size) + 0.5) * cells / size
+    px, py = np.meshgrid(c, c, indexing="xy")  # py = row (v), px = column (u)
+    f1 = np.full((size, size), np.inf)
+    for ox in (-1, 0, 1):
+        for oy in (-1, 0, 1):
+            cx = np.floor(px).astype(np.int64) + ox
+            cy = np.floor(py).astype(np.int64) + oy
+            p = pts[np.mod(cx, cells), np.mod(cy, cells)]
+            dx = px - (cx + (p[..., 0] - np.floor(p[..., 0])))
+            dy = py - (cy + (p[..., 1] - np.floor(p[..., 1])))
+            f1 = np.minimum(f1, np.sqrt(dx * dx + dy * dy))
+    img = np.full((size, size, 4), 255, np.uint8)
+    img[..., 3] = np.where(f1 < radius, 0, 255).astype(np.uint8)
+    return img
+
+
+def _grid_displacement(amplitude: float, seed: int, freq: float):
+    """fBm offsets (3 octaves) of ring/grid vertices along their normals."""
+    def f(points, normals):
+        return amplitude * fbm3(points * freq, seed, octaves=3)
+    return f
+
+
+def batch_scene(index: int, seed: int = 2024):
+    """Scene `index` of the batched workload: a log-uniform triangle budget in
+    [1k, 1.5M] split over 1-4 fBm-displaced torus knots or UV spheres (each
+    alpha-tested with probability 0.1: mask mode, cutoff 0.5, the Worley
+    texture), a 2-triangle ground, a quad light and the synthetic HDRI at a
+    per-scene rotation; materials as random_material.  Returns (scene, width,
+    height, spp) with resolution uniform over BATCH_RESOLUTIONS and spp
+    uniform in 1..64."""
+    from . import scene as S
+    rng = np.random.Generator(np.random.PCG64([seed, index]))
+    lo, hi = BATCH_TRI_RANGE
+    total = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+    width, height = BATCH_RESOLUTIONS[int(rng.integers(0, len(BATCH_RESOLUTIONS)))]
+    spp = int(rng.integers(1, 65))
+    k = int(rng.integers(1, 5))
+    meshes, textures = [], [worley_alpha()]
+    for j in range(k):
+        budget = max(64, total // k)
+        disp = _grid_displacement(0.05, seed * 1000 + index * 8 + j, 3.0)
+        if rng.random() < 0.5:
+            radial = int(min(64, max(8, round(math.sqrt(budget / 8)))))
+            tubular = max(8, budget // (2 * radial))
+            p, q = KNOT_PQ[int(rng.integers(0, len(KNOT_PQ)))]
+            m = S.torus_knot(p, q, 1.0, 0.3, tubular, radial, uvs=True, displace=disp)
+        else:
+            rings = max(4, int(round(math.sqrt(budget / 4))))
+            segments = max(8, budget // (2 * rings))
+            m = S.uv_sphere(rings, segments, 1.1, displace=disp)
+        ang = 2 * math.pi * j / k
+        m = S.transform(m, translate=(2.6 * math.cos(ang) if k > 1 else 0.0, 1.2,
+                                      2.6 * math.sin(ang) if k > 1 else 0.0),
+                        rotate_y=float(rng.uniform(0, 2 * math.pi)))
+        mat = random_material(rng)
+        if rng.random() < 0.1:
+            mat = replace(mat, alpha_mode="mask", alpha_cutoff=0.5, base_color_map=0, transmission=0.0)
+        meshes.append(S.MeshInstance(m, mat))
+    meshes.append(S.MeshInstance(S.floor((0.0, 0.0, 0.0), 10.0), S.matte((0.5, 0.5, 0.5), 1.0)))
+    lamp = S.quad((-1.0, 5.5, -1.0), (1.0, 5.5, -1.0), (1.0, 5.5, 1.0), (-1.0, 5.5, 1.0))
+    meshes.append(S.MeshInstance(lamp, S.light((1.0, 0.9, 0.75), 12.0)))
+    env = S.Environment(map=(1024, 512, _batch_hdri()), rotation=float(rng.uniform(0, 360)), intensity=1.0,
+                        background=True)
+    cam = S.Camera(position=(0.0, 4.5, 9.0), target=(0.0, 1.0, 0.0), fov=45.0, aperture=0.0, focus_distance=9.0)
+    return S.Scene(camera=cam, meshes=meshes, env=env, name=f"batch{index}", textures=textures), width, height, spp
+
+
+_HDRI_CACHE: dict = {}
+
+
+def _batch_hdri():
+    if "map" not in _HDRI_CACHE:
+        _HDRI_CACHE["map"] = synthetic_hdri(1024, 512)
+    return _HDRI_CACHE["map"]

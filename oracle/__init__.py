@@ -413,6 +413,11 @@ class PtOracle:
         s.env_background = 1 if env.background else 0
         s.backdrop[:] = list(env.backdrop)
         s.sheen_lut = arr(sheen_lut_half(), np.float32)
+        if textures is None and (getattr(packed, "color_layers", None) or getattr(packed, "data_layers", None)):
+            # The same layer arrays the renderer uploads (a data format, built
+            # from the scene's images by the host packer).
+            from lucent_amd.scene import texture_arrays
+            textures = texture_arrays(scene, packed)
         if textures is not None:
             size, color, data = textures
             s.color_maps = arr(color, np.uint32)

@@ -801,25 +801,25 @@ static surf_t surface_at(const or_scene *sc, const hit_t *h, d3 wo, d3 *em) {
   }
   int tex = sc->tex_size > 0, mb = (int)m[16], mmr = (int)m[17], mn = (int)m[18], me = (int)m[19];
   double o[4];
-  if (mb >= 0 && tex) {
+  if (mb >= 0 && mb < sc->color_layers && tex) {
     sample_layer(sc->color_maps, sc->tex_size, mb, tu, tv, 1, o);
     s.albedo = mul(s.albedo, V(o[0], o[1], o[2]));
   }
   s.metallic = m[3];
   s.roughness = m[4];
-  if (mmr >= 0 && tex) {
+  if (mmr >= 0 && mmr < sc->data_layers && tex) {
     sample_layer(sc->data_maps, sc->tex_size, mmr, tu, tv, 0, o);
     s.roughness *= o[1];
     s.metallic *= o[2];
   }
   *em = Vf(m + 8);
-  if (me >= 0 && tex) {
+  if (me >= 0 && me < sc->color_layers && tex) {
     sample_layer(sc->color_maps, sc->tex_size, me, tu, tv, 1, o);
     *em = mul(*em, V(o[0], o[1], o[2]));
   }
   s.nc = h->n;
   s.n = h->n;
-  if (mn >= 0 && tex && dot(h->dpdu, h->dpdu) > 0) {
+  if (mn >= 0 && mn < sc->data_layers && tex && dot(h->dpdu, h->dpdu) > 0) {
     d3 N = h->front ? h->n : neg(h->n);
     d3 T = nrm(sub(h->dpdu, scl(N, dot(N, h->dpdu))));
     d3 B = cross(N, T);

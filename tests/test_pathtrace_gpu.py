@@ -158,3 +158,113 @@ def test_reset_keeps_seed_advancing(ctx):
     h = r.film.read()["history"]
     assert np.all(h == 1)
     r.close()
+
+
+def _film_after(ctx, scene, w, h, spp, seed, settings):
+    r = Renderer(ctx, w, h, settings)
+    r.load_scene(scene)
+    r.film.set_seed(seed)
+    r.reset()
+    r.trace_samples(spp)
+    out = r.film.read()
+    info = r.film.info()
+    r.close()
+    return out, info
+
+
+@pytest.mark.parametrize("clamp", [0.0, 8.0])
+def test_persistent_tiles_bit_identical_to_per_sample_launches(ctx, clamp):
+    """The persistent tile kernel (all samples of a pixel in one lane, refilled
+    as paths end, running means in registers) and the reference's structure
+    (one launch per sample) give the same film bit for bit, including ragged
+    edge tiles (70x45 is not a multiple of 8)."""
+    sc = feature_scene()
+    a, ia = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, clamp, per_sample_launch=True))
+    b, ib = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, clamp))
+    assert ia == ib
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_batch_equals_individual_films(ctx):
+    """lucent_trace_batch over several scenes, sizes and sample counts: every
+    film equals what lucent_trace_samples makes of it alone; a second batch
+    continues each film's seeds and history."""
+    from lucent_amd.renderer import trace_batch
+
+    scenes = [feature_scene(), workloads.spheres_scene(), feature_scene()]
+    sizes = [(64, 40), (48, 48), (33, 17)]
+    spps = [(3, 2), (1, 4), (5, 0)]
+    settings = RenderSettings(8, 0.0)
+    rs = []
+    for sc, (w, h) in zip(scenes, sizes):
+        r = Renderer(ctx, w, h, settings)
+        r.load_scene(sc)
+        r.film.set_seed(11)
+        r.reset()
+        rs.append(r)
+    trace_batch(ctx, [(r, n[0]) for r, n in zip(rs, spps)], settings)
+    trace_batch(ctx, [(r, n[1]) for r, n in zip(rs, spps)], settings)
+    for r, sc, (w, h), n in zip(rs, scenes, sizes, spps):
+        got = r.film.read()
+        want, info = _film_after(ctx, sc, w, h, n[0] + n[1], 11, RenderSettings(8, 0.0, per_sample_launch=True))
+        assert r.film.info() == info
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (w, h, k)
+    from lucent_amd._native import LucentError
+    with pytest.raises(LucentError, match="twice"):
+        trace_batch(ctx, [(rs[0], 1), (rs[0], 1)], settings)
+    for r in rs:
+        r.close()
+
+
+def alpha_scene():
+    """Alpha-tested geometry as the batched workload has it: a Worley-masked
+    (mask mode, cutoff 0.5) knot in front of a textured-alpha quad and a lamp,
+    so that closest hits and shadow rays both pass through the holes."""
+    tex = workloads.worley_alpha(64, cells=4, seed=5)
+    knot = S.transform(S.torus_knot(2, 3, 1.0, 0.35, 120, 16, uvs=True), translate=(0.0, 1.2, 0.0))
+    masked = S.Material(albedo=(0.8, 0.5, 0.3), roughness=0.5, alpha_mode="mask", alpha_cutoff=0.5, base_color_map=0)
+    wall = S.quad((-3.0, 0.0, -2.0), (3.0, 0.0, -2.0), (3.0, 3.5, -2.0), (-3.0, 3.5, -2.0))
+    wall.uvs = np.tile(np.array([0, 0, 1, 0, 1, 1, 0, 0, 1, 1, 0, 1], np.float32) * 2.0, 1)
+    cut = S.Material(albedo=(0.3, 0.6, 0.8), roughness=0.8, alpha_mode="mask", alpha_cutoff=0.5, base_color_map=0)
+    lamp = S.quad((-0.5, 3.8, -0.5), (0.5, 3.8, -0.5), (0.5, 3.8, 0.5), (-0.5, 3.8, 0.5))
+    w, h = 128, 64
+    env = S.Environment(map=(w, h, workloads.synthetic_hdri(w, h, sigma_deg=4.0, peak=300.0)), rotation=10.0,
+                        intensity=1.0, background=True)
+    return S.Scene(
+        camera=S.Camera(position=(0.5, 2.0, 5.5), target=(0.0, 1.2, 0.0), fov=45.0, aperture=0.0, focus_distance=5.5),
+        meshes=[S.MeshInstance(knot, masked), S.MeshInstance(wall, cut),
+                S.MeshInstance(lamp, S.light((1.0, 0.9, 0.8), 8.0))],
+        env=env, ground=S.Ground(True, 0.0, S.matte((0.5, 0.55, 0.6), 0.9)), textures=[tex])
+
+
+def test_alpha_masked_scene_matches_oracle(ctx):
+    sc = alpha_scene()
+    W, H, spp = 48, 32, 16
+    gpu, packed = render(ctx, sc, W, H, spp)
+    assert packed.color_layers == [0]
+    assert np.count_nonzero(packed.tri_geometry.reshape(-1, 12)[:, 11]) == packed.tri_count - 2
+    xy = all_pixels(W, H)
+    ref, per = oracle_pixels(packed, sc, W, H, spp, xy)
+    compare(gpu, ref, per, xy, spp, "alpha-masked scene")
+
+
+def test_batch_with_alpha_scene_bit_identical(ctx):
+    from lucent_amd.renderer import trace_batch
+
+    scenes = [alpha_scene(), workloads.batch_scene(1)[0]]
+    sizes = [(40, 24), (64, 48)]
+    settings = RenderSettings(8, 0.0)
+    rs = []
+    for sc, (w, h) in zip(scenes, sizes):
+        r = Renderer(ctx, w, h, settings)
+        r.load_scene(sc)
+        rs.append(r)
+    trace_batch(ctx, [(rs[0], 6), (rs[1], 3)], settings)
+    for r, sc, (w, h), n in zip(rs, scenes, sizes, (6, 3)):
+        want, _ = _film_after(ctx, sc, w, h, n, 0, RenderSettings(8, 0.0, per_sample_launch=True))
+        got = r.film.read()
+        for k in want:
+            assert np.array_equal(got[k], want[k]), k
+        r.close()
