@@ -203,6 +203,9 @@ def bench_path_trace(args) -> dict:
         wall = time.perf_counter() - wall0
         return wall, [s_.elapsed_time(e) for s_, e in zip(starts, ends)]
 
+    from lucent_amd.renderer import autotune
+    tune = autotune(ctx, [(r, SPP)], r.settings, trial_samples=4)
+    log(f"autotuned occupancy {tune['occupancy']} ({tune['ms']} ms for 4 spp)")
     log(f"warmup {args.warmup}")
     if args.warmup:
         timed_steps(args.warmup, False)
@@ -227,7 +230,8 @@ def bench_path_trace(args) -> dict:
     traffic = profiled_traffic("path_trace_1440x900")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_path": round(b_path - 72, 1),
-                "film_bytes_per_pixel": 72, "kernel": "k_trace_tiles", "avg_launch_ms": round(launch_s * 1e3, 3)}
+                "film_bytes_per_pixel": 72, "kernel": "k_trace_tiles", "avg_launch_ms": round(launch_s * 1e3, 3),
+                "occupancy": tune["occupancy"]}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_path_trace(r.packed, scene, W, H)
@@ -244,6 +248,7 @@ def bench_path_trace(args) -> dict:
                        "scene_load_s": round(load_s, 3)},
             "roofline": roofline, "cpu_baseline": cpu, "image_mean_rgb": mean_rgb,
             "mpaths_per_s": round(value / 1e6, 2), "mrays_per_s_all_rays": round(value * rays_per_path / 1e6, 1),
+            "autotune": tune,
             "per_sample_launches": {"samples_per_s": round(per_sample_value, 1),
                                     "trace_ms": round(float(np.mean(trace_ms_ps)), 3),
                                     "avg_sample_launch_ms": round(float(np.mean(trace_ms_ps)) / SPP, 4),
@@ -425,6 +430,9 @@ def bench_batch(args) -> dict:
         ctx.synchronize()
 
     settings = RenderSettings(max_bounces=workloads.BATCH_DEPTH, firefly_clamp=0.0)
+    from lucent_amd.renderer import autotune
+    tune = autotune(ctx, jobs, settings, trial_samples=1)
+    log(f"autotuned occupancy {tune['occupancy']} ({tune['ms']} ms for 1 spp of every scene)")
 
     def run_batched():
         trace_batch(ctx, jobs, settings)
@@ -461,7 +469,7 @@ def bench_batch(args) -> dict:
     return {"metric": "samples/s, 64-scene batch: one persistent launch vs per-sample launches per scene",
             "value": round(paths / tb, 1), "unit": "samples/s", "plain_samples_per_s": round(paths / tp, 1),
             "speedup_vs_plain": round(tp / tb, 3), "batched_s": round(tb, 3), "plain_s": round(tp, 3),
-            "films_bit_identical": same, "scenes": len(jobs), "triangles": tris, "paths": paths,
+            "films_bit_identical": same, "autotune": tune, "scenes": len(jobs), "triangles": tris, "paths": paths,
             "setup_s": round(setup_s, 1)}
 
 

@@ -271,9 +271,12 @@ typedef struct lucent_camera {
 
 typedef struct lucent_render_settings {
   uint32_t max_bounces;
-  float firefly_clamp; /* 0 disables (uClamp) */
-  uint32_t flags;      /* LUCENT_TRACE_* */
-  uint32_t waves_per_cu; /* persistent launch width; 0 = default (8) */
+  float firefly_clamp;   /* 0 disables (uClamp) */
+  uint32_t flags;        /* LUCENT_TRACE_* */
+  uint32_t waves_per_cu; /* persistent launch width; 0 = 4 x occupancy */
+  uint32_t occupancy;    /* persistent kernel variant: registers for 2..6 waves per SIMD; 0 = 2.
+                            3..6 spill registers to scratch but hide more memory latency.
+                            Results do not depend on it (lucent_autotune_tiles picks one). */
 } lucent_render_settings;
 
 lucent_status lucent_trace_samples(lucent_ctx *ctx, lucent_scene *scene, lucent_film *film,
@@ -299,6 +302,17 @@ typedef struct lucent_batch_item {
 
 lucent_status lucent_trace_batch(lucent_ctx *ctx, const lucent_batch_item *items, uint32_t count,
                                  const lucent_render_settings *settings, void *stream);
+
+/* Autotuner of the persistent launch: runs min(trial_samples, samples) samples
+ * of every item (at least 1) once with each register/occupancy variant
+ * (2..6 waves per SIMD) into scratch films of the items' sizes (the items'
+ * films are not touched), after one untimed round, and returns the fastest
+ * variant for lucent_render_settings.occupancy.  ms_out, if not NULL,
+ * receives 5 timings (variants 2..6).  The variants give identical films.
+ * No counterpart in the reference. */
+lucent_status lucent_autotune_tiles(lucent_ctx *ctx, const lucent_batch_item *items, uint32_t count,
+                                    const lucent_render_settings *settings, uint32_t trial_samples,
+                                    uint32_t *occupancy_out, float *ms_out);
 
 /* ------------------------------------------------------------------------
  * Denoiser — Renderer.runDenoiser (src/renderer.ts:788-830): variance.frag,

@@ -97,7 +97,7 @@ TRACE_PER_SAMPLE = 0x1
 
 class RenderSettingsStruct(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("firefly_clamp", C.c_float), ("flags", C.c_uint32),
-                ("waves_per_cu", C.c_uint32)]
+                ("waves_per_cu", C.c_uint32), ("occupancy", C.c_uint32)]
 
 
 class BatchItemStruct(C.Structure):
@@ -141,6 +141,7 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_film_device_ptrs": (_I32, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "lucent_trace_samples": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _P]),
     "lucent_trace_batch": (_I32, [_P, _P, _U32, C.POINTER(RenderSettingsStruct), _P]),
+    "lucent_autotune_tiles": (_I32, [_P, _P, _U32, C.POINTER(RenderSettingsStruct), _U32, C.POINTER(_U32), _P]),
     "lucent_denoise": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(DenoiseParamsStruct), _P]),
     "lucent_film_read_denoised": (_I32, [_P, _P, _P, _P]),
     "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,

@@ -22,7 +22,7 @@ struct lucent_film {
   uint32_t seed = 0;     // Renderer.seed
   void* base = nullptr;
   PtFilm f;
-  void* filter = nullptr;       // two RGBA32F filter targets + one remodulated RGBA32F target
+  void* filter = nullptr;       // two RGBA32F filter targets, one remodulated RGBA32F target, 32 B/px of guides
   float* denoised = nullptr;    // the target holding the last denoise result
   float* remodulated = nullptr;
 };
@@ -345,6 +345,7 @@ lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_
   it.spp = count;
   TileLaunch cfg;
   cfg.waves_per_cu = settings->waves_per_cu;
+  cfg.occupancy = settings->occupancy;
   if ((s = launch_trace_tiles(ctx, &it, 1, cfg, st)) != LUCENT_OK) return s;
   film->seed += count;
   film->samples += count;
@@ -396,12 +397,84 @@ lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items
   }
   TileLaunch cfg;
   cfg.waves_per_cu = settings->waves_per_cu;
+  cfg.occupancy = settings->occupancy;
   Status s = launch_trace_tiles(ctx, tiles.data(), uint32_t(tiles.size()), cfg, st);
   if (s != LUCENT_OK) return s;
   for (uint32_t i = 0; i < count; ++i) {
     items[i].film->seed += items[i].samples;
     items[i].film->samples += items[i].samples;
   }
+  return LUCENT_OK;
+}
+
+lucent_status lucent_autotune_tiles(lucent_ctx* ctx, const lucent_batch_item* items, uint32_t count,
+                                    const lucent_render_settings* settings, uint32_t trial_samples,
+                                    uint32_t* occupancy_out, float* ms_out) {
+  clear_error();
+  if (!ctx || !items || count == 0 || !settings || !occupancy_out) return fail(LUCENT_E_INVALID, "null argument");
+  if (trial_samples == 0) return fail(LUCENT_E_INVALID, "trial_samples must be positive");
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!items[i].scene || !items[i].film) return fail(LUCENT_E_INVALID, "batch item without scene or film");
+    Status s = check_traceable(items[i].scene);
+    if (s != LUCENT_OK) return s;
+  }
+  hipStream_t st = static_cast<hipStream_t>(ctx->stream);
+  // Scratch films of the items' sizes: the callers' films are not touched.
+  std::vector<lucent_film*> films(count, nullptr);
+  std::vector<TileItem> tiles;
+  Status s = LUCENT_OK;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  float best_ms = 0.0f;
+  uint32_t best = 0;
+  for (uint32_t i = 0; i < count && s == LUCENT_OK; ++i) {
+    s = lucent_film_create(ctx, items[i].film->width, items[i].film->height, &films[i]);
+    if (s != LUCENT_OK) break;
+    TileItem it;
+    it.scene = items[i].scene->pt;
+    fill_frame(&items[i].camera, settings, films[i]->width, films[i]->height, &it.frame);
+    it.frame.seed = items[i].film->seed;
+    it.film = films[i]->f;
+    it.spp = std::max(1u, std::min(trial_samples, items[i].samples ? items[i].samples : trial_samples));
+    tiles.push_back(it);
+  }
+  if (s == LUCENT_OK) {
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e != hipSuccess) s = fail_hip(e, "hipEventCreate");
+  }
+  for (int pass = 0; pass < 2 && s == LUCENT_OK; ++pass) {  // pass 0 loads every code object
+    for (uint32_t occ = 2; occ <= 6 && s == LUCENT_OK; ++occ) {
+      TileLaunch cfg;
+      cfg.occupancy = occ;
+      cfg.waves_per_cu = settings->waves_per_cu;
+      hipError_t e = hipEventRecord(ev[0], st);
+      if (e != hipSuccess) {
+        s = fail_hip(e, "hipEventRecord");
+        break;
+      }
+      if ((s = launch_trace_tiles(ctx, tiles.data(), uint32_t(tiles.size()), cfg, st)) != LUCENT_OK) break;
+      e = hipEventRecord(ev[1], st);
+      if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+      if (e != hipSuccess) {
+        s = fail_hip(e, "autotune timing");
+        break;
+      }
+      if (pass == 1) {
+        if (ms_out) ms_out[occ - 2] = ms;
+        if (best == 0 || ms < best_ms) {
+          best = occ;
+          best_ms = ms;
+        }
+      }
+    }
+  }
+  for (hipEvent_t e : ev)
+    if (e) hipEventDestroy(e);
+  for (lucent_film* f : films) lucent_film_destroy(f);
+  if (s != LUCENT_OK) return s;
+  *occupancy_out = best;
   return LUCENT_OK;
 }
 
@@ -535,7 +608,7 @@ lucent_status lucent_denoise(lucent_ctx* ctx, lucent_film* film, const lucent_ca
   if (p->iterations > 12) return fail(LUCENT_E_INVALID, "at most 12 a-trous iterations");
   const size_t px = size_t(film->width) * film->height;
   if (!film->filter) {
-    Status s = dev_alloc(&film->filter, px * 16 * 3);
+    Status s = dev_alloc(&film->filter, px * (16 * 3 + 32));
     if (s != LUCENT_OK) return s;
   }
   DenoiseArgs a;
@@ -546,6 +619,7 @@ lucent_status lucent_denoise(lucent_ctx* ctx, lucent_film* film, const lucent_ca
   a.ping = static_cast<float*>(film->filter);
   a.pong = a.ping + px * 4;
   a.remodulated = a.pong + px * 4;
+  a.guides = a.remodulated + px * 4;
   a.result = &film->denoised;
   a.width = int(film->width);
   a.height = int(film->height);

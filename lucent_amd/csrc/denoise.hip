@@ -7,7 +7,12 @@
 // Guides are read from the film's FP16 targets; every tap is accumulated in
 // float32.  One lane per pixel, 16x8 pixel tiles per block (two wave64s), so
 // a block's taps at one offset form two 8-row spans that the L2 serves for
-// the neighbouring blocks too.
+// the neighbouring blocks too.  The variance pass also stores, per pixel, the
+// unit normal, the coverage flag and the world position (camera ray through
+// the pixel centre times the stored depth): the à-trous taps then read these
+// 32 bytes instead of renormalising the neighbour's normal and rebuilding
+// its position at every tap, and evaluate their edge-stopping exponentials
+// with the hardware log2/exp2 and per-pixel reciprocals.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -45,12 +50,27 @@ __device__ inline void unit_normal(float x, float y, float z, float& ox, float& 
   }
 }
 
+// primaryDir (atrous.frag:22-26) — the pixel centre's camera ray.
+__device__ inline void primary_dir(const Cam& c, int px, int py, int w, int h, float& dx, float& dy, float& dz) {
+  const float ux = (float(px) + 0.5f) / float(w) * 2.0f - 1.0f;
+  const float uy = (float(py) + 0.5f) / float(h) * 2.0f - 1.0f;
+  const float aspect = float(w) / float(h);
+  const float sx = ux * aspect * c.tan_half_fov, sy = uy * c.tan_half_fov;
+  float x = c.fwd[0] + c.right[0] * sx + c.up[0] * sy;
+  float y = c.fwd[1] + c.right[1] * sx + c.up[1] * sy;
+  float z = c.fwd[2] + c.right[2] * sx + c.up[2] * sy;
+  const float inv = rsqrtf(x * x + y * y + z * z);
+  dx = x * inv;
+  dy = y * inv;
+  dz = z * inv;
+}
+
 // variance.frag:15-55.
 __global__ __launch_bounds__(kTx* kTy) void k_variance(const float* __restrict__ color,
                                                        const uint16_t* __restrict__ albedo,
                                                        const uint16_t* __restrict__ normal,
                                                        const float* __restrict__ history, float* __restrict__ out,
-                                                       int w, int h) {
+                                                       float* __restrict__ guides, int w, int h, Cam cam) {
   const int x = blockIdx.x * kTx + threadIdx.x % kTx;
   const int y = blockIdx.y * kTy + threadIdx.x / kTx;
   if (x >= w || y >= h) return;
@@ -58,6 +78,15 @@ __global__ __launch_bounds__(kTx* kTy) void k_variance(const float* __restrict__
   const float4 c = reinterpret_cast<const float4*>(color)[i];
   const float4 a = load_half4(albedo, i);
   float4* o = reinterpret_cast<float4*>(out);
+  {
+    const float4 np = load_half4(normal, i);
+    float nx, ny, nz, dx, dy, dz;
+    unit_normal(np.x, np.y, np.z, nx, ny, nz);
+    primary_dir(cam, x, y, w, h, dx, dy, dz);
+    float4* g = reinterpret_cast<float4*>(guides) + 2 * i;
+    g[0] = make_float4(nx, ny, nz, a.w < 0.5f ? 0.0f : 1.0f);
+    g[1] = make_float4(cam.pos[0] + dx * np.w, cam.pos[1] + dy * np.w, cam.pos[2] + dz * np.w, np.w);
+  }
   if (a.w < 0.5f) {
     o[i] = make_float4(c.x, c.y, c.z, 0.0f);
     return;
@@ -99,44 +128,28 @@ __global__ __launch_bounds__(kTx* kTy) void k_variance(const float* __restrict__
   o[i] = make_float4(ir, ig, ib, variance);
 }
 
-// primaryDir (atrous.frag:22-26) — the pixel centre's camera ray.
-__device__ inline void primary_dir(const Cam& c, int px, int py, int w, int h, float& dx, float& dy, float& dz) {
-  const float ux = (float(px) + 0.5f) / float(w) * 2.0f - 1.0f;
-  const float uy = (float(py) + 0.5f) / float(h) * 2.0f - 1.0f;
-  const float aspect = float(w) / float(h);
-  const float sx = ux * aspect * c.tan_half_fov, sy = uy * c.tan_half_fov;
-  float x = c.fwd[0] + c.right[0] * sx + c.up[0] * sy;
-  float y = c.fwd[1] + c.right[1] * sx + c.up[1] * sy;
-  float z = c.fwd[2] + c.right[2] * sx + c.up[2] * sy;
-  const float inv = rsqrtf(x * x + y * y + z * z);
-  dx = x * inv;
-  dy = y * inv;
-  dz = z * inv;
-}
+__device__ inline float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 
-// atrous.frag:30-82.
-__global__ __launch_bounds__(kTx* kTy) void k_atrous(const float* __restrict__ in, const uint16_t* __restrict__ albedo,
-                                                     const uint16_t* __restrict__ normal, float* __restrict__ out,
-                                                     int w, int h, int step, Cam cam, float sigma_l, float sigma_n,
-                                                     float sigma_p) {
+// atrous.frag:30-82, with the guides of k_variance.
+__global__ __launch_bounds__(kTx* kTy) void k_atrous(const float* __restrict__ in, const float* __restrict__ guides,
+                                                     float* __restrict__ out, int w, int h, int step, float tan_half_fov,
+                                                     float sigma_l, float sigma_n, float sigma_p) {
   const int x = blockIdx.x * kTx + threadIdx.x % kTx;
   const int y = blockIdx.y * kTy + threadIdx.x / kTx;
   if (x >= w || y >= h) return;
   const size_t i = size_t(y) * w + x;
   const float4* I = reinterpret_cast<const float4*>(in);
+  const float4* G = reinterpret_cast<const float4*>(guides);
   const float4 cp = I[i];
   float4* O = reinterpret_cast<float4*>(out);
-  if (load_half4(albedo, i).w < 0.5f) {
+  const float4 gn = G[2 * i];
+  if (gn.w < 0.5f) {
     O[i] = cp;
     return;
   }
-  const float4 np = load_half4(normal, i);
-  float nx, ny, nz;
-  unit_normal(np.x, np.y, np.z, nx, ny, nz);
-  float dx, dy, dz;
-  primary_dir(cam, x, y, w, h, dx, dy, dz);
-  const float Px = cam.pos[0] + dx * np.w, Py = cam.pos[1] + dy * np.w, Pz = cam.pos[2] + dz * np.w;
-  const float footprint = 2.0f * cam.tan_half_fov * np.w / float(h);
+  const float4 gp = G[2 * i + 1];
+  // footprint = 2 tan(fov/2) depth / H, depth = the stored primary hit distance.
+  const float footprint = 2.0f * tan_half_fov * gp.w / float(h);
 
   // 3x3 binomial blur of the variance (atrous.frag:43-54).
   float var = 0.0f, wv = 0.0f;
@@ -148,7 +161,18 @@ __global__ __launch_bounds__(kTx* kTy) void k_atrous(const float* __restrict__ i
       var += I[size_t(qy) * w + qx].w * k;
       wv += k;
     }
-  const float denom = sigma_l * sqrtf(fmaxf(var / wv, 0.0f)) + 1e-6f;
+  const float inv_denom = 1.0f / (sigma_l * sqrtf(fmaxf(var / wv, 0.0f)) + 1e-6f);
+  // 1 / (sigma_p * footprint * step * |offset| + 1e-6) for |offset|^2 = 1, 2, 4, 5, 8.
+  const float base = sigma_p * footprint * float(step);
+  const float inv_reach[9] = {0.0f,
+                              1.0f / (base * 1.0f + 1e-6f),
+                              1.0f / (base * 1.41421356f + 1e-6f),
+                              0.0f,
+                              1.0f / (base * 2.0f + 1e-6f),
+                              1.0f / (base * 2.23606798f + 1e-6f),
+                              0.0f,
+                              0.0f,
+                              1.0f / (base * 2.82842712f + 1e-6f)};
   const float lp = lum3(cp.x, cp.y, cp.z);
   const float hk[3] = {0.375f, 0.25f, 0.0625f};
   float sr = cp.x * (hk[0] * hk[0]), sg = cp.y * (hk[0] * hk[0]), sb = cp.z * (hk[0] * hk[0]);
@@ -160,19 +184,16 @@ __global__ __launch_bounds__(kTx* kTy) void k_atrous(const float* __restrict__ i
       const int qx = x + ox * step, qy = y + oy * step;
       if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
       const size_t q = size_t(qy) * w + qx;
-      if (load_half4(albedo, q).w < 0.5f) continue;
+      const float4 qn = G[2 * q];
+      if (qn.w < 0.5f) continue;
+      const float4 qp = G[2 * q + 1];
       const float4 cq = I[q];
-      const float4 nq = load_half4(normal, q);
       const float k = hk[ox < 0 ? -ox : ox] * hk[oy < 0 ? -oy : oy];
-      float ux, uy, uz;
-      unit_normal(nq.x, nq.y, nq.z, ux, uy, uz);
-      const float wn = powf(fmaxf(0.0f, nx * ux + ny * uy + nz * uz), sigma_n);
-      float qdx, qdy, qdz;
-      primary_dir(cam, qx, qy, w, h, qdx, qdy, qdz);
-      const float Qx = cam.pos[0] + qdx * nq.w, Qy = cam.pos[1] + qdy * nq.w, Qz = cam.pos[2] + qdz * nq.w;
-      const float reach = sigma_p * footprint * float(step) * sqrtf(float(ox * ox + oy * oy));
-      const float wp = expf(-fabsf(nx * (Qx - Px) + ny * (Qy - Py) + nz * (Qz - Pz)) / (reach + 1e-6f));
-      const float wl = expf(-fabsf(lp - lum3(cq.x, cq.y, cq.z)) / denom);
+      const float c = fmaxf(0.0f, gn.x * qn.x + gn.y * qn.y + gn.z * qn.z);
+      const float wn = sigma_n == 0.0f ? 1.0f : __builtin_amdgcn_exp2f(sigma_n * __builtin_amdgcn_logf(c));
+      const float plane = fabsf(gn.x * (qp.x - gp.x) + gn.y * (qp.y - gp.y) + gn.z * (qp.z - gp.z));
+      const float wp = fast_exp(-plane * inv_reach[ox * ox + oy * oy]);
+      const float wl = fast_exp(-fabsf(lp - lum3(cq.x, cq.y, cq.z)) * inv_denom);
       const float wgt = k * wn * wp * wl;
       sr += cq.x * wgt;
       sg += cq.y * wgt;
@@ -202,8 +223,6 @@ Status launch_denoise(const DenoiseArgs& a, void* stream_) {
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   const dim3 grid((a.width + kTx - 1) / kTx, (a.height + kTy - 1) / kTy);
   const dim3 block(kTx * kTy);
-  hipLaunchKernelGGL(k_variance, grid, block, 0, stream, a.color, a.albedo, a.normal, a.history, a.ping, a.width,
-                     a.height);
   Cam cam;
   for (int k = 0; k < 3; ++k) {
     cam.pos[k] = a.cam_pos[k];
@@ -212,11 +231,13 @@ Status launch_denoise(const DenoiseArgs& a, void* stream_) {
     cam.fwd[k] = a.cam_fwd[k];
   }
   cam.tan_half_fov = a.tan_half_fov;
+  hipLaunchKernelGGL(k_variance, grid, block, 0, stream, a.color, a.albedo, a.normal, a.history, a.ping, a.guides,
+                     a.width, a.height, cam);
   float* read = a.ping;
   float* write = a.pong;
   for (uint32_t it = 0; it < a.iterations; ++it) {
-    hipLaunchKernelGGL(k_atrous, grid, block, 0, stream, read, a.albedo, a.normal, write, a.width, a.height,
-                       int(1u << it), cam, a.sigma_l, a.sigma_n, a.sigma_p);
+    hipLaunchKernelGGL(k_atrous, grid, block, 0, stream, read, a.guides, write, a.width, a.height, int(1u << it),
+                       a.tan_half_fov, a.sigma_l, a.sigma_n, a.sigma_p);
     float* t = read;
     read = write;
     write = t;

@@ -15,6 +15,7 @@ struct DenoiseArgs {
   float* ping = nullptr;             // filter target A (RGBA32F)
   float* pong = nullptr;             // filter target B (RGBA32F)
   float* remodulated = nullptr;      // optional RGBA32F output: illumination * albedo
+  float* guides = nullptr;           // 8 floats per pixel: unit normal + coverage, world position
   float** result = nullptr;          // receives the target holding the final iteration
   int width = 0, height = 0;
   uint32_t iterations = 5;

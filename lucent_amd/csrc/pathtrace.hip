@@ -9,13 +9,13 @@
 // * k_trace_tiles — persistent waves pull 8x8 tile jobs from an atomic queue
 //   (one or many films / scenes per launch) and run ALL of a job's samples:
 //   each lane owns one pixel, advances its path one bounce at a time and, as
-//   soon as the path ends, folds it into the pixel's running mean (kept in
-//   registers) and starts the pixel's next sample.  Lanes are refilled
-//   instead of idling behind the longest path of the wave.  Per pixel the
-//   samples run in seed order with the film's exact running-mean and FP16
-//   rounding steps, so the film is bit-identical to spp k_trace_sample
-//   launches; the film is read and written once per job instead of once per
-//   sample.
+//   soon as the path ends, folds it into the pixel's running mean and
+//   starts the pixel's next sample.  Lanes are refilled instead of idling
+//   behind the longest path of the wave.  Per pixel the samples run in seed
+//   order through the same accumulation step, so the film is bit-identical
+//   to spp k_trace_sample launches.  The running means stay in the film
+//   (72 B of traffic per sample) rather than in registers: registers, not
+//   bandwidth, bound this kernel's occupancy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -55,7 +55,6 @@ __device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t
   const PtScene& sc = it.scene;
   const PtFrame& f = it.frame;
   const uint32_t idx = y * f.width + x;
-  pt::PixelAccum acc = pt::load_accum(it.film, idx);
   uint32_t k = 0;
   pt::Rng rng = pt::seed_rng(idx, f.seed);
   pt::v3 ro, rd;
@@ -64,14 +63,13 @@ __device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t
   pt::path_begin(p, ro, rd);
   for (;;) {
     if (!pt::path_step(sc, f, p, rng, st)) {
-      pt::fold_sample(acc, p.radiance, p.guide);
+      pt::accumulate(it.film, idx, p.radiance, p.guide);
       if (++k >= it.spp) break;
       rng = pt::seed_rng(idx, f.seed + k);
       pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
       pt::path_begin(p, ro, rd);
     }
   }
-  pt::store_accum(it.film, idx, acc);
 }
 
 template <class StackT>
@@ -83,8 +81,10 @@ __device__ __forceinline__ void run_tile(const TileItem& it, uint32_t tile, Stac
 
 // Jobs are numbered film by film: film i owns jobs [job_begin[i], job_begin[i+1]).
 // kSingle: one film, passed by value (its fields live in the kernel arguments).
-template <bool kSingle>
-__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_tiles(TileItem single, const TileItem* __restrict__ items,
+// kWaves: the occupancy the register allocation targets (waves per SIMD);
+// more waves hide more memory latency at the price of register spills.
+template <bool kSingle, int kWaves>
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_tiles(TileItem single, const TileItem* __restrict__ items,
                                                          const uint32_t* __restrict__ job_begin, uint32_t n_items,
                                                          uint32_t n_jobs, uint32_t* __restrict__ counter,
                                                          uint32_t* spill, uint32_t spill_depth) {
@@ -152,7 +152,8 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   }
   const uint32_t n_jobs = job_begin[n_items];
   if (n_jobs == 0) return LUCENT_OK;
-  const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 8;
+  const uint32_t waves = (cfg.occupancy >= 2 && cfg.occupancy <= 6) ? cfg.occupancy : 2;
+  const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
   Status s = ensure_spill(ctx, size_t(grid) * kPtBlock * depth * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
@@ -175,12 +176,31 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
   const TileItem* d_items = reinterpret_cast<const TileItem*>(table + items_off);
   const uint32_t* d_begin = reinterpret_cast<const uint32_t*>(table + begin_off);
-  if (n_items == 1)
-    hipLaunchKernelGGL(k_trace_tiles<true>, dim3(grid), dim3(kPtBlock), 0, stream, items[0], nullptr, nullptr, 1u,
-                       n_jobs, counter, static_cast<uint32_t*>(ctx->spill), depth);
-  else
-    hipLaunchKernelGGL(k_trace_tiles<false>, dim3(grid), dim3(kPtBlock), 0, stream, TileItem{}, d_items, d_begin,
-                       n_items, n_jobs, counter, static_cast<uint32_t*>(ctx->spill), depth);
+  uint32_t* spill = static_cast<uint32_t*>(ctx->spill);
+  auto launch = [&](auto single_kernel, auto multi_kernel) {
+    if (n_items == 1)
+      hipLaunchKernelGGL(single_kernel, dim3(grid), dim3(kPtBlock), 0, stream, items[0], nullptr, nullptr, 1u, n_jobs,
+                         counter, spill, depth);
+    else
+      hipLaunchKernelGGL(multi_kernel, dim3(grid), dim3(kPtBlock), 0, stream, TileItem{}, d_items, d_begin, n_items,
+                         n_jobs, counter, spill, depth);
+  };
+  switch (waves) {
+    case 3:
+      launch(k_trace_tiles<true, 3>, k_trace_tiles<false, 3>);
+      break;
+    case 4:
+      launch(k_trace_tiles<true, 4>, k_trace_tiles<false, 4>);
+      break;
+    case 5:
+      launch(k_trace_tiles<true, 5>, k_trace_tiles<false, 5>);
+      break;
+    case 6:
+      launch(k_trace_tiles<true, 6>, k_trace_tiles<false, 6>);
+      break;
+    default:
+      launch(k_trace_tiles<true, 2>, k_trace_tiles<false, 2>);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "k_trace_tiles launch");
   return LUCENT_OK;

@@ -27,7 +27,8 @@ Status launch_trace_sample(Context* ctx, const PtScene& sc, const PtFrame& f, co
 
 // Launch shape of the persistent tile kernel (0 = default).
 struct TileLaunch {
-  uint32_t waves_per_cu = 0;
+  uint32_t waves_per_cu = 0;  // persistent waves per CU (0: 4 x occupancy)
+  uint32_t occupancy = 0;     // register-allocation target in waves per SIMD: 2, 3 or 4 (0: 2)
 };
 uint32_t tile_jobs(uint32_t width, uint32_t height);
 // Runs items[i].spp samples of every pixel of each item's film, seeds from

@@ -68,7 +68,7 @@ class DiffRenderer:
         h, w = self.height, self.width
         out = torch.empty(h * w * 4, dtype=torch.float32, device=torch.device("cuda:0"))
         cam = camera_struct(self.scene.camera)
-        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0)
+        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0, 0)
         torch.cuda.synchronize()
         _native.check(_native.lib().lucent_detached_forward(
             self.ctx.handle, self.renderer.device_scene.handle, C.byref(cam), C.byref(st), w, h, spp, seed0,
@@ -89,7 +89,7 @@ class DiffRenderer:
         ew, eh = self.env_size
         g_env = torch.zeros(ew * eh * 3, dtype=torch.float64, device=dev) if env else None
         cam = camera_struct(self.scene.camera)
-        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0)
+        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0, 0)
         torch.cuda.synchronize()
         _native.check(_native.lib().lucent_backward(
             self.ctx.handle, self.renderer.device_scene.handle, C.byref(cam), C.byref(st), w, h, spp, seed0,

@@ -28,11 +28,12 @@ class RenderSettings:
     firefly_clamp: float = 8.0
     per_sample_launch: bool = False
     waves_per_cu: int = 0
+    occupancy: int = 0
 
     def struct(self) -> "_native.RenderSettingsStruct":
         return _native.RenderSettingsStruct(self.max_bounces, self.firefly_clamp,
                                             _native.TRACE_PER_SAMPLE if self.per_sample_launch else 0,
-                                            self.waves_per_cu)
+                                            self.waves_per_cu, self.occupancy)
 
 
 def lighting_struct(scene: Scene, keep: list) -> _native.LightingStruct:
@@ -205,11 +206,7 @@ class Renderer:
             self.device_scene.destroy()
 
 
-def trace_batch(ctx: Context, jobs, settings: RenderSettings, stream: Optional[int] = None) -> None:
-    """lucent_trace_batch: `jobs` is a list of (Renderer, samples); every
-    renderer's film gets `samples` more samples, all in one persistent launch.
-    The renderers must share `ctx`; their own settings are ignored in favour
-    of `settings`."""
+def _batch_items(ctx: Context, jobs):
     arr = (_native.BatchItemStruct * max(1, len(jobs)))()
     for i, (r, n) in enumerate(jobs):
         if r.ctx is not ctx:
@@ -218,5 +215,26 @@ def trace_batch(ctx: Context, jobs, settings: RenderSettings, stream: Optional[i
         arr[i].film = r.film.handle
         arr[i].camera = camera_struct(r.scene.camera)
         arr[i].samples = int(n)
+    return arr
+
+
+def autotune(ctx: Context, jobs, settings: RenderSettings, trial_samples: int = 2) -> dict:
+    """lucent_autotune_tiles: times the persistent kernel's occupancy variants
+    on `jobs` (a list of (Renderer, samples)) and sets settings.occupancy to
+    the fastest.  Returns {"occupancy": n, "ms": {variant: ms}}."""
+    best = C.c_uint32()
+    ms = (C.c_float * 5)()
     st = settings.struct()
-    _native.check(_native.lib().lucent_trace_batch(ctx.handle, arr, len(jobs), C.byref(st), stream))
+    _native.check(_native.lib().lucent_autotune_tiles(ctx.handle, _batch_items(ctx, jobs), len(jobs), C.byref(st),
+                                                      trial_samples, C.byref(best), ms))
+    settings.occupancy = best.value
+    return {"occupancy": best.value, "ms": {k + 2: round(float(ms[k]), 3) for k in range(5)}}
+
+
+def trace_batch(ctx: Context, jobs, settings: RenderSettings, stream: Optional[int] = None) -> None:
+    """lucent_trace_batch: `jobs` is a list of (Renderer, samples); every
+    renderer's film gets `samples` more samples, all in one persistent launch.
+    The renderers must share `ctx`; their own settings are ignored in favour
+    of `settings`."""
+    st = settings.struct()
+    _native.check(_native.lib().lucent_trace_batch(ctx.handle, _batch_items(ctx, jobs), len(jobs), C.byref(st), stream))

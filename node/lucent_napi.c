@@ -459,6 +459,7 @@ static napi_value js_trace_samples(napi_env env, napi_callback_info info) {
   rs.firefly_clamp = (float)num(env, prop(env, argv[4], "fireflyClamp"), 8);
   rs.flags = num(env, prop(env, argv[4], "perSampleLaunch"), 0) != 0 ? LUCENT_TRACE_PER_SAMPLE : 0u;
   rs.waves_per_cu = (uint32_t)num(env, prop(env, argv[4], "wavesPerCU"), 0);
+  rs.occupancy = (uint32_t)num(env, prop(env, argv[4], "occupancy"), 0);
   CHECK_LUCENT(lucent_trace_samples(ctx, sh->scene, fh->film, &cam, &rs, (uint32_t)num(env, argv[5], 1), NULL));
   CHECK_LUCENT(lucent_synchronize(ctx));
   return NULL;

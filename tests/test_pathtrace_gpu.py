@@ -172,18 +172,36 @@ def _film_after(ctx, scene, w, h, spp, seed, settings):
     return out, info
 
 
-@pytest.mark.parametrize("clamp", [0.0, 8.0])
-def test_persistent_tiles_bit_identical_to_per_sample_launches(ctx, clamp):
+@pytest.mark.parametrize("clamp,occupancy", [(0.0, 2), (8.0, 2), (0.0, 4), (0.0, 6)])
+def test_persistent_tiles_bit_identical_to_per_sample_launches(ctx, clamp, occupancy):
     """The persistent tile kernel (all samples of a pixel in one lane, refilled
-    as paths end, running means in registers) and the reference's structure
-    (one launch per sample) give the same film bit for bit, including ragged
-    edge tiles (70x45 is not a multiple of 8)."""
+    as paths end), in each register/occupancy variant, and the reference's
+    structure (one launch per sample) give the same film bit for bit,
+    including ragged edge tiles (70x45 is not a multiple of 8)."""
     sc = feature_scene()
     a, ia = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, clamp, per_sample_launch=True))
-    b, ib = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, clamp))
+    b, ib = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, clamp, occupancy=occupancy))
     assert ia == ib
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_autotune_picks_a_variant_and_leaves_films_alone(ctx):
+    from lucent_amd.renderer import autotune
+    r = Renderer(ctx, 48, 40, RenderSettings(8, 0.0))
+    r.load_scene(feature_scene())
+    r.trace_samples(2)
+    before = r.film.read()
+    info = r.film.info()
+    st = RenderSettings(8, 0.0)
+    res = autotune(ctx, [(r, 8)], st, trial_samples=2)
+    assert st.occupancy == res["occupancy"] and 2 <= res["occupancy"] <= 6
+    assert all(v > 0 for v in res["ms"].values())
+    after = r.film.read()
+    assert r.film.info() == info
+    for k in before:
+        assert np.array_equal(before[k], after[k])
+    r.close()
 
 
 def test_batch_equals_individual_films(ctx):
