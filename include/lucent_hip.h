@@ -268,6 +268,11 @@ typedef struct lucent_camera {
  * LUCENT_TRACE_PER_SAMPLE: one launch over the film per sample, as the
  * reference's traceSamples draws one full-screen pass per sample. */
 #define LUCENT_TRACE_PER_SAMPLE 0x1u
+/* Wavefront launch structure: separate closest-hit, shadow and shading passes
+ * over a queue of live paths (one per pixel), the traversal passes persistent
+ * with idle lanes refilled from the queue.  Same film bit for bit.  Scenes
+ * with alpha-blended materials use the default structure. */
+#define LUCENT_TRACE_WAVEFRONT 0x2u
 
 typedef struct lucent_render_settings {
   uint32_t max_bounces;

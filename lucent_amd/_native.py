@@ -93,6 +93,7 @@ class CameraStruct(C.Structure):
 
 
 TRACE_PER_SAMPLE = 0x1
+TRACE_WAVEFRONT = 0x2
 
 
 class RenderSettingsStruct(C.Structure):

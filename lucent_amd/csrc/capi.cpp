@@ -79,6 +79,9 @@ static Status grow(void** buf, size_t* have, size_t bytes, void* stream) {
 
 Status ensure_scratch(Context* ctx, size_t bytes) { return grow(&ctx->scratch, &ctx->scratch_bytes, bytes, ctx->stream); }
 Status ensure_spill(Context* ctx, size_t bytes) { return grow(&ctx->spill, &ctx->spill_bytes, bytes, ctx->stream); }
+Status ensure_wavefront(Context* ctx, size_t bytes) {
+  return grow(&ctx->wavefront, &ctx->wavefront_bytes, bytes, ctx->stream);
+}
 Status ensure_tile_table(Context* ctx, size_t bytes) {
   return grow(&ctx->tile_table, &ctx->tile_table_bytes, bytes, ctx->stream);
 }
@@ -207,6 +210,7 @@ void lucent_destroy(lucent_ctx* ctx) {
   if (ctx->spill) hipFree(ctx->spill);
   if (ctx->d_flags) hipFree(ctx->d_flags);
   if (ctx->tile_table) hipFree(ctx->tile_table);
+  if (ctx->wavefront) hipFree(ctx->wavefront);
   if (ctx->tile_event) hipEventDestroy(static_cast<hipEvent_t>(ctx->tile_event));
   hipStreamDestroy(static_cast<hipStream_t>(ctx->stream));
   delete ctx;
@@ -314,6 +318,7 @@ lucent_status lucent_scene_upload(lucent_ctx* ctx, const lucent_packed_scene* ps
       return s;
     }
     pt.materials = m;
+    sc->blend_alpha = any_blend_alpha(ps->materials, ps->material_count);
   }
   pt.material_count = ps->material_count;
   if (ps->sphere_count) {

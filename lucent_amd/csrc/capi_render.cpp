@@ -12,6 +12,7 @@
 #include "denoise.hpp"
 #include "render_host.hpp"
 #include "scene.hpp"
+#include "wavefront.hpp"
 
 using namespace lucent;
 
@@ -343,6 +344,14 @@ lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_
   it.frame.seed = film->seed;
   it.film = film->f;
   it.spp = count;
+  if ((settings->flags & LUCENT_TRACE_WAVEFRONT) && !scene->blend_alpha) {
+    WavefrontLaunch wl;
+    wl.waves_per_cu = settings->waves_per_cu;
+    if ((s = launch_wavefront(ctx, it, wl, st)) != LUCENT_OK) return s;
+    film->seed += count;
+    film->samples += count;
+    return LUCENT_OK;
+  }
   TileLaunch cfg;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
@@ -566,6 +575,7 @@ lucent_status lucent_scene_set_materials(lucent_ctx* ctx, lucent_scene* scene, c
   if ((s = dev_upload(const_cast<float*>(scene->pt.materials), materials, size_t(n) * 40 * sizeof(float), ctx->stream)) !=
       LUCENT_OK)
     return s;
+  scene->blend_alpha = any_blend_alpha(materials, n);
   return stream_sync(ctx->stream);
 }
 

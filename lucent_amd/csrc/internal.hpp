@@ -64,6 +64,9 @@ struct Context {
   void* tile_table = nullptr;
   size_t tile_table_bytes = 0;
   void* tile_event = nullptr;   // hipEvent_t: the table upload has completed
+  // Wavefront path tracer: path state and queues.
+  void* wavefront = nullptr;
+  size_t wavefront_bytes = 0;
   uint32_t* d_flags = nullptr;  // device status word(s)
   float* sheen_lut = nullptr;   // 16x16 half-rounded sheen albedo table
 };
@@ -85,6 +88,7 @@ Status stream_sync(void* stream);
 Status ensure_scratch(Context* ctx, size_t bytes);
 Status ensure_spill(Context* ctx, size_t bytes);
 Status ensure_tile_table(Context* ctx, size_t bytes);
+Status ensure_wavefront(Context* ctx, size_t bytes);
 
 // Kernel launchers (traverse.hip).
 struct RayQuery {

@@ -461,39 +461,45 @@ __device__ inline bool occluded(const PtScene& s, v3 ro, v3 rd, float tmax, Rng&
   }
 }
 
-// traceScene (scene.glsl:300-441): spheres, ground, then triangles.
-template <class StackT>
-__device__ inline bool trace_scene(const PtScene& s, v3 ro, v3 rd, Hit& hit, Rng& rng, StackT& st) {
-  float closest = INF;
-  int best_sphere = -1;
-  bool ground = false;
+// The closest primitive found by traceScene's search (scene.glsl:300-441):
+// kind 0 = miss, 1 = triangle (prim = leaf slot), 2 = sphere, 3 = ground.
+enum : uint32_t { kHitMiss = 0, kHitTri = 1, kHitSphere = 2, kHitGround = 3 };
+struct HitRecord {
+  float t, u, v;
+  uint32_t kind, prim;
+};
+
+// Spheres, then the ground, with t < closest (scene.glsl:307-320).
+__device__ inline void search_analytic(const PtScene& s, v3 ro, v3 rd, HitRecord& r) {
+  r.t = INF;
+  r.kind = kHitMiss;
+  r.prim = 0;
+  r.u = r.v = 0.0f;
   for (uint32_t i = 0; i < s.sphere_count; ++i) {
-    const float t = hit_sphere(ro, rd, reinterpret_cast<const float4*>(s.spheres)[i], closest);
-    if (t < closest) {
-      closest = t;
-      best_sphere = int(i);
+    const float t = hit_sphere(ro, rd, reinterpret_cast<const float4*>(s.spheres)[i], r.t);
+    if (t < r.t) {
+      r.t = t;
+      r.kind = kHitSphere;
+      r.prim = i;
     }
   }
-  const float tg = hit_ground(s, ro, rd, closest);
-  if (tg < closest) {
-    closest = tg;
-    best_sphere = -1;
-    ground = true;
+  const float tg = hit_ground(s, ro, rd, r.t);
+  if (tg < r.t) {
+    r.t = tg;
+    r.kind = kHitGround;
   }
-  float bu = 0.0f, bv = 0.0f;
-  const int best_tri = trace_bvh(s, ro, rd, closest, bu, bv, rng, st);
-  if (best_tri >= 0) {
-    best_sphere = -1;
-    ground = false;
-  }
-  if (best_tri < 0 && best_sphere < 0 && !ground) return false;
+}
 
+// The hit record of traceScene from its search result.
+__device__ inline bool build_hit(const PtScene& s, v3 ro, v3 rd, const HitRecord& r, Hit& hit) {
+  if (r.kind == kHitMiss) return false;
+  const float closest = r.t;
   hit.t = closest;
   hit.position = ro + rd * closest;
   hit.uv_u = hit.uv_v = 0.0f;
   hit.dpdu = hit.dpdv = mk(0.0f, 0.0f, 0.0f);
   hit.light = -1;
-  if (ground) {
+  if (r.kind == kHitGround) {
     hit.front_face = rd.y < 0.0f;
     hit.normal = mk(0.0f, hit.front_face ? 1.0f : -1.0f, 0.0f);
     hit.geom_normal = hit.normal;
@@ -502,17 +508,18 @@ __device__ inline bool trace_scene(const PtScene& s, v3 ro, v3 rd, Hit& hit, Rng
     hit.uv_v = hit.position.z;
     return true;
   }
-  if (best_sphere >= 0) {
-    const float4 sp = reinterpret_cast<const float4*>(s.spheres)[best_sphere];
+  if (r.kind == kHitSphere) {
+    const float4 sp = reinterpret_cast<const float4*>(s.spheres)[r.prim];
     const v3 n = (hit.position - mk(sp)) / sp.w;
     hit.front_face = dot(rd, n) < 0.0f;
     hit.normal = hit.front_face ? n : -n;
     hit.geom_normal = hit.normal;
-    hit.material = s.sphere_material[best_sphere];
-    hit.light = s.sphere_light[best_sphere];
+    hit.material = s.sphere_material[r.prim];
+    hit.light = s.sphere_light[r.prim];
     return true;
   }
-  const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris) + size_t(best_tri) * 3;
+  const float bu = r.u, bv = r.v;
+  const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris) + size_t(r.prim) * 3;
   const float4 p0 = tris[0], p1 = tris[1], p2 = tris[2];
   v3 n0 = mk(0.f, 0.f, 0.f), n1 = n0, n2 = n0;
   float u0 = 0, v0 = 0, u1 = 0, v1 = 0, u2 = 0, v2 = 0;
@@ -521,7 +528,7 @@ __device__ inline bool trace_scene(const PtScene& s, v3 ro, v3 rd, Hit& hit, Rng
   v3 ns;
   const float w = 1.0f - bu - bv;
   if (s.bvh.attrs) {
-    const float4* a = reinterpret_cast<const float4*>(s.bvh.attrs) + size_t(best_tri) * 4;
+    const float4* a = reinterpret_cast<const float4*>(s.bvh.attrs) + size_t(r.prim) * 4;
     const float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
     n0 = mk(a0);
     n1 = mk(a1);
@@ -537,9 +544,9 @@ __device__ inline bool trace_scene(const PtScene& s, v3 ro, v3 rd, Hit& hit, Rng
     const float d1x = u1 - u0, d1y = v1 - v0, d2x = u2 - u0, d2y = v2 - v0;
     const float det = d1x * d2y - d1y * d2x;
     if (fabsf(det) > 1e-12f) {
-      const float r = 1.0f / det;
-      hit.dpdu = (e1 * d2y - e2 * d1y) * r;
-      hit.dpdv = (e2 * d1x - e1 * d2x) * r;
+      const float rr = 1.0f / det;
+      hit.dpdu = (e1 * d2y - e2 * d1y) * rr;
+      hit.dpdv = (e2 * d1x - e1 * d2x) * rr;
     }
     ns = normalize(n0 * w + n1 * bu + n2 * bv);
   } else {
@@ -556,6 +563,24 @@ __device__ inline bool trace_scene(const PtScene& s, v3 ro, v3 rd, Hit& hit, Rng
   hit.material = __float_as_int(p0.w);
   hit.light = __float_as_int(p1.w);
   return true;
+}
+
+// traceScene (scene.glsl:300-441): spheres, ground, then triangles.
+template <class StackT>
+__device__ inline bool trace_scene(const PtScene& s, v3 ro, v3 rd, Hit& hit, Rng& rng, StackT& st) {
+  HitRecord r;
+  search_analytic(s, ro, rd, r);
+  float bu = 0.0f, bv = 0.0f;
+  float closest = r.t;
+  const int best_tri = trace_bvh(s, ro, rd, closest, bu, bv, rng, st);
+  if (best_tri >= 0) {
+    r.t = closest;
+    r.kind = kHitTri;
+    r.prim = uint32_t(best_tri);
+    r.u = bu;
+    r.v = bv;
+  }
+  return build_hit(s, ro, rd, r, hit);
 }
 
 // ---------------------------------------------------------------------------
@@ -1054,11 +1079,30 @@ __device__ __forceinline__ void path_begin(PathState& p, v3 ro, v3 rd) {
 
 // One iteration of tracePath's bounce loop; false once the path has ended
 // (p.radiance and p.guide are then final).
+// Rays of a path step are cast through a tracer policy:
+//   bool closest(v3 ro, v3 rd, Hit& hit, Rng& rng)  - traceScene;
+//   void nee(int which, v3 o, v3 d, float tmax, Rng& rng, PathState& p, F contribution)
+//     - a shadow ray for light sample `which` (0 sun, 1 environment, 2 light)
+//       whose unoccluded contribution(), added to p.radiance, is the term.
+// InlineTracer traces both on the spot (the megakernels); the wavefront's
+// tracer reads the closest hit from its extend pass and defers the shadow
+// rays, adding their contributions in the same order afterwards.
 template <class StackT>
-__device__ __forceinline__ bool path_step(const PtScene& sc, const PtFrame& f, PathState& p, Rng& rng, StackT& st) {
+struct InlineTracer {
+  const PtScene& sc;
+  StackT& st;
+  __device__ __forceinline__ bool closest(v3 ro, v3 rd, Hit& hit, Rng& rng) { return trace_scene(sc, ro, rd, hit, rng, st); }
+  template <class F>
+  __device__ __forceinline__ void nee(int, v3 o, v3 d, float tmax, Rng& rng, PathState& p, F contribution) {
+    if (!occluded(sc, o, d, tmax, rng, st)) p.radiance = p.radiance + contribution();
+  }
+};
+
+template <class Tracer>
+__device__ __forceinline__ bool path_step_with(const PtScene& sc, const PtFrame& f, PathState& p, Rng& rng, Tracer& tr) {
   const int bounce = p.bounce;
   Hit hit;
-  if (!trace_scene(sc, p.ro, p.rd, hit, rng, st)) {
+  if (!tr.closest(p.ro, p.rd, hit, rng)) {
     const v3 env = (bounce == 0 && sc.env_background == 0) ? mk(sc.backdrop) : environment(sc, p.rd, p.specular, p.bsdf_pdf);
     p.radiance = p.radiance + clamp_contribution(p.throughput * env, bounce, f.clamp);
     if (!p.guide.done) {
@@ -1115,36 +1159,38 @@ __device__ __forceinline__ bool path_step(const PtScene& sc, const PtFrame& f, P
       const float u0 = rng.rand(), u1 = rng.rand();
       const v3 wi = sample_sun_direction(sc, u0, u1);
       const float ndl = dot(n, wi);
-      if (ndl > 0.0f && dot(ng, wi) > 0.0f && !occluded(sc, shadow_origin, wi, INF, rng, st)) {
-        float pb;
-        const v3 fv = eval_opaque(sc, s, wo, wi, pb);
-        const float w = mis_weight(sc.sun_pdf, pb);
-        p.radiance = p.radiance + clamp_contribution(p.throughput * fv * ndl * mk(sc.sun_radiance) * (w / sc.sun_pdf), bounce, f.clamp);
-      }
+      if (ndl > 0.0f && dot(ng, wi) > 0.0f)
+        tr.nee(0, shadow_origin, wi, INF, rng, p, [&]() {
+          float pb;
+          const v3 fv = eval_opaque(sc, s, wo, wi, pb);
+          const float w = mis_weight(sc.sun_pdf, pb);
+          return clamp_contribution(p.throughput * fv * ndl * mk(sc.sun_radiance) * (w / sc.sun_pdf), bounce, f.clamp);
+        });
     }
     if (sc.env_sampling != 0) {
       const LightSample ls = sample_environment(sc, rng);
       const float ndl = dot(n, ls.dir);
-      if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f && !occluded(sc, shadow_origin, ls.dir, INF, rng, st)) {
-        float pb;
-        const v3 fv = eval_opaque(sc, s, wo, ls.dir, pb);
-        const float w = mis_weight(ls.pdf, pb);
-        p.radiance = p.radiance + clamp_contribution(p.throughput * fv * ndl * ls.radiance * (w / ls.pdf), bounce, f.clamp);
-      }
+      if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f)
+        tr.nee(1, shadow_origin, ls.dir, INF, rng, p, [&]() {
+          float pb;
+          const v3 fv = eval_opaque(sc, s, wo, ls.dir, pb);
+          const float w = mis_weight(ls.pdf, pb);
+          return clamp_contribution(p.throughput * fv * ndl * ls.radiance * (w / ls.pdf), bounce, f.clamp);
+        });
     }
     if (sc.light_count > 0) {
       const int li = min(int(rng.rand() * float(sc.light_count)), int(sc.light_count) - 1);
       const float u0 = rng.rand(), u1 = rng.rand();
       const LightSample ls = sample_light(sc, li, hit.position, u0, u1);
       const float ndl = dot(n, ls.dir);
-      if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f &&
-          !occluded(sc, shadow_origin, ls.dir, ls.dist * (1.0f - 1e-4f), rng, st)) {
-        const float pl = ls.pdf / float(sc.light_count);
-        float pb;
-        const v3 fv = eval_opaque(sc, s, wo, ls.dir, pb);
-        const float w = mis_weight(pl, pb);
-        p.radiance = p.radiance + clamp_contribution(p.throughput * fv * ndl * ls.radiance * (w / pl), bounce, f.clamp);
-      }
+      if (ls.pdf > 0.0f && ndl > 0.0f && dot(ng, ls.dir) > 0.0f)
+        tr.nee(2, shadow_origin, ls.dir, ls.dist * (1.0f - 1e-4f), rng, p, [&]() {
+          const float pl = ls.pdf / float(sc.light_count);
+          float pb;
+          const v3 fv = eval_opaque(sc, s, wo, ls.dir, pb);
+          const float w = mis_weight(pl, pb);
+          return clamp_contribution(p.throughput * fv * ndl * ls.radiance * (w / pl), bounce, f.clamp);
+        });
     }
     const BsdfSample bs = sample_opaque(sc, s, wo, rng);
     if (bs.pdf <= 0.0f || dot(bs.dir, ng) <= 0.0f) return false;
@@ -1161,6 +1207,12 @@ __device__ __forceinline__ bool path_step(const PtScene& sc, const PtFrame& f, P
   }
   p.bounce = bounce + 1;
   return p.bounce <= int(f.max_bounces);
+}
+
+template <class StackT>
+__device__ __forceinline__ bool path_step(const PtScene& sc, const PtFrame& f, PathState& p, Rng& rng, StackT& st) {
+  InlineTracer<StackT> tr{sc, st};
+  return path_step_with(sc, f, p, rng, tr);
 }
 
 template <class StackT>

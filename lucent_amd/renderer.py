@@ -29,11 +29,13 @@ class RenderSettings:
     per_sample_launch: bool = False
     waves_per_cu: int = 0
     occupancy: int = 0
+    wavefront: bool = False
 
     def struct(self) -> "_native.RenderSettingsStruct":
-        return _native.RenderSettingsStruct(self.max_bounces, self.firefly_clamp,
-                                            _native.TRACE_PER_SAMPLE if self.per_sample_launch else 0,
-                                            self.waves_per_cu, self.occupancy)
+        flags = (_native.TRACE_PER_SAMPLE if self.per_sample_launch else 0) | \
+                (_native.TRACE_WAVEFRONT if self.wavefront else 0)
+        return _native.RenderSettingsStruct(self.max_bounces, self.firefly_clamp, flags, self.waves_per_cu,
+                                            self.occupancy)
 
 
 def lighting_struct(scene: Scene, keep: list) -> _native.LightingStruct:

@@ -286,3 +286,27 @@ def test_batch_with_alpha_scene_bit_identical(ctx):
         for k in want:
             assert np.array_equal(got[k], want[k]), k
         r.close()
+
+
+@pytest.mark.parametrize("clamp", [0.0, 8.0])
+def test_wavefront_bit_identical_to_per_sample_launches(ctx, clamp):
+    """The wavefront structure (separate closest-hit, shadow and shading
+    passes over a path queue, persistent traversal with lane refill) gives
+    the per-sample launches' film bit for bit: same traversal order, same
+    shading code, same random numbers."""
+    sc = feature_scene()
+    a, ia = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, clamp, per_sample_launch=True))
+    b, ib = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, clamp, wavefront=True))
+    assert ia == ib
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("which", ["alpha", "knots"])
+def test_wavefront_bit_identical_on_alpha_and_benchmark_scenes(ctx, which):
+    sc = alpha_scene() if which == "alpha" else workloads.knot_scene()
+    w, h, spp = (40, 24, 6) if which == "alpha" else (144, 90, 3)
+    a, _ = _film_after(ctx, sc, w, h, spp, 2, RenderSettings(8, 0.0, per_sample_launch=True))
+    b, _ = _film_after(ctx, sc, w, h, spp, 2, RenderSettings(8, 0.0, wavefront=True))
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
