@@ -32,7 +32,6 @@ namespace {
 using namespace pt;
 
 constexpr int kAdjBlock = 64;
-constexpr int kAdjLds = 16;
 constexpr int kTile = 8;
 constexpr int kNumParams = 5;        // albedo r, g, b, roughness, metallic
 constexpr int kLdsMaterials = 64;    // materials whose gradients are pre-reduced in LDS
@@ -471,7 +470,7 @@ __device__ inline v3 detached_path(const PtScene& sc, const PtScene& sv, const P
 __global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, PtFrame f, float inv_spp, int first,
                                                         float* __restrict__ out, uint32_t* spill, uint32_t spill_depth,
                                                         uint32_t tiles_x) {
-  __shared__ uint32_t lds[kAdjLds * kAdjBlock];
+  __shared__ uint32_t lds[kPtStackWords * kAdjBlock];
   const uint32_t tile = blockIdx.x;
   const uint32_t x = (tile % tiles_x) * kTile + (threadIdx.x & (kTile - 1));
   const uint32_t y = (tile / tiles_x) * kTile + (threadIdx.x / kTile);
@@ -480,7 +479,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, 
   Rng rng = seed_rng(idx, f.seed);
   v3 ro, rd;
   camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-  Stack<kAdjLds, kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0,
+  PtStack<kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0,
                                0};
   v3 L = detached_path(sc, sv, f, ro, rd, rng, st);
   if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z)) L = mk(0.f, 0.f, 0.f);
@@ -494,7 +493,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(PtScene sc, PtFrame f, co
                                                        float* __restrict__ env_acc, size_t env_stride,
                                                        float* __restrict__ mat_overflow, uint32_t* spill,
                                                        uint32_t spill_depth, uint32_t tiles_x) {
-  __shared__ uint32_t lds[kAdjLds * kAdjBlock];
+  __shared__ uint32_t lds[kPtStackWords * kAdjBlock];
   __shared__ float lds_grad[kLdsMaterials * kNumParams];
   for (int i = threadIdx.x; i < kLdsMaterials * kNumParams; i += kAdjBlock) lds_grad[i] = 0.0f;
   __syncthreads();
@@ -509,7 +508,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(PtScene sc, PtFrame f, co
       Rng rng = seed_rng(idx, f.seed);
       v3 ro, rd;
       camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-      Stack<kAdjLds, kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth,
+      PtStack<kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth,
                                    0, 0};
       Rng replay_rng = rng;
       Guide gd;
@@ -576,7 +575,7 @@ Status launch_detached(Context* ctx, const DetachedArgs& a, void* stream_) {
   const uint32_t tiles_x = (a.frame.width + kTile - 1) / kTile;
   const uint32_t tiles_y = (a.frame.height + kTile - 1) / kTile;
   const uint32_t blocks = tiles_x * tiles_y;
-  const uint32_t depth = a.scene->bvh.stack_depth + 1;
+  const uint32_t depth = pt_spill_words(a.scene->bvh);
   Status s = ensure_spill(ctx, size_t(blocks) * kAdjBlock * depth * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
   PtScene sv = *a.scene;
@@ -606,7 +605,7 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
   float* mat_partials = base;
   float* mat_overflow = base + partials;
   float* env_acc = mat_overflow + overflow;
-  const uint32_t depth = sc.bvh.stack_depth + 1;
+  const uint32_t depth = pt_spill_words(sc.bvh);
   Status s = ensure_spill(ctx, size_t(blocks) * kAdjBlock * depth * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
   const size_t zero_n = overflow + kEnvReplicas * env_stride;

@@ -129,8 +129,17 @@ Status finish_scene(Context* ctx, Scene* sc, const float* nodes, uint32_t node_c
   sc->stats.node_count = node_count;
   sc->stats.pair_count = img.pair_count;
   sc->stats.depth = img.depth;
+  HostBvh4Image img4;
+  if ((s = convert_bvh4(nodes, node_count, sc->bvh.tri_count, &img4)) != LUCENT_OK) return s;
+  sc->bvh.node4_count = img4.node_count;
+  sc->bvh.root4 = img4.root;
+  // Each level pushes at most 3 entries beyond the one it pops.
+  sc->bvh.stack4 = 3 * img4.depth + 2;
   const uint32_t n = sc->bvh.tri_count;
   if ((s = upload_owned(sc, img.pairs.data(), img.pairs.size() * sizeof(float), &sc->bvh.pairs)) != LUCENT_OK) return s;
+  if (img4.node_count > 0 &&
+      (s = upload_owned(sc, img4.nodes.data(), img4.nodes.size() * sizeof(float), &sc->bvh.nodes4)) != LUCENT_OK)
+    return s;
   if ((s = upload_owned(sc, tris.data(), tris.size() * sizeof(float), &sc->bvh.tris)) != LUCENT_OK) return s;
   if (attrs) {
     if ((s = upload_owned(sc, attrs, size_t(n) * 16 * sizeof(float), &sc->bvh.attrs)) != LUCENT_OK) return s;

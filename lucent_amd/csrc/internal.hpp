@@ -46,7 +46,13 @@ struct DevBvh {
   uint32_t root_link = 0;   // pair index, or first slot when the root is a leaf
   uint32_t root_count = 0;  // > 0 when the root itself is a leaf
   uint32_t stack_depth = 0; // entries a traversal stack may need
-  uint32_t pad = 0;
+  // 4-wide nodes of the path tracer: 16 floats per node (frame origin, scale,
+  // 8-bit child codes per axis, 4 links); a link is a node index, a leaf
+  // (bit 31 | (count - 1) << 26 | first slot) or 0xFFFFFFFF for no child.
+  const float* nodes4 = nullptr;
+  uint32_t node4_count = 0;
+  uint32_t root4 = 0xFFFFFFFFu;   // link of the root (a node or a leaf)
+  uint32_t stack4 = 1;            // (link, distance) entries a 4-wide traversal may need
 };
 
 struct Context {
@@ -79,6 +85,13 @@ struct HostBvhImage {
   uint32_t depth = 0;
 };
 Status convert_bvh(const float* nodes, uint32_t node_count, uint32_t tri_count, HostBvhImage* out);
+struct HostBvh4Image {
+  std::vector<float> nodes;  // 16 floats per node
+  uint32_t node_count = 0;
+  uint32_t root = 0xFFFFFFFFu;
+  uint32_t depth = 0;        // levels of 4-wide nodes
+};
+Status convert_bvh4(const float* nodes, uint32_t node_count, uint32_t tri_count, HostBvh4Image* out);
 
 // Device memory helpers (implemented over the HIP runtime).
 Status dev_alloc(void** p, size_t bytes);

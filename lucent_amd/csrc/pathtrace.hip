@@ -29,12 +29,11 @@ namespace lucent {
 namespace {
 
 constexpr int kPtBlock = 64;
-constexpr int kPtLds = 16;
 constexpr int kTile = 8;
 
 __global__ __launch_bounds__(kPtBlock) void k_trace_sample(PtScene sc, PtFrame f, PtFilm film, uint32_t* spill,
                                                           uint32_t spill_depth, uint32_t tiles_x) {
-  __shared__ uint32_t lds[kPtLds * kPtBlock];
+  __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
   const uint32_t tile = blockIdx.x;
   const uint32_t x = (tile % tiles_x) * kTile + (threadIdx.x & (kTile - 1));
   const uint32_t y = (tile / tiles_x) * kTile + (threadIdx.x / kTile);
@@ -43,7 +42,7 @@ __global__ __launch_bounds__(kPtBlock) void k_trace_sample(PtScene sc, PtFrame f
   pt::Rng rng = pt::seed_rng(idx, f.seed);
   pt::v3 ro, rd;
   pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-  pt::Stack<kPtLds, kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
+  pt::PtStack<kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
                                  0, 0};
   pt::Guide g;
   const pt::v3 c = pt::trace_path(sc, f, ro, rd, rng, st, g);
@@ -88,9 +87,9 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
                                                          const uint32_t* __restrict__ job_begin, uint32_t n_items,
                                                          uint32_t n_jobs, uint32_t* __restrict__ counter,
                                                          uint32_t* spill, uint32_t spill_depth) {
-  __shared__ uint32_t lds[kPtLds * kPtBlock];
+  __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
   __shared__ uint32_t next_job;
-  pt::Stack<kPtLds, kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
+  pt::PtStack<kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
                                  0, 0};
   for (;;) {
     if (threadIdx.x == 0) next_job = atomicAdd(counter, 1u);
@@ -125,7 +124,7 @@ Status launch_trace_sample(Context* ctx, const PtScene& sc, const PtFrame& f, co
   const uint32_t tiles_y = (f.height + kTile - 1) / kTile;
   const uint64_t blocks = uint64_t(tiles_x) * tiles_y;
   if (blocks == 0) return LUCENT_OK;
-  const uint32_t depth = sc.bvh.stack_depth + 1;
+  const uint32_t depth = pt::pt_spill_words(sc.bvh);
   Status s = ensure_spill(ctx, size_t(blocks) * kPtBlock * depth * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
   hipLaunchKernelGGL(k_trace_sample, dim3(uint32_t(blocks)), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream), sc, f,
@@ -148,7 +147,7 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
     items[i].tiles_x = (items[i].frame.width + kTile - 1) / kTile;
     const uint32_t jobs = items[i].spp ? tile_jobs(items[i].frame.width, items[i].frame.height) : 0;
     job_begin[i + 1] = job_begin[i] + jobs;
-    depth = std::max(depth, items[i].scene.bvh.stack_depth + 1);
+    depth = std::max(depth, pt::pt_spill_words(items[i].scene.bvh));
   }
   const uint32_t n_jobs = job_begin[n_items];
   if (n_jobs == 0) return LUCENT_OK;

@@ -361,57 +361,185 @@ struct Stack {
   }
 };
 
-// traceScene's triangle part (scene.glsl:321-376): closest accepted hit with
-// t < closest, near child first.  Returns the best slot or -1.
+// ---------------------------------------------------------------------------
+// Traversal of the 4-wide BVH (DevBvh::nodes4).  A stack of (link, entry
+// distance) pairs; one step pops an entry or tests one triangle of the leaf
+// being worked through.  A popped node has its four child boxes slab-tested
+// (hitAABB, scene.glsl:261-269: tn = max(min(t0,t1).xyz, 0), tf = min(max
+// (t0,t1).xyz, tmax), hit iff tf >= tn) and its hit children pushed, farthest
+// first, so the nearest is popped next; entries at or beyond the closest hit
+// so far are dropped when popped (a triangle must be strictly nearer to be
+// accepted).  Shadow rays push hit children unsorted and stop at the first
+// accepted triangle.  The closest hit equals that of any complete traversal
+// (ties between equal distances aside).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kLinkLeafBit = 0x80000000u, kLinkNone = 0xFFFFFFFFu;
+__device__ inline bool link_is_leaf(uint32_t l) { return (l & kLinkLeafBit) != 0u; }
+__device__ inline uint32_t link_first(uint32_t l) { return l & 0x03FFFFFFu; }
+__device__ inline uint32_t link_count(uint32_t l) { return ((l >> 26) & 31u) + 1u; }
+
+// (link, distance) stack: kLds entries per lane in LDS (two lane-major
+// planes), deeper ones spilled to 2 words per entry of a per-lane area.
+template <int kLds, int kBlock>
+struct Stack4 {
+  uint32_t* lds;    // this lane's word of plane 0; plane 1 starts kLds * kBlock words later
+  uint32_t* spill;  // 2 words per entry
+  uint32_t sp, base;
+  __device__ inline void reset() { sp = base = 0; }
+  __device__ inline bool empty() const { return sp == 0; }
+  __device__ inline void push(uint32_t link, float dist) {
+    if (sp - base == kLds) {
+      const uint32_t slot = (base % kLds) * kBlock;
+      spill[2 * base] = lds[slot];
+      spill[2 * base + 1] = lds[kLds * kBlock + slot];
+      ++base;
+    }
+    const uint32_t slot = (sp % kLds) * kBlock;
+    lds[slot] = link;
+    lds[kLds * kBlock + slot] = __float_as_uint(dist);
+    ++sp;
+  }
+  __device__ inline void pop(uint32_t& link, float& dist) {
+    --sp;
+    if (sp >= base) {
+      const uint32_t slot = (sp % kLds) * kBlock;
+      link = lds[slot];
+      dist = __uint_as_float(lds[kLds * kBlock + slot]);
+      return;
+    }
+    --base;
+    link = spill[2 * sp];
+    dist = __uint_as_float(spill[2 * sp + 1]);
+  }
+};
+
+constexpr int kStack4Lds = 12;
+constexpr int kPtStackWords = 2 * kStack4Lds;  // LDS words per lane
+template <int kBlock>
+using PtStack = Stack4<kStack4Lds, kBlock>;
+// Spill words per lane for a scene's 4-wide BVH.
+__host__ __device__ inline uint32_t pt_spill_words(const DevBvh& b) { return 2u * b.stack4; }
+
+// Entry distances of the four children of 4-wide node n (INF: no child or missed).
+__device__ inline void node4_hits(const float* nodes, uint32_t n, v3 o, v3 inv, float tmax, float tn[4], uint32_t link[4]) {
+  const float4* r = reinterpret_cast<const float4*>(nodes) + size_t(n) * 4;
+  const float4 A = r[0], Bq = r[1], Cq = r[2], Lf = r[3];
+  const float sx = A.w, sy = Bq.x, sz = Bq.y;
+  const uint32_t lx = __float_as_uint(Bq.z), hx = __float_as_uint(Bq.w);
+  const uint32_t ly = __float_as_uint(Cq.x), hy = __float_as_uint(Cq.y);
+  const uint32_t lz = __float_as_uint(Cq.z), hz = __float_as_uint(Cq.w);
+  link[0] = __float_as_uint(Lf.x);
+  link[1] = __float_as_uint(Lf.y);
+  link[2] = __float_as_uint(Lf.z);
+  link[3] = __float_as_uint(Lf.w);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int sh = 8 * i;
+    const float bx0 = fmaf(float((lx >> sh) & 255u), sx, A.x), bx1 = fmaf(float((hx >> sh) & 255u), sx, A.x);
+    const float by0 = fmaf(float((ly >> sh) & 255u), sy, A.y), by1 = fmaf(float((hy >> sh) & 255u), sy, A.y);
+    const float bz0 = fmaf(float((lz >> sh) & 255u), sz, A.z), bz1 = fmaf(float((hz >> sh) & 255u), sz, A.z);
+    const float x0 = (bx0 - o.x) * inv.x, x1 = (bx1 - o.x) * inv.x;
+    const float y0 = (by0 - o.y) * inv.y, y1 = (by1 - o.y) * inv.y;
+    const float z0 = (bz0 - o.z) * inv.z, z1 = (bz1 - o.z) * inv.z;
+    const float nn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+    const float ff = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    tn[i] = (link[i] != kLinkNone && ff >= nn) ? nn : INF;
+  }
+}
+
+struct Trav4 {
+  v3 ro, rd, inv;
+  float tmax;  // closest hit so far (closest queries) or the ray's extent (shadow rays)
+  float bu, bv;
+  int best;
+  uint32_t la, na;  // leaf being tested: next slot, triangles left
+};
+
+template <class StackT>
+__device__ inline void trav4_begin(const PtScene& s, Trav4& t, v3 ro, v3 rd, float tmax, StackT& st) {
+  t.ro = ro;
+  t.rd = rd;
+  t.inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+  t.tmax = tmax;
+  t.bu = t.bv = 0.0f;
+  t.best = -1;
+  t.la = t.na = 0;
+  st.reset();
+  if (s.bvh.tri_count > 0) st.push(s.bvh.root4, 0.0f);
+}
+
+template <class StackT>
+__device__ inline bool trav4_done(const Trav4& t, const StackT& st) { return t.na == 0 && st.empty(); }
+
+// One step; for shadow rays (kAny) true on the first accepted triangle.
+template <bool kAny, class StackT>
+__device__ inline bool trav4_step(const PtScene& s, Trav4& t, StackT& st, Rng& rng) {
+  if (t.na > 0) {
+    const uint32_t k = t.la++;
+    --t.na;
+    const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris);
+    const float4 p0 = tris[size_t(k) * 3], p1 = tris[size_t(k) * 3 + 1], p2 = tris[size_t(k) * 3 + 2];
+    float th, u, v;
+    if (hit_triangle(t.ro, t.rd, mk(p0), mk(p1), mk(p2), t.tmax, th, u, v) &&
+        (__float_as_int(p2.w) == 0 || alpha_opaque(s, k, u, v, __float_as_int(p0.w), rng))) {
+      if (kAny) return true;
+      t.tmax = th;
+      t.best = int(k);
+      t.bu = u;
+      t.bv = v;
+    }
+    return false;
+  }
+  uint32_t link;
+  float dist;
+  st.pop(link, dist);
+  if (!kAny && dist >= t.tmax) return false;
+  if (link_is_leaf(link)) {
+    t.la = link_first(link);
+    t.na = link_count(link);
+    return false;
+  }
+  float tn[4];
+  uint32_t l[4];
+  node4_hits(s.bvh.nodes4, link, t.ro, t.inv, t.tmax, tn, l);
+  if (!kAny) {
+    // Farthest first: the nearest child ends on top.
+    auto cs = [&](int a, int b) {
+      if (tn[a] < tn[b]) {
+        const float td = tn[a];
+        tn[a] = tn[b];
+        tn[b] = td;
+        const uint32_t ld = l[a];
+        l[a] = l[b];
+        l[b] = ld;
+      }
+    };
+    cs(0, 1);
+    cs(2, 3);
+    cs(0, 2);
+    cs(1, 3);
+    cs(1, 2);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (tn[i] != INF) st.push(l[i], tn[i]);
+  return false;
+}
+
+// traceScene's triangle part (scene.glsl:321-376): the closest accepted hit
+// with t < closest.  Returns the best slot or -1.
 template <class StackT>
 __device__ inline int trace_bvh(const PtScene& s, v3 ro, v3 rd, float& closest, float& bu, float& bv, Rng& rng,
                                 StackT& st) {
-  const DevBvh& b = s.bvh;
-  int best = -1;
-  if (b.tri_count == 0) return -1;
-  const float4* tris = reinterpret_cast<const float4*>(b.tris);
-  const v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-  auto leaf = [&](uint32_t first, uint32_t count) {
-    for (uint32_t k = first; k < first + count; ++k) {
-      const float4 a = tris[size_t(k) * 3], c1 = tris[size_t(k) * 3 + 1], c2 = tris[size_t(k) * 3 + 2];
-      float t, u, v;
-      if (hit_triangle(ro, rd, mk(a), mk(c1), mk(c2), closest, t, u, v) &&
-          (__float_as_int(c2.w) == 0 || alpha_opaque(s, k, u, v, __float_as_int(a.w), rng))) {
-        closest = t;
-        best = int(k);
-        bu = u;
-        bv = v;
-      }
-    }
-  };
-  st.reset();
-  if (b.root_count > 0) {
-    leaf(b.root_link, b.root_count);
-    return best;
+  Trav4 t;
+  trav4_begin(s, t, ro, rd, closest, st);
+  while (!trav4_done(t, st)) trav4_step<false>(s, t, st, rng);
+  if (t.best >= 0) {
+    closest = t.tmax;
+    bu = t.bu;
+    bv = t.bv;
   }
-  uint32_t cur = b.root_link;
-  while (true) {
-    const PairRec r = load_pair_rec(b.pairs, cur);
-    float t0, t1;
-    slab2(r, ro, inv, closest, t0, t1);
-    const bool h0 = t0 != INF, h1 = t1 != INF;
-    if (h0 && r.c0 > 0) leaf(r.l0, r.c0);
-    if (h1 && r.c1 > 0) leaf(r.l1, r.c1);
-    const bool g0 = h0 && r.c0 == 0, g1 = h1 && r.c1 == 0;
-    if (g0 && g1) {
-      const bool first0 = t0 <= t1;
-      cur = first0 ? r.l0 : r.l1;
-      st.push(first0 ? r.l1 : r.l0);
-    } else if (g0) {
-      cur = r.l0;
-    } else if (g1) {
-      cur = r.l1;
-    } else {
-      if (st.sp == 0) break;
-      cur = st.pop();
-    }
-  }
-  return best;
+  return t.best;
 }
 
 // occluded (scene.glsl:444-485): any accepted hit below tmax.
@@ -422,43 +550,11 @@ __device__ inline bool occluded(const PtScene& s, v3 ro, v3 rd, float tmax, Rng&
     if (hit_sphere(ro, rd, sp, tmax) < tmax) return true;
   }
   if (hit_ground(s, ro, rd, tmax) < tmax) return true;
-  const DevBvh& b = s.bvh;
-  if (b.tri_count == 0) return false;
-  const float4* tris = reinterpret_cast<const float4*>(b.tris);
-  const v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-  auto leaf = [&](uint32_t first, uint32_t count) -> bool {
-    for (uint32_t k = first; k < first + count; ++k) {
-      const float4 a = tris[size_t(k) * 3], c1 = tris[size_t(k) * 3 + 1], c2 = tris[size_t(k) * 3 + 2];
-      float t, u, v;
-      if (hit_triangle(ro, rd, mk(a), mk(c1), mk(c2), tmax, t, u, v) &&
-          (__float_as_int(c2.w) == 0 || alpha_opaque(s, k, u, v, __float_as_int(a.w), rng)))
-        return true;
-    }
-    return false;
-  };
-  if (b.root_count > 0) return leaf(b.root_link, b.root_count);
-  st.reset();
-  uint32_t cur = b.root_link;
-  while (true) {
-    const PairRec r = load_pair_rec(b.pairs, cur);
-    float t0, t1;
-    slab2(r, ro, inv, tmax, t0, t1);
-    const bool h0 = t0 != INF, h1 = t1 != INF;
-    if (h0 && r.c0 > 0 && leaf(r.l0, r.c0)) return true;
-    if (h1 && r.c1 > 0 && leaf(r.l1, r.c1)) return true;
-    const bool g0 = h0 && r.c0 == 0, g1 = h1 && r.c1 == 0;
-    if (g0 && g1) {
-      cur = r.l0;
-      st.push(r.l1);
-    } else if (g0) {
-      cur = r.l0;
-    } else if (g1) {
-      cur = r.l1;
-    } else {
-      if (st.sp == 0) return false;
-      cur = st.pop();
-    }
-  }
+  Trav4 t;
+  trav4_begin(s, t, ro, rd, tmax, st);
+  while (!trav4_done(t, st))
+    if (trav4_step<true>(s, t, st, rng)) return true;
+  return false;
 }
 
 // The closest primitive found by traceScene's search (scene.glsl:300-441):

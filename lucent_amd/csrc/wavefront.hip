@@ -39,9 +39,7 @@ namespace {
 using namespace pt;
 
 constexpr int kWfBlock = 64;
-constexpr int kWfLds = 16;
 constexpr int kTile = 8;
-constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kFlagSpecular = 1u, kFlagGuideDone = 2u, kFlagTerminating = 4u;
 constexpr int kBounceShift = 8;
 // A wave refills its idle lanes once at least this many are idle.
@@ -64,96 +62,6 @@ __device__ inline bool slot_pixel(const PtFrame& f, uint32_t tiles_x, uint32_t s
   x = (tile % tiles_x) * kTile + (l & (kTile - 1));
   y = (tile / tiles_x) * kTile + (l / kTile);
   return x < f.width && y < f.height;
-}
-
-// ---------------------------------------------------------------------------
-// Traversal state machines.  One step = one pair visit or one triangle test,
-// in the order of trace_bvh / occluded (pt_device.hpp): at a pair visit both
-// children are slab-tested against the current closest distance, the hit
-// leaf children are queued (child 0 first), the next pair is chosen (near
-// child first for closest hits, left first for shadow rays) or popped; the
-// queued triangles are then tested one per step before that next pair.
-// ---------------------------------------------------------------------------
-struct Trav {
-  v3 ro, rd, inv;
-  float tmax;  // closest distance so far (closest) or the ray's extent (shadow)
-  float bu, bv;
-  int best;
-  uint32_t cur;     // next pair, kNone when none is left
-  uint32_t la, na;  // queued leaf A: next slot, triangles left
-  uint32_t lb, nb;  // queued leaf B
-};
-
-template <class StackT>
-__device__ inline void trav_begin(const PtScene& s, Trav& t, v3 ro, v3 rd, float tmax, StackT& st) {
-  t.ro = ro;
-  t.rd = rd;
-  t.inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-  t.tmax = tmax;
-  t.bu = t.bv = 0.0f;
-  t.best = -1;
-  t.cur = kNone;
-  t.la = t.na = t.lb = t.nb = 0;
-  st.reset();
-  const DevBvh& b = s.bvh;
-  if (b.tri_count == 0) return;
-  if (b.root_count > 0) {
-    t.la = b.root_link;
-    t.na = b.root_count;
-  } else {
-    t.cur = b.root_link;
-  }
-}
-
-__device__ inline bool trav_done(const Trav& t) { return t.na == 0 && t.nb == 0 && t.cur == kNone; }
-
-// One step; for shadow rays (kAny) returns true on the first accepted hit.
-template <bool kAny, class StackT>
-__device__ inline bool trav_step(const PtScene& s, Trav& t, StackT& st, Rng& rng) {
-  if (t.na > 0 || t.nb > 0) {
-    const bool a = t.na > 0;
-    const uint32_t k = a ? t.la : t.lb;
-    if (a) {
-      ++t.la;
-      --t.na;
-    } else {
-      ++t.lb;
-      --t.nb;
-    }
-    const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris);
-    const float4 p0 = tris[size_t(k) * 3], p1 = tris[size_t(k) * 3 + 1], p2 = tris[size_t(k) * 3 + 2];
-    float th, u, v;
-    if (hit_triangle(t.ro, t.rd, mk(p0), mk(p1), mk(p2), t.tmax, th, u, v) &&
-        (__float_as_int(p2.w) == 0 || alpha_opaque(s, k, u, v, __float_as_int(p0.w), rng))) {
-      if (kAny) return true;
-      t.tmax = th;
-      t.best = int(k);
-      t.bu = u;
-      t.bv = v;
-    }
-    return false;
-  }
-  const PairRec r = load_pair_rec(s.bvh.pairs, t.cur);
-  float t0, t1;
-  slab2(r, t.ro, t.inv, t.tmax, t0, t1);
-  const bool h0 = t0 != INF, h1 = t1 != INF;
-  t.la = r.l0;
-  t.na = (h0 && r.c0 > 0) ? r.c0 : 0u;
-  t.lb = r.l1;
-  t.nb = (h1 && r.c1 > 0) ? r.c1 : 0u;
-  const bool g0 = h0 && r.c0 == 0, g1 = h1 && r.c1 == 0;
-  if (g0 && g1) {
-    const bool first0 = kAny ? true : (t0 <= t1);
-    t.cur = first0 ? r.l0 : r.l1;
-    st.push(first0 ? r.l1 : r.l0);
-  } else if (g0) {
-    t.cur = r.l0;
-  } else if (g1) {
-    t.cur = r.l1;
-  } else {
-    t.cur = st.sp == 0 ? kNone : st.pop();
-  }
-  return false;
 }
 
 // ---------------------------------------------------------------------------
@@ -269,16 +177,16 @@ __global__ void k_wf_advance(WfCtl* ctl) {
 
 template <bool kShadow>
 __global__ __launch_bounds__(kWfBlock) void k_wf_trace(TileItem it, WfBuffers B) {
-  __shared__ uint32_t lds[kWfLds * kWfBlock];
+  __shared__ uint32_t lds[kPtStackWords * kWfBlock];
   const PtScene& sc = it.scene;
-  Stack<kWfLds, kWfBlock> st{lds + threadIdx.x, B.spill + size_t(blockIdx.x * kWfBlock + threadIdx.x) * B.spill_depth,
+  PtStack<kWfBlock> st{lds + threadIdx.x, B.spill + size_t(blockIdx.x * kWfBlock + threadIdx.x) * B.spill_depth,
                              0, 0};
   const uint32_t p = __builtin_amdgcn_readfirstlane(B.ctl->parity);
   const uint32_t items = __builtin_amdgcn_readfirstlane(kShadow ? B.ctl->scount[p] : B.ctl->count[p]);
   const uint32_t* queue = kShadow ? B.squeue[p] : B.queue[p];
   uint32_t* cursor = &B.ctl->cursor[kShadow ? 1 : 0];
   Rng rng{0};  // alpha-blended materials never reach this kernel
-  Trav t;
+  Trav4 t;
   HitRecord rec;
   uint32_t item = 0;
   bool active = false, exhausted = false;
@@ -299,7 +207,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(TileItem it, WfBuffers B)
               blocked = hit_sphere(ro, rd, reinterpret_cast<const float4*>(sc.spheres)[k], o.w) < o.w;
             if (!blocked) blocked = hit_ground(sc, ro, rd, o.w) < o.w;
             if (!blocked) {
-              trav_begin(sc, t, ro, rd, o.w, st);
+              trav4_begin(sc, t, ro, rd, o.w, st);
               active = true;
             }
           } else {
@@ -309,7 +217,7 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(TileItem it, WfBuffers B)
               const float4 o = B.ray_o[slot], d = B.ray_d[slot];
               const v3 ro = mk(o.x, o.y, o.z), rd = mk(d.x, d.y, d.z);
               search_analytic(sc, ro, rd, rec);
-              trav_begin(sc, t, ro, rd, rec.t, st);
+              trav4_begin(sc, t, ro, rd, rec.t, st);
               active = true;
             }
           }
@@ -322,13 +230,13 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(TileItem it, WfBuffers B)
       continue;
     }
     if (active) {
-      const bool blocked = trav_step<kShadow>(sc, t, st, rng);
+      const bool blocked = trav4_step<kShadow>(sc, t, st, rng);
       if (kShadow) {
-        if (blocked || trav_done(t)) {
+        if (blocked || trav4_done(t, st)) {
           if (!blocked) B.nee[item].w = 1.0f;
           active = false;
         }
-      } else if (trav_done(t)) {
+      } else if (trav4_done(t, st)) {
         HitRecord r = rec;
         if (t.best >= 0) {
           r.t = t.tmax;
@@ -450,7 +358,7 @@ Status launch_wavefront(Context* ctx, TileItem& it, const WavefrontLaunch& cfg, 
   const uint32_t w_shadow = cus * (cfg.waves_per_cu ? cfg.waves_per_cu : 24);
   const uint32_t w_shade = cus * (cfg.waves_per_cu ? cfg.waves_per_cu : 8);
   const uint32_t waves = std::max(w_extend, w_shadow);
-  const uint32_t depth = it.scene.bvh.stack_depth + 1;
+  const uint32_t depth = pt_spill_words(it.scene.bvh);
   if ((s = ensure_spill(ctx, size_t(waves) * kWfBlock * depth * sizeof(uint32_t))) != LUCENT_OK) return s;
   // Carve the buffers.
   char* base = static_cast<char*>(ctx->wavefront);
