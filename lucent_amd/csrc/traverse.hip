@@ -1,19 +1,23 @@
 // Batched BVH ray queries on gfx950.
 //
 // One lane per ray, grid-stride over the batch.  The traversal walks the
-// 64-byte child-pair records (internal.hpp): one record fetch tests both
-// children, leaves are tested as soon as their box is hit, and the nearer of
-// two interior children is descended first while the farther goes on a
-// per-lane stack.  The stack's top kLdsStack entries live in LDS (lane-major
-// columns, so the 64 lanes of a wave hit 64 distinct banks); deeper entries
-// spill to a per-lane global area sized from the tree depth, so no entry is
-// ever dropped.
+// 4-wide nodes (internal.hpp: a 64-byte record per node, child boxes as 8-bit
+// codes rounded outward in a per-node frame): one record fetch tests four
+// children, the hit children go on a per-lane stack of (link, entry
+// distance) pairs, nearest on top, and a popped leaf has all its triangles
+// tested.  The stack's top kLdsStack entries live in LDS (lane-major
+// columns, so the lanes of a wave hit distinct banks); deeper entries spill
+// to a per-lane global area sized from the tree depth, so no entry is ever
+// dropped.
 //
 // Exact mode reproduces intersectBVH (reference src/math/bvh.ts:265-320):
-//  * box tests run in float32 but are made conservative — the far distance
-//    and the current closest hit are inflated by 2^-20 relative, far more
-//    than the <= 6 ulp error of the float32 slab, so every box the FP64 CPU
-//    test accepts is accepted here (extra boxes only cost time);
+//  * box tests run in float32 but are made conservative — the decoded boxes
+//    contain the reference's float32 boxes, the far distance and the current
+//    closest hit are inflated by 2^-20 relative, far more than the <= 6 ulp
+//    error of the float32 slab, and a popped entry is skipped only when its
+//    entry distance, shrunk by the same margin, lies strictly beyond the
+//    closest hit; so every box the FP64 CPU test accepts is visited here
+//    (extra boxes only cost time);
 //  * triangle tests run in FP64 with the exact operation order of
 //    src/math/ray.ts:22-45 (no contraction), so t, u and v are bit-identical;
 //  * the CPU's left-first walk with strict `t < closest` keeps the lowest
@@ -32,108 +36,110 @@ namespace lucent {
 namespace {
 
 constexpr int kBlock = 128;
-constexpr int kLdsStack = 16;
+constexpr int kLdsStack = 12;
 constexpr float kMargin32 = 1.0f + 1.0f / 1048576.0f;       // 1 + 2^-20
 constexpr double kMargin64 = 1.0 + 1.0 / 1099511627776.0;  // 1 + 2^-40
 constexpr float kGlslEps = 1e-6f;
 
 enum class Prec { kExact, kGlsl };
 
-// kLds entries of the stack stay in LDS as a ring; older entries spill.
+// (link, entry distance) pairs; kLds entries stay in LDS as a ring (two
+// lane-major planes), older ones spill to 2 words per entry.
 template <int kLds>
 struct LaneStack {
-  uint32_t* lds;    // this lane's column: entry k at lds[k * kBlock]
+  uint32_t* lds;    // this lane's column of plane 0: entry k at lds[k * kBlock]; plane 1 kLds * kBlock later
   uint32_t* spill;  // this lane's global area
   uint32_t sp = 0, base = 0;
 
-  __device__ void push(uint32_t x) {
+  __device__ void push(uint32_t link, float dist) {
     if (sp - base == kLds) {
-      spill[base] = lds[(base % kLds) * kBlock];
+      const uint32_t slot = (base % kLds) * kBlock;
+      spill[2 * base] = lds[slot];
+      spill[2 * base + 1] = lds[kLds * kBlock + slot];
       ++base;
     }
-    lds[(sp % kLds) * kBlock] = x;
+    const uint32_t slot = (sp % kLds) * kBlock;
+    lds[slot] = link;
+    lds[kLds * kBlock + slot] = __float_as_uint(dist);
     ++sp;
   }
-  __device__ uint32_t pop() {
+  __device__ void pop(uint32_t& link, float& dist) {
     --sp;
-    if (sp >= base) return lds[(sp % kLds) * kBlock];
+    if (sp >= base) {
+      const uint32_t slot = (sp % kLds) * kBlock;
+      link = lds[slot];
+      dist = __uint_as_float(lds[kLds * kBlock + slot]);
+      return;
+    }
     --base;
-    return spill[sp];
+    link = spill[2 * sp];
+    dist = __uint_as_float(spill[2 * sp + 1]);
   }
   __device__ bool empty() const { return sp == 0; }
 };
 
-struct PairRecord {
-  float4 x, y, z;  // (lo0, lo1, hi0, hi1) per axis
-  uint4 link;      // (link0, count0, link1, count1)
+constexpr uint32_t kLinkLeaf = 0x80000000u, kLinkNone = 0xFFFFFFFFu;
+constexpr float kShrink32 = 1.0f - 1.0f / 1048576.0f;  // 1 - 2^-20
+
+// A 4-wide node's children: decoded boxes (fmaf(code, scale, origin), the
+// expression the host rounded the codes outward for) and links.
+struct Node4 {
+  float lo[4][3], hi[4][3];
+  uint32_t link[4];
 };
 
-__device__ inline PairRecord load_pair(const float4* pairs, uint32_t p) {
-  const float4* r = pairs + size_t(p) * 4;
-  PairRecord rec;
-  rec.x = r[0];
-  rec.y = r[1];
-  rec.z = r[2];
-  const float4 l = r[3];
-  rec.link = make_uint4(__float_as_uint(l.x), __float_as_uint(l.y), __float_as_uint(l.z), __float_as_uint(l.w));
-  return rec;
-}
-
-// Slab test of both children in float32.  min/max follow IEEE minNum/maxNum,
-// so a NaN slab (zero direction component with the origin on the plane)
-// leaves the interval unconstrained, as the reference's comparisons do.
-template <bool kConservative>
-__device__ inline void slab_pair(const PairRecord& r, const float o[3], const float inv[3], float limit, float& tn0,
-                                 float& tn1, bool& h0, bool& h1) {
-  const float x0 = (r.x.x - o[0]) * inv[0], x1 = (r.x.z - o[0]) * inv[0];
-  const float y0 = (r.y.x - o[1]) * inv[1], y1 = (r.y.z - o[1]) * inv[1];
-  const float z0 = (r.z.x - o[2]) * inv[2], z1 = (r.z.z - o[2]) * inv[2];
-  const float X0 = (r.x.y - o[0]) * inv[0], X1 = (r.x.w - o[0]) * inv[0];
-  const float Y0 = (r.y.y - o[1]) * inv[1], Y1 = (r.y.w - o[1]) * inv[1];
-  const float Z0 = (r.z.y - o[2]) * inv[2], Z1 = (r.z.w - o[2]) * inv[2];
-  tn0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-  tn1 = fmaxf(fmaxf(fminf(X0, X1), fminf(Y0, Y1)), fmaxf(fminf(Z0, Z1), 0.0f));
-  float tf0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-  float tf1 = fminf(fminf(fmaxf(X0, X1), fmaxf(Y0, Y1)), fmaxf(Z0, Z1));
-  if (kConservative) {
-    tf0 = fminf(tf0, limit) * kMargin32;
-    tf1 = fminf(tf1, limit) * kMargin32;
-  } else {
-    tf0 = fminf(tf0, limit);
-    tf1 = fminf(tf1, limit);
-  }
-  h0 = tf0 >= tn0;
-  h1 = tf1 >= tn1;
-}
-
-// Double-precision slab of both children (for double-precision rays, whose
-// origin cannot be rounded to float32 without an absolute error).
-__device__ inline void slab_pair64(const PairRecord& r, const double o[3], const double inv[3], double limit,
-                                   double& tn0, double& tn1, bool& h0, bool& h1) {
-#pragma clang fp contract(off)
-  const float lo[2][3] = {{r.x.x, r.y.x, r.z.x}, {r.x.y, r.y.y, r.z.y}};
-  const float hi[2][3] = {{r.x.z, r.y.z, r.z.z}, {r.x.w, r.y.w, r.z.w}};
-  double tn[2], tf[2];
-  for (int c = 0; c < 2; ++c) {
-    double a = 0.0, b = limit;
+__device__ inline void load_node4(const float4* nodes, uint32_t n, Node4& nd) {
+  const float4* r = nodes + size_t(n) * 4;
+  const float4 A = r[0], B = r[1], C = r[2], L = r[3];
+  const float org[3] = {A.x, A.y, A.z}, sc[3] = {A.w, B.x, B.y};
+  const uint32_t ql[3] = {__float_as_uint(B.z), __float_as_uint(C.x), __float_as_uint(C.z)};
+  const uint32_t qh[3] = {__float_as_uint(B.w), __float_as_uint(C.y), __float_as_uint(C.w)};
+  nd.link[0] = __float_as_uint(L.x);
+  nd.link[1] = __float_as_uint(L.y);
+  nd.link[2] = __float_as_uint(L.z);
+  nd.link[3] = __float_as_uint(L.w);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-      double t0 = (double(lo[c][k]) - o[k]) * inv[k];
-      double t1 = (double(hi[c][k]) - o[k]) * inv[k];
-      if (t0 > t1) {
-        const double tt = t0;
-        t0 = t1;
-        t1 = tt;
-      }
-      if (t0 > a) a = t0;
-      if (t1 < b) b = t1;
+      nd.lo[i][k] = fmaf(float((ql[k] >> (8 * i)) & 255u), sc[k], org[k]);
+      nd.hi[i][k] = fmaf(float((qh[k] >> (8 * i)) & 255u), sc[k], org[k]);
     }
-    tn[c] = a;
-    tf[c] = b * kMargin64;
+}
+
+// Slab test of child i in float32 (conservative: the far distance inflated).
+template <bool kConservative>
+__device__ inline bool slab4(const Node4& nd, int i, const float o[3], const float inv[3], float limit, float& tn) {
+  float a = 0.0f, b = limit;
+  float lo = 0.0f, hi = 0.0f;
+  const float x0 = (nd.lo[i][0] - o[0]) * inv[0], x1 = (nd.hi[i][0] - o[0]) * inv[0];
+  const float y0 = (nd.lo[i][1] - o[1]) * inv[1], y1 = (nd.hi[i][1] - o[1]) * inv[1];
+  const float z0 = (nd.lo[i][2] - o[2]) * inv[2], z1 = (nd.hi[i][2] - o[2]) * inv[2];
+  lo = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), a));
+  hi = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+  hi = kConservative ? fminf(hi, b) * kMargin32 : fminf(hi, b);
+  tn = lo;
+  return hi >= lo;
+}
+
+// Double-precision slab of child i (double-precision rays).
+__device__ inline bool slab4_64(const Node4& nd, int i, const double o[3], const double inv[3], double limit,
+                                double& tn) {
+#pragma clang fp contract(off)
+  double a = 0.0, b = limit;
+  for (int k = 0; k < 3; ++k) {
+    double t0 = (double(nd.lo[i][k]) - o[k]) * inv[k];
+    double t1 = (double(nd.hi[i][k]) - o[k]) * inv[k];
+    if (t0 > t1) {
+      const double tt = t0;
+      t0 = t1;
+      t1 = tt;
+    }
+    if (t0 > a) a = t0;
+    if (t1 < b) b = t1;
   }
-  tn0 = tn[0];
-  tn1 = tn[1];
-  h0 = tf[0] >= tn[0];
-  h1 = tf[1] >= tn[1];
+  tn = a;
+  return b * kMargin64 >= a;
 }
 
 // Möller–Trumbore in FP64, operation for operation as src/math/ray.ts:22-45
@@ -223,8 +229,8 @@ __global__ __launch_bounds__(kBlock) void k_intersect(DevBvh bvh, const void* __
                                                       uint32_t* __restrict__ id_out, double* __restrict__ u_out,
                                                       double* __restrict__ v_out, uint32_t* __restrict__ spill,
                                                       uint32_t spill_depth) {
-  __shared__ uint32_t lds_stack[kLds * kBlock];
-  const float4* pairs = reinterpret_cast<const float4*>(bvh.pairs);
+  __shared__ uint32_t lds_stack[2 * kLds * kBlock];
+  const float4* nodes = reinterpret_cast<const float4*>(bvh.nodes4);
   const float4* tris = reinterpret_cast<const float4*>(bvh.tris);
   const uint32_t lane_global = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t stride = gridDim.x * kBlock;
@@ -261,59 +267,63 @@ __global__ __launch_bounds__(kBlock) void k_intersect(DevBvh bvh, const void* __
     }
     const double lower = kPrec == Prec::kExact ? tmin : double(kGlslEps);
     Best best{tmax, 0.0, 0.0, 0xFFFFFFFFu};
-    bool done = false;
 
     LaneStack<kLds> st{lds_stack + threadIdx.x, spill + size_t(lane_global) * spill_depth};
-    bool have = false;
-    uint32_t cur = 0;
-    if (bvh.tri_count == 0) {
-      done = true;
-    } else if (bvh.root_count > 0) {
-      test_leaf<kPrec, kAny>(tris, bvh.root_link, bvh.root_count, rod, rdd, rof, rdf, lower, tmax, best);
-      done = true;
-    } else {
-      cur = bvh.root_link;
-      have = true;
-    }
-    while (!done) {
-      if (!have) {
-        if (st.empty()) break;
-        cur = st.pop();
+    if (bvh.tri_count > 0) st.push(bvh.root4, 0.0f);
+    while (!st.empty()) {
+      uint32_t link;
+      float dist;
+      st.pop(link, dist);
+      // dist is a lower bound of the entry distance (shrunk by 2^-20 in exact
+      // mode); entries strictly beyond the closest hit cannot hold a hit that
+      // wins, ties included.
+      if (!kAny && double(dist) > best.t) continue;
+      if (link & kLinkLeaf) {
+        const uint32_t first = link & 0x03FFFFFFu, count = ((link >> 26) & 31u) + 1u;
+        if (test_leaf<kPrec, kAny>(tris, first, count, rod, rdd, rof, rdf, lower, tmax, best)) break;
+        continue;
       }
-      have = false;
-      const PairRecord rec = load_pair(pairs, cur);
-      bool h0, h1;
-      float tn0, tn1;
-      if (kF64Rays && kPrec == Prec::kExact) {
-        double a0, a1;
-        slab_pair64(rec, rod, invd, best.t, a0, a1, h0, h1);
-        tn0 = float(a0);
-        tn1 = float(a1);
-      } else if (kPrec == Prec::kExact) {
-        slab_pair<true>(rec, rof, invf, __double2float_ru(best.t), tn0, tn1, h0, h1);
-      } else {
-        slab_pair<false>(rec, rof, invf, float(best.t), tn0, tn1, h0, h1);
+      Node4 nd;
+      load_node4(nodes, link, nd);
+      float tn[4];
+      uint32_t l[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        l[i] = nd.link[i];
+        bool hit;
+        float t;
+        if (kF64Rays && kPrec == Prec::kExact) {
+          double td;
+          hit = slab4_64(nd, i, rod, invd, best.t, td);
+          t = __double2float_rd(td) * kShrink32;
+        } else if (kPrec == Prec::kExact) {
+          hit = slab4<true>(nd, i, rof, invf, __double2float_ru(best.t), t);
+          t *= kShrink32;
+        } else {
+          hit = slab4<false>(nd, i, rof, invf, float(best.t), t);
+        }
+        tn[i] = (l[i] != kLinkNone && hit) ? t : INFINITY;
       }
-      const uint32_t link0 = rec.link.x, cnt0 = rec.link.y, link1 = rec.link.z, cnt1 = rec.link.w;
-      if (h0 && cnt0 > 0) {
-        if (test_leaf<kPrec, kAny>(tris, link0, cnt0, rod, rdd, rof, rdf, lower, tmax, best)) break;
+      if (!kAny) {
+        auto cs = [&](int a, int b) {
+          if (tn[a] < tn[b]) {
+            const float td = tn[a];
+            tn[a] = tn[b];
+            tn[b] = td;
+            const uint32_t ld = l[a];
+            l[a] = l[b];
+            l[b] = ld;
+          }
+        };
+        cs(0, 1);
+        cs(2, 3);
+        cs(0, 2);
+        cs(1, 3);
+        cs(1, 2);
       }
-      if (h1 && cnt1 > 0) {
-        if (test_leaf<kPrec, kAny>(tris, link1, cnt1, rod, rdd, rof, rdf, lower, tmax, best)) break;
-      }
-      const bool g0 = h0 && cnt0 == 0, g1 = h1 && cnt1 == 0;
-      if (g0 && g1) {
-        const bool first0 = tn0 <= tn1;
-        cur = first0 ? link0 : link1;
-        st.push(first0 ? link1 : link0);
-        have = true;
-      } else if (g0) {
-        cur = link0;
-        have = true;
-      } else if (g1) {
-        cur = link1;
-        have = true;
-      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (tn[i] != INFINITY) st.push(l[i], tn[i]);
     }
     if (best.slot != 0xFFFFFFFFu) {
       t_out[r] = best.t;
@@ -338,7 +348,7 @@ Status launch_variant(Context* ctx, const DevBvh& bvh, const RayQuery& q, hipStr
   uint64_t resident = uint64_t(per_cu) * uint64_t(ctx->compute_units > 0 ? ctx->compute_units : 256);
   uint32_t grid = uint32_t(blocks_needed < resident ? blocks_needed : resident);
   if (grid == 0) return LUCENT_OK;
-  const uint32_t depth = bvh.stack_depth + 1;
+  const uint32_t depth = 2 * bvh.stack4;  // words per lane: (link, distance) entries
   Status s = ensure_spill(ctx, size_t(grid) * kBlock * depth * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, bvh, q.org, q.dir, q.n, q.tmin, q.tmax, q.t, q.id,
