@@ -356,11 +356,14 @@ def bench_adjoint(args) -> dict:
     ctx = Context(0)
     sc = workloads.spheres_scene()
     d = DiffRenderer(ctx, sc, W, H, workloads.DIFF_DEPTH)
+    from lucent_amd.renderer import autotune
+    tune = autotune(ctx, [(d.renderer, SPP)], d.renderer.settings, trial_samples=2)
+    log(f"forward occupancy {tune['occupancy']}")
     base_mats, base_env = list(d.materials), d.env.copy()
     pm, penv = workloads.perturb_params(base_mats, base_env, seed=5)
     d.set_materials(pm)
     d.set_env_texels(penv)
-    target = d.forward(128, 1000)
+    target = d.forward(128, 1000, record=False)
     d.set_materials(base_mats)
     d.set_env_texels(base_env)
     img = d.forward(SPP, 0)
@@ -385,7 +388,8 @@ def bench_adjoint(args) -> dict:
             "value": round(1.0 / (t_fwd + t_bwd), 3), "unit": "grad/s", "forward_ms": round(t_fwd * 1e3, 2),
             "backward_ms": round(t_bwd * 1e3, 2), "params": {"material_scalars": int(gm[:16].size),
                                                                "env_channels": int(ge.size)},
-            "loss": float(((img - target) ** 2).sum())}
+            "loss": float(((img - target) ** 2).sum()), "forward_occupancy": tune["occupancy"],
+            "backward": "replays the forward pass's recorded per-sample radiance"}
 
 
 def bench_batch(args) -> dict:

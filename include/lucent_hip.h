@@ -350,12 +350,25 @@ lucent_status lucent_film_read_denoised(lucent_ctx *ctx, const lucent_film *film
  * sampler are held constant (the estimator is unbiased for any of them).
  * No counterpart in the reference (which has no differentiation); the
  * parameters are those of material.ts:117-131 and lights.glsl:45-47.
+ * d_sample_radiance: the record of lucent_forward for these seeds, or NULL
+ * to re-trace each primal path.
  * All pointers are device pointers; grads are overwritten (not accumulated).
  * ---------------------------------------------------------------------- */
 lucent_status lucent_backward(lucent_ctx *ctx, lucent_scene *scene, const lucent_camera *camera,
                               const lucent_render_settings *settings, uint32_t width, uint32_t height,
-                              uint32_t spp, uint32_t seed0, const float *d_dl_di, double *d_grad_materials,
-                              double *d_grad_env, void *stream);
+                              uint32_t spp, uint32_t seed0, const float *d_dl_di, const float *d_sample_radiance,
+                              double *d_grad_materials, double *d_grad_env, void *stream);
+
+/* The forward pass for lucent_backward: resets the film, sets its seed to
+ * seed0 and traces spp samples of every pixel (lucent_trace_samples with the
+ * persistent tile kernel, clamp 0), and, if d_sample_radiance is not NULL,
+ * records each path's radiance there (device, float4 per pixel per sample,
+ * sample k at k * width * height; 16 * spp * width * height bytes).  Passing
+ * that record to lucent_backward (same scene parameters, seeds and size)
+ * spares it re-tracing every primal path.  Asynchronous on `stream`. */
+lucent_status lucent_forward(lucent_ctx *ctx, lucent_scene *scene, lucent_film *film, const lucent_camera *camera,
+                             const lucent_render_settings *settings, uint32_t spp, uint32_t seed0,
+                             float *d_sample_radiance, void *stream);
 
 /* The estimator lucent_backward differentiates, evaluated at other material
  * parameters: the samples of seeds seed0 .. seed0+spp-1 take every random

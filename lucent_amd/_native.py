@@ -146,7 +146,9 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_denoise": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(DenoiseParamsStruct), _P]),
     "lucent_film_read_denoised": (_I32, [_P, _P, _P, _P]),
     "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,
-                                _U32, _P, _P, _P, _P]),
+                                _U32, _P, _P, _P, _P, _P]),
+    "lucent_forward": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _P,
+                               _P]),
     "lucent_detached_forward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32,
                                        _U32, _U32, _P, _U32, _P, _P]),
     "lucent_scene_set_materials": (_I32, [_P, _P, _P, _U32]),

@@ -1,8 +1,8 @@
 // Path-replay adjoint of the progressive path tracer on gfx950.
 //
-// For each pixel sample the kernel first re-runs the primal path from its
-// RNG seed (trace.frag semantics, pt_device.hpp) to obtain the path radiance
-// L, then replays the same path (same random numbers, so the same vertices
+// For each pixel sample the kernel takes the path radiance L recorded by the
+// forward pass (lucent_forward), or re-runs the primal path from its RNG seed
+// (trace.frag semantics, pt_device.hpp) to obtain it, then replays the same path (same random numbers, so the same vertices
 // and decisions) and differentiates each contribution:
 //   * NEE terms  beta * f(θ) * Li * k     -> d/dθ through f and through the
 //                                            environment texels behind Li;
@@ -489,7 +489,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, 
 }
 
 __global__ __launch_bounds__(kAdjBlock) void k_adjoint(PtScene sc, PtFrame f, const float* __restrict__ dl_di,
-                                                       float inv_spp, float* __restrict__ mat_partials,
+                                                       const float4* __restrict__ recorded_L, float inv_spp,
+                                                       float* __restrict__ mat_partials,
                                                        float* __restrict__ env_acc, size_t env_stride,
                                                        float* __restrict__ mat_overflow, uint32_t* spill,
                                                        uint32_t spill_depth, uint32_t tiles_x) {
@@ -511,8 +512,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(PtScene sc, PtFrame f, co
       PtStack<kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth,
                                    0, 0};
       Rng replay_rng = rng;
-      Guide gd;
-      const v3 L = trace_path(sc, f, ro, rd, rng, st, gd);
+      v3 L;
+      if (recorded_L) {  // the forward pass kept this path's radiance
+        const float4 r = recorded_L[idx];
+        L = mk(r.x, r.y, r.z);
+      } else {
+        Guide gd;
+        L = trace_path(sc, f, ro, rd, rng, st, gd);
+      }
       if (!(isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))) {
         GradSink g{lds_grad, env_acc, env_stride, blockIdx.x % kEnvReplicas, int(sc.material_count), mat_overflow};
         replay(sc, f, ro, rd, replay_rng, st, L, dl, g);
@@ -613,7 +620,10 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
     hipLaunchKernelGGL(k_zero, dim3(uint32_t((zero_n + 255) / 256)), dim3(256), 0, stream, mat_overflow, zero_n);
     PtFrame f = a.frame;
     f.seed = a.seed0 + k;
-    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(kAdjBlock), 0, stream, sc, f, a.dl_di, 1.0f / float(a.spp),
+    const float4* rec = a.recorded_radiance
+                            ? reinterpret_cast<const float4*>(a.recorded_radiance) + size_t(k) * a.frame.width * a.frame.height
+                            : nullptr;
+    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(kAdjBlock), 0, stream, sc, f, a.dl_di, rec, 1.0f / float(a.spp),
                        mat_partials, env_acc, env_stride, mat_overflow, static_cast<uint32_t*>(ctx->spill), depth,
                        tiles_x);
     const uint32_t entries = uint32_t(min(int(sc.material_count), kLdsMaterials)) * kNumParams;

@@ -14,6 +14,7 @@ struct AdjointArgs {
   uint32_t spp = 1;
   uint32_t seed0 = 0;
   const float* dl_di = nullptr;     // dLoss/dImage, float4 per pixel (rows bottom-up)
+  const float* recorded_radiance = nullptr;  // optional: float4 per pixel per sample from the forward pass
   double* grad_materials = nullptr; // material_count * 5 (albedo rgb, roughness, metallic)
   double* grad_env = nullptr;       // env_w * env_h * 3, or null
   void* scratch = nullptr;          // adjoint_scratch_bytes(...)

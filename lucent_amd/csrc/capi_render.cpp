@@ -507,10 +507,38 @@ static Status fill_frame(const lucent_camera* cam, const lucent_render_settings*
   return LUCENT_OK;
 }
 
+lucent_status lucent_forward(lucent_ctx* ctx, lucent_scene* scene, lucent_film* film, const lucent_camera* cam,
+                             const lucent_render_settings* settings, uint32_t spp, uint32_t seed0,
+                             float* d_sample_radiance, void* stream) {
+  clear_error();
+  if (!ctx || !scene || !film || !cam || !settings) return fail(LUCENT_E_INVALID, "null argument");
+  if (spp == 0) return fail(LUCENT_E_INVALID, "zero samples");
+  if (settings->firefly_clamp > 0.0f)
+    return fail(LUCENT_E_UNSUPPORTED, "the firefly clamp is not differentiable; use firefly_clamp = 0");
+  Status s = check_traceable(scene);
+  if (s != LUCENT_OK) return s;
+  film->seed = seed0;
+  if ((s = lucent_film_reset(ctx, film)) != LUCENT_OK) return s;
+  TileItem it;
+  it.scene = scene->pt;
+  fill_frame(cam, settings, film->width, film->height, &it.frame);
+  it.frame.seed = seed0;
+  it.film = film->f;
+  it.spp = spp;
+  it.sample_radiance = d_sample_radiance;
+  TileLaunch cfg;
+  cfg.waves_per_cu = settings->waves_per_cu;
+  cfg.occupancy = settings->occupancy;
+  if ((s = launch_trace_tiles(ctx, &it, 1, cfg, stream ? stream : ctx->stream)) != LUCENT_OK) return s;
+  film->seed += spp;
+  film->samples += spp;
+  return LUCENT_OK;
+}
+
 lucent_status lucent_backward(lucent_ctx* ctx, lucent_scene* scene, const lucent_camera* cam,
                               const lucent_render_settings* settings, uint32_t width, uint32_t height, uint32_t spp,
-                              uint32_t seed0, const float* d_dl_di, double* d_grad_materials, double* d_grad_env,
-                              void* stream) {
+                              uint32_t seed0, const float* d_dl_di, const float* d_sample_radiance,
+                              double* d_grad_materials, double* d_grad_env, void* stream) {
   clear_error();
   if (!ctx || !scene || !cam || !settings || !d_dl_di || !d_grad_materials) return fail(LUCENT_E_INVALID, "null argument");
   if (!scene->lighting_set) return fail(LUCENT_E_INVALID, "call lucent_scene_set_lighting before lucent_backward");
@@ -532,6 +560,7 @@ lucent_status lucent_backward(lucent_ctx* ctx, lucent_scene* scene, const lucent
   a.spp = spp;
   a.seed0 = seed0;
   a.dl_di = d_dl_di;
+  a.recorded_radiance = d_sample_radiance;
   a.grad_materials = d_grad_materials;
   a.grad_env = d_grad_env;
   a.scratch = ctx->scratch;

@@ -62,6 +62,9 @@ __device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t
   pt::path_begin(p, ro, rd);
   for (;;) {
     if (!pt::path_step(sc, f, p, rng, st)) {
+      if (it.sample_radiance)
+        reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
+            make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
       pt::accumulate(it.film, idx, p.radiance, p.guide);
       if (++k >= it.spp) break;
       rng = pt::seed_rng(idx, f.seed + k);

@@ -86,6 +86,9 @@ struct TileItem {
   PtFilm film;
   uint32_t spp = 0;
   uint32_t tiles_x = 0;
+  // Optional: each path's radiance as tracePath returns it, float4 per pixel
+  // per sample, sample-major (sample k of the launch at k * width * height).
+  float* sample_radiance = nullptr;
 };
 
 }  // namespace lucent

@@ -1,8 +1,9 @@
 """Differentiable rendering: forward/backward pair over the C ABI.
 
-forward() is Renderer.traceSamples into a fresh film (the image is the
-running mean of `spp` samples from seeds seed0..seed0+spp-1); backward()
-replays those same samples (lucent_backward) and returns the gradient of
+forward() is lucent_forward: Renderer.traceSamples into a fresh film (the
+image is the running mean of `spp` samples from seeds seed0..seed0+spp-1),
+keeping each path's radiance on the device; backward() replays those same
+samples (lucent_backward) and returns the gradient of
 sum_pixels dL_dI · image with respect to every material's (albedo rgb,
 roughness, metallic) and every environment texel channel.
 """
@@ -32,6 +33,8 @@ class DiffRenderer:
         ew, eh, env = scene.env.map
         self.env_size = (ew, eh)
         self.env = np.asarray(env, np.float32).copy()
+        self._version = 0      # bumped by every parameter change
+        self._record = None    # (spp, seed0, version, device buffer) of the last forward
 
     @property
     def settings(self):
@@ -42,19 +45,32 @@ class DiffRenderer:
         _native.check(_native.lib().lucent_scene_set_materials(self.ctx.handle, self.renderer.device_scene.handle,
                                                                packed.ctypes.data, len(materials)))
         self.materials = list(materials)
+        self._version += 1
 
     def set_env_texels(self, rgb) -> None:
         arr = np.ascontiguousarray(np.asarray(rgb, np.float32).reshape(-1))
         _native.check(_native.lib().lucent_scene_update_env_texels(self.ctx.handle, self.renderer.device_scene.handle,
                                                                    arr.ctypes.data, *self.env_size))
         self.env = arr.copy()
+        self._version += 1
 
-    def forward(self, spp: int, seed0: int = 0, stream: Optional[int] = None) -> np.ndarray:
-        """Image (H, W, 3) as float64, rows bottom-up."""
+    def forward(self, spp: int, seed0: int = 0, record: bool = True) -> np.ndarray:
+        """Image (H, W, 3) as float64, rows bottom-up (lucent_forward).  With
+        `record`, each path's radiance is kept on the device so that a
+        backward() with the same parameters and seeds need not re-trace."""
+        import torch
         r = self.renderer
-        r.film.set_seed(seed0)
-        r.reset()
-        r.trace_samples(spp, stream=stream)
+        buf = None
+        if record:
+            buf = torch.empty(spp * self.height * self.width * 4, dtype=torch.float32, device=torch.device("cuda:0"))
+        cam = camera_struct(self.scene.camera)
+        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0, self.settings.occupancy)
+        torch.cuda.synchronize()
+        _native.check(_native.lib().lucent_forward(self.ctx.handle, r.device_scene.handle, r.film.handle, C.byref(cam),
+                                                   C.byref(st), spp, seed0, buf.data_ptr() if buf is not None else None,
+                                                   None))
+        self.ctx.synchronize()
+        self._record = (spp, seed0, self._version, buf) if record else None
         return r.film.read()["color"][..., :3].astype(np.float64)
 
     def detached_forward(self, value_materials, spp: int, seed0: int = 0) -> np.ndarray:
@@ -76,8 +92,10 @@ class DiffRenderer:
         self.ctx.synchronize()
         return out.cpu().numpy().reshape(h, w, 4)[..., :3].astype(np.float64)
 
-    def backward(self, dl_di: np.ndarray, spp: int, seed0: int = 0, env: bool = True):
-        """Gradients (material_count x 5, env (h, w, 3)) of sum dl_di · image."""
+    def backward(self, dl_di: np.ndarray, spp: int, seed0: int = 0, env: bool = True, use_record: bool = True):
+        """Gradients (material_count x 5, env (h, w, 3)) of sum dl_di · image.
+        Uses the radiance recorded by the last forward() when it was made with
+        the same samples, seeds and parameters; otherwise re-traces."""
         import torch
         dev = torch.device("cuda:0")
         h, w = self.height, self.width
@@ -91,9 +109,12 @@ class DiffRenderer:
         cam = camera_struct(self.scene.camera)
         st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0, 0)
         torch.cuda.synchronize()
+        rec = self._record
+        use = use_record and rec is not None and rec[:3] == (spp, seed0, self._version)
         _native.check(_native.lib().lucent_backward(
             self.ctx.handle, self.renderer.device_scene.handle, C.byref(cam), C.byref(st), w, h, spp, seed0,
-            d_dl.data_ptr(), g_mat.data_ptr(), g_env.data_ptr() if env else None, None))
+            d_dl.data_ptr(), rec[3].data_ptr() if use else None, g_mat.data_ptr(), g_env.data_ptr() if env else None,
+            None))
         self.ctx.synchronize()
         gm = g_mat.cpu().numpy().reshape(nm, 5)
         ge = g_env.cpu().numpy().reshape(eh, ew, 3) if env else None

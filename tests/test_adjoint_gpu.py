@@ -136,7 +136,7 @@ def test_loss_gradient_matches_central_differences(diff):
     pm, penv = workloads.perturb_params(base_mats, base_env, seed=5)
     diff.set_materials(pm)
     diff.set_env_texels(penv)
-    target = diff.forward(128, 1000)
+    target = diff.forward(128, 1000, record=False)
     diff.set_materials(base_mats)
     diff.set_env_texels(base_env)
     img = diff.forward(SPP, 0)
@@ -214,3 +214,21 @@ def test_detached_estimator_is_unbiased(ctx):
         assert worst < 1.0
     finally:
         d.close()
+
+
+def test_recorded_forward_spares_the_retrace(diff):
+    """lucent_forward's per-sample radiance record gives lucent_backward the
+    same L the re-traced primal path would: gradients agree to float32
+    reduction-order noise, and the forward film equals trace_samples'."""
+    rng = np.random.default_rng(7)
+    dl = rng.standard_normal((H, W, 3))
+    img = diff.forward(SPP, 0)
+    g1, e1 = diff.backward(dl, SPP, 0)             # uses the record
+    g0, e0 = diff.backward(dl, SPP, 0, use_record=False)
+    assert np.allclose(g1, g0, rtol=1e-5, atol=1e-6 * np.abs(g0).max())
+    assert np.allclose(e1, e0, rtol=1e-4, atol=1e-6 * np.abs(e0).max())
+    r = diff.renderer
+    r.film.set_seed(0)
+    r.reset()
+    r.trace_samples(SPP)
+    assert np.array_equal(r.film.read()["color"][..., :3].astype(np.float64), img)
