@@ -115,7 +115,7 @@ void lucent_scene_destroy(lucent_scene *scene);
 typedef struct lucent_scene_stats {
   uint32_t tri_count;
   uint32_t node_count;  /* reference nodes */
-  uint32_t pair_count;  /* 64-byte child-pair records on the device */
+  uint32_t node4_count; /* 64-byte 4-wide nodes on the device */
   uint32_t depth;       /* deepest leaf, root = 0 */
   uint64_t device_bytes;
 } lucent_scene_stats;

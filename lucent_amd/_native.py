@@ -62,7 +62,7 @@ class SceneStatsStruct(C.Structure):
     _fields_ = [
         ("tri_count", C.c_uint32),
         ("node_count", C.c_uint32),
-        ("pair_count", C.c_uint32),
+        ("node4_count", C.c_uint32),
         ("depth", C.c_uint32),
         ("device_bytes", C.c_uint64),
     ]

@@ -118,25 +118,21 @@ bool finite_all(const float* p, size_t n) {
 // Shared tail of both upload forms: BVH conversion plus leaf-order geometry.
 Status finish_scene(Context* ctx, Scene* sc, const float* nodes, uint32_t node_count, const uint32_t* tri_indices,
                     const std::vector<float>& tris, const float* attrs) {
-  HostBvhImage img;
-  Status s = convert_bvh(nodes, node_count, sc->bvh.tri_count, &img);
+  uint32_t depth = 0;
+  Status s = validate_bvh(nodes, node_count, sc->bvh.tri_count, &depth);
   if (s != LUCENT_OK) return s;
-  sc->bvh.pair_count = img.pair_count;
-  sc->bvh.root_link = img.root_link;
-  sc->bvh.root_count = img.root_count;
-  sc->bvh.stack_depth = img.depth;
+  sc->bvh.depth = depth;
   sc->stats.tri_count = sc->bvh.tri_count;
   sc->stats.node_count = node_count;
-  sc->stats.pair_count = img.pair_count;
-  sc->stats.depth = img.depth;
+  sc->stats.depth = depth;
   HostBvh4Image img4;
   if ((s = convert_bvh4(nodes, node_count, sc->bvh.tri_count, &img4)) != LUCENT_OK) return s;
   sc->bvh.node4_count = img4.node_count;
   sc->bvh.root4 = img4.root;
   // Each level pushes at most 3 entries beyond the one it pops.
   sc->bvh.stack4 = 3 * img4.depth + 2;
+  sc->stats.node4_count = img4.node_count;
   const uint32_t n = sc->bvh.tri_count;
-  if ((s = upload_owned(sc, img.pairs.data(), img.pairs.size() * sizeof(float), &sc->bvh.pairs)) != LUCENT_OK) return s;
   if (img4.node_count > 0 &&
       (s = upload_owned(sc, img4.nodes.data(), img4.nodes.size() * sizeof(float), &sc->bvh.nodes4)) != LUCENT_OK)
     return s;

@@ -25,27 +25,23 @@ Status build_bvh(const float* positions, uint32_t tri_count, uint32_t max_leaf, 
 // Device scene layout (all device pointers; plain types so that host C++
 // compiled without HIP can fill it in).
 //
-//   pairs    : one 64-byte record per interior node, describing its two
-//              children: three float4 of bounds, one per axis,
-//              (lo_child0, lo_child1, hi_child0, hi_child1), then a uint4
-//              (link0, count0, link1, count1).  count > 0: leaf holding
-//              triangles [link, link+count) in leaf order; count == 0:
-//              interior, link = the child's own pair record.
+//   nodes4   : one 64-byte record per 4-wide node (scene_host.cpp,
+//              convert_bvh4): the frame origin and per-axis power-of-two
+//              scales, 8-bit lower/upper codes of the four child boxes per
+//              axis (decoded as fmaf(code, scale, origin), rounded outward),
+//              and four links: a node index, a leaf (bit 31 | (count-1) << 26
+//              | first slot) or 0xFFFFFFFF for no child.
 //   tris     : three float4 per leaf slot: (v0, material) (v1, light)
 //              (v2, alpha flag), the w fields as int32 bits.
 //   attrs    : four float4 per slot, the reference ATTR texels verbatim.
 //   tri_ids  : slot -> original triangle id.
 // ---------------------------------------------------------------------------
 struct DevBvh {
-  const float* pairs = nullptr;   // float4[4 * pair_count]
   const float* tris = nullptr;    // float4[3 * tri_count]
   const float* attrs = nullptr;   // float4[4 * tri_count] or null
   const uint32_t* tri_ids = nullptr;
   uint32_t tri_count = 0;
-  uint32_t pair_count = 0;
-  uint32_t root_link = 0;   // pair index, or first slot when the root is a leaf
-  uint32_t root_count = 0;  // > 0 when the root itself is a leaf
-  uint32_t stack_depth = 0; // entries a traversal stack may need
+  uint32_t depth = 0;       // binary tree depth (deepest leaf, root = 0)
   // 4-wide nodes of the path tracer: 16 floats per node (frame origin, scale,
   // 8-bit child codes per axis, 4 links); a link is a node index, a leaf
   // (bit 31 | (count - 1) << 26 | first slot) or 0xFFFFFFFF for no child.
@@ -77,14 +73,9 @@ struct Context {
   float* sheen_lut = nullptr;   // 16x16 half-rounded sheen albedo table
 };
 
-// Host-side conversion of the reference layout into the device layout.
-struct HostBvhImage {
-  std::vector<float> pairs;  // 16 floats per pair (links as uint32 bits)
-  uint32_t pair_count = 0;
-  uint32_t root_link = 0, root_count = 0;
-  uint32_t depth = 0;
-};
-Status convert_bvh(const float* nodes, uint32_t node_count, uint32_t tri_count, HostBvhImage* out);
+// Host-side checks and conversion of the reference layout into the device layout.
+// validate_bvh: leftFirst/count integral and in range, a tree; depth of the deepest leaf.
+Status validate_bvh(const float* nodes, uint32_t node_count, uint32_t tri_count, uint32_t* depth);
 struct HostBvh4Image {
   std::vector<float> nodes;  // 16 floats per node
   uint32_t node_count = 0;

@@ -303,64 +303,6 @@ __device__ inline bool alpha_opaque(const PtScene& s, uint32_t slot, float bu, f
   return rng.rand() < alpha;
 }
 
-struct PairRec {
-  float4 x, y, z;
-  uint32_t l0, c0, l1, c1;
-};
-__device__ inline PairRec load_pair_rec(const float* pairs, uint32_t p) {
-  const float4* r = reinterpret_cast<const float4*>(pairs) + size_t(p) * 4;
-  PairRec rec;
-  rec.x = r[0];
-  rec.y = r[1];
-  rec.z = r[2];
-  const float4 l = r[3];
-  rec.l0 = __float_as_uint(l.x);
-  rec.c0 = __float_as_uint(l.y);
-  rec.l1 = __float_as_uint(l.z);
-  rec.c1 = __float_as_uint(l.w);
-  return rec;
-}
-
-// hitAABB (scene.glsl:261-269) for both children of a pair record.
-__device__ inline void slab2(const PairRec& r, v3 o, v3 inv, float tmax, float& tn0, float& tn1) {
-  const float x0 = (r.x.x - o.x) * inv.x, x1 = (r.x.z - o.x) * inv.x;
-  const float y0 = (r.y.x - o.y) * inv.y, y1 = (r.y.z - o.y) * inv.y;
-  const float z0 = (r.z.x - o.z) * inv.z, z1 = (r.z.z - o.z) * inv.z;
-  const float X0 = (r.x.y - o.x) * inv.x, X1 = (r.x.w - o.x) * inv.x;
-  const float Y0 = (r.y.y - o.y) * inv.y, Y1 = (r.y.w - o.y) * inv.y;
-  const float Z0 = (r.z.y - o.z) * inv.z, Z1 = (r.z.w - o.z) * inv.z;
-  const float n0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-  const float f0 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-  const float n1 = fmaxf(fmaxf(fminf(X0, X1), fminf(Y0, Y1)), fmaxf(fminf(Z0, Z1), 0.0f));
-  const float f1 = fminf(fminf(fmaxf(X0, X1), fmaxf(Y0, Y1)), fminf(fmaxf(Z0, Z1), tmax));
-  tn0 = f0 >= n0 ? n0 : INF;
-  tn1 = f1 >= n1 ? n1 : INF;
-}
-
-// Per-lane traversal stack: kLds entries in LDS (lane-major), deeper ones in
-// a per-lane global spill area, so no entry is dropped.
-template <int kLds, int kBlock>
-struct Stack {
-  uint32_t* lds;
-  uint32_t* spill;
-  uint32_t sp, base;
-  __device__ inline void reset() { sp = base = 0; }
-  __device__ inline void push(uint32_t x) {
-    if (sp - base == kLds) {
-      spill[base] = lds[(base % kLds) * kBlock];
-      ++base;
-    }
-    lds[(sp % kLds) * kBlock] = x;
-    ++sp;
-  }
-  __device__ inline uint32_t pop() {
-    --sp;
-    if (sp >= base) return lds[(sp % kLds) * kBlock];
-    --base;
-    return spill[sp];
-  }
-};
-
 // ---------------------------------------------------------------------------
 // Traversal of the 4-wide BVH (DevBvh::nodes4).  A stack of (link, entry
 // distance) pairs; one step pops an entry or tests one triangle of the leaf
