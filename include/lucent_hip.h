@@ -69,6 +69,18 @@ lucent_status lucent_build_bvh(const float *positions, uint32_t tri_count, uint3
                                float *nodes_out, uint32_t *node_count_out, uint32_t *tri_indices_out,
                                double *bounds6_out);
 
+/* GPU builder with the same outputs and layout as lucent_build_bvh (root
+ * node 0, children of an interior node adjacent, leaves of at most max_leaf
+ * triangles, triIndices in leaf order), built on the context's device as a
+ * linear BVH: Morton codes of the triangle centroids, radix sort, the
+ * radix tree of the sorted codes and bottom-up boxes.  Its trees cost more to
+ * traverse than the SAH builder's; its build is O(n log n) on the device.
+ * max_leaf 0 means 4.  No counterpart in the reference (it builds on the CPU,
+ * in a worker for large scenes: src/math/bvh-async.ts:8-30). */
+lucent_status lucent_build_bvh_gpu(lucent_ctx *ctx, const float *positions, uint32_t tri_count, uint32_t max_leaf,
+                                   float *nodes_out, uint32_t *node_count_out, uint32_t *tri_indices_out,
+                                   double *bounds6_out);
+
 /* ------------------------------------------------------------------------
  * Scene upload — PackedScene (src/scene/scene.ts:106-134) as uploaded by
  * Renderer.loadScene (src/renderer.ts:279-312).  Arrays use the reference's

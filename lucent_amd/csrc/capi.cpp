@@ -243,6 +243,16 @@ lucent_status lucent_build_bvh(const float* positions, uint32_t tri_count, uint3
   return s;
 }
 
+lucent_status lucent_build_bvh_gpu(lucent_ctx* ctx, const float* positions, uint32_t tri_count, uint32_t max_leaf,
+                                   float* nodes_out, uint32_t* node_count_out, uint32_t* tri_indices_out,
+                                   double* bounds6_out) {
+  clear_error();
+  if (!ctx || !nodes_out || !node_count_out || (tri_count > 0 && (!positions || !tri_indices_out)))
+    return fail(LUCENT_E_INVALID, "null argument");
+  if (tri_count >= 0x7FFFFFFFu) return fail(LUCENT_E_INVALID, "too many triangles");
+  return build_bvh_gpu(ctx, positions, tri_count, max_leaf, nodes_out, node_count_out, tri_indices_out, bounds6_out);
+}
+
 lucent_status lucent_scene_from_bvh(lucent_ctx* ctx, const float* positions, uint32_t tri_count,
                                     const float* bvh_nodes, uint32_t bvh_node_count, const uint32_t* tri_indices,
                                     lucent_scene** out) {
