@@ -45,6 +45,7 @@ struct D5 {
 __device__ inline D5 dc(float v) {
   D5 r;
   r.v = v;
+#pragma unroll
   for (int i = 0; i < kNumParams; ++i) r.d[i] = 0.0f;
   return r;
 }
@@ -56,24 +57,28 @@ __device__ inline D5 dvar(float v, int i) {
 __device__ inline D5 operator+(D5 a, D5 b) {
   D5 r;
   r.v = a.v + b.v;
+#pragma unroll
   for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] + b.d[i];
   return r;
 }
 __device__ inline D5 operator-(D5 a, D5 b) {
   D5 r;
   r.v = a.v - b.v;
+#pragma unroll
   for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] - b.d[i];
   return r;
 }
 __device__ inline D5 operator*(D5 a, D5 b) {
   D5 r;
   r.v = a.v * b.v;
+#pragma unroll
   for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
   return r;
 }
 __device__ inline D5 operator*(D5 a, float s) {
   D5 r;
   r.v = a.v * s;
+#pragma unroll
   for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] * s;
   return r;
 }
@@ -85,6 +90,7 @@ __device__ inline D5 operator/(D5 a, D5 b) {
   D5 r;
   const float inv = 1.0f / b.v;
   r.v = a.v * inv;
+#pragma unroll
   for (int i = 0; i < kNumParams; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * inv;
   return r;
 }
@@ -92,6 +98,7 @@ __device__ inline D5 dsqrt(D5 a) {
   D5 r;
   r.v = sqrtf(a.v);
   const float k = r.v > 0.0f ? 0.5f / r.v : 0.0f;
+#pragma unroll
   for (int i = 0; i < kNumParams; ++i) r.d[i] = a.d[i] * k;
   return r;
 }
@@ -100,7 +107,7 @@ __device__ inline D5 one_minus(D5 a) { return dc(1.0f) - a; }
 // f of evalOpaque (bsdf.glsl:272-324) and its derivatives with respect to
 // the hit material's albedo, roughness and metallic.  The sheen energy scale
 // and the coat layering do not depend on these parameters.
-__device__ inline bool eval_opaque_grad(const PtScene& sc, const Surface& s, v3 wo, v3 wi, float f[3],
+__device__ __forceinline__ bool eval_opaque_grad(const PtScene& sc, const Surface& s, v3 wo, v3 wi, float f[3],
                                         float df[3][kNumParams]) {
   const v3 n = s.n;
   const float ndl = dot(n, wi), ndv = dot(n, wo);
@@ -144,6 +151,7 @@ __device__ inline bool eval_opaque_grad(const PtScene& sc, const Surface& s, v3 
     coat_mul = 1.0f - s.clearcoat * fresnel_schlick1(fmaxf(cv, 0.0f), 0.04f);
     coat_add = s.clearcoat * coat;
   }
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     const D5 f0 = dc(d0) * one_m + alb[c] * metal;
     const D5 F = f0 + one_minus(f0) * fres;
@@ -152,7 +160,8 @@ __device__ inline bool eval_opaque_grad(const PtScene& sc, const Surface& s, v3 
     D5 fc = (diff + spec) * scale + sheen[c];
     fc = fc * coat_mul + coat_add;
     f[c] = fc.v;
-    for (int i = 0; i < kNumParams; ++i) df[c][i] = fc.d[i];
+  #pragma unroll
+  for (int i = 0; i < kNumParams; ++i) df[c][i] = fc.d[i];
   }
   return true;
 }
@@ -164,7 +173,7 @@ struct ParamScale {
   float rough, metal;
 };
 
-__device__ inline ParamScale param_scale(const PtScene& sc, const Hit& hit) {
+__device__ __forceinline__ ParamScale param_scale(const PtScene& sc, const Hit& hit) {
   const Material m = fetch_material(sc, hit.material);
   ParamScale p{{1.0f, 1.0f, 1.0f}, 1.0f, 1.0f};
   float tu, tv;
@@ -198,21 +207,24 @@ struct GradSink {
   float* global_mat;  // fallback for materials beyond the LDS table (float atomics)
 };
 
-__device__ inline void add_material(GradSink& g, int mat, const float v[kNumParams]) {
+__device__ __forceinline__ void add_material(GradSink& g, int mat, const float v[kNumParams]) {
   if (mat < kLdsMaterials) {
-    for (int i = 0; i < kNumParams; ++i)
+  #pragma unroll
+  for (int i = 0; i < kNumParams; ++i)
       if (v[i] != 0.0f) atomicAdd(&g.lds[mat * kNumParams + i], v[i]);
   } else {
-    for (int i = 0; i < kNumParams; ++i)
+  #pragma unroll
+  for (int i = 0; i < kNumParams; ++i)
       if (v[i] != 0.0f) unsafeAtomicAdd(&g.global_mat[mat * kNumParams + i], v[i]);
   }
 }
 
-__device__ inline void add_env(const PtScene& sc, GradSink& g, float u, float v, v3 weight) {
+__device__ __forceinline__ void add_env(const PtScene& sc, GradSink& g, float u, float v, v3 weight) {
   int idx[4];
   float w[4];
   env_taps(sc, u, v, idx, w);
   float* base = g.env + g.replica * g.env_stride;
+#pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (w[k] == 0.0f) continue;
     const float s = w[k] * sc.env_intensity;
@@ -223,11 +235,12 @@ __device__ inline void add_env(const PtScene& sc, GradSink& g, float u, float v,
 }
 
 // Gradient of δL · (beta * f * Li * k) through f.
-__device__ inline void add_f_term(GradSink& g, int mat, const ParamScale& ps, v3 coeff, const float df[3][kNumParams]) {
+__device__ __forceinline__ void add_f_term(GradSink& g, int mat, const ParamScale& ps, v3 coeff, const float df[3][kNumParams]) {
   const float c[3] = {coeff.x, coeff.y, coeff.z};
   float out[kNumParams] = {0, 0, 0, 0, 0};
   for (int ch = 0; ch < 3; ++ch)
-    for (int i = 0; i < kNumParams; ++i) out[i] += c[ch] * df[ch][i];
+  #pragma unroll
+  for (int i = 0; i < kNumParams; ++i) out[i] += c[ch] * df[ch][i];
   out[0] *= ps.albedo[0];
   out[1] *= ps.albedo[1];
   out[2] *= ps.albedo[2];
@@ -238,7 +251,7 @@ __device__ inline void add_f_term(GradSink& g, int mat, const ParamScale& ps, v3
 
 // The replay of tracePath (trace.frag:136-272) accumulating d(dL · L)/dθ.
 template <class StackT>
-__device__ inline void replay(const PtScene& sc, const PtFrame& f, v3 ro, v3 rd, Rng& rng, StackT& st, v3 L, v3 dl,
+__device__ __forceinline__ void replay(const PtScene& sc, const PtFrame& f, v3 ro, v3 rd, Rng& rng, StackT& st, v3 L, v3 dl,
                               GradSink& g) {
   v3 lr = L;
   v3 beta = mk(1.f, 1.f, 1.f);
@@ -488,7 +501,10 @@ __global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, 
   *o = make_float4(prev.x + L.x * inv_spp, prev.y + L.y * inv_spp, prev.z + L.z * inv_spp, 0.0f);
 }
 
-__global__ __launch_bounds__(kAdjBlock) void k_adjoint(PtScene sc, PtFrame f, const float* __restrict__ dl_di,
+// kWaves: the occupancy the register allocation targets (waves per SIMD).
+template <int kWaves>
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_adjoint(
+    PtScene sc, PtFrame f, const float* __restrict__ dl_di,
                                                        const float4* __restrict__ recorded_L, float inv_spp,
                                                        float* __restrict__ mat_partials,
                                                        float* __restrict__ env_acc, size_t env_stride,
@@ -623,7 +639,8 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
     const float4* rec = a.recorded_radiance
                             ? reinterpret_cast<const float4*>(a.recorded_radiance) + size_t(k) * a.frame.width * a.frame.height
                             : nullptr;
-    hipLaunchKernelGGL(k_adjoint, dim3(blocks), dim3(kAdjBlock), 0, stream, sc, f, a.dl_di, rec, 1.0f / float(a.spp),
+    auto kern = a.occupancy == 3 ? k_adjoint<3> : a.occupancy == 4 ? k_adjoint<4> : k_adjoint<2>;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kAdjBlock), 0, stream, sc, f, a.dl_di, rec, 1.0f / float(a.spp),
                        mat_partials, env_acc, env_stride, mat_overflow, static_cast<uint32_t*>(ctx->spill), depth,
                        tiles_x);
     const uint32_t entries = uint32_t(min(int(sc.material_count), kLdsMaterials)) * kNumParams;

@@ -561,6 +561,7 @@ lucent_status lucent_backward(lucent_ctx* ctx, lucent_scene* scene, const lucent
   a.seed0 = seed0;
   a.dl_di = d_dl_di;
   a.recorded_radiance = d_sample_radiance;
+  a.occupancy = settings->occupancy;
   a.grad_materials = d_grad_materials;
   a.grad_env = d_grad_env;
   a.scratch = ctx->scratch;
