@@ -165,9 +165,11 @@ def lib() -> C.CDLL:
     """Loads the library once; raises if it has not been built."""
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
-            raise LucentError(-3, f"{LIB_PATH} is missing: run `python -m lucent_amd.build` first")
-        l = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | C.RTLD_GLOBAL)
+        # LUCENT_HIP_LIBRARY may point at another build of the same library.
+        path = Path(os.environ.get("LUCENT_HIP_LIBRARY", str(LIB_PATH)))
+        if not path.exists():
+            raise LucentError(-3, f"{path} is missing: run `python -m lucent_amd.build` first")
+        l = C.CDLL(str(path), mode=os.RTLD_NOW | C.RTLD_GLOBAL)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(l, name)
             fn.restype = res
