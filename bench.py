@@ -528,7 +528,8 @@ def main() -> int:
     elif args.workload == "denoise":
         d = bench_denoise(args)
         res = {"value": d["value"], "unit": d["unit"], "ms_per_step": d["value"], "dtype": "f32/f16",
-               "config": {"workload": d["metric"]}, "roofline": d["roofline"], "cpu_baseline": d["cpu_baseline"]}
+               "higher_is_better": False, "config": {"workload": d["metric"]}, "roofline": d["roofline"],
+               "cpu_baseline": d["cpu_baseline"]}
     else:
         res = bench_path_trace(args)
         if args.workload == "all":
@@ -540,7 +541,7 @@ def main() -> int:
             extra["adjoint"] = bench_adjoint(args)
     line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
+            "higher_is_better": res.pop("higher_is_better", True), "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
             "data": "synthetic", "config": res.pop("config"), "roofline": res.pop("roofline", None),
             "cpu_baseline": res.pop("cpu_baseline", None)}
     if "roofline" not in line or line["roofline"] is None:

@@ -23,7 +23,7 @@ struct lucent_film {
   uint32_t seed = 0;     // Renderer.seed
   void* base = nullptr;
   PtFilm f;
-  void* filter = nullptr;       // two RGBA32F filter targets, one remodulated RGBA32F target, 32 B/px of guides
+  void* filter = nullptr;       // two RGBA32F filter targets, the remodulated RGBA32F target, 16 B/px of guides, two variance planes
   float* denoised = nullptr;    // the target holding the last denoise result
   float* remodulated = nullptr;
 };
@@ -648,7 +648,7 @@ lucent_status lucent_denoise(lucent_ctx* ctx, lucent_film* film, const lucent_ca
   if (p->iterations > 12) return fail(LUCENT_E_INVALID, "at most 12 a-trous iterations");
   const size_t px = size_t(film->width) * film->height;
   if (!film->filter) {
-    Status s = dev_alloc(&film->filter, px * (16 * 3 + 32));
+    Status s = dev_alloc(&film->filter, px * (16 * 4 + 8));
     if (s != LUCENT_OK) return s;
   }
   DenoiseArgs a;
@@ -660,6 +660,8 @@ lucent_status lucent_denoise(lucent_ctx* ctx, lucent_film* film, const lucent_ca
   a.pong = a.ping + px * 4;
   a.remodulated = a.pong + px * 4;
   a.guides = a.remodulated + px * 4;
+  a.var_a = a.guides + px * 4;
+  a.var_b = a.var_a + px;
   a.result = &film->denoised;
   a.width = int(film->width);
   a.height = int(film->height);

@@ -5,16 +5,17 @@
 // kernel and normal / plane-distance / luminance edge stops.
 //
 // Guides are read from the film's FP16 targets; every tap is accumulated in
-// float32.  One lane per pixel, 16x8 pixel tiles per block (two wave64s), so
-// a block's taps at one offset form two 8-row spans that the L2 serves for
-// the neighbouring blocks too.  The variance pass also stores, per pixel, the
-// unit normal, the coverage flag and the world position (camera ray through
-// the pixel centre times the stored depth): the à-trous taps then read these
-// 32 bytes instead of renormalising the neighbour's normal and rebuilding
-// its position at every tap, and evaluate their edge-stopping exponentials
-// with the hardware log2/exp2 and per-pixel reciprocals.
+// float32.  The variance pass stores a 16-byte guide per pixel (unit normal,
+// depth over the length of the unnormalised camera ray) and a compact copy
+// of the variance.  Each à-trous iteration with a step S <= 16 runs on tiles
+// of 64 columns by R rows spaced S apart: every tap of the tile's outputs
+// lies on the same dilated grid, so a block stages (64+4S) x (R+4) pixels of
+// illumination and guides in LDS once and its 64 R outputs take their 24
+// taps from there.  Larger steps read the taps from global memory.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+
+#include <utility>
 
 #include "denoise.hpp"
 
@@ -50,27 +51,29 @@ __device__ inline void unit_normal(float x, float y, float z, float& ox, float& 
   }
 }
 
-// primaryDir (atrous.frag:22-26) — the pixel centre's camera ray.
-__device__ inline void primary_dir(const Cam& c, int px, int py, int w, int h, float& dx, float& dy, float& dz) {
+// primaryDir (atrous.frag:22-26) before its normalisation: the camera ray
+// through the pixel centre as fwd + right*sx + up*sy.
+__device__ inline void ray_vector(const Cam& c, int px, int py, int w, int h, float& vx, float& vy, float& vz) {
   const float ux = (float(px) + 0.5f) / float(w) * 2.0f - 1.0f;
   const float uy = (float(py) + 0.5f) / float(h) * 2.0f - 1.0f;
   const float aspect = float(w) / float(h);
   const float sx = ux * aspect * c.tan_half_fov, sy = uy * c.tan_half_fov;
-  float x = c.fwd[0] + c.right[0] * sx + c.up[0] * sy;
-  float y = c.fwd[1] + c.right[1] * sx + c.up[1] * sy;
-  float z = c.fwd[2] + c.right[2] * sx + c.up[2] * sy;
-  const float inv = rsqrtf(x * x + y * y + z * z);
-  dx = x * inv;
-  dy = y * inv;
-  dz = z * inv;
+  vx = c.fwd[0] + c.right[0] * sx + c.up[0] * sy;
+  vy = c.fwd[1] + c.right[1] * sx + c.up[1] * sy;
+  vz = c.fwd[2] + c.right[2] * sx + c.up[2] * sy;
 }
 
-// variance.frag:15-55.
+// variance.frag:15-55.  Besides the demodulated illumination and its
+// variance (out), writes per pixel the à-trous guide (unit normal, tau) with
+// tau = depth / |v|, v the unnormalised camera ray through the pixel centre,
+// so that the shading point is cam.pos + v * tau; an uncovered pixel's guide is
+// all zero.  var_out is a compact copy of the variance channel.
 __global__ __launch_bounds__(kTx* kTy) void k_variance(const float* __restrict__ color,
                                                        const uint16_t* __restrict__ albedo,
                                                        const uint16_t* __restrict__ normal,
                                                        const float* __restrict__ history, float* __restrict__ out,
-                                                       float* __restrict__ guides, int w, int h, Cam cam) {
+                                                       float* __restrict__ guides, float* __restrict__ var_out, int w,
+                                                       int h, Cam cam) {
   const int x = blockIdx.x * kTx + threadIdx.x % kTx;
   const int y = blockIdx.y * kTy + threadIdx.x / kTx;
   if (x >= w || y >= h) return;
@@ -78,18 +81,20 @@ __global__ __launch_bounds__(kTx* kTy) void k_variance(const float* __restrict__
   const float4 c = reinterpret_cast<const float4*>(color)[i];
   const float4 a = load_half4(albedo, i);
   float4* o = reinterpret_cast<float4*>(out);
-  {
-    const float4 np = load_half4(normal, i);
-    float nx, ny, nz, dx, dy, dz;
-    unit_normal(np.x, np.y, np.z, nx, ny, nz);
-    primary_dir(cam, x, y, w, h, dx, dy, dz);
-    float4* g = reinterpret_cast<float4*>(guides) + 2 * i;
-    g[0] = make_float4(nx, ny, nz, a.w < 0.5f ? 0.0f : 1.0f);
-    g[1] = make_float4(cam.pos[0] + dx * np.w, cam.pos[1] + dy * np.w, cam.pos[2] + dz * np.w, np.w);
-  }
+  float4* g = reinterpret_cast<float4*>(guides) + i;
   if (a.w < 0.5f) {
+    *g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     o[i] = make_float4(c.x, c.y, c.z, 0.0f);
+    var_out[i] = 0.0f;
     return;
+  }
+  const float4 np = load_half4(normal, i);
+  float nx, ny, nz;
+  unit_normal(np.x, np.y, np.z, nx, ny, nz);
+  {
+    float vx, vy, vz;
+    ray_vector(cam, x, y, w, h, vx, vy, vz);
+    *g = make_float4(nx, ny, nz, np.w / sqrtf(vx * vx + vy * vy + vz * vz));
   }
   const float ir = c.x / fmaxf(a.x, 1e-3f), ig = c.y / fmaxf(a.y, 1e-3f), ib = c.z / fmaxf(a.z, 1e-3f);
   const float samples = fmaxf(history[i], 1.0f);
@@ -99,9 +104,6 @@ __global__ __launch_bounds__(kTx* kTy) void k_variance(const float* __restrict__
     const float l = lum3(c.x, c.y, c.z);
     variance = fmaxf(0.0f, c.w - l * l) / (la * la) / samples;
   } else {
-    const float4 np = load_half4(normal, i);
-    float nx, ny, nz;
-    unit_normal(np.x, np.y, np.z, nx, ny, nz);
     float s1 = 0.0f, s2 = 0.0f, ws = 0.0f;
     for (int dy = -3; dy <= 3; ++dy)
       for (int dx = -3; dx <= 3; ++dx) {
@@ -126,95 +128,334 @@ __global__ __launch_bounds__(kTx* kTy) void k_variance(const float* __restrict__
     variance = fmaxf(0.0f, s2 / fmaxf(ws, 1e-6f) - m1 * m1) / samples;
   }
   o[i] = make_float4(ir, ig, ib, variance);
+  var_out[i] = variance;
 }
 
-__device__ inline float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+// ---------------------------------------------------------------------------
+// One à-trous iteration (atrous.frag:30-82).  Per tap q of centre p the
+// reference's weight is
+//   k * max(0, n_p·n_q)^sigma_n * exp(-|n_p·(Q-P)| / (reach + 1e-6))
+//                              * exp(-|l_p - l_q| / denom),
+// evaluated here as one exp2 of the summed base-2 exponents (log2 k is a
+// constant per tap).  With Q = C + v_q tau_q and v_q linear in the pixel
+// coordinates, n_p·(Q-P) = tau_q (a0 + ox ax + oy ay) - tau_p a0, so with
+// a0 + oy ay precomputed per row a tap spends two operations on the plane
+// term.  An uncovered or
+// outside neighbour has a zero normal, so its normal weight, and with it the
+// whole weight, is exp2(-inf) = 0 (for sigma_n = 0 coverage is tested
+// explicitly).
 
-// atrous.frag:30-82, with the guides of k_variance.
-__global__ __launch_bounds__(kTx* kTy) void k_atrous(const float* __restrict__ in, const float* __restrict__ guides,
-                                                     float* __restrict__ out, int w, int h, int step, float tan_half_fov,
-                                                     float sigma_l, float sigma_n, float sigma_p) {
+struct Centre {
+  float nx, ny, nz;  // unit normal
+  float ax;          // n_p·v_q = row[oy+2] + ox*ax
+  float row[5];      // a0 + oy*ay
+  float d;           // tau_p * a0
+  float lp;          // luminance of the centre illumination
+  float il;          // log2(e) / (sigma_l sqrt(blurred variance) + 1e-6)
+  float irl[5];      // log2(e) / (reach + 1e-6) for |offset|^2 = 1, 2, 4, 5, 8
+  float sr, sg, sb, svar, wsum;
+};
+
+constexpr float kLog2e = 1.4426950408889634f;
+
+__device__ inline void centre_begin(Centre& c, float4 cp, float4 gp, float var_blur, const Cam& cam, int x, int y,
+                                    int w, int h, int step, float sigma_l, float sigma_p) {
+  c.nx = gp.x;
+  c.ny = gp.y;
+  c.nz = gp.z;
+  float vx, vy, vz;
+  ray_vector(cam, x, y, w, h, vx, vy, vz);
+  const float a0 = c.nx * vx + c.ny * vy + c.nz * vz;
+  const float aspect = float(w) / float(h);
+  const float dsx = 2.0f / float(w) * aspect * cam.tan_half_fov * float(step);
+  const float dsy = 2.0f / float(h) * cam.tan_half_fov * float(step);
+  c.ax = (c.nx * cam.right[0] + c.ny * cam.right[1] + c.nz * cam.right[2]) * dsx;
+  const float ay = (c.nx * cam.up[0] + c.ny * cam.up[1] + c.nz * cam.up[2]) * dsy;
+  c.row[0] = a0 - 2.0f * ay;
+  c.row[1] = a0 - ay;
+  c.row[2] = a0;
+  c.row[3] = a0 + ay;
+  c.row[4] = a0 + 2.0f * ay;
+  c.d = gp.w * a0;
+  // footprint = 2 tan(fov/2) depth / H with depth = tau_p |v_p|.
+  const float depth = gp.w * sqrtf(vx * vx + vy * vy + vz * vz);
+  const float footprint = 2.0f * cam.tan_half_fov * depth / float(h);
+  const float base = sigma_p * footprint * float(step);
+  c.irl[0] = kLog2e / (base * 1.0f + 1e-6f);
+  c.irl[1] = kLog2e / (base * 1.41421356f + 1e-6f);
+  c.irl[2] = kLog2e / (base * 2.0f + 1e-6f);
+  c.irl[3] = kLog2e / (base * 2.23606798f + 1e-6f);
+  c.irl[4] = kLog2e / (base * 2.82842712f + 1e-6f);
+  c.il = kLog2e / (sigma_l * sqrtf(fmaxf(var_blur, 0.0f)) + 1e-6f);
+  c.lp = lum3(cp.x, cp.y, cp.z);
+  constexpr float h0 = 0.375f;
+  c.sr = cp.x * (h0 * h0);
+  c.sg = cp.y * (h0 * h0);
+  c.sb = cp.z * (h0 * h0);
+  c.svar = cp.w * (h0 * h0) * (h0 * h0);
+  c.wsum = h0 * h0;
+}
+
+__host__ __device__ constexpr float log2_tap(int o) { return o == 0 ? -1.4150374992788437f : o == 1 ? -2.0f : -4.0f; }
+__host__ __device__ constexpr int iabs(int v) { return v < 0 ? -v : v; }
+__host__ __device__ constexpr int reach_slot(int ox, int oy) {
+  return ox * ox + oy * oy == 1 ? 0 : ox * ox + oy * oy == 2 ? 1 : ox * ox + oy * oy == 4 ? 2 : ox * ox + oy * oy == 5 ? 3 : 4;
+}
+
+template <int OX, int OY, bool kNormalStop>
+__device__ __forceinline__ void tap(Centre& c, float4 cq, float4 gq, float sigma_n) {
+  constexpr float log2k = log2_tap(iabs(OX)) + log2_tap(iabs(OY));
+  const float dot = c.nx * gq.x + c.ny * gq.y + c.nz * gq.z;
+  float e;
+  if constexpr (kNormalStop) {
+    e = sigma_n * __builtin_amdgcn_logf(fmaxf(0.0f, dot)) + log2k;
+  } else {
+    const bool covered = gq.x != 0.0f || gq.y != 0.0f || gq.z != 0.0f;
+    e = covered ? log2k : -INFINITY;
+  }
+  float nv = c.row[OY + 2];
+  if constexpr (OX == 1) nv += c.ax;
+  if constexpr (OX == -1) nv -= c.ax;
+  if constexpr (OX == 2 || OX == -2) nv = fmaf(float(OX), c.ax, nv);
+  const float plane = fabsf(fmaf(gq.w, nv, -c.d));
+  // |l_q - l_p| as one multiply-add chain from -l_p.
+  const float dl = fabsf(fmaf(cq.z, 0.0722f, fmaf(cq.y, 0.7152f, fmaf(cq.x, 0.2126f, -c.lp))));
+  e -= plane * c.irl[reach_slot(OX, OY)];
+  e -= dl * c.il;
+  const float wgt = __builtin_amdgcn_exp2f(e);
+  c.sr += cq.x * wgt;
+  c.sg += cq.y * wgt;
+  c.sb += cq.z * wgt;
+  c.svar += cq.w * (wgt * wgt);
+  c.wsum += wgt;
+}
+
+__device__ inline void centre_store(const Centre& c, size_t i, float4* out, float* var_out, const uint16_t* albedo,
+                                    float4* remodulated) {
+  const float inv = 1.0f / c.wsum;
+  const float4 r = make_float4(c.sr * inv, c.sg * inv, c.sb * inv, c.svar * (inv * inv));
+  out[i] = r;
+  var_out[i] = r.w;
+  if (remodulated) {
+    // display.frag:32-40 with weight 1: illumination times max(albedo, 1e-3).
+    const float4 a = load_half4(albedo, i);
+    remodulated[i] = make_float4(r.x * fmaxf(a.x, 1e-3f), r.y * fmaxf(a.y, 1e-3f), r.z * fmaxf(a.z, 1e-3f), 1.0f);
+  }
+}
+
+__device__ inline void passthrough(float4 cp, size_t i, float4* out, float* var_out, const uint16_t* albedo,
+                                   float4* remodulated) {
+  out[i] = cp;
+  var_out[i] = cp.w;
+  if (remodulated) {
+    const float4 a = load_half4(albedo, i);
+    remodulated[i] = make_float4(cp.x * fmaxf(a.x, 1e-3f), cp.y * fmaxf(a.y, 1e-3f), cp.z * fmaxf(a.z, 1e-3f), 1.0f);
+  }
+}
+
+// 3x3 binomial blur of the variance over the in-image neighbours
+// (atrous.frag:43-54); v[dy+1][dx+1] are the neighbours' variances (zero
+// outside the image).
+__device__ inline float blur_variance(const float v[3][3], int x, int y, int w, int h) {
+  const float kx[3] = {x > 0 ? 0.25f : 0.0f, 0.5f, x + 1 < w ? 0.25f : 0.0f};
+  const float ky[3] = {y > 0 ? 0.25f : 0.0f, 0.5f, y + 1 < h ? 0.25f : 0.0f};
+  float var = 0.0f;
+#pragma unroll
+  for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) var += v[dy][dx] * (kx[dx] * ky[dy]);
+  return var / ((kx[0] + kx[1] + kx[2]) * (ky[0] + ky[1] + ky[2]));
+}
+
+// Tile of the LDS kernel: 64 consecutive columns times R rows spaced by the
+// step (rows y0, y0+S, ..., y0+(R-1)S), so that every tap of every output
+// lies on the tile's own dilated grid: R+4 staged rows of 64+4S columns.
+// 4 waves; lane = column, wave = row.  Four rows stage 2x (step 1) to 4x
+// (step 16) the pixels they filter but leave LDS for 4-6 blocks per CU,
+// which measured faster than taller tiles with less halo.
+constexpr int kTileW = 64, kWaves = 4;
+
+__host__ __device__ constexpr int tile_rows(int) { return 4; }
+
+template <int S>
+struct AtrousTile {
+  static constexpr int R = tile_rows(S);
+  static constexpr int cols = kTileW + 4 * S;
+  static constexpr int rows = R + 4;
+  static constexpr int vcols = kTileW + 2;
+};
+
+template <int S, bool kNormalStop>
+__global__ __launch_bounds__(kWaves * 64) void k_atrous_tile(const float4* __restrict__ in,
+                                                             const float4* __restrict__ guides,
+                                                             const float* __restrict__ var_in,
+                                                             float4* __restrict__ out, float* __restrict__ var_out,
+                                                             const uint16_t* __restrict__ albedo,
+                                                             float4* __restrict__ remodulated, int w, int h, Cam cam,
+                                                             float sigma_l, float sigma_n, float sigma_p) {
+  using T = AtrousTile<S>;
+  __shared__ float4 sc[T::rows * T::cols];
+  __shared__ float4 sg[T::rows * T::cols];
+  __shared__ float sv[T::R * 3 * T::vcols];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * kTileW;
+  const int y0 = int(blockIdx.y % S) + int(blockIdx.y / S) * T::R * S;
+  for (int j = wave; j < T::rows; j += kWaves) {
+    const int gy = y0 + (j - 2) * S;
+    for (int cc = lane; cc < T::cols; cc += 64) {
+      const int gx = x0 - 2 * S + cc;
+      float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g = c;
+      if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
+        const size_t q = size_t(gy) * w + gx;
+        c = in[q];
+        g = guides[q];
+      }
+      sc[j * T::cols + cc] = c;
+      sg[j * T::cols + cc] = g;
+    }
+  }
+  for (int j = wave; j < T::R * 3; j += kWaves) {
+    const int gy = y0 + (j / 3) * S + (j % 3) - 1;
+    for (int cc = lane; cc < T::vcols; cc += 64) {
+      const int gx = x0 - 1 + cc;
+      sv[j * T::vcols + cc] = gx >= 0 && gx < w && gy >= 0 && gy < h ? var_in[size_t(gy) * w + gx] : 0.0f;
+    }
+  }
+  __syncthreads();
+  const int x = x0 + lane;
+  for (int r = wave; r < T::R; r += kWaves) {
+    const int y = y0 + r * S;
+    if (x >= w || y >= h) continue;
+    const size_t i = size_t(y) * w + x;
+    const int ci = (r + 2) * T::cols + lane + 2 * S;
+    const float4 cp = sc[ci];
+    const float4 gp = sg[ci];
+    if (gp.x == 0.0f && gp.y == 0.0f && gp.z == 0.0f) {
+      passthrough(cp, i, out, var_out, albedo, remodulated);
+      continue;
+    }
+    float v[3][3];
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) v[dy][dx] = sv[(r * 3 + dy) * T::vcols + lane + dx];
+    Centre c;
+    centre_begin(c, cp, gp, blur_variance(v, x, y, w, h), cam, x, y, w, h, S, sigma_l, sigma_p);
+    const float4* pc = sc + ci;
+    const float4* pg = sg + ci;
+#define LUCENT_TAP(OX, OY) tap<OX, OY, kNormalStop>(c, pc[OY * T::cols + OX * S], pg[OY * T::cols + OX * S], sigma_n)
+#define LUCENT_ROW(OY) \
+  LUCENT_TAP(-2, OY);  \
+  LUCENT_TAP(-1, OY);  \
+  LUCENT_TAP(0, OY);   \
+  LUCENT_TAP(1, OY);   \
+  LUCENT_TAP(2, OY)
+    LUCENT_ROW(-2);
+    LUCENT_ROW(-1);
+    LUCENT_TAP(-2, 0);
+    LUCENT_TAP(-1, 0);
+    LUCENT_TAP(1, 0);
+    LUCENT_TAP(2, 0);
+    LUCENT_ROW(1);
+    LUCENT_ROW(2);
+#undef LUCENT_ROW
+#undef LUCENT_TAP
+    centre_store(c, i, out, var_out, albedo, remodulated);
+  }
+}
+
+// Steps above the tile kernel's (iterations beyond 5): one lane per pixel,
+// taps read from global memory, outside taps skipped.
+template <bool kNormalStop>
+__global__ __launch_bounds__(kTx* kTy) void k_atrous_direct(const float4* __restrict__ in,
+                                                            const float4* __restrict__ guides,
+                                                            const float* __restrict__ var_in,
+                                                            float4* __restrict__ out, float* __restrict__ var_out,
+                                                            const uint16_t* __restrict__ albedo,
+                                                            float4* __restrict__ remodulated, int w, int h, int step,
+                                                            Cam cam, float sigma_l, float sigma_n, float sigma_p) {
   const int x = blockIdx.x * kTx + threadIdx.x % kTx;
   const int y = blockIdx.y * kTy + threadIdx.x / kTx;
   if (x >= w || y >= h) return;
   const size_t i = size_t(y) * w + x;
-  const float4* I = reinterpret_cast<const float4*>(in);
-  const float4* G = reinterpret_cast<const float4*>(guides);
-  const float4 cp = I[i];
-  float4* O = reinterpret_cast<float4*>(out);
-  const float4 gn = G[2 * i];
-  if (gn.w < 0.5f) {
-    O[i] = cp;
+  const float4 cp = in[i];
+  const float4 gp = guides[i];
+  if (gp.x == 0.0f && gp.y == 0.0f && gp.z == 0.0f) {
+    passthrough(cp, i, out, var_out, albedo, remodulated);
     return;
   }
-  const float4 gp = G[2 * i + 1];
-  // footprint = 2 tan(fov/2) depth / H, depth = the stored primary hit distance.
-  const float footprint = 2.0f * tan_half_fov * gp.w / float(h);
-
-  // 3x3 binomial blur of the variance (atrous.frag:43-54).
-  float var = 0.0f, wv = 0.0f;
-  for (int oy = -1; oy <= 1; ++oy)
-    for (int ox = -1; ox <= 1; ++ox) {
-      const int qx = x + ox, qy = y + oy;
-      if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
-      const float k = (ox == 0 ? 0.5f : 0.25f) * (oy == 0 ? 0.5f : 0.25f);
-      var += I[size_t(qy) * w + qx].w * k;
-      wv += k;
+  float v[3][3];
+  for (int dy = 0; dy < 3; ++dy)
+    for (int dx = 0; dx < 3; ++dx) {
+      const int qx = x + dx - 1, qy = y + dy - 1;
+      v[dy][dx] = qx >= 0 && qx < w && qy >= 0 && qy < h ? var_in[size_t(qy) * w + qx] : 0.0f;
     }
-  const float inv_denom = 1.0f / (sigma_l * sqrtf(fmaxf(var / wv, 0.0f)) + 1e-6f);
-  // 1 / (sigma_p * footprint * step * |offset| + 1e-6) for |offset|^2 = 1, 2, 4, 5, 8.
-  const float base = sigma_p * footprint * float(step);
-  const float inv_reach[9] = {0.0f,
-                              1.0f / (base * 1.0f + 1e-6f),
-                              1.0f / (base * 1.41421356f + 1e-6f),
-                              0.0f,
-                              1.0f / (base * 2.0f + 1e-6f),
-                              1.0f / (base * 2.23606798f + 1e-6f),
-                              0.0f,
-                              0.0f,
-                              1.0f / (base * 2.82842712f + 1e-6f)};
-  const float lp = lum3(cp.x, cp.y, cp.z);
-  const float hk[3] = {0.375f, 0.25f, 0.0625f};
-  float sr = cp.x * (hk[0] * hk[0]), sg = cp.y * (hk[0] * hk[0]), sb = cp.z * (hk[0] * hk[0]);
-  float svar = cp.w * (hk[0] * hk[0]) * (hk[0] * hk[0]);
-  float wsum = hk[0] * hk[0];
-  for (int oy = -2; oy <= 2; ++oy)
-    for (int ox = -2; ox <= 2; ++ox) {
-      if (ox == 0 && oy == 0) continue;
-      const int qx = x + ox * step, qy = y + oy * step;
-      if (qx < 0 || qy < 0 || qx >= w || qy >= h) continue;
-      const size_t q = size_t(qy) * w + qx;
-      const float4 qn = G[2 * q];
-      if (qn.w < 0.5f) continue;
-      const float4 qp = G[2 * q + 1];
-      const float4 cq = I[q];
-      const float k = hk[ox < 0 ? -ox : ox] * hk[oy < 0 ? -oy : oy];
-      const float c = fmaxf(0.0f, gn.x * qn.x + gn.y * qn.y + gn.z * qn.z);
-      const float wn = sigma_n == 0.0f ? 1.0f : __builtin_amdgcn_exp2f(sigma_n * __builtin_amdgcn_logf(c));
-      const float plane = fabsf(gn.x * (qp.x - gp.x) + gn.y * (qp.y - gp.y) + gn.z * (qp.z - gp.z));
-      const float wp = fast_exp(-plane * inv_reach[ox * ox + oy * oy]);
-      const float wl = fast_exp(-fabsf(lp - lum3(cq.x, cq.y, cq.z)) * inv_denom);
-      const float wgt = k * wn * wp * wl;
-      sr += cq.x * wgt;
-      sg += cq.y * wgt;
-      sb += cq.z * wgt;
-      svar += cq.w * wgt * wgt;
-      wsum += wgt;
-    }
-  O[i] = make_float4(sr / wsum, sg / wsum, sb / wsum, svar / (wsum * wsum));
+  Centre c;
+  centre_begin(c, cp, gp, blur_variance(v, x, y, w, h), cam, x, y, w, h, step, sigma_l, sigma_p);
+  const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#define LUCENT_TAP(OX, OY)                                                                        \
+  {                                                                                               \
+    const int qx = x + OX * step, qy = y + OY * step;                                             \
+    const bool inside = qx >= 0 && qx < w && qy >= 0 && qy < h;                                   \
+    const size_t q = inside ? size_t(qy) * w + qx : i;                                            \
+    tap<OX, OY, kNormalStop>(c, inside ? in[q] : zero, inside ? guides[q] : zero, sigma_n);       \
+  }
+#define LUCENT_ROW(OY) \
+  LUCENT_TAP(-2, OY);  \
+  LUCENT_TAP(-1, OY);  \
+  LUCENT_TAP(0, OY);   \
+  LUCENT_TAP(1, OY);   \
+  LUCENT_TAP(2, OY)
+  LUCENT_ROW(-2);
+  LUCENT_ROW(-1);
+  LUCENT_TAP(-2, 0);
+  LUCENT_TAP(-1, 0);
+  LUCENT_TAP(1, 0);
+  LUCENT_TAP(2, 0);
+  LUCENT_ROW(1);
+  LUCENT_ROW(2);
+#undef LUCENT_ROW
+#undef LUCENT_TAP
+  centre_store(c, i, out, var_out, albedo, remodulated);
 }
 
-// display.frag:32-40 with weight 1: denoised illumination times max(albedo, 1e-3).
-__global__ __launch_bounds__(kTx* kTy) void k_remodulate(const float* __restrict__ in, const uint16_t* __restrict__ albedo,
-                                                         float* __restrict__ out, int w, int h) {
-  const int x = blockIdx.x * kTx + threadIdx.x % kTx;
-  const int y = blockIdx.y * kTy + threadIdx.x / kTx;
-  if (x >= w || y >= h) return;
-  const size_t i = size_t(y) * w + x;
-  const float4 d = reinterpret_cast<const float4*>(in)[i];
-  const float4 a = load_half4(albedo, i);
-  reinterpret_cast<float4*>(out)[i] =
-      make_float4(d.x * fmaxf(a.x, 1e-3f), d.y * fmaxf(a.y, 1e-3f), d.z * fmaxf(a.z, 1e-3f), 1.0f);
+struct PassArgs {
+  const float4* in;
+  const float4* guides;
+  const float* var_in;
+  float4* out;
+  float* var_out;
+  const uint16_t* albedo;
+  float4* remodulated;
+  int w, h, step;
+  Cam cam;
+  float sigma_l, sigma_n, sigma_p;
+};
+
+template <int S, bool kNormalStop>
+void launch_tile(const PassArgs& p, hipStream_t stream) {
+  const int rows_per_class = (p.h + S - 1) / S;
+  const int R = AtrousTile<S>::R;
+  const dim3 grid((p.w + kTileW - 1) / kTileW, S * ((rows_per_class + R - 1) / R));
+  hipLaunchKernelGGL((k_atrous_tile<S, kNormalStop>), grid, dim3(kWaves * 64), 0, stream, p.in, p.guides, p.var_in,
+                     p.out, p.var_out, p.albedo, p.remodulated, p.w, p.h, p.cam, p.sigma_l, p.sigma_n, p.sigma_p);
+}
+
+template <bool kNormalStop>
+void launch_pass(const PassArgs& p, hipStream_t stream) {
+  switch (p.step) {
+    case 1: return launch_tile<1, kNormalStop>(p, stream);
+    case 2: return launch_tile<2, kNormalStop>(p, stream);
+    case 4: return launch_tile<4, kNormalStop>(p, stream);
+    case 8: return launch_tile<8, kNormalStop>(p, stream);
+    case 16: return launch_tile<16, kNormalStop>(p, stream);
+    default: {
+      const dim3 grid((p.w + kTx - 1) / kTx, (p.h + kTy - 1) / kTy);
+      hipLaunchKernelGGL((k_atrous_direct<kNormalStop>), grid, dim3(kTx * kTy), 0, stream, p.in, p.guides, p.var_in,
+                         p.out, p.var_out, p.albedo, p.remodulated, p.w, p.h, p.step, p.cam, p.sigma_l, p.sigma_n,
+                         p.sigma_p);
+    }
+  }
 }
 
 }  // namespace
@@ -232,19 +473,35 @@ Status launch_denoise(const DenoiseArgs& a, void* stream_) {
   }
   cam.tan_half_fov = a.tan_half_fov;
   hipLaunchKernelGGL(k_variance, grid, block, 0, stream, a.color, a.albedo, a.normal, a.history, a.ping, a.guides,
-                     a.width, a.height, cam);
+                     a.var_a, a.width, a.height, cam);
+  PassArgs p;
+  p.guides = reinterpret_cast<const float4*>(a.guides);
+  p.albedo = a.albedo;
+  p.w = a.width;
+  p.h = a.height;
+  p.cam = cam;
+  p.sigma_l = a.sigma_l;
+  p.sigma_n = a.sigma_n;
+  p.sigma_p = a.sigma_p;
   float* read = a.ping;
   float* write = a.pong;
+  float* var_read = a.var_a;
+  float* var_write = a.var_b;
   for (uint32_t it = 0; it < a.iterations; ++it) {
-    hipLaunchKernelGGL(k_atrous, grid, block, 0, stream, read, a.guides, write, a.width, a.height, int(1u << it),
-                       a.tan_half_fov, a.sigma_l, a.sigma_n, a.sigma_p);
-    float* t = read;
-    read = write;
-    write = t;
+    p.in = reinterpret_cast<const float4*>(read);
+    p.out = reinterpret_cast<float4*>(write);
+    p.var_in = var_read;
+    p.var_out = var_write;
+    p.step = int(1u << it);
+    p.remodulated = it + 1 == a.iterations ? reinterpret_cast<float4*>(a.remodulated) : nullptr;
+    if (a.sigma_n == 0.0f)
+      launch_pass<false>(p, stream);
+    else
+      launch_pass<true>(p, stream);
+    std::swap(read, write);
+    std::swap(var_read, var_write);
   }
   *a.result = read;
-  if (a.remodulated)
-    hipLaunchKernelGGL(k_remodulate, grid, block, 0, stream, read, a.albedo, a.remodulated, a.width, a.height);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "denoise launch");
   return LUCENT_OK;

@@ -14,8 +14,10 @@ struct DenoiseArgs {
   const float* history = nullptr;    // film history (R32F)
   float* ping = nullptr;             // filter target A (RGBA32F)
   float* pong = nullptr;             // filter target B (RGBA32F)
-  float* remodulated = nullptr;      // optional RGBA32F output: illumination * albedo
-  float* guides = nullptr;           // 8 floats per pixel: unit normal + coverage, world position
+  float* remodulated = nullptr;      // RGBA32F output of the last iteration: illumination * albedo
+  float* guides = nullptr;           // 4 floats per pixel: unit normal (zero if uncovered), depth / |ray vector|
+  float* var_a = nullptr;            // compact variance planes (1 float per pixel)
+  float* var_b = nullptr;
   float** result = nullptr;          // receives the target holding the final iteration
   int width = 0, height = 0;
   uint32_t iterations = 5;
