@@ -300,6 +300,23 @@ lucent_status lucent_trace_samples(lucent_ctx *ctx, lucent_scene *scene, lucent_
                                    const lucent_camera *camera, const lucent_render_settings *settings,
                                    uint32_t count, void *stream);
 
+/* The first sample after a camera move with temporal reuse (replaces the
+ * uReproject = 1 draw of Renderer.traceSamples, renderer.ts:441-451 and
+ * 715-730, and trace.frag:274-355): one sample of every pixel of `film`
+ * (seed film->seed, camera `camera`) whose running means restart from
+ * `prev`'s, reprojected from `prev_camera`'s view by a bilinear fetch over
+ * the previous pixels that saw the same surface (or the same sky), with the
+ * pixel's history capped at history_cap times the primary surface's reuse
+ * factor (uHistoryCap; the reference uses 8).  Disoccluded pixels start
+ * over.  `prev` is typically `film` itself (its targets are first copied to
+ * the context's scratch memory) but may be another film of any size, e.g.
+ * the previous preview resolution.  Afterwards film's sample count is
+ * min(prev's, history_cap) + 1 (renderer.ts:445).  Asynchronous on `stream`. */
+lucent_status lucent_trace_reprojected(lucent_ctx *ctx, lucent_scene *scene, lucent_film *film,
+                                       const lucent_camera *camera, const lucent_film *prev,
+                                       const lucent_camera *prev_camera, float history_cap,
+                                       const lucent_render_settings *settings, void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched multi-scene tracing: for every item, `samples` samples of every
  * pixel of its film (seeds continue from the film's seed, history advances by
@@ -350,6 +367,34 @@ lucent_status lucent_denoise(lucent_ctx *ctx, lucent_film *film, const lucent_ca
 /* Either pointer may be NULL; each receives 4 floats per pixel. */
 lucent_status lucent_film_read_denoised(lucent_ctx *ctx, const lucent_film *film, float *illum_var4,
                                         float *radiance4);
+
+/* ------------------------------------------------------------------------
+ * Presentation (display.frag:1-68, Renderer.render's display draw,
+ * renderer.ts:572-593): the film's running mean, blended with the denoised
+ * illumination times max(albedo, 1e-3) by denoise_weight where covered,
+ * Catmull-Rom upsampled when the view is larger than the film (the moving
+ * preview), times exposure, tone mapped, sRGB encoded, dithered with
+ * interleaved gradient noise and stored as RGBA8 (clamp, x255, round to
+ * nearest even).  Rows bottom-up, like the film.
+ * ---------------------------------------------------------------------- */
+#define LUCENT_TONEMAP_ACES 0u
+#define LUCENT_TONEMAP_REINHARD 1u
+#define LUCENT_TONEMAP_NONE 2u
+
+typedef struct lucent_display_params {
+  float denoise_weight; /* [0, 1]; > 0 needs lucent_denoise first (denoiseWeight, renderer.ts:143-144) */
+  float exposure;       /* settings.exposure, default 1 */
+  uint32_t tonemap;     /* LUCENT_TONEMAP_* (settings.tonemap, default ACES) */
+} lucent_display_params;
+
+/* Into device memory d_rgba8 (4 * view_w * view_h bytes); asynchronous on `stream`. */
+lucent_status lucent_display(lucent_ctx *ctx, const lucent_film *film, const lucent_display_params *params,
+                             uint32_t view_w, uint32_t view_h, uint8_t *d_rgba8, void *stream);
+/* Renderer.screenshot (renderer.ts:856-862) without the PNG encoding: the
+ * displayed image into host memory rgba8; synchronous on the context's
+ * stream (uses the context's scratch memory). */
+lucent_status lucent_screenshot(lucent_ctx *ctx, const lucent_film *film, const lucent_display_params *params,
+                                uint32_t view_w, uint32_t view_h, uint8_t *rgba8);
 
 /* ------------------------------------------------------------------------
  * Differentiable rendering.  The forward pass is lucent_trace_samples; the

@@ -109,6 +109,10 @@ class DenoiseParamsStruct(C.Structure):
     _fields_ = [("sigma_l", C.c_float), ("sigma_n", C.c_float), ("sigma_p", C.c_float), ("iterations", C.c_uint32)]
 
 
+class DisplayParamsStruct(C.Structure):
+    _fields_ = [("denoise_weight", C.c_float), ("exposure", C.c_float), ("tonemap", C.c_uint32)]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _U32 = C.c_uint32
@@ -146,6 +150,10 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_autotune_tiles": (_I32, [_P, _P, _U32, C.POINTER(RenderSettingsStruct), _U32, C.POINTER(_U32), _P]),
     "lucent_denoise": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(DenoiseParamsStruct), _P]),
     "lucent_film_read_denoised": (_I32, [_P, _P, _P, _P]),
+    "lucent_trace_reprojected": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), _P, C.POINTER(CameraStruct), C.c_float,
+                                        C.POINTER(RenderSettingsStruct), _P]),
+    "lucent_display": (_I32, [_P, _P, C.POINTER(DisplayParamsStruct), _U32, _U32, _P, _P]),
+    "lucent_screenshot": (_I32, [_P, _P, C.POINTER(DisplayParamsStruct), _U32, _U32, _P]),
     "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,
                                 _U32, _P, _P, _P, _P, _P]),
     "lucent_forward": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _P,

@@ -10,6 +10,7 @@
 
 #include "adjoint.hpp"
 #include "denoise.hpp"
+#include "display.hpp"
 #include "render_host.hpp"
 #include "scene.hpp"
 #include "wavefront.hpp"
@@ -317,6 +318,52 @@ static Status check_traceable(const lucent_scene* scene) {
   if (!scene->lighting_set) return fail(LUCENT_E_INVALID, "call lucent_scene_set_lighting before tracing");
   if (scene->pt.material_count == 0 && scene->bvh.tri_count > 0)
     return fail(LUCENT_E_INVALID, "scene has triangles but no materials (use lucent_scene_upload)");
+  return LUCENT_OK;
+}
+
+lucent_status lucent_trace_reprojected(lucent_ctx* ctx, lucent_scene* scene, lucent_film* film,
+                                       const lucent_camera* cam, const lucent_film* prev,
+                                       const lucent_camera* prev_cam, float history_cap,
+                                       const lucent_render_settings* settings, void* stream) {
+  clear_error();
+  if (!ctx || !scene || !film || !cam || !prev || !prev_cam || !settings) return fail(LUCENT_E_INVALID, "null argument");
+  if (!(history_cap >= 0.0f) || !std::isfinite(history_cap)) return fail(LUCENT_E_INVALID, "history_cap must be finite and >= 0");
+  Status s = check_traceable(scene);
+  if (s != LUCENT_OK) return s;
+  void* st = stream ? stream : ctx->stream;
+  PrevView pv;
+  pv.width = prev->width;
+  pv.height = prev->height;
+  pv.history_cap = history_cap;
+  pv.film = prev->f;
+  if (prev == film) {
+    // The sample reads neighbours of the previous means: trace from a copy.
+    const size_t px = size_t(film->width) * film->height, bytes = px * (16 + 8 + 8 + 4);
+    if ((s = ensure_scratch(ctx, bytes)) != LUCENT_OK) return s;
+    const hipError_t e = hipMemcpyAsync(ctx->scratch, film->base, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(st));
+    if (e != hipSuccess) return fail_hip(e, "hipMemcpyAsync(previous film)");
+    char* b = static_cast<char*>(ctx->scratch);
+    pv.film.color = reinterpret_cast<float*>(b);
+    pv.film.albedo = reinterpret_cast<uint16_t*>(b + px * 16);
+    pv.film.normal = reinterpret_cast<uint16_t*>(b + px * 24);
+    pv.film.history = reinterpret_cast<float*>(b + px * 32);
+  }
+  double fwd[3], right[3], up[3], tan_half;
+  camera_basis(*prev_cam, fwd, right, up, &tan_half);
+  for (int i = 0; i < 3; ++i) {
+    pv.cam_pos[i] = float(prev_cam->position[i]);
+    pv.cam_fwd[i] = float(fwd[i]);
+    pv.cam_right[i] = float(right[i]);
+    pv.cam_up[i] = float(up[i]);
+  }
+  pv.tan_half_fov = float(tan_half);
+  PtFrame f;
+  fill_frame(cam, settings, film->width, film->height, &f);
+  f.seed = film->seed;
+  if ((s = launch_trace_reprojected(ctx, scene->pt, f, film->f, pv, st)) != LUCENT_OK) return s;
+  film->seed++;
+  const uint32_t cap = history_cap >= 4294967295.0f ? 4294967295u : uint32_t(history_cap);
+  film->samples = std::min<uint32_t>(prev->samples, cap) + 1;
   return LUCENT_OK;
 }
 
@@ -680,6 +727,54 @@ lucent_status lucent_denoise(lucent_ctx* ctx, lucent_film* film, const lucent_ca
   a.tan_half_fov = float(tan_half);
   film->remodulated = a.remodulated;
   return launch_denoise(a, stream ? stream : ctx->stream);
+}
+
+static Status display_args(const lucent_film* film, const lucent_display_params* p, uint32_t view_w, uint32_t view_h,
+                           DisplayArgs* a) {
+  if (view_w == 0 || view_h == 0 || uint64_t(view_w) * view_h > (1ull << 28))
+    return fail(LUCENT_E_INVALID, "view size must be positive and at most 2^28 pixels");
+  if (p->tonemap > 2) return fail(LUCENT_E_INVALID, "tonemap must be 0 (ACES), 1 (Reinhard) or 2 (none)");
+  if (!(p->denoise_weight >= 0.0f && p->denoise_weight <= 1.0f))
+    return fail(LUCENT_E_INVALID, "denoise_weight must lie in [0, 1]");
+  if (p->denoise_weight > 0.0f && !film->denoised)
+    return fail(LUCENT_E_INVALID, "denoise_weight > 0 needs a lucent_denoise of the film first");
+  a->color = film->f.color;
+  a->albedo = film->f.albedo;
+  a->denoised = p->denoise_weight > 0.0f ? film->denoised : nullptr;
+  a->render_w = int(film->width);
+  a->render_h = int(film->height);
+  a->view_w = int(view_w);
+  a->view_h = int(view_h);
+  a->denoise_weight = p->denoise_weight;
+  a->exposure = p->exposure;
+  a->tonemap = int(p->tonemap);
+  return LUCENT_OK;
+}
+
+lucent_status lucent_display(lucent_ctx* ctx, const lucent_film* film, const lucent_display_params* p, uint32_t view_w,
+                             uint32_t view_h, uint8_t* d_rgba8, void* stream) {
+  clear_error();
+  if (!ctx || !film || !p || !d_rgba8) return fail(LUCENT_E_INVALID, "null argument");
+  DisplayArgs a;
+  Status s = display_args(film, p, view_w, view_h, &a);
+  if (s != LUCENT_OK) return s;
+  a.out = d_rgba8;
+  return launch_display(a, stream ? stream : ctx->stream);
+}
+
+lucent_status lucent_screenshot(lucent_ctx* ctx, const lucent_film* film, const lucent_display_params* p,
+                                uint32_t view_w, uint32_t view_h, uint8_t* rgba8) {
+  clear_error();
+  if (!ctx || !film || !p || !rgba8) return fail(LUCENT_E_INVALID, "null argument");
+  DisplayArgs a;
+  Status s = display_args(film, p, view_w, view_h, &a);
+  if (s != LUCENT_OK) return s;
+  const size_t bytes = size_t(view_w) * view_h * 4;
+  if ((s = ensure_scratch(ctx, bytes)) != LUCENT_OK) return s;
+  a.out = static_cast<uint8_t*>(ctx->scratch);
+  if ((s = launch_display(a, ctx->stream)) != LUCENT_OK) return s;
+  if ((s = dev_download(rgba8, ctx->scratch, bytes, ctx->stream)) != LUCENT_OK) return s;
+  return stream_sync(ctx->stream);
 }
 
 lucent_status lucent_film_read_denoised(lucent_ctx* ctx, const lucent_film* film, float* illum_var4,

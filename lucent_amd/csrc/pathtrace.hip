@@ -49,6 +49,99 @@ __global__ __launch_bounds__(kPtBlock) void k_trace_sample(PtScene sc, PtFrame f
   pt::accumulate(film, idx, c, g);
 }
 
+// projectToPrevious (trace.frag:274-282): the continuous pixel position of
+// d (a point relative to the previous camera, or a direction) in the
+// previous view, if it lies inside it.
+__device__ inline bool project_to_previous(const PrevView& pv, pt::v3 d, float& px, float& py) {
+  const float z = pt::dot(d, pt::mk(pv.cam_fwd));
+  if (z <= 1e-6f) return false;
+  const float aspect = float(pv.width) / float(pv.height);
+  const float ndx = pt::dot(d, pt::mk(pv.cam_right)) / (z * pv.tan_half_fov * aspect);
+  const float ndy = pt::dot(d, pt::mk(pv.cam_up)) / (z * pv.tan_half_fov);
+  px = (ndx * 0.5f + 0.5f) * float(pv.width);
+  py = (ndy * 0.5f + 0.5f) * float(pv.height);
+  return px >= 0.0f && py >= 0.0f && px < float(pv.width) && py < float(pv.height);
+}
+
+// reprojectHistory (trace.frag:284-322): bilinear fetch of the previous
+// running means from the taps that saw the same surface (or the same sky);
+// false when too little of the footprint survives (disocclusion).
+__device__ inline bool reproject_history(const PrevView& pv, const pt::Guide& g, pt::v3 rd, pt::PixelAccum& a) {
+  a.color = make_float4(0.f, 0.f, 0.f, 0.f);
+  a.albedo = a.color;
+  a.normal = a.color;
+  a.n = 0.0f;
+  const bool surface = g.coverage > 0.5f;
+  const pt::v3 d = surface ? g.position - pt::mk(pv.cam_pos) : rd;
+  float px, py;
+  if (!project_to_previous(pv, d, px, py)) return false;
+  const float expected = pt::length(d);
+  const float fx = px - 0.5f, fy = py - 0.5f;
+  const float flx = floorf(fx), fly = floorf(fy);
+  const int bx = int(flx), by = int(fly);
+  const float tx = fx - flx, ty = fy - fly;
+  float wsum = 0.0f;
+  for (int i = 0; i < 4; ++i) {
+    const int ox = i & 1, oy = i >> 1;
+    const int qx = bx + ox, qy = by + oy;
+    if (qx < 0 || qy < 0 || qx >= int(pv.width) || qy >= int(pv.height)) continue;
+    const pt::PixelAccum q = pt::load_accum(pv.film, uint32_t(qy) * pv.width + uint32_t(qx));
+    bool same;
+    if (surface) {
+      const pt::v3 qn = pt::normalize(pt::mk(q.normal.x + 1e-6f, q.normal.y + 1e-6f, q.normal.z + 1e-6f));
+      same = q.albedo.w > 0.5f && fabsf(q.normal.w - expected) < 0.02f * expected && pt::dot(qn, g.normal) > 0.85f;
+    } else {
+      same = q.albedo.w < 0.5f;
+    }
+    if (!same) continue;
+    const float w = (ox == 1 ? tx : 1.0f - tx) * (oy == 1 ? ty : 1.0f - ty);
+    a.color = make_float4(a.color.x + q.color.x * w, a.color.y + q.color.y * w, a.color.z + q.color.z * w,
+                          a.color.w + q.color.w * w);
+    a.albedo = make_float4(a.albedo.x + q.albedo.x * w, a.albedo.y + q.albedo.y * w, a.albedo.z + q.albedo.z * w,
+                           a.albedo.w + q.albedo.w * w);
+    a.normal = make_float4(a.normal.x + q.normal.x * w, a.normal.y + q.normal.y * w, a.normal.z + q.normal.z * w,
+                           a.normal.w + q.normal.w * w);
+    a.n += q.n * w;
+    wsum += w;
+  }
+  if (wsum < 1e-3f) return false;
+  a.color = make_float4(a.color.x / wsum, a.color.y / wsum, a.color.z / wsum, a.color.w / wsum);
+  a.albedo = make_float4(a.albedo.x / wsum, a.albedo.y / wsum, a.albedo.z / wsum, a.albedo.w / wsum);
+  a.normal = make_float4(a.normal.x / wsum, a.normal.y / wsum, a.normal.z / wsum, a.normal.w / wsum);
+  a.n = fminf(a.n / wsum, pv.history_cap * g.reuse);
+  return true;
+}
+
+// The sample after a camera move (trace.frag:324-355 with uReproject = 1):
+// one path per pixel; its running means restart from the previous view's,
+// reprojected, or from nothing where the pixel was disoccluded.  `film` and
+// pv.film must not alias.
+__global__ __launch_bounds__(kPtBlock) void k_trace_reprojected(PtScene sc, PtFrame f, PtFilm film, PrevView pv,
+                                                                uint32_t* spill, uint32_t spill_depth,
+                                                                uint32_t tiles_x) {
+  __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t x = (tile % tiles_x) * kTile + (threadIdx.x & (kTile - 1));
+  const uint32_t y = (tile / tiles_x) * kTile + (threadIdx.x / kTile);
+  if (x >= f.width || y >= f.height) return;
+  const uint32_t idx = y * f.width + x;
+  pt::Rng rng = pt::seed_rng(idx, f.seed);
+  pt::v3 ro, rd;
+  pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
+  pt::PtStack<kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
+                                 0, 0};
+  pt::PrimaryTracer<pt::PtStack<kPtBlock>> tr{{sc, st}};
+  pt::PathState p;
+  pt::path_begin(p, ro, rd);
+  if (int(f.max_bounces) >= 0)
+    while (pt::path_step_with(sc, f, p, rng, tr)) {
+    }
+  pt::PixelAccum a;
+  reproject_history(pv, p.guide, rd, a);  // on failure a is all zero: the pixel starts over
+  pt::fold_sample(a, p.radiance, p.guide);
+  pt::store_accum(film, idx, a);
+}
+
 template <class StackT>
 __device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t x, uint32_t y, StackT& st) {
   const PtScene& sc = it.scene;
@@ -134,6 +227,22 @@ Status launch_trace_sample(Context* ctx, const PtScene& sc, const PtFrame& f, co
                      film, static_cast<uint32_t*>(ctx->spill), depth, tiles_x);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "k_trace_sample launch");
+  return LUCENT_OK;
+}
+
+Status launch_trace_reprojected(Context* ctx, const PtScene& sc, const PtFrame& f, const PtFilm& film,
+                                const PrevView& prev, void* stream) {
+  const uint32_t tiles_x = (f.width + kTile - 1) / kTile;
+  const uint32_t tiles_y = (f.height + kTile - 1) / kTile;
+  const uint64_t blocks = uint64_t(tiles_x) * tiles_y;
+  if (blocks == 0) return LUCENT_OK;
+  const uint32_t depth = pt::pt_spill_words(sc.bvh);
+  Status s = ensure_spill(ctx, size_t(blocks) * kPtBlock * depth * sizeof(uint32_t));
+  if (s != LUCENT_OK) return s;
+  hipLaunchKernelGGL(k_trace_reprojected, dim3(uint32_t(blocks)), dim3(kPtBlock), 0, static_cast<hipStream_t>(stream),
+                     sc, f, film, prev, static_cast<uint32_t*>(ctx->spill), depth, tiles_x);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(e, "k_trace_reprojected launch");
   return LUCENT_OK;
 }
 

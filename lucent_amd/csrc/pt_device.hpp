@@ -48,6 +48,11 @@ __device__ inline float max3(v3 v) { return fmaxf(v.x, fmaxf(v.y, v.z)); }
 __device__ inline v3 mixv(v3 a, v3 b, float t) { return a * (1.0f - t) + b * t; }
 __device__ inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 __device__ inline float clampf(float x, float a, float b) { return fminf(fmaxf(x, a), b); }
+// GLSL smoothstep.
+__device__ inline float smoothstep(float e0, float e1, float x) {
+  const float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
 __device__ inline v3 reflect(v3 i, v3 n) { return i - n * (2.0f * dot(n, i)); }
 __device__ inline v3 refract(v3 i, v3 n, float eta) {
   const float d = dot(n, i);
@@ -1066,6 +1071,9 @@ struct Guide {
   v3 albedo, normal;
   float depth, coverage;
   bool done;
+  // Set only by tracers with kRecordPrimary (the reprojecting sample):
+  v3 position;  // primary hit
+  float reuse;  // 0-1: how view-independent the primary surface is (trace.frag:187-193)
 };
 
 // cameraRay (trace.frag:53-62); frag = gl_FragCoord (pixel centre, rows bottom-up).
@@ -1113,6 +1121,8 @@ __device__ __forceinline__ void path_begin(PathState& p, v3 ro, v3 rd) {
   p.guide.depth = FAR;
   p.guide.coverage = 0.0f;
   p.guide.done = false;
+  p.guide.position = mk(0.f, 0.f, 0.f);
+  p.guide.reuse = 1.0f;
 }
 
 // One iteration of tracePath's bounce loop; false once the path has ended
@@ -1127,6 +1137,7 @@ __device__ __forceinline__ void path_begin(PathState& p, v3 ro, v3 rd) {
 // rays, adding their contributions in the same order afterwards.
 template <class StackT>
 struct InlineTracer {
+  static constexpr bool kRecordPrimary = false;  // fill Guide::position and Guide::reuse
   const PtScene& sc;
   StackT& st;
   __device__ __forceinline__ bool closest(v3 ro, v3 rd, Hit& hit, Rng& rng) { return trace_scene(sc, ro, rd, hit, rng, st); }
@@ -1134,6 +1145,13 @@ struct InlineTracer {
   __device__ __forceinline__ void nee(int, v3 o, v3 d, float tmax, Rng& rng, PathState& p, F contribution) {
     if (!occluded(sc, o, d, tmax, rng, st)) p.radiance = p.radiance + contribution();
   }
+};
+
+// InlineTracer that also records the primary hit position and the surface's
+// reuse factor into the guide (for the reprojecting sample).
+template <class StackT>
+struct PrimaryTracer : InlineTracer<StackT> {
+  static constexpr bool kRecordPrimary = true;
 };
 
 template <class Tracer>
@@ -1153,6 +1171,7 @@ __device__ __forceinline__ bool path_step_with(const PtScene& sc, const PtFrame&
   if (bounce == 0) {
     p.guide.depth = hit.t;
     p.guide.coverage = 1.0f;
+    if constexpr (Tracer::kRecordPrimary) p.guide.position = hit.position;
   }
   const v3 wo = -p.rd;
   v3 emission;
@@ -1174,6 +1193,15 @@ __device__ __forceinline__ bool path_step_with(const PtScene& sc, const PtFrame&
     if (hit.light >= 0) return false;
   }
   const bool glass = s.transmission > 0.0f && rng.rand() < s.transmission;
+  if constexpr (Tracer::kRecordPrimary) {
+    if (bounce == 0) {
+      // Reflections and refractions do not move with the surface (trace.frag:187-193).
+      const float sharp = 1.0f - smoothstep(0.1f, 0.7f, s.roughness);
+      float glossy = fmaxf(sharp * (0.5f * (1.0f - s.metallic) + 1.0f * s.metallic), s.transmission);
+      glossy = fmaxf(glossy, s.clearcoat * 0.5f * (1.0f - smoothstep(0.08f, 0.5f, s.clearcoat_roughness)));
+      p.guide.reuse = 1.0f - glossy;
+    }
+  }
   const bool mirror = !glass && s.metallic > 0.9f && s.roughness < 0.1f;
   if (!p.guide.done && !((glass || mirror) && bounce < 3)) {
     p.guide.albedo = p.tint * s.albedo;

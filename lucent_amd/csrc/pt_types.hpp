@@ -78,6 +78,19 @@ struct PtFilm {
   float* history = nullptr;     // float per pixel
 };
 
+// The previous view of a reprojecting sample (renderer.ts:715-726): the
+// running means it accumulated (read only), its resolution and camera.
+struct PrevView {
+  PtFilm film;
+  uint32_t width = 0, height = 0;
+  float cam_pos[3] = {0, 0, 0};
+  float cam_right[3] = {1, 0, 0};
+  float cam_up[3] = {0, 1, 0};
+  float cam_fwd[3] = {0, 0, -1};
+  float tan_half_fov = 1.0f;
+  float history_cap = 8.0f;  // uHistoryCap
+};
+
 // One film's share of a persistent tile launch: `spp` samples of every pixel
 // with seeds frame.seed .. frame.seed + spp - 1, in that order per pixel.
 struct TileItem {

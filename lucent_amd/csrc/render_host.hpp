@@ -24,6 +24,9 @@ float round_to_half(float f);
 void camera_basis(const lucent_camera& cam, double fwd[3], double right[3], double up[3], double* tan_half_fov);
 
 Status launch_trace_sample(Context* ctx, const PtScene& sc, const PtFrame& f, const PtFilm& film, void* stream);
+// One reprojecting sample of every pixel of `film` (seed f.seed); film and prev.film must not alias.
+Status launch_trace_reprojected(Context* ctx, const PtScene& sc, const PtFrame& f, const PtFilm& film,
+                                const PrevView& prev, void* stream);
 
 // Launch shape of the persistent tile kernel (0 = default).
 struct TileLaunch {

@@ -120,6 +120,7 @@ __device__ inline void begin_sample(const PtFrame& f, const WfBuffers& B, uint32
 // The shade pass's tracer: the closest hit comes from the extend pass; light
 // samples are stored for the next shadow pass.
 struct WfTracer {
+  static constexpr bool kRecordPrimary = false;
   const PtScene& sc;
   const WfBuffers& B;
   uint32_t slot;

@@ -1,7 +1,8 @@
-"""Progressive renderer — the host mirror of Renderer (src/renderer.ts:171-871)
-for the tracing path: load_scene / upload_lighting / resize (film) / reset /
-trace_samples / benchmark / read-back.  Display, preview scaling and the GPU
-budget scheduler are not part of this path.
+"""Progressive renderer — the host mirror of Renderer (src/renderer.ts:171-871):
+load_scene / upload_lighting / upload_camera (with temporal reprojection) /
+reset / trace_samples / run_denoiser / render / screenshot / benchmark /
+read-back.  Preview scaling and the GPU budget scheduler are not part of it:
+render() traces the sample count it is given.
 """
 from __future__ import annotations
 
@@ -30,6 +31,11 @@ class RenderSettings:
     waves_per_cu: int = 0
     occupancy: int = 0
     wavefront: bool = False
+    # Presentation part (renderer.ts:42-84 defaults).
+    temporal: bool = True
+    denoise: bool = True
+    exposure: float = 1.0
+    tonemap: str = "aces"
 
     def struct(self) -> "_native.RenderSettingsStruct":
         flags = (_native.TRACE_PER_SAMPLE if self.per_sample_launch else 0) | \
@@ -62,6 +68,20 @@ def lighting_struct(scene: Scene, keep: list) -> _native.LightingStruct:
     L.env.background = 1 if e.background else 0
     L.env.backdrop[:] = list(e.backdrop)
     return L
+
+
+HISTORY_CAP = 8  # renderer.ts:139
+TONEMAPS = {"aces": 0, "reinhard": 1, "none": 2}
+
+
+def atrous_levels(spp: int) -> int:
+    """atrousLevels (renderer.ts:142)."""
+    return 5 if spp <= 4 else 4 if spp <= 16 else 3 if spp <= 64 else 2 if spp <= 256 else 1
+
+
+def denoise_weight(spp: int) -> float:
+    """denoiseWeight (renderer.ts:144): full until 128 spp, fading to the raw image by 1024."""
+    return min(1.0, max(0.0, (1024 - spp) / (1024 - 128)))
 
 
 def camera_struct(cam) -> _native.CameraStruct:
@@ -133,6 +153,9 @@ class Renderer:
         self.scene: Optional[Scene] = None
         self.packed: Optional[PackedScene] = None
         self.device_scene: Optional[DeviceScene] = None
+        self.history_camera = None     # the camera the film's samples were traced with
+        self.reproject_from = None     # pending reprojection: the previous view's camera
+        self.denoised_frame = -1
 
     @property
     def frame(self) -> int:
@@ -165,14 +188,77 @@ class Renderer:
         self.reset()
 
     def reset(self) -> None:
+        """Renderer.reset (renderer.ts:463-474)."""
         self.film.reset()
+        self.history_camera = None
+        self.reproject_from = None
+        self.denoised_frame = -1
+
+    def upload_camera(self, camera, moving: bool = True) -> None:
+        """Renderer.uploadCamera + invalidateView (renderer.ts:424-451): with
+        temporal reuse and samples in the film, the next sample reprojects the
+        history instead of starting over.  `moving` only matters to the
+        reference's preview scaling."""
+        self.scene.camera = camera
+        if self.settings.temporal and self.history_camera is not None and self.frame > 0:
+            self.reproject_from = self.history_camera
+            self.denoised_frame = -1
+        else:
+            self.reset()
 
     def trace_samples(self, count: int, stream: Optional[int] = None) -> None:
-        """Renderer.traceSamples (renderer.ts:702-737)."""
+        """Renderer.traceSamples (renderer.ts:702-737); the first sample after a
+        camera move reprojects the history (uReproject = 1)."""
+        if count <= 0:
+            return
         cam = camera_struct(self.scene.camera)
         st = self.settings.struct()
-        _native.check(_native.lib().lucent_trace_samples(self.ctx.handle, self.device_scene.handle, self.film.handle,
-                                                         C.byref(cam), C.byref(st), count, stream))
+        lib = _native.lib()
+        if self.reproject_from is not None:
+            prev = camera_struct(self.reproject_from)
+            _native.check(lib.lucent_trace_reprojected(self.ctx.handle, self.device_scene.handle, self.film.handle,
+                                                       C.byref(cam), self.film.handle, C.byref(prev),
+                                                       float(HISTORY_CAP), C.byref(st), stream))
+            self.reproject_from = None
+            count -= 1
+        if count:
+            _native.check(lib.lucent_trace_samples(self.ctx.handle, self.device_scene.handle, self.film.handle,
+                                                   C.byref(cam), C.byref(st), count, stream))
+        self.history_camera = self.scene.camera
+
+    def display_params(self) -> "_native.DisplayParamsStruct":
+        spp = self.frame
+        w = denoise_weight(spp) if (self.settings.denoise and spp > 0 and self.denoised_frame == spp) else 0.0
+        return _native.DisplayParamsStruct(w, self.settings.exposure, TONEMAPS[self.settings.tonemap])
+
+    def render(self, samples: int = 1, stream: Optional[int] = None) -> int:
+        """Renderer.render (renderer.ts:552-600) without the budget scheduler:
+        traces `samples` samples, runs the denoiser when its weight is above 0
+        and the film changed, and leaves the image to display()/screenshot()."""
+        self.trace_samples(samples, stream)
+        spp = self.frame
+        if self.settings.denoise and spp > 0 and denoise_weight(spp) > 0 and self.denoised_frame != spp:
+            self.run_denoiser(stream=stream)
+            self.denoised_frame = spp
+        return samples
+
+    def display(self, d_rgba8: int, view_w: Optional[int] = None, view_h: Optional[int] = None,
+                stream: Optional[int] = None) -> None:
+        """The display draw (display.frag) into device memory d_rgba8."""
+        p = self.display_params()
+        _native.check(_native.lib().lucent_display(self.ctx.handle, self.film.handle, C.byref(p),
+                                                   view_w or self.film.width, view_h or self.film.height, d_rgba8,
+                                                   stream))
+
+    def screenshot(self, view_w: Optional[int] = None, view_h: Optional[int] = None) -> np.ndarray:
+        """Renderer.screenshot (renderer.ts:856-862): the displayed image as
+        RGBA8 (view_h, view_w, 4), rows bottom-up like the film."""
+        vw, vh = view_w or self.film.width, view_h or self.film.height
+        out = np.empty((vh, vw, 4), np.uint8)
+        p = self.display_params()
+        _native.check(_native.lib().lucent_screenshot(self.ctx.handle, self.film.handle, C.byref(p), vw, vh,
+                                                      out.ctypes.data))
+        return out
 
     def benchmark(self, samples: int) -> dict:
         """Renderer.benchmark (renderer.ts:745-777): 1 warm-up sample, then

@@ -231,11 +231,20 @@ class OrFrame(C.Structure):
                 ("aperture", _F), ("focus_dist", _F)]
 
 
+class OrPrev(C.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("cam_pos", _F * 3), ("cam_right", _F * 3), ("cam_up", _F * 3),
+                ("cam_fwd", _F * 3), ("tan_half_fov", _F), ("history_cap", _F), ("color4", _P), ("albedo4", _P),
+                ("normal4", _P), ("history", _P)]
+
+
 def _pt_lib():
     l = lib()
     if not getattr(l, "_pt_ready", False):
         l.or_trace_pixels.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), _U32, _U32, _P, _U32, _P, _P, _P]
         l.or_trace_pixels.restype = C.c_int
+        l.or_trace_reprojected.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), C.POINTER(OrPrev), _U32, _P, _U32,
+                                           _P, _P]
+        l.or_trace_reprojected.restype = C.c_int
         l.or_alias_table.argtypes = [_P, _U32, _P, _P, _P, C.POINTER(_D)]
         l.or_alias_table.restype = C.c_int
         l.or_env_sampler.argtypes = [_P, _U32, _U32, _U32, _P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_D)]
@@ -470,6 +479,38 @@ class PtOracle:
             return out, ps, dict(zip(self.COUNT_NAMES, tot[: len(self.COUNT_NAMES)].tolist()))
         return out, ps
 
+    def trace_reprojected(self, frame: OrFrame, prev_film: dict, prev_frame: OrFrame, history_cap: float, seed: int,
+                          xy: np.ndarray, threads: int | None = None):
+        """trace.frag main() with uReproject = 1 (trace.frag:274-355) at the listed
+        pixels: one sample with seed `seed`, history reprojected from prev_film
+        (float arrays color/albedo/normal (H, W, 4) and history (H, W), rows
+        bottom-up) seen through prev_frame's camera.  Returns (out (n, 13): colour
+        rgb + lum^2, albedo + coverage, normal + depth, history; reprojected (n,)
+        bool)."""
+        xy = np.ascontiguousarray(xy, np.uint32).reshape(-1, 2)
+        n = xy.shape[0]
+        arrs = [np.ascontiguousarray(prev_film[k], np.float32) for k in ("color", "albedo", "normal", "history")]
+        pv = OrPrev()
+        pv.width, pv.height = prev_frame.width, prev_frame.height
+        for k in ("cam_pos", "cam_right", "cam_up", "cam_fwd"):
+            getattr(pv, k)[:] = list(getattr(prev_frame, k))
+        pv.tan_half_fov = prev_frame.tan_half_fov
+        pv.history_cap = history_cap
+        pv.color4, pv.albedo4, pv.normal4, pv.history = [a.ctypes.data for a in arrs]
+        out = np.zeros((n, 13))
+        flags = np.zeros(n, np.uint8)
+        threads = threads or min(16, os.cpu_count() or 1)
+        chunks = _split(n, threads * 4)
+
+        def run(ab):
+            a, b = ab
+            return _pt_lib().or_trace_reprojected(C.byref(self.scene), C.byref(frame), C.byref(pv), seed,
+                                                  _ptr(xy) + a * 8, b - a, _ptr(out) + a * 104, _ptr(flags) + a)
+
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(run, chunks))
+        return out, flags.astype(bool)
+
     def set_materials(self, packed_materials) -> None:
         """Replaces the material texels the oracle scene points to (same count)."""
         m = np.ascontiguousarray(packed_materials, np.float32).reshape(-1)
@@ -549,3 +590,60 @@ def camera13(camera) -> np.ndarray:
     fw, r, u, t = camera_basis(camera.position, camera.target, camera.fov)
     return np.concatenate([np.float32(camera.position), np.float32(r), np.float32(u), np.float32(fw),
                            [np.float32(t)]]).astype(np.float32)
+
+
+def display(color, albedo, denoised, view_w, view_h, denoise_weight=0.0, exposure=1.0, tonemap=0):
+    """FP64 restatement of display.frag (display.frag:16-67) over a film read
+    back as float arrays (H, W, 4), rows bottom-up; denoised is the denoised
+    illumination (H, W, >=3) or None.  Returns (rgba8 (view_h, view_w, 4)
+    uint8 as a UNORM8 target stores the colour: clamp, x255, round half to
+    even; the float colour before that conversion (view_h, view_w, 3))."""
+    color = np.asarray(color, np.float64)
+    albedo = np.asarray(albedo, np.float64)
+    rh, rw = color.shape[:2]
+
+    def radiance(px, py):
+        px = np.clip(px, 0, rw - 1)
+        py = np.clip(py, 0, rh - 1)
+        raw = color[py, px, :3]
+        if denoise_weight > 0:
+            a = albedo[py, px]
+            d = np.asarray(denoised, np.float64)[py, px, :3]
+            mixed = raw * (1 - denoise_weight) + d * np.maximum(a[..., :3], 1e-3) * denoise_weight
+            return np.where((a[..., 3] >= 0.5)[..., None], mixed, raw)
+        return raw
+
+    yy, xx = np.mgrid[0:view_h, 0:view_w]
+    if rw == view_w and rh == view_h:
+        c = radiance(xx, yy)
+    else:
+        sx = (xx + 0.5) * (rw / view_w) - 0.5
+        sy = (yy + 0.5) * (rh / view_h) - 0.5
+        ix, iy = np.floor(sx).astype(np.int64), np.floor(sy).astype(np.int64)
+        fx, fy = sx - np.floor(sx), sy - np.floor(sy)
+
+        def cr(f):
+            return [f * (-0.5 + f * (1 - 0.5 * f)), 1 + f * f * (-2.5 + 1.5 * f), f * (0.5 + f * (2 - 1.5 * f)),
+                    f * f * (-0.5 + 0.5 * f)]
+
+        wx, wy = cr(fx), cr(fy)
+        c = np.zeros((view_h, view_w, 3))
+        for j in range(4):
+            for i in range(4):
+                c += radiance(ix + i - 1, iy + j - 1) * (wx[i] * wy[j])[..., None]
+        c = np.maximum(c, 0)
+    c = c * exposure
+    if tonemap == 0:
+        c = np.clip((c * (2.51 * c + 0.03)) / (c * (2.43 * c + 0.59) + 0.14), 0, 1)
+    elif tonemap == 1:
+        c = c / (1 + c)
+    c = np.clip(c, 0, 1)
+    c = np.where(c >= 0.0031308, 1.055 * np.power(c, 1 / 2.4) - 0.055, c * 12.92)
+    px, py = xx + 0.5, yy + 0.5
+    a = 0.06711056 * px + 0.00583715 * py
+    b = 52.9829189 * (a - np.floor(a))
+    c = c + ((b - np.floor(b)) - 0.5)[..., None] / 255
+    rgba = np.empty((view_h, view_w, 4), np.uint8)
+    rgba[..., :3] = np.rint(np.clip(c, 0, 1) * 255).astype(np.uint8)
+    rgba[..., 3] = 255
+    return rgba, c

@@ -851,7 +851,14 @@ typedef struct {
   d3 albedo, normal;
   double depth, coverage;
   int done;
+  d3 position;  /* primary hit (trace.frag:158-159) */
+  double reuse; /* trace.frag:187-193 */
 } guide_t;
+
+static double smoothstep_d(double e0, double e1, double x) {
+  double t = clampd((x - e0) / (e1 - e0), 0, 1);
+  return t * t * (3 - 2 * t);
+}
 
 static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t *r, guide_t *g) {
   d3 rad = V(0, 0, 0), thr = V(1, 1, 1), tint = V(1, 1, 1);
@@ -862,6 +869,8 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
   g->depth = 1e4;
   g->coverage = 0;
   g->done = 0;
+  g->position = V(0, 0, 0);
+  g->reuse = 1;
   for (int bounce = 0; bounce <= (int)f->max_bounces; bounce++) {
     hit_t h;
     if (!trace_scene(sc, ro, rd, &h, r)) {
@@ -879,6 +888,7 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
     if (bounce == 0) {
       g->depth = h.t;
       g->coverage = 1;
+      g->position = h.pos;
     }
     d3 wo = neg(rd), em;
     surf_t s = surface_at(sc, &h, wo, &em);
@@ -899,6 +909,12 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
       if (h.light >= 0) break;
     }
     int glass = s.transmission > 0 && rnd(r) < s.transmission;
+    if (bounce == 0) {
+      double sharp = 1 - smoothstep_d(0.1, 0.7, s.roughness);
+      double glossy = maxd(sharp * (0.5 * (1 - s.metallic) + 1.0 * s.metallic), s.transmission);
+      glossy = maxd(glossy, s.clearcoat * 0.5 * (1 - smoothstep_d(0.08, 0.5, s.cc_rough)));
+      g->reuse = 1 - glossy;
+    }
     int mirror = !glass && s.metallic > 0.9 && s.roughness < 0.1;
     if (!g->done && !((glass || mirror) && bounce < 3)) {
       g->albedo = mul(tint, s.albedo);
@@ -973,6 +989,27 @@ static d3 trace_path(const or_scene *sc, const or_frame *f, d3 ro, d3 rd, rng_t 
   return rad;
 }
 
+/* main() of trace.frag up to tracePath (trace.frag:324-330): seed, camera
+ * ray (cameraRay, trace.frag:53-62), path.  Returns the radiance and the
+ * guide; *rd_out is the camera ray direction. */
+static d3 sample_pixel(const or_scene *sc, const or_frame *f, uint32_t x, uint32_t y, uint32_t seed, guide_t *g,
+                       d3 *rd_out) {
+  rng_t r = {(x + y * f->width) * 1973u + seed * 9277u + 26699u};
+  pcg(&r);
+  double r0 = rnd(&r), r1 = rnd(&r);
+  double ux = (x + 0.5 + r0) / f->width * 2 - 1, uy = (y + 0.5 + r1) / f->height * 2 - 1;
+  double aspect = (double)f->width / f->height;
+  d3 F = Vf(f->cam_fwd), R = Vf(f->cam_right), U = Vf(f->cam_up), P = Vf(f->cam_pos);
+  d3 dir = nrm(add(add(F, scl(R, ux * aspect * f->tan_half_fov)), scl(U, uy * f->tan_half_fov)));
+  d3 focus = add(P, scl(dir, f->focus_dist / dot(dir, F)));
+  double l0 = rnd(&r), l1 = rnd(&r), lx, ly;
+  concentric(l0, l1, &lx, &ly);
+  d3 ro = add(add(P, scl(R, lx * f->aperture)), scl(U, ly * f->aperture));
+  d3 rd = nrm(sub(focus, ro));
+  *rd_out = rd;
+  return trace_path(sc, f, ro, rd, &r, g);
+}
+
 /* Traces `samples` samples (seeds seed0, seed0+1, ...) at each listed pixel
  * (x, y with rows bottom-up) and writes per pixel: mean rgb, mean lum^2,
  * mean albedo rgb, mean coverage, mean normal xyz, mean depth (12 doubles),
@@ -986,21 +1023,9 @@ int or_trace_pixels(const or_scene *sc, const or_frame *f, uint32_t seed0, uint3
     uint32_t x = xy[2 * p], y = xy[2 * p + 1];
     double acc[12] = {0};
     for (uint32_t k = 0; k < samples; k++) {
-      rng_t r = {(x + y * f->width) * 1973u + (seed0 + k) * 9277u + 26699u};
-      pcg(&r);
-      /* cameraRay (trace.frag:53-62) */
-      double r0 = rnd(&r), r1 = rnd(&r);
-      double ux = (x + 0.5 + r0) / f->width * 2 - 1, uy = (y + 0.5 + r1) / f->height * 2 - 1;
-      double aspect = (double)f->width / f->height;
-      d3 F = Vf(f->cam_fwd), R = Vf(f->cam_right), U = Vf(f->cam_up), P = Vf(f->cam_pos);
-      d3 dir = nrm(add(add(F, scl(R, ux * aspect * f->tan_half_fov)), scl(U, uy * f->tan_half_fov)));
-      d3 focus = add(P, scl(dir, f->focus_dist / dot(dir, F)));
-      double l0 = rnd(&r), l1 = rnd(&r), lx, ly;
-      concentric(l0, l1, &lx, &ly);
-      d3 ro = add(add(P, scl(R, lx * f->aperture)), scl(U, ly * f->aperture));
-      d3 rd = nrm(sub(focus, ro));
       guide_t g;
-      d3 c = trace_path(sc, f, ro, rd, &r, &g);
+      d3 rd;
+      d3 c = sample_pixel(sc, f, x, y, seed0 + k, &g, &rd);
       if (!isfinite(c.x) || !isfinite(c.y) || !isfinite(c.z)) c = V(0, 0, 0);
       double l = lum(c);
       double vals[12] = {c.x, c.y, c.z, l * l, g.albedo.x, g.albedo.y, g.albedo.z, g.coverage,
@@ -1016,6 +1041,93 @@ int or_trace_pixels(const or_scene *sc, const or_frame *f, uint32_t seed0, uint3
     for (int i = 0; i < 12; i++) out12[(size_t)p * 12 + i] = acc[i] / samples;
   }
   g_counts = NULL;
+  return 0;
+}
+
+
+/* ---- temporal reprojection (trace.frag:274-355 with uReproject = 1) ---- */
+typedef struct {
+  uint32_t width, height;
+  float cam_pos[3], cam_right[3], cam_up[3], cam_fwd[3];
+  float tan_half_fov, history_cap;
+  const float *color4, *albedo4, *normal4, *history; /* previous running means, rows bottom-up */
+} or_prev;
+
+/* projectToPrevious (trace.frag:274-282). */
+static int project_prev(const or_prev *pv, d3 d, double *px, double *py) {
+  double z = dot(d, Vf(pv->cam_fwd));
+  if (z <= 1e-6) return 0;
+  double aspect = (double)pv->width / pv->height;
+  double nx = dot(d, Vf(pv->cam_right)) / (z * pv->tan_half_fov * aspect);
+  double ny = dot(d, Vf(pv->cam_up)) / (z * pv->tan_half_fov);
+  *px = (nx * 0.5 + 0.5) * pv->width;
+  *py = (ny * 0.5 + 0.5) * pv->height;
+  return *px >= 0 && *py >= 0 && *px < pv->width && *py < pv->height;
+}
+
+/* reprojectHistory (trace.frag:284-322); prev13 = colour 4, albedo 4,
+ * normal 4, history. */
+static int reproject_history(const or_prev *pv, const guide_t *g, d3 rd, double prev13[13]) {
+  for (int i = 0; i < 13; i++) prev13[i] = 0;
+  int surface = g->coverage > 0.5;
+  d3 d = surface ? sub(g->position, Vf(pv->cam_pos)) : rd;
+  double px, py;
+  if (!project_prev(pv, d, &px, &py)) return 0;
+  double expected = len(d);
+  double fx = px - 0.5, fy = py - 0.5;
+  int bx = (int)floor(fx), by = (int)floor(fy);
+  double tx = fx - floor(fx), ty = fy - floor(fy), wsum = 0;
+  for (int i = 0; i < 4; i++) {
+    int ox = i & 1, oy = i >> 1, qx = bx + ox, qy = by + oy;
+    if (qx < 0 || qy < 0 || qx >= (int)pv->width || qy >= (int)pv->height) continue;
+    size_t q = (size_t)qy * pv->width + qx;
+    const float *a = pv->albedo4 + q * 4, *n = pv->normal4 + q * 4, *c = pv->color4 + q * 4;
+    int same;
+    if (surface) {
+      d3 qn = nrm(V(n[0] + 1e-6, n[1] + 1e-6, n[2] + 1e-6));
+      same = a[3] > 0.5 && fabs(n[3] - expected) < 0.02 * expected && dot(qn, g->normal) > 0.85;
+    } else {
+      same = a[3] < 0.5;
+    }
+    if (!same) continue;
+    double w = (ox == 1 ? tx : 1 - tx) * (oy == 1 ? ty : 1 - ty);
+    for (int k = 0; k < 4; k++) {
+      prev13[k] += c[k] * w;
+      prev13[4 + k] += a[k] * w;
+      prev13[8 + k] += n[k] * w;
+    }
+    prev13[12] += pv->history[q] * w;
+    wsum += w;
+  }
+  if (wsum < 1e-3) return 0;
+  for (int k = 0; k < 12; k++) prev13[k] /= wsum;
+  prev13[12] = mind(prev13[12] / wsum, pv->history_cap * g->reuse);
+  return 1;
+}
+
+/* One reprojecting sample (seed `seed`) at each listed pixel of the frame:
+ * out13 per pixel = colour rgb + lum^2, albedo rgb + coverage, normal xyz +
+ * depth, history after the sample (FP64, no FP16 rounding); flags[p] = 1 if
+ * the history was reprojected, 0 if the pixel started over. */
+int or_trace_reprojected(const or_scene *sc, const or_frame *f, const or_prev *pv, uint32_t seed, const uint32_t *xy,
+                         uint32_t npix, double *out13, uint8_t *flags) {
+  for (uint32_t p = 0; p < npix; p++) {
+    uint32_t x = xy[2 * p], y = xy[2 * p + 1];
+    guide_t g;
+    d3 rd;
+    d3 c = sample_pixel(sc, f, x, y, seed, &g, &rd);
+    if (!isfinite(c.x) || !isfinite(c.y) || !isfinite(c.z)) c = V(0, 0, 0);
+    double l = lum(c), prev[13];
+    int ok = reproject_history(pv, &g, rd, prev);
+    if (!ok) prev[12] = 0;
+    double k = 1.0 / (prev[12] + 1.0), j = 1 - k;
+    double now[12] = {c.x, c.y, c.z, l * l, g.albedo.x, g.albedo.y, g.albedo.z, g.coverage,
+                      g.normal.x, g.normal.y, g.normal.z, g.depth};
+    double *o = out13 + (size_t)p * 13;
+    for (int i = 0; i < 12; i++) o[i] = prev[i] * j + now[i] * k;
+    o[12] = prev[12] + 1;
+    if (flags) flags[p] = (uint8_t)ok;
+  }
   return 0;
 }
 
