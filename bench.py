@@ -497,13 +497,73 @@ def bench_copy(args) -> dict:
             "dtype": "u8", "config": {"workload": "device copy of 1 GiB (counter calibration)", "bytes": n}}
 
 
+def bench_frame_graph(args) -> dict:
+    """Interactive frames on the knot scene: 1 sample, a 3-level denoise and
+    the display pass per frame, issued as eager calls (9 kernel launches
+    through 3 entry points) or as one replay of a captured frame graph, at
+    640x480 and at a 128x96 thumbnail.  Both leave bit-identical films;
+    frames/s over 200 frames."""
+    import ctypes as C
+
+    import torch
+
+    from lucent_amd import _native, workloads
+    from lucent_amd.renderer import FrameGraph, Renderer, RenderSettings
+    from lucent_amd.runtime import Context
+
+    N = 200
+    ctx = Context(0)
+    lib = _native.lib()
+    res = {}
+    for W, H in ((640, 480), (128, 96)):
+        out = {}
+        for mode in ("eager", "graph"):
+            r = Renderer(ctx, W, H, RenderSettings(8, 0.0))
+            r.load_scene(workloads.knot_scene())
+            r.film.set_seed(1)
+            r.reset()
+            buf = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
+            torch.cuda.synchronize()
+            dn, dp = (4.0, 64.0, 1.5, 3), _native.DisplayParamsStruct(1.0, 1.0, 0)
+            g = None
+            if mode == "eager":
+                def frame():
+                    r.trace_samples(1)
+                    r.run_denoiser(*dn)
+                    _native.check(lib.lucent_display(ctx.handle, r.film.handle, C.byref(dp), W, H, buf.data_ptr(),
+                                                     None))
+            else:
+                g = FrameGraph(r, 1, dn, (1.0, 1.0, 0), buf.data_ptr())
+                frame = g.launch
+            for _ in range(5):
+                frame()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(N):
+                frame()
+            ctx.synchronize()
+            dt = (time.perf_counter() - t0) / N
+            out[mode] = {"ms_per_frame": round(dt * 1e3, 4), "frames_per_s": round(1 / dt, 1)}
+            out[mode + "_film"] = r.film.read()["color"]
+            if g is not None:
+                g.close()
+            r.close()
+        same = bool(np.array_equal(out.pop("eager_film"), out.pop("graph_film")))
+        out["speedup"] = round(out["eager"]["ms_per_frame"] / out["graph"]["ms_per_frame"], 3)
+        out["film_bit_identical"] = same
+        res[f"{W}x{H}"] = out
+    ctx.close()
+    return {"metric": "interactive frames/s (1 spp + denoise + display), graph replay", "value":
+            res["128x96"]["graph"]["frames_per_s"], "unit": "frames/s @128x96", **res}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "closest-hit", "path-trace", "batch", "denoise", "adjoint", "copy"])
+                    choices=["all", "closest-hit", "path-trace", "batch", "denoise", "adjoint", "frame-graph", "copy"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -525,6 +585,10 @@ def main() -> int:
         d = bench_batch(args)
         res = {"value": d.pop("value"), "unit": d.pop("unit"), "ms_per_step": d["batched_s"] * 1e3, "dtype": "f32",
                "config": {"workload": d.pop("metric")}, **d}
+    elif args.workload == "frame-graph":
+        d = bench_frame_graph(args)
+        res = {"value": d.pop("value"), "unit": d.pop("unit"), "ms_per_step": d["128x96"]["graph"]["ms_per_frame"],
+               "dtype": "f32/f16", "config": {"workload": d.pop("metric")}, **d}
     elif args.workload == "denoise":
         d = bench_denoise(args)
         res = {"value": d["value"], "unit": d["unit"], "ms_per_step": d["value"], "dtype": "f32/f16",
@@ -539,6 +603,7 @@ def main() -> int:
             extra["batch"] = bench_batch(args)
             extra["denoise"] = bench_denoise(args)
             extra["adjoint"] = bench_adjoint(args)
+            extra["frame_graph"] = bench_frame_graph(args)
     line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
             "higher_is_better": res.pop("higher_is_better", True), "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),

@@ -397,6 +397,35 @@ lucent_status lucent_screenshot(lucent_ctx *ctx, const lucent_film *film, const 
                                 uint32_t view_w, uint32_t view_h, uint8_t *rgba8);
 
 /* ------------------------------------------------------------------------
+ * Frame graphs: one progressive frame (`samples` samples of every pixel of
+ * `film` through the persistent tile launch, then optionally lucent_denoise
+ * with *denoise and lucent_display with *display into d_rgba8) captured
+ * once into a HIP graph and replayed by lucent_frame_graph_launch with one
+ * launch call.  The samples' seeds continue on the device (the graph
+ * advances a device copy of the film's seed), so n replays leave the film,
+ * the denoised image and the display buffer bit-identical to n calls of
+ * lucent_trace_samples, lucent_denoise and lucent_display.  Camera,
+ * settings, denoise iterations (0 = atrousLevels of the sample count after
+ * the first replay) and display parameters are fixed at capture; the graph
+ * holds the scene's, film's and output's device addresses.  Re-create it
+ * after moving the camera, after lucent_scene_set_lighting /
+ * lucent_scene_set_textures, or when any of them is destroyed.  A replay
+ * fails if the film was advanced by other calls since the last replay.
+ * No counterpart in the reference (one draw per sample and pass,
+ * renderer.ts:552-600, 702-737).
+ * ---------------------------------------------------------------------- */
+typedef struct lucent_frame_graph lucent_frame_graph;
+
+lucent_status lucent_frame_graph_create(lucent_ctx *ctx, lucent_scene *scene, lucent_film *film,
+                                        const lucent_camera *camera, const lucent_render_settings *settings,
+                                        uint32_t samples, const lucent_denoise_params *denoise,
+                                        const lucent_display_params *display, uint32_t view_w, uint32_t view_h,
+                                        uint8_t *d_rgba8, lucent_frame_graph **out);
+/* Enqueues one replay on `stream` (NULL: the context's stream). */
+lucent_status lucent_frame_graph_launch(lucent_frame_graph *graph, void *stream);
+void lucent_frame_graph_destroy(lucent_frame_graph *graph);
+
+/* ------------------------------------------------------------------------
  * Differentiable rendering.  The forward pass is lucent_trace_samples; the
  * backward pass replays each of the `spp` paths per pixel (seeds seed0 ..
  * seed0+spp-1, the seeds the forward used) from its RNG seed and returns the

@@ -154,6 +154,11 @@ SIGNATURES: dict[str, tuple] = {
                                         C.POINTER(RenderSettingsStruct), _P]),
     "lucent_display": (_I32, [_P, _P, C.POINTER(DisplayParamsStruct), _U32, _U32, _P, _P]),
     "lucent_screenshot": (_I32, [_P, _P, C.POINTER(DisplayParamsStruct), _U32, _U32, _P]),
+    "lucent_frame_graph_create": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32,
+                                         C.POINTER(DenoiseParamsStruct), C.POINTER(DisplayParamsStruct), _U32, _U32,
+                                         _P, C.POINTER(_P)]),
+    "lucent_frame_graph_launch": (_I32, [_P, _P]),
+    "lucent_frame_graph_destroy": (None, [_P]),
     "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,
                                 _U32, _P, _P, _P, _P, _P]),
     "lucent_forward": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _P,

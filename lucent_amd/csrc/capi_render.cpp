@@ -688,44 +688,52 @@ static uint32_t atrous_levels(uint32_t spp) {
   return spp <= 4 ? 5 : spp <= 16 ? 4 : spp <= 64 ? 3 : spp <= 256 ? 2 : 1;
 }
 
-lucent_status lucent_denoise(lucent_ctx* ctx, lucent_film* film, const lucent_camera* cam,
-                             const lucent_denoise_params* p, void* stream) {
-  clear_error();
-  if (!ctx || !film || !cam || !p) return fail(LUCENT_E_INVALID, "null argument");
+// DenoiseArgs over the film's targets; iterations 0 = atrousLevels(samples).
+static Status denoise_args(lucent_film* film, const lucent_camera* cam, const lucent_denoise_params* p, uint32_t samples,
+                           DenoiseArgs* a) {
   if (p->iterations > 12) return fail(LUCENT_E_INVALID, "at most 12 a-trous iterations");
   const size_t px = size_t(film->width) * film->height;
   if (!film->filter) {
     Status s = dev_alloc(&film->filter, px * (16 * 4 + 8));
     if (s != LUCENT_OK) return s;
   }
-  DenoiseArgs a;
-  a.color = film->f.color;
-  a.albedo = film->f.albedo;
-  a.normal = film->f.normal;
-  a.history = film->f.history;
-  a.ping = static_cast<float*>(film->filter);
-  a.pong = a.ping + px * 4;
-  a.remodulated = a.pong + px * 4;
-  a.guides = a.remodulated + px * 4;
-  a.var_a = a.guides + px * 4;
-  a.var_b = a.var_a + px;
-  a.result = &film->denoised;
-  a.width = int(film->width);
-  a.height = int(film->height);
-  a.iterations = p->iterations ? p->iterations : atrous_levels(film->samples);
-  a.sigma_l = p->sigma_l;
-  a.sigma_n = p->sigma_n;
-  a.sigma_p = p->sigma_p;
+  a->color = film->f.color;
+  a->albedo = film->f.albedo;
+  a->normal = film->f.normal;
+  a->history = film->f.history;
+  a->ping = static_cast<float*>(film->filter);
+  a->pong = a->ping + px * 4;
+  a->remodulated = a->pong + px * 4;
+  a->guides = a->remodulated + px * 4;
+  a->var_a = a->guides + px * 4;
+  a->var_b = a->var_a + px;
+  a->result = &film->denoised;
+  a->width = int(film->width);
+  a->height = int(film->height);
+  a->iterations = p->iterations ? p->iterations : atrous_levels(samples);
+  a->sigma_l = p->sigma_l;
+  a->sigma_n = p->sigma_n;
+  a->sigma_p = p->sigma_p;
   double fwd[3], right[3], up[3], tan_half;
   camera_basis(*cam, fwd, right, up, &tan_half);
   for (int i = 0; i < 3; ++i) {
-    a.cam_pos[i] = float(cam->position[i]);
-    a.cam_fwd[i] = float(fwd[i]);
-    a.cam_right[i] = float(right[i]);
-    a.cam_up[i] = float(up[i]);
+    a->cam_pos[i] = float(cam->position[i]);
+    a->cam_fwd[i] = float(fwd[i]);
+    a->cam_right[i] = float(right[i]);
+    a->cam_up[i] = float(up[i]);
   }
-  a.tan_half_fov = float(tan_half);
-  film->remodulated = a.remodulated;
+  a->tan_half_fov = float(tan_half);
+  film->remodulated = a->remodulated;
+  return LUCENT_OK;
+}
+
+lucent_status lucent_denoise(lucent_ctx* ctx, lucent_film* film, const lucent_camera* cam,
+                             const lucent_denoise_params* p, void* stream) {
+  clear_error();
+  if (!ctx || !film || !cam || !p) return fail(LUCENT_E_INVALID, "null argument");
+  DenoiseArgs a;
+  Status s = denoise_args(film, cam, p, film->samples, &a);
+  if (s != LUCENT_OK) return s;
   return launch_denoise(a, stream ? stream : ctx->stream);
 }
 
@@ -788,6 +796,115 @@ lucent_status lucent_film_read_denoised(lucent_ctx* ctx, const lucent_film* film
   if (illum_var4 && (s = dev_download(illum_var4, film->denoised, px * 16, ctx->stream)) != LUCENT_OK) return s;
   if (radiance4 && (s = dev_download(radiance4, film->remodulated, px * 16, ctx->stream)) != LUCENT_OK) return s;
   return stream_sync(ctx->stream);
+}
+
+
+struct lucent_frame_graph {
+  lucent_ctx* ctx = nullptr;
+  lucent_film* film = nullptr;
+  uint32_t samples = 0;
+  uint32_t expected_seed = 0;
+  void* buf = nullptr;  // [seed word, job counter] in the first 256 bytes, then the spill area
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture = nullptr;
+};
+
+void lucent_frame_graph_destroy(lucent_frame_graph* g) {
+  if (!g) return;
+  if (g->ctx) hipStreamSynchronize(static_cast<hipStream_t>(g->ctx->stream));
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  if (g->capture) hipStreamDestroy(g->capture);
+  if (g->buf) {
+    hipDeviceSynchronize();
+    hipFree(g->buf);
+  }
+  delete g;
+}
+
+lucent_status lucent_frame_graph_create(lucent_ctx* ctx, lucent_scene* scene, lucent_film* film,
+                                        const lucent_camera* cam, const lucent_render_settings* settings,
+                                        uint32_t samples, const lucent_denoise_params* denoise,
+                                        const lucent_display_params* display, uint32_t view_w, uint32_t view_h,
+                                        uint8_t* d_rgba8, lucent_frame_graph** out) {
+  clear_error();
+  if (!ctx || !scene || !film || !cam || !settings || !out) return fail(LUCENT_E_INVALID, "null argument");
+  *out = nullptr;
+  if (samples == 0) return fail(LUCENT_E_INVALID, "a frame graph traces at least one sample");
+  if (settings->flags & (LUCENT_TRACE_PER_SAMPLE | LUCENT_TRACE_WAVEFRONT))
+    return fail(LUCENT_E_UNSUPPORTED, "frame graphs capture the persistent tile launch only");
+  if (display && !d_rgba8) return fail(LUCENT_E_INVALID, "display needs an output buffer");
+  Status s = check_traceable(scene);
+  if (s != LUCENT_OK) return s;
+  // Finish queued work on the film before its counters are read.
+  if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+  TileItem it;
+  it.scene = scene->pt;
+  fill_frame(cam, settings, film->width, film->height, &it.frame);
+  it.frame.seed = film->seed;
+  it.film = film->f;
+  it.spp = samples;
+  TileLaunch cfg;
+  cfg.waves_per_cu = settings->waves_per_cu;
+  cfg.occupancy = settings->occupancy;
+  DenoiseArgs da;
+  if (denoise && (s = denoise_args(film, cam, denoise, film->samples + samples, &da)) != LUCENT_OK) return s;
+  DisplayArgs disp;
+  if (display) {
+    if (display->denoise_weight > 0.0f && !denoise && !film->denoised)
+      return fail(LUCENT_E_INVALID, "denoise_weight > 0 needs a denoise step in the graph or a lucent_denoise first");
+    lucent_film probe = *film;
+    if (denoise) probe.denoised = static_cast<float*>(film->filter);  // replaced after the denoise is captured
+    if ((s = display_args(&probe, display, view_w, view_h, &disp)) != LUCENT_OK) return s;
+    disp.out = d_rgba8;
+  }
+  auto* g = new lucent_frame_graph();
+  g->ctx = ctx;
+  g->film = film;
+  g->samples = samples;
+  g->expected_seed = film->seed;
+  const size_t spill = tile_spill_bytes(ctx, it, cfg);
+  auto bail = [&](Status st) {
+    lucent_frame_graph_destroy(g);
+    return st;
+  };
+  if ((s = dev_alloc(&g->buf, 256 + spill)) != LUCENT_OK) return bail(s);
+  uint32_t* seed_word = static_cast<uint32_t*>(g->buf);
+  cfg.counter = seed_word + 1;
+  cfg.spill = reinterpret_cast<uint32_t*>(static_cast<char*>(g->buf) + 256);
+  cfg.spill_bytes = spill;
+  it.seed_src = seed_word;
+  hipError_t e = hipMemcpy(seed_word, &film->seed, sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return bail(fail_hip(e, "frame graph capture start"));
+  // Trace, advance the device seed, denoise, display.
+  Status cs = launch_trace_tiles(ctx, &it, 1, cfg, g->capture);
+  if (cs == LUCENT_OK) cs = launch_add_u32(seed_word, samples, g->capture);
+  if (cs == LUCENT_OK && denoise) cs = launch_denoise(da, g->capture);
+  if (denoise && disp.denoised) disp.denoised = film->denoised;  // the target the captured iterations end in
+  if (cs == LUCENT_OK && display) cs = launch_display(disp, g->capture);
+  e = hipStreamEndCapture(g->capture, &g->graph);
+  if (cs != LUCENT_OK) return bail(cs);
+  if (e != hipSuccess) return bail(fail_hip(e, "frame graph capture"));
+  if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess)
+    return bail(fail_hip(e, "hipGraphInstantiate"));
+  *out = g;
+  return LUCENT_OK;
+}
+
+lucent_status lucent_frame_graph_launch(lucent_frame_graph* g, void* stream) {
+  clear_error();
+  if (!g) return fail(LUCENT_E_INVALID, "null graph");
+  if (g->film->seed != g->expected_seed)
+    return fail(LUCENT_E_INVALID, "the film was advanced outside this graph; create a new graph");
+  const hipError_t e = hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream ? stream : g->ctx->stream));
+  if (e != hipSuccess) return fail_hip(e, "hipGraphLaunch");
+  g->film->seed += g->samples;
+  g->film->samples += g->samples;
+  g->expected_seed = g->film->seed;
+  return LUCENT_OK;
 }
 
 }  // extern "C"

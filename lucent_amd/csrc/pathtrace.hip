@@ -143,12 +143,13 @@ __global__ __launch_bounds__(kPtBlock) void k_trace_reprojected(PtScene sc, PtFr
 }
 
 template <class StackT>
-__device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t x, uint32_t y, StackT& st) {
+__device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t x, uint32_t y, StackT& st,
+                                                    uint32_t seed0) {
   const PtScene& sc = it.scene;
   const PtFrame& f = it.frame;
   const uint32_t idx = y * f.width + x;
   uint32_t k = 0;
-  pt::Rng rng = pt::seed_rng(idx, f.seed);
+  pt::Rng rng = pt::seed_rng(idx, seed0);
   pt::v3 ro, rd;
   pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
   pt::PathState p;
@@ -160,7 +161,7 @@ __device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t
             make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
       pt::accumulate(it.film, idx, p.radiance, p.guide);
       if (++k >= it.spp) break;
-      rng = pt::seed_rng(idx, f.seed + k);
+      rng = pt::seed_rng(idx, seed0 + k);
       pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
       pt::path_begin(p, ro, rd);
     }
@@ -169,9 +170,11 @@ __device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t
 
 template <class StackT>
 __device__ __forceinline__ void run_tile(const TileItem& it, uint32_t tile, StackT& st) {
+  // The first seed, read once per job in uniform control flow (a scalar).
+  const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : it.frame.seed;
   const uint32_t x = (tile % it.tiles_x) * kTile + (threadIdx.x & (kTile - 1));
   const uint32_t y = (tile / it.tiles_x) * kTile + (threadIdx.x / kTile);
-  if (x < it.frame.width && y < it.frame.height) trace_pixel_samples(it, x, y, st);
+  if (x < it.frame.width && y < it.frame.height) trace_pixel_samples(it, x, y, st, seed0);
 }
 
 // Jobs are numbered film by film: film i owns jobs [job_begin[i], job_begin[i+1]).
@@ -212,6 +215,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+__global__ void k_add_u32(uint32_t* p, uint32_t v) { *p += v; }
 
 }  // namespace
 
@@ -246,6 +250,21 @@ Status launch_trace_reprojected(Context* ctx, const PtScene& sc, const PtFrame& 
   return LUCENT_OK;
 }
 
+size_t tile_spill_bytes(const Context* ctx, const TileItem& item, const TileLaunch& cfg) {
+  const uint32_t waves = (cfg.occupancy >= 2 && cfg.occupancy <= 6) ? cfg.occupancy : 2;
+  const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
+  const uint32_t n_jobs = item.spp ? tile_jobs(item.frame.width, item.frame.height) : 0;
+  const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
+  return size_t(grid) * kPtBlock * std::max<uint32_t>(1, pt::pt_spill_words(item.scene.bvh)) * sizeof(uint32_t);
+}
+
+Status launch_add_u32(uint32_t* p, uint32_t v, void* stream) {
+  hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), p, v);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(e, "k_add_u32 launch");
+  return LUCENT_OK;
+}
+
 uint32_t tile_jobs(uint32_t width, uint32_t height) {
   return ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
 }
@@ -266,15 +285,28 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   const uint32_t waves = (cfg.occupancy >= 2 && cfg.occupancy <= 6) ? cfg.occupancy : 2;
   const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
-  Status s = ensure_spill(ctx, size_t(grid) * kPtBlock * depth * sizeof(uint32_t));
-  if (s != LUCENT_OK) return s;
+  const size_t spill_bytes = size_t(grid) * kPtBlock * depth * sizeof(uint32_t);
+  uint32_t* spill = cfg.spill;
+  uint32_t* counter = cfg.counter;
+  Status s;
+  if (cfg.counter) {
+    // Caller-owned buffers (a captured graph): one film, nothing to upload.
+    if (n_items != 1) return fail(LUCENT_E_INVALID, "caller-owned tile buffers take one film");
+    if (cfg.spill_bytes < spill_bytes) return fail(LUCENT_E_INVALID, "caller-owned spill buffer too small");
+  } else {
+    if ((s = ensure_spill(ctx, spill_bytes)) != LUCENT_OK) return s;
+    spill = static_cast<uint32_t*>(ctx->spill);
+  }
   // Device table: [counter | job_begin (n+1) | items], 256-byte aligned parts.
   const size_t begin_off = 256;
   const size_t items_off = (begin_off + job_begin.size() * sizeof(uint32_t) + 255) / 256 * 256;
   const size_t bytes = items_off + (n_items > 1 ? size_t(n_items) * sizeof(TileItem) : 0);
-  if ((s = ensure_tile_table(ctx, bytes)) != LUCENT_OK) return s;
-  char* table = static_cast<char*>(ctx->tile_table);
-  uint32_t* counter = reinterpret_cast<uint32_t*>(table);
+  char* table = nullptr;
+  if (!cfg.counter) {
+    if ((s = ensure_tile_table(ctx, bytes)) != LUCENT_OK) return s;
+    table = static_cast<char*>(ctx->tile_table);
+    counter = reinterpret_cast<uint32_t*>(table);
+  }
   if (n_items > 1) {
     std::vector<char> host(bytes - begin_off);
     std::memcpy(host.data(), job_begin.data(), job_begin.size() * sizeof(uint32_t));
@@ -285,9 +317,8 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
     if (e != hipSuccess) return fail_hip(e, "tile table upload");
   }
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
-  const TileItem* d_items = reinterpret_cast<const TileItem*>(table + items_off);
-  const uint32_t* d_begin = reinterpret_cast<const uint32_t*>(table + begin_off);
-  uint32_t* spill = static_cast<uint32_t*>(ctx->spill);
+  const TileItem* d_items = n_items > 1 ? reinterpret_cast<const TileItem*>(table + items_off) : nullptr;
+  const uint32_t* d_begin = n_items > 1 ? reinterpret_cast<const uint32_t*>(table + begin_off) : nullptr;
   auto launch = [&](auto single_kernel, auto multi_kernel) {
     if (n_items == 1)
       hipLaunchKernelGGL(single_kernel, dim3(grid), dim3(kPtBlock), 0, stream, items[0], nullptr, nullptr, 1u, n_jobs,

@@ -102,6 +102,9 @@ struct TileItem {
   // Optional: each path's radiance as tracePath returns it, float4 per pixel
   // per sample, sample-major (sample k of the launch at k * width * height).
   float* sample_radiance = nullptr;
+  // Optional: a device word holding the first seed, read instead of
+  // frame.seed (a replayed graph advances it on the device).
+  const uint32_t* seed_src = nullptr;
 };
 
 }  // namespace lucent

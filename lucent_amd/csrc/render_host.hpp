@@ -31,8 +31,17 @@ Status launch_trace_reprojected(Context* ctx, const PtScene& sc, const PtFrame& 
 // Launch shape of the persistent tile kernel (0 = default).
 struct TileLaunch {
   uint32_t waves_per_cu = 0;  // persistent waves per CU (0: 4 x occupancy)
-  uint32_t occupancy = 0;     // register-allocation target in waves per SIMD: 2, 3 or 4 (0: 2)
+  uint32_t occupancy = 0;     // register-allocation target in waves per SIMD: 2..6 (0: 2)
+  // Optional caller-owned job counter and spill buffer (one film only);
+  // otherwise the context's buffers are used.
+  uint32_t* counter = nullptr;
+  uint32_t* spill = nullptr;
+  size_t spill_bytes = 0;
 };
+// Spill bytes the launch of one film with `cfg` needs.
+size_t tile_spill_bytes(const Context* ctx, const TileItem& item, const TileLaunch& cfg);
+// *p += v on the device (a graph node).
+Status launch_add_u32(uint32_t* p, uint32_t v, void* stream);
 uint32_t tile_jobs(uint32_t width, uint32_t height);
 // Runs items[i].spp samples of every pixel of each item's film, seeds from
 // items[i].frame.seed; fills items[i].tiles_x.

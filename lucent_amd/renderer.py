@@ -294,6 +294,46 @@ class Renderer:
             self.device_scene.destroy()
 
 
+class FrameGraph:
+    """One progressive frame captured into a HIP graph (lucent_frame_graph_*):
+    `samples` samples of the renderer's film, then optionally the denoiser and
+    the display pass into device memory d_rgba8.  launch() replays it; n
+    replays equal n eager frames bit for bit.  Camera, settings and filter
+    parameters are fixed at capture."""
+
+    def __init__(self, renderer: "Renderer", samples: int, denoise: Optional[tuple] = None,
+                 display: Optional[tuple] = None, d_rgba8: Optional[int] = None, view: Optional[tuple] = None):
+        r = renderer
+        self.renderer = r
+        cam = camera_struct(r.scene.camera)
+        st = r.settings.struct()
+        dn = _native.DenoiseParamsStruct(*denoise) if denoise is not None else None
+        dp = _native.DisplayParamsStruct(*display) if display is not None else None
+        vw, vh = view or (r.film.width, r.film.height)
+        h = C.c_void_p()
+        _native.check(_native.lib().lucent_frame_graph_create(
+            r.ctx.handle, r.device_scene.handle, r.film.handle, C.byref(cam), C.byref(st), samples,
+            C.byref(dn) if dn is not None else None, C.byref(dp) if dp is not None else None, vw, vh, d_rgba8,
+            C.byref(h)))
+        self._h = h
+        self.samples = samples
+
+    def launch(self, stream: Optional[int] = None) -> None:
+        _native.check(_native.lib().lucent_frame_graph_launch(self._h, stream))
+        self.renderer.history_camera = self.renderer.scene.camera
+
+    def close(self) -> None:
+        if self._h is not None:
+            _native.lib().lucent_frame_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _batch_items(ctx: Context, jobs):
     arr = (_native.BatchItemStruct * max(1, len(jobs)))()
     for i, (r, n) in enumerate(jobs):

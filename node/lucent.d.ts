@@ -22,12 +22,15 @@ export interface Camera { position: number[]; target: number[]; fov: number; ape
 export interface FilmTargets { color: Float32Array; albedo: Float32Array; normal: Float32Array; history: Float32Array; }
 export declare class LucentRenderer {
   constructor(opts: { width: number; height: number; device?: number });
-  settings: { maxBounces: number; fireflyClamp: number };
+  settings: { maxBounces: number; fireflyClamp: number; temporal: boolean; denoise: boolean; exposure: number;
+    tonemap: 'aces' | 'reinhard' | 'none' };
   loadScene(packed: object, scene: object): void;
   uploadLighting(scene: object): void;
-  uploadCamera(cam: Camera): void;
+  uploadCamera(cam: Camera, moving?: boolean): void;
   reset(): void;
   traceSamples(count: number): void;
+  render(samples?: number): boolean;
+  screenshot(viewWidth?: number, viewHeight?: number): { width: number; height: number; data: Uint8Array };
   benchmark(samples: number): { traceMs: number; mpathsPerSec: number; denoiseMs: number; width: number; height: number };
   runDenoiser(params?: { sigmaL: number; sigmaN: number; sigmaP: number; iterations: number }):
     { illumVar: Float32Array; radiance: Float32Array };

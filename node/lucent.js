@@ -7,7 +7,8 @@
  *   buildBVH(positions, options)          src/math/bvh.ts:52
  *   GpuTraversal#intersect / pickDistance src/math/bvh.ts:265, src/scene/scene.ts:327
  *   LucentRenderer                        src/renderer.ts:171 (loadScene, uploadLighting,
- *                                         uploadCamera, reset, traceSamples, benchmark,
+ *                                         uploadCamera with temporal reprojection, reset,
+ *                                         traceSamples, render, screenshot, benchmark,
  *                                         runDenoiser, stats, dispose)
  * Types are in lucent.d.ts.
  */
@@ -17,6 +18,11 @@ const path = require('path');
 const addon = require(path.join(__dirname, 'build', 'lucent_napi.node'));
 
 const QUERY_ANY = 1;
+/** Samples a reprojected pixel may carry (HISTORY_CAP, renderer.ts:139). */
+const HISTORY_CAP = 8;
+const TONEMAPS = { aces: 0, reinhard: 1, none: 2 };
+/** atrousLevels / denoiseWeight (renderer.ts:142-144). */
+const denoiseWeight = (spp) => Math.min(1, Math.max(0, (1024 - spp) / (1024 - 128)));
 const PREC_GLSL_FP32 = 2;
 const MISS = 0xffffffff;
 
@@ -91,11 +97,15 @@ class LucentRenderer {
     this.width = opts.width;
     this.height = opts.height;
     this.film = addon.createFilm(this.ctx, opts.width, opts.height);
-    /** RenderSettings (renderer.ts:42-84), integrator part. */
-    this.settings = { maxBounces: 8, fireflyClamp: 8 };
+    /** RenderSettings (renderer.ts:42-84): integrator and presentation parts. */
+    this.settings = { maxBounces: 8, fireflyClamp: 8, temporal: true, denoise: true, exposure: 1, tonemap: 'aces' };
     this.scene = null;
     this.camera = null;
     this.seed = 0;
+    /** The camera the film's samples were traced with, and a pending reprojection's previous camera. */
+    this.historyCamera = null;
+    this.reprojectFrom = null;
+    this.denoisedFrame = -1;
   }
 
   /**
@@ -118,20 +128,70 @@ class LucentRenderer {
     this.reset();
   }
 
-  /** uploadCamera (renderer.ts:424). */
-  uploadCamera(cam) {
+  /**
+   * uploadCamera + invalidateView (renderer.ts:424-451): with temporal reuse
+   * and samples in the film the next sample reprojects the history,
+   * otherwise the film starts over.  `moving` only drives the reference's
+   * preview scaling.
+   */
+  uploadCamera(cam, moving) {
     this.camera = cam;
-    this.reset();
+    if (this.settings.temporal && this.historyCamera && addon.filmInfo(this.film).samples > 0) {
+      this.reprojectFrom = this.historyCamera;
+      this.denoisedFrame = -1;
+    } else {
+      this.reset();
+    }
   }
 
   /** reset (renderer.ts:463): the RNG stream keeps advancing. */
   reset() {
     addon.resetFilm(this.ctx, this.film);
+    this.historyCamera = null;
+    this.reprojectFrom = null;
+    this.denoisedFrame = -1;
   }
 
-  /** traceSamples (renderer.ts:702): `count` progressive samples. */
+  /** traceSamples (renderer.ts:702): `count` progressive samples, the first reprojecting after a camera move. */
   traceSamples(count) {
-    addon.traceSamples(this.ctx, this.scene, this.film, this.camera, this.settings, count);
+    if (count <= 0) return;
+    if (this.reprojectFrom) {
+      addon.traceReprojected(this.ctx, this.scene, this.film, this.camera, this.film, this.reprojectFrom, HISTORY_CAP,
+        this.settings);
+      this.reprojectFrom = null;
+      count -= 1;
+    }
+    if (count > 0) addon.traceSamples(this.ctx, this.scene, this.film, this.camera, this.settings, count);
+    this.historyCamera = this.camera;
+  }
+
+  /**
+   * render (renderer.ts:552) without the GPU budget scheduler: traces
+   * `samples` samples and denoises when the denoise weight is above 0.
+   */
+  render(samples) {
+    this.traceSamples(samples === undefined ? 1 : samples);
+    const spp = addon.filmInfo(this.film).samples;
+    if (this.settings.denoise && spp > 0 && denoiseWeight(spp) > 0 && this.denoisedFrame !== spp) {
+      this.runDenoiser();
+      this.denoisedFrame = spp;
+    }
+    return true;
+  }
+
+  /**
+   * screenshot (renderer.ts:857) without the PNG encoding: the displayed
+   * image (display.frag) as RGBA8, rows bottom to top.
+   * @param {number} [viewWidth] defaults to the film size; larger views are Catmull-Rom upsampled
+   * @param {number} [viewHeight]
+   */
+  screenshot(viewWidth, viewHeight) {
+    const spp = addon.filmInfo(this.film).samples;
+    const w = this.settings.denoise && spp > 0 && this.denoisedFrame === spp ? denoiseWeight(spp) : 0;
+    const vw = viewWidth || this.width, vh = viewHeight || this.height;
+    const data = addon.screenshot(this.ctx, this.film, { denoiseWeight: w, exposure: this.settings.exposure,
+      tonemap: TONEMAPS[this.settings.tonemap] }, vw, vh);
+    return { width: vw, height: vh, data };
   }
 
   /** benchmark (renderer.ts:745): warm-up sample, then `samples` timed. */

@@ -442,6 +442,61 @@ static int read_camera(napi_env env, napi_value c, lucent_camera *cam) {
 }
 
 /* traceSamples(ctx, scene, film, camera, {maxBounces, fireflyClamp}, count) — traceSamples (renderer.ts:702). */
+/* RenderSettings fields of the integrator (renderer.ts:42-84). */
+static void read_settings(napi_env env, napi_value o, lucent_render_settings *rs) {
+  rs->max_bounces = (uint32_t)num(env, prop(env, o, "maxBounces"), 8);
+  rs->firefly_clamp = (float)num(env, prop(env, o, "fireflyClamp"), 8);
+  rs->flags = num(env, prop(env, o, "perSampleLaunch"), 0) != 0 ? LUCENT_TRACE_PER_SAMPLE : 0u;
+  rs->waves_per_cu = (uint32_t)num(env, prop(env, o, "wavesPerCU"), 0);
+  rs->occupancy = (uint32_t)num(env, prop(env, o, "occupancy"), 0);
+}
+
+/* traceReprojected(ctx, scene, film, camera, prevFilm, prevCamera, historyCap, settings):
+ * the uReproject = 1 sample of traceSamples (renderer.ts:715-730). */
+static napi_value js_trace_reprojected(napi_env env, napi_callback_info info) {
+  size_t argc = 8;
+  napi_value argv[8];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  scene_holder *sh = argc > 1 ? (scene_holder *)get_external(env, argv[1]) : NULL;
+  film_holder *fh = argc > 2 ? (film_holder *)get_external(env, argv[2]) : NULL;
+  film_holder *ph = argc > 4 ? (film_holder *)get_external(env, argv[4]) : NULL;
+  lucent_camera cam, prev;
+  if (!ctx || !sh || !fh || !ph || argc < 8 || !read_camera(env, argv[3], &cam) || !read_camera(env, argv[5], &prev)) {
+    napi_throw_type_error(env, NULL, "traceReprojected(ctx, scene, film, camera, prevFilm, prevCamera, historyCap, settings)");
+    return NULL;
+  }
+  lucent_render_settings rs;
+  read_settings(env, argv[7], &rs);
+  CHECK_LUCENT(lucent_trace_reprojected(ctx, sh->scene, fh->film, &cam, ph->film, &prev, (float)num(env, argv[6], 8), &rs,
+                                        NULL));
+  CHECK_LUCENT(lucent_synchronize(ctx));
+  return NULL;
+}
+
+/* screenshot(ctx, film, {denoiseWeight, exposure, tonemap}, viewW, viewH) -> Uint8Array RGBA8
+ * (rows bottom to top): the display pass (display.frag). */
+static napi_value js_screenshot(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  film_holder *fh = argc > 1 ? (film_holder *)get_external(env, argv[1]) : NULL;
+  if (!ctx || !fh || argc < 5) {
+    napi_throw_type_error(env, NULL, "screenshot(ctx, film, params, viewW, viewH)");
+    return NULL;
+  }
+  lucent_display_params p;
+  p.denoise_weight = (float)num(env, prop(env, argv[2], "denoiseWeight"), 0);
+  p.exposure = (float)num(env, prop(env, argv[2], "exposure"), 1);
+  p.tonemap = (uint32_t)num(env, prop(env, argv[2], "tonemap"), 0);
+  uint32_t vw = (uint32_t)num(env, argv[3], 0), vh = (uint32_t)num(env, argv[4], 0);
+  void *px;
+  napi_value out = new_typed(env, napi_uint8_array, (size_t)vw * vh * 4, 1, &px);
+  CHECK_LUCENT(lucent_screenshot(ctx, fh->film, &p, vw, vh, (uint8_t *)px));
+  return out;
+}
+
 static napi_value js_trace_samples(napi_env env, napi_callback_info info) {
   size_t argc = 6;
   napi_value argv[6];
@@ -455,11 +510,7 @@ static napi_value js_trace_samples(napi_env env, napi_callback_info info) {
     return NULL;
   }
   lucent_render_settings rs;
-  rs.max_bounces = (uint32_t)num(env, prop(env, argv[4], "maxBounces"), 8);
-  rs.firefly_clamp = (float)num(env, prop(env, argv[4], "fireflyClamp"), 8);
-  rs.flags = num(env, prop(env, argv[4], "perSampleLaunch"), 0) != 0 ? LUCENT_TRACE_PER_SAMPLE : 0u;
-  rs.waves_per_cu = (uint32_t)num(env, prop(env, argv[4], "wavesPerCU"), 0);
-  rs.occupancy = (uint32_t)num(env, prop(env, argv[4], "occupancy"), 0);
+  read_settings(env, argv[4], &rs);
   CHECK_LUCENT(lucent_trace_samples(ctx, sh->scene, fh->film, &cam, &rs, (uint32_t)num(env, argv[5], 1), NULL));
   CHECK_LUCENT(lucent_synchronize(ctx));
   return NULL;
@@ -538,6 +589,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"resetFilm", 0, js_reset_film, 0, 0, 0, napi_default, 0},
       {"filmInfo", 0, js_film_info, 0, 0, 0, napi_default, 0},
       {"traceSamples", 0, js_trace_samples, 0, 0, 0, napi_default, 0},
+      {"traceReprojected", 0, js_trace_reprojected, 0, 0, 0, napi_default, 0},
+      {"screenshot", 0, js_screenshot, 0, 0, 0, napi_default, 0},
       {"readFilm", 0, js_read_film, 0, 0, 0, napi_default, 0},
       {"denoise", 0, js_denoise, 0, 0, 0, napi_default, 0},
   };

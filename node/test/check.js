@@ -1,7 +1,8 @@
 // Test driver for the N-API facade (run by tests/test_node_addon.py).
 //   node check.js bvh <dir>     buildBVH over <dir>/positions.f32 -> nodes.f32, tri.u32
 //   node check.js hits <dir>    GpuTraversal over positions + bvh, rays org.f64/dir.f64 -> t/u/v.f64, tri.u32
-//   node check.js render <dir>  LucentRenderer over the packed scene in <dir> -> color.f32, albedo.f32
+//   node check.js render <dir>  LucentRenderer over the packed scene in <dir> -> color.f32, albedo.f32,
+//                               den.f32, then a camera move -> moved.f32, shot.u8
 'use strict';
 const fs = require('fs');
 const path = require('path');
@@ -40,7 +41,7 @@ if (mode === 'bvh') {
   const scene = Object.assign({}, meta.scene);
   scene.env = Object.assign({}, meta.scene.env, { map: { width: meta.envW, height: meta.envH, data: f32('env.f32') } });
   const r = new lucent.LucentRenderer({ width: meta.width, height: meta.height });
-  r.settings = { maxBounces: 8, fireflyClamp: 0 };
+  r.settings = { maxBounces: 8, fireflyClamp: 0, temporal: true, denoise: true, exposure: 1, tonemap: 'aces' };
   r.loadScene(packed, scene);
   lucent.addon.resetFilm(r.ctx, r.film, meta.seed);
   r.traceSamples(meta.spp);
@@ -49,5 +50,11 @@ if (mode === 'bvh') {
   write('albedo.f32', out.albedo);
   const d = r.runDenoiser({ sigmaL: 4, sigmaN: 64, sigmaP: 1.5, iterations: 3 });
   write('den.f32', d.illumVar);
-  process.stdout.write(JSON.stringify(r.stats()) + '\n');
+  const before = r.stats();
+  // A camera move with temporal reuse: one reprojecting sample, one plain sample, then the displayed image.
+  r.uploadCamera(meta.camera2);
+  r.traceSamples(2);
+  write('moved.f32', r.read().color);
+  write('shot.u8', r.screenshot().data);
+  process.stdout.write(JSON.stringify(Object.assign(before, { movedSamples: r.stats().samples })) + '\n');
 }

@@ -74,6 +74,7 @@ def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
     from test_pathtrace_gpu import feature_scene
     from lucent_amd.renderer import Renderer, RenderSettings
     from lucent_amd.scene import pack_scene
+    from lucent_amd import scene as S
     sc = feature_scene()
     W, H, SPP, SEED = 48, 32, 3, 9
     p = pack_scene(sc)
@@ -94,9 +95,14 @@ def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
                 "backdrop": list(sc.env.backdrop)},
         "ground": {"enabled": sc.ground.enabled, "height": sc.ground.height},
     }
+    c2 = S.Camera(position=(0.4, 2.6, 5.9), target=sc.camera.target, fov=sc.camera.fov, aperture=sc.camera.aperture,
+                  focus_distance=sc.camera.focus_distance)
+    scene_json["camera2"] = {"position": list(c2.position), "target": list(c2.target), "fov": c2.fov,
+                             "aperture": c2.aperture, "focusDistance": c2.focus_distance}
     meta = {"triCount": p.tri_count, "nodeCount": p.bvh_node_count, "materialCount": p.material_count,
             "sphereCount": p.sphere_count, "lightCount": p.light_count, "groundMaterial": p.ground_material,
-            "envW": w, "envH": h, "width": W, "height": H, "spp": SPP, "seed": SEED, "scene": scene_json}
+            "envW": w, "envH": h, "width": W, "height": H, "spp": SPP, "seed": SEED, "scene": scene_json,
+            "camera2": scene_json.pop("camera2")}
     (tmp_path / "meta.json").write_text(json.dumps(meta))
     stats = node("render", tmp_path)
     assert stats["samples"] == SPP and stats["seed"] == SEED + SPP
@@ -108,7 +114,14 @@ def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
     ref = r.film.read()
     r.run_denoiser(4.0, 64.0, 1.5, 3)
     den = r.read_denoised()
+    r.upload_camera(c2)
+    r.trace_samples(2)
+    moved = r.film.read()["color"]
+    shot = r.screenshot()
     r.close()
+    assert stats["movedSamples"] == SPP + 2
+    assert np.array_equal(np.fromfile(tmp_path / "moved.f32", np.float32).reshape(H, W, 4), moved)
+    assert np.array_equal(np.fromfile(tmp_path / "shot.u8", np.uint8).reshape(H, W, 4), shot)
     assert np.array_equal(np.fromfile(tmp_path / "color.f32", np.float32).reshape(H, W, 4), ref["color"])
     assert np.array_equal(np.fromfile(tmp_path / "albedo.f32", np.float32).reshape(H, W, 4), ref["albedo"])
     assert np.array_equal(np.fromfile(tmp_path / "den.f32", np.float32).reshape(H, W, 4), den["illum_var"])
