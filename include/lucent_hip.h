@@ -397,6 +397,19 @@ lucent_status lucent_screenshot(lucent_ctx *ctx, const lucent_film *film, const 
                                 uint32_t view_w, uint32_t view_h, uint8_t *rgba8);
 
 /* ------------------------------------------------------------------------
+ * Asset ingestion.  lucent_decode_hdr replaces parseHDR
+ * (src/loaders/hdr.ts:7-93): a Radiance RGBE file (flat or run-length
+ * encoded scanlines, any axis order) in memory -> width, height and RGB
+ * floats with rows top to bottom, the environment-map layout
+ * (HDRImage, environment.ts:6-10).  Pass rgb = NULL to query the size
+ * only; otherwise rgb_capacity (floats) must hold width * height * 3.
+ * Errors (LUCENT_E_INVALID) carry parseHDR's messages.  Host-only, no
+ * context needed.
+ * ---------------------------------------------------------------------- */
+lucent_status lucent_decode_hdr(const uint8_t *bytes, size_t length, uint32_t *width, uint32_t *height, float *rgb,
+                                size_t rgb_capacity);
+
+/* ------------------------------------------------------------------------
  * Frame graphs: one progressive frame (`samples` samples of every pixel of
  * `film` through the persistent tile launch, then optionally lucent_denoise
  * with *denoise and lucent_display with *display into d_rgba8) captured

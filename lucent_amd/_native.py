@@ -158,6 +158,7 @@ SIGNATURES: dict[str, tuple] = {
                                          C.POINTER(DenoiseParamsStruct), C.POINTER(DisplayParamsStruct), _U32, _U32,
                                          _P, C.POINTER(_P)]),
     "lucent_frame_graph_launch": (_I32, [_P, _P]),
+    "lucent_decode_hdr": (_I32, [_P, C.c_size_t, C.POINTER(_U32), C.POINTER(_U32), _P, C.c_size_t]),
     "lucent_frame_graph_destroy": (None, [_P]),
     "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,
                                 _U32, _P, _P, _P, _P, _P]),

@@ -20,6 +20,8 @@ export declare class GpuTraversal {
 }
 export interface Camera { position: number[]; target: number[]; fov: number; aperture: number; focusDistance: number; }
 export interface FilmTargets { color: Float32Array; albedo: Float32Array; normal: Float32Array; history: Float32Array; }
+export declare function parseHDR(buffer: ArrayBuffer | Uint8Array): { width: number; height: number; data: Float32Array };
+
 export declare class LucentRenderer {
   constructor(opts: { width: number; height: number; device?: number });
   settings: { maxBounces: number; fireflyClamp: number; temporal: boolean; denoise: boolean; exposure: number;

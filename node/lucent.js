@@ -5,6 +5,7 @@
  * Node 12).  It mirrors the reference's TypeScript entry points on this
  * path so that a caller swaps implementations, not call sites:
  *   buildBVH(positions, options)          src/math/bvh.ts:52
+ *   parseHDR(buffer)                      src/loaders/hdr.ts:7
  *   GpuTraversal#intersect / pickDistance src/math/bvh.ts:265, src/scene/scene.ts:327
  *   LucentRenderer                        src/renderer.ts:171 (loadScene, uploadLighting,
  *                                         uploadCamera with temporal reprojection, reset,
@@ -33,6 +34,17 @@ const MISS = 0xffffffff;
  */
 function buildBVH(positions, options) {
   return addon.buildBVH(positions, options || {});
+}
+
+/**
+ * parseHDR (src/loaders/hdr.ts:7): a Radiance .hdr file -> HDRImage
+ * {width, height, data} (RGB floats, rows top to bottom).  Throws Error
+ * with parseHDR's messages.
+ * @param {ArrayBuffer|Uint8Array} buffer
+ */
+function parseHDR(buffer) {
+  const bytes = buffer instanceof Uint8Array ? buffer : new Uint8Array(buffer);
+  return addon.decodeHDR(bytes);
 }
 
 /** Batched closest-hit queries against one uploaded BVH (intersectBVH on the GPU). */
@@ -248,4 +260,4 @@ class LucentRenderer {
   }
 }
 
-module.exports = { buildBVH, GpuTraversal, LucentRenderer, cameraBasis, version: addon.version, addon, MISS };
+module.exports = { buildBVH, parseHDR, GpuTraversal, LucentRenderer, cameraBasis, version: addon.version, addon, MISS };

@@ -442,6 +442,39 @@ static int read_camera(napi_env env, napi_value c, lucent_camera *cam) {
 }
 
 /* traceSamples(ctx, scene, film, camera, {maxBounces, fireflyClamp}, count) — traceSamples (renderer.ts:702). */
+/* decodeHDR(bytes: Uint8Array | Buffer) -> {width, height, data: Float32Array} (parseHDR, hdr.ts:7). */
+static napi_value js_decode_hdr(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  bool is_ta = false;
+  if (argc < 1 || napi_is_typedarray(env, argv[0], &is_ta) != napi_ok || !is_ta) {
+    napi_throw_type_error(env, NULL, "decodeHDR(bytes: Uint8Array)");
+    return NULL;
+  }
+  napi_typedarray_type t;
+  size_t len;
+  void *data;
+  napi_value ab;
+  size_t off;
+  CHECK_NAPI(napi_get_typedarray_info(env, argv[0], &t, &len, &data, &ab, &off));
+  if (t != napi_uint8_array) {
+    napi_throw_type_error(env, NULL, "decodeHDR(bytes: Uint8Array)");
+    return NULL;
+  }
+  uint32_t w = 0, h = 0;
+  CHECK_LUCENT(lucent_decode_hdr((const uint8_t *)data, len, &w, &h, NULL, 0));
+  void *rgb;
+  napi_value arr = new_typed(env, napi_float32_array, (size_t)w * h * 3, 4, &rgb);
+  CHECK_LUCENT(lucent_decode_hdr((const uint8_t *)data, len, &w, &h, (float *)rgb, (size_t)w * h * 3));
+  napi_value out;
+  CHECK_NAPI(napi_create_object(env, &out));
+  napi_set_named_property(env, out, "width", make_number(env, w));
+  napi_set_named_property(env, out, "height", make_number(env, h));
+  napi_set_named_property(env, out, "data", arr);
+  return out;
+}
+
 /* RenderSettings fields of the integrator (renderer.ts:42-84). */
 static void read_settings(napi_env env, napi_value o, lucent_render_settings *rs) {
   rs->max_bounces = (uint32_t)num(env, prop(env, o, "maxBounces"), 8);
@@ -580,6 +613,7 @@ static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"version", 0, js_version, 0, 0, 0, napi_default, 0},
       {"buildBVH", 0, js_build_bvh, 0, 0, 0, napi_default, 0},
+      {"decodeHDR", 0, js_decode_hdr, 0, 0, 0, napi_default, 0},
       {"createContext", 0, js_create_context, 0, 0, 0, napi_default, 0},
       {"sceneFromBVH", 0, js_scene_from_bvh, 0, 0, 0, napi_default, 0},
       {"uploadScene", 0, js_upload_scene, 0, 0, 0, napi_default, 0},

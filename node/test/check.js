@@ -1,6 +1,7 @@
 // Test driver for the N-API facade (run by tests/test_node_addon.py).
 //   node check.js bvh <dir>     buildBVH over <dir>/positions.f32 -> nodes.f32, tri.u32
 //   node check.js hits <dir>    GpuTraversal over positions + bvh, rays org.f64/dir.f64 -> t/u/v.f64, tri.u32
+//   node check.js hdr <dir>     parseHDR(in.hdr) -> rgb.f32
 //   node check.js render <dir>  LucentRenderer over the packed scene in <dir> -> color.f32, albedo.f32,
 //                               den.f32, then a camera move -> moved.f32, shot.u8
 'use strict';
@@ -22,6 +23,12 @@ if (mode === 'bvh') {
   let err = '';
   try { lucent.buildBVH(new Float32Array(10)); } catch (e) { err = e.message; }
   process.stdout.write(JSON.stringify({ nodeCount: bvh.nodeCount, bounds: bvh.bounds, error: err }) + '\n');
+} else if (mode === 'hdr') {
+  const img = lucent.parseHDR(fs.readFileSync(path.join(dir, 'in.hdr')));
+  write('rgb.f32', img.data);
+  let err = '';
+  try { lucent.parseHDR(Buffer.from('P6\n1 1\n255\n')); } catch (e) { err = e.message; }
+  process.stdout.write(JSON.stringify({ width: img.width, height: img.height, error: err }) + '\n');
 } else if (mode === 'hits') {
   const bvh = { nodes: f32('nodes.f32'), triIndices: u32('tri.u32') };
   const tr = new lucent.GpuTraversal(bvh, f32('positions.f32'));

@@ -67,6 +67,21 @@ def test_intersect_through_node_bit_identical(tmp_path):
         assert meta["first"]["triIndex"] == int(tri[0]) and meta["first"]["t"] == t[0]
 
 
+def test_parse_hdr_through_node(tmp_path):
+    """parseHDR through the addon gives the C-ABI decoder's floats and its error messages."""
+    from lucent_amd.assets import decode_hdr
+    width = 8
+    data = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 2 +X 8\n" + bytes(
+        [2, 2, 0, width, 128 + 8, 200, 4, 1, 2, 3, 4, 128 + 4, 9, 128 + 8, 0, 128 + 8, 128] +
+        [7, 8, 9, 130] * width)
+    (tmp_path / "in.hdr").write_bytes(data)
+    meta = node("hdr", tmp_path)
+    w, h, rgb = decode_hdr(data)
+    assert (meta["width"], meta["height"]) == (w, h) == (8, 2)
+    assert np.array_equal(np.fromfile(tmp_path / "rgb.f32", np.float32).reshape(h, w, 3), rgb)
+    assert "Radiance" in meta["error"]
+
+
 @pytest.mark.gpu
 def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
     import sys
