@@ -1,12 +1,15 @@
 """Progressive renderer — the host mirror of Renderer (src/renderer.ts:171-871):
 load_scene / upload_lighting / upload_camera (with temporal reprojection) /
 reset / trace_samples / run_denoiser / render / screenshot / benchmark /
-read-back.  Preview scaling and the GPU budget scheduler are not part of it:
-render() traces the sample count it is given.
+read-back, with the GPU-time budget scheduler (perf.ts) behind render().
+The moving preview's automatic resolution drop is not part of it; a
+caller that renders a preview film reprojects from it with
+lucent_trace_reprojected.
 """
 from __future__ import annotations
 
 import ctypes as C
+import math
 import time
 from dataclasses import dataclass
 from typing import Optional
@@ -14,6 +17,7 @@ from typing import Optional
 import numpy as np
 
 from . import _native
+from .perf import GpuBudget, affordable_samples
 from .runtime import Context, Scene as DeviceScene
 from .scene import PackedScene, Scene, pack_scene, texture_arrays
 
@@ -36,6 +40,12 @@ class RenderSettings:
     denoise: bool = True
     exposure: float = 1.0
     tonemap: str = "aces"
+    # Scheduling part (renderer.ts:42-84, perf.ts): samples per render();
+    # 0 lets the GPU-time budget decide.
+    samples_per_frame: int = 0
+    gpu_budget: float = 0.5
+    fps_cap: float = 60.0
+    max_samples: int = 0
 
     def struct(self) -> "_native.RenderSettingsStruct":
         flags = (_native.TRACE_PER_SAMPLE if self.per_sample_launch else 0) | \
@@ -71,6 +81,7 @@ def lighting_struct(scene: Scene, keep: list) -> _native.LightingStruct:
 
 
 HISTORY_CAP = 8  # renderer.ts:139
+MOTION_HOLD_MS = 160  # renderer.ts:133
 TONEMAPS = {"aces": 0, "reinhard": 1, "none": 2}
 
 
@@ -156,6 +167,10 @@ class Renderer:
         self.history_camera = None     # the camera the film's samples were traced with
         self.reproject_from = None     # pending reprojection: the previous view's camera
         self.denoised_frame = -1
+        self.budget = GpuBudget(self.settings.gpu_budget)
+        self.ms_per_sample = 0.0
+        self.last_sync_measure = -math.inf
+        self.last_motion = -math.inf
 
     @property
     def frame(self) -> int:
@@ -200,6 +215,8 @@ class Renderer:
         history instead of starting over.  `moving` only matters to the
         reference's preview scaling."""
         self.scene.camera = camera
+        if moving:
+            self.last_motion = time.perf_counter() * 1e3
         if self.settings.temporal and self.history_camera is not None and self.frame > 0:
             self.reproject_from = self.history_camera
             self.denoised_frame = -1
@@ -231,11 +248,54 @@ class Renderer:
         w = denoise_weight(spp) if (self.settings.denoise and spp > 0 and self.denoised_frame == spp) else 0.0
         return _native.DisplayParamsStruct(w, self.settings.exposure, TONEMAPS[self.settings.tonemap])
 
-    def render(self, samples: int = 1, stream: Optional[int] = None) -> int:
-        """Renderer.render (renderer.ts:552-600) without the budget scheduler:
-        traces `samples` samples, runs the denoiser when its weight is above 0
-        and the film changed, and leaves the image to display()/screenshot()."""
-        self.trace_samples(samples, stream)
+    def _measure_sample(self) -> float:
+        """measureSample (renderer.ts:664-670): one sample timed between two waits."""
+        self.ctx.synchronize()
+        t0 = time.perf_counter()
+        self.trace_samples(1)
+        self.ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    def trace_budgeted(self, now_ms: float) -> int:
+        """traceBudgeted (renderer.ts:618-649): spends this frame's share of the
+        GPU-time budget on samples; the sample cost is re-measured every 2 s."""
+        st = self.settings
+        remaining = st.max_samples - self.frame if st.max_samples > 0 else math.inf
+        if st.samples_per_frame > 0:
+            n = int(min(st.samples_per_frame, remaining))
+            self.trace_samples(n)
+            return n
+        self.budget.budget = st.gpu_budget
+        frame_ms = 1000.0 / (st.fps_cap if st.fps_cap > 0 else 60.0)
+        if self.ms_per_sample <= 0 or now_ms - self.last_sync_measure > 2000:
+            ms = self._measure_sample()
+            self.ms_per_sample = self.ms_per_sample * 0.6 + ms * 0.4 if self.ms_per_sample > 0 else ms
+            self.last_sync_measure = now_ms
+            self.budget.allowance(now_ms, 0)
+            self.budget.spend(ms)
+            return 1
+        allowance = self.budget.allowance(now_ms, max(frame_ms * st.gpu_budget * 2, self.ms_per_sample * 1.05))
+        # Moving the camera always gets a sample; the budget is repaid once it stops.
+        interactive = now_ms - self.last_motion < MOTION_HOLD_MS
+        n = int(min(1 if interactive else affordable_samples(allowance, self.ms_per_sample), remaining))
+        if n <= 0:
+            return 0
+        self.trace_samples(n)
+        self.budget.spend(n * self.ms_per_sample)
+        return n
+
+    def render(self, samples: Optional[int] = None, stream: Optional[int] = None,
+               now_ms: Optional[float] = None) -> int:
+        """Renderer.render (renderer.ts:552-600): traces `samples` samples, or
+        with samples None what the settings' samples_per_frame or GPU-time
+        budget pay for (nothing once max_samples is reached), runs the
+        denoiser when its weight is above 0 and the film changed, and leaves
+        the image to display()/screenshot().  Returns the samples traced."""
+        if samples is None:
+            done = self.settings.max_samples > 0 and self.frame >= self.settings.max_samples
+            samples = 0 if done else self.trace_budgeted(time.perf_counter() * 1e3 if now_ms is None else now_ms)
+        else:
+            self.trace_samples(samples, stream)
         spp = self.frame
         if self.settings.denoise and spp > 0 and denoise_weight(spp) > 0 and self.denoised_frame != spp:
             self.run_denoiser(stream=stream)
