@@ -242,7 +242,9 @@ __device__ inline bool hit_triangle(v3 ro, v3 rd, v3 v0, v3 v1, v3 v2, float tma
   const v3 p = cross(rd, e2);
   const float det = dot(e1, p);
   if (fabsf(det) < 1e-12f) return false;
-  const float inv_det = 1.0f / det;
+  // GLSL's division is accurate to 2.5 ulp; the hardware reciprocal (1 ulp)
+  // stays within that and replaces the IEEE division sequence.
+  const float inv_det = __builtin_amdgcn_rcpf(det);
   const v3 s = ro - v0;
   const float u = dot(s, p) * inv_det;
   if (u < 0.0f || u > 1.0f) return false;
@@ -326,35 +328,40 @@ __device__ inline uint32_t link_first(uint32_t l) { return l & 0x03FFFFFFu; }
 __device__ inline uint32_t link_count(uint32_t l) { return ((l >> 26) & 31u) + 1u; }
 
 // (link, distance) stack: kLds entries per lane in LDS (two lane-major
-// planes), deeper ones spilled to 2 words per entry of a per-lane area.
+// planes, a ring), deeper ones spilled to 2 words per entry of a per-lane
+// area.  Entries [base, sp) are in the ring at slots top-1, ..., bot
+// (mod kLds); [0, base) are spilled.  The slot indices are kept
+// incrementally (no division by kLds).
 template <int kLds, int kBlock>
 struct Stack4 {
   uint32_t* lds;    // this lane's word of plane 0; plane 1 starts kLds * kBlock words later
   uint32_t* spill;  // 2 words per entry
   uint32_t sp, base;
-  __device__ inline void reset() { sp = base = 0; }
+  uint32_t top, bot;  // sp % kLds, base % kLds
+  __device__ inline void reset() { sp = base = top = bot = 0; }
   __device__ inline bool empty() const { return sp == 0; }
   __device__ inline void push(uint32_t link, float dist) {
     if (sp - base == kLds) {
-      const uint32_t slot = (base % kLds) * kBlock;
-      spill[2 * base] = lds[slot];
-      spill[2 * base + 1] = lds[kLds * kBlock + slot];
+      spill[2 * base] = lds[bot * kBlock];
+      spill[2 * base + 1] = lds[(kLds + bot) * kBlock];
       ++base;
+      bot = bot + 1 == kLds ? 0 : bot + 1;
     }
-    const uint32_t slot = (sp % kLds) * kBlock;
-    lds[slot] = link;
-    lds[kLds * kBlock + slot] = __float_as_uint(dist);
+    lds[top * kBlock] = link;
+    lds[(kLds + top) * kBlock] = __float_as_uint(dist);
     ++sp;
+    top = top + 1 == kLds ? 0 : top + 1;
   }
   __device__ inline void pop(uint32_t& link, float& dist) {
     --sp;
+    top = top == 0 ? kLds - 1 : top - 1;
     if (sp >= base) {
-      const uint32_t slot = (sp % kLds) * kBlock;
-      link = lds[slot];
-      dist = __uint_as_float(lds[kLds * kBlock + slot]);
+      link = lds[top * kBlock];
+      dist = __uint_as_float(lds[(kLds + top) * kBlock]);
       return;
     }
     --base;
+    bot = top;
     link = spill[2 * sp];
     dist = __uint_as_float(spill[2 * sp + 1]);
   }
@@ -367,7 +374,12 @@ using PtStack = Stack4<kStack4Lds, kBlock>;
 // Spill words per lane for a scene's 4-wide BVH.
 __host__ __device__ inline uint32_t pt_spill_words(const DevBvh& b) { return 2u * b.stack4; }
 
-// Entry distances of the four children of 4-wide node n (INF: no child or missed).
+// Entry distances of the four children of 4-wide node n (INF: no child or
+// missed).  A child's slab distance (origin + code * scale - o) * inv is one
+// multiply-add per plane, code * (scale * inv) + (origin - o) * inv, and each
+// child box is widened by 1/16 of a quantisation step on both sides, which
+// covers the rounding of that split form (a few ulps of the terms) with room
+// to spare: no box that the ray touches is missed.
 __device__ inline void node4_hits(const float* nodes, uint32_t n, v3 o, v3 inv, float tmax, float tn[4], uint32_t link[4]) {
   const float4* r = reinterpret_cast<const float4*>(nodes) + size_t(n) * 4;
   const float4 A = r[0], Bq = r[1], Cq = r[2], Lf = r[3];
@@ -379,15 +391,17 @@ __device__ inline void node4_hits(const float* nodes, uint32_t n, v3 o, v3 inv, 
   link[1] = __float_as_uint(Lf.y);
   link[2] = __float_as_uint(Lf.z);
   link[3] = __float_as_uint(Lf.w);
+  const float kx = sx * inv.x, ky = sy * inv.y, kz = sz * inv.z;
+  const float dx = A.x - o.x, dy = A.y - o.y, dz = A.z - o.z;
+  const float lox = fmaf(sx, -0.0625f, dx) * inv.x, hix = fmaf(sx, 0.0625f, dx) * inv.x;
+  const float loy = fmaf(sy, -0.0625f, dy) * inv.y, hiy = fmaf(sy, 0.0625f, dy) * inv.y;
+  const float loz = fmaf(sz, -0.0625f, dz) * inv.z, hiz = fmaf(sz, 0.0625f, dz) * inv.z;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int sh = 8 * i;
-    const float bx0 = fmaf(float((lx >> sh) & 255u), sx, A.x), bx1 = fmaf(float((hx >> sh) & 255u), sx, A.x);
-    const float by0 = fmaf(float((ly >> sh) & 255u), sy, A.y), by1 = fmaf(float((hy >> sh) & 255u), sy, A.y);
-    const float bz0 = fmaf(float((lz >> sh) & 255u), sz, A.z), bz1 = fmaf(float((hz >> sh) & 255u), sz, A.z);
-    const float x0 = (bx0 - o.x) * inv.x, x1 = (bx1 - o.x) * inv.x;
-    const float y0 = (by0 - o.y) * inv.y, y1 = (by1 - o.y) * inv.y;
-    const float z0 = (bz0 - o.z) * inv.z, z1 = (bz1 - o.z) * inv.z;
+    const float x0 = fmaf(float((lx >> sh) & 255u), kx, lox), x1 = fmaf(float((hx >> sh) & 255u), kx, hix);
+    const float y0 = fmaf(float((ly >> sh) & 255u), ky, loy), y1 = fmaf(float((hy >> sh) & 255u), ky, hiy);
+    const float z0 = fmaf(float((lz >> sh) & 255u), kz, loz), z1 = fmaf(float((hz >> sh) & 255u), kz, hiz);
     const float nn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
     const float ff = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
     tn[i] = (link[i] != kLinkNone && ff >= nn) ? nn : INF;
@@ -449,27 +463,38 @@ __device__ inline bool trav4_step(const PtScene& s, Trav4& t, StackT& st, Rng& r
   float tn[4];
   uint32_t l[4];
   node4_hits(s.bvh.nodes4, link, t.ro, t.inv, t.tmax, tn, l);
-  if (!kAny) {
-    // Farthest first: the nearest child ends on top.
-    auto cs = [&](int a, int b) {
-      if (tn[a] < tn[b]) {
-        const float td = tn[a];
-        tn[a] = tn[b];
-        tn[b] = td;
-        const uint32_t ld = l[a];
-        l[a] = l[b];
-        l[b] = ld;
-      }
-    };
-    cs(0, 1);
-    cs(2, 3);
-    cs(0, 2);
-    cs(1, 3);
-    cs(1, 2);
-  }
+  if (kAny) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-    if (tn[i] != INF) st.push(l[i], tn[i]);
+    for (int i = 0; i < 4; ++i)
+      if (tn[i] != INF) st.push(l[i], tn[i]);
+    return false;
+  }
+  // Farthest first, so that the nearest child ends on top.  Entry distances
+  // are >= 0, so their bit patterns order like the floats: the keys carry
+  // the child index in the two low bits (the distance truncated by up to 3
+  // ulps, which only weakens culling) and sort with integer min/max.
+  uint32_t k[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) k[i] = (__float_as_uint(tn[i]) & ~3u) | uint32_t(i);
+  auto cx = [&](int a, int b) {
+    const uint32_t lo = min(k[a], k[b]), hi = max(k[a], k[b]);
+    k[a] = hi;
+    k[b] = lo;
+  };
+  cx(0, 1);
+  cx(2, 3);
+  cx(0, 2);
+  cx(1, 3);
+  cx(1, 2);
+  constexpr uint32_t kMissKey = 0x7149F2CAu & ~3u;  // INF's bits, truncated
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if ((k[i] & ~3u) != kMissKey) {
+      const uint32_t j = k[i] & 3u;
+      const uint32_t lk = (j & 1u) ? ((j & 2u) ? l[3] : l[1]) : ((j & 2u) ? l[2] : l[0]);
+      st.push(lk, __uint_as_float(k[i] & ~3u));
+    }
+  }
   return false;
 }
 
