@@ -35,6 +35,13 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
+# The path-tracing translation units evaluate trace.frag's sin/cos/log with
+# the hardware approximations (GLSL leaves their precision to the
+# implementation).  All three share the flag, so the same path still rounds
+# identically in every kernel.  (Relaxed fp32 division is not used: its
+# lowering depends on the surrounding code, which breaks that identity.)
+PT_SOURCES = {"pathtrace.hip", "wavefront.hip", "adjoint.hip"}
+PT_FLAGS = ["-fgpu-approx-transcendentals"]
 CXX_FLAGS = [
     "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
     "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", "-Wall", "-Wno-unknown-pragmas",
@@ -55,7 +62,8 @@ def _run(cmd: list[str]) -> None:
 def _compile(src: Path) -> Path:
     obj = BUILD_DIR / (src.name + ".o")
     if src.suffix == ".hip":
-        cmd = [_hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+        extra = PT_FLAGS if src.name in PT_SOURCES else []
+        cmd = [_hipcc(), *HIPCC_FLAGS, *extra, "-c", str(src), "-o", str(obj)]
     else:
         cmd = ["g++", *CXX_FLAGS, "-c", str(src), "-o", str(obj)]
     _run(cmd)

@@ -119,7 +119,7 @@ __device__ __forceinline__ bool eval_opaque_grad(const PtScene& sc, const Surfac
   const v3 H = normalize(wo + wi);
   const float ndh = fmaxf(dot(n, H), 0.0f);
   const float vdh = fmaxf(dot(wo, H), 0.0f);
-  const float fres = powf(1.0f - clampf(vdh, 0.0f, 1.0f), 5.0f);
+  const float fres = pow5(1.0f - clampf(vdh, 0.0f, 1.0f));
   const float d0 = dielectric_f0(s.ior);
   const D5 a2 = a * a;
   const D5 dd = (a2 + (-1.0f)) * (ndh * ndh) + 1.0f;

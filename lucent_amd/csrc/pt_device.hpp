@@ -61,6 +61,9 @@ __device__ inline v3 refract(v3 i, v3 n, float eta) {
   return i * eta - n * (eta * d + sqrtf(k));
 }
 __device__ inline v3 vexp(v3 a) { return {expf(a.x), expf(a.y), expf(a.z)}; }
+// GLSL pow(x, y) for x >= 0 as the specification defines it, exp2(y * log2(x)),
+// on the hardware exp2/log2 (pow(0, y > 0) = exp2(-inf) = 0).
+__device__ inline float gpow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
 __device__ inline float luminance(v3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
 
 // ---------------------------------------------------------------------------
@@ -139,7 +142,7 @@ __device__ inline v3 offset_ray(v3 p, v3 n) { return mk(offset_axis(p.x, n.x), o
 // Textures: bilinear RGBA8 arrays, REPEAT, level 0 (context.ts:238-242).
 // ---------------------------------------------------------------------------
 __device__ inline float srgb_to_linear(float c) {
-  return c <= 0.04045f ? c * (1.0f / 12.92f) : powf((c + 0.055f) * (1.0f / 1.055f), 2.4f);
+  return c <= 0.04045f ? c * (1.0f / 12.92f) : gpow((c + 0.055f) * (1.0f / 1.055f), 2.4f);
 }
 __device__ inline float4 texel_rgba8(const uint32_t* maps, int size, int layer, int x, int y, bool srgb) {
   x = ((x % size) + size) % size;
@@ -839,7 +842,11 @@ __device__ inline float g1_ggx(float ndx, float a) {
   const float a2 = a * a;
   return 2.0f * ndx / (ndx + sqrtf(a2 + (1.0f - a2) * ndx * ndx));
 }
-__device__ inline float pow5(float x) { return powf(x, 5.0f); }
+// GLSL pow(x, 5.0): products (GLSL leaves pow's precision to the implementation).
+__device__ inline float pow5(float x) {
+  const float x2 = x * x;
+  return x2 * x2 * x;
+}
 __device__ inline v3 fresnel_schlick3(float c, v3 f0) {
   const float f = pow5(1.0f - clampf(c, 0.0f, 1.0f));
   return f0 + (mk(1.f, 1.f, 1.f) - f0) * f;
@@ -867,7 +874,7 @@ __device__ inline float alpha_from_roughness(float r) { return fmaxf(r * r, 1e-3
 __device__ inline float d_charlie(float alpha, float ndh) {
   const float inv = 1.0f / alpha;
   const float sin2 = fmaxf(1.0f - ndh * ndh, 0.0f);
-  return (2.0f + inv) * powf(sin2, inv * 0.5f) / TWO_PI;
+  return (2.0f + inv) * gpow(sin2, inv * 0.5f) / TWO_PI;
 }
 __device__ inline float v_neubelt(float ndl, float ndv) { return 1.0f / (4.0f * (ndl + ndv - ndl * ndv)); }
 // Bilinear R16F LUT, CLAMP_TO_EDGE (renderer.ts:241-242, bsdf.glsl:247-250).
