@@ -115,6 +115,18 @@ typedef struct lucent_packed_scene {
 
 lucent_status lucent_scene_upload(lucent_ctx *ctx, const lucent_packed_scene *packed, lucent_scene **out);
 
+/* The device tree of scenes uploaded afterwards on this context.  The
+ * caller's BVH (bvh.ts:6-16 layout) is always validated; unless max_leaf is
+ * 0 the device tree is then rebuilt from the same triangles with a binned
+ * SAH of `bins` bins and leaves of at most max_leaf triangles (default 2
+ * and 32: fewer node visits and triangle tests than buildBVH's 4 and 12 for
+ * every traversal here), and hit records keep their meaning: original
+ * triangle ids, and exact queries still resolve ties to the lowest slot of
+ * the caller's tree.  max_leaf = 0 keeps the caller's tree as the device
+ * tree.  No counterpart in the reference (its shaders walk the tree they
+ * are given). */
+lucent_status lucent_set_tree_build(lucent_ctx *ctx, uint32_t max_leaf, uint32_t bins);
+
 /* Convenience form for traversal only: the BVH of buildBVH plus the positions
  * it was built over (the arguments of intersectBVH, src/math/bvh.ts:265-271);
  * triangles are gathered into leaf order on upload, as packScene does. */
@@ -126,9 +138,9 @@ void lucent_scene_destroy(lucent_scene *scene);
 
 typedef struct lucent_scene_stats {
   uint32_t tri_count;
-  uint32_t node_count;  /* reference nodes */
+  uint32_t node_count;  /* nodes of the caller's tree */
   uint32_t node4_count; /* 64-byte 4-wide nodes on the device */
-  uint32_t depth;       /* deepest leaf, root = 0 */
+  uint32_t depth;       /* deepest leaf of the device's binary tree, root = 0 */
   uint64_t device_bytes;
 } lucent_scene_stats;
 lucent_status lucent_scene_stats_get(const lucent_scene *scene, lucent_scene_stats *out);

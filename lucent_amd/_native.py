@@ -159,6 +159,7 @@ SIGNATURES: dict[str, tuple] = {
                                          _P, C.POINTER(_P)]),
     "lucent_frame_graph_launch": (_I32, [_P, _P]),
     "lucent_decode_hdr": (_I32, [_P, C.c_size_t, C.POINTER(_U32), C.POINTER(_U32), _P, C.c_size_t]),
+    "lucent_set_tree_build": (_I32, [_P, _U32, _U32]),
     "lucent_frame_graph_destroy": (None, [_P]),
     "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,
                                 _U32, _P, _P, _P, _P, _P]),

@@ -43,6 +43,10 @@ struct DevBvh {
   const float* tris = nullptr;    // float4[3 * tri_count]
   const float* attrs = nullptr;   // float4[4 * tri_count] or null
   const uint32_t* tri_ids = nullptr;
+  // slot -> the caller's slot (position in its triIndices) when the device
+  // tree was rebuilt, or null when the slots are the caller's; exact queries
+  // break ties by it, as the caller's left-first traversal does.
+  const uint32_t* ref_slots = nullptr;
   uint32_t tri_count = 0;
   uint32_t depth = 0;       // binary tree depth (deepest leaf, root = 0)
   // 4-wide nodes of the path tracer: 16 floats per node (frame origin, scale,
@@ -74,6 +78,9 @@ struct Context {
   size_t wavefront_bytes = 0;
   uint32_t* d_flags = nullptr;  // device status word(s)
   float* sheen_lut = nullptr;   // 16x16 half-rounded sheen albedo table
+  // Device tree built at scene upload (binned SAH over the caller's
+  // triangles); tree_leaf = 0 keeps the caller's tree as the device tree.
+  uint32_t tree_leaf = 2, tree_bins = 32;
 };
 
 // Host-side checks and conversion of the reference layout into the device layout.

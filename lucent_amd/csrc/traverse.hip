@@ -192,29 +192,33 @@ __device__ inline bool mt_glsl(const float ro[3], const float rd[3], const float
 
 struct Best {
   double t, u, v;
-  uint32_t slot;
+  uint32_t slot;  // device slot
+  uint32_t key;   // the caller's slot of it: ties go to the lowest
 };
 
 template <Prec kPrec, bool kAny>
-__device__ inline bool test_leaf(const float4* tris, uint32_t first, uint32_t count, const double rod[3],
-                                 const double rdd[3], const float rof[3], const float rdf[3], double tmin,
-                                 double tmax, Best& best) {
+__device__ inline bool test_leaf(const float4* tris, const uint32_t* ref_slots, uint32_t first, uint32_t count,
+                                 const double rod[3], const double rdd[3], const float rof[3], const float rdf[3],
+                                 double tmin, double tmax, Best& best) {
   for (uint32_t s = first; s < first + count; ++s) {
     const float4 a = tris[size_t(s) * 3], b = tris[size_t(s) * 3 + 1], c = tris[size_t(s) * 3 + 2];
     if (kPrec == Prec::kExact) {
       double t, u, v;
       if (!mt_exact(rod, rdd, a, b, c, t, u, v)) continue;
       if (!(t > tmin && t < tmax)) continue;
-      if (t < best.t || (t == best.t && s < best.slot)) {
-        best = Best{t, u, v, s};
-        if (kAny) return true;
+      if (t <= best.t) {
+        const uint32_t key = ref_slots ? ref_slots[s] : s;
+        if (t < best.t || key < best.key) {
+          best = Best{t, u, v, s, key};
+          if (kAny) return true;
+        }
       }
     } else {
       float t, u, v;
       if (!mt_glsl(rof, rdf, a, b, c, t, u, v)) continue;
       if (!(t > kGlslEps && double(t) < tmax)) continue;
       if (double(t) < best.t) {
-        best = Best{double(t), double(u), double(v), s};
+        best = Best{double(t), double(u), double(v), s, s};
         if (kAny) return true;
       }
     }
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_intersect(DevBvh bvh, const void* __
       invd[2] = 1.0 / rdd[2];
     }
     const double lower = kPrec == Prec::kExact ? tmin : double(kGlslEps);
-    Best best{tmax, 0.0, 0.0, 0xFFFFFFFFu};
+    Best best{tmax, 0.0, 0.0, 0xFFFFFFFFu, 0xFFFFFFFFu};
 
     LaneStack<kLds> st{lds_stack + threadIdx.x, spill + size_t(lane_global) * spill_depth};
     if (bvh.tri_count > 0) st.push(bvh.root4, 0.0f);
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_intersect(DevBvh bvh, const void* __
       if (!kAny && double(dist) > best.t) continue;
       if (link & kLinkLeaf) {
         const uint32_t first = link & 0x03FFFFFFu, count = ((link >> 26) & 31u) + 1u;
-        if (test_leaf<kPrec, kAny>(tris, first, count, rod, rdd, rof, rdf, lower, tmax, best)) break;
+        if (test_leaf<kPrec, kAny>(tris, bvh.ref_slots, first, count, rod, rdd, rof, rdf, lower, tmax, best)) break;
         continue;
       }
       Node4 nd;

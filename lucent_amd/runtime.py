@@ -43,6 +43,11 @@ class Context:
     def synchronize(self) -> None:
         _native.check(_native.lib().lucent_synchronize(self.handle))
 
+    def set_tree_build(self, max_leaf: int = 2, bins: int = 32) -> None:
+        """lucent_set_tree_build: the device tree of scenes uploaded from now
+        on (max_leaf 0 keeps the caller's BVH as the device tree)."""
+        _native.check(_native.lib().lucent_set_tree_build(self.handle, max_leaf, bins))
+
     def close(self) -> None:
         if self._h is not None:
             _native.lib().lucent_destroy(self._h)

@@ -95,6 +95,30 @@ def test_degenerate_and_tiny_trees(ctx):
     assert gpu.tri.tolist() == [0, MISS] and gpu.t[1] == math.inf
 
 
+def test_rebuilt_device_tree_keeps_hit_records(ctx):
+    """The device tree rebuilt at upload (default) and the caller's own tree
+    give bit-identical exact hit records, ties included (64 coincident
+    triangles: the caller's lowest slot wins either way)."""
+    soup, org, dirs = reference_soup_case()
+    b = build_bvh(soup)
+    one = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0] * 64, np.float32)
+    b1 = build_bvh(one)
+    res = {}
+    try:
+        for leaf in (0, 2, 1):
+            ctx.set_tree_build(leaf, 32)
+            res[leaf] = (ctx.scene_from_bvh(soup, b).intersect(org, dirs),
+                         ctx.scene_from_bvh(one, b1).intersect(np.array([[0.2, 0.2, -1.0], [0.1, 0.3, 2.0]]),
+                                                               np.array([[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]])))
+    finally:
+        ctx.set_tree_build(2, 32)
+    for leaf in (2, 1):
+        for got, ref in zip(res[leaf], res[0]):
+            for f in ("t", "tri", "u", "v"):
+                assert np.array_equal(getattr(got, f), getattr(ref, f)), (leaf, f)
+    assert res[2][1].tri.tolist() == [b1.tri_indices[0]] * 2
+
+
 def test_empty_scene_misses(ctx):
     empty = np.zeros(0, np.float32)
     b = build_bvh(empty)
