@@ -180,6 +180,10 @@ Status finish_scene(Context* ctx, Scene* sc, const float* nodes, uint32_t node_c
   if (img4.node_count > 0 &&
       (s = upload_owned(sc, img4.nodes.data(), img4.nodes.size() * sizeof(float), &sc->bvh.nodes4)) != LUCENT_OK)
     return s;
+  if (img4.node_count > 0 &&
+      (s = upload_owned(sc, img4.nodes_f32.data(), img4.nodes_f32.size() * sizeof(float), &sc->bvh.nodes4f)) !=
+          LUCENT_OK)
+    return s;
   if ((s = upload_owned(sc, t.data(), t.size() * sizeof(float), &sc->bvh.tris)) != LUCENT_OK) return s;
   if (a && (s = upload_owned(sc, a, size_t(n) * 16 * sizeof(float), &sc->bvh.attrs)) != LUCENT_OK) return s;
   const float* idp = nullptr;

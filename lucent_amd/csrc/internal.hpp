@@ -53,6 +53,7 @@ struct DevBvh {
   // 8-bit child codes per axis, 4 links); a link is a node index, a leaf
   // (bit 31 | (count - 1) << 26 | first slot) or 0xFFFFFFFF for no child.
   const float* nodes4 = nullptr;
+  const float* nodes4f = nullptr;  // the same nodes, float32 child bounds (32 floats per node)
   uint32_t node4_count = 0;
   uint32_t root4 = 0xFFFFFFFFu;   // link of the root (a node or a leaf)
   uint32_t stack4 = 1;            // (link, distance) entries a 4-wide traversal may need
@@ -88,6 +89,11 @@ struct Context {
 Status validate_bvh(const float* nodes, uint32_t node_count, uint32_t tri_count, uint32_t* depth);
 struct HostBvh4Image {
   std::vector<float> nodes;  // 16 floats per node
+  // The same nodes with float32 child bounds, 32 floats per node: the
+  // frame origin (xyz, 0), then per axis the four children's lower and
+  // upper offsets from the origin (rounded outward and widened by 2^-12 of
+  // the node's extent), then the four links.
+  std::vector<float> nodes_f32;
   uint32_t node_count = 0;
   uint32_t root = 0xFFFFFFFFu;
   uint32_t depth = 0;        // levels of 4-wide nodes

@@ -411,6 +411,32 @@ __device__ inline void node4_hits(const float* nodes, uint32_t n, v3 o, v3 inv, 
   }
 }
 
+// The same test on the float32 records (DevBvh::nodes4f): child bounds are
+// offsets from the node origin, so a plane is one multiply-add onto the
+// per-node (origin - o) * inv, with no decoding.
+__device__ inline void node4_hits_f32(const float* nodes, uint32_t n, v3 o, v3 inv, float tmax, float tn[4],
+                                      uint32_t link[4]) {
+  const float4* r = reinterpret_cast<const float4*>(nodes) + size_t(n) * 8;
+  const float4 A = r[0], lx = r[1], hx = r[2], ly = r[3], hy = r[4], lz = r[5], hz = r[6], L = r[7];
+  link[0] = __float_as_uint(L.x);
+  link[1] = __float_as_uint(L.y);
+  link[2] = __float_as_uint(L.z);
+  link[3] = __float_as_uint(L.w);
+  const float bx = (A.x - o.x) * inv.x, by = (A.y - o.y) * inv.y, bz = (A.z - o.z) * inv.z;
+  const float LX[4] = {lx.x, lx.y, lx.z, lx.w}, HX[4] = {hx.x, hx.y, hx.z, hx.w};
+  const float LY[4] = {ly.x, ly.y, ly.z, ly.w}, HY[4] = {hy.x, hy.y, hy.z, hy.w};
+  const float LZ[4] = {lz.x, lz.y, lz.z, lz.w}, HZ[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float x0 = fmaf(LX[i], inv.x, bx), x1 = fmaf(HX[i], inv.x, bx);
+    const float y0 = fmaf(LY[i], inv.y, by), y1 = fmaf(HY[i], inv.y, by);
+    const float z0 = fmaf(LZ[i], inv.z, bz), z1 = fmaf(HZ[i], inv.z, bz);
+    const float nn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+    const float ff = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    tn[i] = (link[i] != kLinkNone && ff >= nn) ? nn : INF;
+  }
+}
+
 struct Trav4 {
   v3 ro, rd, inv;
   float tmax;  // closest hit so far (closest queries) or the ray's extent (shadow rays)
@@ -465,7 +491,11 @@ __device__ inline bool trav4_step(const PtScene& s, Trav4& t, StackT& st, Rng& r
   }
   float tn[4];
   uint32_t l[4];
+#ifdef LUCENT_PT_QUANTIZED_NODES
   node4_hits(s.bvh.nodes4, link, t.ro, t.inv, t.tmax, tn, l);
+#else
+  node4_hits_f32(s.bvh.nodes4f, link, t.ro, t.inv, t.tmax, tn, l);
+#endif
   if (kAny) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)

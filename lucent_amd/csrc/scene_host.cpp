@@ -180,8 +180,33 @@ Status convert_bvh4(const float* nodes, uint32_t node_count, uint32_t tri_count,
     std::memcpy(r + 10, &qlo[2], 4);
     std::memcpy(r + 11, &qhi[2], 4);
     std::memcpy(r + 12, links, sizeof(links));
+    if (out->nodes_f32.size() < size_t(out->node_count) * 32)
+      out->nodes_f32.resize(size_t(out->node_count) * 32 * 2, 0.0f);
+    float* f = out->nodes_f32.data() + size_t(it.node) * 32;
+    f[0] = lo[0];
+    f[1] = lo[1];
+    f[2] = lo[2];
+    f[3] = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+      const double margin = (double(hi[a]) - double(lo[a])) * 0x1p-12;
+      for (int i = 0; i < 4; ++i) {
+        float ol = 0.0f, oh = 0.0f;
+        if (i < n) {
+          const double dl = double(nodes[size_t(ch[i]) * 8 + a]) - lo[a] - margin;
+          const double dh = double(nodes[size_t(ch[i]) * 8 + 4 + a]) - lo[a] + margin;
+          ol = float(dl);
+          if (double(ol) > dl) ol = std::nextafter(ol, -INFINITY);
+          oh = float(dh);
+          if (double(oh) < dh) oh = std::nextafter(oh, INFINITY);
+        }
+        f[4 + 8 * a + i] = ol;
+        f[8 + 8 * a + i] = oh;
+      }
+    }
+    std::memcpy(f + 28, links, sizeof(links));
   }
   out->nodes.resize(size_t(out->node_count) * 16);
+  out->nodes_f32.resize(size_t(out->node_count) * 32);
   out->root = 0;
   out->depth = max_depth;
   return LUCENT_OK;
