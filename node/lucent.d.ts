@@ -20,6 +20,7 @@ export declare class GpuTraversal {
 }
 export interface Camera { position: number[]; target: number[]; fov: number; aperture: number; focusDistance: number; }
 export interface FilmTargets { color: Float32Array; albedo: Float32Array; normal: Float32Array; history: Float32Array; }
+export declare function buildBVHAsync(positions: Float32Array, options?: BVHOptions): Promise<BVH>;
 export declare function parseHDR(buffer: ArrayBuffer | Uint8Array): { width: number; height: number; data: Float32Array };
 
 export declare class LucentRenderer {
@@ -31,6 +32,9 @@ export declare class LucentRenderer {
   uploadCamera(cam: Camera, moving?: boolean): void;
   reset(): void;
   traceSamples(count: number): void;
+  traceSamplesAsync(count: number): Promise<void>;
+  runDenoiserAsync(params?: { sigmaL: number; sigmaN: number; sigmaP: number; iterations: number }):
+    Promise<{ illumVar: Float32Array; radiance: Float32Array }>;
   render(samples?: number): boolean;
   screenshot(viewWidth?: number, viewHeight?: number): { width: number; height: number; data: Uint8Array };
   benchmark(samples: number): { traceMs: number; mpathsPerSec: number; denoiseMs: number; width: number; height: number };

@@ -5,6 +5,7 @@
  * Node 12).  It mirrors the reference's TypeScript entry points on this
  * path so that a caller swaps implementations, not call sites:
  *   buildBVH(positions, options)          src/math/bvh.ts:52
+ *   buildBVHAsync(positions, options)     src/math/bvh-async.ts:8
  *   parseHDR(buffer)                      src/loaders/hdr.ts:7
  *   GpuTraversal#intersect / pickDistance src/math/bvh.ts:265, src/scene/scene.ts:327
  *   LucentRenderer                        src/renderer.ts:171 (loadScene, uploadLighting,
@@ -34,6 +35,20 @@ const MISS = 0xffffffff;
  */
 function buildBVH(positions, options) {
   return addon.buildBVH(positions, options || {});
+}
+
+/**
+ * buildBVHAsync (src/math/bvh-async.ts:8-30): the same build on a worker
+ * thread; the Promise rejects with the same errors.
+ * @param {Float32Array} positions 9 floats per triangle
+ * @param {{maxLeafSize?: number, bins?: number}} [options]
+ */
+function buildBVHAsync(positions, options) {
+  try {
+    return addon.buildBVHAsync(positions, options || {});
+  } catch (e) {
+    return Promise.reject(e);
+  }
 }
 
 /**
@@ -118,6 +133,20 @@ class LucentRenderer {
     this.historyCamera = null;
     this.reprojectFrom = null;
     this.denoisedFrame = -1;
+    /** Asynchronous calls run one after another (a context is not re-entrant). */
+    this._queue = Promise.resolve();
+    this._inflight = 0;
+  }
+
+  _sync() {
+    if (this._inflight > 0) throw new Error('an asynchronous call is still running on this renderer');
+  }
+
+  _enqueue(start) {
+    this._inflight++;
+    const run = this._queue.then(start);
+    this._queue = run.then(() => undefined, () => undefined);
+    return run.then((v) => { this._inflight--; return v; }, (e) => { this._inflight--; throw e; });
   }
 
   /**
@@ -166,6 +195,7 @@ class LucentRenderer {
 
   /** traceSamples (renderer.ts:702): `count` progressive samples, the first reprojecting after a camera move. */
   traceSamples(count) {
+    this._sync();
     if (count <= 0) return;
     if (this.reprojectFrom) {
       addon.traceReprojected(this.ctx, this.scene, this.film, this.camera, this.film, this.reprojectFrom, HISTORY_CAP,
@@ -222,7 +252,34 @@ class LucentRenderer {
 
   /** runDenoiser (renderer.ts:788): sigmaL 4, sigmaN 64, sigmaP 1.5 by default. */
   runDenoiser(params) {
+    this._sync();
     return addon.denoise(this.ctx, this.film, this.camera, params || { sigmaL: 4, sigmaN: 64, sigmaP: 1.5, iterations: 0 });
+  }
+
+  /**
+   * traceSamples on a worker thread: resolves once the samples are in the
+   * film.  Calls queue behind one another.
+   * @param {number} count
+   */
+  traceSamplesAsync(count) {
+    return this._enqueue(() => {
+      if (count <= 0) return undefined;
+      if (this.reprojectFrom) {
+        addon.traceReprojected(this.ctx, this.scene, this.film, this.camera, this.film, this.reprojectFrom, HISTORY_CAP,
+          this.settings);
+        this.reprojectFrom = null;
+        count -= 1;
+      }
+      this.historyCamera = this.camera;
+      return count > 0 ? addon.traceSamplesAsync(this.ctx, this.scene, this.film, this.camera, this.settings, count)
+        : undefined;
+    });
+  }
+
+  /** runDenoiser on a worker thread: resolves to {illumVar, radiance}. */
+  runDenoiserAsync(params) {
+    return this._enqueue(() => addon.denoiseAsync(this.ctx, this.film, this.camera,
+      params || { sigmaL: 4, sigmaN: 64, sigmaP: 1.5, iterations: 0 }));
   }
 
   /** The accumulation targets (rows bottom to top, like readPixels). */
@@ -260,4 +317,4 @@ class LucentRenderer {
   }
 }
 
-module.exports = { buildBVH, parseHDR, GpuTraversal, LucentRenderer, cameraBasis, version: addon.version, addon, MISS };
+module.exports = { buildBVH, buildBVHAsync, parseHDR, GpuTraversal, LucentRenderer, cameraBasis, version: addon.version, addon, MISS };

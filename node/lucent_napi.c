@@ -130,6 +130,30 @@ static napi_value js_version(napi_env env, napi_callback_info info) {
 }
 
 /* buildBVH(positions: Float32Array, {maxLeafSize, bins}) — src/math/bvh.ts:52. */
+/* {nodes: Float32Array, nodeCount, triIndices, bounds: {min, max}} of buildBVH (bvh.ts:16-24). */
+static napi_value bvh_result(napi_env env, const float *nodes, uint32_t count, napi_value idx, const double bounds[6]) {
+  void *nd;
+  napi_value nodes_ta = new_typed(env, napi_float32_array, (size_t)count * 8, 4, &nd);
+  if (!nodes_ta) return NULL;
+  memcpy(nd, nodes, (size_t)count * 8 * sizeof(float));
+  napi_value out, bmin, bmax, b;
+  CHECK_NAPI(napi_create_object(env, &out));
+  CHECK_NAPI(napi_set_named_property(env, out, "nodes", nodes_ta));
+  CHECK_NAPI(napi_set_named_property(env, out, "nodeCount", make_number(env, count)));
+  CHECK_NAPI(napi_set_named_property(env, out, "triIndices", idx));
+  CHECK_NAPI(napi_create_array_with_length(env, 3, &bmin));
+  CHECK_NAPI(napi_create_array_with_length(env, 3, &bmax));
+  for (int i = 0; i < 3; i++) {
+    napi_set_element(env, bmin, i, make_number(env, bounds[i]));
+    napi_set_element(env, bmax, i, make_number(env, bounds[3 + i]));
+  }
+  CHECK_NAPI(napi_create_object(env, &b));
+  CHECK_NAPI(napi_set_named_property(env, b, "min", bmin));
+  CHECK_NAPI(napi_set_named_property(env, b, "max", bmax));
+  CHECK_NAPI(napi_set_named_property(env, out, "bounds", b));
+  return out;
+}
+
 static napi_value js_build_bvh(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -164,25 +188,8 @@ static napi_value js_build_bvh(napi_env env, napi_callback_info info) {
     free(nodes);
     return throw_status(env, st);
   }
-  void *nd;
-  napi_value nodes_ta = new_typed(env, napi_float32_array, (size_t)count * 8, 4, &nd);
-  memcpy(nd, nodes, (size_t)count * 8 * sizeof(float));
+  napi_value out = bvh_result(env, nodes, count, idx, bounds);
   free(nodes);
-  napi_value out, bmin, bmax, b;
-  CHECK_NAPI(napi_create_object(env, &out));
-  CHECK_NAPI(napi_set_named_property(env, out, "nodes", nodes_ta));
-  CHECK_NAPI(napi_set_named_property(env, out, "nodeCount", make_number(env, count)));
-  CHECK_NAPI(napi_set_named_property(env, out, "triIndices", idx));
-  CHECK_NAPI(napi_create_array_with_length(env, 3, &bmin));
-  CHECK_NAPI(napi_create_array_with_length(env, 3, &bmax));
-  for (int i = 0; i < 3; i++) {
-    napi_set_element(env, bmin, i, make_number(env, bounds[i]));
-    napi_set_element(env, bmax, i, make_number(env, bounds[3 + i]));
-  }
-  CHECK_NAPI(napi_create_object(env, &b));
-  CHECK_NAPI(napi_set_named_property(env, b, "min", bmin));
-  CHECK_NAPI(napi_set_named_property(env, b, "max", bmax));
-  CHECK_NAPI(napi_set_named_property(env, out, "bounds", b));
   return out;
 }
 
@@ -609,10 +616,247 @@ static napi_value js_denoise(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* ---- asynchronous calls --------------------------------------------------
+ * buildBVHAsync (bvh-async.ts:8-30), traceSamplesAsync and denoiseAsync run
+ * the C-ABI call on a libuv worker thread (napi_async_work) and settle a
+ * Promise; the arguments' JavaScript objects are referenced until then.
+ * A context is not re-entrant: the JS facade queues calls per renderer. */
+typedef enum { ASYNC_BVH, ASYNC_TRACE, ASYNC_DENOISE } async_kind;
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  async_kind kind;
+  napi_ref refs[4];
+  int nrefs;
+  lucent_status status;
+  char msg[512];
+  /* buildBVH */
+  const float *pos;
+  uint32_t tris, max_leaf, bins, count;
+  float *nodes;
+  uint32_t *idx;
+  double bounds[6];
+  /* trace / denoise */
+  lucent_ctx *ctx;
+  lucent_scene *scene;
+  lucent_film *film;
+  lucent_camera cam;
+  lucent_render_settings rs;
+  uint32_t samples;
+  lucent_denoise_params dp;
+  float *iv, *rad;
+  size_t px;
+} async_call;
+
+static void async_execute(napi_env env, void *data) {
+  async_call *a = (async_call *)data;
+  (void)env;
+  switch (a->kind) {
+    case ASYNC_BVH:
+      a->status = lucent_build_bvh(a->pos, a->tris, a->max_leaf, a->bins, a->nodes, &a->count, a->idx, a->bounds);
+      break;
+    case ASYNC_TRACE:
+      a->status = lucent_trace_samples(a->ctx, a->scene, a->film, &a->cam, &a->rs, a->samples, NULL);
+      if (a->status == LUCENT_OK) a->status = lucent_synchronize(a->ctx);
+      break;
+    case ASYNC_DENOISE:
+      a->status = lucent_denoise(a->ctx, a->film, &a->cam, &a->dp, NULL);
+      if (a->status == LUCENT_OK) a->status = lucent_film_read_denoised(a->ctx, a->film, a->iv, a->rad);
+      break;
+  }
+  /* The message is thread-local: read it on this thread. */
+  if (a->status != LUCENT_OK) snprintf(a->msg, sizeof a->msg, "%s", lucent_last_error(a->ctx));
+}
+
+static void async_complete(napi_env env, napi_status status, void *data) {
+  async_call *a = (async_call *)data;
+  if (a->status != LUCENT_OK || status != napi_ok) {
+    char code[32];
+    snprintf(code, sizeof code, "LUCENT_%d", (int)a->status);
+    napi_value c, m, err;
+    napi_create_string_utf8(env, code, NAPI_AUTO_LENGTH, &c);
+    napi_create_string_utf8(env, status != napi_ok ? "asynchronous call cancelled" : a->msg, NAPI_AUTO_LENGTH, &m);
+    napi_create_error(env, c, m, &err);
+    napi_reject_deferred(env, a->deferred, err);
+  } else {
+    napi_value result = NULL;
+    if (a->kind == ASYNC_BVH) {
+      void *idx_data;
+      napi_value idx = new_typed(env, napi_uint32_array, a->tris, 4, &idx_data);
+      if (idx) memcpy(idx_data, a->idx, (size_t)a->tris * sizeof(uint32_t));
+      result = idx ? bvh_result(env, a->nodes, a->count, idx, a->bounds) : NULL;
+    } else if (a->kind == ASYNC_DENOISE) {
+      void *iv, *rad;
+      napi_value ia = new_typed(env, napi_float32_array, a->px * 4, 4, &iv);
+      napi_value ra = new_typed(env, napi_float32_array, a->px * 4, 4, &rad);
+      if (ia && ra) {
+        memcpy(iv, a->iv, a->px * 16);
+        memcpy(rad, a->rad, a->px * 16);
+        napi_create_object(env, &result);
+        napi_set_named_property(env, result, "illumVar", ia);
+        napi_set_named_property(env, result, "radiance", ra);
+      }
+    } else {
+      napi_get_undefined(env, &result);
+    }
+    if (result) {
+      napi_resolve_deferred(env, a->deferred, result);
+    } else {
+      napi_value c, m, err;
+      napi_create_string_utf8(env, "LUCENT_ASYNC", NAPI_AUTO_LENGTH, &c);
+      napi_create_string_utf8(env, "could not build the result", NAPI_AUTO_LENGTH, &m);
+      napi_create_error(env, c, m, &err);
+      napi_reject_deferred(env, a->deferred, err);
+    }
+  }
+  for (int i = 0; i < a->nrefs; i++) napi_delete_reference(env, a->refs[i]);
+  free(a->nodes);
+  free(a->idx);
+  free(a->iv);
+  free(a->rad);
+  napi_delete_async_work(env, a->work);
+  free(a);
+}
+
+static napi_value start_async(napi_env env, async_call *a, napi_value *args, int nargs, const char *name) {
+  napi_value promise, res_name;
+  for (int i = 0; i < nargs; i++) {
+    if (napi_create_reference(env, args[i], 1, &a->refs[a->nrefs]) == napi_ok) a->nrefs++;
+  }
+  if (napi_create_promise(env, &a->deferred, &promise) != napi_ok ||
+      napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &res_name) != napi_ok ||
+      napi_create_async_work(env, NULL, res_name, async_execute, async_complete, a, &a->work) != napi_ok ||
+      napi_queue_async_work(env, a->work) != napi_ok) {
+    for (int i = 0; i < a->nrefs; i++) napi_delete_reference(env, a->refs[i]);
+    free(a->nodes);
+    free(a->idx);
+    free(a->iv);
+    free(a->rad);
+    free(a);
+    napi_throw_error(env, NULL, "could not start the asynchronous call");
+    return NULL;
+  }
+  return promise;
+}
+
+/* buildBVHAsync(positions, options) -> Promise<BVH> (bvh-async.ts:8-30). */
+static napi_value js_build_bvh_async(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  void *pos;
+  size_t len;
+  if (argc < 1 || !typed(env, argv[0], napi_float32_array, &pos, &len)) {
+    napi_throw_type_error(env, NULL, "buildBVHAsync: positions must be a Float32Array");
+    return NULL;
+  }
+  if (len % 9 != 0) {
+    char msg[96];
+    snprintf(msg, sizeof msg, "positions length %zu is not a multiple of 9", len);
+    napi_throw_error(env, NULL, msg);
+    return NULL;
+  }
+  async_call *a = (async_call *)calloc(1, sizeof(async_call));
+  if (!a) {
+    napi_throw_error(env, NULL, "out of memory");
+    return NULL;
+  }
+  a->kind = ASYNC_BVH;
+  a->pos = (const float *)pos;
+  a->tris = (uint32_t)(len / 9);
+  a->max_leaf = argc > 1 ? (uint32_t)num(env, prop(env, argv[1], "maxLeafSize"), 4) : 4;
+  a->bins = argc > 1 ? (uint32_t)num(env, prop(env, argv[1], "bins"), 12) : 12;
+  a->nodes = (float *)malloc((a->tris > 0 ? 2 * (size_t)a->tris - 1 : 1) * 8 * sizeof(float));
+  a->idx = (uint32_t *)malloc((a->tris > 0 ? a->tris : 1) * sizeof(uint32_t));
+  if (!a->nodes || !a->idx) {
+    free(a->nodes);
+    free(a->idx);
+    free(a);
+    napi_throw_error(env, NULL, "out of memory");
+    return NULL;
+  }
+  return start_async(env, a, argv, 1, "lucent.buildBVHAsync");
+}
+
+/* traceSamplesAsync(ctx, scene, film, camera, settings, count) -> Promise<void>. */
+static napi_value js_trace_samples_async(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  scene_holder *sh = argc > 1 ? (scene_holder *)get_external(env, argv[1]) : NULL;
+  film_holder *fh = argc > 2 ? (film_holder *)get_external(env, argv[2]) : NULL;
+  async_call *a = (async_call *)calloc(1, sizeof(async_call));
+  if (!a) {
+    napi_throw_error(env, NULL, "out of memory");
+    return NULL;
+  }
+  if (!ctx || !sh || !fh || argc < 6 || !read_camera(env, argv[3], &a->cam)) {
+    free(a);
+    napi_throw_type_error(env, NULL, "traceSamplesAsync(ctx, scene, film, camera, settings, count)");
+    return NULL;
+  }
+  a->kind = ASYNC_TRACE;
+  a->ctx = ctx;
+  a->scene = sh->scene;
+  a->film = fh->film;
+  read_settings(env, argv[4], &a->rs);
+  a->samples = (uint32_t)num(env, argv[5], 1);
+  return start_async(env, a, argv, 3, "lucent.traceSamplesAsync");
+}
+
+/* denoiseAsync(ctx, film, camera, params) -> Promise<{illumVar, radiance}>. */
+static napi_value js_denoise_async(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_ctx *ctx = argc > 0 ? get_ctx(env, argv[0]) : NULL;
+  film_holder *fh = argc > 1 ? (film_holder *)get_external(env, argv[1]) : NULL;
+  async_call *a = (async_call *)calloc(1, sizeof(async_call));
+  if (!a) {
+    napi_throw_error(env, NULL, "out of memory");
+    return NULL;
+  }
+  if (!ctx || !fh || argc < 3 || !read_camera(env, argv[2], &a->cam)) {
+    free(a);
+    napi_throw_type_error(env, NULL, "denoiseAsync(ctx, film, camera, params)");
+    return NULL;
+  }
+  a->kind = ASYNC_DENOISE;
+  a->ctx = ctx;
+  a->film = fh->film;
+  a->dp.sigma_l = 4.0f;
+  a->dp.sigma_n = 64.0f;
+  a->dp.sigma_p = 1.5f;
+  a->dp.iterations = 0;
+  if (argc > 3) {
+    a->dp.sigma_l = (float)num(env, prop(env, argv[3], "sigmaL"), 4);
+    a->dp.sigma_n = (float)num(env, prop(env, argv[3], "sigmaN"), 64);
+    a->dp.sigma_p = (float)num(env, prop(env, argv[3], "sigmaP"), 1.5);
+    a->dp.iterations = (uint32_t)num(env, prop(env, argv[3], "iterations"), 0);
+  }
+  uint32_t w, h, n, sd;
+  lucent_film_info(fh->film, &w, &h, &n, &sd);
+  a->px = (size_t)w * h;
+  a->iv = (float *)malloc(a->px * 16);
+  a->rad = (float *)malloc(a->px * 16);
+  if (!a->iv || !a->rad) {
+    free(a->iv);
+    free(a->rad);
+    free(a);
+    napi_throw_error(env, NULL, "out of memory");
+    return NULL;
+  }
+  return start_async(env, a, argv, 2, "lucent.denoiseAsync");
+}
+
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor props[] = {
       {"version", 0, js_version, 0, 0, 0, napi_default, 0},
       {"buildBVH", 0, js_build_bvh, 0, 0, 0, napi_default, 0},
+      {"buildBVHAsync", 0, js_build_bvh_async, 0, 0, 0, napi_default, 0},
+      {"traceSamplesAsync", 0, js_trace_samples_async, 0, 0, 0, napi_default, 0},
+      {"denoiseAsync", 0, js_denoise_async, 0, 0, 0, napi_default, 0},
       {"decodeHDR", 0, js_decode_hdr, 0, 0, 0, napi_default, 0},
       {"createContext", 0, js_create_context, 0, 0, 0, napi_default, 0},
       {"sceneFromBVH", 0, js_scene_from_bvh, 0, 0, 0, napi_default, 0},

@@ -2,6 +2,7 @@
 //   node check.js bvh <dir>     buildBVH over <dir>/positions.f32 -> nodes.f32, tri.u32
 //   node check.js hits <dir>    GpuTraversal over positions + bvh, rays org.f64/dir.f64 -> t/u/v.f64, tri.u32
 //   node check.js hdr <dir>     parseHDR(in.hdr) -> rgb.f32
+//   node check.js async <dir>   the render scene through the queued async calls vs the sync ones
 //   node check.js render <dir>  LucentRenderer over the packed scene in <dir> -> color.f32, albedo.f32,
 //                               den.f32, then a camera move -> moved.f32, shot.u8
 'use strict';
@@ -16,13 +17,21 @@ const u32 = (n) => { const b = fs.readFileSync(path.join(dir, n)); return new Ui
 const i32 = (n) => { const b = fs.readFileSync(path.join(dir, n)); return new Int32Array(b.buffer, b.byteOffset, b.length / 4); };
 const write = (n, ta) => fs.writeFileSync(path.join(dir, n), Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength));
 
+const same = (a, b) => a.length === b.length && Buffer.from(a.buffer, a.byteOffset, a.byteLength)
+  .equals(Buffer.from(b.buffer, b.byteOffset, b.byteLength));
+
 if (mode === 'bvh') {
   const bvh = lucent.buildBVH(f32('positions.f32'), { maxLeafSize: 4, bins: 12 });
   write('nodes.f32', bvh.nodes);
   write('tri.u32', bvh.triIndices);
   let err = '';
   try { lucent.buildBVH(new Float32Array(10)); } catch (e) { err = e.message; }
-  process.stdout.write(JSON.stringify({ nodeCount: bvh.nodeCount, bounds: bvh.bounds, error: err }) + '\n');
+  lucent.buildBVHAsync(f32('positions.f32'), { maxLeafSize: 4, bins: 12 }).then((abvh) => {
+    const asyncSame = abvh.nodeCount === bvh.nodeCount && same(abvh.nodes, bvh.nodes) && same(abvh.triIndices, bvh.triIndices);
+    return lucent.buildBVHAsync(new Float32Array(10)).then(() => 'resolved', (e) => e.message).then((asyncErr) => {
+      process.stdout.write(JSON.stringify({ nodeCount: bvh.nodeCount, bounds: bvh.bounds, error: err, asyncSame, asyncErr }) + '\n');
+    });
+  });
 } else if (mode === 'hdr') {
   const img = lucent.parseHDR(fs.readFileSync(path.join(dir, 'in.hdr')));
   write('rgb.f32', img.data);
@@ -36,6 +45,40 @@ if (mode === 'bvh') {
   write('t.f64', r.t); write('u.f64', r.u); write('v.f64', r.v); write('hit.u32', r.triIndex);
   const one = tr.intersect({ origin: Array.from(f64('org.f64').subarray(0, 3)), dir: Array.from(f64('dir.f64').subarray(0, 3)) });
   process.stdout.write(JSON.stringify({ first: one }) + '\n');
+} else if (mode === 'async') {
+  // The same scene as 'render': sync calls on one renderer, queued async calls on another.
+  const meta = JSON.parse(fs.readFileSync(path.join(dir, 'meta.json'), 'utf8'));
+  const packed = {
+    triGeometry: f32('geo.f32'), triAttributes: f32('attr.f32'), triCount: meta.triCount,
+    bvhNodes: f32('nodes.f32'), bvhNodeCount: meta.nodeCount, materials: f32('mat.f32'),
+    materialCount: meta.materialCount, spheres: f32('sph.f32'), sphereMaterials: i32('sphm.i32'),
+    sphereLights: i32('sphl.i32'), sphereCount: meta.sphereCount, lights: f32('lights.f32'),
+    lightCount: meta.lightCount, groundMaterial: meta.groundMaterial, cpu: { bvh: { triIndices: u32('tri.u32') } },
+  };
+  const scene = Object.assign({}, meta.scene);
+  scene.env = Object.assign({}, meta.scene.env, { map: { width: meta.envW, height: meta.envH, data: f32('env.f32') } });
+  const mk = () => {
+    const r = new lucent.LucentRenderer({ width: meta.width, height: meta.height });
+    r.settings = { maxBounces: 8, fireflyClamp: 0, temporal: true, denoise: true, exposure: 1, tonemap: 'aces' };
+    r.loadScene(packed, scene);
+    lucent.addon.resetFilm(r.ctx, r.film, meta.seed);
+    return r;
+  };
+  const params = { sigmaL: 4, sigmaN: 64, sigmaP: 1.5, iterations: 3 };
+  const a = mk();
+  a.traceSamples(meta.spp);
+  const da = a.runDenoiser(params);
+  const b = mk();
+  const p1 = b.traceSamplesAsync(1);
+  const p2 = b.traceSamplesAsync(meta.spp - 1);
+  let guard = '';
+  try { b.traceSamples(1); } catch (e) { guard = e.message; }
+  const p3 = b.runDenoiserAsync(params);
+  Promise.all([p1, p2, p3]).then(([, , db]) => {
+    const filmSame = same(a.read().color, b.read().color);
+    const denSame = same(da.illumVar, db.illumVar) && same(da.radiance, db.radiance);
+    process.stdout.write(JSON.stringify({ filmSame, denSame, guard, samples: b.stats().samples }) + '\n');
+  });
 } else if (mode === 'render') {
   const meta = JSON.parse(fs.readFileSync(path.join(dir, 'meta.json'), 'utf8'));
   const packed = {

@@ -39,6 +39,9 @@ def test_build_bvh_through_node(tmp_path):
     assert np.array_equal(tri, ref.tri_indices)
     assert np.array_equal(tri, build_bvh(soup).tri_indices)
     assert meta["error"] == "positions length 10 is not a multiple of 9"
+    # buildBVHAsync (bvh-async.ts): the same tree from a worker thread, the same error as a rejection.
+    assert meta["asyncSame"] is True
+    assert meta["asyncErr"] == "positions length 10 is not a multiple of 9"
 
 
 @pytest.mark.gpu
@@ -82,16 +85,14 @@ def test_parse_hdr_through_node(tmp_path):
     assert "Radiance" in meta["error"]
 
 
-@pytest.mark.gpu
-def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
+def write_render_inputs(tmp_path, W=48, H=32, SPP=3, SEED=9):
+    """The feature scene's PackedScene arrays, lighting and cameras as files for node/test/check.js."""
     import sys
     sys.path.insert(0, str(Path(__file__).parent))
     from test_pathtrace_gpu import feature_scene
-    from lucent_amd.renderer import Renderer, RenderSettings
     from lucent_amd.scene import pack_scene
     from lucent_amd import scene as S
     sc = feature_scene()
-    W, H, SPP, SEED = 48, 32, 3, 9
     p = pack_scene(sc)
     for name, arr in (("geo.f32", p.tri_geometry), ("attr.f32", p.tri_attributes), ("nodes.f32", p.bvh_nodes),
                       ("mat.f32", p.materials), ("sph.f32", p.spheres), ("sphm.i32", p.sphere_materials),
@@ -112,13 +113,21 @@ def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
     }
     c2 = S.Camera(position=(0.4, 2.6, 5.9), target=sc.camera.target, fov=sc.camera.fov, aperture=sc.camera.aperture,
                   focus_distance=sc.camera.focus_distance)
-    scene_json["camera2"] = {"position": list(c2.position), "target": list(c2.target), "fov": c2.fov,
-                             "aperture": c2.aperture, "focusDistance": c2.focus_distance}
+    cam2 = {"position": list(c2.position), "target": list(c2.target), "fov": c2.fov, "aperture": c2.aperture,
+            "focusDistance": c2.focus_distance}
     meta = {"triCount": p.tri_count, "nodeCount": p.bvh_node_count, "materialCount": p.material_count,
             "sphereCount": p.sphere_count, "lightCount": p.light_count, "groundMaterial": p.ground_material,
             "envW": w, "envH": h, "width": W, "height": H, "spp": SPP, "seed": SEED, "scene": scene_json,
-            "camera2": scene_json.pop("camera2")}
+            "camera2": cam2}
     (tmp_path / "meta.json").write_text(json.dumps(meta))
+    return sc, c2
+
+
+@pytest.mark.gpu
+def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
+    from lucent_amd.renderer import Renderer, RenderSettings
+    W, H, SPP, SEED = 48, 32, 3, 9
+    sc, c2 = write_render_inputs(tmp_path, W, H, SPP, SEED)
     stats = node("render", tmp_path)
     assert stats["samples"] == SPP and stats["seed"] == SEED + SPP
     r = Renderer(ctx, W, H, RenderSettings(8, 0.0))
@@ -140,3 +149,15 @@ def test_renderer_through_node_matches_python_binding(tmp_path, ctx):
     assert np.array_equal(np.fromfile(tmp_path / "color.f32", np.float32).reshape(H, W, 4), ref["color"])
     assert np.array_equal(np.fromfile(tmp_path / "albedo.f32", np.float32).reshape(H, W, 4), ref["albedo"])
     assert np.array_equal(np.fromfile(tmp_path / "den.f32", np.float32).reshape(H, W, 4), den["illum_var"])
+
+
+@pytest.mark.gpu
+def test_async_calls_through_node_equal_sync_calls(tmp_path, ctx):
+    """traceSamplesAsync and runDenoiserAsync on worker threads (napi_async_work),
+    queued on one renderer, leave the film and the denoised image bit-identical
+    to the synchronous calls; a synchronous call while they run is refused."""
+    write_render_inputs(tmp_path)
+    meta = node("async", tmp_path)
+    assert meta["filmSame"] is True and meta["denSame"] is True
+    assert "asynchronous call is still running" in meta["guard"]
+    assert meta["samples"] == json.loads((tmp_path / "meta.json").read_text())["spp"]
