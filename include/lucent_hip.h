@@ -297,6 +297,11 @@ typedef struct lucent_camera {
  * with idle lanes refilled from the queue.  Same film bit for bit.  Scenes
  * with alpha-blended materials use the default structure. */
 #define LUCENT_TRACE_WAVEFRONT 0x2u
+/* Default structure only: trace each shadow ray where the integrator asks for
+ * it (trace.frag's order of loops) instead of feeding a lane's closest-hit
+ * and shadow rays through one traversal loop.  Same film bit for bit; scenes
+ * with alpha-blended materials always trace this way. */
+#define LUCENT_TRACE_INLINE_SHADOWS 0x4u
 
 typedef struct lucent_render_settings {
   uint32_t max_bounces;

@@ -94,6 +94,7 @@ class CameraStruct(C.Structure):
 
 TRACE_PER_SAMPLE = 0x1
 TRACE_WAVEFRONT = 0x2
+TRACE_INLINE_SHADOWS = 0x4
 
 
 class RenderSettingsStruct(C.Structure):

@@ -400,6 +400,7 @@ lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_
     return LUCENT_OK;
   }
   TileLaunch cfg;
+  cfg.fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS) && !scene->blend_alpha;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
   if ((s = launch_trace_tiles(ctx, &it, 1, cfg, st)) != LUCENT_OK) return s;
@@ -432,6 +433,8 @@ lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items
     }
     return LUCENT_OK;
   }
+  bool fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS);
+  for (uint32_t i = 0; i < count; ++i) fused = fused && !items[i].scene->blend_alpha;
   // Longest jobs first: a tile job costs about samples x (log of the scene size).
   std::vector<uint32_t> order(count);
   for (uint32_t i = 0; i < count; ++i) order[i] = i;
@@ -452,6 +455,7 @@ lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items
     tiles.push_back(it);
   }
   TileLaunch cfg;
+  cfg.fused = fused;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
   Status s = launch_trace_tiles(ctx, tiles.data(), uint32_t(tiles.size()), cfg, st);
@@ -498,9 +502,12 @@ lucent_status lucent_autotune_tiles(lucent_ctx* ctx, const lucent_batch_item* it
     if (e == hipSuccess) e = hipEventCreate(&ev[1]);
     if (e != hipSuccess) s = fail_hip(e, "hipEventCreate");
   }
+  bool fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS);
+  for (uint32_t i = 0; i < count; ++i) fused = fused && !items[i].scene->blend_alpha;
   for (int pass = 0; pass < 2 && s == LUCENT_OK; ++pass) {  // pass 0 loads every code object
     for (uint32_t occ = 2; occ <= 6 && s == LUCENT_OK; ++occ) {
       TileLaunch cfg;
+      cfg.fused = fused;
       cfg.occupancy = occ;
       cfg.waves_per_cu = settings->waves_per_cu;
       hipError_t e = hipEventRecord(ev[0], st);
@@ -574,6 +581,7 @@ lucent_status lucent_forward(lucent_ctx* ctx, lucent_scene* scene, lucent_film* 
   it.spp = spp;
   it.sample_radiance = d_sample_radiance;
   TileLaunch cfg;
+  cfg.fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS) && !scene->blend_alpha;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
   if ((s = launch_trace_tiles(ctx, &it, 1, cfg, stream ? stream : ctx->stream)) != LUCENT_OK) return s;
@@ -846,6 +854,7 @@ lucent_status lucent_frame_graph_create(lucent_ctx* ctx, lucent_scene* scene, lu
   it.film = film->f;
   it.spp = samples;
   TileLaunch cfg;
+  cfg.fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS) && !scene->blend_alpha;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
   DenoiseArgs da;

@@ -7,19 +7,21 @@
 //   share BVH nodes in the caches.  A wave lasts as long as its longest path.
 //
 // * k_trace_tiles — persistent waves pull 8x8 tile jobs from an atomic queue
-//   (one or many films / scenes per launch) and run ALL of a job's samples:
-//   each lane owns one pixel, advances its path one bounce at a time and, as
+//   (one or many films / scenes per launch) and run ALL of their samples:
+//   a lane takes one pixel, advances its path one bounce at a time and, as
 //   soon as the path ends, folds it into the pixel's running mean and
-//   starts the pixel's next sample.  Lanes are refilled instead of idling
-//   behind the longest path of the wave.  Per pixel the samples run in seed
-//   order through the same accumulation step, so the film is bit-identical
-//   to spp k_trace_sample launches.  The running means stay in the film
+//   starts the pixel's next sample; after the pixel's last sample it takes
+//   the next free pixel of the wave's jobs.  Lanes are refilled instead of
+//   idling behind the longest path or the busiest pixel of the wave.  Per
+//   pixel the samples run in seed order through the same accumulation step,
+//   so the film is bit-identical to spp k_trace_sample launches.  The running means stay in the film
 //   (72 B of traffic per sample) rather than in registers: registers, not
 //   bandwidth, bound this kernel's occupancy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "pt_device.hpp"
@@ -29,6 +31,9 @@ namespace lucent {
 namespace {
 
 constexpr int kPtBlock = 64;
+#ifndef LUCENT_SHADE_BATCH
+#define LUCENT_SHADE_BATCH 32
+#endif
 constexpr int kTile = 8;
 
 __global__ __launch_bounds__(kPtBlock) void k_trace_sample(PtScene sc, PtFrame f, PtFilm film, uint32_t* spill,
@@ -142,63 +147,303 @@ __global__ __launch_bounds__(kPtBlock) void k_trace_reprojected(PtScene sc, PtFr
   pt::store_accum(film, idx, a);
 }
 
-template <class StackT>
-__device__ __forceinline__ void trace_pixel_samples(const TileItem& it, uint32_t x, uint32_t y, StackT& st,
-                                                    uint32_t seed0) {
-  const PtScene& sc = it.scene;
-  const PtFrame& f = it.frame;
-  const uint32_t idx = y * f.width + x;
-  uint32_t k = 0;
-  pt::Rng rng = pt::seed_rng(idx, seed0);
+__device__ __forceinline__ void begin_sample(const PtFrame& f, uint32_t x, uint32_t y, uint32_t idx, uint32_t seed,
+                                             pt::Rng& rng, pt::PathState& p) {
+  rng = pt::seed_rng(idx, seed);
   pt::v3 ro, rd;
   pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-  pt::PathState p;
   pt::path_begin(p, ro, rd);
+}
+
+// Takes the next job from the launch's queue: one atomic by lane 0, the
+// result broadcast.  Called with all 64 lanes active.
+__device__ __forceinline__ uint32_t take_job(uint32_t* counter) {
+  uint32_t v = 0;
+  if (threadIdx.x == 0) v = atomicAdd(counter, 1u);
+  return __builtin_amdgcn_readlane(v, 0);
+}
+
+// Runs one item's jobs on this wave, starting with `job`, already taken.
+// Each lane samples one pixel at a time: all of the pixel's spp samples in
+// seed order, one bounce per loop turn, then it takes the next pixel slot of
+// the job being handed out (slots are the valid pixels of an 8x8 tile; a
+// ballot ranks the lanes asking, so a turn hands out as many slots as there
+// are idle lanes).  When the job runs out the wave takes another from the
+// queue.  A job of a later item (or the end of the queue) stops the hand-out;
+// the wave finishes its pixels and returns that job.  So lanes do not idle
+// behind pixels whose paths are long (a tile straddling the sky and an
+// object, a glass surface) until the queue is empty.
+template <class StackT>
+__device__ __forceinline__ uint32_t run_item(const TileItem& it, uint32_t job, uint32_t job_base, uint32_t job_end,
+                                             uint32_t* counter, StackT& st) {
+  const PtFrame& f = it.frame;
+  // The first seed, read once per item in uniform control flow (a scalar).
+  const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
+  const uint32_t lane = threadIdx.x;
+  // The job being handed out (wave-uniform): tile corner, width, slots.
+  uint32_t x0 = 0, y0 = 0, tw = 1, slots = 0, handed = 0;
+  auto open_job = [&](uint32_t j) {
+    const uint32_t t = j - job_base;
+    x0 = (t % it.tiles_x) * kTile;
+    y0 = (t / it.tiles_x) * kTile;
+    tw = min(uint32_t(kTile), f.width - x0);
+    slots = tw * min(uint32_t(kTile), f.height - y0);
+    handed = 0;
+  };
+  open_job(job);
+  bool taking = true;
+  uint32_t next = 0;
+  // This lane's pixel and its current sample.
+  bool active = false;
+  uint32_t x = 0, y = 0, idx = 0, k = 0;
+  pt::Rng rng{};
+  pt::PathState p;
   for (;;) {
-    if (!pt::path_step(sc, f, p, rng, st)) {
+    uint64_t idle = __ballot(!active);
+    while (taking && idle != 0) {
+      if (handed == slots) {
+        const uint32_t j = take_job(counter);
+        if (j >= job_end) {
+          taking = false;
+          next = j;
+          break;
+        }
+        open_job(j);
+        continue;
+      }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
+      const uint32_t avail = slots - handed;
+      if (((idle >> lane) & 1u) && rank < avail) {
+        const uint32_t s = handed + rank;
+        x = x0 + s % tw;
+        y = y0 + s / tw;
+        idx = y * f.width + x;
+        k = 0;
+        begin_sample(f, x, y, idx, seed0, rng, p);
+        active = true;
+      }
+      handed += min(uint32_t(__popcll(idle)), avail);
+      idle = __ballot(!active);
+    }
+    if (active && !pt::path_step(it.scene, f, p, rng, st)) {
       if (it.sample_radiance)
         reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
             make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
       pt::accumulate(it.film, idx, p.radiance, p.guide);
-      if (++k >= it.spp) break;
-      rng = pt::seed_rng(idx, seed0 + k);
-      pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-      pt::path_begin(p, ro, rd);
+      if (++k >= it.spp)
+        active = false;
+      else
+        begin_sample(f, x, y, idx, seed0 + k, rng, p);
     }
+    if (!taking && __ballot(active) == 0) return next;
   }
 }
 
+// Per-lane words of the tile kernels' scratch area: three light-sample
+// slots of the fused loop (origin + extent, direction, unoccluded
+// contribution: 12 words each), then the traversal stack's spill words.
+constexpr uint32_t kShadowWords = 36;
+__host__ __device__ inline uint32_t tile_lane_words(uint32_t stack_words) {
+  return kShadowWords + ((stack_words + 3u) & ~3u);
+}
+
+// The fused loop's path-step tracer: the closest hit was traced before the
+// step; each light sample goes to slot `which` of the lane's scratch area
+// (its ray and its contribution, evaluated here: an unoccluded sample adds
+// exactly this value), to be traced after the step in slot order.
+struct DeferTracer {
+  static constexpr bool kRecordPrimary = false;
+  const PtScene& sc;
+  float4* slots;
+  pt::HitRecord rec;
+  uint32_t mask;  // slots filled (bit `which`)
+  __device__ __forceinline__ bool closest(pt::v3 ro, pt::v3 rd, pt::Hit& hit, pt::Rng&) {
+    return pt::build_hit(sc, ro, rd, rec, hit);
+  }
+  template <class F>
+  __device__ __forceinline__ void nee(int which, pt::v3 o, pt::v3 d, float tmax, pt::Rng&, pt::PathState&,
+                                      F contribution) {
+    const pt::v3 c = contribution();
+    slots[3 * which] = make_float4(o.x, o.y, o.z, tmax);
+    slots[3 * which + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+    slots[3 * which + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    mask |= 1u << which;
+  }
+};
+
+// Lanes whose rays have ended wait for shading until this many of them are
+// ready (or no lane is tracing), so that the shading code runs with most
+// lanes of the wave.
+constexpr int kShadeBatch = LUCENT_SHADE_BATCH;
+
+// run_item for scenes whose traversal draws no random numbers (no
+// alpha-blended material): every ray of a lane — closest-hit rays and the
+// shadow rays of its light samples — goes through ONE traversal loop, one
+// node or leaf step per turn, instead of a closest-hit loop and a shadow
+// loop per light sample that the lanes without such a ray sit out.  A lane
+// moves through: trace the closest hit; wait; shade the step (light samples
+// into its slots, the continuation ray into the path state); trace the
+// samples' shadow rays in slot order, adding each unoccluded contribution;
+// trace the continuation or, at the path's end, wait and fold the sample
+// into the film.  The random numbers are drawn in the same order as
+// path_step draws them and the radiance is summed in the same order, so the
+// film is bit-identical to run_item's.
+enum : uint32_t { kLaneIdle = 0, kLaneTrace = 1, kLaneShade = 2, kLaneFinish = 3 };
+
 template <class StackT>
-__device__ __forceinline__ void run_tile(const TileItem& it, uint32_t tile, StackT& st) {
-  // The first seed, read once per job in uniform control flow (a scalar).
-  const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : it.frame.seed;
-  const uint32_t x = (tile % it.tiles_x) * kTile + (threadIdx.x & (kTile - 1));
-  const uint32_t y = (tile / it.tiles_x) * kTile + (threadIdx.x / kTile);
-  if (x < it.frame.width && y < it.frame.height) trace_pixel_samples(it, x, y, st, seed0);
+__device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t job, uint32_t job_base, uint32_t job_end,
+                                                   uint32_t* counter, StackT& st, float4* slots) {
+  const PtFrame& f = it.frame;
+  const PtScene& sc = it.scene;
+  const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
+  const uint32_t lane = threadIdx.x;
+  uint32_t x0 = 0, y0 = 0, tw = 1, slots_in_job = 0, handed = 0;
+  auto open_job = [&](uint32_t j) {
+    const uint32_t t = j - job_base;
+    x0 = (t % it.tiles_x) * kTile;
+    y0 = (t / it.tiles_x) * kTile;
+    tw = min(uint32_t(kTile), f.width - x0);
+    slots_in_job = tw * min(uint32_t(kTile), f.height - y0);
+    handed = 0;
+  };
+  open_job(job);
+  bool taking = true;
+  uint32_t next = 0;
+  uint32_t state = kLaneIdle;
+  uint32_t x = 0, y = 0, idx = 0, k = 0;
+  pt::Rng rng{};
+  pt::PathState p;
+  pt::Trav4 t;
+  pt::HitRecord rec;
+  bool any = false, cont = false;
+  uint32_t which = 0, pending = 0;
+  auto start_closest = [&]() {
+    pt::search_analytic(sc, p.ro, p.rd, rec);
+    pt::trav4_begin(sc, t, p.ro, p.rd, rec.t, st);
+    any = false;
+    state = kLaneTrace;
+  };
+  // The next ray of the step: the lowest pending light sample not blocked by
+  // a sphere or the ground (occluded's order), else the continuation, else
+  // the path is finished.
+  auto next_ray = [&]() {
+    while (pending != 0) {
+      which = uint32_t(__builtin_ctz(pending));
+      pending &= pending - 1u;
+      const float4 o = slots[3 * which], d = slots[3 * which + 1];
+      const pt::v3 ro = pt::mk(o), rd = pt::mk(d);
+      bool blocked = false;
+      for (uint32_t i = 0; i < sc.sphere_count && !blocked; ++i)
+        blocked = pt::hit_sphere(ro, rd, reinterpret_cast<const float4*>(sc.spheres)[i], o.w) < o.w;
+      if (!blocked) blocked = pt::hit_ground(sc, ro, rd, o.w) < o.w;
+      if (!blocked) {
+        pt::trav4_begin(sc, t, ro, rd, o.w, st);
+        any = true;
+        state = kLaneTrace;
+        return;
+      }
+    }
+    if (cont)
+      start_closest();
+    else
+      state = kLaneFinish;
+  };
+  for (;;) {
+    uint64_t idle = __ballot(state == kLaneIdle);
+    while (taking && idle != 0) {
+      if (handed == slots_in_job) {
+        const uint32_t j = take_job(counter);
+        if (j >= job_end) {
+          taking = false;
+          next = j;
+          break;
+        }
+        open_job(j);
+        continue;
+      }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
+      const uint32_t avail = slots_in_job - handed;
+      if (((idle >> lane) & 1u) && rank < avail) {
+        const uint32_t s = handed + rank;
+        x = x0 + s % tw;
+        y = y0 + s / tw;
+        idx = y * f.width + x;
+        k = 0;
+        begin_sample(f, x, y, idx, seed0, rng, p);
+        start_closest();
+      }
+      handed += min(uint32_t(__popcll(idle)), avail);
+      idle = __ballot(state == kLaneIdle);
+    }
+    if (state == kLaneTrace) {
+      bool blocked = false;
+      if (!pt::trav4_done(t, st)) blocked = pt::trav4_step_mixed(sc, t, st, rng, any);
+      if (blocked || pt::trav4_done(t, st)) {
+        if (any) {
+          if (!blocked) p.radiance = p.radiance + pt::mk(slots[3 * which + 2]);
+          next_ray();
+        } else {
+          if (t.best >= 0) {
+            rec.t = t.tmax;
+            rec.kind = pt::kHitTri;
+            rec.prim = uint32_t(t.best);
+            rec.u = t.bu;
+            rec.v = t.bv;
+          }
+          state = kLaneShade;
+        }
+      }
+    }
+    const uint64_t ready = __ballot(state == kLaneShade || state == kLaneFinish);
+    if (ready != 0 && (__popcll(ready) >= kShadeBatch || __ballot(state == kLaneTrace) == 0)) {
+      if (state == kLaneShade) {
+        DeferTracer tr{sc, slots, rec, 0u};
+        cont = pt::path_step_with(sc, f, p, rng, tr);
+        pending = tr.mask;
+        next_ray();
+      }
+      if (state == kLaneFinish) {
+        if (it.sample_radiance)
+          reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
+              make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
+        pt::accumulate(it.film, idx, p.radiance, p.guide);
+        if (++k >= it.spp) {
+          state = kLaneIdle;
+        } else {
+          begin_sample(f, x, y, idx, seed0 + k, rng, p);
+          start_closest();
+        }
+      }
+    }
+    if (!taking && __ballot(state != kLaneIdle) == 0) return next;
+  }
 }
 
 // Jobs are numbered film by film: film i owns jobs [job_begin[i], job_begin[i+1]).
 // kSingle: one film, passed by value (its fields live in the kernel arguments).
 // kWaves: the occupancy the register allocation targets (waves per SIMD);
 // more waves hide more memory latency at the price of register spills.
-template <bool kSingle, int kWaves>
+// kFused: run_item_fused (no item draws random numbers inside traversal).
+// Each lane owns lane_words words of `scratch`: its light-sample slots, then
+// its stack's spill words.
+template <bool kSingle, int kWaves, bool kFused>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_tiles(TileItem single, const TileItem* __restrict__ items,
                                                          const uint32_t* __restrict__ job_begin, uint32_t n_items,
                                                          uint32_t n_jobs, uint32_t* __restrict__ counter,
-                                                         uint32_t* spill, uint32_t spill_depth) {
+                                                         uint32_t* scratch, uint32_t lane_words) {
   __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
-  __shared__ uint32_t next_job;
-  pt::PtStack<kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
-                                 0, 0};
-  for (;;) {
-    if (threadIdx.x == 0) next_job = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t job = __builtin_amdgcn_readfirstlane(next_job);
-    __syncthreads();
-    if (job >= n_jobs) break;
-    uint32_t item = 0;
-    if (!kSingle) {
-      uint32_t hi = n_items;
+  uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
+  float4* slots = reinterpret_cast<float4*>(area);
+  pt::PtStack<kPtBlock> st{lds + threadIdx.x, area + kShadowWords, 0, 0};
+  uint32_t job = take_job(counter);
+  while (job < n_jobs) {
+    if constexpr (kSingle) {
+      if constexpr (kFused)
+        job = run_item_fused(single, job, 0u, n_jobs, counter, st, slots);
+      else
+        job = run_item(single, job, 0u, n_jobs, counter, st);
+    } else {
+      uint32_t item = 0, hi = n_items;
       while (hi - item > 1) {
         const uint32_t mid = (item + hi) / 2;
         if (job_begin[mid] <= job)
@@ -206,11 +451,11 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
         else
           hi = mid;
       }
+      if constexpr (kFused)
+        job = run_item_fused(items[item], job, job_begin[item], job_begin[item + 1], counter, st, slots);
+      else
+        job = run_item(items[item], job, job_begin[item], job_begin[item + 1], counter, st);
     }
-    if constexpr (kSingle)
-      run_tile(single, job, st);
-    else
-      run_tile(items[item], job - job_begin[item], st);
   }
 }
 
@@ -255,7 +500,7 @@ size_t tile_spill_bytes(const Context* ctx, const TileItem& item, const TileLaun
   const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t n_jobs = item.spp ? tile_jobs(item.frame.width, item.frame.height) : 0;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
-  return size_t(grid) * kPtBlock * std::max<uint32_t>(1, pt::pt_spill_words(item.scene.bvh)) * sizeof(uint32_t);
+  return size_t(grid) * kPtBlock * tile_lane_words(pt::pt_spill_words(item.scene.bvh)) * sizeof(uint32_t);
 }
 
 Status launch_add_u32(uint32_t* p, uint32_t v, void* stream) {
@@ -285,7 +530,8 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   const uint32_t waves = (cfg.occupancy >= 2 && cfg.occupancy <= 6) ? cfg.occupancy : 2;
   const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
-  const size_t spill_bytes = size_t(grid) * kPtBlock * depth * sizeof(uint32_t);
+  const uint32_t lane_words = tile_lane_words(depth);
+  const size_t spill_bytes = size_t(grid) * kPtBlock * lane_words * sizeof(uint32_t);
   uint32_t* spill = cfg.spill;
   uint32_t* counter = cfg.counter;
   Status s;
@@ -322,27 +568,34 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   auto launch = [&](auto single_kernel, auto multi_kernel) {
     if (n_items == 1)
       hipLaunchKernelGGL(single_kernel, dim3(grid), dim3(kPtBlock), 0, stream, items[0], nullptr, nullptr, 1u, n_jobs,
-                         counter, spill, depth);
+                         counter, spill, lane_words);
     else
       hipLaunchKernelGGL(multi_kernel, dim3(grid), dim3(kPtBlock), 0, stream, TileItem{}, d_items, d_begin, n_items,
-                         n_jobs, counter, spill, depth);
+                         n_jobs, counter, spill, lane_words);
   };
-  switch (waves) {
-    case 3:
-      launch(k_trace_tiles<true, 3>, k_trace_tiles<false, 3>);
-      break;
-    case 4:
-      launch(k_trace_tiles<true, 4>, k_trace_tiles<false, 4>);
-      break;
-    case 5:
-      launch(k_trace_tiles<true, 5>, k_trace_tiles<false, 5>);
-      break;
-    case 6:
-      launch(k_trace_tiles<true, 6>, k_trace_tiles<false, 6>);
-      break;
-    default:
-      launch(k_trace_tiles<true, 2>, k_trace_tiles<false, 2>);
-  }
+  auto launch_waves = [&](auto fused) {
+    constexpr bool kF = decltype(fused)::value;
+    switch (waves) {
+      case 3:
+        launch(k_trace_tiles<true, 3, kF>, k_trace_tiles<false, 3, kF>);
+        break;
+      case 4:
+        launch(k_trace_tiles<true, 4, kF>, k_trace_tiles<false, 4, kF>);
+        break;
+      case 5:
+        launch(k_trace_tiles<true, 5, kF>, k_trace_tiles<false, 5, kF>);
+        break;
+      case 6:
+        launch(k_trace_tiles<true, 6, kF>, k_trace_tiles<false, 6, kF>);
+        break;
+      default:
+        launch(k_trace_tiles<true, 2, kF>, k_trace_tiles<false, 2, kF>);
+    }
+  };
+  if (cfg.fused)
+    launch_waves(std::true_type{});
+  else
+    launch_waves(std::false_type{});
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "k_trace_tiles launch");
   return LUCENT_OK;

@@ -37,6 +37,10 @@ struct TileLaunch {
   uint32_t* counter = nullptr;
   uint32_t* spill = nullptr;
   size_t spill_bytes = 0;
+  // All rays of a lane through one traversal loop (false: each shadow ray
+  // traced where the integrator asks for it).  Only for scenes whose
+  // traversal draws no random numbers (no alpha-blended material).
+  bool fused = true;
 };
 // Spill bytes the launch of one film with `cfg` needs.
 size_t tile_spill_bytes(const Context* ctx, const TileItem& item, const TileLaunch& cfg);

@@ -27,14 +27,17 @@ class RenderSettings:
     """Integrator part of RenderSettings (renderer.ts:42-84); the reference's
     default firefly clamp is 8, the benchmark configurations use 0 (unbiased).
     per_sample_launch selects the reference's launch structure (one pass over
-    the film per sample) instead of the persistent tile kernel; both give
-    bit-identical films."""
+    the film per sample) instead of the persistent tile kernel;
+    inline_shadows makes the tile kernel trace each shadow ray where the
+    integrator asks for it instead of in the lane's common traversal loop.
+    All structures give bit-identical films."""
     max_bounces: int = 8
     firefly_clamp: float = 8.0
     per_sample_launch: bool = False
     waves_per_cu: int = 0
     occupancy: int = 0
     wavefront: bool = False
+    inline_shadows: bool = False
     # Presentation part (renderer.ts:42-84 defaults).
     temporal: bool = True
     denoise: bool = True
@@ -49,7 +52,8 @@ class RenderSettings:
 
     def struct(self) -> "_native.RenderSettingsStruct":
         flags = (_native.TRACE_PER_SAMPLE if self.per_sample_launch else 0) | \
-                (_native.TRACE_WAVEFRONT if self.wavefront else 0)
+                (_native.TRACE_WAVEFRONT if self.wavefront else 0) | \
+                (_native.TRACE_INLINE_SHADOWS if self.inline_shadows else 0)
         return _native.RenderSettingsStruct(self.max_bounces, self.firefly_clamp, flags, self.waves_per_cu,
                                             self.occupancy)
 

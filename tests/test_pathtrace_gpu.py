@@ -288,6 +288,34 @@ def test_batch_with_alpha_scene_bit_identical(ctx):
         r.close()
 
 
+def blend_scene():
+    """alpha_scene with the knot alpha-blended (blend mode draws a random
+    number per candidate hit inside traversal)."""
+    import dataclasses
+    sc = alpha_scene()
+    m = sc.meshes[0]
+    sc.meshes[0] = S.MeshInstance(m.mesh, dataclasses.replace(m.material, alpha_mode="blend", alpha=0.6))
+    return sc
+
+
+@pytest.mark.parametrize("which,occupancy", [("feature", 3), ("feature", 4), ("knots", 3), ("alpha", 3), ("blend", 3)])
+def test_fused_traversal_loop_bit_identical_to_inline_shadows(ctx, which, occupancy):
+    """The tile kernel's default structure (a lane's closest-hit and shadow
+    rays through one traversal loop, shading in batches of lanes) and the
+    inline structure (each shadow ray traced where the integrator asks for
+    it) both give the per-sample launches' film bit for bit.  A scene with an
+    alpha-blended material takes the inline structure."""
+    sc = {"feature": feature_scene, "knots": workloads.knot_scene, "alpha": alpha_scene, "blend": blend_scene}[which]()
+    w, h, spp = (144, 90, 3) if which == "knots" else (70, 45, 6)
+    a, ia = _film_after(ctx, sc, w, h, spp, 4, RenderSettings(8, 0.0, per_sample_launch=True))
+    b, ib = _film_after(ctx, sc, w, h, spp, 4, RenderSettings(8, 0.0, occupancy=occupancy))
+    c, ic = _film_after(ctx, sc, w, h, spp, 4, RenderSettings(8, 0.0, occupancy=occupancy, inline_shadows=True))
+    assert ia == ib == ic
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], c[k]), k
+
+
 @pytest.mark.parametrize("clamp", [0.0, 8.0])
 def test_wavefront_bit_identical_to_per_sample_launches(ctx, clamp):
     """The wavefront structure (separate closest-hit, shadow and shading
