@@ -23,6 +23,8 @@
 // spread contention and are reduced into FP64 by a final kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "adjoint.hpp"
 #include "pt_device.hpp"
 
@@ -250,14 +252,34 @@ __device__ __forceinline__ void add_f_term(GradSink& g, int mat, const ParamScal
 }
 
 // The replay of tracePath (trace.frag:136-272) accumulating d(dL · L)/dθ.
+// A path's replay state between bounces: the ray, the radiance still ahead
+// of this vertex (lr), the throughput.
+struct ReplayState {
+  v3 ro, rd, lr, beta;
+  float bsdf_pdf;
+  int bounce;
+  bool specular;
+};
+
+__device__ __forceinline__ void replay_begin(ReplayState& r, v3 ro, v3 rd, v3 L) {
+  r.ro = ro;
+  r.rd = rd;
+  r.lr = L;
+  r.beta = mk(1.f, 1.f, 1.f);
+  r.bsdf_pdf = 0.0f;
+  r.bounce = 0;
+  r.specular = true;
+}
+
+// One bounce of the replay; false once the path has ended.
 template <class StackT>
-__device__ __forceinline__ void replay(const PtScene& sc, const PtFrame& f, v3 ro, v3 rd, Rng& rng, StackT& st, v3 L, v3 dl,
-                              GradSink& g) {
-  v3 lr = L;
-  v3 beta = mk(1.f, 1.f, 1.f);
-  bool specular = true;
-  float bsdf_pdf = 0.0f;
-  for (int bounce = 0; bounce <= int(f.max_bounces); ++bounce) {
+__device__ __forceinline__ bool replay_step(const PtScene& sc, const PtFrame& f, ReplayState& r, Rng& rng, StackT& st,
+                                            v3 dl, GradSink& g) {
+  v3 ro = r.ro, rd = r.rd, lr = r.lr, beta = r.beta;
+  bool specular = r.specular;
+  float bsdf_pdf = r.bsdf_pdf;
+  const int bounce = r.bounce;
+  {
     Hit hit;
     if (!trace_scene(sc, ro, rd, hit, rng, st)) {
       if (!(bounce == 0 && sc.env_background == 0)) {
@@ -267,7 +289,7 @@ __device__ __forceinline__ void replay(const PtScene& sc, const PtFrame& f, v3 r
         equirect_uv(env_to_local(sc, rd), u, v);
         add_env(sc, g, u, v, dl * beta * w);
       }
-      return;
+      return false;
     }
     const v3 wo = -rd;
     v3 emission;
@@ -278,14 +300,14 @@ __device__ __forceinline__ void replay(const PtScene& sc, const PtFrame& f, v3 r
       float w = 1.0f;
       if (!specular && hit.light >= 0) w = mis_weight(bsdf_pdf, light_pdf(sc, hit.light, ro, rd, hit.t) / float(sc.light_count));
       lr = lr - beta * emission * w;
-      if (hit.light >= 0) return;
+      if (hit.light >= 0) return false;
     }
     const bool glass = s.transmission > 0.0f && rng.rand() < s.transmission;
     const ParamScale ps = param_scale(sc, hit);
     if (glass) {
       v3 wi, weight;
       bool reflected;
-      if (!sample_dielectric(s, hit.front_face, ng, wo, rng, wi, weight, reflected)) return;
+      if (!sample_dielectric(s, hit.front_face, ng, wo, rng, wi, weight, reflected)) return false;
       if (!reflected) {
         // Refraction weight = albedo: later terms scale with it.
         float out[kNumParams] = {0, 0, 0, 0, 0};
@@ -350,7 +372,7 @@ __device__ __forceinline__ void replay(const PtScene& sc, const PtFrame& f, v3 r
         }
       }
       const BsdfSample bs = sample_opaque(sc, s, wo, rng);
-      if (bs.pdf <= 0.0f || dot(bs.dir, ng) <= 0.0f) return;
+      if (bs.pdf <= 0.0f || dot(bs.dir, ng) <= 0.0f) return false;
       // Every later term carries this vertex's f: δL · Lr * (df/dθ) / f.
       if (eval_opaque_grad(sc, s, wo, bs.dir, fv, dfv)) {
         v3 coeff = mk(0.f, 0.f, 0.f);
@@ -367,10 +389,18 @@ __device__ __forceinline__ void replay(const PtScene& sc, const PtFrame& f, v3 r
     }
     if (bounce >= 2) {
       const float p = fminf(0.95f, max3(beta));
-      if (rng.rand() >= p) return;
+      if (rng.rand() >= p) return false;
       beta = beta / p;
     }
   }
+  r.ro = ro;
+  r.rd = rd;
+  r.lr = lr;
+  r.beta = beta;
+  r.specular = specular;
+  r.bsdf_pdf = bsdf_pdf;
+  r.bounce = bounce + 1;
+  return r.bounce <= int(f.max_bounces);
 }
 
 // The detached estimator as a function of the material parameters: every
@@ -378,7 +408,7 @@ __device__ __forceinline__ void replay(const PtScene& sc, const PtFrame& f, v3 r
 // from the scene's materials `sc` (θ0), the BSDF values, refraction tints,
 // absorption and emission from `sv` (θ, same geometry and lighting).  At θ = θ0
 // this is tracePath (trace.frag:136-272, clamp 0) up to rounding; its
-// derivative in θ at θ0 is the sum replay() accumulates.  Two throughputs run
+// derivative in θ at θ0 is the sum replay_step accumulates.  Two throughputs run
 // side by side: beta0 drives roulette, beta carries the θ values.
 template <class StackT>
 __device__ inline v3 detached_path(const PtScene& sc, const PtScene& sv, const PtFrame& f, v3 ro, v3 rd, Rng& rng,
@@ -501,46 +531,107 @@ __global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, 
   *o = make_float4(prev.x + L.x * inv_spp, prev.y + L.y * inv_spp, prev.z + L.z * inv_spp, 0.0f);
 }
 
+// Persistent replay over all samples: waves take 8x8 tile jobs from an
+// atomic queue; each lane replays one pixel's samples one bounce per loop
+// turn (sample k: seed seed0 + k, radiance recorded_L[k]) and, after the
+// pixel's last sample, takes the next free pixel of the wave's jobs, so
+// lanes do not idle behind the longest path of the wave.  Material
+// gradients gather in the block's LDS table over everything the block
+// replays and are written once as the block's partials.
 // kWaves: the occupancy the register allocation targets (waves per SIMD).
 template <int kWaves>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_adjoint(
-    PtScene sc, PtFrame f, const float* __restrict__ dl_di,
-                                                       const float4* __restrict__ recorded_L, float inv_spp,
-                                                       float* __restrict__ mat_partials,
-                                                       float* __restrict__ env_acc, size_t env_stride,
-                                                       float* __restrict__ mat_overflow, uint32_t* spill,
-                                                       uint32_t spill_depth, uint32_t tiles_x) {
+    PtScene sc, PtFrame f, uint32_t spp, uint32_t n_jobs, uint32_t* __restrict__ counter, const float* __restrict__ dl_di,
+    const float4* __restrict__ recorded_L, float inv_spp, float* __restrict__ mat_partials, float* __restrict__ env_acc,
+    size_t env_stride, float* __restrict__ mat_overflow, uint32_t* spill, uint32_t spill_depth, uint32_t tiles_x) {
   __shared__ uint32_t lds[kPtStackWords * kAdjBlock];
   __shared__ float lds_grad[kLdsMaterials * kNumParams];
   for (int i = threadIdx.x; i < kLdsMaterials * kNumParams; i += kAdjBlock) lds_grad[i] = 0.0f;
   __syncthreads();
-  const uint32_t tile = blockIdx.x;
-  const uint32_t x = (tile % tiles_x) * kTile + (threadIdx.x & (kTile - 1));
-  const uint32_t y = (tile / tiles_x) * kTile + (threadIdx.x / kTile);
-  if (x < f.width && y < f.height) {
-    const uint32_t idx = y * f.width + x;
-    const float4 d = reinterpret_cast<const float4*>(dl_di)[idx];
-    const v3 dl = mk(d.x * inv_spp, d.y * inv_spp, d.z * inv_spp);
-    if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
-      Rng rng = seed_rng(idx, f.seed);
+  PtStack<kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0, 0};
+  GradSink g{lds_grad, env_acc, env_stride, blockIdx.x % kEnvReplicas, int(sc.material_count), mat_overflow};
+  const uint32_t lane = threadIdx.x;
+  const size_t frame_px = size_t(f.width) * f.height;
+  auto take = [&]() {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(counter, 1u);
+    return __builtin_amdgcn_readlane(v, 0);
+  };
+  uint32_t job = take();
+  uint32_t x0 = 0, y0 = 0, tw = 1, slots = 0, handed = 0;
+  auto open_job = [&](uint32_t j) {
+    x0 = (j % tiles_x) * kTile;
+    y0 = (j / tiles_x) * kTile;
+    tw = min(uint32_t(kTile), f.width - x0);
+    slots = tw * min(uint32_t(kTile), f.height - y0);
+    handed = 0;
+  };
+  bool taking = job < n_jobs;
+  if (taking) open_job(job);
+  bool active = false;
+  uint32_t x = 0, y = 0, idx = 0, k = 0;
+  v3 dl = mk(0.f, 0.f, 0.f);
+  Rng rng{};
+  ReplayState r;
+  // Starts the pixel's sample k or a later one (a sample whose radiance is
+  // not finite contributes no gradient, as the film's accumulation drops
+  // it); inactive after the last.
+  auto start_sample = [&]() {
+    for (; k < spp; ++k) {
+      rng = seed_rng(idx, f.seed + k);
       v3 ro, rd;
       camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-      PtStack<kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth,
-                                   0, 0};
-      Rng replay_rng = rng;
+      const Rng replay_rng = rng;
       v3 L;
       if (recorded_L) {  // the forward pass kept this path's radiance
-        const float4 r = recorded_L[idx];
-        L = mk(r.x, r.y, r.z);
+        const float4 rr = recorded_L[size_t(k) * frame_px + idx];
+        L = mk(rr.x, rr.y, rr.z);
       } else {
         Guide gd;
         L = trace_path(sc, f, ro, rd, rng, st, gd);
       }
       if (!(isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))) {
-        GradSink g{lds_grad, env_acc, env_stride, blockIdx.x % kEnvReplicas, int(sc.material_count), mat_overflow};
-        replay(sc, f, ro, rd, replay_rng, st, L, dl, g);
+        rng = replay_rng;
+        replay_begin(r, ro, rd, L);
+        active = true;
+        return;
       }
     }
+    active = false;
+  };
+  for (;;) {
+    uint64_t idle = __ballot(!active);
+    while (taking && idle != 0) {
+      if (handed == slots) {
+        const uint32_t j = take();
+        if (j >= n_jobs) {
+          taking = false;
+          break;
+        }
+        open_job(j);
+        continue;
+      }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
+      const uint32_t avail = slots - handed;
+      if (((idle >> lane) & 1u) && rank < avail) {
+        const uint32_t s = handed + rank;
+        x = x0 + s % tw;
+        y = y0 + s / tw;
+        idx = y * f.width + x;
+        const float4 d = reinterpret_cast<const float4*>(dl_di)[idx];
+        dl = mk(d.x * inv_spp, d.y * inv_spp, d.z * inv_spp);
+        k = 0;
+        if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) start_sample();
+      }
+      handed += min(uint32_t(__popcll(idle)), avail);
+      // Lanes whose pixel has no loss gradient (or no finite sample) ask again.
+      idle = __ballot(!active);
+    }
+    if (active && !replay_step(sc, f, r, rng, st, dl, g)) {
+      ++k;
+      start_sample();
+    }
+    if (!taking && __ballot(active) == 0) break;
   }
   __syncthreads();
   const int nm = min(int(sc.material_count), kLdsMaterials);
@@ -590,7 +681,7 @@ size_t adjoint_scratch_bytes(uint32_t width, uint32_t height, uint32_t material_
   const size_t partials = blocks * kLdsMaterials * kNumParams;
   const size_t overflow = size_t(material_count) * kNumParams;
   const size_t env = size_t(kEnvReplicas) * env_texels * 3;
-  return (partials + overflow + env) * sizeof(float) + 1024;
+  return (partials + overflow + env) * sizeof(float) + sizeof(uint32_t) + 1024;  // + the replay's job counter
 }
 
 Status launch_detached(Context* ctx, const DetachedArgs& a, void* stream_) {
@@ -619,39 +710,40 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
   const PtScene& sc = *a.scene;
   const uint32_t tiles_x = (a.frame.width + kTile - 1) / kTile;
   const uint32_t tiles_y = (a.frame.height + kTile - 1) / kTile;
-  const uint32_t blocks = tiles_x * tiles_y;
+  const uint32_t n_jobs = tiles_x * tiles_y;
+  const uint32_t waves = (a.occupancy == 3 || a.occupancy == 4) ? a.occupancy : 2;
+  const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * 4 * waves);
   const size_t env_texels = size_t(sc.env_w) * sc.env_h;
-  const size_t partials = size_t(blocks) * kLdsMaterials * kNumParams;
+  const size_t partials = size_t(n_jobs) * kLdsMaterials * kNumParams;
   const size_t overflow = size_t(sc.material_count) * kNumParams;
   const size_t env_stride = env_texels * 3;
+  // Scratch: [block partials | overflow | env replicas | job counter] (adjoint_scratch_bytes).
   float* base = static_cast<float*>(a.scratch);
   float* mat_partials = base;
   float* mat_overflow = base + partials;
   float* env_acc = mat_overflow + overflow;
+  uint32_t* counter = reinterpret_cast<uint32_t*>(env_acc + kEnvReplicas * env_stride);
   const uint32_t depth = pt_spill_words(sc.bvh);
-  Status s = ensure_spill(ctx, size_t(blocks) * kAdjBlock * depth * sizeof(uint32_t));
+  Status s = ensure_spill(ctx, size_t(grid) * kAdjBlock * depth * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
-  const size_t zero_n = overflow + kEnvReplicas * env_stride;
-  for (uint32_t k = 0; k < a.spp; ++k) {
-    hipLaunchKernelGGL(k_zero, dim3(uint32_t((zero_n + 255) / 256)), dim3(256), 0, stream, mat_overflow, zero_n);
-    PtFrame f = a.frame;
-    f.seed = a.seed0 + k;
-    const float4* rec = a.recorded_radiance
-                            ? reinterpret_cast<const float4*>(a.recorded_radiance) + size_t(k) * a.frame.width * a.frame.height
-                            : nullptr;
-    auto kern = a.occupancy == 3 ? k_adjoint<3> : a.occupancy == 4 ? k_adjoint<4> : k_adjoint<2>;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kAdjBlock), 0, stream, sc, f, a.dl_di, rec, 1.0f / float(a.spp),
-                       mat_partials, env_acc, env_stride, mat_overflow, static_cast<uint32_t*>(ctx->spill), depth,
-                       tiles_x);
-    const uint32_t entries = uint32_t(min(int(sc.material_count), kLdsMaterials)) * kNumParams;
-    const uint32_t all = uint32_t(sc.material_count) * kNumParams;
-    if (all > 0)
-      hipLaunchKernelGGL(k_reduce_materials, dim3(all), dim3(256), 0, stream, mat_partials, blocks, entries, mat_overflow,
-                         all, a.grad_materials);
-    if (env_stride > 0 && a.grad_env)
-      hipLaunchKernelGGL(k_reduce_env, dim3(uint32_t((env_stride + 255) / 256)), dim3(256), 0, stream, env_acc, env_stride,
-                         env_stride, a.grad_env);
-  }
+  // Overflow, env replicas and the counter (0.0f has the bits of 0u).
+  const size_t zero_n = overflow + kEnvReplicas * env_stride + 1;
+  hipLaunchKernelGGL(k_zero, dim3(uint32_t((zero_n + 255) / 256)), dim3(256), 0, stream, mat_overflow, zero_n);
+  PtFrame f = a.frame;
+  f.seed = a.seed0;
+  const float4* rec = reinterpret_cast<const float4*>(a.recorded_radiance);
+  auto kern = waves == 3 ? k_adjoint<3> : waves == 4 ? k_adjoint<4> : k_adjoint<2>;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), 0, stream, sc, f, a.spp, n_jobs, counter, a.dl_di, rec,
+                     1.0f / float(a.spp), mat_partials, env_acc, env_stride, mat_overflow,
+                     static_cast<uint32_t*>(ctx->spill), depth, tiles_x);
+  const uint32_t entries = uint32_t(min(int(sc.material_count), kLdsMaterials)) * kNumParams;
+  const uint32_t all = uint32_t(sc.material_count) * kNumParams;
+  if (all > 0)
+    hipLaunchKernelGGL(k_reduce_materials, dim3(all), dim3(256), 0, stream, mat_partials, grid, entries, mat_overflow, all,
+                       a.grad_materials);
+  if (env_stride > 0 && a.grad_env)
+    hipLaunchKernelGGL(k_reduce_env, dim3(uint32_t((env_stride + 255) / 256)), dim3(256), 0, stream, env_acc, env_stride,
+                       env_stride, a.grad_env);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "adjoint launch");
   return LUCENT_OK;

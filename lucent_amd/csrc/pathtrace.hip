@@ -310,15 +310,19 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
   bool taking = true;
   uint32_t next = 0;
   uint32_t state = kLaneIdle;
-  uint32_t x = 0, y = 0, idx = 0, k = 0;
+  uint32_t idx = 0, k = 0;
   pt::Rng rng{};
   pt::PathState p;
   pt::Trav4 t;
-  pt::HitRecord rec;
+  // The analytic part of the closest hit (kind << 30 | prim); its distance
+  // is t.tmax until a triangle is accepted.
+  uint32_t analytic = 0;
   bool any = false, cont = false;
   uint32_t which = 0, pending = 0;
   auto start_closest = [&]() {
+    pt::HitRecord rec;
     pt::search_analytic(sc, p.ro, p.rd, rec);
+    analytic = rec.kind << 30 | rec.prim;
     pt::trav4_begin(sc, t, p.ro, p.rd, rec.t, st);
     any = false;
     state = kLaneTrace;
@@ -365,11 +369,9 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
       const uint32_t avail = slots_in_job - handed;
       if (((idle >> lane) & 1u) && rank < avail) {
         const uint32_t s = handed + rank;
-        x = x0 + s % tw;
-        y = y0 + s / tw;
-        idx = y * f.width + x;
+        idx = (y0 + s / tw) * f.width + x0 + s % tw;
         k = 0;
-        begin_sample(f, x, y, idx, seed0, rng, p);
+        begin_sample(f, idx % f.width, idx / f.width, idx, seed0, rng, p);
         start_closest();
       }
       handed += min(uint32_t(__popcll(idle)), avail);
@@ -383,13 +385,6 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
           if (!blocked) p.radiance = p.radiance + pt::mk(slots[3 * which + 2]);
           next_ray();
         } else {
-          if (t.best >= 0) {
-            rec.t = t.tmax;
-            rec.kind = pt::kHitTri;
-            rec.prim = uint32_t(t.best);
-            rec.u = t.bu;
-            rec.v = t.bv;
-          }
           state = kLaneShade;
         }
       }
@@ -397,6 +392,18 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
     const uint64_t ready = __ballot(state == kLaneShade || state == kLaneFinish);
     if (ready != 0 && (__popcll(ready) >= kShadeBatch || __ballot(state == kLaneTrace) == 0)) {
       if (state == kLaneShade) {
+        pt::HitRecord rec;
+        rec.t = t.tmax;
+        if (t.best >= 0) {
+          rec.kind = pt::kHitTri;
+          rec.prim = uint32_t(t.best);
+          rec.u = t.bu;
+          rec.v = t.bv;
+        } else {
+          rec.kind = analytic >> 30;
+          rec.prim = analytic & 0x3FFFFFFFu;
+          rec.u = rec.v = 0.0f;
+        }
         DeferTracer tr{sc, slots, rec, 0u};
         cont = pt::path_step_with(sc, f, p, rng, tr);
         pending = tr.mask;
@@ -410,7 +417,7 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
         if (++k >= it.spp) {
           state = kLaneIdle;
         } else {
-          begin_sample(f, x, y, idx, seed0 + k, rng, p);
+          begin_sample(f, idx % f.width, idx / f.width, idx, seed0 + k, rng, p);
           start_closest();
         }
       }
