@@ -17,10 +17,10 @@
 // over the opaque BSDF (bsdf.glsl:272-324) with θ = (albedo rgb, roughness,
 // metallic) of the hit material.
 //
-// Reductions: material gradients are summed per block in LDS and written as
-// per-block partials, then reduced in FP64 by a second kernel (deterministic);
-// environment-texel gradients go through float atomics into 8 replicas to
-// spread contention and are reduced into FP64 by a final kernel.
+// Reductions, FP64 throughout: material gradients are summed per block in
+// LDS and written as per-block partials, then reduced by a second kernel;
+// environment-texel gradients go through atomics into 8 replicas to spread
+// contention and are summed by a final kernel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -200,24 +200,26 @@ __device__ __forceinline__ ParamScale param_scale(const PtScene& sc, const Hit& 
   return p;
 }
 
+// Gradient sums are FP64 throughout (LDS and global double atomics): a
+// persistent launch sums every sample of the frame into them.
 struct GradSink {
-  float* lds;  // kLdsMaterials * 5 floats (block-shared) or null
-  float* env;  // kEnvReplicas replicas of env_w * env_h * 3 floats
+  double* lds;  // kLdsMaterials * 5 (block-shared)
+  double* env;  // kEnvReplicas replicas of env_w * env_h * 3
   size_t env_stride;
   uint32_t replica;
   int material_count;
-  float* global_mat;  // fallback for materials beyond the LDS table (float atomics)
+  double* global_mat;  // materials beyond the LDS table
 };
 
 __device__ __forceinline__ void add_material(GradSink& g, int mat, const float v[kNumParams]) {
   if (mat < kLdsMaterials) {
   #pragma unroll
   for (int i = 0; i < kNumParams; ++i)
-      if (v[i] != 0.0f) atomicAdd(&g.lds[mat * kNumParams + i], v[i]);
+      if (v[i] != 0.0f) atomicAdd(&g.lds[mat * kNumParams + i], double(v[i]));
   } else {
   #pragma unroll
   for (int i = 0; i < kNumParams; ++i)
-      if (v[i] != 0.0f) unsafeAtomicAdd(&g.global_mat[mat * kNumParams + i], v[i]);
+      if (v[i] != 0.0f) unsafeAtomicAdd(&g.global_mat[mat * kNumParams + i], double(v[i]));
   }
 }
 
@@ -225,14 +227,14 @@ __device__ __forceinline__ void add_env(const PtScene& sc, GradSink& g, float u,
   int idx[4];
   float w[4];
   env_taps(sc, u, v, idx, w);
-  float* base = g.env + g.replica * g.env_stride;
+  double* base = g.env + g.replica * g.env_stride;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (w[k] == 0.0f) continue;
     const float s = w[k] * sc.env_intensity;
-    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 0], weight.x * s);
-    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 1], weight.y * s);
-    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 2], weight.z * s);
+    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 0], double(weight.x * s));
+    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 1], double(weight.y * s));
+    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 2], double(weight.z * s));
   }
 }
 
@@ -532,21 +534,24 @@ __global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, 
 }
 
 // Persistent replay over all samples: waves take 8x8 tile jobs from an
-// atomic queue; each lane replays one pixel's samples one bounce per loop
-// turn (sample k: seed seed0 + k, radiance recorded_L[k]) and, after the
-// pixel's last sample, takes the next free pixel of the wave's jobs, so
-// lanes do not idle behind the longest path of the wave.  Material
-// gradients gather in the block's LDS table over everything the block
-// replays and are written once as the block's partials.
+// atomic queue and hand out the job's pixel samples to their lanes, one path
+// per lane at a time, one bounce per loop turn (the job's pixels at sample
+// 0, then at sample 1, ...; sample k: seed seed0 + k, radiance
+// recorded_L[k]).  A lane whose path ends takes the next sample of the job
+// or of the next job, so lanes neither idle behind the longest path of the
+// wave nor behind the longest pixel at the end of the queue (at 512x512 a
+// lane sees only about one pixel).  Material gradients gather in the block's
+// LDS table over everything the block replays and are written once as the
+// block's partials.
 // kWaves: the occupancy the register allocation targets (waves per SIMD).
 template <int kWaves>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_adjoint(
     PtScene sc, PtFrame f, uint32_t spp, uint32_t n_jobs, uint32_t* __restrict__ counter, const float* __restrict__ dl_di,
-    const float4* __restrict__ recorded_L, float inv_spp, float* __restrict__ mat_partials, float* __restrict__ env_acc,
-    size_t env_stride, float* __restrict__ mat_overflow, uint32_t* spill, uint32_t spill_depth, uint32_t tiles_x) {
+    const float4* __restrict__ recorded_L, float inv_spp, double* __restrict__ mat_partials, double* __restrict__ env_acc,
+    size_t env_stride, double* __restrict__ mat_overflow, uint32_t* spill, uint32_t spill_depth, uint32_t tiles_x) {
   __shared__ uint32_t lds[kPtStackWords * kAdjBlock];
-  __shared__ float lds_grad[kLdsMaterials * kNumParams];
-  for (int i = threadIdx.x; i < kLdsMaterials * kNumParams; i += kAdjBlock) lds_grad[i] = 0.0f;
+  __shared__ double lds_grad[kLdsMaterials * kNumParams];
+  for (int i = threadIdx.x; i < kLdsMaterials * kNumParams; i += kAdjBlock) lds_grad[i] = 0.0;
   __syncthreads();
   PtStack<kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0, 0};
   GradSink g{lds_grad, env_acc, env_stride, blockIdx.x % kEnvReplicas, int(sc.material_count), mat_overflow};
@@ -558,51 +563,27 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWave
     return __builtin_amdgcn_readlane(v, 0);
   };
   uint32_t job = take();
-  uint32_t x0 = 0, y0 = 0, tw = 1, slots = 0, handed = 0;
+  // The job being handed out (wave-uniform): tile corner, width, pixels,
+  // and the next of its px * spp samples.
+  uint32_t x0 = 0, y0 = 0, tw = 1, px = 0, units = 0, handed = 0;
   auto open_job = [&](uint32_t j) {
     x0 = (j % tiles_x) * kTile;
     y0 = (j / tiles_x) * kTile;
     tw = min(uint32_t(kTile), f.width - x0);
-    slots = tw * min(uint32_t(kTile), f.height - y0);
+    px = tw * min(uint32_t(kTile), f.height - y0);
+    units = px * spp;
     handed = 0;
   };
   bool taking = job < n_jobs;
   if (taking) open_job(job);
   bool active = false;
-  uint32_t x = 0, y = 0, idx = 0, k = 0;
   v3 dl = mk(0.f, 0.f, 0.f);
   Rng rng{};
   ReplayState r;
-  // Starts the pixel's sample k or a later one (a sample whose radiance is
-  // not finite contributes no gradient, as the film's accumulation drops
-  // it); inactive after the last.
-  auto start_sample = [&]() {
-    for (; k < spp; ++k) {
-      rng = seed_rng(idx, f.seed + k);
-      v3 ro, rd;
-      camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-      const Rng replay_rng = rng;
-      v3 L;
-      if (recorded_L) {  // the forward pass kept this path's radiance
-        const float4 rr = recorded_L[size_t(k) * frame_px + idx];
-        L = mk(rr.x, rr.y, rr.z);
-      } else {
-        Guide gd;
-        L = trace_path(sc, f, ro, rd, rng, st, gd);
-      }
-      if (!(isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))) {
-        rng = replay_rng;
-        replay_begin(r, ro, rd, L);
-        active = true;
-        return;
-      }
-    }
-    active = false;
-  };
   for (;;) {
     uint64_t idle = __ballot(!active);
     while (taking && idle != 0) {
-      if (handed == slots) {
+      if (handed == units) {
         const uint32_t j = take();
         if (j >= n_jobs) {
           taking = false;
@@ -612,25 +593,41 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWave
         continue;
       }
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
-      const uint32_t avail = slots - handed;
+      const uint32_t avail = units - handed;
       if (((idle >> lane) & 1u) && rank < avail) {
-        const uint32_t s = handed + rank;
-        x = x0 + s % tw;
-        y = y0 + s / tw;
-        idx = y * f.width + x;
+        const uint32_t u = handed + rank;
+        const uint32_t slot = u % px, k = u / px;
+        const uint32_t x = x0 + slot % tw, y = y0 + slot / tw;
+        const uint32_t idx = y * f.width + x;
         const float4 d = reinterpret_cast<const float4*>(dl_di)[idx];
         dl = mk(d.x * inv_spp, d.y * inv_spp, d.z * inv_spp);
-        k = 0;
-        if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) start_sample();
+        // A pixel without loss gradient, or a sample whose radiance is not
+        // finite (the film's accumulation drops it), needs no replay: the
+        // lane asks again.
+        if (dl.x != 0.0f || dl.y != 0.0f || dl.z != 0.0f) {
+          rng = seed_rng(idx, f.seed + k);
+          v3 ro, rd;
+          camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
+          const Rng replay_rng = rng;
+          v3 L;
+          if (recorded_L) {  // the forward pass kept this path's radiance
+            const float4 rr = recorded_L[size_t(k) * frame_px + idx];
+            L = mk(rr.x, rr.y, rr.z);
+          } else {
+            Guide gd;
+            L = trace_path(sc, f, ro, rd, rng, st, gd);
+          }
+          if (!(isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z))) {
+            rng = replay_rng;
+            replay_begin(r, ro, rd, L);
+            active = true;
+          }
+        }
       }
       handed += min(uint32_t(__popcll(idle)), avail);
-      // Lanes whose pixel has no loss gradient (or no finite sample) ask again.
       idle = __ballot(!active);
     }
-    if (active && !replay_step(sc, f, r, rng, st, dl, g)) {
-      ++k;
-      start_sample();
-    }
+    if (active && !replay_step(sc, f, r, rng, st, dl, g)) active = false;
     if (!taking && __ballot(active) == 0) break;
   }
   __syncthreads();
@@ -640,14 +637,14 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWave
 }
 
 // FP64 reductions: per-block material partials, then the env replicas.
-__global__ void k_reduce_materials(const float* __restrict__ partials, uint32_t blocks, uint32_t entries,
-                                   const float* __restrict__ overflow, uint32_t overflow_entries,
+__global__ void k_reduce_materials(const double* __restrict__ partials, uint32_t blocks, uint32_t entries,
+                                   const double* __restrict__ overflow, uint32_t overflow_entries,
                                    double* __restrict__ out) {
   const uint32_t e = blockIdx.x;
   __shared__ double part[256];
   double acc = 0.0;
   if (e < entries)
-    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) acc += double(partials[size_t(b) * kLdsMaterials * kNumParams + e]);
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) acc += partials[size_t(b) * kLdsMaterials * kNumParams + e];
   part[threadIdx.x] = acc;
   __syncthreads();
   for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -656,22 +653,22 @@ __global__ void k_reduce_materials(const float* __restrict__ partials, uint32_t 
   }
   if (threadIdx.x == 0) {
     double v = e < entries ? part[0] : 0.0;
-    if (e < overflow_entries) v += double(overflow[e]);
+    if (e < overflow_entries) v += overflow[e];
     out[e] += v;
   }
 }
 
-__global__ void k_reduce_env(const float* __restrict__ acc, size_t stride, size_t n, double* __restrict__ out) {
+__global__ void k_reduce_env(const double* __restrict__ acc, size_t stride, size_t n, double* __restrict__ out) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int r = 0; r < kEnvReplicas; ++r) s += double(acc[r * stride + i]);
+  for (int r = 0; r < kEnvReplicas; ++r) s += acc[r * stride + i];
   out[i] += s;
 }
 
-__global__ void k_zero(float* p, size_t n) {
+__global__ void k_zero(double* p, size_t n) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0f;
+  if (i < n) p[i] = 0.0;
 }
 
 }  // namespace
@@ -681,7 +678,7 @@ size_t adjoint_scratch_bytes(uint32_t width, uint32_t height, uint32_t material_
   const size_t partials = blocks * kLdsMaterials * kNumParams;
   const size_t overflow = size_t(material_count) * kNumParams;
   const size_t env = size_t(kEnvReplicas) * env_texels * 3;
-  return (partials + overflow + env) * sizeof(float) + sizeof(uint32_t) + 1024;  // + the replay's job counter
+  return (partials + overflow + env + 1) * sizeof(double) + 1024;  // + the replay's job counter
 }
 
 Status launch_detached(Context* ctx, const DetachedArgs& a, void* stream_) {
@@ -718,15 +715,15 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
   const size_t overflow = size_t(sc.material_count) * kNumParams;
   const size_t env_stride = env_texels * 3;
   // Scratch: [block partials | overflow | env replicas | job counter] (adjoint_scratch_bytes).
-  float* base = static_cast<float*>(a.scratch);
-  float* mat_partials = base;
-  float* mat_overflow = base + partials;
-  float* env_acc = mat_overflow + overflow;
+  double* base = static_cast<double*>(a.scratch);
+  double* mat_partials = base;
+  double* mat_overflow = base + partials;
+  double* env_acc = mat_overflow + overflow;
   uint32_t* counter = reinterpret_cast<uint32_t*>(env_acc + kEnvReplicas * env_stride);
   const uint32_t depth = pt_spill_words(sc.bvh);
   Status s = ensure_spill(ctx, size_t(grid) * kAdjBlock * depth * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
-  // Overflow, env replicas and the counter (0.0f has the bits of 0u).
+  // Overflow, env replicas and the counter (0.0 has the bits of 0u).
   const size_t zero_n = overflow + kEnvReplicas * env_stride + 1;
   hipLaunchKernelGGL(k_zero, dim3(uint32_t((zero_n + 255) / 256)), dim3(256), 0, stream, mat_overflow, zero_n);
   PtFrame f = a.frame;

@@ -22,6 +22,15 @@ from .scene import Scene, pack_materials
 PARAM_NAMES = ("albedo_r", "albedo_g", "albedo_b", "roughness", "metallic")
 
 
+class _DeviceArray:
+    """A device buffer described by __cuda_array_interface__, so that torch
+    can view it without a copy (torch.as_tensor(..., device="cuda"))."""
+
+    def __init__(self, ptr: int, shape: tuple, typestr: str = "<f4"):
+        self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (ptr, False), "version": 2,
+                                         "strides": None}
+
+
 class DiffRenderer:
     def __init__(self, ctx: Context, scene: Scene, width: int, height: int, max_bounces: int = 6):
         self.ctx = ctx
@@ -55,7 +64,7 @@ class DiffRenderer:
         self.env = arr.copy()
         self._version += 1
 
-    def forward(self, spp: int, seed0: int = 0, record: bool = True) -> np.ndarray:
+    def forward(self, spp: int, seed0: int = 0, record: bool = True, device: bool = False):
         """Image (H, W, 3) as float64, rows bottom-up (lucent_forward).  With
         `record`, each path's radiance is kept on the device so that a
         backward() with the same parameters and seeds need not re-trace."""
@@ -72,7 +81,18 @@ class DiffRenderer:
                                                    None))
         self.ctx.synchronize()
         self._record = (spp, seed0, self._version, buf) if record else None
+        if device:
+            return self.image_view()
         return r.film.read()["color"][..., :3].astype(np.float64)
+
+    def image_view(self):
+        """The film's colour target as a torch float32 (H, W, 4) view on the
+        device (rgb, luminance²), valid until the next trace: forward(...,
+        device=True) returns it so that a loss and its gradient can stay in
+        device memory."""
+        import torch
+        ptr = self.renderer.film.device_ptrs()["color"]
+        return torch.as_tensor(_DeviceArray(ptr, (self.height, self.width, 4)), device=torch.device("cuda:0"))
 
     def detached_forward(self, value_materials, spp: int, seed0: int = 0) -> np.ndarray:
         """Image (H, W, 3) of the detached estimator (lucent_detached_forward):
@@ -93,16 +113,25 @@ class DiffRenderer:
         self.ctx.synchronize()
         return out.cpu().numpy().reshape(h, w, 4)[..., :3].astype(np.float64)
 
-    def backward(self, dl_di: np.ndarray, spp: int, seed0: int = 0, env: bool = True, use_record: bool = True):
+    def backward(self, dl_di, spp: int, seed0: int = 0, env: bool = True, use_record: bool = True,
+                 device: bool = False):
         """Gradients (material_count x 5, env (h, w, 3)) of sum dl_di · image.
-        Uses the radiance recorded by the last forward() when it was made with
-        the same samples, seeds and parameters; otherwise re-traces."""
+        dl_di: (h, w, 3) host array, or a torch float32 (h, w, 4) tensor on the
+        device (its fourth channel is ignored).  Uses the radiance recorded by
+        the last forward() when it was made with the same samples, seeds and
+        parameters; otherwise re-traces.  device=True returns the float64
+        gradients as device tensors instead of host arrays."""
         import torch
         dev = torch.device("cuda:0")
         h, w = self.height, self.width
-        d = np.zeros((h, w, 4), np.float32)
-        d[..., :3] = dl_di
-        d_dl = torch.from_numpy(d).to(dev)
+        if isinstance(dl_di, torch.Tensor):
+            if dl_di.device.type != "cuda" or dl_di.dtype != torch.float32 or tuple(dl_di.shape) != (h, w, 4):
+                raise ValueError(f"device dl_di must be a float32 ({h}, {w}, 4) CUDA tensor")
+            d_dl = dl_di.contiguous()
+        else:
+            d = np.zeros((h, w, 4), np.float32)
+            d[..., :3] = dl_di
+            d_dl = torch.from_numpy(d).to(dev)
         nm = len(self.materials)
         g_mat = torch.zeros(nm * 5, dtype=torch.float64, device=dev)
         ew, eh = self.env_size
@@ -117,6 +146,8 @@ class DiffRenderer:
             d_dl.data_ptr(), rec[3].data_ptr() if use else None, g_mat.data_ptr(), g_env.data_ptr() if env else None,
             None))
         self.ctx.synchronize()
+        if device:
+            return g_mat.reshape(nm, 5), (g_env.reshape(eh, ew, 3) if env else None)
         gm = g_mat.cpu().numpy().reshape(nm, 5)
         ge = g_env.cpu().numpy().reshape(eh, ew, 3) if env else None
         return gm, ge

@@ -232,3 +232,27 @@ def test_recorded_forward_spares_the_retrace(diff):
     r.reset()
     r.trace_samples(SPP)
     assert np.array_equal(r.film.read()["color"][..., :3].astype(np.float64), img)
+
+
+def test_device_resident_gradient_evaluation(diff):
+    """forward(device=True) views the film's colour target on the device and
+    backward takes a device dL/dI and returns device gradients: the same
+    image bit for bit and the same gradients (up to float32 atomic-order
+    noise) as the host-array path."""
+    import torch
+    rng = np.random.default_rng(11)
+    dl = rng.standard_normal((H, W, 3))
+    img = diff.forward(SPP, 0)
+    g_host, e_host = diff.backward(dl, SPP, 0)
+    img_dev = diff.forward(SPP, 0, device=True)
+    assert tuple(img_dev.shape) == (H, W, 4) and img_dev.is_cuda
+    assert np.array_equal(img_dev[..., :3].cpu().numpy().astype(np.float64), img)
+    d_dl = torch.zeros((H, W, 4), dtype=torch.float32, device=img_dev.device)
+    d_dl[..., :3] = torch.from_numpy(dl.astype(np.float32)).to(img_dev.device)
+    g_dev, e_dev = diff.backward(d_dl, SPP, 0, device=True)
+    assert g_dev.is_cuda and e_dev.is_cuda
+    g, e = g_dev.cpu().numpy(), e_dev.cpu().numpy()
+    assert np.allclose(g, g_host, rtol=1e-5, atol=1e-6 * np.abs(g_host).max())
+    assert np.allclose(e, e_host, rtol=1e-4, atol=1e-6 * np.abs(e_host).max())
+    with pytest.raises(ValueError, match="float32"):
+        diff.backward(d_dl.double(), SPP, 0)
