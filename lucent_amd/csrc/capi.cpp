@@ -83,6 +83,7 @@ Status ensure_spill(Context* ctx, size_t bytes) { return grow(&ctx->spill, &ctx-
 Status ensure_wavefront(Context* ctx, size_t bytes) {
   return grow(&ctx->wavefront, &ctx->wavefront_bytes, bytes, ctx->stream);
 }
+Status ensure_units(Context* ctx, size_t bytes) { return grow(&ctx->units, &ctx->units_bytes, bytes, ctx->stream); }
 Status ensure_tile_table(Context* ctx, size_t bytes) {
   return grow(&ctx->tile_table, &ctx->tile_table_bytes, bytes, ctx->stream);
 }
@@ -263,6 +264,7 @@ void lucent_destroy(lucent_ctx* ctx) {
   if (ctx->d_flags) hipFree(ctx->d_flags);
   if (ctx->tile_table) hipFree(ctx->tile_table);
   if (ctx->wavefront) hipFree(ctx->wavefront);
+  if (ctx->units) hipFree(ctx->units);
   if (ctx->tile_event) hipEventDestroy(static_cast<hipEvent_t>(ctx->tile_event));
   hipStreamDestroy(static_cast<hipStream_t>(ctx->stream));
   delete ctx;

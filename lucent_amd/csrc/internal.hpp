@@ -77,6 +77,9 @@ struct Context {
   // Wavefront path tracer: path state and queues.
   void* wavefront = nullptr;
   size_t wavefront_bytes = 0;
+  // Per-sample tile launches: path records and per-pixel counters.
+  void* units = nullptr;
+  size_t units_bytes = 0;
   uint32_t* d_flags = nullptr;  // device status word(s)
   float* sheen_lut = nullptr;   // 16x16 half-rounded sheen albedo table
   // Device tree built at scene upload (binned SAH over the caller's
@@ -109,6 +112,7 @@ Status ensure_scratch(Context* ctx, size_t bytes);
 Status ensure_spill(Context* ctx, size_t bytes);
 Status ensure_tile_table(Context* ctx, size_t bytes);
 Status ensure_wavefront(Context* ctx, size_t bytes);
+Status ensure_units(Context* ctx, size_t bytes);
 
 // Kernel launchers (traverse.hip).
 struct RayQuery {

@@ -426,6 +426,189 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
   }
 }
 
+// A finished path as the film's accumulation step takes it: three float4
+// (colour, depth), (albedo, coverage), (normal, 0).
+constexpr uint32_t kRecordFloat4 = 3;
+
+// run_item_fused for one film with the work handed out per pixel SAMPLE
+// (the job's pixels at sample 0, then at sample 1, ...) instead of per
+// pixel, so that at the end of the queue lanes do not idle behind the last
+// pixels' remaining samples (a lane sees only a few pixels of a 1440x900
+// film, one of a 512x512 one).  A finished path is stored as a record
+// (slot job * 64 + pixel, sample k) and counted; the lane that brings a
+// pixel's count to spp folds the pixel's records into the film in sample
+// order, with the accumulation step of the other structures, so the film is
+// bit-identical.  All samples of a job are handed out by the wave that took
+// it, so records and counters are shared within one wave only
+// (workgroup-scope ordering).
+template <class StackT>
+__device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job, uint32_t n_jobs, uint32_t* counter,
+                                                StackT& st, float4* slots, float4* __restrict__ records,
+                                                uint32_t* __restrict__ arrived) {
+  const PtFrame& f = it.frame;
+  const PtScene& sc = it.scene;
+  const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
+  const uint32_t spp = it.spp;
+  const uint32_t lane = threadIdx.x;
+  // The job being handed out (wave-uniform): tile corner, width, pixels,
+  // and the next of its px * spp samples.
+  uint32_t cur = 0, x0 = 0, y0 = 0, tw = 1, px = 0, units = 0, handed = 0;
+  auto open_job = [&](uint32_t j) {
+    cur = j;
+    x0 = (j % it.tiles_x) * kTile;
+    y0 = (j / it.tiles_x) * kTile;
+    tw = min(uint32_t(kTile), f.width - x0);
+    px = tw * min(uint32_t(kTile), f.height - y0);
+    units = px * spp;
+    handed = 0;
+  };
+  open_job(job);
+  bool taking = true;
+  uint32_t state = kLaneIdle;
+  uint32_t idx = 0, k = 0, rslot = 0;
+  pt::Rng rng{};
+  pt::PathState p;
+  pt::Trav4 t;
+  uint32_t analytic = 0;
+  bool any = false, cont = false;
+  uint32_t which = 0, pending = 0;
+  auto start_closest = [&]() {
+    pt::HitRecord rec;
+    pt::search_analytic(sc, p.ro, p.rd, rec);
+    analytic = rec.kind << 30 | rec.prim;
+    pt::trav4_begin(sc, t, p.ro, p.rd, rec.t, st);
+    any = false;
+    state = kLaneTrace;
+  };
+  auto next_ray = [&]() {
+    while (pending != 0) {
+      which = uint32_t(__builtin_ctz(pending));
+      pending &= pending - 1u;
+      const float4 o = slots[3 * which], d = slots[3 * which + 1];
+      const pt::v3 ro = pt::mk(o), rd = pt::mk(d);
+      bool blocked = false;
+      for (uint32_t i = 0; i < sc.sphere_count && !blocked; ++i)
+        blocked = pt::hit_sphere(ro, rd, reinterpret_cast<const float4*>(sc.spheres)[i], o.w) < o.w;
+      if (!blocked) blocked = pt::hit_ground(sc, ro, rd, o.w) < o.w;
+      if (!blocked) {
+        pt::trav4_begin(sc, t, ro, rd, o.w, st);
+        any = true;
+        state = kLaneTrace;
+        return;
+      }
+    }
+    if (cont)
+      start_closest();
+    else
+      state = kLaneFinish;
+  };
+  for (;;) {
+    uint64_t idle = __ballot(state == kLaneIdle);
+    while (taking && idle != 0) {
+      if (handed == units) {
+        const uint32_t j = take_job(counter);
+        if (j >= n_jobs) {
+          taking = false;
+          break;
+        }
+        open_job(j);
+        continue;
+      }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
+      const uint32_t avail = units - handed;
+      if (((idle >> lane) & 1u) && rank < avail) {
+        const uint32_t u = handed + rank;
+        const uint32_t slot = u % px;
+        k = u / px;
+        rslot = cur * kPtBlock + slot;
+        idx = (y0 + slot / tw) * f.width + x0 + slot % tw;
+        begin_sample(f, idx % f.width, idx / f.width, idx, seed0 + k, rng, p);
+        start_closest();
+      }
+      handed += min(uint32_t(__popcll(idle)), avail);
+      idle = __ballot(state == kLaneIdle);
+    }
+    if (state == kLaneTrace) {
+      bool blocked = false;
+      if (!pt::trav4_done(t, st)) blocked = pt::trav4_step_mixed(sc, t, st, rng, any);
+      if (blocked || pt::trav4_done(t, st)) {
+        if (any) {
+          if (!blocked) p.radiance = p.radiance + pt::mk(slots[3 * which + 2]);
+          next_ray();
+        } else {
+          state = kLaneShade;
+        }
+      }
+    }
+    const uint64_t ready = __ballot(state == kLaneShade || state == kLaneFinish);
+    if (ready != 0 && (__popcll(ready) >= kShadeBatch || __ballot(state == kLaneTrace) == 0)) {
+      if (state == kLaneShade) {
+        pt::HitRecord rec;
+        rec.t = t.tmax;
+        if (t.best >= 0) {
+          rec.kind = pt::kHitTri;
+          rec.prim = uint32_t(t.best);
+          rec.u = t.bu;
+          rec.v = t.bv;
+        } else {
+          rec.kind = analytic >> 30;
+          rec.prim = analytic & 0x3FFFFFFFu;
+          rec.u = rec.v = 0.0f;
+        }
+        DeferTracer tr{sc, slots, rec, 0u};
+        cont = pt::path_step_with(sc, f, p, rng, tr);
+        pending = tr.mask;
+        next_ray();
+      }
+      if (state == kLaneFinish) {
+        if (it.sample_radiance)
+          reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
+              make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
+        float4* r = records + (size_t(rslot) * spp + k) * kRecordFloat4;
+        r[0] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, p.guide.depth);
+        r[1] = make_float4(p.guide.albedo.x, p.guide.albedo.y, p.guide.albedo.z, p.guide.coverage);
+        r[2] = make_float4(p.guide.normal.x, p.guide.normal.y, p.guide.normal.z, 0.0f);
+        const uint32_t before =
+            __hip_atomic_fetch_add(arrived + rslot, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (before + 1u == spp) {
+          pt::PixelAccum a = pt::load_accum(it.film, idx);
+          const float4* q = records + size_t(rslot) * spp * kRecordFloat4;
+          for (uint32_t s = 0; s < spp; ++s, q += kRecordFloat4) {
+            const float4 c = q[0], al = q[1], n = q[2];
+            pt::Guide g = p.guide;
+            g.albedo = pt::mk(al.x, al.y, al.z);
+            g.coverage = al.w;
+            g.normal = pt::mk(n.x, n.y, n.z);
+            g.depth = c.w;
+            pt::fold_sample(a, pt::mk(c.x, c.y, c.z), g);
+          }
+          pt::store_accum(it.film, idx, a);
+        }
+        state = kLaneIdle;
+      }
+    }
+    if (!taking && __ballot(state != kLaneIdle) == 0) return;
+  }
+}
+
+// One film, work handed out per pixel sample (run_units_fused).
+template <int kWaves>
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_units(
+    TileItem single, uint32_t n_jobs, uint32_t* __restrict__ counter, uint32_t* scratch, uint32_t lane_words,
+    float4* __restrict__ records, uint32_t* __restrict__ arrived) {
+  __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
+  uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
+  float4* slots = reinterpret_cast<float4*>(area);
+  pt::PtStack<kPtBlock> st{lds + threadIdx.x, area + kShadowWords, 0, 0};
+  const uint32_t job = take_job(counter);
+  if (job < n_jobs) run_units_fused(single, job, n_jobs, counter, st, slots, records, arrived);
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // Jobs are numbered film by film: film i owns jobs [job_begin[i], job_begin[i+1]).
 // kSingle: one film, passed by value (its fields live in the kernel arguments).
 // kWaves: the occupancy the register allocation targets (waves per SIMD);
@@ -521,9 +704,72 @@ uint32_t tile_jobs(uint32_t width, uint32_t height) {
   return ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
 }
 
+// Path records of one per-sample launch at most (larger sample counts run
+// as several launches, each continuing the film).
+constexpr size_t kUnitBytes = size_t(1) << 30;
+
+Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hipStream_t stream) {
+  item.tiles_x = (item.frame.width + kTile - 1) / kTile;
+  const uint32_t n_jobs = tile_jobs(item.frame.width, item.frame.height);
+  const uint32_t waves = (cfg.occupancy >= 2 && cfg.occupancy <= 6) ? cfg.occupancy : 2;
+  const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
+  const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
+  const uint32_t lane_words = tile_lane_words(pt::pt_spill_words(item.scene.bvh));
+  Status s = ensure_spill(ctx, size_t(grid) * kPtBlock * lane_words * sizeof(uint32_t));
+  if (s != LUCENT_OK) return s;
+  const size_t slots = size_t(n_jobs) * kPtBlock;
+  const size_t per_sample = slots * kRecordFloat4 * sizeof(float4);
+  const uint32_t chunk = uint32_t(std::max<size_t>(1, std::min<size_t>(item.spp, kUnitBytes / per_sample)));
+  const size_t counters_off = (size_t(chunk) * per_sample + 255) / 256 * 256;
+  if ((s = ensure_units(ctx, counters_off + slots * sizeof(uint32_t))) != LUCENT_OK) return s;
+  if ((s = ensure_tile_table(ctx, 256)) != LUCENT_OK) return s;
+  float4* records = static_cast<float4*>(ctx->units);
+  uint32_t* arrived = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->units) + counters_off);
+  uint32_t* counter = static_cast<uint32_t*>(ctx->tile_table);
+  uint32_t* scratch = static_cast<uint32_t*>(ctx->spill);
+  const size_t frame_px = size_t(item.frame.width) * item.frame.height;
+  const uint32_t spp = item.spp;
+  for (uint32_t done = 0; done < spp; done += chunk) {
+    TileItem it = item;
+    it.spp = std::min(chunk, spp - done);
+    it.frame.seed = item.frame.seed + done;
+    if (item.sample_radiance) it.sample_radiance = item.sample_radiance + size_t(done) * frame_px * 4;
+    hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((slots + 255) / 256)), dim3(256), 0, stream, arrived, slots, 0u);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
+    switch (waves) {
+      case 3:
+        hipLaunchKernelGGL(k_trace_units<3>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+                           lane_words, records, arrived);
+        break;
+      case 4:
+        hipLaunchKernelGGL(k_trace_units<4>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+                           lane_words, records, arrived);
+        break;
+      case 5:
+        hipLaunchKernelGGL(k_trace_units<5>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+                           lane_words, records, arrived);
+        break;
+      case 6:
+        hipLaunchKernelGGL(k_trace_units<6>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+                           lane_words, records, arrived);
+        break;
+      default:
+        hipLaunchKernelGGL(k_trace_units<2>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+                           lane_words, records, arrived);
+    }
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(e, "k_trace_units launch");
+  return LUCENT_OK;
+}
+
 Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const TileLaunch& cfg, void* stream_) {
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   if (n_items == 0) return LUCENT_OK;
+  if (n_items == 1 && cfg.fused && cfg.units && !cfg.counter) {
+    if (items[0].spp == 0) return LUCENT_OK;
+    return launch_trace_units(ctx, items[0], cfg, stream);
+  }
   std::vector<uint32_t> job_begin(n_items + 1, 0);
   uint32_t depth = 1;
   for (uint32_t i = 0; i < n_items; ++i) {

@@ -41,6 +41,9 @@ struct TileLaunch {
   // traced where the integrator asks for it).  Only for scenes whose
   // traversal draws no random numbers (no alpha-blended material).
   bool fused = true;
+  // One film, fused: hand out pixel samples rather than whole pixels (the
+  // context's buffers only; launches of up to kUnitBytes of path records).
+  bool units = true;
 };
 // Spill bytes the launch of one film with `cfg` needs.
 size_t tile_spill_bytes(const Context* ctx, const TileItem& item, const TileLaunch& cfg);
