@@ -164,9 +164,10 @@ def bench_closest_hit(args) -> dict:
 def bench_path_trace(args) -> dict:
     """Progressive path trace: 1440x900, 64 spp per step, depth 8, RR from
     bounce 2, clamp 0.  The step is Renderer.traceSamples(64) after a reset,
-    run by the persistent tile kernel (one launch per step); the reference's
-    launch structure (one launch per sample) is timed beside it on the same
-    film and seeds (the two films are bit-identical)."""
+    run by the persistent per-sample tile kernel (one k_trace_units launch
+    per step); the reference's launch structure (one launch per sample) is
+    timed beside it on the same film and seeds (the two films are
+    bit-identical)."""
     import torch
 
     from lucent_amd import workloads
@@ -230,7 +231,7 @@ def bench_path_trace(args) -> dict:
     traffic = profiled_traffic("path_trace_1440x900")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_path": round(b_path - 72, 1),
-                "film_bytes_per_pixel": 72, "kernel": "k_trace_tiles", "avg_launch_ms": round(launch_s * 1e3, 3),
+                "film_bytes_per_pixel": 72, "kernel": "k_trace_units", "avg_launch_ms": round(launch_s * 1e3, 3),
                 "occupancy": tune["occupancy"]}
     cpu = None
     if not args.no_cpu_baseline:

@@ -704,9 +704,10 @@ uint32_t tile_jobs(uint32_t width, uint32_t height) {
   return ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
 }
 
-// Path records of one per-sample launch at most (larger sample counts run
-// as several launches, each continuing the film).
-constexpr size_t kUnitBytes = size_t(1) << 30;
+// Path records of one per-sample launch at most (48 B per path: a
+// 1440x900 film's 64 samples take 4.0 GB of the 288 GB; larger sample
+// counts run as several launches, each continuing the film).
+constexpr size_t kUnitBytes = size_t(9) << 29;
 
 Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hipStream_t stream) {
   item.tiles_x = (item.frame.width + kTile - 1) / kTile;
