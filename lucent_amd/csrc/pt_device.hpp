@@ -461,74 +461,86 @@ __device__ inline void trav4_begin(const PtScene& s, Trav4& t, v3 ro, v3 rd, flo
 template <class StackT>
 __device__ inline bool trav4_done(const Trav4& t, const StackT& st) { return t.na == 0 && st.empty(); }
 
-// One step; for shadow rays (kAny) true on the first accepted triangle.
-template <bool kAny, class StackT>
-__device__ inline bool trav4_step(const PtScene& s, Trav4& t, StackT& st, Rng& rng) {
-  if (t.na > 0) {
-    const uint32_t k = t.la++;
-    --t.na;
-    const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris);
-    const float4 p0 = tris[size_t(k) * 3], p1 = tris[size_t(k) * 3 + 1], p2 = tris[size_t(k) * 3 + 2];
-    float th, u, v;
-    if (hit_triangle(t.ro, t.rd, mk(p0), mk(p1), mk(p2), t.tmax, th, u, v) &&
-        (__float_as_int(p2.w) == 0 || alpha_opaque(s, k, u, v, __float_as_int(p0.w), rng))) {
-      if (kAny) return true;
-      t.tmax = th;
-      t.best = int(k);
-      t.bu = u;
-      t.bv = v;
-    }
-    return false;
-  }
-  uint32_t link;
-  float dist;
-  st.pop(link, dist);
-  if (!kAny && dist >= t.tmax) return false;
-  if (link_is_leaf(link)) {
-    t.la = link_first(link);
-    t.na = link_count(link);
-    return false;
-  }
-  float tn[4];
-  uint32_t l[4];
-#ifdef LUCENT_PT_QUANTIZED_NODES
-  node4_hits(s.bvh.nodes4, link, t.ro, t.inv, t.tmax, tn, l);
-#else
-  node4_hits_f32(s.bvh.nodes4f, link, t.ro, t.inv, t.tmax, tn, l);
-#endif
-  if (kAny) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (tn[i] != INF) st.push(l[i], tn[i]);
-    return false;
-  }
-  // Farthest first, so that the nearest child ends on top.  Entry distances
-  // are >= 0, so their bit patterns order like the floats: the keys carry
-  // the child index in the two low bits (the distance truncated by up to 3
-  // ulps, which only weakens culling) and sort with integer min/max.
-  uint32_t k[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) k[i] = (__float_as_uint(tn[i]) & ~3u) | uint32_t(i);
-  auto cx = [&](int a, int b) {
-    const uint32_t lo = min(k[a], k[b]), hi = max(k[a], k[b]);
-    k[a] = hi;
-    k[b] = lo;
-  };
-  cx(0, 1);
-  cx(2, 3);
-  cx(0, 2);
-  cx(1, 3);
-  cx(1, 2);
-  constexpr uint32_t kMissKey = 0x7149F2CAu & ~3u;  // INF's bits, truncated
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    if ((k[i] & ~3u) != kMissKey) {
-      const uint32_t j = k[i] & 3u;
-      const uint32_t lk = (j & 1u) ? ((j & 2u) ? l[3] : l[1]) : ((j & 2u) ? l[2] : l[0]);
-      st.push(lk, __uint_as_float(k[i] & ~3u));
-    }
+// Sort key of a missed child: INF's bits with the two index bits cleared.
+constexpr uint32_t kMissKey = 0x7149F2CAu & ~3u;
+
+// Tests the pending leaf's next triangle (t.na > 0); for shadow rays (any)
+// true when it is accepted, for closest rays it updates the hit.
+__device__ __forceinline__ bool test_next_triangle(const PtScene& s, Trav4& t, Rng& rng, bool any) {
+  const uint32_t k = t.la++;
+  --t.na;
+  const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris);
+  const float4 p0 = tris[size_t(k) * 3], p1 = tris[size_t(k) * 3 + 1], p2 = tris[size_t(k) * 3 + 2];
+  float th, u, v;
+  if (hit_triangle(t.ro, t.rd, mk(p0), mk(p1), mk(p2), t.tmax, th, u, v) &&
+      (__float_as_int(p2.w) == 0 || alpha_opaque(s, k, u, v, __float_as_int(p0.w), rng))) {
+    if (any) return true;
+    t.tmax = th;
+    t.best = int(k);
+    t.bu = u;
+    t.bv = v;
   }
   return false;
+}
+
+// One step: with no leaf pending, pop an entry and visit it (a node's
+// children are pushed; a leaf's first triangle is tested in the same step),
+// else test the pending leaf's next triangle.  For shadow rays (kAny) true
+// on the first accepted triangle.
+template <bool kAny, class StackT>
+__device__ inline bool trav4_step(const PtScene& s, Trav4& t, StackT& st, Rng& rng) {
+  if (t.na == 0) {
+    uint32_t link;
+    float dist;
+    st.pop(link, dist);
+    if (!kAny && dist >= t.tmax) return false;
+    if (!link_is_leaf(link)) {
+      float tn[4];
+      uint32_t l[4];
+#ifdef LUCENT_PT_QUANTIZED_NODES
+      node4_hits(s.bvh.nodes4, link, t.ro, t.inv, t.tmax, tn, l);
+#else
+      node4_hits_f32(s.bvh.nodes4f, link, t.ro, t.inv, t.tmax, tn, l);
+#endif
+      if (kAny) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (tn[i] != INF) st.push(l[i], tn[i]);
+        return false;
+      }
+      // Farthest first, so that the nearest child ends on top.  Entry
+      // distances are >= 0, so their bit patterns order like the floats: the
+      // keys carry the child index in the two low bits (the distance
+      // truncated by up to 3 ulps, which only weakens culling) and sort
+      // descending with a branch-free network of integer min/max; missed
+      // children (kMissKey) sort first and are not pushed.
+      uint32_t k[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) k[i] = (__float_as_uint(tn[i]) & ~3u) | uint32_t(i);
+      auto cx = [&](int a, int b) {
+        const uint32_t lo = min(k[a], k[b]), hi = max(k[a], k[b]);
+        k[a] = hi;
+        k[b] = lo;
+      };
+      cx(0, 1);
+      cx(2, 3);
+      cx(0, 2);
+      cx(1, 3);
+      cx(1, 2);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if ((k[i] & ~3u) != kMissKey) {
+          const uint32_t j = k[i] & 3u;
+          const uint32_t lk = (j & 1u) ? ((j & 2u) ? l[3] : l[1]) : ((j & 2u) ? l[2] : l[0]);
+          st.push(lk, __uint_as_float(k[i] & ~3u));
+        }
+      }
+      return false;
+    }
+    t.la = link_first(link);
+    t.na = link_count(link);
+  }
+  return test_next_triangle(s, t, rng, kAny);
 }
 
 // trav4_step for a ray whose kind is a run-time value: one body for closest
@@ -539,67 +551,52 @@ __device__ inline bool trav4_step(const PtScene& s, Trav4& t, StackT& st, Rng& r
 // sorting network with keys that keep their index order.
 template <class StackT>
 __device__ inline bool trav4_step_mixed(const PtScene& s, Trav4& t, StackT& st, Rng& rng, bool any) {
-  if (t.na > 0) {
-    const uint32_t k = t.la++;
-    --t.na;
-    const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris);
-    const float4 p0 = tris[size_t(k) * 3], p1 = tris[size_t(k) * 3 + 1], p2 = tris[size_t(k) * 3 + 2];
-    float th, u, v;
-    if (hit_triangle(t.ro, t.rd, mk(p0), mk(p1), mk(p2), t.tmax, th, u, v) &&
-        (__float_as_int(p2.w) == 0 || alpha_opaque(s, k, u, v, __float_as_int(p0.w), rng))) {
-      if (any) return true;
-      t.tmax = th;
-      t.best = int(k);
-      t.bu = u;
-      t.bv = v;
+  if (t.na == 0) {
+    uint32_t link;
+    float dist;
+    st.pop(link, dist);
+    if (!any && dist >= t.tmax) return false;
+    if (!link_is_leaf(link)) {
+      float tn[4];
+      uint32_t l[4];
+#ifdef LUCENT_PT_QUANTIZED_NODES
+      node4_hits(s.bvh.nodes4, link, t.ro, t.inv, t.tmax, tn, l);
+#else
+      node4_hits_f32(s.bvh.nodes4f, link, t.ro, t.inv, t.tmax, tn, l);
+#endif
+      // Closest rays: the entry distance's bits with the child index in the
+      // two low bits (as trav4_step).  Shadow rays: (3 - i) above the index,
+      // so the descending sort leaves them in index order; missed children
+      // sort first with kMissKey either way and are not pushed.
+      uint32_t k[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        k[i] = any ? (tn[i] != INF ? ((3u - uint32_t(i)) << 2) | uint32_t(i) : kMissKey | uint32_t(i))
+                   : (__float_as_uint(tn[i]) & ~3u) | uint32_t(i);
+      auto cx = [&](int a, int b) {
+        const uint32_t lo = min(k[a], k[b]), hi = max(k[a], k[b]);
+        k[a] = hi;
+        k[b] = lo;
+      };
+      cx(0, 1);
+      cx(2, 3);
+      cx(0, 2);
+      cx(1, 3);
+      cx(1, 2);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if ((k[i] & ~3u) != kMissKey) {
+          const uint32_t j = k[i] & 3u;
+          const uint32_t lk = (j & 1u) ? ((j & 2u) ? l[3] : l[1]) : ((j & 2u) ? l[2] : l[0]);
+          st.push(lk, __uint_as_float(k[i] & ~3u));
+        }
+      }
+      return false;
     }
-    return false;
-  }
-  uint32_t link;
-  float dist;
-  st.pop(link, dist);
-  if (!any && dist >= t.tmax) return false;
-  if (link_is_leaf(link)) {
     t.la = link_first(link);
     t.na = link_count(link);
-    return false;
   }
-  float tn[4];
-  uint32_t l[4];
-#ifdef LUCENT_PT_QUANTIZED_NODES
-  node4_hits(s.bvh.nodes4, link, t.ro, t.inv, t.tmax, tn, l);
-#else
-  node4_hits_f32(s.bvh.nodes4f, link, t.ro, t.inv, t.tmax, tn, l);
-#endif
-  constexpr uint32_t kMissKey = 0x7149F2CAu & ~3u;  // INF's bits, truncated
-  // Closest rays: the entry distance's bits with the child index in the two
-  // low bits (see trav4_step).  Shadow rays: (3 - i) above the index, so the
-  // descending sort leaves them in index order; missed children sort first
-  // with kMissKey either way and are not pushed.
-  uint32_t k[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    k[i] = any ? (tn[i] != INF ? ((3u - uint32_t(i)) << 2) | uint32_t(i) : kMissKey | uint32_t(i))
-               : (__float_as_uint(tn[i]) & ~3u) | uint32_t(i);
-  auto cx = [&](int a, int b) {
-    const uint32_t lo = min(k[a], k[b]), hi = max(k[a], k[b]);
-    k[a] = hi;
-    k[b] = lo;
-  };
-  cx(0, 1);
-  cx(2, 3);
-  cx(0, 2);
-  cx(1, 3);
-  cx(1, 2);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    if ((k[i] & ~3u) != kMissKey) {
-      const uint32_t j = k[i] & 3u;
-      const uint32_t lk = (j & 1u) ? ((j & 2u) ? l[3] : l[1]) : ((j & 2u) ? l[2] : l[0]);
-      st.push(lk, __uint_as_float(k[i] & ~3u));
-    }
-  }
-  return false;
+  return test_next_triangle(s, t, rng, any);
 }
 
 // traceScene's triangle part (scene.glsl:321-376): the closest accepted hit

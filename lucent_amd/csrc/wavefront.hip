@@ -231,7 +231,8 @@ __global__ __launch_bounds__(kWfBlock) void k_wf_trace(TileItem it, WfBuffers B)
       continue;
     }
     if (active) {
-      const bool blocked = trav4_step<kShadow>(sc, t, st, rng);
+      // A scene without triangles starts its rays done: nothing to pop.
+      const bool blocked = !trav4_done(t, st) && trav4_step<kShadow>(sc, t, st, rng);
       if (kShadow) {
         if (blocked || trav4_done(t, st)) {
           if (!blocked) B.nee[item].w = 1.0f;

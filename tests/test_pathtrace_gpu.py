@@ -316,6 +316,23 @@ def test_fused_traversal_loop_bit_identical_to_inline_shadows(ctx, which, occupa
         assert np.array_equal(a[k], c[k]), k
 
 
+def test_scene_without_triangles_in_every_structure(ctx):
+    """Spheres and the ground only: rays start with an empty traversal stack
+    (nothing to pop) in the per-sample, per-sample-hand-out, tile and
+    wavefront structures, which all give the same film."""
+    sc = S.Scene(camera=S.Camera(position=(0.0, 1.5, 6.0), target=(0.0, 1.0, 0.0), fov=45.0, aperture=0.0,
+                                 focus_distance=6.0),
+                 meshes=[], spheres=[S.Sphere((0.0, 1.0, 0.0), 1.0, S.matte((0.7, 0.3, 0.2), 0.4)),
+                                     S.Sphere((1.8, 0.6, 0.5), 0.6, S.light((1.0, 0.9, 0.8), 6.0))],
+                 ground=S.Ground(True, 0.0, S.matte((0.5, 0.55, 0.6), 0.9)))
+    a, ia = _film_after(ctx, sc, 40, 24, 4, 3, RenderSettings(8, 0.0, per_sample_launch=True))
+    for st in (RenderSettings(8, 0.0), RenderSettings(8, 0.0, inline_shadows=True), RenderSettings(8, 0.0, wavefront=True)):
+        b, ib = _film_after(ctx, sc, 40, 24, 4, 3, st)
+        assert ia == ib
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+
+
 @pytest.mark.parametrize("clamp", [0.0, 8.0])
 def test_wavefront_bit_identical_to_per_sample_launches(ctx, clamp):
     """The wavefront structure (separate closest-hit, shadow and shading
