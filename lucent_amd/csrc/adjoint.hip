@@ -524,7 +524,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, 
   Rng rng = seed_rng(idx, f.seed);
   v3 ro, rd;
   camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-  PtStack<kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0,
+  PtStack<kAdjBlock> st{as_lds(lds + threadIdx.x), spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0,
                                0};
   v3 L = detached_path(sc, sv, f, ro, rd, rng, st);
   if (isnan(L.x) || isnan(L.y) || isnan(L.z) || isinf(L.x) || isinf(L.y) || isinf(L.z)) L = mk(0.f, 0.f, 0.f);
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWave
   __shared__ double lds_grad[kLdsMaterials * kNumParams];
   for (int i = threadIdx.x; i < kLdsMaterials * kNumParams; i += kAdjBlock) lds_grad[i] = 0.0;
   __syncthreads();
-  PtStack<kAdjBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0, 0};
+  PtStack<kAdjBlock> st{as_lds(lds + threadIdx.x), spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0, 0};
   GradSink g{lds_grad, env_acc, env_stride, blockIdx.x % kEnvReplicas, int(sc.material_count), mat_overflow};
   const uint32_t lane = threadIdx.x;
   const size_t frame_px = size_t(f.width) * f.height;

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kPtBlock) void k_trace_sample(PtScene sc, PtFrame f
   pt::Rng rng = pt::seed_rng(idx, f.seed);
   pt::v3 ro, rd;
   pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-  pt::PtStack<kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
+  pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
                                  0, 0};
   pt::Guide g;
   const pt::v3 c = pt::trace_path(sc, f, ro, rd, rng, st, g);
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kPtBlock) void k_trace_reprojected(PtScene sc, PtFr
   pt::Rng rng = pt::seed_rng(idx, f.seed);
   pt::v3 ro, rd;
   pt::camera_ray(f, float(x) + 0.5f, float(y) + 0.5f, rng, ro, rd);
-  pt::PtStack<kPtBlock> st{lds + threadIdx.x, spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
+  pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), spill + size_t(blockIdx.x * kPtBlock + threadIdx.x) * spill_depth,
                                  0, 0};
   pt::PrimaryTracer<pt::PtStack<kPtBlock>> tr{{sc, st}};
   pt::PathState p;
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
   __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
   uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
   float4* slots = reinterpret_cast<float4*>(area);
-  pt::PtStack<kPtBlock> st{lds + threadIdx.x, area + kShadowWords, 0, 0};
+  pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
   const uint32_t job = take_job(counter);
   if (job < n_jobs) run_units_fused(single, job, n_jobs, counter, st, slots, records, arrived);
 }
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
   __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
   uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
   float4* slots = reinterpret_cast<float4*>(area);
-  pt::PtStack<kPtBlock> st{lds + threadIdx.x, area + kShadowWords, 0, 0};
+  pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
   uint32_t job = take_job(counter);
   while (job < n_jobs) {
     if constexpr (kSingle) {

@@ -335,9 +335,16 @@ __device__ inline uint32_t link_count(uint32_t l) { return ((l >> 26) & 31u) + 1
 // area.  Entries [base, sp) are in the ring at slots top-1, ..., bot
 // (mod kLds); [0, base) are spilled.  The slot indices are kept
 // incrementally (no division by kLds).
+// A pointer into LDS typed as such: every stack access compiles to a ds_
+// instruction.  (Through a generic pointer the compiler merges the LDS read
+// and the rare spill read of a pop into one flat load, which waits on both
+// the LDS and the vector-memory counters.)
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ inline lds_u32* as_lds(uint32_t* p) { return (lds_u32*)p; }
+
 template <int kLds, int kBlock>
 struct Stack4 {
-  uint32_t* lds;    // this lane's word of plane 0; plane 1 starts kLds * kBlock words later
+  lds_u32* lds;     // this lane's word of plane 0; plane 1 starts kLds * kBlock words later
   uint32_t* spill;  // 2 words per entry
   uint32_t sp, base;
   uint32_t top, bot;  // sp % kLds, base % kLds

@@ -45,9 +45,13 @@ enum class Prec { kExact, kGlsl };
 
 // (link, entry distance) pairs; kLds entries stay in LDS as a ring (two
 // lane-major planes), older ones spill to 2 words per entry.
+// LDS pointer typed as such, so that stack accesses are ds_ instructions
+// (not flat loads merging the LDS and spill reads of a pop).
+typedef __attribute__((address_space(3))) uint32_t lds_word;
+
 template <int kLds>
 struct LaneStack {
-  uint32_t* lds;    // this lane's column of plane 0: entry k at lds[k * kBlock]; plane 1 kLds * kBlock later
+  lds_word* lds;    // this lane's column of plane 0: entry k at lds[k * kBlock]; plane 1 kLds * kBlock later
   uint32_t* spill;  // this lane's global area
   uint32_t sp = 0, base = 0;
 
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_intersect(DevBvh bvh, const void* __
     const double lower = kPrec == Prec::kExact ? tmin : double(kGlslEps);
     Best best{tmax, 0.0, 0.0, 0xFFFFFFFFu, 0xFFFFFFFFu};
 
-    LaneStack<kLds> st{lds_stack + threadIdx.x, spill + size_t(lane_global) * spill_depth};
+    LaneStack<kLds> st{(lds_word*)(lds_stack + threadIdx.x), spill + size_t(lane_global) * spill_depth};
     if (bvh.tri_count > 0) st.push(bvh.root4, 0.0f);
     while (!st.empty()) {
       uint32_t link;

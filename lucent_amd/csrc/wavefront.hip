@@ -180,7 +180,7 @@ template <bool kShadow>
 __global__ __launch_bounds__(kWfBlock) void k_wf_trace(TileItem it, WfBuffers B) {
   __shared__ uint32_t lds[kPtStackWords * kWfBlock];
   const PtScene& sc = it.scene;
-  PtStack<kWfBlock> st{lds + threadIdx.x, B.spill + size_t(blockIdx.x * kWfBlock + threadIdx.x) * B.spill_depth,
+  PtStack<kWfBlock> st{as_lds(lds + threadIdx.x), B.spill + size_t(blockIdx.x * kWfBlock + threadIdx.x) * B.spill_depth,
                              0, 0};
   const uint32_t p = __builtin_amdgcn_readfirstlane(B.ctl->parity);
   const uint32_t items = __builtin_amdgcn_readfirstlane(kShadow ? B.ctl->scount[p] : B.ctl->count[p]);
