@@ -26,7 +26,9 @@ export declare function parseHDR(buffer: ArrayBuffer | Uint8Array): { width: num
 export declare class LucentRenderer {
   constructor(opts: { width: number; height: number; device?: number });
   settings: { maxBounces: number; fireflyClamp: number; temporal: boolean; denoise: boolean; exposure: number;
-    tonemap: 'aces' | 'reinhard' | 'none' };
+    tonemap: 'aces' | 'reinhard' | 'none';
+    /* launch structure (every one gives the same film): */
+    perSampleLaunch?: boolean; wavefront?: boolean; inlineShadows?: boolean; occupancy?: number; wavesPerCU?: number };
   loadScene(packed: object, scene: object): void;
   uploadLighting(scene: object): void;
   uploadCamera(cam: Camera, moving?: boolean): void;

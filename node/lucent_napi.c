@@ -486,7 +486,9 @@ static napi_value js_decode_hdr(napi_env env, napi_callback_info info) {
 static void read_settings(napi_env env, napi_value o, lucent_render_settings *rs) {
   rs->max_bounces = (uint32_t)num(env, prop(env, o, "maxBounces"), 8);
   rs->firefly_clamp = (float)num(env, prop(env, o, "fireflyClamp"), 8);
-  rs->flags = num(env, prop(env, o, "perSampleLaunch"), 0) != 0 ? LUCENT_TRACE_PER_SAMPLE : 0u;
+  rs->flags = (num(env, prop(env, o, "perSampleLaunch"), 0) != 0 ? LUCENT_TRACE_PER_SAMPLE : 0u) |
+              (num(env, prop(env, o, "wavefront"), 0) != 0 ? LUCENT_TRACE_WAVEFRONT : 0u) |
+              (num(env, prop(env, o, "inlineShadows"), 0) != 0 ? LUCENT_TRACE_INLINE_SHADOWS : 0u);
   rs->waves_per_cu = (uint32_t)num(env, prop(env, o, "wavesPerCU"), 0);
   rs->occupancy = (uint32_t)num(env, prop(env, o, "occupancy"), 0);
 }
