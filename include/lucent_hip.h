@@ -127,6 +127,13 @@ lucent_status lucent_scene_upload(lucent_ctx *ctx, const lucent_packed_scene *pa
  * are given). */
 lucent_status lucent_set_tree_build(lucent_ctx *ctx, uint32_t max_leaf, uint32_t bins);
 
+/* Device memory a single-film tile launch may use for path records (48 B
+ * per pixel sample; default 4.5 GiB, which holds 1440x900 at 64 samples):
+ * requests above it run as several launches, each continuing the film
+ * (bit-identical either way).  bytes = 0 restores the default.  No
+ * counterpart in the reference. */
+lucent_status lucent_set_record_budget(lucent_ctx *ctx, uint64_t bytes);
+
 /* Convenience form for traversal only: the BVH of buildBVH plus the positions
  * it was built over (the arguments of intersectBVH, src/math/bvh.ts:265-271);
  * triangles are gathered into leaf order on upload, as packScene does. */

@@ -161,6 +161,7 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_frame_graph_launch": (_I32, [_P, _P]),
     "lucent_decode_hdr": (_I32, [_P, C.c_size_t, C.POINTER(_U32), C.POINTER(_U32), _P, C.c_size_t]),
     "lucent_set_tree_build": (_I32, [_P, _U32, _U32]),
+    "lucent_set_record_budget": (_I32, [_P, C.c_uint64]),
     "lucent_frame_graph_destroy": (None, [_P]),
     "lucent_backward": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _U32, _U32,
                                 _U32, _P, _P, _P, _P, _P]),

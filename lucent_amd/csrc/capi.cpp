@@ -348,6 +348,13 @@ lucent_status lucent_set_tree_build(lucent_ctx* ctx, uint32_t max_leaf, uint32_t
   return LUCENT_OK;
 }
 
+lucent_status lucent_set_record_budget(lucent_ctx* ctx, uint64_t bytes) {
+  clear_error();
+  if (!ctx) return fail(LUCENT_E_INVALID, "null ctx");
+  ctx->record_budget = bytes ? size_t(bytes) : size_t(9) << 29;
+  return LUCENT_OK;
+}
+
 lucent_status lucent_scene_upload(lucent_ctx* ctx, const lucent_packed_scene* ps, lucent_scene** out) {
   clear_error();
   if (!ctx || !ps || !out) return fail(LUCENT_E_INVALID, "null ctx/packed/out");

@@ -77,9 +77,11 @@ struct Context {
   // Wavefront path tracer: path state and queues.
   void* wavefront = nullptr;
   size_t wavefront_bytes = 0;
-  // Per-sample tile launches: path records and per-pixel counters.
+  // Per-sample tile launches: path records and per-pixel counters, and the
+  // most record bytes one launch may use.
   void* units = nullptr;
   size_t units_bytes = 0;
+  size_t record_budget = size_t(9) << 29;
   uint32_t* d_flags = nullptr;  // device status word(s)
   float* sheen_lut = nullptr;   // 16x16 half-rounded sheen albedo table
   // Device tree built at scene upload (binned SAH over the caller's

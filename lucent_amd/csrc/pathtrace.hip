@@ -704,11 +704,6 @@ uint32_t tile_jobs(uint32_t width, uint32_t height) {
   return ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
 }
 
-// Path records of one per-sample launch at most (48 B per path: a
-// 1440x900 film's 64 samples take 4.0 GB of the 288 GB; larger sample
-// counts run as several launches, each continuing the film).
-constexpr size_t kUnitBytes = size_t(9) << 29;
-
 Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hipStream_t stream) {
   item.tiles_x = (item.frame.width + kTile - 1) / kTile;
   const uint32_t n_jobs = tile_jobs(item.frame.width, item.frame.height);
@@ -720,7 +715,10 @@ Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hi
   if (s != LUCENT_OK) return s;
   const size_t slots = size_t(n_jobs) * kPtBlock;
   const size_t per_sample = slots * kRecordFloat4 * sizeof(float4);
-  const uint32_t chunk = uint32_t(std::max<size_t>(1, std::min<size_t>(item.spp, kUnitBytes / per_sample)));
+  // Path records of one launch at most ctx->record_budget bytes (48 B per
+  // path: a 1440x900 film's 64 samples take 4.0 GB of the 288 GB); larger
+  // sample counts run as several launches, each continuing the film.
+  const uint32_t chunk = uint32_t(std::max<size_t>(1, std::min<size_t>(item.spp, ctx->record_budget / per_sample)));
   const size_t counters_off = (size_t(chunk) * per_sample + 255) / 256 * 256;
   if ((s = ensure_units(ctx, counters_off + slots * sizeof(uint32_t))) != LUCENT_OK) return s;
   if ((s = ensure_tile_table(ctx, 256)) != LUCENT_OK) return s;

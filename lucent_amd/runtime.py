@@ -48,6 +48,11 @@ class Context:
         on (max_leaf 0 keeps the caller's BVH as the device tree)."""
         _native.check(_native.lib().lucent_set_tree_build(self.handle, max_leaf, bins))
 
+    def set_record_budget(self, nbytes: int = 0) -> None:
+        """lucent_set_record_budget: path-record bytes one single-film tile
+        launch may use (0: the default, 4.5 GiB)."""
+        _native.check(_native.lib().lucent_set_record_budget(self.handle, nbytes))
+
     def close(self) -> None:
         if self._h is not None:
             _native.lib().lucent_destroy(self._h)

@@ -256,3 +256,21 @@ def test_device_resident_gradient_evaluation(diff):
     assert np.allclose(e, e_host, rtol=1e-4, atol=1e-6 * np.abs(e_host).max())
     with pytest.raises(ValueError, match="float32"):
         diff.backward(d_dl.double(), SPP, 0)
+
+
+def test_recorded_radiance_survives_a_split_launch(ctx, diff):
+    """With a record budget below the forward's 32 samples, lucent_forward
+    runs several launches: the per-sample radiance record and the image are
+    bit-identical to the single launch's."""
+    import torch
+    img = diff.forward(SPP, 0)
+    rec = diff._record[3].clone()
+    per_sample = ((W + 7) // 8) * ((H + 7) // 8) * 64 * 48
+    ctx.set_record_budget(5 * per_sample)
+    try:
+        img2 = diff.forward(SPP, 0)
+        rec2 = diff._record[3]
+    finally:
+        ctx.set_record_budget(0)
+    assert np.array_equal(img, img2)
+    assert torch.equal(rec, rec2)

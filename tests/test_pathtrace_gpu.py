@@ -316,6 +316,26 @@ def test_fused_traversal_loop_bit_identical_to_inline_shadows(ctx, which, occupa
         assert np.array_equal(a[k], c[k]), k
 
 
+def test_record_budget_splits_the_launch(ctx):
+    """A single-film launch whose path records exceed the record budget runs
+    as several launches, each continuing the film (here one sample per
+    launch, then three): the film is still bit-identical to the per-sample
+    launch structure's."""
+    sc = feature_scene()
+    w, h, spp = 70, 45, 7
+    per_sample = ((w + 7) // 8) * ((h + 7) // 8) * 64 * 48
+    a, ia = _film_after(ctx, sc, w, h, spp, 5, RenderSettings(8, 0.0, per_sample_launch=True))
+    for budget in (1, 3 * per_sample):
+        ctx.set_record_budget(budget)
+        try:
+            b, ib = _film_after(ctx, sc, w, h, spp, 5, RenderSettings(8, 0.0))
+        finally:
+            ctx.set_record_budget(0)
+        assert ia == ib
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (budget, k)
+
+
 def test_scene_without_triangles_in_every_structure(ctx):
     """Spheres and the ground only: rays start with an empty traversal stack
     (nothing to pop) in the per-sample, per-sample-hand-out, tile and
