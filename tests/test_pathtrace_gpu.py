@@ -316,6 +316,20 @@ def test_fused_traversal_loop_bit_identical_to_inline_shadows(ctx, which, occupa
         assert np.array_equal(a[k], c[k]), k
 
 
+@pytest.mark.parametrize("w,h,spp", [(1, 1, 9), (3, 2, 5), (9, 1, 3), (2, 2, 130)])
+def test_tiny_films_in_every_structure(ctx, w, h, spp):
+    """Films smaller than a tile (one pixel, a partial tile, many samples on
+    four pixels): the per-sample hand-out, the pixel hand-out and the inline
+    structure give the per-sample launches' film bit for bit."""
+    sc = feature_scene()
+    a, ia = _film_after(ctx, sc, w, h, spp, 1, RenderSettings(8, 0.0, per_sample_launch=True))
+    for st in (RenderSettings(8, 0.0), RenderSettings(8, 0.0, inline_shadows=True)):
+        b, ib = _film_after(ctx, sc, w, h, spp, 1, st)
+        assert ia == ib
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+
+
 def test_record_budget_splits_the_launch(ctx):
     """A single-film launch whose path records exceed the record budget runs
     as several launches, each continuing the film (here one sample per
