@@ -290,27 +290,14 @@ constexpr int kShadeBatch = LUCENT_SHADE_BATCH;
 // film is bit-identical to run_item's.
 enum : uint32_t { kLaneIdle = 0, kLaneTrace = 1, kLaneShade = 2, kLaneFinish = 3 };
 
+// One lane's path in the fused loop: its state, the ray being traced and
+// the light samples of the current step.
 template <class StackT>
-__device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t job, uint32_t job_base, uint32_t job_end,
-                                                   uint32_t* counter, StackT& st, float4* slots) {
-  const PtFrame& f = it.frame;
-  const PtScene& sc = it.scene;
-  const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
-  const uint32_t lane = threadIdx.x;
-  uint32_t x0 = 0, y0 = 0, tw = 1, slots_in_job = 0, handed = 0;
-  auto open_job = [&](uint32_t j) {
-    const uint32_t t = j - job_base;
-    x0 = (t % it.tiles_x) * kTile;
-    y0 = (t / it.tiles_x) * kTile;
-    tw = min(uint32_t(kTile), f.width - x0);
-    slots_in_job = tw * min(uint32_t(kTile), f.height - y0);
-    handed = 0;
-  };
-  open_job(job);
-  bool taking = true;
-  uint32_t next = 0;
+struct FusedPath {
+  const PtScene& sc;
+  StackT& st;
+  float4* slots;  // the lane's light-sample slots
   uint32_t state = kLaneIdle;
-  uint32_t idx = 0, k = 0;
   pt::Rng rng{};
   pt::PathState p;
   pt::Trav4 t;
@@ -319,18 +306,24 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
   uint32_t analytic = 0;
   bool any = false, cont = false;
   uint32_t which = 0, pending = 0;
-  auto start_closest = [&]() {
+
+  __device__ __forceinline__ void start_closest() {
     pt::HitRecord rec;
     pt::search_analytic(sc, p.ro, p.rd, rec);
     analytic = rec.kind << 30 | rec.prim;
     pt::trav4_begin(sc, t, p.ro, p.rd, rec.t, st);
     any = false;
     state = kLaneTrace;
-  };
+  }
+  // A new path (camera ray of pixel idx = (x, y) with this seed).
+  __device__ __forceinline__ void start(const PtFrame& f, uint32_t x, uint32_t y, uint32_t idx, uint32_t seed) {
+    begin_sample(f, x, y, idx, seed, rng, p);
+    start_closest();
+  }
   // The next ray of the step: the lowest pending light sample not blocked by
   // a sphere or the ground (occluded's order), else the continuation, else
   // the path is finished.
-  auto next_ray = [&]() {
+  __device__ __forceinline__ void next_ray() {
     while (pending != 0) {
       which = uint32_t(__builtin_ctz(pending));
       pending &= pending - 1u;
@@ -351,9 +344,72 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
       start_closest();
     else
       state = kLaneFinish;
+  }
+  // One traversal step of the ray being traced; a finished shadow ray adds
+  // its contribution when unoccluded and moves on, a finished closest-hit
+  // ray waits for shading.
+  __device__ __forceinline__ void trace_turn() {
+    bool blocked = false;
+    if (!pt::trav4_done(t, st)) blocked = pt::trav4_step_mixed(sc, t, st, rng, any);
+    if (blocked || pt::trav4_done(t, st)) {
+      if (any) {
+        if (!blocked) p.radiance = p.radiance + pt::mk(slots[3 * which + 2]);
+        next_ray();
+      } else {
+        state = kLaneShade;
+      }
+    }
+  }
+  // Shades the step at the closest hit: light samples into the slots, the
+  // continuation into the path state.
+  __device__ __forceinline__ void shade(const PtFrame& f) {
+    pt::HitRecord rec;
+    rec.t = t.tmax;
+    if (t.best >= 0) {
+      rec.kind = pt::kHitTri;
+      rec.prim = uint32_t(t.best);
+      rec.u = t.bu;
+      rec.v = t.bv;
+    } else {
+      rec.kind = analytic >> 30;
+      rec.prim = analytic & 0x3FFFFFFFu;
+      rec.u = rec.v = 0.0f;
+    }
+    DeferTracer tr{sc, slots, rec, 0u};
+    cont = pt::path_step_with(sc, f, p, rng, tr);
+    pending = tr.mask;
+    next_ray();
+  }
+};
+
+// True when the wave should shade now: enough lanes wait, or none traces.
+__device__ __forceinline__ bool shade_now(uint32_t state) {
+  const uint64_t ready = __ballot(state == kLaneShade || state == kLaneFinish);
+  return ready != 0 && (__popcll(ready) >= kShadeBatch || __ballot(state == kLaneTrace) == 0);
+}
+
+template <class StackT>
+__device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t job, uint32_t job_base, uint32_t job_end,
+                                                   uint32_t* counter, StackT& st, float4* slots) {
+  const PtFrame& f = it.frame;
+  const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
+  const uint32_t lane = threadIdx.x;
+  uint32_t x0 = 0, y0 = 0, tw = 1, slots_in_job = 0, handed = 0;
+  auto open_job = [&](uint32_t j) {
+    const uint32_t t = j - job_base;
+    x0 = (t % it.tiles_x) * kTile;
+    y0 = (t / it.tiles_x) * kTile;
+    tw = min(uint32_t(kTile), f.width - x0);
+    slots_in_job = tw * min(uint32_t(kTile), f.height - y0);
+    handed = 0;
   };
+  open_job(job);
+  bool taking = true;
+  uint32_t next = 0;
+  uint32_t idx = 0, k = 0;
+  FusedPath<StackT> lp{it.scene, st, slots};
   for (;;) {
-    uint64_t idle = __ballot(state == kLaneIdle);
+    uint64_t idle = __ballot(lp.state == kLaneIdle);
     while (taking && idle != 0) {
       if (handed == slots_in_job) {
         const uint32_t j = take_job(counter);
@@ -371,58 +427,26 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
         const uint32_t s = handed + rank;
         idx = (y0 + s / tw) * f.width + x0 + s % tw;
         k = 0;
-        begin_sample(f, idx % f.width, idx / f.width, idx, seed0, rng, p);
-        start_closest();
+        lp.start(f, idx % f.width, idx / f.width, idx, seed0);
       }
       handed += min(uint32_t(__popcll(idle)), avail);
-      idle = __ballot(state == kLaneIdle);
+      idle = __ballot(lp.state == kLaneIdle);
     }
-    if (state == kLaneTrace) {
-      bool blocked = false;
-      if (!pt::trav4_done(t, st)) blocked = pt::trav4_step_mixed(sc, t, st, rng, any);
-      if (blocked || pt::trav4_done(t, st)) {
-        if (any) {
-          if (!blocked) p.radiance = p.radiance + pt::mk(slots[3 * which + 2]);
-          next_ray();
-        } else {
-          state = kLaneShade;
-        }
-      }
-    }
-    const uint64_t ready = __ballot(state == kLaneShade || state == kLaneFinish);
-    if (ready != 0 && (__popcll(ready) >= kShadeBatch || __ballot(state == kLaneTrace) == 0)) {
-      if (state == kLaneShade) {
-        pt::HitRecord rec;
-        rec.t = t.tmax;
-        if (t.best >= 0) {
-          rec.kind = pt::kHitTri;
-          rec.prim = uint32_t(t.best);
-          rec.u = t.bu;
-          rec.v = t.bv;
-        } else {
-          rec.kind = analytic >> 30;
-          rec.prim = analytic & 0x3FFFFFFFu;
-          rec.u = rec.v = 0.0f;
-        }
-        DeferTracer tr{sc, slots, rec, 0u};
-        cont = pt::path_step_with(sc, f, p, rng, tr);
-        pending = tr.mask;
-        next_ray();
-      }
-      if (state == kLaneFinish) {
+    if (lp.state == kLaneTrace) lp.trace_turn();
+    if (shade_now(lp.state)) {
+      if (lp.state == kLaneShade) lp.shade(f);
+      if (lp.state == kLaneFinish) {
         if (it.sample_radiance)
           reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
-              make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
-        pt::accumulate(it.film, idx, p.radiance, p.guide);
-        if (++k >= it.spp) {
-          state = kLaneIdle;
-        } else {
-          begin_sample(f, idx % f.width, idx / f.width, idx, seed0 + k, rng, p);
-          start_closest();
-        }
+              make_float4(lp.p.radiance.x, lp.p.radiance.y, lp.p.radiance.z, 0.0f);
+        pt::accumulate(it.film, idx, lp.p.radiance, lp.p.guide);
+        if (++k >= it.spp)
+          lp.state = kLaneIdle;
+        else
+          lp.start(f, idx % f.width, idx / f.width, idx, seed0 + k);
       }
     }
-    if (!taking && __ballot(state != kLaneIdle) == 0) return next;
+    if (!taking && __ballot(lp.state != kLaneIdle) == 0) return next;
   }
 }
 
@@ -446,7 +470,6 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
                                                 StackT& st, float4* slots, float4* __restrict__ records,
                                                 uint32_t* __restrict__ arrived) {
   const PtFrame& f = it.frame;
-  const PtScene& sc = it.scene;
   const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
   const uint32_t spp = it.spp;
   const uint32_t lane = threadIdx.x;
@@ -464,46 +487,10 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
   };
   open_job(job);
   bool taking = true;
-  uint32_t state = kLaneIdle;
   uint32_t idx = 0, k = 0, rslot = 0;
-  pt::Rng rng{};
-  pt::PathState p;
-  pt::Trav4 t;
-  uint32_t analytic = 0;
-  bool any = false, cont = false;
-  uint32_t which = 0, pending = 0;
-  auto start_closest = [&]() {
-    pt::HitRecord rec;
-    pt::search_analytic(sc, p.ro, p.rd, rec);
-    analytic = rec.kind << 30 | rec.prim;
-    pt::trav4_begin(sc, t, p.ro, p.rd, rec.t, st);
-    any = false;
-    state = kLaneTrace;
-  };
-  auto next_ray = [&]() {
-    while (pending != 0) {
-      which = uint32_t(__builtin_ctz(pending));
-      pending &= pending - 1u;
-      const float4 o = slots[3 * which], d = slots[3 * which + 1];
-      const pt::v3 ro = pt::mk(o), rd = pt::mk(d);
-      bool blocked = false;
-      for (uint32_t i = 0; i < sc.sphere_count && !blocked; ++i)
-        blocked = pt::hit_sphere(ro, rd, reinterpret_cast<const float4*>(sc.spheres)[i], o.w) < o.w;
-      if (!blocked) blocked = pt::hit_ground(sc, ro, rd, o.w) < o.w;
-      if (!blocked) {
-        pt::trav4_begin(sc, t, ro, rd, o.w, st);
-        any = true;
-        state = kLaneTrace;
-        return;
-      }
-    }
-    if (cont)
-      start_closest();
-    else
-      state = kLaneFinish;
-  };
+  FusedPath<StackT> lp{it.scene, st, slots};
   for (;;) {
-    uint64_t idle = __ballot(state == kLaneIdle);
+    uint64_t idle = __ballot(lp.state == kLaneIdle);
     while (taking && idle != 0) {
       if (handed == units) {
         const uint32_t j = take_job(counter);
@@ -522,45 +509,16 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
         k = u / px;
         rslot = cur * kPtBlock + slot;
         idx = (y0 + slot / tw) * f.width + x0 + slot % tw;
-        begin_sample(f, idx % f.width, idx / f.width, idx, seed0 + k, rng, p);
-        start_closest();
+        lp.start(f, idx % f.width, idx / f.width, idx, seed0 + k);
       }
       handed += min(uint32_t(__popcll(idle)), avail);
-      idle = __ballot(state == kLaneIdle);
+      idle = __ballot(lp.state == kLaneIdle);
     }
-    if (state == kLaneTrace) {
-      bool blocked = false;
-      if (!pt::trav4_done(t, st)) blocked = pt::trav4_step_mixed(sc, t, st, rng, any);
-      if (blocked || pt::trav4_done(t, st)) {
-        if (any) {
-          if (!blocked) p.radiance = p.radiance + pt::mk(slots[3 * which + 2]);
-          next_ray();
-        } else {
-          state = kLaneShade;
-        }
-      }
-    }
-    const uint64_t ready = __ballot(state == kLaneShade || state == kLaneFinish);
-    if (ready != 0 && (__popcll(ready) >= kShadeBatch || __ballot(state == kLaneTrace) == 0)) {
-      if (state == kLaneShade) {
-        pt::HitRecord rec;
-        rec.t = t.tmax;
-        if (t.best >= 0) {
-          rec.kind = pt::kHitTri;
-          rec.prim = uint32_t(t.best);
-          rec.u = t.bu;
-          rec.v = t.bv;
-        } else {
-          rec.kind = analytic >> 30;
-          rec.prim = analytic & 0x3FFFFFFFu;
-          rec.u = rec.v = 0.0f;
-        }
-        DeferTracer tr{sc, slots, rec, 0u};
-        cont = pt::path_step_with(sc, f, p, rng, tr);
-        pending = tr.mask;
-        next_ray();
-      }
-      if (state == kLaneFinish) {
+    if (lp.state == kLaneTrace) lp.trace_turn();
+    if (shade_now(lp.state)) {
+      if (lp.state == kLaneShade) lp.shade(f);
+      if (lp.state == kLaneFinish) {
+        const pt::PathState& p = lp.p;
         if (it.sample_radiance)
           reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
               make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
@@ -584,10 +542,10 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
           }
           pt::store_accum(it.film, idx, a);
         }
-        state = kLaneIdle;
+        lp.state = kLaneIdle;
       }
     }
-    if (!taking && __ballot(state != kLaneIdle) == 0) return;
+    if (!taking && __ballot(lp.state != kLaneIdle) == 0) return;
   }
 }
 
