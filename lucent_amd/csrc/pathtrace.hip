@@ -388,6 +388,18 @@ __device__ __forceinline__ bool shade_now(uint32_t state) {
   return ready != 0 && (__popcll(ready) >= kShadeBatch || __ballot(state == kLaneTrace) == 0);
 }
 
+// Up to kTraceTurns traversal steps while some lane traces, between the
+// wave's checks for shading and new work: the checks cost little, but a
+// loop of traversal steps alone keeps the wave on its hot code (measured:
+// 8 steps per check, 128 -> 116 ms for 1440x900 at 64 samples).
+constexpr int kTraceTurns = 8;
+
+template <class Lane>
+__device__ __forceinline__ void trace_turns(Lane& lp) {
+  for (int turn = 0; turn < kTraceTurns && __ballot(lp.state == kLaneTrace) != 0; ++turn)
+    if (lp.state == kLaneTrace) lp.trace_turn();
+}
+
 template <class StackT>
 __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t job, uint32_t job_base, uint32_t job_end,
                                                    uint32_t* counter, StackT& st, float4* slots) {
@@ -432,7 +444,7 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
       handed += min(uint32_t(__popcll(idle)), avail);
       idle = __ballot(lp.state == kLaneIdle);
     }
-    if (lp.state == kLaneTrace) lp.trace_turn();
+    trace_turns(lp);
     if (shade_now(lp.state)) {
       if (lp.state == kLaneShade) lp.shade(f);
       if (lp.state == kLaneFinish) {
@@ -514,7 +526,7 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
       handed += min(uint32_t(__popcll(idle)), avail);
       idle = __ballot(lp.state == kLaneIdle);
     }
-    if (lp.state == kLaneTrace) lp.trace_turn();
+    trace_turns(lp);
     if (shade_now(lp.state)) {
       if (lp.state == kLaneShade) lp.shade(f);
       if (lp.state == kLaneFinish) {
