@@ -249,7 +249,7 @@ def bench_path_trace(args) -> dict:
                        "scene_load_s": round(load_s, 3)},
             "roofline": roofline, "cpu_baseline": cpu, "image_mean_rgb": mean_rgb,
             "mpaths_per_s": round(value / 1e6, 2), "mrays_per_s_all_rays": round(value * rays_per_path / 1e6, 1),
-            "autotune": tune,
+            "autotune": {"occupancy": tune["occupancy"], "ms": {k: round(v, 3) for k, v in tune["ms"].items()}},
             "per_sample_launches": {"samples_per_s": round(per_sample_value, 1),
                                     "trace_ms": round(float(np.mean(trace_ms_ps)), 3),
                                     "avg_sample_launch_ms": round(float(np.mean(trace_ms_ps)) / SPP, 4),
@@ -481,7 +481,8 @@ def bench_batch(args) -> dict:
     return {"metric": "samples/s, 64-scene batch: one persistent launch vs per-sample launches per scene",
             "value": round(paths / tb, 1), "unit": "samples/s", "plain_samples_per_s": round(paths / tp, 1),
             "speedup_vs_plain": round(tp / tb, 3), "batched_s": round(tb, 3), "plain_s": round(tp, 3),
-            "films_bit_identical": same, "autotune": tune, "scenes": len(jobs), "triangles": tris, "paths": paths,
+            "films_bit_identical": same,
+            "autotune": {"occupancy": tune["occupancy"], "ms": {k: round(v, 3) for k, v in tune["ms"].items()}}, "scenes": len(jobs), "triangles": tris, "paths": paths,
             "setup_s": round(setup_s, 1)}
 
 

@@ -526,7 +526,9 @@ lucent_status lucent_autotune_tiles(lucent_ctx* ctx, const lucent_batch_item* it
       }
       if (pass == 1) {
         if (ms_out) ms_out[occ - 2] = ms;
-        if (best == 0 || ms < best_ms) {
+        // A higher occupancy must win by more than 2 %: its register spills
+        // cost memory traffic that a near-tie does not repay.
+        if (best == 0 || ms < best_ms * 0.98f) {
           best = occ;
           best_ms = ms;
         }

@@ -420,7 +420,7 @@ def autotune(ctx: Context, jobs, settings: RenderSettings, trial_samples: int = 
     _native.check(_native.lib().lucent_autotune_tiles(ctx.handle, _batch_items(ctx, jobs), len(jobs), C.byref(st),
                                                       trial_samples, C.byref(best), ms))
     settings.occupancy = best.value
-    return {"occupancy": best.value, "ms": {k + 2: round(float(ms[k]), 3) for k in range(5)}}
+    return {"occupancy": best.value, "ms": {k + 2: float(ms[k]) for k in range(5)}}
 
 
 def trace_batch(ctx: Context, jobs, settings: RenderSettings, stream: Optional[int] = None) -> None:
