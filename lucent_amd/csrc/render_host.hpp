@@ -42,7 +42,8 @@ struct TileLaunch {
   // traversal draws no random numbers (no alpha-blended material).
   bool fused = true;
   // One film, fused: hand out pixel samples rather than whole pixels (the
-  // context's buffers only; launches of up to kUnitBytes of path records).
+  // context's buffers only; launches of up to the context's record budget
+  // of path records, lucent_set_record_budget).
   bool units = true;
 };
 // Spill bytes the launch of one film with `cfg` needs.
