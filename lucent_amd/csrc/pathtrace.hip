@@ -31,6 +31,9 @@ namespace lucent {
 namespace {
 
 constexpr int kPtBlock = 64;
+#ifndef LUCENT_RING_JOBS
+#define LUCENT_RING_JOBS 4
+#endif
 #ifndef LUCENT_SHADE_BATCH
 #define LUCENT_SHADE_BATCH 32
 #endif
@@ -561,6 +564,148 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
   }
 }
 
+// Jobs whose records a wave of the batch kernel keeps (a ring).
+constexpr uint32_t kRingJobs = LUCENT_RING_JOBS;
+
+// run_units_fused for a launch over many films (a batch): the wave keeps
+// the records of its last kRingJobs jobs in its own area (kRingJobs x 64
+// pixels x s_max samples), and opens a job into a ring slot only once the
+// job that used the slot before has folded all its pixels (its samples were
+// all handed out kRingJobs - 1 jobs earlier, so the wave seldom waits).  A change of film waits only for the paths in flight, not for the
+// lanes' pixels' remaining samples.  Returns the first job taken that
+// belongs to another film (or >= the queue's end).
+template <class StackT>
+__device__ __forceinline__ uint32_t run_units_ring(const TileItem& it, uint32_t job, uint32_t job_base, uint32_t job_end,
+                                                   uint32_t* counter, StackT& st, float4* slots,
+                                                   float4* __restrict__ records, uint32_t* __restrict__ arrived,
+                                                   uint32_t s_max) {
+  const PtFrame& f = it.frame;
+  const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
+  const uint32_t spp = it.spp;
+  const uint32_t lane = threadIdx.x;
+  // The job being handed out and the ring (wave-uniform).
+  uint32_t x0 = 0, y0 = 0, tw = 1, px = 0, units = 0, handed = 0;
+  uint32_t cur_ring = 0, next_ring = 0;
+  uint32_t left[kRingJobs] = {};  // pixels not yet folded per ring slot
+  auto open_job = [&](uint32_t j) {
+    const uint32_t t = j - job_base;
+    x0 = (t % it.tiles_x) * kTile;
+    y0 = (t / it.tiles_x) * kTile;
+    tw = min(uint32_t(kTile), f.width - x0);
+    px = tw * min(uint32_t(kTile), f.height - y0);
+    units = px * spp;
+    handed = 0;
+    cur_ring = next_ring;
+    next_ring = next_ring + 1u == kRingJobs ? 0u : next_ring + 1u;
+#pragma unroll
+    for (uint32_t r = 0; r < kRingJobs; ++r)
+      if (r == cur_ring) left[r] = px;
+    __hip_atomic_store(arrived + cur_ring * kPtBlock + lane, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  open_job(job);
+  bool taking = true;
+  uint32_t next = 0;
+  uint32_t idx = 0, k = 0, rslot = 0;
+  FusedPath<StackT> lp{it.scene, st, slots};
+  for (;;) {
+    uint64_t idle = __ballot(lp.state == kLaneIdle);
+    while (taking && idle != 0) {
+      if (handed == units) {
+        uint32_t busy = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < kRingJobs; ++r)
+          if (r == next_ring) busy = left[r];
+        if (busy != 0) break;  // that slot's job is still folding
+        const uint32_t j = take_job(counter);
+        if (j >= job_end) {
+          taking = false;
+          next = j;
+          break;
+        }
+        open_job(j);
+        continue;
+      }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
+      const uint32_t avail = units - handed;
+      if (((idle >> lane) & 1u) && rank < avail) {
+        const uint32_t u = handed + rank;
+        const uint32_t slot = u % px;
+        k = u / px;
+        rslot = cur_ring * kPtBlock + slot;
+        idx = (y0 + slot / tw) * f.width + x0 + slot % tw;
+        lp.start(f, idx % f.width, idx / f.width, idx, seed0 + k);
+      }
+      handed += min(uint32_t(__popcll(idle)), avail);
+      idle = __ballot(lp.state == kLaneIdle);
+    }
+    trace_turns(lp);
+    if (shade_now(lp.state)) {
+      if (lp.state == kLaneShade) lp.shade(f);
+      bool folded = false;
+      if (lp.state == kLaneFinish) {
+        const pt::PathState& p = lp.p;
+        if (it.sample_radiance)
+          reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
+              make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
+        float4* r = records + (size_t(rslot) * s_max + k) * kRecordFloat4;
+        r[0] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, p.guide.depth);
+        r[1] = make_float4(p.guide.albedo.x, p.guide.albedo.y, p.guide.albedo.z, p.guide.coverage);
+        r[2] = make_float4(p.guide.normal.x, p.guide.normal.y, p.guide.normal.z, 0.0f);
+        const uint32_t before =
+            __hip_atomic_fetch_add(arrived + rslot, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (before + 1u == spp) {
+          pt::PixelAccum a = pt::load_accum(it.film, idx);
+          const float4* q = records + size_t(rslot) * s_max * kRecordFloat4;
+          for (uint32_t s = 0; s < spp; ++s, q += kRecordFloat4) {
+            const float4 c = q[0], al = q[1], n = q[2];
+            pt::Guide g = p.guide;
+            g.albedo = pt::mk(al.x, al.y, al.z);
+            g.coverage = al.w;
+            g.normal = pt::mk(n.x, n.y, n.z);
+            g.depth = c.w;
+            pt::fold_sample(a, pt::mk(c.x, c.y, c.z), g);
+          }
+          pt::store_accum(it.film, idx, a);
+          folded = true;
+        }
+        lp.state = kLaneIdle;
+      }
+#pragma unroll
+      for (uint32_t r = 0; r < kRingJobs; ++r)
+        left[r] -= uint32_t(__popcll(__ballot(folded && rslot / kPtBlock == r)));
+    }
+    if (!taking && __ballot(lp.state != kLaneIdle) == 0) return next;
+  }
+}
+
+// Many films, work handed out per pixel sample (run_units_ring).  Each wave
+// owns kRingJobs * 64 * s_max records and kRingJobs * 64 counters.
+template <int kWaves>
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_batch_units(
+    const TileItem* __restrict__ items, const uint32_t* __restrict__ job_begin, uint32_t n_items, uint32_t n_jobs,
+    uint32_t* __restrict__ counter, uint32_t* scratch, uint32_t lane_words, float4* __restrict__ records,
+    uint32_t* __restrict__ arrived, uint32_t s_max) {
+  __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
+  uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
+  float4* slots = reinterpret_cast<float4*>(area);
+  pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
+  float4* my_records = records + size_t(blockIdx.x) * kRingJobs * kPtBlock * s_max * kRecordFloat4;
+  uint32_t* my_arrived = arrived + size_t(blockIdx.x) * kRingJobs * kPtBlock;
+  uint32_t job = take_job(counter);
+  while (job < n_jobs) {
+    uint32_t item = 0, hi = n_items;
+    while (hi - item > 1) {
+      const uint32_t mid = (item + hi) / 2;
+      if (job_begin[mid] <= job)
+        item = mid;
+      else
+        hi = mid;
+    }
+    job = run_units_ring(items[item], job, job_begin[item], job_begin[item + 1], counter, st, slots, my_records,
+                         my_arrived, s_max);
+  }
+}
+
 // One film, work handed out per pixel sample (run_units_fused).
 template <int kWaves>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_units(
@@ -787,6 +932,31 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
   const TileItem* d_items = n_items > 1 ? reinterpret_cast<const TileItem*>(table + items_off) : nullptr;
   const uint32_t* d_begin = n_items > 1 ? reinterpret_cast<const uint32_t*>(table + begin_off) : nullptr;
+  if (n_items > 1 && cfg.fused && cfg.units) {
+    // Per-sample hand-out with a record ring of kRingJobs jobs per wave,
+    // when the records fit the context's budget (else the pixel hand-out
+    // below).
+    uint32_t s_max = 0;
+    for (uint32_t i = 0; i < n_items; ++i) s_max = std::max(s_max, items[i].spp);
+    const size_t ring_records = size_t(grid) * kRingJobs * kPtBlock * s_max * kRecordFloat4 * sizeof(float4);
+    const size_t counters_off = (ring_records + 255) / 256 * 256;
+    if (s_max > 0 && ring_records <= ctx->record_budget) {
+      if ((s = ensure_units(ctx, counters_off + size_t(grid) * kRingJobs * kPtBlock * sizeof(uint32_t))) != LUCENT_OK)
+        return s;
+      float4* records = static_cast<float4*>(ctx->units);
+      uint32_t* arrived = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->units) + counters_off);
+      auto batch_kernel = waves == 3 ? k_trace_batch_units<3>
+                          : waves == 4 ? k_trace_batch_units<4>
+                          : waves == 5 ? k_trace_batch_units<5>
+                          : waves == 6 ? k_trace_batch_units<6>
+                                       : k_trace_batch_units<2>;
+      hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kPtBlock), 0, stream, d_items, d_begin, n_items, n_jobs, counter,
+                         spill, lane_words, records, arrived, s_max);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return fail_hip(e, "k_trace_batch_units launch");
+      return LUCENT_OK;
+    }
+  }
   auto launch = [&](auto single_kernel, auto multi_kernel) {
     if (n_items == 1)
       hipLaunchKernelGGL(single_kernel, dim3(grid), dim3(kPtBlock), 0, stream, items[0], nullptr, nullptr, 1u, n_jobs,
