@@ -127,9 +127,10 @@ lucent_status lucent_scene_upload(lucent_ctx *ctx, const lucent_packed_scene *pa
  * are given). */
 lucent_status lucent_set_tree_build(lucent_ctx *ctx, uint32_t max_leaf, uint32_t bins);
 
-/* Device memory a single-film tile launch may use for path records (48 B
- * per pixel sample; default 4.5 GiB, which holds 1440x900 at 64 samples):
- * requests above it run as several launches, each continuing the film
+/* Device memory a tile launch may use for path records (48 B per pixel
+ * sample; default 4.5 GiB, which holds 1440x900 at 64 samples): a single
+ * film above it runs as several launches, each continuing the film; a batch
+ * whose per-wave record rings exceed it hands out whole pixels instead
  * (bit-identical either way).  bytes = 0 restores the default.  No
  * counterpart in the reference. */
 lucent_status lucent_set_record_budget(lucent_ctx *ctx, uint64_t bytes);
