@@ -210,10 +210,22 @@ def test_autotune_picks_a_variant_and_leaves_films_alone(ctx):
     r.close()
 
 
-def test_batch_equals_individual_films(ctx):
+@pytest.mark.parametrize("budget", [0, 1])
+def test_batch_equals_individual_films(ctx, budget):
     """lucent_trace_batch over several scenes, sizes and sample counts: every
     film equals what lucent_trace_samples makes of it alone; a second batch
-    continues each film's seeds and history."""
+    continues each film's seeds and history.  With the default record budget
+    the batch hands out pixel samples (per-wave record rings); with a budget
+    of 1 byte it hands out whole pixels."""
+    from lucent_amd.renderer import trace_batch
+    ctx.set_record_budget(budget)
+    try:
+        _batch_equals_individual_films(ctx)
+    finally:
+        ctx.set_record_budget(0)
+
+
+def _batch_equals_individual_films(ctx):
     from lucent_amd.renderer import trace_batch
 
     scenes = [feature_scene(), workloads.spheres_scene(), feature_scene()]
