@@ -363,13 +363,13 @@ lucent_status lucent_trace_batch(lucent_ctx *ctx, const lucent_batch_item *items
                                  const lucent_render_settings *settings, void *stream);
 
 /* Autotuner of the persistent launch: runs min(trial_samples, samples) samples
- * of every item (at least 1) once with each register/occupancy variant
+ * of every item (at least 1) twice with each register/occupancy variant
  * (2..6 waves per SIMD) into scratch films of the items' sizes (the items'
  * films are not touched), after one untimed round, and returns the fastest
- * variant for lucent_render_settings.occupancy (a higher occupancy must be
- * more than 2 % faster than a lower one: its register spills cost memory
- * traffic that a near-tie does not repay).  ms_out, if not NULL,
- * receives 5 timings (variants 2..6).  The variants give identical films.
+ * variant by its better time for lucent_render_settings.occupancy (a higher
+ * occupancy must be more than 1 % faster than a lower one: its register
+ * spills cost memory traffic that a near-tie does not repay).  ms_out, if
+ * not NULL, receives the 5 better times (variants 2..6).  The variants give identical films.
  * No counterpart in the reference. */
 lucent_status lucent_autotune_tiles(lucent_ctx *ctx, const lucent_batch_item *items, uint32_t count,
                                     const lucent_render_settings *settings, uint32_t trial_samples,

@@ -544,11 +544,27 @@ __global__ __launch_bounds__(kAdjBlock) void k_detached(PtScene sc, PtScene sv, 
 // LDS table over everything the block replays and are written once as the
 // block's partials.
 // kWaves: the occupancy the register allocation targets (waves per SIMD).
+// The scene and frame of one replay launch, in device memory: k_adjoint
+// reads them through a pointer, so the compiler loads their fields with
+// scalar loads where they are used instead of keeping more of them live
+// (by value, 52 B per lane spilled at 3 waves per SIMD).
+struct AdjointView {
+  PtScene sc;
+  PtFrame f;
+};
+
+__global__ void k_store_view(AdjointView* dst, PtScene sc, PtFrame f) {
+  dst->sc = sc;
+  dst->f = f;
+}
+
 template <int kWaves>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_adjoint(
-    PtScene sc, PtFrame f, uint32_t spp, uint32_t n_jobs, uint32_t* __restrict__ counter, const float* __restrict__ dl_di,
+    const AdjointView* __restrict__ view, uint32_t spp, uint32_t n_jobs, uint32_t* __restrict__ counter, const float* __restrict__ dl_di,
     const float4* __restrict__ recorded_L, float inv_spp, double* __restrict__ mat_partials, double* __restrict__ env_acc,
     size_t env_stride, double* __restrict__ mat_overflow, uint32_t* spill, uint32_t spill_depth, uint32_t tiles_x) {
+  const PtScene& sc = view->sc;
+  const PtFrame& f = view->f;
   __shared__ uint32_t lds[kPtStackWords * kAdjBlock];
   __shared__ double lds_grad[kLdsMaterials * kNumParams];
   for (int i = threadIdx.x; i < kLdsMaterials * kNumParams; i += kAdjBlock) lds_grad[i] = 0.0;
@@ -678,7 +694,8 @@ size_t adjoint_scratch_bytes(uint32_t width, uint32_t height, uint32_t material_
   const size_t partials = blocks * kLdsMaterials * kNumParams;
   const size_t overflow = size_t(material_count) * kNumParams;
   const size_t env = size_t(kEnvReplicas) * env_texels * 3;
-  return (partials + overflow + env + 1) * sizeof(double) + 1024;  // + the replay's job counter
+  // + the replay's job counter, then its AdjointView at a 256-byte boundary.
+  return (partials + overflow + env + 1) * sizeof(double) + 256 + sizeof(AdjointView);
 }
 
 Status launch_detached(Context* ctx, const DetachedArgs& a, void* stream_) {
@@ -714,12 +731,14 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
   const size_t partials = size_t(n_jobs) * kLdsMaterials * kNumParams;
   const size_t overflow = size_t(sc.material_count) * kNumParams;
   const size_t env_stride = env_texels * 3;
-  // Scratch: [block partials | overflow | env replicas | job counter] (adjoint_scratch_bytes).
+  // Scratch: [block partials | overflow | env replicas | job counter | view] (adjoint_scratch_bytes).
   double* base = static_cast<double*>(a.scratch);
   double* mat_partials = base;
   double* mat_overflow = base + partials;
   double* env_acc = mat_overflow + overflow;
   uint32_t* counter = reinterpret_cast<uint32_t*>(env_acc + kEnvReplicas * env_stride);
+  const size_t view_off = ((reinterpret_cast<char*>(counter + 2) - static_cast<char*>(a.scratch)) + 255) / 256 * 256;
+  AdjointView* view = reinterpret_cast<AdjointView*>(static_cast<char*>(a.scratch) + view_off);
   const uint32_t depth = pt_spill_words(sc.bvh);
   Status s = ensure_spill(ctx, size_t(grid) * kAdjBlock * depth * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
@@ -729,8 +748,9 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
   PtFrame f = a.frame;
   f.seed = a.seed0;
   const float4* rec = reinterpret_cast<const float4*>(a.recorded_radiance);
+  hipLaunchKernelGGL(k_store_view, dim3(1), dim3(1), 0, stream, view, sc, f);
   auto kern = waves == 3 ? k_adjoint<3> : waves == 4 ? k_adjoint<4> : k_adjoint<2>;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), 0, stream, sc, f, a.spp, n_jobs, counter, a.dl_di, rec,
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), 0, stream, view, a.spp, n_jobs, counter, a.dl_di, rec,
                      1.0f / float(a.spp), mat_partials, env_acc, env_stride, mat_overflow,
                      static_cast<uint32_t*>(ctx->spill), depth, tiles_x);
   const uint32_t entries = uint32_t(min(int(sc.material_count), kLdsMaterials)) * kNumParams;

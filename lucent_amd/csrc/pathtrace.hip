@@ -706,17 +706,21 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
   }
 }
 
-// One film, work handed out per pixel sample (run_units_fused).
+// One film, work handed out per pixel sample (run_units_fused).  The film's
+// descriptor is read through a pointer to device memory: the compiler then
+// loads its fields with scalar loads where they are used, where a by-value
+// kernel argument kept more of them live (185 instead of 170 VGPRs unbounded;
+// 12 B per lane of spill at 3 waves per SIMD, none this way).
 template <int kWaves>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_units(
-    TileItem single, uint32_t n_jobs, uint32_t* __restrict__ counter, uint32_t* scratch, uint32_t lane_words,
-    float4* __restrict__ records, uint32_t* __restrict__ arrived) {
+    const TileItem* __restrict__ single, uint32_t n_jobs, uint32_t* __restrict__ counter, uint32_t* scratch,
+    uint32_t lane_words, float4* __restrict__ records, uint32_t* __restrict__ arrived) {
   __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
   uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
   float4* slots = reinterpret_cast<float4*>(area);
   pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
   const uint32_t job = take_job(counter);
-  if (job < n_jobs) run_units_fused(single, job, n_jobs, counter, st, slots, records, arrived);
+  if (job < n_jobs) run_units_fused(*single, job, n_jobs, counter, st, slots, records, arrived);
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -765,6 +769,12 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+// Stores one launch's film descriptor for k_trace_units and resets its job
+// counter (stream-ordered, so chunks of one film reuse the same slot).
+__global__ void k_begin_units(TileItem* dst, TileItem v, uint32_t* counter) {
+  *dst = v;
+  *counter = 0u;
+}
 __global__ void k_add_u32(uint32_t* p, uint32_t v) { *p += v; }
 
 }  // namespace
@@ -836,10 +846,12 @@ Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hi
   const uint32_t chunk = uint32_t(std::max<size_t>(1, std::min<size_t>(item.spp, ctx->record_budget / per_sample)));
   const size_t counters_off = (size_t(chunk) * per_sample + 255) / 256 * 256;
   if ((s = ensure_units(ctx, counters_off + slots * sizeof(uint32_t))) != LUCENT_OK) return s;
-  if ((s = ensure_tile_table(ctx, 256)) != LUCENT_OK) return s;
+  // Tile table: [counter | the chunk's TileItem], 256-byte aligned parts.
+  if ((s = ensure_tile_table(ctx, 256 + sizeof(TileItem))) != LUCENT_OK) return s;
   float4* records = static_cast<float4*>(ctx->units);
   uint32_t* arrived = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->units) + counters_off);
   uint32_t* counter = static_cast<uint32_t*>(ctx->tile_table);
+  TileItem* d_item = reinterpret_cast<TileItem*>(static_cast<char*>(ctx->tile_table) + 256);
   uint32_t* scratch = static_cast<uint32_t*>(ctx->spill);
   const size_t frame_px = size_t(item.frame.width) * item.frame.height;
   const uint32_t spp = item.spp;
@@ -849,26 +861,26 @@ Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hi
     it.frame.seed = item.frame.seed + done;
     if (item.sample_radiance) it.sample_radiance = item.sample_radiance + size_t(done) * frame_px * 4;
     hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((slots + 255) / 256)), dim3(256), 0, stream, arrived, slots, 0u);
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
+    hipLaunchKernelGGL(k_begin_units, dim3(1), dim3(1), 0, stream, d_item, it, counter);
     switch (waves) {
       case 3:
-        hipLaunchKernelGGL(k_trace_units<3>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+        hipLaunchKernelGGL(k_trace_units<3>, dim3(grid), dim3(kPtBlock), 0, stream, d_item, n_jobs, counter, scratch,
                            lane_words, records, arrived);
         break;
       case 4:
-        hipLaunchKernelGGL(k_trace_units<4>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+        hipLaunchKernelGGL(k_trace_units<4>, dim3(grid), dim3(kPtBlock), 0, stream, d_item, n_jobs, counter, scratch,
                            lane_words, records, arrived);
         break;
       case 5:
-        hipLaunchKernelGGL(k_trace_units<5>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+        hipLaunchKernelGGL(k_trace_units<5>, dim3(grid), dim3(kPtBlock), 0, stream, d_item, n_jobs, counter, scratch,
                            lane_words, records, arrived);
         break;
       case 6:
-        hipLaunchKernelGGL(k_trace_units<6>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+        hipLaunchKernelGGL(k_trace_units<6>, dim3(grid), dim3(kPtBlock), 0, stream, d_item, n_jobs, counter, scratch,
                            lane_words, records, arrived);
         break;
       default:
-        hipLaunchKernelGGL(k_trace_units<2>, dim3(grid), dim3(kPtBlock), 0, stream, it, n_jobs, counter, scratch,
+        hipLaunchKernelGGL(k_trace_units<2>, dim3(grid), dim3(kPtBlock), 0, stream, d_item, n_jobs, counter, scratch,
                            lane_words, records, arrived);
     }
   }
