@@ -43,7 +43,7 @@ class DiffRenderer:
         self.env_size = (ew, eh)
         self.env = np.asarray(env, np.float32).copy()
         self._version = 0      # bumped by every parameter change
-        self.adjoint_occupancy = 3  # register variant of the replay kernel (2, 3, 4); 3 measured fastest
+        self.adjoint_occupancy = 4  # register variant of the replay kernel (2, 3, 4); 4 measured fastest
         self._record = None    # (spp, seed0, version, device buffer) of the last forward
 
     @property
