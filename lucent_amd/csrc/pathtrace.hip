@@ -37,6 +37,9 @@ constexpr int kPtBlock = 64;
 #ifndef LUCENT_SHADE_BATCH
 #define LUCENT_SHADE_BATCH 32
 #endif
+#ifndef LUCENT_TRACE_TURNS
+#define LUCENT_TRACE_TURNS 8
+#endif
 constexpr int kTile = 8;
 
 __global__ __launch_bounds__(kPtBlock) void k_trace_sample(PtScene sc, PtFrame f, PtFilm film, uint32_t* spill,
@@ -395,7 +398,7 @@ __device__ __forceinline__ bool shade_now(uint32_t state) {
 // wave's checks for shading and new work: the checks cost little, but a
 // loop of traversal steps alone keeps the wave on its hot code (measured:
 // 8 steps per check, 128 -> 116 ms for 1440x900 at 64 samples).
-constexpr int kTraceTurns = 8;
+constexpr int kTraceTurns = LUCENT_TRACE_TURNS;
 
 template <class Lane>
 __device__ __forceinline__ void trace_turns(Lane& lp) {
