@@ -316,9 +316,9 @@ typedef struct lucent_render_settings {
   float firefly_clamp;   /* 0 disables (uClamp) */
   uint32_t flags;        /* LUCENT_TRACE_* */
   uint32_t waves_per_cu; /* persistent launch width; 0 = 4 x occupancy */
-  uint32_t occupancy;    /* persistent kernel variant: registers for 2..6 waves per SIMD; 0 = 2.
+  uint32_t occupancy;    /* persistent kernel variant: registers for 2..6 waves per SIMD; 0 = 4.
                             4..6 spill registers to scratch but hide more memory latency.
-                            lucent_backward's replay takes 2..4 (4 measured fastest).
+                            lucent_backward's replay takes 2..4 (0 = 4, measured fastest).
                             Results do not depend on it (lucent_autotune_tiles picks one). */
 } lucent_render_settings;
 

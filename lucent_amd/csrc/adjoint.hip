@@ -725,7 +725,7 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
   const uint32_t tiles_x = (a.frame.width + kTile - 1) / kTile;
   const uint32_t tiles_y = (a.frame.height + kTile - 1) / kTile;
   const uint32_t n_jobs = tiles_x * tiles_y;
-  const uint32_t waves = (a.occupancy == 3 || a.occupancy == 4) ? a.occupancy : 2;
+  const uint32_t waves = (a.occupancy == 2 || a.occupancy == 3) ? a.occupancy : 4;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * 4 * waves);
   const size_t env_texels = size_t(sc.env_w) * sc.env_h;
   const size_t partials = size_t(n_jobs) * kLdsMaterials * kNumParams;

@@ -813,8 +813,12 @@ Status launch_trace_reprojected(Context* ctx, const PtScene& sc, const PtFrame& 
   return LUCENT_OK;
 }
 
+// The persistent kernels' register variant: 2..6 waves per SIMD as asked,
+// else 4 (the fastest on the measured workloads without autotuning).
+static uint32_t tile_waves(uint32_t occupancy) { return occupancy >= 2 && occupancy <= 6 ? occupancy : 4; }
+
 size_t tile_spill_bytes(const Context* ctx, const TileItem& item, const TileLaunch& cfg) {
-  const uint32_t waves = (cfg.occupancy >= 2 && cfg.occupancy <= 6) ? cfg.occupancy : 2;
+  const uint32_t waves = tile_waves(cfg.occupancy);
   const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t n_jobs = item.spp ? tile_jobs(item.frame.width, item.frame.height) : 0;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
@@ -835,7 +839,7 @@ uint32_t tile_jobs(uint32_t width, uint32_t height) {
 Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hipStream_t stream) {
   item.tiles_x = (item.frame.width + kTile - 1) / kTile;
   const uint32_t n_jobs = tile_jobs(item.frame.width, item.frame.height);
-  const uint32_t waves = (cfg.occupancy >= 2 && cfg.occupancy <= 6) ? cfg.occupancy : 2;
+  const uint32_t waves = tile_waves(cfg.occupancy);
   const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
   const uint32_t lane_words = tile_lane_words(pt::pt_spill_words(item.scene.bvh));
@@ -909,7 +913,7 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   }
   const uint32_t n_jobs = job_begin[n_items];
   if (n_jobs == 0) return LUCENT_OK;
-  const uint32_t waves = (cfg.occupancy >= 2 && cfg.occupancy <= 6) ? cfg.occupancy : 2;
+  const uint32_t waves = tile_waves(cfg.occupancy);
   const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
   const uint32_t lane_words = tile_lane_words(depth);
