@@ -887,10 +887,13 @@ lucent_status lucent_frame_graph_create(lucent_ctx* ctx, lucent_scene* scene, lu
     lucent_frame_graph_destroy(g);
     return st;
   };
-  if ((s = dev_alloc(&g->buf, 256 + spill)) != LUCENT_OK) return bail(s);
+  // [seed word | job counter | film descriptor slot | spill], 256-byte aligned parts.
+  const size_t item_bytes = (sizeof(TileItem) + 255) / 256 * 256;
+  if ((s = dev_alloc(&g->buf, 256 + item_bytes + spill)) != LUCENT_OK) return bail(s);
   uint32_t* seed_word = static_cast<uint32_t*>(g->buf);
   cfg.counter = seed_word + 1;
-  cfg.spill = reinterpret_cast<uint32_t*>(static_cast<char*>(g->buf) + 256);
+  cfg.item = reinterpret_cast<TileItem*>(static_cast<char*>(g->buf) + 256);
+  cfg.spill = reinterpret_cast<uint32_t*>(static_cast<char*>(g->buf) + 256 + item_bytes);
   cfg.spill_bytes = spill;
   it.seed_src = seed_word;
   hipError_t e = hipMemcpy(seed_word, &film->seed, sizeof(uint32_t), hipMemcpyHostToDevice);

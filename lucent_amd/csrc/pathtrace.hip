@@ -739,7 +739,7 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 // Each lane owns lane_words words of `scratch`: its light-sample slots, then
 // its stack's spill words.
 template <bool kSingle, int kWaves, bool kFused>
-__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_tiles(TileItem single, const TileItem* __restrict__ items,
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_tiles(const TileItem* __restrict__ single, const TileItem* __restrict__ items,
                                                          const uint32_t* __restrict__ job_begin, uint32_t n_items,
                                                          uint32_t n_jobs, uint32_t* __restrict__ counter,
                                                          uint32_t* scratch, uint32_t lane_words) {
@@ -751,9 +751,9 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
   while (job < n_jobs) {
     if constexpr (kSingle) {
       if constexpr (kFused)
-        job = run_item_fused(single, job, 0u, n_jobs, counter, st, slots);
+        job = run_item_fused(*single, job, 0u, n_jobs, counter, st, slots);
       else
-        job = run_item(single, job, 0u, n_jobs, counter, st);
+        job = run_item(*single, job, 0u, n_jobs, counter, st);
     } else {
       uint32_t item = 0, hi = n_items;
       while (hi - item > 1) {
@@ -925,6 +925,7 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
     // Caller-owned buffers (a captured graph): one film, nothing to upload.
     if (n_items != 1) return fail(LUCENT_E_INVALID, "caller-owned tile buffers take one film");
     if (cfg.spill_bytes < spill_bytes) return fail(LUCENT_E_INVALID, "caller-owned spill buffer too small");
+    if (!cfg.item) return fail(LUCENT_E_INVALID, "caller-owned tile buffers need a film descriptor slot");
   } else {
     if ((s = ensure_spill(ctx, spill_bytes)) != LUCENT_OK) return s;
     spill = static_cast<uint32_t*>(ctx->spill);
@@ -932,7 +933,7 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   // Device table: [counter | job_begin (n+1) | items], 256-byte aligned parts.
   const size_t begin_off = 256;
   const size_t items_off = (begin_off + job_begin.size() * sizeof(uint32_t) + 255) / 256 * 256;
-  const size_t bytes = items_off + (n_items > 1 ? size_t(n_items) * sizeof(TileItem) : 0);
+  const size_t bytes = items_off + size_t(n_items) * sizeof(TileItem);
   char* table = nullptr;
   if (!cfg.counter) {
     if ((s = ensure_tile_table(ctx, bytes)) != LUCENT_OK) return s;
@@ -948,7 +949,16 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
     if (e == hipSuccess) e = hipEventSynchronize(static_cast<hipEvent_t>(ctx->tile_event));  // `host` must outlive the copy
     if (e != hipSuccess) return fail_hip(e, "tile table upload");
   }
-  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
+  // One film: its descriptor goes to device memory with the counter reset
+  // (a kernel, so that a captured graph holds it), and the kernel reads it
+  // through a pointer (fewer registers than a by-value argument).
+  TileItem* d_single = nullptr;
+  if (n_items == 1) {
+    d_single = cfg.counter ? cfg.item : reinterpret_cast<TileItem*>(table + items_off);
+    hipLaunchKernelGGL(k_begin_units, dim3(1), dim3(1), 0, stream, d_single, items[0], counter);
+  } else {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
+  }
   const TileItem* d_items = n_items > 1 ? reinterpret_cast<const TileItem*>(table + items_off) : nullptr;
   const uint32_t* d_begin = n_items > 1 ? reinterpret_cast<const uint32_t*>(table + begin_off) : nullptr;
   if (n_items > 1 && cfg.fused && cfg.units) {
@@ -978,10 +988,10 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   }
   auto launch = [&](auto single_kernel, auto multi_kernel) {
     if (n_items == 1)
-      hipLaunchKernelGGL(single_kernel, dim3(grid), dim3(kPtBlock), 0, stream, items[0], nullptr, nullptr, 1u, n_jobs,
+      hipLaunchKernelGGL(single_kernel, dim3(grid), dim3(kPtBlock), 0, stream, d_single, nullptr, nullptr, 1u, n_jobs,
                          counter, spill, lane_words);
     else
-      hipLaunchKernelGGL(multi_kernel, dim3(grid), dim3(kPtBlock), 0, stream, TileItem{}, d_items, d_begin, n_items,
+      hipLaunchKernelGGL(multi_kernel, dim3(grid), dim3(kPtBlock), 0, stream, nullptr, d_items, d_begin, n_items,
                          n_jobs, counter, spill, lane_words);
   };
   auto launch_waves = [&](auto fused) {

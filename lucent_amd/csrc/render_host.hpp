@@ -31,10 +31,11 @@ Status launch_trace_reprojected(Context* ctx, const PtScene& sc, const PtFrame& 
 // Launch shape of the persistent tile kernel (0 = default).
 struct TileLaunch {
   uint32_t waves_per_cu = 0;  // persistent waves per CU (0: 4 x occupancy)
-  uint32_t occupancy = 0;     // register-allocation target in waves per SIMD: 2..6 (0: 2)
-  // Optional caller-owned job counter and spill buffer (one film only);
-  // otherwise the context's buffers are used.
+  uint32_t occupancy = 0;     // register-allocation target in waves per SIMD: 2..6 (0: 4)
+  // Optional caller-owned job counter, film descriptor slot and spill buffer
+  // (one film only); otherwise the context's buffers are used.
   uint32_t* counter = nullptr;
+  TileItem* item = nullptr;
   uint32_t* spill = nullptr;
   size_t spill_bytes = 0;
   // All rays of a lane through one traversal loop (false: each shadow ray
