@@ -443,11 +443,16 @@ lucent_status lucent_decode_hdr(const uint8_t *bytes, size_t length, uint32_t *w
  * with *denoise and lucent_display with *display into d_rgba8) captured
  * once into a HIP graph and replayed by lucent_frame_graph_launch with one
  * launch call.  The samples' seeds continue on the device (the graph
- * advances a device copy of the film's seed), so n replays leave the film,
- * the denoised image and the display buffer bit-identical to n calls of
- * lucent_trace_samples, lucent_denoise and lucent_display.  Camera,
- * settings, denoise iterations (0 = atrousLevels of the sample count after
- * the first replay) and display parameters are fixed at capture; the graph
+ * advances a device copy of the film's seed), so n replays leave the film
+ * and the display buffer bit-identical to n calls of lucent_trace_samples
+ * and lucent_display, and the denoised image bit-identical to n calls of
+ * lucent_denoise with the captured iteration count.  Camera, settings,
+ * denoise iterations and display parameters are fixed at capture; with
+ * iterations = 0 the count captured is atrousLevels of the sample count
+ * after the first replay, and it stays that count on later replays, where
+ * eager calls with iterations = 0 would re-evaluate it once the sample
+ * count crosses 4, 16, 64 or 256 (pass an explicit count to match eager
+ * calls across those thresholds).  The graph
  * holds the scene's, film's and output's device addresses.  Re-create it
  * after moving the camera, after lucent_scene_set_lighting /
  * lucent_scene_set_textures, or when any of them is destroyed.  A replay

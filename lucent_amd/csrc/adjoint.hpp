@@ -18,7 +18,7 @@ struct AdjointArgs {
   double* grad_materials = nullptr; // material_count * 5 (albedo rgb, roughness, metallic)
   double* grad_env = nullptr;       // env_w * env_h * 3, or null
   void* scratch = nullptr;          // adjoint_scratch_bytes(...)
-  uint32_t occupancy = 0;           // register-allocation variant of the replay kernel: 2 (default), 3 or 4
+  uint32_t occupancy = 0;           // register-allocation variant of the replay kernel: 4 (default; any value but 2 or 3), 2 or 3
 };
 
 struct DetachedArgs {

@@ -138,6 +138,8 @@ class LucentRenderer {
     this._inflight = 0;
   }
 
+  /** Every synchronous member that touches the context, scene or film
+   * refuses to run while an asynchronous call may be using them. */
   _sync() {
     if (this._inflight > 0) throw new Error('an asynchronous call is still running on this renderer');
   }
@@ -154,6 +156,7 @@ class LucentRenderer {
    * (packScene output), `scene` the Scene it came from (camera, lighting, ground).
    */
   loadScene(packed, scene) {
+    this._sync();
     const p = Object.assign({}, packed, {
       groundMaterial: scene.ground && scene.ground.enabled ? packed.groundMaterial : -1,
       groundHeight: scene.ground ? scene.ground.height : 0,
@@ -165,6 +168,7 @@ class LucentRenderer {
 
   /** uploadLighting (renderer.ts:355): {sun, sky, env}; env.map = {width, height, data} or null. */
   uploadLighting(scene) {
+    this._sync();
     addon.setLighting(this.ctx, this.scene, { sun: scene.sun, sky: scene.sky, env: scene.env });
     this.reset();
   }
@@ -176,6 +180,7 @@ class LucentRenderer {
    * preview scaling.
    */
   uploadCamera(cam, moving) {
+    this._sync();
     this.camera = cam;
     if (this.settings.temporal && this.historyCamera && addon.filmInfo(this.film).samples > 0) {
       this.reprojectFrom = this.historyCamera;
@@ -187,6 +192,7 @@ class LucentRenderer {
 
   /** reset (renderer.ts:463): the RNG stream keeps advancing. */
   reset() {
+    this._sync();
     addon.resetFilm(this.ctx, this.film);
     this.historyCamera = null;
     this.reprojectFrom = null;
@@ -212,6 +218,7 @@ class LucentRenderer {
    * `samples` samples and denoises when the denoise weight is above 0.
    */
   render(samples) {
+    this._sync();
     this.traceSamples(samples === undefined ? 1 : samples);
     const spp = addon.filmInfo(this.film).samples;
     if (this.settings.denoise && spp > 0 && denoiseWeight(spp) > 0 && this.denoisedFrame !== spp) {
@@ -228,6 +235,7 @@ class LucentRenderer {
    * @param {number} [viewHeight]
    */
   screenshot(viewWidth, viewHeight) {
+    this._sync();
     const spp = addon.filmInfo(this.film).samples;
     const w = this.settings.denoise && spp > 0 && this.denoisedFrame === spp ? denoiseWeight(spp) : 0;
     const vw = viewWidth || this.width, vh = viewHeight || this.height;
@@ -238,6 +246,7 @@ class LucentRenderer {
 
   /** benchmark (renderer.ts:745): warm-up sample, then `samples` timed. */
   benchmark(samples) {
+    this._sync();
     this.reset();
     this.traceSamples(1);
     let t0 = process.hrtime.bigint();
@@ -284,11 +293,13 @@ class LucentRenderer {
 
   /** The accumulation targets (rows bottom to top, like readPixels). */
   read() {
+    this._sync();
     return addon.readFilm(this.ctx, this.film);
   }
 
   /** stats (renderer.ts:832), the fields this path knows. */
   stats() {
+    this._sync();
     const f = addon.filmInfo(this.film);
     return { samples: f.samples, width: f.width, height: f.height, seed: f.seed };
   }
@@ -311,6 +322,7 @@ class LucentRenderer {
   }
 
   dispose() {
+    this._sync();
     this.scene = null;
     this.film = null;
     this.ctx = null;

@@ -1,4 +1,5 @@
 // Test driver for the N-API facade (run by tests/test_node_addon.py).
+//   node check.js guard <dir>   the async guard of every synchronous LucentRenderer member (CPU)
 //   node check.js bvh <dir>     buildBVH over <dir>/positions.f32 -> nodes.f32, tri.u32
 //   node check.js hits <dir>    GpuTraversal over positions + bvh, rays org.f64/dir.f64 -> t/u/v.f64, tri.u32
 //   node check.js hdr <dir>     parseHDR(in.hdr) -> rgb.f32
@@ -20,7 +21,24 @@ const write = (n, ta) => fs.writeFileSync(path.join(dir, n), Buffer.from(ta.buff
 const same = (a, b) => a.length === b.length && Buffer.from(a.buffer, a.byteOffset, a.byteLength)
   .equals(Buffer.from(b.buffer, b.byteOffset, b.byteLength));
 
-if (mode === 'bvh') {
+if (mode === 'guard') {
+  // CPU only: a renderer with an asynchronous call in flight refuses every
+  // synchronous member that touches its context, scene or film.
+  const r = Object.create(lucent.LucentRenderer.prototype);
+  r._inflight = 1;
+  r.settings = { temporal: true };
+  const calls = {
+    loadScene: () => r.loadScene({}, {}), uploadLighting: () => r.uploadLighting({}),
+    uploadCamera: () => r.uploadCamera({}), reset: () => r.reset(), traceSamples: () => r.traceSamples(1),
+    render: () => r.render(1), screenshot: () => r.screenshot(), benchmark: () => r.benchmark(1),
+    runDenoiser: () => r.runDenoiser(), read: () => r.read(), stats: () => r.stats(), dispose: () => r.dispose(),
+  };
+  const out = {};
+  for (const [name, fn] of Object.entries(calls)) {
+    try { fn(); out[name] = 'ran'; } catch (e) { out[name] = e.message; }
+  }
+  process.stdout.write(JSON.stringify(out) + '\n');
+} else if (mode === 'bvh') {
   const bvh = lucent.buildBVH(f32('positions.f32'), { maxLeafSize: 4, bins: 12 });
   write('nodes.f32', bvh.nodes);
   write('tri.u32', bvh.triIndices);

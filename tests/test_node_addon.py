@@ -44,6 +44,18 @@ def test_build_bvh_through_node(tmp_path):
     assert meta["asyncErr"] == "positions length 10 is not a multiple of 9"
 
 
+def test_sync_members_refuse_while_an_async_call_runs(tmp_path):
+    """A libuv worker may be running lucent_trace_samples on the renderer's
+    context: every synchronous member that touches the context, scene or film
+    (uploadLighting frees env buffers the worker reads; reset races the film's
+    seed and sample count) throws instead of running."""
+    out = node("guard", tmp_path)
+    for name, msg in out.items():
+        assert "asynchronous call is still running" in msg, f"{name}: {msg}"
+    assert set(out) >= {"loadScene", "uploadLighting", "uploadCamera", "reset", "traceSamples", "render",
+                        "screenshot", "benchmark", "runDenoiser", "read", "stats", "dispose"}
+
+
 @pytest.mark.gpu
 def test_intersect_through_node_bit_identical(tmp_path):
     rand = O.Lcg(42)
