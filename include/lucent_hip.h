@@ -189,6 +189,22 @@ lucent_status lucent_intersect_device(lucent_ctx *ctx, const lucent_scene *scene
                                       const float *d_dir3, uint32_t n, double tmin, double tmax, uint32_t flags,
                                       double *d_t, uint32_t *d_tri_id, double *d_u, double *d_v, void *stream);
 
+/* lucent_intersect_device without any device allocation: no counterpart in
+ * the reference.  For callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * traversal stacks spill into a fixed 64 MiB device array that the library
+ * holds per device for the life of the process (its address is looked up
+ * once per device in lucent_create); the launch's grid shrinks to what that
+ * array holds, and a tree whose stacks need more than it holds for one
+ * workgroup (about 128 * 8 * (3 * depth4 - 10) bytes) is rejected with
+ * LUCENT_E_UNSUPPORTED.  Results equal lucent_intersect_device's.  All calls
+ * on one device share the array: calls that overlap on one device (other
+ * streams, other contexts, other threads) race. */
+lucent_status lucent_intersect_device_noalloc(lucent_ctx *ctx, const lucent_scene *scene, const float *d_org3,
+                                              const float *d_dir3, uint32_t n, double tmin, double tmax,
+                                              uint32_t flags, double *d_t, uint32_t *d_tri_id, double *d_u,
+                                              double *d_v, void *stream);
+
 /* ------------------------------------------------------------------------
  * Lighting — the Scene's sun, sky and environment (src/scene/scene.ts:21-38,
  * src/scene/environment.ts:12-22) as uploaded by Renderer.uploadLighting

@@ -135,6 +135,7 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_intersect": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P]),
     "lucent_intersect_f64": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P]),
     "lucent_intersect_device": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P, _P]),
+    "lucent_intersect_device_noalloc": (_I32, [_P, _P, _P, _P, _U32, _D, _D, _U32, _P, _P, _P, _P, _P]),
     "lucent_scene_set_lighting": (_I32, [_P, _P, C.POINTER(LightingStruct)]),
     "lucent_env_sampler": (_I32, [_P, _U32, _U32, _U32, _P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_D)]),
     "lucent_alias_table": (_I32, [_P, _U32, _P, _P, _P, C.POINTER(_D)]),

@@ -252,6 +252,15 @@ lucent_status lucent_create(int device, lucent_ctx** out) {
     return fail_hip(e, "hipEventCreate");
   }
   ctx->tile_event = ev;
+  // The no-allocation query's scratch array: its address on this device.
+  Status st = noalloc_scratch_init(device);
+  if (st != LUCENT_OK) {
+    hipEventDestroy(ev);
+    hipFree(flags);
+    hipStreamDestroy(s);
+    delete ctx;
+    return st;
+  }
   *out = ctx;
   return LUCENT_OK;
 }
@@ -504,6 +513,18 @@ lucent_status lucent_intersect_device(lucent_ctx* ctx, const lucent_scene* scene
   if (!d_org3 || !d_dir3 || !d_t || !d_tri_id || !d_u || !d_v) return fail(LUCENT_E_INVALID, "null array");
   RayQuery q{d_org3, d_dir3, false, n, tmin, tmax, flags, d_t, d_tri_id, d_u, d_v};
   return launch_intersect(ctx, scene->bvh, q, stream ? stream : ctx->stream);
+}
+
+lucent_status lucent_intersect_device_noalloc(lucent_ctx* ctx, const lucent_scene* scene, const float* d_org3,
+                                              const float* d_dir3, uint32_t n, double tmin, double tmax,
+                                              uint32_t flags, double* d_t, uint32_t* d_tri_id, double* d_u,
+                                              double* d_v, void* stream) {
+  clear_error();
+  if (!ctx || !scene) return fail(LUCENT_E_INVALID, "null ctx/scene");
+  if (n == 0) return LUCENT_OK;
+  if (!d_org3 || !d_dir3 || !d_t || !d_tri_id || !d_u || !d_v) return fail(LUCENT_E_INVALID, "null array");
+  RayQuery q{d_org3, d_dir3, false, n, tmin, tmax, flags, d_t, d_tri_id, d_u, d_v};
+  return launch_intersect_noalloc(ctx, scene->bvh, q, stream ? stream : ctx->stream);
 }
 
 }  // extern "C"

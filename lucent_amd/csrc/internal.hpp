@@ -130,5 +130,12 @@ struct RayQuery {
   double* v;
 };
 Status launch_intersect(Context* ctx, const DevBvh& bvh, const RayQuery& q, void* stream);
+// The same query with the traversal stacks' spill area in a fixed device
+// array of kNoAllocScratchBytes (no device allocation); noalloc_scratch_init
+// looks up that array's address on `device` (current) once, at context set-up.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kMaxNoAllocDevices = 64;
+Status noalloc_scratch_init(int device);
+Status launch_intersect_noalloc(Context* ctx, const DevBvh& bvh, const RayQuery& q, void* stream);
 
 }  // namespace lucent

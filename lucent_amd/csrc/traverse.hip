@@ -29,6 +29,8 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "internal.hpp"
 
@@ -347,8 +349,16 @@ __global__ __launch_bounds__(kBlock) void k_intersect(DevBvh bvh, const void* __
   }
 }
 
+// Where a launch's traversal stacks spill: the context's growable area, or
+// a fixed area (the no-allocation entry point) that bounds the grid.
+struct SpillArea {
+  uint32_t* fixed = nullptr;  // null: the context's area, grown to fit
+  size_t fixed_bytes = 0;
+};
+
 template <Prec kPrec, bool kAny, bool kF64Rays, int kLds = kLdsStack>
-Status launch_variant(Context* ctx, const DevBvh& bvh, const RayQuery& q, hipStream_t stream) {
+Status launch_variant(Context* ctx, const DevBvh& bvh, const RayQuery& q, hipStream_t stream,
+                      const SpillArea& area) {
   auto kern = k_intersect<kPrec, kAny, kF64Rays, kLds>;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 4;
@@ -356,37 +366,104 @@ Status launch_variant(Context* ctx, const DevBvh& bvh, const RayQuery& q, hipStr
   uint64_t resident = uint64_t(per_cu) * uint64_t(ctx->compute_units > 0 ? ctx->compute_units : 256);
   uint32_t grid = uint32_t(blocks_needed < resident ? blocks_needed : resident);
   if (grid == 0) return LUCENT_OK;
-  const uint32_t depth = 2 * bvh.stack4;  // words per lane: (link, distance) entries
-  Status s = ensure_spill(ctx, size_t(grid) * kBlock * depth * sizeof(uint32_t));
-  if (s != LUCENT_OK) return s;
+  uint32_t* spill = nullptr;
+  uint32_t depth = 0;  // words per lane
+  if (!area.fixed) {
+    depth = 2 * bvh.stack4;  // (link, distance) entries
+    Status s = ensure_spill(ctx, size_t(grid) * kBlock * depth * sizeof(uint32_t));
+    if (s != LUCENT_OK) return s;
+    spill = static_cast<uint32_t*>(ctx->spill);
+  } else {
+    // Only entries beyond the kLds held in LDS reach the spill area: a push
+    // spills entry `base` when kLds entries are above it, so spill indexes
+    // stay below 2 * (stack4 - kLds).  The grid-stride loop lets a smaller
+    // grid cover every ray, so the grid shrinks to what the area holds.
+    depth = bvh.stack4 > uint32_t(kLds) ? 2 * (bvh.stack4 - kLds) : 2;
+    const size_t per_block = size_t(kBlock) * depth * sizeof(uint32_t);
+    const size_t fit = area.fixed_bytes / per_block;
+    if (fit == 0)
+      return fail(LUCENT_E_UNSUPPORTED, "lucent_intersect_device_noalloc: the tree needs " +
+                                            std::to_string(per_block) + " B of stack spill per block, above the " +
+                                            std::to_string(area.fixed_bytes) + " B scratch area");
+    if (grid > fit) grid = uint32_t(fit);
+    spill = area.fixed;
+  }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, bvh, q.org, q.dir, q.n, q.tmin, q.tmax, q.t, q.id,
-                     q.u, q.v, static_cast<uint32_t*>(ctx->spill), depth);
+                     q.u, q.v, spill, depth);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "k_intersect launch");
   return LUCENT_OK;
 }
 
-}  // namespace
-
-Status launch_intersect(Context* ctx, const DevBvh& bvh, const RayQuery& q, void* stream_) {
-  hipStream_t stream = static_cast<hipStream_t>(stream_);
+Status launch_intersect_in(Context* ctx, const DevBvh& bvh, const RayQuery& q, hipStream_t stream,
+                          const SpillArea& a) {
   const bool any = (q.flags & LUCENT_QUERY_ANY) != 0;
   const bool glsl = (q.flags & LUCENT_PREC_GLSL_FP32) != 0;
   if (q.flags & LUCENT_FLAG_TINY_LDS_STACK) {
     // Test aid: a 2-entry LDS ring forces the global spill path on any tree.
     if (q.f64_rays || any || glsl) return fail(LUCENT_E_INVALID, "LUCENT_FLAG_TINY_LDS_STACK: exact closest-hit float32 rays only");
-    return launch_variant<Prec::kExact, false, false, 2>(ctx, bvh, q, stream);
+    return launch_variant<Prec::kExact, false, false, 2>(ctx, bvh, q, stream, a);
   }
   if (q.f64_rays) {
-    if (glsl) return any ? launch_variant<Prec::kGlsl, true, true>(ctx, bvh, q, stream)
-                         : launch_variant<Prec::kGlsl, false, true>(ctx, bvh, q, stream);
-    return any ? launch_variant<Prec::kExact, true, true>(ctx, bvh, q, stream)
-               : launch_variant<Prec::kExact, false, true>(ctx, bvh, q, stream);
+    if (glsl) return any ? launch_variant<Prec::kGlsl, true, true>(ctx, bvh, q, stream, a)
+                         : launch_variant<Prec::kGlsl, false, true>(ctx, bvh, q, stream, a);
+    return any ? launch_variant<Prec::kExact, true, true>(ctx, bvh, q, stream, a)
+               : launch_variant<Prec::kExact, false, true>(ctx, bvh, q, stream, a);
   }
-  if (glsl) return any ? launch_variant<Prec::kGlsl, true, false>(ctx, bvh, q, stream)
-                       : launch_variant<Prec::kGlsl, false, false>(ctx, bvh, q, stream);
-  return any ? launch_variant<Prec::kExact, true, false>(ctx, bvh, q, stream)
-             : launch_variant<Prec::kExact, false, false>(ctx, bvh, q, stream);
+  if (glsl) return any ? launch_variant<Prec::kGlsl, true, false>(ctx, bvh, q, stream, a)
+                       : launch_variant<Prec::kGlsl, false, false>(ctx, bvh, q, stream, a);
+  return any ? launch_variant<Prec::kExact, true, false>(ctx, bvh, q, stream, a)
+             : launch_variant<Prec::kExact, false, false>(ctx, bvh, q, stream, a);
+}
+
+}  // namespace
+
+Status launch_intersect(Context* ctx, const DevBvh& bvh, const RayQuery& q, void* stream_) {
+  return launch_intersect_in(ctx, bvh, q, static_cast<hipStream_t>(stream_), SpillArea{});
+}
+
+}  // namespace lucent
+
+// ---------------------------------------------------------------------------
+// The no-allocation query (lucent_intersect_device_noalloc): its traversal
+// stacks spill into this fixed device array instead of the context's
+// growable area, so a call allocates nothing.  Every device the library runs
+// on holds its own copy for the life of the process; calls on one device
+// share it, so calls that overlap on one device race.  Host code uses the
+// address hipGetSymbolAddress returns for the current device (kept per
+// device by noalloc_scratch_init), never the array's name.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint32_t scratch[lucent::kNoAllocScratchBytes / sizeof(uint32_t)];
+
+namespace lucent {
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxNoAllocDevices] = {};
+}  // namespace
+
+Status noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxNoAllocDevices) return fail(LUCENT_E_UNSUPPORTED, "device index above the no-allocation table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return LUCENT_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(scratch)");
+  if (!p) return fail(LUCENT_E_HIP, "hipGetSymbolAddress(scratch) returned null");
+  g_scratch_ptr[device] = p;
+  return LUCENT_OK;
+}
+
+Status launch_intersect_noalloc(Context* ctx, const DevBvh& bvh, const RayQuery& q, void* stream_) {
+  void* p = nullptr;
+  if (ctx->device >= 0 && ctx->device < kMaxNoAllocDevices) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch_ptr[ctx->device];
+  }
+  if (!p) return fail(LUCENT_E_HIP, "no-allocation scratch area was not set up for this device");
+  SpillArea a;
+  a.fixed = static_cast<uint32_t*>(p);
+  a.fixed_bytes = kNoAllocScratchBytes;
+  return launch_intersect_in(ctx, bvh, q, static_cast<hipStream_t>(stream_), a);
 }
 
 }  // namespace lucent

@@ -182,13 +182,15 @@ class Scene:
         return Hits(t, tri, u, v)
 
     def intersect_device(self, org, dirs, t, tri, u, v, tmin: float = 1e-5, tmax: float = float("inf"),
-                         precision: str = "exact", any_hit: bool = False, stream: Optional[int] = None) -> None:
-        """Device-pointer form: arguments are torch CUDA tensors (or raw int addresses)."""
+                         precision: str = "exact", any_hit: bool = False, stream: Optional[int] = None,
+                         noalloc: bool = False, flags: int = 0) -> None:
+        """Device-pointer form: arguments are torch CUDA tensors (or raw int addresses).
+        noalloc=True calls the variant that allocates no device memory."""
         def addr(x):
             return x if isinstance(x, int) else x.data_ptr()
         n = (org.numel() // 3) if hasattr(org, "numel") else None
         if n is None:
             raise ValueError("pass torch tensors for the device form")
-        _native.check(_native.lib().lucent_intersect_device(
-            self.ctx.handle, self.handle, addr(org), addr(dirs), n, tmin, tmax, self._flags(precision, any_hit),
-            addr(t), addr(tri), addr(u), addr(v), stream))
+        fn = _native.lib().lucent_intersect_device_noalloc if noalloc else _native.lib().lucent_intersect_device
+        _native.check(fn(self.ctx.handle, self.handle, addr(org), addr(dirs), n, tmin, tmax,
+                         self._flags(precision, any_hit) | flags, addr(t), addr(tri), addr(u), addr(v), stream))
