@@ -190,6 +190,40 @@ class Renderer:
         self.upload_textures(scene)
         self.upload_lighting(scene)
 
+    def pick_distance(self, origin, direction) -> Optional[float]:
+        """pickDistance (scene.ts:327-341): the nearest hit along the ray —
+        triangles through the exact GPU query (intersectBVH's FP64 hit
+        record), then the spheres with tMax = that distance, then the ground
+        plane if it is enabled — or None."""
+        o = np.asarray(origin, np.float64).reshape(1, 3)
+        d = np.asarray(direction, np.float64).reshape(1, 3)
+        best = math.inf
+        if self.packed.tri_count > 0:
+            hit = self.device_scene.intersect(o, d)
+            if hit.tri[0] != 0xFFFFFFFF:
+                best = float(hit.t[0])
+        for s in self.scene.spheres:
+            # intersectSphere (ray.ts:48-67), tMin 1e-5.
+            oc = o[0] - np.asarray(s.center, np.float64)
+            b = float(oc @ d[0])
+            c = float(oc @ oc) - s.radius * s.radius
+            disc = b * b - c
+            if disc < 0:
+                continue
+            sq = math.sqrt(disc)
+            t = -b - sq
+            if t <= 1e-5 or t >= best:
+                t = -b + sq
+                if t <= 1e-5 or t >= best:
+                    continue
+            best = t
+        g = self.scene.ground
+        if g.enabled and abs(d[0, 1]) > 1e-9:
+            t = (g.height - o[0, 1]) / d[0, 1]
+            if 1e-5 < t < best:
+                best = t
+        return best if math.isfinite(best) else None
+
     def upload_textures(self, scene: Scene) -> None:
         """The texture arrays of loadScene (renderer.ts:295, 315-335)."""
         size, color, data = texture_arrays(scene, self.packed)

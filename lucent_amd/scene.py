@@ -473,27 +473,71 @@ def _next_pow2(n: int) -> int:
     return p
 
 
-def resample_rgba8(img: np.ndarray, size: int) -> np.ndarray:
-    """Bilinear resample of an RGBA8 image to size x size (the role of the
-    resample pass in createTextureArray, context.ts:243-250)."""
+def _lerp_weights(n_src: int, n_dst: int) -> np.ndarray:
+    """(n_dst, n_src) LINEAR-filter weights sampling n_src texels at the
+    centres of n_dst target texels, clamped to the edge."""
+    x = (np.arange(n_dst) + 0.5) * (n_src / n_dst) - 0.5
+    i0 = np.floor(x).astype(np.int64)
+    f = x - i0
+    w = np.zeros((n_dst, n_src))
+    rows = np.arange(n_dst)
+    np.add.at(w, (rows, np.clip(i0, 0, n_src - 1)), 1.0 - f)
+    np.add.at(w, (rows, np.clip(i0 + 1, 0, n_src - 1)), f)
+    return w
+
+
+def _halve(a: np.ndarray) -> np.ndarray:
+    """Next mip level: 2x2 box average (edge texels repeat on odd sizes)."""
+    h, w = a.shape[:2]
+    nh, nw = max(1, h // 2), max(1, w // 2)
+    y0 = np.minimum(2 * np.arange(nh), h - 1)
+    y1 = np.minimum(2 * np.arange(nh) + 1, h - 1)
+    x0 = np.minimum(2 * np.arange(nw), w - 1)
+    x1 = np.minimum(2 * np.arange(nw) + 1, w - 1)
+    return (a[y0][:, x0] + a[y0][:, x1] + a[y1][:, x0] + a[y1][:, x1]) / 4
+
+
+def resample_rgba8(img: np.ndarray, size: int, srgb: bool = False) -> np.ndarray:
+    """One texture-array layer as createTextureArray draws it
+    (context.ts:225-273, resample.frag): `texture(src, fragCoord / size)`
+    over a mipmapped, edge-clamped source, LINEAR_MIPMAP_LINEAR.  An image of
+    the layer size is copied; a smaller one is bilinearly magnified; a
+    larger one is sampled from its box-filtered mip chain at
+    lod = log2(max(w, h) / size), blending the two nearest levels.  sRGB
+    layers filter in linear space, as SRGB8_ALPHA8 textures do."""
     img = np.asarray(img, np.uint8)
     h, w = img.shape[:2]
     if h == size and w == size:
-        return img
-    ys = (np.arange(size) + 0.5) * h / size - 0.5
-    xs = (np.arange(size) + 0.5) * w / size - 0.5
-    y0 = np.floor(ys).astype(np.int64)
-    x0 = np.floor(xs).astype(np.int64)
-    ty = (ys - y0)[:, None, None]
-    tx = (xs - x0)[None, :, None]
-    f = img.astype(np.float64)
+        return img.copy()
+    # Unorm channels filter in units of one code (exact for the dyadic
+    # weights of 2x scaling, so ties round half to even alike everywhere);
+    # sRGB colour channels filter decoded to linear [0, 1].
+    v = img.astype(np.float64)
+    if srgb:
+        c = v[..., :3] / 255.0
+        v[..., :3] = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+    lod = math.log2(max(w, h) / size)
 
-    def at(yy, xx):
-        return f[np.mod(yy, h)][:, np.mod(xx, w)]
+    def sample(level):
+        wy = _lerp_weights(level.shape[0], size)
+        wx = _lerp_weights(level.shape[1], size)
+        return np.einsum("yi,ijc,xj->yxc", wy, level, wx)
 
-    top = at(y0, x0) * (1 - tx) + at(y0, x0 + 1) * tx
-    bot = at(y0 + 1, x0) * (1 - tx) + at(y0 + 1, x0 + 1) * tx
-    return np.clip(np.rint(top * (1 - ty) + bot * ty), 0, 255).astype(np.uint8)
+    if lod <= 0:
+        out = sample(v)
+    else:
+        lo = int(math.floor(lod))
+        levels = [v]
+        while len(levels) <= lo + 1 and max(levels[-1].shape[:2]) > 1:
+            levels.append(_halve(levels[-1]))
+        a = levels[min(lo, len(levels) - 1)]
+        b = levels[min(lo + 1, len(levels) - 1)]
+        t = lod - lo
+        out = sample(a) * (1 - t) + sample(b) * t
+    if srgb:
+        c = np.clip(out[..., :3], 0.0, 1.0)
+        out[..., :3] = np.where(c < 0.0031308, c * 12.92, 1.055 * np.power(c, 1 / 2.4) - 0.055) * 255.0
+    return np.rint(np.clip(out, 0.0, 255.0)).astype(np.uint8)
 
 
 def texture_arrays(scene: Scene, packed: PackedScene):
@@ -509,10 +553,10 @@ def texture_arrays(scene: Scene, packed: PackedScene):
     while size > 256 and len(used) * size * size * 4 > TEXTURE_BUDGET_BYTES:
         size //= 2
 
-    def array(layers):
+    def array(layers, srgb):
         if not layers:
             return np.zeros(0, np.uint32)
-        out = np.stack([resample_rgba8(scene.textures[i], size) for i in layers])
+        out = np.stack([resample_rgba8(scene.textures[i], size, srgb) for i in layers])
         return np.ascontiguousarray(out).view(np.uint32).reshape(-1)
 
-    return size, array(packed.color_layers), array(packed.data_layers)
+    return size, array(packed.color_layers, True), array(packed.data_layers, False)

@@ -335,6 +335,22 @@ def eval_opaque(mat40, wo, wi, lut=None):
     return f, pdf.value
 
 
+def eval_opaque_batch(mat40, wo, wi, lut=None):
+    """eval_opaque for arrays of local-frame directions (n, 3); returns (f (n, 3), pdf (n,))."""
+    l = _pt_lib()
+    l.or_eval_opaque_batch.argtypes = [_P, _P, _U32, _P, _P, _P, _P]
+    l.or_eval_opaque_batch.restype = None
+    m = np.ascontiguousarray(mat40, np.float32)
+    lut = sheen_lut_half() if lut is None else np.ascontiguousarray(lut, np.float32)
+    wo = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float64), np.shape(wi)), np.float64)
+    wi = np.ascontiguousarray(wi, np.float64)
+    n = wi.shape[0]
+    f = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    l.or_eval_opaque_batch(_ptr(m), _ptr(lut), n, _ptr(wo), _ptr(wi), _ptr(f), _ptr(pdf))
+    return f, pdf
+
+
 def sample_opaque(mat40, wo, state, lut=None):
     m = np.ascontiguousarray(mat40, np.float32)
     lut = sheen_lut_half() if lut is None else np.ascontiguousarray(lut, np.float32)
@@ -368,10 +384,31 @@ def camera_basis(position, target, fov):
     return f, r, u, math.tan(fov * math.pi / 360)
 
 
-class PtOracle:
-    """Holds the arrays an OrScene points to, built from a packed scene and its lighting."""
+def bake_sky(sky, sun, width: int = 1024) -> np.ndarray:
+    """bakeSky (environment.ts:106-132), float32 (width/2, width, 3) rows top to bottom."""
+    l = _pt_lib()
+    if not getattr(l, "_sky_ready", False):
+        l.or_bake_sky.argtypes = [C.c_int, _P, _P, _P, _D, _D, _D, _P, _D, _D, C.c_int, _P]
+        l.or_bake_sky.restype = None
+        l.or_env_pdf_dir.argtypes = [C.POINTER(OrScene), _P]
+        l.or_env_pdf_dir.restype = _D
+        l._sky_ready = True
+    out = np.zeros((width // 2, width, 3), np.float32)
+    z, h, g, c = (np.ascontiguousarray(x, np.float64) for x in (sky.zenith, sky.horizon, sky.ground, sun.color))
+    l.or_bake_sky(1 if sky.enabled else 0, _ptr(z), _ptr(h), _ptr(g), sky.intensity, sun.elevation, sun.azimuth,
+                  _ptr(c), sun.intensity, sun.angular_radius, width, _ptr(out))
+    return out
 
-    def __init__(self, packed, scene, textures=None):
+
+class PtOracle:
+    """Holds the arrays an OrScene points to, all built by the oracle itself
+    from a scene description: packScene (oracle/packing.py), the oracle's
+    buildBVH, its texture layers, env sampler, sky bake and sheen LUT, and
+    the lighting uniforms as Renderer.uploadLighting sets them
+    (renderer.ts:355-413)."""
+
+    def __init__(self, scene, bvh=None):
+        from . import packing
         keep = []
 
         def arr(a, dt):
@@ -379,6 +416,8 @@ class PtOracle:
             keep.append(x)
             return x.ctypes.data
 
+        packed = packing.pack_scene(scene, bvh)
+        self.packed = packed
         s = OrScene()
         s.geo = arr(packed.tri_geometry, np.float32)
         s.attr = arr(packed.tri_attributes, np.float32)
@@ -395,7 +434,6 @@ class PtOracle:
         s.light_count = packed.light_count
         s.ground_material = packed.ground_material_active
         s.ground_height = packed.ground_height
-        # Lighting as Renderer.uploadLighting sets it (renderer.ts:355-413).
         sun, env = scene.sun, scene.env
         el, az = math.radians(sun.elevation), math.radians(sun.azimuth)
         sd = [math.cos(el) * math.sin(az), math.sin(el), math.cos(el) * math.cos(az)]
@@ -406,37 +444,56 @@ class PtOracle:
         s.sun_radiance[:] = [c * sun.intensity / solid for c in sun.color]
         s.sun_cos = ca
         s.sun_pdf = 1 / solid
-        if env.map is None:
-            raise NotImplementedError("oracle scenes use an explicit environment map")
-        w, h, rgb = env.map
-        rgb = np.ascontiguousarray(rgb, np.float32)
+        if env.map is not None:
+            w, h, rgb = env.map
+            rgb = np.ascontiguousarray(rgb, np.float32)
+            rot, intensity = math.radians(env.rotation), env.intensity
+        else:
+            # The procedural sky, baked in world orientation at unit intensity.
+            rgb = bake_sky(scene.sky, sun)
+            h, w = rgb.shape[:2]
+            rot, intensity = 0.0, 1.0
+        self.env_rgb = rgb
         table, gw, gh, total = env_sampler(rgb, w, h, 1024)
         s.env_rgb = arr(rgb, np.float32)
         s.env_w, s.env_h = w, h
         s.env_table = arr(table, np.float32)
         s.grid_w, s.grid_h = gw, gh
-        rot = math.radians(env.rotation)
         s.env_cos, s.env_sin = math.cos(rot), math.sin(rot)
-        s.env_intensity = env.intensity
+        s.env_intensity = intensity
         s.env_sampling = 1 if total > 0 else 0
         s.env_background = 1 if env.background else 0
         s.backdrop[:] = list(env.backdrop)
         s.sheen_lut = arr(sheen_lut_half(), np.float32)
-        if textures is None and (getattr(packed, "color_layers", None) or getattr(packed, "data_layers", None)):
-            # The same layer arrays the renderer uploads (a data format, built
-            # from the scene's images by the host packer).
-            from lucent_amd.scene import texture_arrays
-            textures = texture_arrays(scene, packed)
-        if textures is not None:
-            size, color, data = textures
+        size, color, data = packing.texture_layers(scene, packed)
+        if size:
             s.color_maps = arr(color, np.uint32)
             s.data_maps = arr(data, np.uint32)
             s.tex_size = size
             s.color_layers = color.size // (size * size)
             s.data_layers = data.size // (size * size)
+        self.textures = (size, color, data)
         self.scene = s
         self.keep = keep
         self.env_table = table
+
+    def env_pdf_batch(self, dirs) -> np.ndarray:
+        """envPdf of many world directions (n, 3)."""
+        l = _pt_lib()
+        l.or_env_pdf_batch.argtypes = [C.POINTER(OrScene), _U32, _P, _P]
+        l.or_env_pdf_batch.restype = None
+        d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+        out = np.zeros(d.shape[0])
+        l.or_env_pdf_batch(C.byref(self.scene), d.shape[0], _ptr(d), _ptr(out))
+        return out
+
+    def env_pdf(self, d) -> float:
+        """envPdf of world direction d (lights.glsl:49-56)."""
+        l = _pt_lib()
+        l.or_env_pdf_dir.argtypes = [C.POINTER(OrScene), _P]
+        l.or_env_pdf_dir.restype = _D
+        dd = np.ascontiguousarray(d, np.float64)
+        return l.or_env_pdf_dir(C.byref(self.scene), _ptr(dd))
 
     def frame(self, camera, width, height, max_bounces=8, clamp=0.0) -> OrFrame:
         f = OrFrame()

@@ -524,6 +524,46 @@ static double env_pdf(const or_scene *s, d3 d) {
   if (cy > s->grid_h - 1) cy = s->grid_h - 1;
   return s->env_table[((size_t)cy * s->grid_w + cx) * 4 + 2] / (2 * PI * PI * st);
 }
+/* envPdf of a world direction (lights.glsl:49-56), exported for the
+ * sphere-integral check. */
+double or_env_pdf_dir(const or_scene *s, const double *d) { return env_pdf(s, V(d[0], d[1], d[2])); }
+void or_env_pdf_batch(const or_scene *s, uint32_t n, const double *d, double *out) {
+  for (uint32_t i = 0; i < n; i++) out[i] = env_pdf(s, V(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
+}
+
+/* bakeSky (environment.ts:106-132): the procedural gradient sky plus the
+ * sun's glow (not its disc) in a width x width/2 equirect map, float32 rows
+ * top to bottom.  Each stage stores into the float32 map as the reference's
+ * Float32Array does: the gradient, then + colour * glow, then * intensity.
+ * mix(a, b, t) = a + (b - a) * t as in the reference (not GLSL's form). */
+void or_bake_sky(int enabled, const double *zenith, const double *horizon, const double *ground, double intensity,
+                 double sun_elev_deg, double sun_azim_deg, const double *sun_color, double sun_intensity,
+                 double sun_angular_radius_deg, int width, float *out) {
+  int height = width / 2;
+  for (size_t i = 0; i < (size_t)width * height * 3; i++) out[i] = 0.0f;
+  if (!enabled || intensity <= 0) return;
+  double el = sun_elev_deg * PI / 180, az = sun_azim_deg * PI / 180;
+  d3 sd = V(cos(el) * sin(az), sin(el), cos(el) * cos(az));
+  int sun_on = sun_intensity > 0 && sun_elev_deg > -sun_angular_radius_deg;
+  for (int y = 0; y < height; y++)
+    for (int x = 0; x < width; x++) {
+      double u = (x + 0.5) / width, v = (y + 0.5) / height;
+      double phi = (u - 0.5) * 2 * PI, th = v * PI, st = sin(th);
+      d3 d = V(st * cos(phi), cos(th), st * sin(phi));
+      double t = d.y;
+      float *o = out + ((size_t)y * width + x) * 3;
+      for (int c = 0; c < 3; c++)
+        o[c] = (float)(t >= 0 ? horizon[c] + (zenith[c] - horizon[c]) * pow(t, 0.55)
+                              : horizon[c] + (ground[c] - horizon[c]) * mind(1, -t * 4));
+      if (sun_on) {
+        double sdot = maxd(0, d.x * sd.x + d.y * sd.y + d.z * sd.z);
+        double glow = 0.12 * pow(sdot, 24) + 0.5 * pow(sdot, 400);
+        for (int c = 0; c < 3; c++) o[c] = (float)((double)o[c] + sun_color[c] * glow);
+      }
+      for (int c = 0; c < 3; c++) o[c] = (float)((double)o[c] * intensity);
+    }
+}
+
 static lsample sample_env(const or_scene *s, rng_t *r) {
   lsample ls;
   int n = s->grid_w * s->grid_h;
@@ -1295,6 +1335,13 @@ void or_eval_opaque(const float *mat40, const float *sheen_lut, const double *wo
 }
 
 /* One opaque BSDF sample (weight f*cos/pdf), drawing from a PCG stream. */
+/* or_eval_opaque over n direction pairs (local frame, normal +z), for
+ * quadratures of the BSDF. */
+void or_eval_opaque_batch(const float *mat40, const float *sheen_lut, uint32_t n, const double *wo,
+                          const double *wi, double *f3, double *pdf) {
+  for (uint32_t i = 0; i < n; i++) or_eval_opaque(mat40, sheen_lut, wo + 3 * i, wi + 3 * i, f3 + 3 * i, pdf + i);
+}
+
 void or_sample_opaque(const float *mat40, const float *sheen_lut, const double *wo, uint32_t rng_state,
                       double *dir, double *w3, double *pdf) {
   or_scene sc;

@@ -26,7 +26,7 @@ from lucent_amd import workloads  # noqa: E402
 def main():
     sc = workloads.knot_scene()
     packed = S.pack_scene(sc)
-    o = O.PtOracle(packed, sc)
+    o = O.PtOracle(sc)
     W, H = workloads.PT_WIDTH, workloads.PT_HEIGHT
     f = o.frame(sc.camera, W, H, workloads.PT_DEPTH, 0.0)
     yy, xx = np.mgrid[H // 2 - 32:H // 2 + 32, W // 2 - 32:W // 2 + 32]

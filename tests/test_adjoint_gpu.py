@@ -35,7 +35,7 @@ def test_matches_fp64_adjoint_on_64_pixels(diff):
     dl[xy[:, 1], xy[:, 0]] = vals
     gm, ge = diff.backward(dl, SPP, 0)
     packed = diff.renderer.packed
-    o = O.PtOracle(packed, diff.scene)
+    o = O.PtOracle(diff.scene)
     f = o.frame(diff.scene.camera, W, H, DEPTH, 0.0)
     om, oe = o.adjoint_pixels(f, 0, SPP, xy, vals, packed.material_count)
     scale = np.abs(om).max()

@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from oracle import packing as OP
 from lucent_amd import scene as S
 from lucent_amd import workloads
 
@@ -36,8 +37,8 @@ def small_scene():
 
 def setup(depth=2):
     sc = small_scene()
-    packed = S.pack_scene(sc)
-    o = O.PtOracle(packed, sc)
+    o = O.PtOracle(sc)
+    packed = o.packed
     W, H = 24, 16
     f = o.frame(sc.camera, W, H, depth, 0.0)
     yy, xx = np.mgrid[0:H, 0:W]
@@ -87,14 +88,14 @@ def test_albedo_gradient_equals_finite_difference_at_depth_2():
                   for j, m in enumerate(mats)]
             mm = [replace(m, albedo=tuple(a - (h if (j == mi and k == c) else 0) for k, a in enumerate(m.albedo)))
                   for j, m in enumerate(mats)]
-            P = S.pack_materials(mp)
-            M = S.pack_materials(mm)
-            hp = float(P[mi * 40 + c]) - float(S.pack_materials(mats)[mi * 40 + c])
-            hm = float(S.pack_materials(mats)[mi * 40 + c]) - float(M[mi * 40 + c])
+            P = OP.pack_materials(mp)
+            M = OP.pack_materials(mm)
+            hp = float(P[mi * 40 + c]) - float(OP.pack_materials(mats)[mi * 40 + c])
+            hm = float(OP.pack_materials(mats)[mi * 40 + c]) - float(M[mi * 40 + c])
             o.set_materials(P)
             fp = objective(o, f, xy, dl, spp)
             o.set_materials(M)
             fm = objective(o, f, xy, dl, spp)
-            o.set_materials(S.pack_materials(mats))
+            o.set_materials(OP.pack_materials(mats))
             fd = (fp - fm) / (hp + hm)
             assert fd == pytest.approx(gm[mi, c], rel=1e-4, abs=1e-7), (mi, c, fd, gm[mi, c])

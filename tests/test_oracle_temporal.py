@@ -20,7 +20,6 @@ import numpy as np
 import oracle as O
 from lucent_amd import scene as S
 from lucent_amd import workloads
-from lucent_amd.scene import pack_scene
 
 W, H = 24, 16
 
@@ -68,7 +67,7 @@ def fresh_samples(o, f, seed, xy):
 def test_sky_reprojection_lands_on_the_jittered_sample_position():
     c = cam((0.0, 0.0, 5.0), (0.0, 0.0, -10.0))
     sc = sky_scene(c)
-    o = O.PtOracle(pack_scene(sc), sc)
+    o = O.PtOracle(sc)
     f = o.frame(c, W, H, 3, 0.0)
     film = uniform_film(W, H, (0, 0, 0, 0), 0.0, (0, 0, 1, 1e4), 5.0)
     yy, xx = np.mgrid[0:H, 0:W]
@@ -97,7 +96,7 @@ def test_sky_reprojection_lands_on_the_jittered_sample_position():
 def test_history_is_capped():
     c = cam((0.0, 0.0, 5.0), (0.0, 0.0, -10.0))
     sc = sky_scene(c)
-    o = O.PtOracle(pack_scene(sc), sc)
+    o = O.PtOracle(sc)
     f = o.frame(c, W, H, 3, 0.0)
     film = uniform_film(W, H, (0.25, 0.5, 0.75, 0.1), 0.0, (0, 0, 1, 1e4), 100.0)
     xy = all_xy(W, H)[:40]
@@ -112,7 +111,7 @@ def test_turned_away_view_starts_over():
     c = cam((0.0, 0.0, 5.0), (0.0, 0.0, -10.0))
     back = cam((0.0, 0.0, 5.0), (0.0, 0.0, 20.0))
     sc = sky_scene(c)
-    o = O.PtOracle(pack_scene(sc), sc)
+    o = O.PtOracle(sc)
     f = o.frame(c, W, H, 3, 0.0)
     fb = o.frame(back, W, H, 3, 0.0)
     film = uniform_film(W, H, (9, 9, 9, 81), 0.0, (0, 0, 1, 1e4), 4.0)
@@ -126,7 +125,7 @@ def test_turned_away_view_starts_over():
 
 def test_surface_over_a_sky_history_is_disoccluded():
     sc = plane_scene(S.matte((0.6, 0.5, 0.4), 1.0))
-    o = O.PtOracle(pack_scene(sc), sc)
+    o = O.PtOracle(sc)
     f = o.frame(sc.camera, W, H, 2, 0.0)
     film = uniform_film(W, H, (9, 9, 9, 81), 0.0, (0, 0, 1, 1e4), 4.0)  # coverage 0 everywhere
     xy = all_xy(W, H)
@@ -151,7 +150,7 @@ def test_reuse_caps_history_by_material():
     xy = all_xy(W, H)
     interior = (xy[:, 0] > 0) & (xy[:, 0] < W - 1) & (xy[:, 1] > 0) & (xy[:, 1] < H - 1)
     sc = plane_scene(S.matte((0.6, 0.5, 0.4), 1.0))
-    o = O.PtOracle(pack_scene(sc), sc)
+    o = O.PtOracle(sc)
     f = o.frame(sc.camera, W, H, 2, 0.0)
     out, ok = o.trace_reprojected(f, surface_history(o, f, 100, 4), f, 8.0, 104, xy)
     assert ok[interior].all()
@@ -160,7 +159,7 @@ def test_reuse_caps_history_by_material():
     t = (r - 0.1) / 0.6
     reuse = t * t * (3 - 2 * t)
     sc = plane_scene(S.metal((0.9, 0.9, 0.9), 0.15))
-    o = O.PtOracle(pack_scene(sc), sc)
+    o = O.PtOracle(sc)
     f = o.frame(sc.camera, W, H, 2, 0.0)
     out, ok = o.trace_reprojected(f, surface_history(o, f, 100, 4), f, 8.0, 104, xy)
     assert ok[interior].all()

@@ -79,7 +79,7 @@ def test_reprojected_sample_matches_oracle(ctx):
     gpu = r.film.read()
     info = r.film.info()
     assert info["samples"] == 9 and info["seed"] == 11 + 9
-    o = O.PtOracle(r.packed, sc)
+    o = O.PtOracle(sc)
     ref, ok = o.trace_reprojected(o.frame(c1, W, H, 8, 0.0), prev, o.frame(c0, W, H, 8, 0.0), 8.0, 11 + 8,
                                   all_xy(W, H))
     compare_reprojected(gpu, ref, ok, "orbit 4 deg")
@@ -104,7 +104,7 @@ def test_reprojection_from_a_preview_film_of_another_size(ctx):
                                                          C.byref(camera_struct(c0)), 8.0, C.byref(st), None))
     gpu = dst.read()
     assert dst.info()["samples"] == 9 and dst.info()["seed"] == 41
-    o = O.PtOracle(r.packed, sc)
+    o = O.PtOracle(sc)
     ref, ok = o.trace_reprojected(o.frame(c1, W, H, 8, 0.0), prev, o.frame(c0, W // 2, H // 2, 8, 0.0), 8.0, 40,
                                   all_xy(W, H))
     compare_reprojected(gpu, ref, ok, "preview 48x32 -> 96x64")
