@@ -3,9 +3,10 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload closest-hit]
 
 Headline (default): the plain progressive path trace at 1440x900 — one step
-is a full 64-spp frame (64 launches, one per sample, as Renderer.traceSamples)
-of the 222k-triangle knot scene; value = camera paths (pixel samples) per
-second.  The closest-hit benchmark is reported beside it: one step = one
+is a full 64-spp frame of the 222k-triangle knot scene (Renderer.traceSamples(64)
+after a reset: one persistent k_trace_units launch; the reference's launch
+structure, one launch per sample, is timed beside it on the same film);
+value = camera paths (pixel samples) per second.  The closest-hit benchmark is reported beside it: one step = one
 batched exact closest-hit launch of 2^20 rays against the 81,920-triangle
 fBm icosphere (inputs resident in HBM).
 
@@ -205,8 +206,9 @@ def bench_path_trace(args) -> dict:
         return wall, [s_.elapsed_time(e) for s_, e in zip(starts, ends)]
 
     from lucent_amd.renderer import autotune
-    tune = autotune(ctx, [(r, SPP)], r.settings, trial_samples=4)
-    log(f"autotuned occupancy {tune['occupancy']} ({tune['ms']} ms for 4 spp)")
+    tune = autotune(ctx, [(r, SPP)], r.settings, trial_samples=SPP)
+    log(f"autotuned occupancy {tune['occupancy']}, tile {tune['tile']}, shade batch {tune['shade_batch']}, "
+        f"trace turns {tune['trace_turns']}")
     log(f"warmup {args.warmup}")
     if args.warmup:
         timed_steps(args.warmup, False)
@@ -232,7 +234,8 @@ def bench_path_trace(args) -> dict:
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_path": round(b_path - 72, 1),
                 "film_bytes_per_pixel": 72, "kernel": "k_trace_units", "avg_launch_ms": round(launch_s * 1e3, 3),
-                "occupancy": tune["occupancy"]}
+                "occupancy": tune["occupancy"], "tile": tune["tile"], "shade_batch": tune["shade_batch"],
+                "trace_turns": tune["trace_turns"]}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_path_trace(r.packed, scene, W, H)
@@ -249,7 +252,7 @@ def bench_path_trace(args) -> dict:
                        "scene_load_s": round(load_s, 3)},
             "roofline": roofline, "cpu_baseline": cpu, "image_mean_rgb": mean_rgb,
             "mpaths_per_s": round(value / 1e6, 2), "mrays_per_s_all_rays": round(value * rays_per_path / 1e6, 1),
-            "autotune": {"occupancy": tune["occupancy"], "ms": {k: round(v, 3) for k, v in tune["ms"].items()}},
+            "autotune": tune,
             "per_sample_launches": {"samples_per_s": round(per_sample_value, 1),
                                     "trace_ms": round(float(np.mean(trace_ms_ps)), 3),
                                     "avg_sample_launch_ms": round(float(np.mean(trace_ms_ps)) / SPP, 4),
@@ -273,7 +276,7 @@ def cpu_baseline_path_trace(packed, scene, W, H) -> dict:
     host cores: a 256x256 crop at the frame centre, 16 spp, one thread per core."""
     import oracle as O
 
-    o = O.PtOracle(packed, scene)
+    o = O.PtOracle(scene)
     f = o.frame(scene.camera, W, H, 8, 0.0)
     yy, xx = np.mgrid[H // 2 - 128:H // 2 + 128, W // 2 - 128:W // 2 + 128]
     xy = np.stack([xx.reshape(-1), yy.reshape(-1)], -1)
@@ -347,8 +350,9 @@ def bench_adjoint(args) -> dict:
     forward render (one persistent launch, per-sample radiance recorded) +
     dL/dI of the L2 loss against a target rendered with ±20% perturbed
     parameters + path-replay backward (one persistent launch + FP64
-    reductions), all in device memory; grad/s = 1 / (forward + backward
-    time)."""
+    reductions), all in device memory on one stream; grad/s = 1 / (forward +
+    backward time).  HIP events on that stream time each pass; the
+    roofline is the backward's (k_adjoint and its two reductions)."""
     import torch
 
     from lucent_amd import workloads
@@ -369,45 +373,104 @@ def bench_adjoint(args) -> dict:
     target = d.forward(128, 1000, record=False, device=True).clone()
     d.set_materials(base_mats)
     d.set_env_texels(base_env)
-    img = d.forward(SPP, 0, device=True)
-    d.backward(2 * (img - target), SPP, 0, device=True)  # warm-up
-    # One gradient evaluation stays in device memory: forward into the film,
-    # dL/dI = 2 (I - target) by torch on the film's colour target, replay
-    # backward into device gradient buffers.
-    reps = 3
-    t_fwd = t_bwd = 0.0
-    for _ in range(reps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        img = d.forward(SPP, 0, device=True)
-        t1 = time.perf_counter()
-        gm, ge = d.backward(2 * (img - target), SPP, 0, device=True)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        t_fwd += t1 - t0
-        t_bwd += t2 - t1
-    t_fwd /= reps
-    t_bwd /= reps
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sp = stream.cuda_stream
+    img = d.forward(SPP, 0, device=True, stream=sp)
+    d.backward(2 * (img - target), SPP, 0, device=True, stream=sp)  # warm-up
+    torch.cuda.synchronize()
+    # The replay's job tile: the fastest of a few shapes (identical gradients
+    # up to the order of the FP64 sums).
+    tile_ms = {}
+    for tile in ((8, 8), (4, 4), (4, 2), (2, 2)):
+        d.adjoint_tile = tile
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        best = float("inf")
+        for _ in range(2):
+            e0.record(stream)
+            d.backward(2 * (img - target), SPP, 0, device=True, stream=sp)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        tile_ms[f"{tile[0]}x{tile[1]}"] = round(best, 3)
+    d.adjoint_tile = tuple(int(v) for v in min(tile_ms, key=tile_ms.get).split("x"))
+    log(f"replay tile {d.adjoint_tile}: {tile_ms}")
+    reps = max(3, min(args.steps, 10))
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(reps)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for e in ev:
+        e[0].record(stream)
+        img = d.forward(SPP, 0, device=True, stream=sp)
+        e[1].record(stream)
+        gm, ge = d.backward(2 * (img - target), SPP, 0, device=True, stream=sp)
+        e[2].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - wall0) / reps
+    t_fwd = float(np.mean([a.elapsed_time(b) for a, b, _ in ev])) / 1e3
+    t_bwd = float(np.mean([b.elapsed_time(c) for _, b, c in ev])) / 1e3
     loss = float(((img[..., :3] - target[..., :3]).double() ** 2).sum())
     params = {"material_scalars": int(gm[:16].numel()), "env_channels": int(ge.numel())}
+    counts = json.loads((ROOT / "tests/golden/adjoint_counts.json").read_text())["per_path"]
+    b_path = path_bytes(counts) - 72
+    paths, px = W * H * SPP, W * H
+    env_lookups = counts["env_nee"] + counts["escaped"]
+    # Replay: the path's fetches again, its recorded radiance (16 B), FP64
+    # atomics of 4 texels x 3 channels per environment lookup; dL/dI (16 B
+    # per pixel); the reduction of 8 env-gradient replicas into the output.
+    b_bwd = paths * (b_path + 16 + 96 * env_lookups) + 16 * px + 9 * 8 * params["env_channels"] + 8 * 80
+    achieved = b_bwd / t_bwd / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic("adjoint_512x512"),
+                "kernel": "k_adjoint (+ k_reduce_materials, k_reduce_env)", "avg_launch_ms": round(t_bwd * 1e3, 3),
+                "bytes_per_path": round(b_path + 16 + 96 * env_lookups, 1)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_adjoint(sc, W, H, SPP, workloads.DIFF_DEPTH)
     d.close()
     ctx.close()
     return {"metric": "adjoint grad/s (forward + backward, 512x512, 32 spp, depth 6)",
             "value": round(1.0 / (t_fwd + t_bwd), 3), "unit": "grad/s", "forward_ms": round(t_fwd * 1e3, 2),
-            "backward_ms": round(t_bwd * 1e3, 2), "params": params,
-            "loss": loss, "forward_occupancy": tune["occupancy"],
+            "backward_ms": round(t_bwd * 1e3, 2), "wall_ms_per_gradient": round(wall * 1e3, 2), "params": params,
+            "loss": loss, "forward_launch": {k: v for k, v in tune.items() if k != "trials"},
+            "replay_tile": list(d.adjoint_tile), "replay_tile_ms": tile_ms, "roofline": roofline, "cpu_baseline": cpu,
             "backward": "replays the forward pass's recorded per-sample radiance; image, loss gradient and "
                         "parameter gradients stay in device memory"}
+
+
+def cpu_baseline_adjoint(sc, W, H, SPP, depth) -> dict:
+    """The FP64 oracle's path-replay adjoint on 64 sampled pixels (all their
+    samples), extrapolated to whole-image gradient evaluations per second."""
+    import oracle as O
+
+    o = O.PtOracle(sc)
+    f = o.frame(sc.camera, W, H, depth, 0.0)
+    n = 16384
+    rng = np.random.default_rng(3)
+    xy = np.stack([rng.integers(0, W, n), rng.integers(0, H, n)], -1).astype(np.uint32)
+    dl = rng.standard_normal((n, 3))
+    threads = min(16, os.cpu_count() or 1)
+    log(f"cpu baseline: FP64 oracle adjoint, {n} pixels x {SPP} spp on {threads} threads")
+    t0 = time.perf_counter()
+    o.adjoint_pixels(f, 0, SPP, xy, dl, o.packed.material_count, threads=threads)
+    dt = time.perf_counter() - t0
+    per_image = dt * (W * H) / n
+    return {"value": round(1.0 / per_image, 6), "unit": "grad/s", "cores": threads, "kind": "port",
+            "sample": f"{n} random pixels x {SPP} spp of the 512x512 gradient (the 64-pixel check, scaled up "
+                      f"to a measurable run), FP64 C path-replay adjoint on {threads} threads: {dt:.3f} s, "
+                      f"extrapolated x{W * H / n:g} to one whole-image gradient"}
 
 
 def bench_batch(args) -> dict:
     """Batched multi-scene tracing: 64 seeded scenes (1k..1.5M triangles,
     11.8M in all, 6% of meshes Worley alpha-masked), resolution uniform over
     256^2 / 640x480 / 1280x720 / 1920x1080, spp uniform in 1..64, depth 8.
-    Plain: lucent_trace_samples per scene with one launch per sample (the
-    reference's structure).  Batched: ONE lucent_trace_batch over all 64
-    films.  Same seeds, same estimator: the films are compared bit for bit,
-    so the per-pixel error is equal by construction."""
+    Batched: ONE lucent_trace_batch over all 64 films.  Two plain paths on
+    the same films and seeds: one lucent_trace_samples(spp) launch per scene
+    (the library's one-film persistent launch), and one launch per sample
+    per scene (the reference's structure, renderer.ts:702-737).  Same seeds,
+    same estimator: the films are compared bit for bit, so the per-pixel
+    error is equal by construction."""
     import torch
 
     from lucent_amd import workloads
@@ -417,38 +480,54 @@ def bench_batch(args) -> dict:
     log(f"building {workloads.BATCH_SCENES} batch scenes")
     t0 = time.time()
     ctx = Context(0)
-    jobs, tris = [], 0
+    jobs, tris, gen_s, load_s = [], 0, 0.0, 0.0
     for i in range(workloads.BATCH_SCENES):
+        g0 = time.time()
         sc, w, h, spp = workloads.batch_scene(i)
+        g1 = time.time()
         r = Renderer(ctx, w, h, RenderSettings(max_bounces=workloads.BATCH_DEPTH, firefly_clamp=0.0))
         r.load_scene(sc)
+        load_s += time.time() - g1
+        gen_s += g1 - g0
         tris += r.packed.tri_count
         r.packed = None  # host copies are not needed any more
         jobs.append((r, spp))
     setup_s = time.time() - t0
     paths = sum(r.film.width * r.film.height * n for r, n in jobs)
-    log(f"{tris} triangles, {paths} paths; set-up {setup_s:.1f} s")
+    pixels = sum(r.film.width * r.film.height for r, _ in jobs)
+    log(f"{tris} triangles, {paths} paths; set-up {setup_s:.1f} s (scene generation {gen_s:.1f} s, "
+        f"pack + BVH + upload {load_s:.1f} s)")
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sp = stream.cuda_stream
 
     def reset_all():
+        torch.cuda.synchronize()
         for r, _ in jobs:
             r.film.set_seed(0)
             r.reset()
         ctx.synchronize()
 
-    def run_plain():
-        for r, n in jobs:
-            r.settings.per_sample_launch = True
-            r.trace_samples(n)
-        ctx.synchronize()
-
     settings = RenderSettings(max_bounces=workloads.BATCH_DEPTH, firefly_clamp=0.0)
     from lucent_amd.renderer import autotune
     tune = autotune(ctx, jobs, settings, trial_samples=1)
-    log(f"autotuned occupancy {tune['occupancy']} ({tune['ms']} ms for 1 spp of every scene)")
+    log(f"autotuned {tune} (1 spp of every scene)")
 
-    def run_batched():
-        trace_batch(ctx, jobs, settings)
-        ctx.synchronize()
+    def run(mode):
+        reset_all()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t = time.perf_counter()
+        e0.record(stream)
+        if mode == "batched":
+            trace_batch(ctx, jobs, settings, stream=sp)
+        else:
+            for r, n in jobs:
+                r.settings.per_sample_launch = mode == "per_sample"
+                r.settings.occupancy = settings.occupancy
+                r.trace_samples(n, stream=sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, e0.elapsed_time(e1) / 1e3
 
     def films_digest():
         out = []
@@ -457,33 +536,86 @@ def bench_batch(args) -> dict:
             out.append(hash(f["color"].tobytes()) ^ hash(f["albedo"].tobytes()) ^ hash(f["normal"].tobytes()))
         return out
 
-    def timed(fn, n):
-        best = []
-        for _ in range(n):
-            reset_all()
-            t = time.perf_counter()
-            fn()
-            best.append(time.perf_counter() - t)
-        return best
-
+    if args.no_plain:
+        run("batched")
+        bt = [run("batched") for _ in range(max(1, min(args.steps, 3)))]
+        for r, _ in jobs:
+            r.close()
+        ctx.close()
+        tb_wall, tb_gpu = min(bt)
+        return {"metric": "samples/s, 64-scene batch, one persistent launch (plain paths not run)",
+                "value": round(paths / tb_wall, 1), "unit": "samples/s", "batched_s": round(tb_wall, 3),
+                "batched_gpu_s": round(tb_gpu, 3), "autotune": tune, "paths": paths, "setup_s": round(setup_s, 1)}
     log("plain: one launch per sample per scene")
-    plain_s = timed(run_plain, 1)
-    plain_digest = films_digest()
-    log(f"plain {plain_s[0]:.2f} s; batched warm-up")
-    timed(run_batched, 1)
+    ps_wall, ps_gpu = run("per_sample")
+    ref_digest = films_digest()
+    log(f"per-sample launches {ps_wall:.2f} s; plain: one lucent_trace_samples launch per scene")
+    run("per_scene")
+    pl = [run("per_scene") for _ in range(2)]
+    same_plain = films_digest() == ref_digest
+    pl_wall, pl_gpu = min(pl)
+    log(f"one launch per scene {pl_wall:.2f} s; batched warm-up")
+    run("batched")
     log("batched: timing")
-    batch_s = timed(run_batched, max(1, min(args.steps, 3)))
-    same = films_digest() == plain_digest
+    bt = [run("batched") for _ in range(max(1, min(args.steps, 3)))]
+    same = films_digest() == ref_digest
+    tb_wall, tb_gpu = min(bt)
     for r, _ in jobs:
         r.close()
     ctx.close()
-    tb, tp = float(np.median(batch_s)), plain_s[0]
-    return {"metric": "samples/s, 64-scene batch: one persistent launch vs per-sample launches per scene",
-            "value": round(paths / tb, 1), "unit": "samples/s", "plain_samples_per_s": round(paths / tp, 1),
-            "speedup_vs_plain": round(tp / tb, 3), "batched_s": round(tb, 3), "plain_s": round(tp, 3),
-            "films_bit_identical": same,
-            "autotune": {"occupancy": tune["occupancy"], "ms": {k: round(v, 3) for k, v in tune["ms"].items()}}, "scenes": len(jobs), "triangles": tris, "paths": paths,
-            "setup_s": round(setup_s, 1)}
+    roofline = None
+    cpath = ROOT / "tests/golden/batch_counts.json"
+    if cpath.exists():
+        per = {s["index"]: s for s in json.loads(cpath.read_text())["scenes"]}
+        b_launch = 0.0
+        for i in range(workloads.BATCH_SCENES):
+            s_ = per[i]
+            n_px = s_["width"] * s_["height"]
+            b_launch += (path_bytes(s_["per_path"]) - 72) * n_px * s_["spp"] + 72 * n_px
+        achieved = b_launch / tb_gpu / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic("batch_64scenes"),
+                    "kernel": "k_trace_batch_units", "avg_launch_ms": round(tb_gpu * 1e3, 2),
+                    "bytes_per_launch": round(b_launch)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_batch()
+    return {"metric": "samples/s, 64-scene batch: one persistent launch vs one launch per scene",
+            "value": round(paths / tb_wall, 1), "unit": "samples/s",
+            "plain_one_launch_per_scene_samples_per_s": round(paths / pl_wall, 1),
+            "speedup_vs_one_launch_per_scene": round(pl_wall / tb_wall, 3),
+            "plain_per_sample_launches_samples_per_s": round(paths / ps_wall, 1),
+            "speedup_vs_per_sample_launches": round(ps_wall / tb_wall, 3),
+            "batched_s": round(tb_wall, 3), "batched_gpu_s": round(tb_gpu, 3),
+            "one_launch_per_scene_s": round(pl_wall, 3), "per_sample_launches_s": round(ps_wall, 3),
+            "films_bit_identical": same and same_plain, "roofline": roofline, "cpu_baseline": cpu,
+            "autotune": tune, "scenes": len(jobs), "triangles": tris, "paths": paths, "pixels": pixels,
+            "setup_s": round(setup_s, 1), "setup_scene_generation_s": round(gen_s, 1),
+            "setup_pack_bvh_upload_s": round(load_s, 1)}
+
+
+def cpu_baseline_batch() -> dict:
+    """The FP64 oracle path tracer on a bounded sample of the batch: a
+    192x192 centre crop of three of its scenes at their own sample counts."""
+    import oracle as O
+    from lucent_amd import workloads
+
+    threads = min(16, os.cpu_count() or 1)
+    n, dt = 0, 0.0
+    picks = (1, 13, 62)
+    for i in picks:
+        sc, w, h, spp = workloads.batch_scene(i)
+        o = O.PtOracle(sc)
+        f = o.frame(sc.camera, w, h, workloads.BATCH_DEPTH, 0.0)
+        yy, xx = np.mgrid[h // 2 - 96:h // 2 + 96, w // 2 - 96:w // 2 + 96]
+        xy = np.stack([xx.reshape(-1), yy.reshape(-1)], -1)
+        t0 = time.perf_counter()
+        o.trace_pixels(f, 0, spp, xy, threads=threads)
+        dt += time.perf_counter() - t0
+        n += xy.shape[0] * spp
+    return {"value": round(n / dt, 1), "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"192x192 centre crops of batch scenes {picks} at their own spp ({n} paths), FP64 C "
+                      f"restatement of trace.frag on {threads} threads, {dt:.2f} s"}
 
 
 def bench_copy(args) -> dict:
@@ -575,6 +707,7 @@ def main() -> int:
                     choices=["all", "closest-hit", "path-trace", "batch", "denoise", "adjoint", "frame-graph", "copy"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="batch: time only the batched launch (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by "

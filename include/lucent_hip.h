@@ -336,6 +336,14 @@ typedef struct lucent_render_settings {
                             4..6 spill registers to scratch but hide more memory latency.
                             lucent_backward's replay takes 2..4 (0 = 4, measured fastest).
                             Results do not depend on it (lucent_autotune_tiles picks one). */
+  /* Shape of the persistent per-sample hand-out (one film, and batches):
+   * a job is a tile_w x tile_h tile of pixels (at most 64) with all its
+   * samples; a wave shades once shade_batch of its 64 lanes wait and runs up
+   * to trace_turns traversal steps between its checks for shading and new
+   * work.  0 = 8, 8, 32, 8.  Results do not depend on them
+   * (lucent_autotune_launch picks them with the occupancy).  lucent_backward
+   * hands out its replays by tiles of tile_w x tile_h too. */
+  uint32_t tile_w, tile_h, shade_batch, trace_turns;
 } lucent_render_settings;
 
 lucent_status lucent_trace_samples(lucent_ctx *ctx, lucent_scene *scene, lucent_film *film,
@@ -391,6 +399,27 @@ lucent_status lucent_trace_batch(lucent_ctx *ctx, const lucent_batch_item *items
 lucent_status lucent_autotune_tiles(lucent_ctx *ctx, const lucent_batch_item *items, uint32_t count,
                                     const lucent_render_settings *settings, uint32_t trial_samples,
                                     uint32_t *occupancy_out, float *ms_out);
+
+/* The launch autotuner: searches the persistent launch's register variant
+ * (occupancy 2..6), then its job tile (8x8, 8x4, 4x4, 4x2, 2x2), shade
+ * batch (16, 24, 32, 40, 48 lanes) and traversal turns per check (4, 8,
+ * 12, 16), one parameter at a time with the others at their best so far,
+ * each candidate timed as lucent_autotune_tiles times a variant (best of two
+ * runs of min(trial_samples, samples) samples of every item after an
+ * untimed round, in scratch films).  A candidate replaces the current choice
+ * only when more than 1 % faster.  Writes the choice into *settings
+ * (occupancy, tile_w, tile_h, shade_batch, trace_turns; its other fields
+ * are read: flags, depth, clamp, waves_per_cu) and, if trials is not NULL,
+ * up to trials_cap timed candidates in order; *n_trials receives their
+ * number.  Every choice gives identical films.  No counterpart in the
+ * reference. */
+typedef struct lucent_autotune_trial {
+  uint32_t occupancy, tile_w, tile_h, shade_batch, trace_turns;
+  float ms;
+} lucent_autotune_trial;
+lucent_status lucent_autotune_launch(lucent_ctx *ctx, const lucent_batch_item *items, uint32_t count,
+                                     lucent_render_settings *settings, uint32_t trial_samples,
+                                     lucent_autotune_trial *trials, uint32_t trials_cap, uint32_t *n_trials);
 
 /* ------------------------------------------------------------------------
  * Denoiser — Renderer.runDenoiser (src/renderer.ts:788-830): variance.frag,

@@ -99,7 +99,13 @@ TRACE_INLINE_SHADOWS = 0x4
 
 class RenderSettingsStruct(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("firefly_clamp", C.c_float), ("flags", C.c_uint32),
-                ("waves_per_cu", C.c_uint32), ("occupancy", C.c_uint32)]
+                ("waves_per_cu", C.c_uint32), ("occupancy", C.c_uint32), ("tile_w", C.c_uint32),
+                ("tile_h", C.c_uint32), ("shade_batch", C.c_uint32), ("trace_turns", C.c_uint32)]
+
+
+class AutotuneTrialStruct(C.Structure):
+    _fields_ = [("occupancy", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32),
+                ("shade_batch", C.c_uint32), ("trace_turns", C.c_uint32), ("ms", C.c_float)]
 
 
 class BatchItemStruct(C.Structure):
@@ -150,6 +156,8 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_trace_samples": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), C.POINTER(RenderSettingsStruct), _U32, _P]),
     "lucent_trace_batch": (_I32, [_P, _P, _U32, C.POINTER(RenderSettingsStruct), _P]),
     "lucent_autotune_tiles": (_I32, [_P, _P, _U32, C.POINTER(RenderSettingsStruct), _U32, C.POINTER(_U32), _P]),
+    "lucent_autotune_launch": (_I32, [_P, _P, _U32, C.POINTER(RenderSettingsStruct), _U32, _P, _U32,
+                                      C.POINTER(_U32)]),
     "lucent_denoise": (_I32, [_P, _P, C.POINTER(CameraStruct), C.POINTER(DenoiseParamsStruct), _P]),
     "lucent_film_read_denoised": (_I32, [_P, _P, _P, _P]),
     "lucent_trace_reprojected": (_I32, [_P, _P, _P, C.POINTER(CameraStruct), _P, C.POINTER(CameraStruct), C.c_float,

@@ -562,7 +562,8 @@ template <int kWaves>
 __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_adjoint(
     const AdjointView* __restrict__ view, uint32_t spp, uint32_t n_jobs, uint32_t* __restrict__ counter, const float* __restrict__ dl_di,
     const float4* __restrict__ recorded_L, float inv_spp, double* __restrict__ mat_partials, double* __restrict__ env_acc,
-    size_t env_stride, double* __restrict__ mat_overflow, uint32_t* spill, uint32_t spill_depth, uint32_t tiles_x) {
+    size_t env_stride, double* __restrict__ mat_overflow, uint32_t* spill, uint32_t spill_depth, uint32_t tiles_x,
+    uint32_t tile_w, uint32_t tile_h) {
   const PtScene& sc = view->sc;
   const PtFrame& f = view->f;
   __shared__ uint32_t lds[kPtStackWords * kAdjBlock];
@@ -583,10 +584,10 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWave
   // and the next of its px * spp samples.
   uint32_t x0 = 0, y0 = 0, tw = 1, px = 0, units = 0, handed = 0;
   auto open_job = [&](uint32_t j) {
-    x0 = (j % tiles_x) * kTile;
-    y0 = (j / tiles_x) * kTile;
-    tw = min(uint32_t(kTile), f.width - x0);
-    px = tw * min(uint32_t(kTile), f.height - y0);
+    x0 = (j % tiles_x) * tile_w;
+    y0 = (j / tiles_x) * tile_h;
+    tw = min(tile_w, f.width - x0);
+    px = tw * min(tile_h, f.height - y0);
     units = px * spp;
     handed = 0;
   };
@@ -722,13 +723,20 @@ Status launch_detached(Context* ctx, const DetachedArgs& a, void* stream_) {
 Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   const PtScene& sc = *a.scene;
-  const uint32_t tiles_x = (a.frame.width + kTile - 1) / kTile;
-  const uint32_t tiles_y = (a.frame.height + kTile - 1) / kTile;
+  // Jobs: tiles of the launch shape (0 = 8 x 8, at most 64 pixels) with all
+  // their samples.  The per-block partials hold one entry per 8 x 8 tile
+  // (adjoint_scratch_bytes), which bounds the grid.
+  const uint32_t tw = a.tile_w ? a.tile_w : kTile, th = a.tile_h ? a.tile_h : kTile;
+  if (tw * th > 64) return fail(LUCENT_E_INVALID, "launch tile holds more than 64 pixels");
+  const uint32_t tiles_x = (a.frame.width + tw - 1) / tw;
+  const uint32_t tiles_y = (a.frame.height + th - 1) / th;
   const uint32_t n_jobs = tiles_x * tiles_y;
+  const uint32_t blocks8 = ((a.frame.width + kTile - 1) / kTile) * ((a.frame.height + kTile - 1) / kTile);
   const uint32_t waves = (a.occupancy == 2 || a.occupancy == 3) ? a.occupancy : 4;
-  const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * 4 * waves);
+  const uint32_t grid =
+      std::min<uint32_t>(std::min(n_jobs, blocks8), uint32_t(std::max(ctx->compute_units, 1)) * 4 * waves);
   const size_t env_texels = size_t(sc.env_w) * sc.env_h;
-  const size_t partials = size_t(n_jobs) * kLdsMaterials * kNumParams;
+  const size_t partials = size_t(blocks8) * kLdsMaterials * kNumParams;
   const size_t overflow = size_t(sc.material_count) * kNumParams;
   const size_t env_stride = env_texels * 3;
   // Scratch: [block partials | overflow | env replicas | job counter | view] (adjoint_scratch_bytes).
@@ -752,7 +760,7 @@ Status launch_adjoint(Context* ctx, const AdjointArgs& a, void* stream_) {
   auto kern = waves == 3 ? k_adjoint<3> : waves == 4 ? k_adjoint<4> : k_adjoint<2>;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(kAdjBlock), 0, stream, view, a.spp, n_jobs, counter, a.dl_di, rec,
                      1.0f / float(a.spp), mat_partials, env_acc, env_stride, mat_overflow,
-                     static_cast<uint32_t*>(ctx->spill), depth, tiles_x);
+                     static_cast<uint32_t*>(ctx->spill), depth, tiles_x, tw, th);
   const uint32_t entries = uint32_t(min(int(sc.material_count), kLdsMaterials)) * kNumParams;
   const uint32_t all = uint32_t(sc.material_count) * kNumParams;
   if (all > 0)

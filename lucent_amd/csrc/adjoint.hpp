@@ -19,6 +19,7 @@ struct AdjointArgs {
   double* grad_env = nullptr;       // env_w * env_h * 3, or null
   void* scratch = nullptr;          // adjoint_scratch_bytes(...)
   uint32_t occupancy = 0;           // register-allocation variant of the replay kernel: 4 (default; any value but 2 or 3), 2 or 3
+  uint32_t tile_w = 0, tile_h = 0;  // job tile of the replay's hand-out (0 = 8 x 8; at most 64 pixels)
 };
 
 struct DetachedArgs {

@@ -403,6 +403,10 @@ lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_
   cfg.fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS) && !scene->blend_alpha;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
+  cfg.tile_w = settings->tile_w;
+  cfg.tile_h = settings->tile_h;
+  cfg.shade_batch = settings->shade_batch;
+  cfg.trace_turns = settings->trace_turns;
   if ((s = launch_trace_tiles(ctx, &it, 1, cfg, st)) != LUCENT_OK) return s;
   film->seed += count;
   film->samples += count;
@@ -458,6 +462,10 @@ lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items
   cfg.fused = fused;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
+  cfg.tile_w = settings->tile_w;
+  cfg.tile_h = settings->tile_h;
+  cfg.shade_batch = settings->shade_batch;
+  cfg.trace_turns = settings->trace_turns;
   Status s = launch_trace_tiles(ctx, tiles.data(), uint32_t(tiles.size()), cfg, st);
   if (s != LUCENT_OK) return s;
   for (uint32_t i = 0; i < count; ++i) {
@@ -550,6 +558,138 @@ lucent_status lucent_autotune_tiles(lucent_ctx* ctx, const lucent_batch_item* it
   return LUCENT_OK;
 }
 
+lucent_status lucent_autotune_launch(lucent_ctx* ctx, const lucent_batch_item* items, uint32_t count,
+                                     lucent_render_settings* settings, uint32_t trial_samples,
+                                     lucent_autotune_trial* trials, uint32_t trials_cap, uint32_t* n_trials) {
+  clear_error();
+  if (!ctx || !items || count == 0 || !settings) return fail(LUCENT_E_INVALID, "null argument");
+  if (trial_samples == 0) return fail(LUCENT_E_INVALID, "trial_samples must be positive");
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!items[i].scene || !items[i].film) return fail(LUCENT_E_INVALID, "batch item without scene or film");
+    Status s = check_traceable(items[i].scene);
+    if (s != LUCENT_OK) return s;
+  }
+  if (n_trials) *n_trials = 0;
+  hipStream_t st = static_cast<hipStream_t>(ctx->stream);
+  std::vector<lucent_film*> films(count, nullptr);
+  std::vector<TileItem> tiles;
+  Status s = LUCENT_OK;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (uint32_t i = 0; i < count && s == LUCENT_OK; ++i) {
+    s = lucent_film_create(ctx, items[i].film->width, items[i].film->height, &films[i]);
+    if (s != LUCENT_OK) break;
+    TileItem it;
+    it.scene = items[i].scene->pt;
+    fill_frame(&items[i].camera, settings, films[i]->width, films[i]->height, &it.frame);
+    it.frame.seed = items[i].film->seed;
+    it.film = films[i]->f;
+    it.spp = std::max(1u, std::min(trial_samples, items[i].samples ? items[i].samples : trial_samples));
+    tiles.push_back(it);
+  }
+  if (s == LUCENT_OK) {
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    if (e != hipSuccess) s = fail_hip(e, "hipEventCreate");
+  }
+  bool fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS);
+  for (uint32_t i = 0; i < count; ++i) fused = fused && !items[i].scene->blend_alpha;
+  // One timed run of a candidate (the launch rewrites a copy of the items).
+  auto run = [&](const TileLaunch& cfg, float* ms) -> Status {
+    std::vector<TileItem> t = tiles;
+    hipError_t e = hipEventRecord(ev[0], st);
+    if (e != hipSuccess) return fail_hip(e, "hipEventRecord");
+    Status r = launch_trace_tiles(ctx, t.data(), uint32_t(t.size()), cfg, st);
+    if (r != LUCENT_OK) return r;
+    e = hipEventRecord(ev[1], st);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ev[0], ev[1]);
+    if (e != hipSuccess) return fail_hip(e, "autotune timing");
+    return LUCENT_OK;
+  };
+  TileLaunch best;
+  best.fused = fused;
+  best.waves_per_cu = settings->waves_per_cu;
+  best.occupancy = 4;
+  best.tile_w = best.tile_h = 8;
+  best.shade_batch = 32;
+  best.trace_turns = 8;
+  float best_ms = 0.0f;
+  uint32_t logged = 0;
+  // Best of two runs, after an untimed one the first time a variant runs.
+  auto measure = [&](const TileLaunch& cfg, bool warm, float* out) -> Status {
+    float a = 0.0f, b = 0.0f;
+    Status r = LUCENT_OK;
+    if (warm && (r = run(cfg, &a)) != LUCENT_OK) return r;
+    if ((r = run(cfg, &a)) != LUCENT_OK) return r;
+    if ((r = run(cfg, &b)) != LUCENT_OK) return r;
+    *out = std::min(a, b);
+    if (trials && logged < trials_cap)
+      trials[logged] = lucent_autotune_trial{cfg.occupancy, cfg.tile_w, cfg.tile_h, cfg.shade_batch, cfg.trace_turns,
+                                             *out};
+    ++logged;
+    return LUCENT_OK;
+  };
+  // Stage 0: the register variant (each a code object of its own).
+  for (uint32_t occ = 2; occ <= 6 && s == LUCENT_OK; ++occ) {
+    TileLaunch c = best;
+    c.occupancy = occ;
+    float ms = 0.0f;
+    if ((s = measure(c, true, &ms)) != LUCENT_OK) break;
+    // A higher occupancy must win by more than 1 %: its register spills
+    // cost memory traffic that a near-tie does not repay.
+    if (occ == 2 || ms < best_ms * 0.99f) {
+      best = c;
+      best_ms = ms;
+    }
+  }
+  // Stages 1-3: the hand-out shape, one parameter at a time.
+  const uint32_t tiles_wh[][2] = {{8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}};
+  const uint32_t batches[] = {16, 24, 32, 40, 48};
+  const uint32_t turns[] = {4, 8, 12, 16};
+  auto consider = [&](const TileLaunch& c) -> Status {
+    float ms = 0.0f;
+    Status r = measure(c, false, &ms);
+    if (r == LUCENT_OK && ms < best_ms * 0.99f) {
+      best = c;
+      best_ms = ms;
+    }
+    return r;
+  };
+  for (const auto& wh : tiles_wh) {
+    if (s != LUCENT_OK) break;
+    if (wh[0] == best.tile_w && wh[1] == best.tile_h) continue;
+    TileLaunch c = best;
+    c.tile_w = wh[0];
+    c.tile_h = wh[1];
+    s = consider(c);
+  }
+  for (uint32_t b : batches) {
+    if (s != LUCENT_OK) break;
+    if (b == best.shade_batch) continue;
+    TileLaunch c = best;
+    c.shade_batch = b;
+    s = consider(c);
+  }
+  for (uint32_t t : turns) {
+    if (s != LUCENT_OK) break;
+    if (t == best.trace_turns) continue;
+    TileLaunch c = best;
+    c.trace_turns = t;
+    s = consider(c);
+  }
+  for (hipEvent_t e : ev)
+    if (e) hipEventDestroy(e);
+  for (lucent_film* f : films) lucent_film_destroy(f);
+  if (s != LUCENT_OK) return s;
+  settings->occupancy = best.occupancy;
+  settings->tile_w = best.tile_w;
+  settings->tile_h = best.tile_h;
+  settings->shade_batch = best.shade_batch;
+  settings->trace_turns = best.trace_turns;
+  if (n_trials) *n_trials = std::min(logged, trials ? trials_cap : 0u);
+  return LUCENT_OK;
+}
+
 static Status fill_frame(const lucent_camera* cam, const lucent_render_settings* settings, uint32_t w, uint32_t h,
                          PtFrame* f) {
   f->width = w;
@@ -593,6 +733,10 @@ lucent_status lucent_forward(lucent_ctx* ctx, lucent_scene* scene, lucent_film* 
   cfg.fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS) && !scene->blend_alpha;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
+  cfg.tile_w = settings->tile_w;
+  cfg.tile_h = settings->tile_h;
+  cfg.shade_batch = settings->shade_batch;
+  cfg.trace_turns = settings->trace_turns;
   if ((s = launch_trace_tiles(ctx, &it, 1, cfg, stream ? stream : ctx->stream)) != LUCENT_OK) return s;
   film->seed += spp;
   film->samples += spp;
@@ -626,6 +770,8 @@ lucent_status lucent_backward(lucent_ctx* ctx, lucent_scene* scene, const lucent
   a.dl_di = d_dl_di;
   a.recorded_radiance = d_sample_radiance;
   a.occupancy = settings->occupancy;
+  a.tile_w = settings->tile_w;
+  a.tile_h = settings->tile_h;
   a.grad_materials = d_grad_materials;
   a.grad_env = d_grad_env;
   a.scratch = ctx->scratch;
@@ -866,6 +1012,10 @@ lucent_status lucent_frame_graph_create(lucent_ctx* ctx, lucent_scene* scene, lu
   cfg.fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS) && !scene->blend_alpha;
   cfg.waves_per_cu = settings->waves_per_cu;
   cfg.occupancy = settings->occupancy;
+  cfg.tile_w = settings->tile_w;
+  cfg.tile_h = settings->tile_h;
+  cfg.shade_batch = settings->shade_batch;
+  cfg.trace_turns = settings->trace_turns;
   DenoiseArgs da;
   if (denoise && (s = denoise_args(film, cam, denoise, film->samples + samples, &da)) != LUCENT_OK) return s;
   DisplayArgs disp;

@@ -279,7 +279,8 @@ struct DeferTracer {
 
 // Lanes whose rays have ended wait for shading until this many of them are
 // ready (or no lane is tracing), so that the shading code runs with most
-// lanes of the wave.
+// lanes of the wave: kShadeBatch for the whole-pixel hand-out, the item's
+// shade_batch for the per-sample hand-out.
 constexpr int kShadeBatch = LUCENT_SHADE_BATCH;
 
 // run_item for scenes whose traversal draws no random numbers (no
@@ -389,9 +390,9 @@ struct FusedPath {
 };
 
 // True when the wave should shade now: enough lanes wait, or none traces.
-__device__ __forceinline__ bool shade_now(uint32_t state) {
+__device__ __forceinline__ bool shade_now(uint32_t state, uint32_t batch = kShadeBatch) {
   const uint64_t ready = __ballot(state == kLaneShade || state == kLaneFinish);
-  return ready != 0 && (__popcll(ready) >= kShadeBatch || __ballot(state == kLaneTrace) == 0);
+  return ready != 0 && (uint32_t(__popcll(ready)) >= batch || __ballot(state == kLaneTrace) == 0);
 }
 
 // Up to kTraceTurns traversal steps while some lane traces, between the
@@ -401,8 +402,8 @@ __device__ __forceinline__ bool shade_now(uint32_t state) {
 constexpr int kTraceTurns = LUCENT_TRACE_TURNS;
 
 template <class Lane>
-__device__ __forceinline__ void trace_turns(Lane& lp) {
-  for (int turn = 0; turn < kTraceTurns && __ballot(lp.state == kLaneTrace) != 0; ++turn)
+__device__ __forceinline__ void trace_turns(Lane& lp, uint32_t turns = kTraceTurns) {
+  for (uint32_t turn = 0; turn < turns && __ballot(lp.state == kLaneTrace) != 0; ++turn)
     if (lp.state == kLaneTrace) lp.trace_turn();
 }
 
@@ -491,15 +492,17 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
   const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
   const uint32_t spp = it.spp;
   const uint32_t lane = threadIdx.x;
+  const uint32_t tile_w = it.tile_w, tile_h = it.tile_h, tile_px = tile_w * tile_h;
+  const uint32_t batch = it.shade_batch, turns = it.trace_turns;
   // The job being handed out (wave-uniform): tile corner, width, pixels,
   // and the next of its px * spp samples.
   uint32_t cur = 0, x0 = 0, y0 = 0, tw = 1, px = 0, units = 0, handed = 0;
   auto open_job = [&](uint32_t j) {
     cur = j;
-    x0 = (j % it.tiles_x) * kTile;
-    y0 = (j / it.tiles_x) * kTile;
-    tw = min(uint32_t(kTile), f.width - x0);
-    px = tw * min(uint32_t(kTile), f.height - y0);
+    x0 = (j % it.tiles_x) * tile_w;
+    y0 = (j / it.tiles_x) * tile_h;
+    tw = min(tile_w, f.width - x0);
+    px = tw * min(tile_h, f.height - y0);
     units = px * spp;
     handed = 0;
   };
@@ -525,15 +528,15 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
         const uint32_t u = handed + rank;
         const uint32_t slot = u % px;
         k = u / px;
-        rslot = cur * kPtBlock + slot;
+        rslot = cur * tile_px + slot;
         idx = (y0 + slot / tw) * f.width + x0 + slot % tw;
         lp.start(f, idx % f.width, idx / f.width, idx, seed0 + k);
       }
       handed += min(uint32_t(__popcll(idle)), avail);
       idle = __ballot(lp.state == kLaneIdle);
     }
-    trace_turns(lp);
-    if (shade_now(lp.state)) {
+    trace_turns(lp, turns);
+    if (shade_now(lp.state, batch)) {
       if (lp.state == kLaneShade) lp.shade(f);
       if (lp.state == kLaneFinish) {
         const pt::PathState& p = lp.p;
@@ -586,16 +589,18 @@ __device__ __forceinline__ uint32_t run_units_ring(const TileItem& it, uint32_t 
   const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
   const uint32_t spp = it.spp;
   const uint32_t lane = threadIdx.x;
+  const uint32_t tile_w = it.tile_w, tile_h = it.tile_h;
+  const uint32_t batch = it.shade_batch, turns = it.trace_turns;
   // The job being handed out and the ring (wave-uniform).
   uint32_t x0 = 0, y0 = 0, tw = 1, px = 0, units = 0, handed = 0;
   uint32_t cur_ring = 0, next_ring = 0;
   uint32_t left[kRingJobs] = {};  // pixels not yet folded per ring slot
   auto open_job = [&](uint32_t j) {
     const uint32_t t = j - job_base;
-    x0 = (t % it.tiles_x) * kTile;
-    y0 = (t / it.tiles_x) * kTile;
-    tw = min(uint32_t(kTile), f.width - x0);
-    px = tw * min(uint32_t(kTile), f.height - y0);
+    x0 = (t % it.tiles_x) * tile_w;
+    y0 = (t / it.tiles_x) * tile_h;
+    tw = min(tile_w, f.width - x0);
+    px = tw * min(tile_h, f.height - y0);
     units = px * spp;
     handed = 0;
     cur_ring = next_ring;
@@ -641,8 +646,8 @@ __device__ __forceinline__ uint32_t run_units_ring(const TileItem& it, uint32_t 
       handed += min(uint32_t(__popcll(idle)), avail);
       idle = __ballot(lp.state == kLaneIdle);
     }
-    trace_turns(lp);
-    if (shade_now(lp.state)) {
+    trace_turns(lp, turns);
+    if (shade_now(lp.state, batch)) {
       if (lp.state == kLaneShade) lp.shade(f);
       bool folded = false;
       if (lp.state == kLaneFinish) {
@@ -836,16 +841,37 @@ uint32_t tile_jobs(uint32_t width, uint32_t height) {
   return ((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
 }
 
+Status apply_launch_shape(const TileLaunch& cfg, TileItem* item) {
+  const uint32_t tw = cfg.tile_w ? cfg.tile_w : kTile, th = cfg.tile_h ? cfg.tile_h : kTile;
+  const uint32_t batch = cfg.shade_batch ? cfg.shade_batch : kShadeBatch;
+  const uint32_t turns = cfg.trace_turns ? cfg.trace_turns : kTraceTurns;
+  if (tw * th > uint32_t(kPtBlock)) return fail(LUCENT_E_INVALID, "launch tile holds more than 64 pixels");
+  if (batch > uint32_t(kPtBlock)) return fail(LUCENT_E_INVALID, "shade batch above 64 lanes");
+  if (turns > 64) return fail(LUCENT_E_INVALID, "trace turns above 64");
+  item->tile_w = tw;
+  item->tile_h = th;
+  item->shade_batch = batch;
+  item->trace_turns = turns;
+  item->tiles_x = (item->frame.width + tw - 1) / tw;
+  return LUCENT_OK;
+}
+
+// Jobs of an item in the per-sample hand-out (its tile shape).
+static uint32_t unit_jobs(const TileItem& it) {
+  return it.tiles_x * ((it.frame.height + it.tile_h - 1) / it.tile_h);
+}
+
 Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hipStream_t stream) {
-  item.tiles_x = (item.frame.width + kTile - 1) / kTile;
-  const uint32_t n_jobs = tile_jobs(item.frame.width, item.frame.height);
+  Status s = apply_launch_shape(cfg, &item);
+  if (s != LUCENT_OK) return s;
+  const uint32_t n_jobs = unit_jobs(item);
   const uint32_t waves = tile_waves(cfg.occupancy);
   const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
   const uint32_t lane_words = tile_lane_words(pt::pt_spill_words(item.scene.bvh));
-  Status s = ensure_spill(ctx, size_t(grid) * kPtBlock * lane_words * sizeof(uint32_t));
+  s = ensure_spill(ctx, size_t(grid) * kPtBlock * lane_words * sizeof(uint32_t));
   if (s != LUCENT_OK) return s;
-  const size_t slots = size_t(n_jobs) * kPtBlock;
+  const size_t slots = size_t(n_jobs) * item.tile_w * item.tile_h;
   const size_t per_sample = slots * kRecordFloat4 * sizeof(float4);
   // Path records of one launch at most ctx->record_budget bytes (48 B per
   // path: a 1440x900 film's 64 samples take 4.0 GB of the 288 GB); larger
@@ -964,13 +990,31 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   if (n_items > 1 && cfg.fused && cfg.units) {
     // Per-sample hand-out with a record ring of kRingJobs jobs per wave,
     // when the records fit the context's budget (else the pixel hand-out
-    // below).
+    // below).  Its jobs are tiles of the launch shape: the job table and the
+    // film descriptors are rewritten for it.
     uint32_t s_max = 0;
     for (uint32_t i = 0; i < n_items; ++i) s_max = std::max(s_max, items[i].spp);
-    const size_t ring_records = size_t(grid) * kRingJobs * kPtBlock * s_max * kRecordFloat4 * sizeof(float4);
+    std::vector<TileItem> shaped(items, items + n_items);
+    std::vector<uint32_t> ubegin(n_items + 1, 0);
+    for (uint32_t i = 0; i < n_items; ++i) {
+      if ((s = apply_launch_shape(cfg, &shaped[i])) != LUCENT_OK) return s;
+      ubegin[i + 1] = ubegin[i] + (shaped[i].spp ? unit_jobs(shaped[i]) : 0);
+    }
+    const uint32_t u_jobs = ubegin[n_items];
+    const uint32_t u_grid = std::min<uint32_t>(u_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
+    const size_t ring_records = size_t(u_grid) * kRingJobs * kPtBlock * s_max * kRecordFloat4 * sizeof(float4);
     const size_t counters_off = (ring_records + 255) / 256 * 256;
     if (s_max > 0 && ring_records <= ctx->record_budget) {
-      if ((s = ensure_units(ctx, counters_off + size_t(grid) * kRingJobs * kPtBlock * sizeof(uint32_t))) != LUCENT_OK)
+      if ((s = ensure_spill(ctx, size_t(u_grid) * kPtBlock * lane_words * sizeof(uint32_t))) != LUCENT_OK) return s;
+      spill = static_cast<uint32_t*>(ctx->spill);
+      std::vector<char> host(bytes - begin_off);
+      std::memcpy(host.data(), ubegin.data(), ubegin.size() * sizeof(uint32_t));
+      std::memcpy(host.data() + (items_off - begin_off), shaped.data(), size_t(n_items) * sizeof(TileItem));
+      hipError_t ue = hipMemcpyAsync(table + begin_off, host.data(), host.size(), hipMemcpyHostToDevice, stream);
+      if (ue == hipSuccess) ue = hipEventRecord(static_cast<hipEvent_t>(ctx->tile_event), stream);
+      if (ue == hipSuccess) ue = hipEventSynchronize(static_cast<hipEvent_t>(ctx->tile_event));
+      if (ue != hipSuccess) return fail_hip(ue, "tile table upload");
+      if ((s = ensure_units(ctx, counters_off + size_t(u_grid) * kRingJobs * kPtBlock * sizeof(uint32_t))) != LUCENT_OK)
         return s;
       float4* records = static_cast<float4*>(ctx->units);
       uint32_t* arrived = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->units) + counters_off);
@@ -979,8 +1023,8 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
                           : waves == 5 ? k_trace_batch_units<5>
                           : waves == 6 ? k_trace_batch_units<6>
                                        : k_trace_batch_units<2>;
-      hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kPtBlock), 0, stream, d_items, d_begin, n_items, n_jobs, counter,
-                         spill, lane_words, records, arrived, s_max);
+      hipLaunchKernelGGL(batch_kernel, dim3(u_grid), dim3(kPtBlock), 0, stream, d_items, d_begin, n_items, u_jobs,
+                         counter, spill, lane_words, records, arrived, s_max);
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) return fail_hip(e, "k_trace_batch_units launch");
       return LUCENT_OK;

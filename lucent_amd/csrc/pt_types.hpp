@@ -99,6 +99,14 @@ struct TileItem {
   PtFilm film;
   uint32_t spp = 0;
   uint32_t tiles_x = 0;
+  // Launch shape of the per-sample hand-out (k_trace_units and the batch's
+  // record ring): a job is a tile_w x tile_h tile (at most 64 pixels) with
+  // all its samples; a wave shades once shade_batch of its lanes wait (or
+  // none traces) and runs up to trace_turns traversal steps between its
+  // checks for shading and new work.  None of them changes any path, so
+  // films are bit-identical for every choice (the autotuner's search space).
+  uint32_t tile_w = 8, tile_h = 8;
+  uint32_t shade_batch = 32, trace_turns = 8;
   // Optional: each path's radiance as tracePath returns it, float4 per pixel
   // per sample, sample-major (sample k of the launch at k * width * height).
   float* sample_radiance = nullptr;

@@ -32,6 +32,9 @@ Status launch_trace_reprojected(Context* ctx, const PtScene& sc, const PtFrame& 
 struct TileLaunch {
   uint32_t waves_per_cu = 0;  // persistent waves per CU (0: 4 x occupancy)
   uint32_t occupancy = 0;     // register-allocation target in waves per SIMD: 2..6 (0: 4)
+  // Per-sample hand-out shape (TileItem): tile of a job, shading batch and
+  // traversal turns per check; 0 = 8 x 8, 32, 8.
+  uint32_t tile_w = 0, tile_h = 0, shade_batch = 0, trace_turns = 0;
   // Optional caller-owned job counter, film descriptor slot and spill buffer
   // (one film only); otherwise the context's buffers are used.
   uint32_t* counter = nullptr;
@@ -52,6 +55,9 @@ size_t tile_spill_bytes(const Context* ctx, const TileItem& item, const TileLaun
 // *p += v on the device (a graph node).
 Status launch_add_u32(uint32_t* p, uint32_t v, void* stream);
 uint32_t tile_jobs(uint32_t width, uint32_t height);
+// Checks a launch shape (tile of at most 64 pixels, shade batch 1..64,
+// trace turns 1..64; 0 = default) and writes it into `item`.
+Status apply_launch_shape(const TileLaunch& cfg, TileItem* item);
 // Runs items[i].spp samples of every pixel of each item's film, seeds from
 // items[i].frame.seed; fills items[i].tiles_x.
 Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const TileLaunch& cfg, void* stream);

@@ -44,6 +44,7 @@ class DiffRenderer:
         self.env = np.asarray(env, np.float32).copy()
         self._version = 0      # bumped by every parameter change
         self.adjoint_occupancy = 4  # register variant of the replay kernel (2, 3, 4); 4 measured fastest
+        self.adjoint_tile = (0, 0)  # job tile of the replay (0 = 8 x 8)
         self._record = None    # (spp, seed0, version, device buffer) of the last forward
 
     @property
@@ -64,22 +65,30 @@ class DiffRenderer:
         self.env = arr.copy()
         self._version += 1
 
-    def forward(self, spp: int, seed0: int = 0, record: bool = True, device: bool = False):
+    def forward(self, spp: int, seed0: int = 0, record: bool = True, device: bool = False,
+                stream: Optional[int] = None):
         """Image (H, W, 3) as float64, rows bottom-up (lucent_forward).  With
         `record`, each path's radiance is kept on the device so that a
-        backward() with the same parameters and seeds need not re-trace."""
+        backward() with the same parameters and seeds need not re-trace.
+        `stream`: a HIP stream to enqueue on (the call then returns without
+        waiting; device=True is required)."""
         import torch
         r = self.renderer
         buf = None
         if record:
             buf = torch.empty(spp * self.height * self.width * 4, dtype=torch.float32, device=torch.device("cuda:0"))
         cam = camera_struct(self.scene.camera)
-        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0, self.settings.occupancy)
-        torch.cuda.synchronize()
+        st = self.settings.struct()
+        st.firefly_clamp, st.flags, st.waves_per_cu = 0.0, 0, 0
+        if stream is not None and not device:
+            raise ValueError("forward(stream=...) returns the device image: pass device=True")
+        if stream is None:
+            torch.cuda.synchronize()
         _native.check(_native.lib().lucent_forward(self.ctx.handle, r.device_scene.handle, r.film.handle, C.byref(cam),
                                                    C.byref(st), spp, seed0, buf.data_ptr() if buf is not None else None,
-                                                   None))
-        self.ctx.synchronize()
+                                                   stream))
+        if stream is None:
+            self.ctx.synchronize()
         self._record = (spp, seed0, self._version, buf) if record else None
         if device:
             return self.image_view()
@@ -114,13 +123,14 @@ class DiffRenderer:
         return out.cpu().numpy().reshape(h, w, 4)[..., :3].astype(np.float64)
 
     def backward(self, dl_di, spp: int, seed0: int = 0, env: bool = True, use_record: bool = True,
-                 device: bool = False):
+                 device: bool = False, stream: Optional[int] = None):
         """Gradients (material_count x 5, env (h, w, 3)) of sum dl_di · image.
         dl_di: (h, w, 3) host array, or a torch float32 (h, w, 4) tensor on the
         device (its fourth channel is ignored).  Uses the radiance recorded by
         the last forward() when it was made with the same samples, seeds and
         parameters; otherwise re-traces.  device=True returns the float64
-        gradients as device tensors instead of host arrays."""
+        gradients as device tensors instead of host arrays; `stream` enqueues
+        on that HIP stream without waiting (device=True required)."""
         import torch
         dev = torch.device("cuda:0")
         h, w = self.height, self.width
@@ -137,15 +147,20 @@ class DiffRenderer:
         ew, eh = self.env_size
         g_env = torch.zeros(ew * eh * 3, dtype=torch.float64, device=dev) if env else None
         cam = camera_struct(self.scene.camera)
-        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0, self.adjoint_occupancy)
-        torch.cuda.synchronize()
+        st = _native.RenderSettingsStruct(self.settings.max_bounces, 0.0, 0, 0, self.adjoint_occupancy,
+                                          *self.adjoint_tile)
+        if stream is not None and not device:
+            raise ValueError("backward(stream=...) returns device gradients: pass device=True")
+        if stream is None:
+            torch.cuda.synchronize()
         rec = self._record
         use = use_record and rec is not None and rec[:3] == (spp, seed0, self._version)
         _native.check(_native.lib().lucent_backward(
             self.ctx.handle, self.renderer.device_scene.handle, C.byref(cam), C.byref(st), w, h, spp, seed0,
             d_dl.data_ptr(), rec[3].data_ptr() if use else None, g_mat.data_ptr(), g_env.data_ptr() if env else None,
-            None))
-        self.ctx.synchronize()
+            stream))
+        if stream is None:
+            self.ctx.synchronize()
         if device:
             return g_mat.reshape(nm, 5), (g_env.reshape(eh, ew, 3) if env else None)
         gm = g_mat.cpu().numpy().reshape(nm, 5)

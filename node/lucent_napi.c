@@ -491,6 +491,10 @@ static void read_settings(napi_env env, napi_value o, lucent_render_settings *rs
               (num(env, prop(env, o, "inlineShadows"), 0) != 0 ? LUCENT_TRACE_INLINE_SHADOWS : 0u);
   rs->waves_per_cu = (uint32_t)num(env, prop(env, o, "wavesPerCU"), 0);
   rs->occupancy = (uint32_t)num(env, prop(env, o, "occupancy"), 0);
+  rs->tile_w = (uint32_t)num(env, prop(env, o, "tileWidth"), 0);
+  rs->tile_h = (uint32_t)num(env, prop(env, o, "tileHeight"), 0);
+  rs->shade_batch = (uint32_t)num(env, prop(env, o, "shadeBatch"), 0);
+  rs->trace_turns = (uint32_t)num(env, prop(env, o, "traceTurns"), 0);
 }
 
 /* traceReprojected(ctx, scene, film, camera, prevFilm, prevCamera, historyCap, settings):

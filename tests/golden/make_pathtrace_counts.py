@@ -19,13 +19,11 @@ ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
 
 import oracle as O  # noqa: E402
-from lucent_amd import scene as S  # noqa: E402
 from lucent_amd import workloads  # noqa: E402
 
 
 def main():
     sc = workloads.knot_scene()
-    packed = S.pack_scene(sc)
     o = O.PtOracle(sc)
     W, H = workloads.PT_WIDTH, workloads.PT_HEIGHT
     f = o.frame(sc.camera, W, H, workloads.PT_DEPTH, 0.0)

@@ -274,3 +274,20 @@ def test_recorded_radiance_survives_a_split_launch(ctx, diff):
         ctx.set_record_budget(0)
     assert np.array_equal(img, img2)
     assert torch.equal(rec, rec2)
+
+
+@pytest.mark.parametrize("tile", [(4, 4), (2, 2), (8, 2)])
+def test_replay_tile_gives_the_same_gradients(diff, tile):
+    """The replay's job tile changes only the order of the FP64 gradient sums."""
+    rng = np.random.default_rng(8)
+    dl = rng.standard_normal((H, W, 3))
+    diff.forward(SPP, 0)
+    diff.adjoint_tile = (0, 0)
+    gm0, ge0 = diff.backward(dl, SPP, 0)
+    diff.adjoint_tile = tile
+    try:
+        gm1, ge1 = diff.backward(dl, SPP, 0)
+    finally:
+        diff.adjoint_tile = (0, 0)
+    assert np.allclose(gm1, gm0, rtol=1e-9, atol=1e-12 * np.abs(gm0).max())
+    assert np.allclose(ge1, ge0, rtol=1e-9, atol=1e-12 * np.abs(ge0).max())

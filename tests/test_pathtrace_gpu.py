@@ -193,7 +193,7 @@ def test_persistent_tiles_bit_identical_to_per_sample_launches(ctx, clamp, occup
 
 
 def test_autotune_picks_a_variant_and_leaves_films_alone(ctx):
-    from lucent_amd.renderer import autotune
+    from lucent_amd.renderer import autotune_occupancy as autotune
     r = Renderer(ctx, 48, 40, RenderSettings(8, 0.0))
     r.load_scene(feature_scene())
     r.trace_samples(2)
@@ -213,6 +213,74 @@ def test_autotune_picks_a_variant_and_leaves_films_alone(ctx):
     assert r.film.info() == info
     for k in before:
         assert np.array_equal(before[k], after[k])
+    r.close()
+
+
+def test_launch_autotuner_searches_the_shape_and_leaves_films_alone(ctx):
+    """lucent_autotune_launch: occupancy first, then tile, shade batch and
+    traversal turns, each candidate kept only when > 1 % faster; the chosen
+    shape is one of the candidates and the caller's film is untouched."""
+    from lucent_amd.renderer import autotune
+    r = Renderer(ctx, 64, 48, RenderSettings(8, 0.0))
+    r.load_scene(feature_scene())
+    r.trace_samples(2)
+    before, info = r.film.read(), r.film.info()
+    st = RenderSettings(8, 0.0)
+    res = autotune(ctx, [(r, 8)], st, trial_samples=2)
+    trials = res["trials"]
+    assert len(trials) == 5 + 4 + 4 + 3
+    assert [t["occupancy"] for t in trials[:5]] == [2, 3, 4, 5, 6]
+    assert all(t["ms"] > 0 for t in trials)
+    chosen = (st.occupancy, st.tile_w, st.tile_h, st.shade_batch, st.trace_turns)
+    assert chosen in [(t["occupancy"], *t["tile"], t["shade_batch"], t["trace_turns"]) for t in trials]
+    print("autotuned", res["occupancy"], res["tile"], res["shade_batch"], res["trace_turns"])
+    after = r.film.read()
+    assert r.film.info() == info
+    for k in before:
+        assert np.array_equal(before[k], after[k])
+    r.close()
+
+
+SHAPES = [dict(tile_w=4, tile_h=4), dict(tile_w=2, tile_h=2), dict(tile_w=8, tile_h=2, shade_batch=16),
+          dict(shade_batch=48, trace_turns=3), dict(tile_w=1, tile_h=1, shade_batch=1, trace_turns=1)]
+
+
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+def test_launch_shape_keeps_films_bit_identical(ctx, shape):
+    """Every launch shape (job tile, shade batch, traversal turns) gives the
+    film of the per-sample launches, for one film and for a batch (whose
+    record ring takes the same shape)."""
+    from lucent_amd.renderer import trace_batch
+    kw = SHAPES[shape]
+    sc = feature_scene()
+    want, info = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, 0.0, per_sample_launch=True))
+    got, gi = _film_after(ctx, sc, 70, 45, 7, 5, RenderSettings(8, 0.0, **kw))
+    assert gi == info
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+    settings = RenderSettings(8, 0.0, **kw)
+    rs = []
+    for (w, h) in ((64, 40), (33, 17)):
+        r = Renderer(ctx, w, h, settings)
+        r.load_scene(sc)
+        r.film.set_seed(11)
+        r.reset()
+        rs.append(r)
+    trace_batch(ctx, [(rs[0], 5), (rs[1], 3)], settings)
+    for r, n in zip(rs, (5, 3)):
+        w_, _ = _film_after(ctx, sc, r.film.width, r.film.height, n, 11, RenderSettings(8, 0.0, per_sample_launch=True))
+        g_ = r.film.read()
+        for k in w_:
+            assert np.array_equal(g_[k], w_[k]), (r.film.width, k)
+        r.close()
+
+
+def test_launch_shape_rejects_oversized_tiles(ctx):
+    from lucent_amd._native import LucentError
+    r = Renderer(ctx, 16, 16, RenderSettings(8, 0.0, tile_w=16, tile_h=8))
+    r.load_scene(feature_scene())
+    with pytest.raises(LucentError, match="64 pixels"):
+        r.trace_samples(1)
     r.close()
 
 
