@@ -50,6 +50,8 @@ const char *lucent_last_error(const lucent_ctx *ctx);
  * Replaces the WebGL2 context set-up of Renderer's constructor
  * (src/renderer.ts:226; src/gl/context.ts:18-29, which throws GLError). */
 lucent_status lucent_create(int device, lucent_ctx **out);
+/* Destroy the context's scenes, films and frame graphs first: they refer to
+ * its stream. */
 void lucent_destroy(lucent_ctx *ctx);
 
 /* Waits for all work enqueued on the context's stream. */

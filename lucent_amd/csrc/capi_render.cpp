@@ -655,24 +655,27 @@ lucent_status lucent_autotune_launch(lucent_ctx* ctx, const lucent_batch_item* i
     }
     return r;
   };
+  const uint32_t tile0[2] = {best.tile_w, best.tile_h};
   for (const auto& wh : tiles_wh) {
     if (s != LUCENT_OK) break;
-    if (wh[0] == best.tile_w && wh[1] == best.tile_h) continue;
+    if (wh[0] == tile0[0] && wh[1] == tile0[1]) continue;  // timed already
     TileLaunch c = best;
     c.tile_w = wh[0];
     c.tile_h = wh[1];
     s = consider(c);
   }
+  const uint32_t batch0 = best.shade_batch;
   for (uint32_t b : batches) {
     if (s != LUCENT_OK) break;
-    if (b == best.shade_batch) continue;
+    if (b == batch0) continue;
     TileLaunch c = best;
     c.shade_batch = b;
     s = consider(c);
   }
+  const uint32_t turns0 = best.trace_turns;
   for (uint32_t t : turns) {
     if (s != LUCENT_OK) break;
-    if (t == best.trace_turns) continue;
+    if (t == turns0) continue;
     TileLaunch c = best;
     c.trace_turns = t;
     s = consider(c);

@@ -123,6 +123,7 @@ class Film:
         _native.check(_native.lib().lucent_film_create(ctx.handle, width, height, C.byref(h)))
         self._h = h
         self.width, self.height = width, height
+        ctx.adopt(self)
 
     @property
     def handle(self):
@@ -422,6 +423,7 @@ class FrameGraph:
             C.byref(h)))
         self._h = h
         self.samples = samples
+        r.ctx.adopt(self)
 
     def launch(self, stream: Optional[int] = None) -> None:
         _native.check(_native.lib().lucent_frame_graph_launch(self._h, stream))

@@ -7,6 +7,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from dataclasses import dataclass
 from typing import Optional
 
@@ -27,6 +28,14 @@ class Context:
         h = C.c_void_p()
         _native.check(_native.lib().lucent_create(device, C.byref(h)))
         self._h = h
+        # Scenes, films and frame graphs made on this context: the C ABI
+        # wants them destroyed before the context, so close() does that for
+        # any still alive.
+        self._children = weakref.WeakSet()
+
+    def adopt(self, child) -> None:
+        """Registers an object with a destroy()/close() method that close() must run first."""
+        self._children.add(child)
 
     @property
     def handle(self) -> C.c_void_p:
@@ -55,6 +64,8 @@ class Context:
 
     def close(self) -> None:
         if self._h is not None:
+            for child in list(self._children):
+                (getattr(child, "close", None) or child.destroy)()
             _native.lib().lucent_destroy(self._h)
             self._h = None
 
@@ -130,6 +141,7 @@ class Scene:
     def __init__(self, ctx: Context, handle: C.c_void_p):
         self.ctx = ctx
         self._h = handle
+        ctx.adopt(self)
 
     @property
     def handle(self):
