@@ -863,6 +863,8 @@ static Status denoise_args(lucent_film* film, const lucent_camera* cam, const lu
     Status s = dev_alloc(&film->filter, px * (16 * 4 + 8));
     if (s != LUCENT_OK) return s;
   }
+  a->compute_units = std::max(film->ctx->compute_units, 1);
+  a->compute_units = std::max(film->ctx->compute_units, 1);
   a->color = film->f.color;
   a->albedo = film->f.albedo;
   a->normal = film->f.normal;

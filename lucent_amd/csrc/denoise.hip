@@ -15,6 +15,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <utility>
 
 #include "denoise.hpp"
@@ -53,10 +54,12 @@ __device__ inline void unit_normal(float x, float y, float z, float& ox, float& 
 
 // primaryDir (atrous.frag:22-26) before its normalisation: the camera ray
 // through the pixel centre as fwd + right*sx + up*sy.
+// (The quotients use the hardware reciprocal, 1 ulp.)
 __device__ inline void ray_vector(const Cam& c, int px, int py, int w, int h, float& vx, float& vy, float& vz) {
-  const float ux = (float(px) + 0.5f) / float(w) * 2.0f - 1.0f;
-  const float uy = (float(py) + 0.5f) / float(h) * 2.0f - 1.0f;
-  const float aspect = float(w) / float(h);
+  const float iw = __builtin_amdgcn_rcpf(float(w)), ih = __builtin_amdgcn_rcpf(float(h));
+  const float ux = (float(px) + 0.5f) * iw * 2.0f - 1.0f;
+  const float uy = (float(py) + 0.5f) * ih * 2.0f - 1.0f;
+  const float aspect = float(w) * ih;
   const float sx = ux * aspect * c.tan_half_fov, sy = uy * c.tan_half_fov;
   vx = c.fwd[0] + c.right[0] * sx + c.up[0] * sy;
   vy = c.fwd[1] + c.right[1] * sx + c.up[1] * sy;
@@ -158,6 +161,11 @@ struct Centre {
 
 constexpr float kLog2e = 1.4426950408889634f;
 
+// The hardware reciprocal (1 ulp): the edge stops and the final
+// normalisation need far less than correctly rounded division, which costs
+// ten instructions per quotient here.
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ inline void centre_begin(Centre& c, float4 cp, float4 gp, float var_blur, const Cam& cam, int x, int y,
                                     int w, int h, int step, float sigma_l, float sigma_p) {
   c.nx = gp.x;
@@ -166,9 +174,9 @@ __device__ inline void centre_begin(Centre& c, float4 cp, float4 gp, float var_b
   float vx, vy, vz;
   ray_vector(cam, x, y, w, h, vx, vy, vz);
   const float a0 = c.nx * vx + c.ny * vy + c.nz * vz;
-  const float aspect = float(w) / float(h);
-  const float dsx = 2.0f / float(w) * aspect * cam.tan_half_fov * float(step);
-  const float dsy = 2.0f / float(h) * cam.tan_half_fov * float(step);
+  const float inv_h = rcp(float(h));
+  const float dsx = 2.0f * inv_h * cam.tan_half_fov * float(step);  // 2/w * (w/h)
+  const float dsy = 2.0f * inv_h * cam.tan_half_fov * float(step);
   c.ax = (c.nx * cam.right[0] + c.ny * cam.right[1] + c.nz * cam.right[2]) * dsx;
   const float ay = (c.nx * cam.up[0] + c.ny * cam.up[1] + c.nz * cam.up[2]) * dsy;
   c.row[0] = a0 - 2.0f * ay;
@@ -179,14 +187,14 @@ __device__ inline void centre_begin(Centre& c, float4 cp, float4 gp, float var_b
   c.d = gp.w * a0;
   // footprint = 2 tan(fov/2) depth / H with depth = tau_p |v_p|.
   const float depth = gp.w * sqrtf(vx * vx + vy * vy + vz * vz);
-  const float footprint = 2.0f * cam.tan_half_fov * depth / float(h);
+  const float footprint = 2.0f * cam.tan_half_fov * depth * inv_h;
   const float base = sigma_p * footprint * float(step);
-  c.irl[0] = kLog2e / (base * 1.0f + 1e-6f);
-  c.irl[1] = kLog2e / (base * 1.41421356f + 1e-6f);
-  c.irl[2] = kLog2e / (base * 2.0f + 1e-6f);
-  c.irl[3] = kLog2e / (base * 2.23606798f + 1e-6f);
-  c.irl[4] = kLog2e / (base * 2.82842712f + 1e-6f);
-  c.il = kLog2e / (sigma_l * sqrtf(fmaxf(var_blur, 0.0f)) + 1e-6f);
+  c.irl[0] = kLog2e * rcp(base * 1.0f + 1e-6f);
+  c.irl[1] = kLog2e * rcp(base * 1.41421356f + 1e-6f);
+  c.irl[2] = kLog2e * rcp(base * 2.0f + 1e-6f);
+  c.irl[3] = kLog2e * rcp(base * 2.23606798f + 1e-6f);
+  c.irl[4] = kLog2e * rcp(base * 2.82842712f + 1e-6f);
+  c.il = kLog2e * rcp(sigma_l * sqrtf(fmaxf(var_blur, 0.0f)) + 1e-6f);
   c.lp = lum3(cp.x, cp.y, cp.z);
   constexpr float h0 = 0.375f;
   c.sr = cp.x * (h0 * h0);
@@ -203,7 +211,7 @@ __host__ __device__ constexpr int reach_slot(int ox, int oy) {
 }
 
 template <int OX, int OY, bool kNormalStop>
-__device__ __forceinline__ void tap(Centre& c, float4 cq, float4 gq, float sigma_n) {
+__device__ __forceinline__ void tap(Centre& c, float4 cq, float4 gq, float lq, float sigma_n) {
   constexpr float log2k = log2_tap(iabs(OX)) + log2_tap(iabs(OY));
   const float dot = c.nx * gq.x + c.ny * gq.y + c.nz * gq.z;
   float e;
@@ -218,8 +226,7 @@ __device__ __forceinline__ void tap(Centre& c, float4 cq, float4 gq, float sigma
   if constexpr (OX == -1) nv -= c.ax;
   if constexpr (OX == 2 || OX == -2) nv = fmaf(float(OX), c.ax, nv);
   const float plane = fabsf(fmaf(gq.w, nv, -c.d));
-  // |l_q - l_p| as one multiply-add chain from -l_p.
-  const float dl = fabsf(fmaf(cq.z, 0.0722f, fmaf(cq.y, 0.7152f, fmaf(cq.x, 0.2126f, -c.lp))));
+  const float dl = fabsf(lq - c.lp);
   e -= plane * c.irl[reach_slot(OX, OY)];
   e -= dl * c.il;
   const float wgt = __builtin_amdgcn_exp2f(e);
@@ -232,7 +239,7 @@ __device__ __forceinline__ void tap(Centre& c, float4 cq, float4 gq, float sigma
 
 __device__ inline void centre_store(const Centre& c, size_t i, float4* out, float* var_out, const uint16_t* albedo,
                                     float4* remodulated) {
-  const float inv = 1.0f / c.wsum;
+  const float inv = rcp(c.wsum);
   const float4 r = make_float4(c.sr * inv, c.sg * inv, c.sb * inv, c.svar * (inv * inv));
   out[i] = r;
   var_out[i] = r.w;
@@ -274,8 +281,12 @@ __device__ inline float blur_variance(const float v[3][3], int x, int y, int w, 
 // (step 16) the pixels they filter but leave LDS for 4-6 blocks per CU,
 // which measured faster than taller tiles with less halo.
 constexpr int kTileW = 64, kWaves = 4;
+constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
 
-__host__ __device__ constexpr int tile_rows(int) { return 4; }
+#ifndef LUCENT_ATROUS_ROWS
+#define LUCENT_ATROUS_ROWS 4
+#endif
+__host__ __device__ constexpr int tile_rows(int) { return LUCENT_ATROUS_ROWS; }
 
 template <int S>
 struct AtrousTile {
@@ -285,82 +296,172 @@ struct AtrousTile {
   static constexpr int vcols = kTileW + 2;
 };
 
+// Tile t in the XCD-aware order: column tiles fastest, then the row bands
+// of one residue class mod S, then the classes.
+template <int S>
+__device__ inline void tile_origin(uint32_t t, int w, int h, int& x0, int& y0) {
+  using T = AtrousTile<S>;
+  const uint32_t cols_n = (uint32_t(w) + kTileW - 1) / kTileW;
+  const uint32_t bands = (uint32_t((h + S - 1) / S) + T::R - 1) / T::R;
+  const uint32_t col = t % cols_n, rest = t / cols_n, band = rest % bands, cls = rest / bands;
+  x0 = int(col) * kTileW;
+  y0 = int(cls) + int(band) * T::R * S;
+}
+
+// A thread's share of one tile's staged pixels, held in registers between
+// the global loads and the LDS stores.
+template <int S>
+struct TileRegs {
+  using T = AtrousTile<S>;
+  static constexpr int kPix = (T::rows * T::cols + kWaves * 64 - 1) / (kWaves * 64);
+  static constexpr int kVar = (T::R * 3 * T::vcols + kWaves * 64 - 1) / (kWaves * 64);
+  float4 c[kPix], g[kPix];
+  float v[kVar];
+};
+
+template <int S>
+__device__ __forceinline__ void tile_load(TileRegs<S>& r, const float4* __restrict__ in,
+                                          const float4* __restrict__ guides, const float* __restrict__ var_in, int w,
+                                          int h, int x0, int y0) {
+  using T = AtrousTile<S>;
+#pragma unroll
+  for (int k = 0; k < TileRegs<S>::kPix; ++k) {
+    const int e = int(threadIdx.x) + k * kWaves * 64;
+    const int j = e / T::cols, cc = e % T::cols;
+    const int gy = y0 + (j - 2) * S, gx = x0 - 2 * S + cc;
+    float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g = c;
+    if (e < T::rows * T::cols && gx >= 0 && gx < w && gy >= 0 && gy < h) {
+      const size_t q = size_t(gy) * w + gx;
+      c = in[q];
+      g = guides[q];
+    }
+    r.c[k] = c;
+    r.g[k] = g;
+  }
+#pragma unroll
+  for (int k = 0; k < TileRegs<S>::kVar; ++k) {
+    const int e = int(threadIdx.x) + k * kWaves * 64;
+    const int j = e / T::vcols, cc = e % T::vcols;
+    const int gy = y0 + (j / 3) * S + (j % 3) - 1, gx = x0 - 1 + cc;
+    r.v[k] = e < T::R * 3 * T::vcols && gx >= 0 && gx < w && gy >= 0 && gy < h ? var_in[size_t(gy) * w + gx] : 0.0f;
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void tile_store(const TileRegs<S>& r, float4* sc, float4* sg, float* sl, float* sv) {
+  using T = AtrousTile<S>;
+#pragma unroll
+  for (int k = 0; k < TileRegs<S>::kPix; ++k) {
+    const int e = int(threadIdx.x) + k * kWaves * 64;
+    if (e < T::rows * T::cols) {
+      sc[e] = r.c[k];
+      sg[e] = r.g[k];
+      sl[e] = lum3(r.c[k].x, r.c[k].y, r.c[k].z);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < TileRegs<S>::kVar; ++k) {
+    const int e = int(threadIdx.x) + k * kWaves * 64;
+    if (e < T::R * 3 * T::vcols) sv[e] = r.v[k];
+  }
+}
+
+// Persistent: each block filters a run of tiles, loading the next tile's
+// pixels into registers while it filters the current one from LDS, so the
+// global-memory latency of the staging hides behind the taps (a block that
+// loaded, synchronised and then filtered exposed that latency once per
+// tile).  Tiles are dealt in the XCD-aware order: workgroups go to the 8
+// XCDs round robin (b mod 8); XCD k's blocks take interleaved tiles of a
+// contiguous run of the order, so the halos that neighbouring tiles share
+// stay in its own L2.
+#ifndef LUCENT_ATROUS_WAVES_PER_EU
+#define LUCENT_ATROUS_WAVES_PER_EU 1
+#endif
 template <int S, bool kNormalStop>
-__global__ __launch_bounds__(kWaves * 64) void k_atrous_tile(const float4* __restrict__ in,
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(LUCENT_ATROUS_WAVES_PER_EU))) void k_atrous_tile(const float4* __restrict__ in,
                                                              const float4* __restrict__ guides,
                                                              const float* __restrict__ var_in,
                                                              float4* __restrict__ out, float* __restrict__ var_out,
                                                              const uint16_t* __restrict__ albedo,
                                                              float4* __restrict__ remodulated, int w, int h, Cam cam,
-                                                             float sigma_l, float sigma_n, float sigma_p) {
+                                                             float sigma_l, float sigma_n, float sigma_p,
+                                                             uint32_t n_tiles) {
   using T = AtrousTile<S>;
   __shared__ float4 sc[T::rows * T::cols];
   __shared__ float4 sg[T::rows * T::cols];
+  __shared__ float sl[T::rows * T::cols];  // luminance of the staged illumination
   __shared__ float sv[T::R * 3 * T::vcols];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x0 = blockIdx.x * kTileW;
-  const int y0 = int(blockIdx.y % S) + int(blockIdx.y / S) * T::R * S;
-  for (int j = wave; j < T::rows; j += kWaves) {
-    const int gy = y0 + (j - 2) * S;
-    for (int cc = lane; cc < T::cols; cc += 64) {
-      const int gx = x0 - 2 * S + cc;
-      float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g = c;
-      if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
-        const size_t q = size_t(gy) * w + gx;
-        c = in[q];
-        g = guides[q];
+  // This block's tiles: XCD k (the blocks b = k mod 8, `mine` of them)
+  // owns a contiguous run [first, end) of the order, in proportion to its
+  // blocks, dealt to its blocks round robin.
+  const uint32_t nb = gridDim.x, b = blockIdx.x, xcd = b % kXcds;
+  uint32_t before = 0;
+  for (uint32_t k = 0; k < xcd; ++k) before += (nb - k + kXcds - 1) / kXcds;
+  const uint32_t mine = (nb - xcd + kXcds - 1) / kXcds;
+  const uint32_t first = uint32_t(uint64_t(n_tiles) * before / nb);
+  const uint32_t end = uint32_t(uint64_t(n_tiles) * (before + mine) / nb);
+  const uint32_t blocks_in_xcd = mine;
+  uint32_t t = first + b / kXcds;
+  if (t >= end) return;
+  TileRegs<S> regs;
+  int x0, y0;
+  tile_origin<S>(t, w, h, x0, y0);
+  tile_load<S>(regs, in, guides, var_in, w, h, x0, y0);
+  for (;;) {
+    tile_store<S>(regs, sc, sg, sl, sv);
+    __syncthreads();
+    const int cx0 = x0, cy0 = y0;
+    const uint32_t next = t + blocks_in_xcd;
+    if (next < end) {
+      tile_origin<S>(next, w, h, x0, y0);
+      tile_load<S>(regs, in, guides, var_in, w, h, x0, y0);
+    }
+    const int x = cx0 + lane;
+    for (int r = wave; r < T::R; r += kWaves) {
+      const int y = cy0 + r * S;
+      if (x >= w || y >= h) continue;
+      const size_t i = size_t(y) * w + x;
+      const int ci = (r + 2) * T::cols + lane + 2 * S;
+      const float4 cp = sc[ci];
+      const float4 gp = sg[ci];
+      if (gp.x == 0.0f && gp.y == 0.0f && gp.z == 0.0f) {
+        passthrough(cp, i, out, var_out, albedo, remodulated);
+        continue;
       }
-      sc[j * T::cols + cc] = c;
-      sg[j * T::cols + cc] = g;
-    }
-  }
-  for (int j = wave; j < T::R * 3; j += kWaves) {
-    const int gy = y0 + (j / 3) * S + (j % 3) - 1;
-    for (int cc = lane; cc < T::vcols; cc += 64) {
-      const int gx = x0 - 1 + cc;
-      sv[j * T::vcols + cc] = gx >= 0 && gx < w && gy >= 0 && gy < h ? var_in[size_t(gy) * w + gx] : 0.0f;
-    }
-  }
-  __syncthreads();
-  const int x = x0 + lane;
-  for (int r = wave; r < T::R; r += kWaves) {
-    const int y = y0 + r * S;
-    if (x >= w || y >= h) continue;
-    const size_t i = size_t(y) * w + x;
-    const int ci = (r + 2) * T::cols + lane + 2 * S;
-    const float4 cp = sc[ci];
-    const float4 gp = sg[ci];
-    if (gp.x == 0.0f && gp.y == 0.0f && gp.z == 0.0f) {
-      passthrough(cp, i, out, var_out, albedo, remodulated);
-      continue;
-    }
-    float v[3][3];
+      float v[3][3];
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy)
+      for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-      for (int dx = 0; dx < 3; ++dx) v[dy][dx] = sv[(r * 3 + dy) * T::vcols + lane + dx];
-    Centre c;
-    centre_begin(c, cp, gp, blur_variance(v, x, y, w, h), cam, x, y, w, h, S, sigma_l, sigma_p);
-    const float4* pc = sc + ci;
-    const float4* pg = sg + ci;
-#define LUCENT_TAP(OX, OY) tap<OX, OY, kNormalStop>(c, pc[OY * T::cols + OX * S], pg[OY * T::cols + OX * S], sigma_n)
+        for (int dx = 0; dx < 3; ++dx) v[dy][dx] = sv[(r * 3 + dy) * T::vcols + lane + dx];
+      Centre c;
+      centre_begin(c, cp, gp, blur_variance(v, x, y, w, h), cam, x, y, w, h, S, sigma_l, sigma_p);
+      const float4* pc = sc + ci;
+      const float4* pg = sg + ci;
+      const float* pl = sl + ci;
+#define LUCENT_TAP(OX, OY) \
+  tap<OX, OY, kNormalStop>(c, pc[OY * T::cols + OX * S], pg[OY * T::cols + OX * S], pl[OY * T::cols + OX * S], sigma_n)
 #define LUCENT_ROW(OY) \
   LUCENT_TAP(-2, OY);  \
   LUCENT_TAP(-1, OY);  \
   LUCENT_TAP(0, OY);   \
   LUCENT_TAP(1, OY);   \
   LUCENT_TAP(2, OY)
-    LUCENT_ROW(-2);
-    LUCENT_ROW(-1);
-    LUCENT_TAP(-2, 0);
-    LUCENT_TAP(-1, 0);
-    LUCENT_TAP(1, 0);
-    LUCENT_TAP(2, 0);
-    LUCENT_ROW(1);
-    LUCENT_ROW(2);
+      LUCENT_ROW(-2);
+      LUCENT_ROW(-1);
+      LUCENT_TAP(-2, 0);
+      LUCENT_TAP(-1, 0);
+      LUCENT_TAP(1, 0);
+      LUCENT_TAP(2, 0);
+      LUCENT_ROW(1);
+      LUCENT_ROW(2);
 #undef LUCENT_ROW
 #undef LUCENT_TAP
-    centre_store(c, i, out, var_out, albedo, remodulated);
+      centre_store(c, i, out, var_out, albedo, remodulated);
+    }
+    if (next >= end) break;
+    t = next;
+    __syncthreads();  // every wave is done with this tile's LDS
   }
 }
 
@@ -398,7 +499,8 @@ __global__ __launch_bounds__(kTx* kTy) void k_atrous_direct(const float4* __rest
     const int qx = x + OX * step, qy = y + OY * step;                                             \
     const bool inside = qx >= 0 && qx < w && qy >= 0 && qy < h;                                   \
     const size_t q = inside ? size_t(qy) * w + qx : i;                                            \
-    tap<OX, OY, kNormalStop>(c, inside ? in[q] : zero, inside ? guides[q] : zero, sigma_n);       \
+    const float4 cq = inside ? in[q] : zero;                                                      \
+    tap<OX, OY, kNormalStop>(c, cq, inside ? guides[q] : zero, lum3(cq.x, cq.y, cq.z), sigma_n);  \
   }
 #define LUCENT_ROW(OY) \
   LUCENT_TAP(-2, OY);  \
@@ -430,15 +532,22 @@ struct PassArgs {
   int w, h, step;
   Cam cam;
   float sigma_l, sigma_n, sigma_p;
+  int compute_units;
 };
 
 template <int S, bool kNormalStop>
 void launch_tile(const PassArgs& p, hipStream_t stream) {
   const int rows_per_class = (p.h + S - 1) / S;
   const int R = AtrousTile<S>::R;
-  const dim3 grid((p.w + kTileW - 1) / kTileW, S * ((rows_per_class + R - 1) / R));
-  hipLaunchKernelGGL((k_atrous_tile<S, kNormalStop>), grid, dim3(kWaves * 64), 0, stream, p.in, p.guides, p.var_in,
-                     p.out, p.var_out, p.albedo, p.remodulated, p.w, p.h, p.cam, p.sigma_l, p.sigma_n, p.sigma_p);
+  const uint32_t n_tiles = uint32_t((p.w + kTileW - 1) / kTileW) * S * ((rows_per_class + R - 1) / R);
+  // Persistent grid: the blocks that fit the device at once.
+  auto kern = k_atrous_tile<S, kNormalStop>;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kWaves * 64, 0) != hipSuccess || per_cu < 1)
+    per_cu = 4;
+  const uint32_t grid = std::min<uint32_t>(n_tiles, uint32_t(per_cu) * uint32_t(std::max(p.compute_units, 1)));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.in, p.guides, p.var_in, p.out, p.var_out,
+                     p.albedo, p.remodulated, p.w, p.h, p.cam, p.sigma_l, p.sigma_n, p.sigma_p, n_tiles);
 }
 
 template <bool kNormalStop>
@@ -483,6 +592,7 @@ Status launch_denoise(const DenoiseArgs& a, void* stream_) {
   p.sigma_l = a.sigma_l;
   p.sigma_n = a.sigma_n;
   p.sigma_p = a.sigma_p;
+  p.compute_units = a.compute_units;
   float* read = a.ping;
   float* write = a.pong;
   float* var_read = a.var_a;

@@ -24,6 +24,7 @@ struct DenoiseArgs {
   float sigma_l = 4.0f, sigma_n = 64.0f, sigma_p = 1.5f;
   float cam_pos[3], cam_right[3], cam_up[3], cam_fwd[3];
   float tan_half_fov = 1.0f;
+  int compute_units = 256;           // sizes the persistent grid of the a-trous passes
 };
 
 Status launch_denoise(const DenoiseArgs& a, void* stream);
