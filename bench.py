@@ -473,7 +473,9 @@ def bench_batch(args) -> dict:
     error is equal by construction."""
     import torch
 
+    from lucent_amd import scene as S
     from lucent_amd import workloads
+    from lucent_amd.bvh import build_bvh_gpu
     from lucent_amd.renderer import Renderer, RenderSettings, trace_batch
     from lucent_amd.runtime import Context
 
@@ -486,7 +488,10 @@ def bench_batch(args) -> dict:
         sc, w, h, spp = workloads.batch_scene(i)
         g1 = time.time()
         r = Renderer(ctx, w, h, RenderSettings(max_bounces=workloads.BATCH_DEPTH, firefly_clamp=0.0))
-        r.load_scene(sc)
+        # The caller's tree comes from the GPU builder (the role of
+        # buildBVHAsync); the upload rebuilds the device tree with the binned
+        # SAH on the host's threads.
+        r.load_scene(sc, build_bvh_gpu(ctx, S.merge_meshes([m.mesh for m in sc.meshes]).positions))
         load_s += time.time() - g1
         gen_s += g1 - g0
         tris += r.packed.tri_count

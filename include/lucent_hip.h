@@ -71,6 +71,17 @@ lucent_status lucent_build_bvh(const float *positions, uint32_t tri_count, uint3
                                float *nodes_out, uint32_t *node_count_out, uint32_t *tri_indices_out,
                                double *bounds6_out);
 
+/* The splits of lucent_build_bvh with independent subtrees built on
+ * `threads` host threads (0 = the host's hardware threads): the same node
+ * layout and leaf order semantics (root 0, children adjacent, leaves of at
+ * most max_leaf triangles), the same set of nodes, numbered subtree by
+ * subtree instead of in one depth-first order; deterministic.  No
+ * counterpart in the reference (buildBVHAsync moves the one-threaded build
+ * to a worker, src/math/bvh-async.ts:8-30). */
+lucent_status lucent_build_bvh_parallel(const float *positions, uint32_t tri_count, uint32_t max_leaf, uint32_t bins,
+                                        uint32_t threads, float *nodes_out, uint32_t *node_count_out,
+                                        uint32_t *tri_indices_out, double *bounds6_out);
+
 /* GPU builder with the same outputs and layout as lucent_build_bvh (root
  * node 0, children of an interior node adjacent, leaves of at most max_leaf
  * triangles, triIndices in leaf order), built on the context's device as a
@@ -125,8 +136,11 @@ lucent_status lucent_scene_upload(lucent_ctx *ctx, const lucent_packed_scene *pa
  * every traversal here), and hit records keep their meaning: original
  * triangle ids, and exact queries still resolve ties to the lowest slot of
  * the caller's tree.  max_leaf = 0 keeps the caller's tree as the device
- * tree.  No counterpart in the reference (its shaders walk the tree they
- * are given). */
+ * tree.  The SAH rebuild runs its independent subtrees on the host's
+ * threads (lucent_build_bvh_parallel); bins = 0 builds the device tree on
+ * the GPU instead (lucent_build_bvh_gpu's linear BVH with leaves of at most
+ * max_leaf).  No counterpart in the reference (its shaders walk the tree
+ * they are given). */
 lucent_status lucent_set_tree_build(lucent_ctx *ctx, uint32_t max_leaf, uint32_t bins);
 
 /* Device memory a tile launch may use for path records (48 B per pixel

@@ -84,3 +84,25 @@ def build_bvh_gpu(ctx, positions, max_leaf_size: int = 4) -> BVH:
     )
     n = count.value
     return BVH(nodes[: n * NODE_STRIDE].copy(), n, tri, (tuple(bounds[:3]), tuple(bounds[3:])))
+
+
+def build_bvh_parallel(positions, max_leaf_size: int = 4, bins: int = 12, threads: int = 0) -> BVH:
+    """lucent_build_bvh_parallel: buildBVH's splits with subtrees built on
+    `threads` host threads (0 = all); the same nodes, numbered subtree by
+    subtree."""
+    pos = _as_f32(positions).reshape(-1)
+    if pos.size % 9 != 0:
+        raise ValueError(f"positions length {pos.size} is not a multiple of 9")
+    tri_count = pos.size // 9
+    nodes = np.zeros(max(1, 2 * tri_count - 1) * NODE_STRIDE, dtype=np.float32)
+    tri = np.zeros(tri_count, dtype=np.uint32)
+    bounds = np.zeros(6, dtype=np.float64)
+    count = C.c_uint32(0)
+    _native.check(
+        _native.lib().lucent_build_bvh_parallel(
+            pos.ctypes.data, tri_count, max_leaf_size, bins, threads, nodes.ctypes.data, C.byref(count),
+            tri.ctypes.data, bounds.ctypes.data,
+        )
+    )
+    n = count.value
+    return BVH(nodes[: n * NODE_STRIDE].copy(), n, tri, (tuple(bounds[:3]), tuple(bounds[3:])))

@@ -14,9 +14,12 @@
 // compiled with -ffp-contract=off and the pragma below).
 #pragma clang fp contract(off)
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -55,6 +58,11 @@ class SahBinnedBuilder {
  public:
   SahBinnedBuilder(const float* positions, uint32_t tris, uint32_t max_leaf, uint32_t bins)
       : pos_(positions), n_(tris), max_leaf_(max_leaf), bins_(bins) {}
+  // A worker over `o`'s prepared triangles and order (its own bins).
+  struct Worker {};
+  SahBinnedBuilder(const SahBinnedBuilder& o, Worker)
+      : pos_(o.pos_), n_(o.n_), max_leaf_(o.max_leaf_), bins_(o.bins_), order_(o.order_), tlo_(o.tlo_),
+        thi_(o.thi_), cen_(o.cen_) {}
 
   void build(float* nodes, uint32_t* node_count, uint32_t* order, double bounds[6]) {
     nodes_ = nodes;
@@ -110,6 +118,71 @@ class SahBinnedBuilder {
     *node_count = allocated;
   }
 
+  // The same splits, with every range of fewer than `cutoff` triangles built
+  // as a subtree of its own on a pool of `threads` threads and stitched in
+  // afterwards: node numbers differ from build()'s (each subtree's nodes are
+  // contiguous), the tree is the same set of splits and is deterministic.
+  void build_parallel(float* nodes, uint32_t* node_count, uint32_t* order, double bounds[6], uint32_t threads,
+                      uint32_t cutoff) {
+    if (n_ <= cutoff || threads < 2) {
+      build(nodes, node_count, order, bounds);
+      return;
+    }
+    nodes_ = nodes;
+    order_ = order;
+    prepare();
+    alloc_bins();
+    uint32_t allocated = 1;
+    std::vector<Task> todo, deferred;
+    todo.push_back({0, 0, n_});
+    while (!todo.empty()) {
+      const Task task = todo.back();
+      todo.pop_back();
+      if (task.end - task.begin < cutoff) {
+        deferred.push_back(task);
+        continue;
+      }
+      const Expand e = expand(task, allocated);
+      if (e.leaf) continue;
+      todo.push_back({e.right, e.mid, task.end});
+      todo.push_back({e.left, task.begin, e.mid});
+    }
+    // Subtrees: local node 0 is the subtree's root (written at its task's
+    // node), local node k > 0 goes to base + k - 1.
+    std::vector<std::vector<float>> local(deferred.size());
+    std::vector<uint32_t> counts(deferred.size(), 0);
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+      SahBinnedBuilder sub(*this, Worker{});
+      for (size_t i = next++; i < deferred.size(); i = next++) {
+        const Task& t = deferred[i];
+        local[i].assign(size_t(2 * (t.end - t.begin)) * 8, 0.0f);
+        sub.nodes_ = local[i].data();
+        counts[i] = sub.build_range(t.begin, t.end);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t k = 0; k + 1 < threads; ++k) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    uint32_t base = allocated;
+    for (size_t i = 0; i < deferred.size(); ++i) {
+      const float* src = local[i].data();
+      for (uint32_t k = 0; k < counts[i]; ++k) {
+        float* dst = nodes_ + size_t(k == 0 ? deferred[i].node : base + k - 1) * 8;
+        std::memcpy(dst, src + size_t(k) * 8, 8 * sizeof(float));
+        if (dst[7] == 0.0f) dst[3] = float(base + uint32_t(dst[3]) - 1);  // interior: its children
+      }
+      base += counts[i] - 1;
+    }
+    bounds_of(0, n_);
+    for (int a = 0; a < 3; ++a) {
+      bounds[a] = node_box_.lo[a];
+      bounds[a + 3] = node_box_.hi[a];
+    }
+    *node_count = base;
+  }
+
  private:
   struct Task {
     uint32_t node, begin, end;
@@ -119,12 +192,69 @@ class SahBinnedBuilder {
     int bin = -1;
     double cost = INFINITY;
   };
+  struct Expand {
+    bool leaf;
+    uint32_t left, right, mid;
+  };
+  void alloc_bins() {
+    bin_lo_.assign(3 * bins_ * 3, 0.0);
+    bin_hi_.assign(3 * bins_ * 3, 0.0);
+    bin_n_.assign(3 * bins_, 0);
+    suffix_area_.assign(bins_, 0.0);
+    suffix_n_.assign(bins_, 0);
+  }
+  // One step of build()'s loop: the task's node becomes a leaf, or an
+  // interior node whose two children are allocated.
+  Expand expand(const Task& task, uint32_t& allocated) {
+    const uint32_t count = task.end - task.begin;
+    bounds_of(task.begin, task.end);
+    if (count <= max_leaf_) {
+      emit(task.node, node_box_.lo, node_box_.hi, task.begin, count);
+      return {true, 0, 0, 0};
+    }
+    Split best = best_split(task.begin, task.end, count);
+    const double leaf_cost =
+        count * Aabb::area(node_box_.lo[0], node_box_.lo[1], node_box_.lo[2], node_box_.hi[0], node_box_.hi[1],
+                           node_box_.hi[2]);
+    const bool unprofitable = best.axis < 0 || best.cost >= leaf_cost;
+    if (unprofitable && count <= max_leaf_ * 4) {
+      emit(task.node, node_box_.lo, node_box_.hi, task.begin, count);
+      return {true, 0, 0, 0};
+    }
+    uint32_t mid = unprofitable ? (task.begin + task.end) >> 1 : partition(task.begin, task.end, best);
+    if (mid == task.begin || mid == task.end) mid = (task.begin + task.end) >> 1;
+    const uint32_t left = allocated++;
+    const uint32_t right = allocated++;
+    emit(task.node, node_box_.lo, node_box_.hi, left, 0);
+    return {false, left, right, mid};
+  }
+  // The subtree over [begin, end) with local numbering from 0 (its root);
+  // returns its node count.
+  uint32_t build_range(uint32_t begin, uint32_t end) {
+    alloc_bins();
+    uint32_t allocated = 1;
+    std::vector<Task> todo;
+    todo.push_back({0, begin, end});
+    while (!todo.empty()) {
+      const Task task = todo.back();
+      todo.pop_back();
+      const Expand e = expand(task, allocated);
+      if (e.leaf) continue;
+      todo.push_back({e.right, e.mid, task.end});
+      todo.push_back({e.left, task.begin, e.mid});
+    }
+    return allocated;
+  }
+
 
   // Per-triangle float32 bounds and centroids (bvh.ts:60-73).
   void prepare() {
     tri_lo_.resize(size_t(n_) * 3);
     tri_hi_.resize(size_t(n_) * 3);
     centroid_.resize(size_t(n_) * 3);
+    tlo_ = tri_lo_.data();
+    thi_ = tri_hi_.data();
+    cen_ = centroid_.data();
     for (uint32_t t = 0; t < n_; ++t) {
       const float* p = pos_ + size_t(t) * 9;
       for (int a = 0; a < 3; ++a) {
@@ -145,7 +275,7 @@ class SahBinnedBuilder {
     for (uint32_t i = begin; i < end; ++i) {
       const size_t t = size_t(order_[i]) * 3;
       for (int a = 0; a < 3; ++a) {
-        const double lo = tri_lo_[t + a], hi = tri_hi_[t + a], c = centroid_[t + a];
+        const double lo = tlo_[t + a], hi = thi_[t + a], c = cen_[t + a];
         if (lo < node_box_.lo[a]) node_box_.lo[a] = lo;
         if (hi > node_box_.hi[a]) node_box_.hi[a] = hi;
         if (c < cen_box_.lo[a]) cen_box_.lo[a] = c;
@@ -155,7 +285,7 @@ class SahBinnedBuilder {
   }
 
   double bin_index(uint32_t tri, int axis) const {
-    const double f = std::floor((double(centroid_[size_t(tri) * 3 + axis]) - cen_box_.lo[axis]) * scale_[axis]);
+    const double f = std::floor((double(cen_[size_t(tri) * 3 + axis]) - cen_box_.lo[axis]) * scale_[axis]);
     return js_min(double(bins_ - 1), f);
   }
 
@@ -185,7 +315,7 @@ class SahBinnedBuilder {
         const uint32_t k = a * bins_ + uint32_t(bin_index(t, a));
         ++bin_n_[k];
         for (int c = 0; c < 3; ++c) {
-          const double lo = tri_lo_[size_t(t) * 3 + c], hi = tri_hi_[size_t(t) * 3 + c];
+          const double lo = tlo_[size_t(t) * 3 + c], hi = thi_[size_t(t) * 3 + c];
           if (lo < bin_lo_[k * 3 + c]) bin_lo_[k * 3 + c] = lo;
           if (hi > bin_hi_[k * 3 + c]) bin_hi_[k * 3 + c] = hi;
         }
@@ -266,7 +396,8 @@ class SahBinnedBuilder {
   uint32_t n_, max_leaf_, bins_;
   float* nodes_ = nullptr;
   uint32_t* order_ = nullptr;
-  std::vector<float> tri_lo_, tri_hi_, centroid_;
+  std::vector<float> tri_lo_, tri_hi_, centroid_;  // owned by the builder that prepared them
+  const float *tlo_ = nullptr, *thi_ = nullptr, *cen_ = nullptr;
   std::vector<double> bin_lo_, bin_hi_, suffix_area_;
   std::vector<uint32_t> bin_n_, suffix_n_;
   Aabb node_box_{}, cen_box_{};
@@ -284,6 +415,22 @@ Status build_bvh(const float* positions, uint32_t tri_count, uint32_t max_leaf, 
   if (!nodes_out || !node_count || !bounds) return fail(LUCENT_E_INVALID, "null output");
   SahBinnedBuilder b(positions, tri_count, max_leaf, bins);
   b.build(nodes_out, node_count, tri_indices_out, bounds);
+  return LUCENT_OK;
+}
+
+Status build_bvh_parallel(const float* positions, uint32_t tri_count, uint32_t max_leaf, uint32_t bins,
+                          float* nodes_out, uint32_t* node_count, uint32_t* tri_indices_out, double bounds[6],
+                          uint32_t threads) {
+  if (bins < 2 || bins > 4096) return fail(LUCENT_E_INVALID, "bins must be in [2, 4096]");
+  if (max_leaf < 1) return fail(LUCENT_E_INVALID, "max_leaf must be >= 1");
+  if (tri_count > (1u << 24)) return fail(LUCENT_E_INVALID, "more than 2^24 triangles cannot be indexed by float32 node fields");
+  if (tri_count > 0 && (!positions || !tri_indices_out)) return fail(LUCENT_E_INVALID, "null array");
+  if (!nodes_out || !node_count || !bounds) return fail(LUCENT_E_INVALID, "null output");
+  // Subtrees of at most ~1/(8 x threads) of the triangles (and at least 4096)
+  // keep the pool busy to the end.
+  const uint32_t cutoff = std::max<uint32_t>(4096, tri_count / (8 * std::max<uint32_t>(threads, 1)));
+  SahBinnedBuilder b(positions, tri_count, max_leaf, bins);
+  b.build_parallel(nodes_out, node_count, tri_indices_out, bounds, threads, cutoff);
   return LUCENT_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -148,9 +149,16 @@ Status finish_scene(Context* ctx, Scene* sc, const float* nodes, uint32_t node_c
     own_nodes.resize(size_t(std::max<uint32_t>(1, 2 * n - 1)) * 8);
     ref.resize(n);
     double bounds[6];
-    if ((s = build_bvh(pos.data(), n, ctx->tree_leaf, ctx->tree_bins, own_nodes.data(), &dev_node_count, ref.data(),
-                       bounds)) != LUCENT_OK)
-      return s;
+    // The host cores build independent subtrees in parallel (any valid tree
+    // serves: exact queries break ties by the caller's slots).
+    // With tree_bins = 0 the device builds a linear BVH instead.
+    const uint32_t threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (ctx->tree_bins == 0)
+      s = build_bvh_gpu(ctx, pos.data(), n, ctx->tree_leaf, own_nodes.data(), &dev_node_count, ref.data(), bounds);
+    else
+      s = build_bvh_parallel(pos.data(), n, ctx->tree_leaf, ctx->tree_bins, own_nodes.data(), &dev_node_count,
+                             ref.data(), bounds, threads);
+    if (s != LUCENT_OK) return s;
     if ((s = validate_bvh(own_nodes.data(), dev_node_count, n, &depth)) != LUCENT_OK) return s;
     dev_nodes = own_nodes.data();
     tris.resize(tris_in.size());
@@ -301,6 +309,18 @@ lucent_status lucent_build_bvh(const float* positions, uint32_t tri_count, uint3
   return s;
 }
 
+lucent_status lucent_build_bvh_parallel(const float* positions, uint32_t tri_count, uint32_t max_leaf, uint32_t bins,
+                                        uint32_t threads, float* nodes_out, uint32_t* node_count_out,
+                                        uint32_t* tri_indices_out, double* bounds6_out) {
+  clear_error();
+  if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+  double bounds[6];
+  Status s = build_bvh_parallel(positions, tri_count, max_leaf, bins, nodes_out, node_count_out, tri_indices_out,
+                                bounds, threads);
+  if (s == LUCENT_OK && bounds6_out) std::memcpy(bounds6_out, bounds, sizeof(bounds));
+  return s;
+}
+
 lucent_status lucent_build_bvh_gpu(lucent_ctx* ctx, const float* positions, uint32_t tri_count, uint32_t max_leaf,
                                    float* nodes_out, uint32_t* node_count_out, uint32_t* tri_indices_out,
                                    double* bounds6_out) {
@@ -351,7 +371,8 @@ lucent_status lucent_set_tree_build(lucent_ctx* ctx, uint32_t max_leaf, uint32_t
   clear_error();
   if (!ctx) return fail(LUCENT_E_INVALID, "null ctx");
   if (max_leaf > 16) return fail(LUCENT_E_INVALID, "max_leaf must be at most 16 (0 keeps the caller's tree)");
-  if (max_leaf > 0 && (bins < 2 || bins > 4096)) return fail(LUCENT_E_INVALID, "bins must be in [2, 4096]");
+  if (max_leaf > 0 && bins != 0 && (bins < 2 || bins > 4096))
+    return fail(LUCENT_E_INVALID, "bins must be 0 (a linear BVH built on the device) or in [2, 4096]");
   ctx->tree_leaf = max_leaf;
   ctx->tree_bins = max_leaf > 0 ? bins : ctx->tree_bins;
   return LUCENT_OK;

@@ -20,6 +20,11 @@ void clear_error();
 
 Status build_bvh(const float* positions, uint32_t tri_count, uint32_t max_leaf, uint32_t bins, float* nodes_out,
                  uint32_t* node_count, uint32_t* tri_indices_out, double bounds[6]);
+// The same splits built with subtrees on `threads` host threads (node
+// numbering differs from build_bvh's; deterministic).
+Status build_bvh_parallel(const float* positions, uint32_t tri_count, uint32_t max_leaf, uint32_t bins,
+                          float* nodes_out, uint32_t* node_count, uint32_t* tri_indices_out, double bounds[6],
+                          uint32_t threads);
 struct Context;
 Status build_bvh_gpu(Context* ctx, const float* positions, uint32_t tri_count, uint32_t max_leaf, float* nodes_out,
                      uint32_t* node_count_out, uint32_t* tri_indices_out, double bounds_out[6]);

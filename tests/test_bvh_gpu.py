@@ -1,8 +1,9 @@
 """GPU linear-BVH builder (lucent_build_bvh_gpu): the reference's node-layout
 invariants (bvh.test.ts:31-86: every triangle in exactly one leaf, children
-adjacent and nested, boxes containing their triangles), exact closest hits
-bit-identical to those over the SAH tree (exact mode does not depend on the
-tree), path-traced films equal to the SAH tree's, and build time plus
+adjacent and nested, boxes containing their triangles); with the LBVH kept as
+the device tree (lucent_set_tree_build(0)), exact closest hits through it
+equal the FP64 oracle's and the path-traced film agrees with the SAH tree's
+and the oracle's; the LBVH as the upload-time device tree; build time plus
 traversal-cost ratio against the SAH builder on 1.5M triangles."""
 import time
 
@@ -67,20 +68,72 @@ def test_tiny_inputs(ctx):
         build_bvh_gpu(ctx, np.zeros(10, np.float32))
 
 
-def test_exact_hits_identical_to_the_sah_tree(ctx):
+class caller_tree:
+    """lucent_set_tree_build(0): the caller's tree is the device tree, so a
+    scene built from a GPU LBVH is traversed through the LBVH itself."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def __enter__(self):
+        self.ctx.set_tree_build(0, 32)
+
+    def __exit__(self, *exc):
+        self.ctx.set_tree_build(2, 32)
+
+
+def test_lbvh_traversal_matches_the_oracle(ctx):
+    """Exact closest hits through the GPU-built tree equal the FP64 oracle's
+    (intersectBVH over the reference builder's tree), ties included: the
+    exact mode breaks ties by the lowest slot of the caller's tree, and the
+    caller's tree is the LBVH here, so ids may differ only at equal t."""
+    import oracle as O
     cfg = workloads.config1(1 << 16)
     pos = cfg["positions"]
-    a = ctx.scene_from_bvh(pos, build_bvh(pos))
-    b = ctx.scene_from_bvh(pos, build_bvh_gpu(ctx, pos))
-    ha = a.intersect(cfg["origins"], cfg["dirs"])
-    hb = b.intersect(cfg["origins"], cfg["dirs"])
-    for f in ("t", "tri", "u", "v"):
-        assert np.array_equal(getattr(ha, f), getattr(hb, f)), f
-    a.destroy()
-    b.destroy()
+    lb = build_bvh_gpu(ctx, pos)
+    with caller_tree(ctx):
+        sc = ctx.scene_from_bvh(pos, lb)
+        st = sc.stats()
+        hb = sc.intersect(cfg["origins"], cfg["dirs"])
+    assert st["node_count"] == lb.node_count
+    ref = O.build_bvh(pos)
+    t, tri, u, v = O.intersect_bvh_batch(ref, pos, cfg["origins"], cfg["dirs"])
+    assert np.array_equal(hb.t.view(np.uint64), t.view(np.uint64))
+    same = hb.tri == tri
+    tie = ~same & (np.abs(hb.t - t) <= 1e-6 * np.abs(t))
+    assert np.all(same | tie)
+    assert np.array_equal(hb.u[same], u[same]) and np.array_equal(hb.v[same], v[same])
+    assert (tri != 0xFFFFFFFF).mean() > 0.99
+    # The LBVH's own leaf order decides ties among its slots: the oracle
+    # traversing the LBVH agrees bit for bit.
+    t2, tri2, u2, v2 = O.intersect_bvh_batch(lb, pos, cfg["origins"], cfg["dirs"])
+    assert np.array_equal(hb.tri, tri2) and np.array_equal(hb.u, u2) and np.array_equal(hb.v, v2)
+    sc.destroy()
 
 
-def test_path_traced_film_matches_the_sah_tree(ctx):
+def test_device_built_upload_tree(ctx):
+    """lucent_set_tree_build(leaf, 0): the upload-time device tree is built on
+    the GPU; exact hits keep their meaning (ties go to the caller's slots)."""
+    import oracle as O
+    cfg = workloads.config1(1 << 15, seed=5)
+    pos = cfg["positions"]
+    ref = O.build_bvh(pos)
+    ctx.set_tree_build(2, 0)
+    try:
+        sc = ctx.scene_from_bvh(pos, build_bvh(pos))
+        hb = sc.intersect(cfg["origins"], cfg["dirs"])
+    finally:
+        ctx.set_tree_build(2, 32)
+    t, tri, u, v = O.intersect_bvh_batch(ref, pos, cfg["origins"], cfg["dirs"])
+    for a, b in ((hb.t, t), (hb.tri, tri), (hb.u, u), (hb.v, v)):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    sc.destroy()
+
+
+def test_path_traced_film_through_the_lbvh(ctx):
+    """The knot scene's film traced through the LBVH itself (no rebuild)
+    equals the one traced through the SAH tree up to equal-distance ties,
+    and agrees with the oracle."""
     from lucent_amd.renderer import Renderer, RenderSettings
     sc = workloads.knot_scene()
     w, h = 144, 90
@@ -88,17 +141,29 @@ def test_path_traced_film_matches_the_sah_tree(ctx):
     for builder in ("sah", "gpu"):
         r = Renderer(ctx, w, h, RenderSettings(8, 0.0))
         pos = S.merge_meshes([m.mesh for m in sc.meshes]).positions
-        r.load_scene(sc, build_bvh(pos) if builder == "sah" else build_bvh_gpu(ctx, pos))
+        if builder == "gpu":
+            with caller_tree(ctx):
+                lb = build_bvh_gpu(ctx, pos)
+                r.load_scene(sc, lb)
+                assert r.device_scene.stats()["node_count"] == lb.node_count
+        else:
+            r.load_scene(sc, build_bvh(pos))
         r.film.set_seed(0)
         r.reset()
         r.trace_samples(4)
-        films.append(r.film.read()["color"])
+        films.append(r.film.read())
         r.close()
-    same = np.all(films[0] == films[1], axis=-1).mean()
-    rel = abs(films[0][..., :3].mean() - films[1][..., :3].mean()) / films[0][..., :3].mean()
+    a, b = films[0]["color"], films[1]["color"]
+    same = np.all(a == b, axis=-1).mean()
+    rel = abs(a[..., :3].mean() - b[..., :3].mean()) / a[..., :3].mean()
     print(f"pixels identical: {same:.5f}, image mean rel diff {rel:.2e}")
     # Different trees visit equal-distance triangles in a different order; nothing else differs.
     assert same > 0.999 and rel < 1e-5
+    from test_pathtrace_gpu import compare, oracle_pixels
+    yy, xx = np.mgrid[30:54, 50:90]
+    xy = np.stack([xx.reshape(-1), yy.reshape(-1)], -1).astype(np.uint32)
+    ref, per = oracle_pixels(None, sc, w, h, 4, xy, seed=0)
+    compare(films[1], ref, per, xy, 4, "knots through the LBVH")
 
 
 def sah_cost(bvh):
