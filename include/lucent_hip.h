@@ -498,6 +498,55 @@ lucent_status lucent_screenshot(lucent_ctx *ctx, const lucent_film *film, const 
 lucent_status lucent_decode_hdr(const uint8_t *bytes, size_t length, uint32_t *width, uint32_t *height, float *rgb,
                                 size_t rgb_capacity);
 
+/* glTF 2.0 / GLB ingestion; together these replace loadGLTF
+ * (src/loaders/gltf.ts:188-502).  The reference's loader awaits a fetch per
+ * external file; a C caller instead runs three phases:
+ *   1. lucent_gltf_parse reads a .glb or .gltf file in memory, checks the
+ *      version and the required extensions (KHR_draco_mesh_compression,
+ *      EXT_meshopt_compression, KHR_texture_basisu, MSFT_texture_dds and
+ *      EXT_mesh_gpu_instancing are refused, gltf.ts:38-44), converts the
+ *      materials and lists the external files the asset needs: its buffers,
+ *      then the images some material samples, in first-use order
+ *      (lucent_gltf_resource_count / _uri; URIs as written, not decoded);
+ *   2. the caller hands each one's bytes to lucent_gltf_set_resource;
+ *   3. lucent_gltf_build walks the default scene (or, without scenes, every
+ *      root node) and produces one world-space triangle soup per triangle
+ *      primitive (modes 4, 5 and 6; positions, normals by the inverse
+ *      transpose, flat normals where NORMAL is missing, TEXCOORD_0), with
+ *      the winding kept under mirroring transforms.
+ * Materials come back in the material texel layout (40 values per
+ * material, the order of pack_materials / material.ts:132-149) as doubles,
+ * so that factors read from the JSON are exact; map fields index the image
+ * list.  A primitive without a material uses an appended glTF default
+ * material.  Images come back still encoded (PNG, JPEG, WebP), as the
+ * reference leaves decoding to the browser.  Warnings (ignored material
+ * extensions, skipped point/line primitives, KTX2 textures, TEXCOORD_1 use)
+ * are listed once each.  Errors: LUCENT_E_INVALID with gltf.ts's messages
+ * for malformed files, LUCENT_E_UNSUPPORTED for refused features.  Host
+ * only, no context needed; pointers returned stay valid until
+ * lucent_gltf_destroy. */
+typedef struct lucent_gltf lucent_gltf;
+typedef struct {
+  uint32_t mesh_count, material_count, image_count, warning_count;
+  uint64_t triangle_count;
+  double bounds[6]; /* min xyz, max xyz of the world-space positions */
+} lucent_gltf_summary;
+lucent_status lucent_gltf_parse(const uint8_t *bytes, size_t length, lucent_gltf **out);
+uint32_t lucent_gltf_resource_count(const lucent_gltf *doc);
+const char *lucent_gltf_resource_uri(const lucent_gltf *doc, uint32_t i);
+lucent_status lucent_gltf_set_resource(lucent_gltf *doc, uint32_t i, const uint8_t *bytes, size_t length);
+lucent_status lucent_gltf_build(lucent_gltf *doc);
+lucent_status lucent_gltf_info(const lucent_gltf *doc, lucent_gltf_summary *out);
+/* Mesh i: tri_count triangles of 3 corners; positions and normals hold
+ * 9 floats per triangle, uvs 6 (NULL when the primitive has no TEXCOORD_0). */
+lucent_status lucent_gltf_mesh(const lucent_gltf *doc, uint32_t i, uint32_t *tri_count, int32_t *material,
+                               const float **positions, const float **normals, const float **uvs);
+lucent_status lucent_gltf_material(const lucent_gltf *doc, uint32_t i, double *packed40);
+lucent_status lucent_gltf_image(const lucent_gltf *doc, uint32_t i, const uint8_t **data, size_t *length,
+                                const char **mime_type, const char **name);
+const char *lucent_gltf_warning(const lucent_gltf *doc, uint32_t i);
+void lucent_gltf_destroy(lucent_gltf *doc);
+
 /* ------------------------------------------------------------------------
  * Frame graphs: one progressive frame (`samples` samples of every pixel of
  * `film` through the persistent tile launch, then optionally lucent_denoise

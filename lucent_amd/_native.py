@@ -120,6 +120,11 @@ class DisplayParamsStruct(C.Structure):
     _fields_ = [("denoise_weight", C.c_float), ("exposure", C.c_float), ("tonemap", C.c_uint32)]
 
 
+class GltfSummaryStruct(C.Structure):
+    _fields_ = [("mesh_count", C.c_uint32), ("material_count", C.c_uint32), ("image_count", C.c_uint32),
+                ("warning_count", C.c_uint32), ("triangle_count", C.c_uint64), ("bounds", C.c_double * 6)]
+
+
 # name -> (restype, argtypes)
 _P = C.c_void_p
 _U32 = C.c_uint32
@@ -170,6 +175,19 @@ SIGNATURES: dict[str, tuple] = {
                                          _P, C.POINTER(_P)]),
     "lucent_frame_graph_launch": (_I32, [_P, _P]),
     "lucent_decode_hdr": (_I32, [_P, C.c_size_t, C.POINTER(_U32), C.POINTER(_U32), _P, C.c_size_t]),
+    "lucent_gltf_parse": (_I32, [_P, C.c_size_t, C.POINTER(_P)]),
+    "lucent_gltf_resource_count": (_U32, [_P]),
+    "lucent_gltf_resource_uri": (C.c_char_p, [_P, _U32]),
+    "lucent_gltf_set_resource": (_I32, [_P, _U32, _P, C.c_size_t]),
+    "lucent_gltf_build": (_I32, [_P]),
+    "lucent_gltf_info": (_I32, [_P, C.POINTER(GltfSummaryStruct)]),
+    "lucent_gltf_mesh": (_I32, [_P, _U32, C.POINTER(_U32), C.POINTER(_I32), C.POINTER(C.POINTER(C.c_float)),
+                                C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_float))]),
+    "lucent_gltf_material": (_I32, [_P, _U32, C.POINTER(_D)]),
+    "lucent_gltf_image": (_I32, [_P, _U32, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t),
+                                 C.POINTER(C.c_char_p), C.POINTER(C.c_char_p)]),
+    "lucent_gltf_warning": (C.c_char_p, [_P, _U32]),
+    "lucent_gltf_destroy": (None, [_P]),
     "lucent_set_tree_build": (_I32, [_P, _U32, _U32]),
     "lucent_set_record_budget": (_I32, [_P, C.c_uint64]),
     "lucent_frame_graph_destroy": (None, [_P]),

@@ -22,6 +22,17 @@ export interface Camera { position: number[]; target: number[]; fov: number; ape
 export interface FilmTargets { color: Float32Array; albedo: Float32Array; normal: Float32Array; history: Float32Array; }
 export declare function buildBVHAsync(positions: Float32Array, options?: BVHOptions): Promise<BVH>;
 export declare function parseHDR(buffer: ArrayBuffer | Uint8Array): { width: number; height: number; data: Float32Array };
+export interface GLTFImage { data: Uint8Array; mimeType: string; name: string; }
+export interface GLTFAsset {
+  meshes: { mesh: { positions: Float32Array; normals: Float32Array; uvs?: Float32Array }; material: number }[];
+  materials: Record<string, unknown>[];
+  images: GLTFImage[];
+  bounds: { min: number[]; max: number[] };
+  triangleCount: number;
+  warnings: string[];
+}
+export declare function loadGLTF(source: ArrayBuffer | Uint8Array,
+  resolve: (uri: string) => Promise<ArrayBuffer | Uint8Array> | ArrayBuffer | Uint8Array): Promise<GLTFAsset>;
 
 export declare class LucentRenderer {
   constructor(opts: { width: number; height: number; device?: number });

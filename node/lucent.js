@@ -7,6 +7,7 @@
  *   buildBVH(positions, options)          src/math/bvh.ts:52
  *   buildBVHAsync(positions, options)     src/math/bvh-async.ts:8
  *   parseHDR(buffer)                      src/loaders/hdr.ts:7
+ *   loadGLTF(source, resolve)             src/loaders/gltf.ts:188
  *   GpuTraversal#intersect / pickDistance src/math/bvh.ts:265, src/scene/scene.ts:327
  *   LucentRenderer                        src/renderer.ts:171 (loadScene, uploadLighting,
  *                                         uploadCamera with temporal reprojection, reset,
@@ -60,6 +61,53 @@ function buildBVHAsync(positions, options) {
 function parseHDR(buffer) {
   const bytes = buffer instanceof Uint8Array ? buffer : new Uint8Array(buffer);
   return addon.decodeHDR(bytes);
+}
+
+const ALPHA_MODES = ['opaque', 'mask', 'blend'];
+
+/** A Material (material.ts:17-57) from its 40 texel values (material.ts:132-149 order). */
+function materialFromTexels(t) {
+  return {
+    albedo: [t[0], t[1], t[2]], metallic: t[3], roughness: t[4], ior: t[5], transmission: t[6],
+    checkerScale: t[7], emission: [t[8], t[9], t[10]], alpha: t[11], albedo2: [t[12], t[13], t[14]],
+    alphaMode: ALPHA_MODES[t[15]], baseColorMap: t[16], metalRoughMap: t[17], normalMap: t[18],
+    emissiveMap: t[19], normalScale: t[20], clearcoat: t[21], clearcoatRoughness: t[22], alphaCutoff: t[23],
+    sheenColor: [t[24], t[25], t[26]], sheenRoughness: t[27], attenuationColor: [t[28], t[29], t[30]],
+    attenuationDistance: t[31] > 0 ? t[31] : Infinity, thinWalled: t[35] !== 0,
+    uvTransform: [t[32], t[33], t[34], t[36], t[37], t[38]],
+  };
+}
+
+/**
+ * loadGLTF (src/loaders/gltf.ts:188): a .glb or .gltf file -> Promise of
+ * GLTFAsset {meshes: [{mesh, material}], materials, images, bounds,
+ * triangleCount, warnings}.  The native loader lists the external files it
+ * needs (buffers, then the images some material samples); `resolve(uri)`
+ * fetches each (an ArrayBuffer or a Promise of one).  Rejects with the
+ * reference's messages.
+ * @param {ArrayBuffer|Uint8Array} source
+ * @param {(uri: string) => Promise<ArrayBuffer>|ArrayBuffer} resolve
+ */
+async function loadGLTF(source, resolve) {
+  const bytes = source instanceof Uint8Array ? source : new Uint8Array(source);
+  const doc = addon.gltfParse(bytes);
+  for (let i = 0; i < doc.uris.length; i++) {
+    const data = await resolve(doc.uris[i]);
+    addon.gltfSetResource(doc.handle, i, data instanceof Uint8Array ? data : new Uint8Array(data));
+  }
+  const a = addon.gltfBuild(doc.handle);
+  return {
+    meshes: a.meshes.map((m) => ({
+      mesh: m.uvs ? { positions: m.positions, normals: m.normals, uvs: m.uvs }
+        : { positions: m.positions, normals: m.normals },
+      material: m.material,
+    })),
+    materials: a.materials.map(materialFromTexels),
+    images: a.images,
+    bounds: a.bounds,
+    triangleCount: a.triangleCount,
+    warnings: a.warnings,
+  };
 }
 
 /** Batched closest-hit queries against one uploaded BVH (intersectBVH on the GPU). */
@@ -329,4 +377,4 @@ class LucentRenderer {
   }
 }
 
-module.exports = { buildBVH, buildBVHAsync, parseHDR, GpuTraversal, LucentRenderer, cameraBasis, version: addon.version, addon, MISS };
+module.exports = { buildBVH, buildBVHAsync, parseHDR, loadGLTF, GpuTraversal, LucentRenderer, cameraBasis, version: addon.version, addon, MISS };

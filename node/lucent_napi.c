@@ -482,6 +482,149 @@ static napi_value js_decode_hdr(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* glTF ingestion (loadGLTF, gltf.ts:188-502) in the C ABI's three phases:
+ * gltfParse(bytes) -> {handle, uris}; gltfSetResource(handle, i, bytes);
+ * gltfBuild(handle) -> {meshes: [{positions, normals, uvs, material}],
+ * materials: [Float64Array(40)], images: [{data, mimeType, name}], bounds,
+ * triangleCount, warnings}.  The handle's finaliser frees the document. */
+static void gltf_finalize(napi_env env, void *data, void *hint) {
+  lucent_gltf_destroy((lucent_gltf *)data);
+}
+static int bytes_arg(napi_env env, napi_value v, const uint8_t **data, size_t *len) {
+  void *p = NULL;
+  if (!typed(env, v, napi_uint8_array, &p, len)) return 0;
+  *data = (const uint8_t *)p;
+  return 1;
+}
+static napi_value js_gltf_parse(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  const uint8_t *data;
+  size_t len;
+  if (argc < 1 || !bytes_arg(env, argv[0], &data, &len)) {
+    napi_throw_type_error(env, NULL, "gltfParse(bytes: Uint8Array)");
+    return NULL;
+  }
+  lucent_gltf *doc = NULL;
+  CHECK_LUCENT(lucent_gltf_parse(data, len, &doc));
+  napi_value ext, uris, out;
+  if (napi_create_external(env, doc, gltf_finalize, NULL, &ext) != napi_ok) {
+    lucent_gltf_destroy(doc);
+    napi_throw_error(env, NULL, "N-API call failed: napi_create_external");
+    return NULL;
+  }
+  const uint32_t n = lucent_gltf_resource_count(doc);
+  CHECK_NAPI(napi_create_array_with_length(env, n, &uris));
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value u;
+    CHECK_NAPI(napi_create_string_utf8(env, lucent_gltf_resource_uri(doc, i), NAPI_AUTO_LENGTH, &u));
+    CHECK_NAPI(napi_set_element(env, uris, i, u));
+  }
+  CHECK_NAPI(napi_create_object(env, &out));
+  CHECK_NAPI(napi_set_named_property(env, out, "handle", ext));
+  CHECK_NAPI(napi_set_named_property(env, out, "uris", uris));
+  return out;
+}
+static napi_value js_gltf_set_resource(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_gltf *doc = argc >= 1 ? (lucent_gltf *)get_external(env, argv[0]) : NULL;
+  const uint8_t *data;
+  size_t len;
+  if (argc < 3 || !doc || !bytes_arg(env, argv[2], &data, &len)) {
+    napi_throw_type_error(env, NULL, "gltfSetResource(handle, index, bytes: Uint8Array)");
+    return NULL;
+  }
+  CHECK_LUCENT(lucent_gltf_set_resource(doc, (uint32_t)num(env, argv[1], -1), data, len));
+  return NULL;
+}
+static napi_value copy_floats(napi_env env, const float *src, size_t n) {
+  void *dst;
+  napi_value ta = new_typed(env, napi_float32_array, n, 4, &dst);
+  if (ta && n) memcpy(dst, src, n * sizeof(float));
+  return ta;
+}
+static napi_value js_gltf_build(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  lucent_gltf *doc = argc >= 1 ? (lucent_gltf *)get_external(env, argv[0]) : NULL;
+  if (!doc) {
+    napi_throw_type_error(env, NULL, "gltfBuild(handle)");
+    return NULL;
+  }
+  CHECK_LUCENT(lucent_gltf_build(doc));
+  lucent_gltf_summary sum;
+  CHECK_LUCENT(lucent_gltf_info(doc, &sum));
+  napi_value out, meshes, materials, images, warnings, bmin, bmax, bounds;
+  CHECK_NAPI(napi_create_object(env, &out));
+  CHECK_NAPI(napi_create_array_with_length(env, sum.mesh_count, &meshes));
+  for (uint32_t i = 0; i < sum.mesh_count; i++) {
+    uint32_t tris;
+    int32_t mat;
+    const float *p, *n, *uv;
+    CHECK_LUCENT(lucent_gltf_mesh(doc, i, &tris, &mat, &p, &n, &uv));
+    napi_value m, nulls;
+    CHECK_NAPI(napi_create_object(env, &m));
+    CHECK_NAPI(napi_set_named_property(env, m, "positions", copy_floats(env, p, (size_t)tris * 9)));
+    CHECK_NAPI(napi_set_named_property(env, m, "normals", copy_floats(env, n, (size_t)tris * 9)));
+    CHECK_NAPI(napi_get_null(env, &nulls));
+    CHECK_NAPI(napi_set_named_property(env, m, "uvs", uv ? copy_floats(env, uv, (size_t)tris * 6) : nulls));
+    CHECK_NAPI(napi_set_named_property(env, m, "material", make_number(env, mat)));
+    CHECK_NAPI(napi_set_element(env, meshes, i, m));
+  }
+  CHECK_NAPI(napi_create_array_with_length(env, sum.material_count, &materials));
+  for (uint32_t i = 0; i < sum.material_count; i++) {
+    void *t;
+    napi_value ta = new_typed(env, napi_float64_array, 40, 8, &t);
+    if (!ta) return NULL;
+    CHECK_LUCENT(lucent_gltf_material(doc, i, (double *)t));
+    CHECK_NAPI(napi_set_element(env, materials, i, ta));
+  }
+  CHECK_NAPI(napi_create_array_with_length(env, sum.image_count, &images));
+  for (uint32_t i = 0; i < sum.image_count; i++) {
+    const uint8_t *d;
+    size_t len;
+    const char *mime, *name;
+    CHECK_LUCENT(lucent_gltf_image(doc, i, &d, &len, &mime, &name));
+    void *dst;
+    napi_value ta = new_typed(env, napi_uint8_array, len, 1, &dst), o, sv;
+    if (!ta) return NULL;
+    if (len) memcpy(dst, d, len);
+    CHECK_NAPI(napi_create_object(env, &o));
+    CHECK_NAPI(napi_set_named_property(env, o, "data", ta));
+    CHECK_NAPI(napi_create_string_utf8(env, mime, NAPI_AUTO_LENGTH, &sv));
+    CHECK_NAPI(napi_set_named_property(env, o, "mimeType", sv));
+    CHECK_NAPI(napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &sv));
+    CHECK_NAPI(napi_set_named_property(env, o, "name", sv));
+    CHECK_NAPI(napi_set_element(env, images, i, o));
+  }
+  CHECK_NAPI(napi_create_array_with_length(env, sum.warning_count, &warnings));
+  for (uint32_t i = 0; i < sum.warning_count; i++) {
+    napi_value w;
+    CHECK_NAPI(napi_create_string_utf8(env, lucent_gltf_warning(doc, i), NAPI_AUTO_LENGTH, &w));
+    CHECK_NAPI(napi_set_element(env, warnings, i, w));
+  }
+  CHECK_NAPI(napi_create_array_with_length(env, 3, &bmin));
+  CHECK_NAPI(napi_create_array_with_length(env, 3, &bmax));
+  for (int i = 0; i < 3; i++) {
+    napi_set_element(env, bmin, i, make_number(env, sum.bounds[i]));
+    napi_set_element(env, bmax, i, make_number(env, sum.bounds[3 + i]));
+  }
+  CHECK_NAPI(napi_create_object(env, &bounds));
+  CHECK_NAPI(napi_set_named_property(env, bounds, "min", bmin));
+  CHECK_NAPI(napi_set_named_property(env, bounds, "max", bmax));
+  CHECK_NAPI(napi_set_named_property(env, out, "meshes", meshes));
+  CHECK_NAPI(napi_set_named_property(env, out, "materials", materials));
+  CHECK_NAPI(napi_set_named_property(env, out, "images", images));
+  CHECK_NAPI(napi_set_named_property(env, out, "bounds", bounds));
+  CHECK_NAPI(napi_set_named_property(env, out, "triangleCount", make_number(env, (double)sum.triangle_count)));
+  CHECK_NAPI(napi_set_named_property(env, out, "warnings", warnings));
+  return out;
+}
+
 /* RenderSettings fields of the integrator (renderer.ts:42-84). */
 static void read_settings(napi_env env, napi_value o, lucent_render_settings *rs) {
   rs->max_bounces = (uint32_t)num(env, prop(env, o, "maxBounces"), 8);
@@ -864,6 +1007,9 @@ static napi_value init(napi_env env, napi_value exports) {
       {"traceSamplesAsync", 0, js_trace_samples_async, 0, 0, 0, napi_default, 0},
       {"denoiseAsync", 0, js_denoise_async, 0, 0, 0, napi_default, 0},
       {"decodeHDR", 0, js_decode_hdr, 0, 0, 0, napi_default, 0},
+      {"gltfParse", 0, js_gltf_parse, 0, 0, 0, napi_default, 0},
+      {"gltfSetResource", 0, js_gltf_set_resource, 0, 0, 0, napi_default, 0},
+      {"gltfBuild", 0, js_gltf_build, 0, 0, 0, napi_default, 0},
       {"createContext", 0, js_create_context, 0, 0, 0, napi_default, 0},
       {"sceneFromBVH", 0, js_scene_from_bvh, 0, 0, 0, napi_default, 0},
       {"uploadScene", 0, js_upload_scene, 0, 0, 0, napi_default, 0},

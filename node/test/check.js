@@ -3,6 +3,7 @@
 //   node check.js bvh <dir>     buildBVH over <dir>/positions.f32 -> nodes.f32, tri.u32
 //   node check.js hits <dir>    GpuTraversal over positions + bvh, rays org.f64/dir.f64 -> t/u/v.f64, tri.u32
 //   node check.js hdr <dir>     parseHDR(in.hdr) -> rgb.f32
+//   node check.js gltf <dir>    loadGLTF(in.gltf, files of <dir>) -> positions.f32 and a JSON summary
 //   node check.js async <dir>   the render scene through the queued async calls vs the sync ones
 //   node check.js render <dir>  LucentRenderer over the packed scene in <dir> -> color.f32, albedo.f32,
 //                               den.f32, then a camera move -> moved.f32, shot.u8
@@ -56,6 +57,20 @@ if (mode === 'guard') {
   let err = '';
   try { lucent.parseHDR(Buffer.from('P6\n1 1\n255\n')); } catch (e) { err = e.message; }
   process.stdout.write(JSON.stringify({ width: img.width, height: img.height, error: err }) + '\n');
+} else if (mode === 'gltf') {
+  const fetched = [];
+  const resolve = (uri) => { fetched.push(uri); return Promise.resolve(fs.readFileSync(path.join(dir, uri))); };
+  lucent.loadGLTF(fs.readFileSync(path.join(dir, 'in.gltf')), resolve).then((a) => {
+    write('positions.f32', a.meshes[0].mesh.positions);
+    return lucent.loadGLTF(Buffer.from('{"asset":{"version":"1.0"}}'), resolve).then(() => '', (e) => e.message)
+      .then((err) => {
+        process.stdout.write(JSON.stringify({
+          fetched, triangleCount: a.triangleCount, bounds: a.bounds, warnings: a.warnings,
+          material: a.materials[a.meshes[0].material], hasUvs: !!a.meshes[0].mesh.uvs,
+          images: a.images.map((im) => ({ mimeType: im.mimeType, name: im.name, data: Array.from(im.data) })), error: err,
+        }) + '\n');
+      });
+  });
 } else if (mode === 'hits') {
   const bvh = { nodes: f32('nodes.f32'), triIndices: u32('tri.u32') };
   const tr = new lucent.GpuTraversal(bvh, f32('positions.f32'));

@@ -259,3 +259,121 @@ def test_loaded_asset_packs_into_a_scene():
                  meshes=[S.MeshInstance(m, a.materials[i]) for m, i in a.meshes], textures=decoded)
     p = S.pack_scene(sc)
     assert p.tri_count == 1 and p.material_count >= 1
+
+
+# ---------------------------------------------------------------------------
+# Native loader edge cases beyond gltf.test.ts: fans, skipped primitives,
+# warnings, unlit and transformed materials, and the error paths.
+# ---------------------------------------------------------------------------
+def quad_glb(prims, extra=None, positions=None):
+    pos = np.float32(positions if positions is not None else [0, 0, 0, 1, 0, 0, 1, 1, 0, 0, 1, 0])
+    blob, _ = pack(pos)
+    js = {"asset": {"version": "2.0"}, "nodes": [{"mesh": 0}], "meshes": [{"primitives": prims}],
+          "accessors": [{"bufferView": 0, "componentType": FLOAT, "count": pos.size // 3, "type": "VEC3"}],
+          "bufferViews": [{"buffer": 0, "byteLength": pos.nbytes}], "buffers": [{"byteLength": len(blob)}]}
+    js.update(extra or {})
+    return make_glb(js, blob)
+
+
+def test_triangle_fan_and_skipped_points():
+    a = load_gltf(quad_glb([{"mode": 6, "attributes": {"POSITION": 0}}, {"mode": 0, "attributes": {"POSITION": 0}}]),
+                  no_fetch)
+    assert a.triangle_count == 2 and len(a.meshes) == 1
+    n = a.meshes[0][0].normals
+    assert abs(n[2] - 1) < 1e-6 and abs(n[11] - 1) < 1e-6  # fan triangles keep +Z winding
+    assert a.warnings == ["point and line primitives are skipped"]
+    assert a.bounds == ((0, 0, 0), (1, 1, 0))
+
+
+def test_unlit_material_emits_its_base_colour_and_transform_is_taken():
+    glb = triangle_glb({}, {"materials": [{
+        "pbrMetallicRoughness": {"baseColorFactor": [0.5, 0.25, 1, 1], "baseColorTexture": {
+            "index": 0, "texCoord": 1,
+            "extensions": {"KHR_texture_transform": {"offset": [0.5, 0.25], "scale": [2, 3],
+                                                     "rotation": math.pi / 2}}}},
+        "extensions": {"KHR_materials_unlit": {}}}]})
+    a = load_gltf(glb, no_fetch)
+    m = a.materials[0]
+    assert m.emission == (0.5, 0.25, 1) and m.albedo == (0, 0, 0) and m.metallic == 0
+    assert m.emissive_map == 0 and m.base_color_map == 0
+    assert np.allclose(m.uv_transform, texture_transform({"offset": [0.5, 0.25], "scale": [2, 3],
+                                                          "rotation": math.pi / 2}))
+    assert a.warnings == ["textures on TEXCOORD_1 or higher are sampled with TEXCOORD_0"]
+
+
+def test_ktx2_images_are_skipped_with_a_warning():
+    glb = triangle_glb({}, {"images": [{"uri": "t.ktx2"}]})
+    a = load_gltf(glb, no_fetch)
+    assert a.materials[0].base_color_map == -1 and a.images == []
+    assert a.warnings == ["KTX2/Basis textures are not supported and were skipped"]
+
+
+def test_primitives_share_one_default_material():
+    a = load_gltf(quad_glb([{"attributes": {"POSITION": 0}}, {"attributes": {"POSITION": 0}, "material": 5}]),
+                  no_fetch)
+    assert [i for _, i in a.meshes] == [0, 0] and len(a.materials) == 1
+
+
+def test_external_buffer_is_requested_before_images():
+    blob, off = pack(np.float32([0, 0, 0, 1, 0, 0, 0, 1, 0]))
+    js = {"asset": {"version": "2.0"}, "nodes": [{"mesh": 0}],
+          "meshes": [{"primitives": [{"attributes": {"POSITION": 0}, "material": 0}]}],
+          "materials": [{"emissiveTexture": {"index": 0}}], "textures": [{"source": 0}],
+          "images": [{"uri": "glow.png", "name": "glow"}],
+          "accessors": [{"bufferView": 0, "componentType": FLOAT, "count": 3, "type": "VEC3"}],
+          "bufferViews": [{"buffer": 0, "byteLength": 36}], "buffers": [{"byteLength": 36, "uri": "geo.bin"}]}
+    fetched = []
+
+    def fetch(uri):
+        fetched.append(uri)
+        return blob if uri == "geo.bin" else b"\x01\x02"
+
+    a = load_gltf(json.dumps(js).encode(), fetch)
+    assert fetched == ["geo.bin", "glow.png"]
+    assert (a.images[0].name, a.images[0].mime_type, a.images[0].data) == ("glow", "image/png", b"\x01\x02")
+
+
+@pytest.mark.parametrize("case,match", [
+    ("index", "out of range"),
+    ("cycle", "deeper than 64"),
+    ("empty", "no triangle meshes"),
+    ("view", "runs past its buffer"),
+    ("accessor", "missing accessor"),
+    ("json", "glTF JSON"),
+])
+def test_malformed_files_raise(case, match):
+    if case == "index":
+        blob, off = pack(np.float32([0, 0, 0, 1, 0, 0, 0, 1, 0]), np.uint16([0, 1, 3]))
+        js = {"asset": {"version": "2.0"}, "nodes": [{"mesh": 0}],
+              "meshes": [{"primitives": [{"attributes": {"POSITION": 0}, "indices": 1}]}],
+              "accessors": [{"bufferView": 0, "componentType": FLOAT, "count": 3, "type": "VEC3"},
+                            {"bufferView": 1, "componentType": 5123, "count": 3, "type": "SCALAR"}],
+              "bufferViews": [{"buffer": 0, "byteOffset": off[0], "byteLength": 36},
+                              {"buffer": 0, "byteOffset": off[1], "byteLength": 6}],
+              "buffers": [{"byteLength": len(blob)}]}
+        data = make_glb(js, blob)
+    elif case == "cycle":
+        data = quad_glb([{"attributes": {"POSITION": 0}}],
+                        {"nodes": [{"mesh": 0, "children": [1]}, {"children": [0]}], "scenes": [{"nodes": [0]}]})
+    elif case == "empty":
+        data = quad_glb([{"mode": 1, "attributes": {"POSITION": 0}}])
+    elif case == "view":
+        data = quad_glb([{"attributes": {"POSITION": 0}}], {"bufferViews": [{"buffer": 0, "byteLength": 4096}]})
+    elif case == "accessor":
+        data = quad_glb([{"attributes": {"POSITION": 3}}])
+    else:
+        data = b'{"asset": {"version": "2.0"}, '
+    with pytest.raises(GLTFError, match=match):
+        load_gltf(data, no_fetch)
+
+
+def test_native_loader_agrees_with_the_scene_packer_material_layout():
+    """The 40 texel values the native loader returns pack back to the same
+    texels pack_materials writes for the Material built from them."""
+    from lucent_amd import scene as S
+    a = load_gltf(triangle_glb({}, {"materials": [{"alphaMode": "BLEND", "emissiveFactor": [0.1, 0.2, 0.3],
+                                                   "normalTexture": {"index": 0, "scale": 2}}]}), no_fetch)
+    m = a.materials[0]
+    assert (m.alpha_mode, m.normal_map, m.normal_scale) == ("blend", 0, 2)
+    t = S.pack_materials([m])
+    assert t[15] == 2 and t[18] == 0 and t[31] == 0  # blend, normal map 0, no attenuation distance

@@ -97,6 +97,40 @@ def test_parse_hdr_through_node(tmp_path):
     assert "Radiance" in meta["error"]
 
 
+def test_load_gltf_through_node(tmp_path):
+    """loadGLTF through the addon: the same world-space positions, material
+    and images as the Python binding of the native loader, external files
+    fetched buffers first, and the reference's version error."""
+    import base64
+    from lucent_amd.gltf import load_gltf
+    pos = np.float32([0, 0, 0, 1, 0, 0, 0, 1, 0, 1, 1, 0])
+    (tmp_path / "geo.bin").write_bytes(pos.tobytes())
+    (tmp_path / "t.png").write_bytes(b"\x89PNG")
+    js = {"asset": {"version": "2.0"}, "nodes": [{"mesh": 0, "translation": [0, 0, 2], "scale": [-1, 1, 1]}],
+          "meshes": [{"primitives": [{"mode": 5, "attributes": {"POSITION": 0}, "material": 0}]}],
+          "materials": [{"emissiveTexture": {"index": 0}, "emissiveFactor": [1, 1, 1],
+                         "extensions": {"KHR_materials_emissive_strength": {"emissiveStrength": 3}}}],
+          "textures": [{"source": 0}], "images": [{"uri": "t.png"}],
+          "accessors": [{"bufferView": 0, "componentType": 5126, "count": 4, "type": "VEC3"}],
+          "bufferViews": [{"buffer": 0, "byteLength": pos.nbytes}],
+          "buffers": [{"byteLength": pos.nbytes, "uri": "geo.bin"}]}
+    data = json.dumps(js).encode()
+    (tmp_path / "in.gltf").write_bytes(data)
+    meta = node("gltf", tmp_path)
+    a = load_gltf(data, lambda uri: (tmp_path / uri).read_bytes())
+    assert meta["fetched"] == ["geo.bin", "t.png"]
+    assert meta["triangleCount"] == a.triangle_count == 2
+    assert np.array_equal(np.fromfile(tmp_path / "positions.f32", np.float32), a.meshes[0][0].positions)
+    assert meta["bounds"] == {"min": list(a.bounds[0]), "max": list(a.bounds[1])}
+    m = a.materials[0]
+    assert meta["material"]["emission"] == list(m.emission) == [3, 3, 3]
+    assert meta["material"]["emissiveMap"] == m.emissive_map == 0
+    assert meta["material"]["attenuationDistance"] is None  # Infinity serialises as null
+    assert meta["images"] == [{"mimeType": "image/png", "name": "t.png", "data": list(b"\x89PNG")}]
+    assert meta["hasUvs"] is False and meta["warnings"] == []
+    assert "only 2.0" in meta["error"]
+
+
 def write_render_inputs(tmp_path, W=48, H=32, SPP=3, SEED=9):
     """The feature scene's PackedScene arrays, lighting and cameras as files for node/test/check.js."""
     import sys
