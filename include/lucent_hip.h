@@ -178,7 +178,7 @@ lucent_status lucent_scene_stats_get(const lucent_scene *scene, lucent_scene_sta
  * ties resolved to the lowest leaf slot as its left-first traversal does);
  * LUCENT_PREC_GLSL_FP32 follows the shader's float32 traversal
  * (src/shaders/scene.glsl:221-238 hitTriangle, EPS = 1e-6 replaces tmin).
- * LUCENT_QUERY_ANY returns any accepted hit (occlusion, scene.glsl:444-485)
+ * LUCENT_QUERY_ANY returns any accepted hit (occlusion, scene.glsl:336-377)
  * with t/u/v of that hit.
  * A miss writes tri_id = 0xFFFFFFFF, t = +inf, u = v = 0 (the reference's null).
  * ---------------------------------------------------------------------- */

@@ -14,7 +14,7 @@
 // environment sampler's pdf are held constant (detached): the estimator is
 // unbiased for any choice of them, so its expectation's gradient is the
 // expectation of these terms.  df/dθ comes from forward-mode dual numbers
-// over the opaque BSDF (bsdf.glsl:272-324) with θ = (albedo rgb, roughness,
+// over the opaque BSDF (bsdf.glsl:114-166) with θ = (albedo rgb, roughness,
 // metallic) of the hit material.
 //
 // Reductions, FP64 throughout: material gradients are summed per block in
@@ -106,7 +106,7 @@ __device__ inline D5 dsqrt(D5 a) {
 }
 __device__ inline D5 one_minus(D5 a) { return dc(1.0f) - a; }
 
-// f of evalOpaque (bsdf.glsl:272-324) and its derivatives with respect to
+// f of evalOpaque (bsdf.glsl:114-166) and its derivatives with respect to
 // the hit material's albedo, roughness and metallic.  The sheen energy scale
 // and the coat layering do not depend on these parameters.
 __device__ __forceinline__ bool eval_opaque_grad(const PtScene& sc, const Surface& s, v3 wo, v3 wi, float f[3],

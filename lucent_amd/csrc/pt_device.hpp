@@ -622,7 +622,7 @@ __device__ inline int trace_bvh(const PtScene& s, v3 ro, v3 rd, float& closest, 
   return t.best;
 }
 
-// occluded (scene.glsl:444-485): any accepted hit below tmax.
+// occluded (scene.glsl:336-377): any accepted hit below tmax.
 template <class StackT>
 __device__ inline bool occluded(const PtScene& s, v3 ro, v3 rd, float tmax, Rng& rng, StackT& st) {
   for (uint32_t i = 0; i < s.sphere_count; ++i) {
@@ -637,7 +637,7 @@ __device__ inline bool occluded(const PtScene& s, v3 ro, v3 rd, float tmax, Rng&
   return false;
 }
 
-// The closest primitive found by traceScene's search (scene.glsl:300-441):
+// The closest primitive found by traceScene's search (scene.glsl:192-333):
 // kind 0 = miss, 1 = triangle (prim = leaf slot), 2 = sphere, 3 = ground.
 enum : uint32_t { kHitMiss = 0, kHitTri = 1, kHitSphere = 2, kHitGround = 3 };
 struct HitRecord {
@@ -741,7 +741,7 @@ __device__ inline bool build_hit(const PtScene& s, v3 ro, v3 rd, const HitRecord
   return true;
 }
 
-// traceScene (scene.glsl:300-441): spheres, ground, then triangles.
+// traceScene (scene.glsl:192-333): spheres, ground, then triangles.
 template <class StackT>
 __device__ inline bool trace_scene(const PtScene& s, v3 ro, v3 rd, Hit& hit, Rng& rng, StackT& st) {
   HitRecord r;
@@ -982,7 +982,7 @@ __device__ inline float d_charlie(float alpha, float ndh) {
   return (2.0f + inv) * gpow(sin2, inv * 0.5f) / TWO_PI;
 }
 __device__ inline float v_neubelt(float ndl, float ndv) { return 1.0f / (4.0f * (ndl + ndv - ndl * ndv)); }
-// Bilinear R16F LUT, CLAMP_TO_EDGE (renderer.ts:241-242, bsdf.glsl:247-250).
+// Bilinear R16F LUT, CLAMP_TO_EDGE (renderer.ts:241-242, bsdf.glsl:89-92).
 __device__ inline float sheen_albedo(const PtScene& s, float ndv, float r) {
   const float fx = ndv * float(kSheenLutSize - 1), fy = r * float(kSheenLutSize - 1);
   const float x = ((fx + 0.5f) / float(kSheenLutSize)) * float(kSheenLutSize) - 0.5f;

@@ -164,7 +164,7 @@ def box(mn, mx, inward=False) -> Mesh:
 
 def torus_knot(p=2, q=3, radius=1.0, tube=0.32, tubular_segments=200, radial_segments=20, uvs=False,
                displace=None) -> Mesh:
-    """mesh.ts:288-349: (p, q) knot, ring normals, 2*tubular*radial triangles.
+    """mesh.ts:95-156: (p, q) knot, ring normals, 2*tubular*radial triangles.
     uvs=True adds (i / tubular, j / radial) texture coordinates per vertex (the
     reference's knot has none).  displace(points, normals) -> offsets, if
     given, moves the ring vertices along their normals before triangulation."""

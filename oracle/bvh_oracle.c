@@ -648,7 +648,7 @@ uint32_t or_bvh_depth(const float *nodes, uint32_t node_count) {
 }
 
 /* ---------------------------------------------------------------------------
- * mesh.ts:288-349 — (p, q) torus knot with ring normals; f64 math stored f32.
+ * mesh.ts:95-156 — (p, q) torus knot with ring normals; f64 math stored f32.
  * Writes tubular*radial*2 triangles (9 floats each) into pos and nrm.
  * ------------------------------------------------------------------------- */
 static void knot_curve(double t, double p, double q, double radius, double *o) {

@@ -6,7 +6,7 @@
  *   pcg/rand/basis/samplers/offsetRay   src/shaders/common.glsl:10-108
  *   traceScene/occluded/alphaOpaque     src/shaders/scene.glsl:136-377
  *   environment and light sampling      src/shaders/lights.glsl:25-158
- *   layered BSDF                        src/shaders/bsdf.glsl:186-406
+ *   layered BSDF                        src/shaders/bsdf.glsl:28-248
  *   cameraRay/surfaceAt/tracePath/main  src/shaders/trace.frag:53-272, 324-355
  *   buildAliasTable                     src/math/alias.ts:17-64
  *   buildEnvSampler                     src/scene/environment.ts:52-85
