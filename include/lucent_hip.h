@@ -276,6 +276,23 @@ lucent_status lucent_scene_set_textures(lucent_ctx *ctx, lucent_scene *scene, ui
 lucent_status lucent_env_sampler(const float *rgb, uint32_t width, uint32_t height, uint32_t max_width,
                                  float *table_out, uint32_t *grid_w, uint32_t *grid_h, double *total);
 
+/* buildEnvSampler on the device, in float64: d_rgba is an RGBA float map
+ * (4 floats per texel, rows top to bottom; the scene's env map layout) in
+ * device memory, d_table_out (4 floats per cell, 16-byte aligned) receives
+ * the table.  Grid, table and total are bit-identical to lucent_env_sampler
+ * on the same texels: the cell weights, pmf and table texels are computed
+ * one thread per cell; the weight sum and Vose's pairing (alias.ts:44-63)
+ * are sequential float64 chains run by one wave, in the host order.
+ * Enqueued on `stream` (NULL = the context's stream) and synchronous;
+ * uses the context's scratch memory.  gpu_ms (may be NULL) receives the
+ * device time between events around the kernels.  Fails with the host
+ * build's message when a weight is negative or not finite, leaving
+ * d_table_out untouched.  No counterpart in the reference (it builds the
+ * table on the CPU, environment.ts:52-85). */
+lucent_status lucent_env_sampler_gpu(lucent_ctx *ctx, const float *d_rgba, uint32_t width, uint32_t height,
+                                     uint32_t max_width, float *d_table_out, uint32_t *grid_w, uint32_t *grid_h,
+                                     double *total, float *gpu_ms, void *stream);
+
 /* buildAliasTable (src/math/alias.ts:17-64), host-only. */
 lucent_status lucent_alias_table(const double *weights, uint32_t n, float *prob_out, uint32_t *alias_out,
                                  double *pmf_out, double *total_out);
@@ -635,9 +652,18 @@ lucent_status lucent_scene_set_materials(lucent_ctx *ctx, lucent_scene *scene, c
                                          uint32_t material_count);
 
 /* Replaces the environment map's texels (same size), keeping the importance
- * sampler built from the map given to lucent_scene_set_lighting. */
+ * sampler built from the map given to lucent_scene_set_lighting (rebuild it
+ * with lucent_scene_rebuild_env_sampler). */
 lucent_status lucent_scene_update_env_texels(lucent_ctx *ctx, lucent_scene *scene, const float *rgb,
                                              uint32_t width, uint32_t height);
+
+/* Rebuilds the scene's environment importance sampler from the map now on
+ * the device (after lucent_scene_update_env_texels) with
+ * lucent_env_sampler_gpu, in place; light sampling switches off if the map
+ * became black (total 0).  Synchronous on the context's stream; traces on
+ * other streams must not overlap it.  total_out may be NULL.  The reference
+ * rebuilds on the CPU when a map is uploaded (renderer.ts:389-406). */
+lucent_status lucent_scene_rebuild_env_sampler(lucent_ctx *ctx, lucent_scene *scene, double *total_out);
 
 /* Environment map size of the scene's lighting (0 before lighting is set). */
 lucent_status lucent_scene_env_size(const lucent_scene *scene, uint32_t *width, uint32_t *height);

@@ -843,6 +843,33 @@ lucent_status lucent_scene_update_env_texels(lucent_ctx* ctx, lucent_scene* scen
   return stream_sync(ctx->stream);
 }
 
+lucent_status lucent_env_sampler_gpu(lucent_ctx* ctx, const float* d_rgba, uint32_t width, uint32_t height,
+                                     uint32_t max_width, float* d_table_out, uint32_t* grid_w, uint32_t* grid_h,
+                                     double* total, float* gpu_ms, void* stream) {
+  clear_error();
+  if (!ctx) return fail(LUCENT_E_INVALID, "null ctx");
+  return build_env_sampler_gpu(ctx, d_rgba, width, height, max_width, d_table_out, grid_w, grid_h, total, stream,
+                               gpu_ms);
+}
+
+lucent_status lucent_scene_rebuild_env_sampler(lucent_ctx* ctx, lucent_scene* scene, double* total_out) {
+  clear_error();
+  if (!ctx || !scene) return fail(LUCENT_E_INVALID, "null ctx/scene");
+  PtScene& pt = scene->pt;
+  if (!scene->lighting_set || !pt.env_map || !pt.env_table) return fail(LUCENT_E_INVALID, "scene has no lighting");
+  uint32_t gw = 0, gh = 0;
+  double total = 0;
+  // Writes the scene's own table in place (same grid: the map keeps its size);
+  // on failure the table is left as it was.
+  Status s = build_env_sampler_gpu(ctx, pt.env_map, uint32_t(pt.env_w), uint32_t(pt.env_h), 1024,
+                                   const_cast<float*>(pt.env_table), &gw, &gh, &total, ctx->stream, nullptr);
+  if (s != LUCENT_OK) return s;
+  if (int32_t(gw) != pt.grid_w || int32_t(gh) != pt.grid_h) return fail(LUCENT_E_INVALID, "sampler grid changed");
+  pt.env_sampling = total > 0 ? 1 : 0;
+  if (total_out) *total_out = total;
+  return LUCENT_OK;
+}
+
 lucent_status lucent_scene_env_size(const lucent_scene* scene, uint32_t* w, uint32_t* h) {
   if (!scene) return fail(LUCENT_E_INVALID, "null scene");
   if (w) *w = uint32_t(scene->pt.env_w);
@@ -863,7 +890,6 @@ static Status denoise_args(lucent_film* film, const lucent_camera* cam, const lu
     Status s = dev_alloc(&film->filter, px * (16 * 4 + 8));
     if (s != LUCENT_OK) return s;
   }
-  a->compute_units = std::max(film->ctx->compute_units, 1);
   a->compute_units = std::max(film->ctx->compute_units, 1);
   a->color = film->f.color;
   a->albedo = film->f.albedo;

@@ -64,16 +64,24 @@ Status build_alias_table(const double* w, uint32_t n, float* prob, uint32_t* ali
 
 static double lum(double r, double g, double b) { return 0.2126 * r + 0.7152 * g + 0.0722 * b; }
 
+void env_grid(uint32_t width, uint32_t height, uint32_t max_width, uint32_t* gw_out, uint32_t* gh_out) {
+  const uint32_t gw = std::max(1u, std::min(width, max_width));
+  *gw_out = gw;
+  *gh_out = std::max(1u, std::min(height, uint32_t(std::round(double(height) * gw / width))));
+}
+
+double env_row_sine(uint32_t cy, uint32_t gh) { return std::sin((M_PI * (cy + 0.5)) / gh); }
+
 Status build_env_sampler(const float* rgb, uint32_t width, uint32_t height, uint32_t max_width, EnvSampler* out) {
   if (!rgb || width == 0 || height == 0) return fail(LUCENT_E_INVALID, "empty environment map");
-  const uint32_t gw = std::max(1u, std::min(width, max_width));
-  const uint32_t gh = std::max(1u, std::min(height, uint32_t(std::round(double(height) * gw / width))));
+  uint32_t gw, gh;
+  env_grid(width, height, max_width, &gw, &gh);
   const size_t n = size_t(gw) * gh;
   std::vector<double> weights(n);
   for (uint32_t cy = 0; cy < gh; ++cy) {
     const uint32_t y0 = uint32_t(std::floor(double(cy) * height / gh));
     const uint32_t y1 = std::max(y0 + 1, uint32_t(std::floor(double(cy + 1) * height / gh)));
-    const double sin_theta = std::sin((M_PI * (cy + 0.5)) / gh);
+    const double sin_theta = env_row_sine(cy, gh);
     for (uint32_t cx = 0; cx < gw; ++cx) {
       const uint32_t x0 = uint32_t(std::floor(double(cx) * width / gw));
       const uint32_t x1 = std::max(x0 + 1, uint32_t(std::floor(double(cx + 1) * width / gw)));

@@ -58,10 +58,15 @@ class DiffRenderer:
         self.materials = list(materials)
         self._version += 1
 
-    def set_env_texels(self, rgb) -> None:
+    def set_env_texels(self, rgb, rebuild_sampler: bool = False) -> None:
+        """New environment texels (same size).  The importance sampler stays
+        the one built with the lighting unless `rebuild_sampler` (rebuilt on
+        the device from the new texels)."""
         arr = np.ascontiguousarray(np.asarray(rgb, np.float32).reshape(-1))
         _native.check(_native.lib().lucent_scene_update_env_texels(self.ctx.handle, self.renderer.device_scene.handle,
                                                                    arr.ctypes.data, *self.env_size))
+        if rebuild_sampler:
+            self.renderer.device_scene.rebuild_env_sampler()
         self.env = arr.copy()
         self._version += 1
 
