@@ -643,6 +643,52 @@ def bench_copy(args) -> dict:
             "dtype": "u8", "config": {"workload": "device copy of 1 GiB (counter calibration)", "bytes": n}}
 
 
+def bench_env_sampler(args) -> dict:
+    """The environment importance sampler (buildEnvSampler + Vose alias
+    table, float64) built on the device from a map already in HBM, against
+    the host build of the same tables: the path-trace HDRI (1024x512) and
+    the differentiable-rendering map.  Device time between events around the
+    seven kernels (median of 9 builds); host time of lucent_env_sampler from
+    the same texels; tables compared bit for bit."""
+    import ctypes as C
+
+    import torch
+
+    from lucent_amd import _native, workloads
+    from lucent_amd.runtime import Context
+
+    ctx = Context(0)
+    lib = _native.lib()
+    res = {}
+    diff_env = workloads.spheres_scene().env.map
+    for name, (w, h, rgb) in {"hdri_1024x512": (1024, 512, workloads.synthetic_hdri(1024, 512, sigma_deg=3.0)),
+                              f"diff_env_{diff_env[0]}x{diff_env[1]}": diff_env}.items():
+        rgb = np.ascontiguousarray(np.asarray(rgb, np.float32).reshape(-1))
+        rgba = np.zeros((h, w, 4), np.float32)
+        rgba[..., :3] = rgb.reshape(h, w, 3)
+        d_map = torch.from_numpy(rgba).to("cuda")
+        dev_ms = []
+        for _ in range(10):
+            table, gw, gh, total, ms = ctx.env_sampler_gpu(d_map)
+            dev_ms.append(ms)
+        host_table = np.zeros(gw * gh * 4, np.float32)
+        hgw, hgh, htotal = C.c_uint32(), C.c_uint32(), C.c_double()
+        host_ms = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            _native.check(lib.lucent_env_sampler(rgb.ctypes.data, w, h, 1024, host_table.ctypes.data, C.byref(hgw),
+                                                 C.byref(hgh), C.byref(htotal)))
+            host_ms.append((time.perf_counter() - t0) * 1e3)
+        same = bool(np.array_equal(table.cpu().numpy().reshape(-1).view(np.uint32), host_table.view(np.uint32))
+                    and total == htotal.value)
+        res[name] = {"grid": [gw, gh], "device_ms": round(float(np.median(dev_ms[1:])), 4),
+                     "host_ms": round(float(np.median(host_ms)), 4), "bit_identical": same}
+        log(f"env sampler {name}: device {res[name]['device_ms']} ms, host {res[name]['host_ms']} ms, "
+            f"identical {same}")
+    ctx.close()
+    return res
+
+
 def bench_frame_graph(args) -> dict:
     """Interactive frames on the knot scene: 1 sample, a 3-level denoise and
     the display pass per frame, issued as eager calls (9 kernel launches
@@ -709,7 +755,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "closest-hit", "path-trace", "batch", "denoise", "adjoint", "frame-graph", "copy"])
+                    choices=["all", "closest-hit", "path-trace", "batch", "denoise", "adjoint", "frame-graph", "copy",
+                             "env-sampler"])
     ap.add_argument("--cpu-sample", type=int, default=1 << 19)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="batch: time only the batched launch (for profiling)")
@@ -732,6 +779,12 @@ def main() -> int:
         d = bench_batch(args)
         res = {"value": d.pop("value"), "unit": d.pop("unit"), "ms_per_step": d["batched_s"] * 1e3, "dtype": "f32",
                "config": {"workload": d.pop("metric")}, **d}
+    elif args.workload == "env-sampler":
+        d = bench_env_sampler(args)
+        first = d["hdri_1024x512"]
+        res = {"value": first["device_ms"], "unit": "ms per sampler build (1024x512 map)",
+               "ms_per_step": first["device_ms"], "dtype": "f64", "higher_is_better": False,
+               "config": {"workload": "environment sampler build from a map in HBM"}, **d}
     elif args.workload == "frame-graph":
         d = bench_frame_graph(args)
         res = {"value": d.pop("value"), "unit": d.pop("unit"), "ms_per_step": d["128x96"]["graph"]["ms_per_frame"],
@@ -751,6 +804,7 @@ def main() -> int:
             extra["denoise"] = bench_denoise(args)
             extra["adjoint"] = bench_adjoint(args)
             extra["frame_graph"] = bench_frame_graph(args)
+            extra["env_sampler"] = bench_env_sampler(args)
     line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
             "higher_is_better": res.pop("higher_is_better", True), "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
