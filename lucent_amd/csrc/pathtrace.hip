@@ -245,34 +245,82 @@ __device__ __forceinline__ uint32_t run_item(const TileItem& it, uint32_t job, u
   }
 }
 
-// Per-lane words of the tile kernels' scratch area: three light-sample
-// slots of the fused loop (origin + extent, direction, unoccluded
-// contribution: 12 words each), then the traversal stack's spill words.
-constexpr uint32_t kShadowWords = 36;
+// The fused loop's three light-sample slots per lane: two float4 each, the
+// shadow ray's direction and length, and its unoccluded contribution.  (The
+// ray's origin is the step's shadow origin, which the path state keeps.)
+constexpr uint32_t kSlotFloat4 = 6;
+// Per-lane words of the tile kernels' scratch area: the light-sample slots
+// (used when they do not live in LDS), then the traversal stack's spill words.
+constexpr uint32_t kShadowWords = 4 * kSlotFloat4;
 __host__ __device__ inline uint32_t tile_lane_words(uint32_t stack_words) {
   return kShadowWords + ((stack_words + 3u) & ~3u);
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) f32x4 lds_f4;
+
+// Slots in LDS, lane-interleaved (element e of a lane is e * kPtBlock
+// float4 after its element 0), so a wave's access is one conflict-free
+// ds_read/ds_write_b128.
+struct LdsSlots {
+  lds_f4* base;
+  __device__ __forceinline__ float4 load(uint32_t e) const {
+    const f32x4 v = base[e * kPtBlock];
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+  __device__ __forceinline__ void store(uint32_t e, float4 v) {
+    f32x4 w;
+    w.x = v.x;
+    w.y = v.y;
+    w.z = v.z;
+    w.w = v.w;
+    base[e * kPtBlock] = w;
+  }
+};
+// Slots in the lane's own scratch area (occupancies whose LDS share holds
+// only the traversal stack).
+struct AreaSlots {
+  float4* base;
+  __device__ __forceinline__ float4 load(uint32_t e) const { return base[e]; }
+  __device__ __forceinline__ void store(uint32_t e, float4 v) { base[e] = v; }
+};
+
+// LDS of the tile kernels per lane at kWaves waves per SIMD (four SIMDs
+// share a CU's 160 KiB: 40 words per lane at 4 waves, 53 at 3, 32 at 5, 26
+// at 6): the traversal stack's ring of kStack entries (2 words each) and, up
+// to 4 waves, the light-sample slots.
+template <int kWaves>
+struct TileLds {
+  static constexpr bool kSlots = kWaves <= 4;
+  static constexpr int kStack = kWaves == 4 ? 8 : 12;
+  static constexpr int kWords = 2 * kStack + (kSlots ? 4 * int(kSlotFloat4) : 0);
+  using Stack = pt::Stack4<kStack, kPtBlock>;
+};
+
 // The fused loop's path-step tracer: the closest hit was traced before the
-// step; each light sample goes to slot `which` of the lane's scratch area
-// (its ray and its contribution, evaluated here: an unoccluded sample adds
-// exactly this value), to be traced after the step in slot order.
+// step; each light sample goes to slot `which` (its ray and its
+// contribution, evaluated here: an unoccluded sample adds exactly this
+// value), to be traced after the step in slot order.  Every light sample of
+// a step starts at the step's shadow origin, and so does the continuation
+// ray (path_step_with sets p.ro to it before returning true), so the origin
+// is stored once, in p.ro; a path that ends at this step no longer reads it.
+template <class SlotsT>
 struct DeferTracer {
   static constexpr bool kRecordPrimary = false;
   const PtScene& sc;
-  float4* slots;
+  SlotsT& slots;
   pt::HitRecord rec;
   uint32_t mask;  // slots filled (bit `which`)
   __device__ __forceinline__ bool closest(pt::v3 ro, pt::v3 rd, pt::Hit& hit, pt::Rng&) {
     return pt::build_hit(sc, ro, rd, rec, hit);
   }
   template <class F>
-  __device__ __forceinline__ void nee(int which, pt::v3 o, pt::v3 d, float tmax, pt::Rng&, pt::PathState&,
+  __device__ __forceinline__ void nee(int which, pt::v3 o, pt::v3 d, float tmax, pt::Rng&, pt::PathState& p,
                                       F contribution) {
     const pt::v3 c = contribution();
-    slots[3 * which] = make_float4(o.x, o.y, o.z, tmax);
-    slots[3 * which + 1] = make_float4(d.x, d.y, d.z, 0.0f);
-    slots[3 * which + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    p.ro = o;
+    slots.store(2 * which, make_float4(d.x, d.y, d.z, tmax));
+    slots.store(2 * which + 1, make_float4(c.x, c.y, c.z, 0.0f));
     mask |= 1u << which;
   }
 };
@@ -299,11 +347,11 @@ enum : uint32_t { kLaneIdle = 0, kLaneTrace = 1, kLaneShade = 2, kLaneFinish = 3
 
 // One lane's path in the fused loop: its state, the ray being traced and
 // the light samples of the current step.
-template <class StackT>
+template <class StackT, class SlotsT>
 struct FusedPath {
   const PtScene& sc;
   StackT& st;
-  float4* slots;  // the lane's light-sample slots
+  SlotsT slots;  // the lane's light-sample slots
   uint32_t state = kLaneIdle;
   pt::Rng rng{};
   pt::PathState p;
@@ -334,14 +382,14 @@ struct FusedPath {
     while (pending != 0) {
       which = uint32_t(__builtin_ctz(pending));
       pending &= pending - 1u;
-      const float4 o = slots[3 * which], d = slots[3 * which + 1];
-      const pt::v3 ro = pt::mk(o), rd = pt::mk(d);
+      const float4 d = slots.load(2 * which);
+      const pt::v3 ro = p.ro, rd = pt::mk(d);
       bool blocked = false;
       for (uint32_t i = 0; i < sc.sphere_count && !blocked; ++i)
-        blocked = pt::hit_sphere(ro, rd, reinterpret_cast<const float4*>(sc.spheres)[i], o.w) < o.w;
-      if (!blocked) blocked = pt::hit_ground(sc, ro, rd, o.w) < o.w;
+        blocked = pt::hit_sphere(ro, rd, reinterpret_cast<const float4*>(sc.spheres)[i], d.w) < d.w;
+      if (!blocked) blocked = pt::hit_ground(sc, ro, rd, d.w) < d.w;
       if (!blocked) {
-        pt::trav4_begin(sc, t, ro, rd, o.w, st);
+        pt::trav4_begin(sc, t, ro, rd, d.w, st);
         any = true;
         state = kLaneTrace;
         return;
@@ -360,7 +408,7 @@ struct FusedPath {
     if (!pt::trav4_done(t, st)) blocked = pt::trav4_step_mixed(sc, t, st, rng, any);
     if (blocked || pt::trav4_done(t, st)) {
       if (any) {
-        if (!blocked) p.radiance = p.radiance + pt::mk(slots[3 * which + 2]);
+        if (!blocked) p.radiance = p.radiance + pt::mk(slots.load(2 * which + 1));
         next_ray();
       } else {
         state = kLaneShade;
@@ -382,7 +430,7 @@ struct FusedPath {
       rec.prim = analytic & 0x3FFFFFFFu;
       rec.u = rec.v = 0.0f;
     }
-    DeferTracer tr{sc, slots, rec, 0u};
+    DeferTracer<SlotsT> tr{sc, slots, rec, 0u};
     cont = pt::path_step_with(sc, f, p, rng, tr);
     pending = tr.mask;
     next_ray();
@@ -407,9 +455,9 @@ __device__ __forceinline__ void trace_turns(Lane& lp, uint32_t turns = kTraceTur
     if (lp.state == kLaneTrace) lp.trace_turn();
 }
 
-template <class StackT>
+template <class StackT, class SlotsT>
 __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t job, uint32_t job_base, uint32_t job_end,
-                                                   uint32_t* counter, StackT& st, float4* slots) {
+                                                   uint32_t* counter, StackT& st, SlotsT slots) {
   const PtFrame& f = it.frame;
   const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
   const uint32_t lane = threadIdx.x;
@@ -426,7 +474,7 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
   bool taking = true;
   uint32_t next = 0;
   uint32_t idx = 0, k = 0;
-  FusedPath<StackT> lp{it.scene, st, slots};
+  FusedPath<StackT, SlotsT> lp{it.scene, st, slots};
   for (;;) {
     uint64_t idle = __ballot(lp.state == kLaneIdle);
     while (taking && idle != 0) {
@@ -484,9 +532,9 @@ constexpr uint32_t kRecordFloat4 = 3;
 // bit-identical.  All samples of a job are handed out by the wave that took
 // it, so records and counters are shared within one wave only
 // (workgroup-scope ordering).
-template <class StackT>
+template <class StackT, class SlotsT>
 __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job, uint32_t n_jobs, uint32_t* counter,
-                                                StackT& st, float4* slots, float4* __restrict__ records,
+                                                StackT& st, SlotsT slots, float4* __restrict__ records,
                                                 uint32_t* __restrict__ arrived) {
   const PtFrame& f = it.frame;
   const uint32_t seed0 = it.seed_src ? __builtin_amdgcn_readfirstlane(*it.seed_src) : f.seed;
@@ -509,7 +557,7 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
   open_job(job);
   bool taking = true;
   uint32_t idx = 0, k = 0, rslot = 0;
-  FusedPath<StackT> lp{it.scene, st, slots};
+  FusedPath<StackT, SlotsT> lp{it.scene, st, slots};
   for (;;) {
     uint64_t idle = __ballot(lp.state == kLaneIdle);
     while (taking && idle != 0) {
@@ -580,9 +628,9 @@ constexpr uint32_t kRingJobs = LUCENT_RING_JOBS;
 // all handed out kRingJobs - 1 jobs earlier, so the wave seldom waits).  A change of film waits only for the paths in flight, not for the
 // lanes' pixels' remaining samples.  Returns the first job taken that
 // belongs to another film (or >= the queue's end).
-template <class StackT>
+template <class StackT, class SlotsT>
 __device__ __forceinline__ uint32_t run_units_ring(const TileItem& it, uint32_t job, uint32_t job_base, uint32_t job_end,
-                                                   uint32_t* counter, StackT& st, float4* slots,
+                                                   uint32_t* counter, StackT& st, SlotsT slots,
                                                    float4* __restrict__ records, uint32_t* __restrict__ arrived,
                                                    uint32_t s_max) {
   const PtFrame& f = it.frame;
@@ -614,7 +662,7 @@ __device__ __forceinline__ uint32_t run_units_ring(const TileItem& it, uint32_t 
   bool taking = true;
   uint32_t next = 0;
   uint32_t idx = 0, k = 0, rslot = 0;
-  FusedPath<StackT> lp{it.scene, st, slots};
+  FusedPath<StackT, SlotsT> lp{it.scene, st, slots};
   for (;;) {
     uint64_t idle = __ballot(lp.state == kLaneIdle);
     while (taking && idle != 0) {
@@ -686,6 +734,16 @@ __device__ __forceinline__ uint32_t run_units_ring(const TileItem& it, uint32_t 
   }
 }
 
+// A lane's light-sample slots in a tile kernel's LDS (after the stack
+// ring's two planes) or in its scratch area.
+template <int kWaves>
+__device__ __forceinline__ auto tile_slots(uint32_t* lds, uint32_t* area) {
+  if constexpr (TileLds<kWaves>::kSlots)
+    return LdsSlots{(lds_f4*)(lds + 2 * TileLds<kWaves>::kStack * kPtBlock) + threadIdx.x};
+  else
+    return AreaSlots{reinterpret_cast<float4*>(area)};
+}
+
 // Many films, work handed out per pixel sample (run_units_ring).  Each wave
 // owns kRingJobs * 64 * s_max records and kRingJobs * 64 counters.
 template <int kWaves>
@@ -693,10 +751,10 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
     const TileItem* __restrict__ items, const uint32_t* __restrict__ job_begin, uint32_t n_items, uint32_t n_jobs,
     uint32_t* __restrict__ counter, uint32_t* scratch, uint32_t lane_words, float4* __restrict__ records,
     uint32_t* __restrict__ arrived, uint32_t s_max) {
-  __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
+  __shared__ __align__(16) uint32_t lds[TileLds<kWaves>::kWords * kPtBlock];
   uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
-  float4* slots = reinterpret_cast<float4*>(area);
-  pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
+  const auto slots = tile_slots<kWaves>(lds, area);
+  typename TileLds<kWaves>::Stack st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
   float4* my_records = records + size_t(blockIdx.x) * kRingJobs * kPtBlock * s_max * kRecordFloat4;
   uint32_t* my_arrived = arrived + size_t(blockIdx.x) * kRingJobs * kPtBlock;
   uint32_t job = take_job(counter);
@@ -723,10 +781,10 @@ template <int kWaves>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves))) void k_trace_units(
     const TileItem* __restrict__ single, uint32_t n_jobs, uint32_t* __restrict__ counter, uint32_t* scratch,
     uint32_t lane_words, float4* __restrict__ records, uint32_t* __restrict__ arrived) {
-  __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
+  __shared__ __align__(16) uint32_t lds[TileLds<kWaves>::kWords * kPtBlock];
   uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
-  float4* slots = reinterpret_cast<float4*>(area);
-  pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
+  const auto slots = tile_slots<kWaves>(lds, area);
+  typename TileLds<kWaves>::Stack st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
   const uint32_t job = take_job(counter);
   if (job < n_jobs) run_units_fused(*single, job, n_jobs, counter, st, slots, records, arrived);
 }
@@ -748,10 +806,10 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(kWaves
                                                          const uint32_t* __restrict__ job_begin, uint32_t n_items,
                                                          uint32_t n_jobs, uint32_t* __restrict__ counter,
                                                          uint32_t* scratch, uint32_t lane_words) {
-  __shared__ uint32_t lds[pt::kPtStackWords * kPtBlock];
+  __shared__ __align__(16) uint32_t lds[TileLds<kWaves>::kWords * kPtBlock];
   uint32_t* area = scratch + size_t(blockIdx.x * kPtBlock + threadIdx.x) * lane_words;
-  float4* slots = reinterpret_cast<float4*>(area);
-  pt::PtStack<kPtBlock> st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
+  const auto slots = tile_slots<kWaves>(lds, area);
+  typename TileLds<kWaves>::Stack st{pt::as_lds(lds + threadIdx.x), area + kShadowWords, 0, 0};
   uint32_t job = take_job(counter);
   while (job < n_jobs) {
     if constexpr (kSingle) {
