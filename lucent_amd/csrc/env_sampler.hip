@@ -70,22 +70,41 @@ __global__ __launch_bounds__(kBlock) void k_env_cells(const float4* __restrict__
   if (!(v >= 0) || !isfinite(v)) atomicMin(info, i);
 }
 
+// The weights' sum in index order.  The next block of weights is loaded
+// into registers while the current one is summed from LDS, and the sum reads
+// two weights per LDS access, sixteen ahead of the chain of additions.
+constexpr uint32_t kTotalPerLane = kStage / 64;
 __global__ __launch_bounds__(64) void k_env_total(const double* __restrict__ w, uint32_t n,
                                                   const uint32_t* __restrict__ info, double* __restrict__ total) {
-  __shared__ double stage[kStage];
+  __shared__ __align__(16) double stage[kStage];
   if (build_failed(info)) return;
+  double regs[kTotalPerLane];
+  auto load = [&](uint32_t base) {
+#pragma unroll
+    for (uint32_t j = 0; j < kTotalPerLane; ++j) {
+      const uint32_t k = base + j * 64 + threadIdx.x;
+      regs[j] = k < n ? w[k] : 0.0;
+    }
+  };
   double t = 0;
+  load(0);
   for (uint32_t base = 0; base < n; base += kStage) {
-    const uint32_t m = min(kStage, n - base);
-    for (uint32_t k = threadIdx.x; k < m; k += 64) stage[k] = w[base + k];
+#pragma unroll
+    for (uint32_t j = 0; j < kTotalPerLane; ++j) stage[j * 64 + threadIdx.x] = regs[j];
     __syncthreads();
+    if (base + kStage < n) load(base + kStage);
+    const uint32_t m = min(kStage, n - base);
+    const double2* s2 = reinterpret_cast<const double2*>(stage);
     uint32_t k = 0;
-    for (; k + 8 <= m; k += 8) {
-      double v[8];
+    for (; k + 16 <= m; k += 16) {
+      double2 v[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = stage[k + j];
+      for (int j = 0; j < 8; ++j) v[j] = s2[k / 2 + j];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) t += v[j];
+      for (int j = 0; j < 8; ++j) {
+        t += v[j].x;
+        t += v[j].y;
+      }
     }
     for (; k < m; ++k) t += stage[k];
     __syncthreads();
@@ -150,7 +169,8 @@ __global__ __launch_bounds__(1024) void k_env_offsets(uint32_t* __restrict__ cou
 __global__ __launch_bounds__(kBlock) void k_env_scatter(const double* __restrict__ scaled, uint32_t n,
                                                         const uint32_t* __restrict__ offsets,
                                                         const uint32_t* __restrict__ info,
-                                                        uint32_t* __restrict__ small, uint32_t* __restrict__ large) {
+                                                        uint32_t* __restrict__ small, uint32_t* __restrict__ large,
+                                                        double* __restrict__ small_val, double* __restrict__ large_val) {
   __shared__ uint32_t wave_small[kBlock / 64], wave_valid[kBlock / 64];
   if (build_failed(info)) return;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -172,23 +192,59 @@ __global__ __launch_bounds__(kBlock) void k_env_scatter(const double* __restrict
   const uint32_t rs = ps + uint32_t(__popcll(bs & below));
   const uint32_t rv = pv + uint32_t(__popcll(bv & below));
   if (!valid) return;
-  if (is_small)
+  const double v = scaled[i];
+  if (is_small) {
     small[offsets[blockIdx.x] + rs] = i;
-  else
+    small_val[offsets[blockIdx.x] + rs] = v;
+  } else {
     large[offsets[gridDim.x + blockIdx.x] + (rv - rs)] = i;
+    large_val[offsets[gridDim.x + blockIdx.x] + (rv - rs)] = v;
+  }
 }
 
-// Stages list positions [base, top) (at most kStage of them) into LDS:
-// the cell index and its scaled weight.
-__device__ inline void stage_list(const uint32_t* __restrict__ list, const double* __restrict__ scaled, uint32_t base,
-                                  uint32_t top, uint32_t* idx, double* val) {
-  for (uint32_t k = threadIdx.x; k < top - base; k += 64) {
-    const uint32_t c = list[base + k];
-    idx[k] = c;
-    val[k] = scaled[c];
+// One worklist as the sweep reads it: entries are taken from the top
+// (descending positions); blocks of kStage entries (cell index and scaled
+// weight, contiguous in global memory) are staged into LDS, and the next
+// entry is read from LDS one step ahead of its use, so that the LDS latency
+// overlaps the step that precedes it.
+struct SweepList {
+  const uint32_t* idx;
+  const double* val;
+  uint32_t* s_idx;
+  double* s_val;
+  uint32_t left;        // entries not yet taken: positions [0, left)
+  uint32_t base;        // first staged position
+  uint32_t next_cell;   // the entry at position left - 1 (valid when left > 0)
+  double next_value;
+
+  __device__ void stage() {
+    const uint32_t top = left;
+    base = top > kStage ? top - kStage : 0;
+    for (uint32_t k = threadIdx.x; k < top - base; k += 64) {
+      s_idx[k] = idx[base + k];
+      s_val[k] = val[base + k];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-}
+  __device__ void peek() {
+    if (left == 0) return;
+    if (left - 1 < base) stage();
+    next_cell = s_idx[left - 1 - base];
+    next_value = s_val[left - 1 - base];
+  }
+  __device__ void begin(uint32_t n) {
+    left = n;
+    base = n;
+    peek();
+  }
+  // Takes the top entry (left > 0) and reads the one below it.
+  __device__ void take(uint32_t& cell, double& value) {
+    cell = next_cell;
+    value = next_value;
+    --left;
+    peek();
+  }
+};
 
 __device__ inline void emit(float* prob, uint32_t* alias, uint32_t cell, float p, uint32_t a) {
   if (threadIdx.x == 0) {
@@ -199,13 +255,17 @@ __device__ inline void emit(float* prob, uint32_t* alias, uint32_t cell, float p
 
 // One wave; every lane holds the same state, so all branches are uniform.
 __global__ __launch_bounds__(64) void k_env_sweep(const uint32_t* __restrict__ small, const uint32_t* __restrict__ large,
-                                                  const double* __restrict__ scaled, const uint32_t* __restrict__ info,
-                                                  float* __restrict__ prob, uint32_t* __restrict__ alias) {
+                                                  const double* __restrict__ small_val,
+                                                  const double* __restrict__ large_val,
+                                                  const uint32_t* __restrict__ info, float* __restrict__ prob,
+                                                  uint32_t* __restrict__ alias) {
   __shared__ uint32_t s_idx[kStage], l_idx[kStage];
   __shared__ double s_val[kStage], l_val[kStage];
   if (build_failed(info)) return;
-  uint32_t sp = info[1], lp = info[2];  // remaining list entries: positions [0, sp) and [0, lp)
-  uint32_t sb = sp, lb = lp;            // staged positions start here (nothing staged yet)
+  SweepList sm{small, small_val, s_idx, s_val, 0, 0, 0, 0.0};
+  SweepList lg{large, large_val, l_idx, l_val, 0, 0, 0, 0.0};
+  sm.begin(info[1]);
+  lg.begin(info[2]);
   bool heavy = false, pending = false;  // a current large cell; a former large cell on top of the small list
   uint32_t l = 0, p = 0;
   double rl = 0, rp = 0;
@@ -217,27 +277,15 @@ __global__ __launch_bounds__(64) void k_env_sweep(const uint32_t* __restrict__ s
       xs = rp;
       pending = false;
     } else {
-      if (sp == 0) break;
-      if (sp == sb) {
-        sb = sp > kStage ? sp - kStage : 0;
-        stage_list(small, scaled, sb, sp, s_idx, s_val);
-      }
-      --sp;
-      s = s_idx[sp - sb];
-      xs = s_val[sp - sb];
+      if (sm.left == 0) break;
+      sm.take(s, xs);
     }
     if (!heavy) {
-      if (lp == 0) {
+      if (lg.left == 0) {
         emit(prob, alias, s, 1.0f, s);  // no large cell left: s keeps its own bucket
         break;
       }
-      if (lp == lb) {
-        lb = lp > kStage ? lp - kStage : 0;
-        stage_list(large, scaled, lb, lp, l_idx, l_val);
-      }
-      --lp;
-      l = l_idx[lp - lb];
-      rl = l_val[lp - lb];
+      lg.take(l, rl);
       heavy = true;
     }
     emit(prob, alias, s, float(xs), l);
@@ -250,11 +298,11 @@ __global__ __launch_bounds__(64) void k_env_sweep(const uint32_t* __restrict__ s
     }
   }
   if (heavy) emit(prob, alias, l, 1.0f, l);
-  for (uint32_t k = threadIdx.x; k < lp; k += 64) {
+  for (uint32_t k = threadIdx.x; k < lg.left; k += 64) {
     prob[large[k]] = 1.0f;
     alias[large[k]] = large[k];
   }
-  for (uint32_t k = threadIdx.x; k < sp; k += 64) {
+  for (uint32_t k = threadIdx.x; k < sm.left; k += 64) {
     prob[small[k]] = 1.0f;
     alias[small[k]] = small[k];
   }
@@ -285,8 +333,9 @@ Status build_env_sampler_gpu(Context* ctx, const float* d_rgba, uint32_t width, 
   const size_t n = size_t(gw) * gh;
   if (n > (size_t(1) << 28)) return fail(LUCENT_E_INVALID, "environment sampler grid larger than 2^28 cells");
   const uint32_t nb = uint32_t((n + kBlock - 1) / kBlock);
-  // Scratch: w/pmf, scaled (f64 n each), small, large, alias (u32 n), prob
-  // (f32 n), block counts (2 nb), info (4 u32), total (f64), row sines (gh).
+  // Scratch: w/pmf, scaled (f64 n each), small, large (u32 n) and their
+  // values (f64 n), alias (u32 n), prob (f32 n), block counts (2 nb), info
+  // (4 u32), total (f64), row sines (gh).
   size_t off = 0;
   auto take = [&](size_t bytes) {
     const size_t at = off;
@@ -294,6 +343,7 @@ Status build_env_sampler_gpu(Context* ctx, const float* d_rgba, uint32_t width, 
     return at;
   };
   const size_t o_w = take(8 * n), o_sc = take(8 * n), o_small = take(4 * n), o_large = take(4 * n);
+  const size_t o_small_val = take(8 * n), o_large_val = take(8 * n);
   const size_t o_alias = take(4 * n), o_prob = take(4 * n), o_counts = take(8 * size_t(nb));
   const size_t o_info = take(16), o_total = take(8), o_sin = take(8 * size_t(gh));
   Status s = ensure_scratch(ctx, off);
@@ -303,6 +353,8 @@ Status build_env_sampler_gpu(Context* ctx, const float* d_rgba, uint32_t width, 
   double* scaled = reinterpret_cast<double*>(base + o_sc);
   uint32_t* small = reinterpret_cast<uint32_t*>(base + o_small);
   uint32_t* large = reinterpret_cast<uint32_t*>(base + o_large);
+  double* small_val = reinterpret_cast<double*>(base + o_small_val);
+  double* large_val = reinterpret_cast<double*>(base + o_large_val);
   uint32_t* alias = reinterpret_cast<uint32_t*>(base + o_alias);
   float* prob = reinterpret_cast<float*>(base + o_prob);
   uint32_t* counts = reinterpret_cast<uint32_t*>(base + o_counts);
@@ -330,8 +382,8 @@ Status build_env_sampler_gpu(Context* ctx, const float* d_rgba, uint32_t width, 
     k_env_total<<<1, 64, 0, st>>>(w, un, info, total);
     k_env_split<<<nb, kBlock, 0, st>>>(w, scaled, total, un, info, counts);
     k_env_offsets<<<1, 1024, 0, st>>>(counts, nb, info);
-    k_env_scatter<<<nb, kBlock, 0, st>>>(scaled, un, counts, info, small, large);
-    k_env_sweep<<<1, 64, 0, st>>>(small, large, scaled, info, prob, alias);
+    k_env_scatter<<<nb, kBlock, 0, st>>>(scaled, un, counts, info, small, large, small_val, large_val);
+    k_env_sweep<<<1, 64, 0, st>>>(small, large, small_val, large_val, info, prob, alias);
     k_env_pack<<<nb, kBlock, 0, st>>>(prob, alias, w, un, info, reinterpret_cast<float4*>(d_table));
     e = hipGetLastError();
   }
