@@ -340,7 +340,8 @@ def bench_denoise(args) -> dict:
     return {"metric": "denoise ms @1920x1080, 8 spp, 5 a-trous iterations", "value": round(ms, 4), "unit": "ms",
             "higher_is_better": False,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_pixel": b_px},
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_pixel": b_px,
+                         "traffic": profiled_traffic("denoise_1920x1080")},
             "cpu_baseline": cpu}
 
 
