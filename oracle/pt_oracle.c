@@ -23,12 +23,18 @@
  *
  * Parity pins: the RNG against the published PCG hash constants and the
  * reference seeding formula; the BSDF against closed forms (GGX
- * normalisation, white furnace of a Lambert surface, R + T = 1 for the
- * dielectric sampler); alias tables against the reference's known answers
- * (test/environment.test.ts:16-44, 112-136); the sheen table against its
- * Monte-Carlo test (test/environment.test.ts:160-183).  The full
- * integrator's converged radiance is otherwise "parity unpinned": the
- * reference shader cannot run in this image.
+ * normalisation, the Lambert value at normal incidence, the dielectric
+ * sampler's reflection frequency = Schlick) and the sampler's histogram
+ * against the evaluated pdf; alias tables against the reference's known
+ * answers (test/environment.test.ts:16-44, 112-136); the sheen table against
+ * its Monte-Carlo test (test/environment.test.ts:160-183); the integrator
+ * (or_trace_pixels) against what the mathematics fixes
+ * (tests/test_oracle_integrator.py): an empty scene and a directly viewed
+ * emitter exactly, a lone convex sphere's directional albedo under a
+ * constant environment (NEE, MIS and BSDF sampling together) against an
+ * independent quadrature, a Lambert plane under a quad light against the
+ * rectangle form factor, the environment pdf integrated over the sphere,
+ * and alpha-masked hits against brute force.
  */
 #include <math.h>
 #include <stdint.h>

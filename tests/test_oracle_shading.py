@@ -222,10 +222,11 @@ def test_bsdf_sampler_consistent_with_evaluation(kw):
     assert np.allclose(est, quad, rtol=0.04, atol=2e-3), (est, quad)
 
 
-def test_lambert_white_furnace_closed_form():
-    # A pure diffuse lobe (F0 = 0 via ior = 1) with albedo 1: ∫ f cos = 1 - ∫ F...; with
-    # ior 1 the Schlick F0 is 0, so the albedo is 1 minus a Fresnel tail; at normal
-    # incidence the specular lobe vanishes with F0 = 0 except the (1-cos)^5 term.
+def test_lambert_value_at_normal_incidence():
+    # A point value, not a furnace integral (the directional-albedo integrals
+    # are pinned in test_oracle_integrator.py / closed_forms.py): with ior 1
+    # the Schlick F0 is 0, so at wi = wo = n the specular lobe vanishes and
+    # the diffuse lobe of albedo 1 is exactly 1/pi.
     m = _mat(albedo=(1.0, 1.0, 1.0), roughness=1.0, ior=1.0)
     wo = np.array([0.0, 0.0, 1.0])
     f, pdf = O.eval_opaque(m, wo, np.array([0.0, 0.0, 1.0]))
