@@ -852,18 +852,42 @@ lucent_status lucent_env_sampler_gpu(lucent_ctx* ctx, const float* d_rgba, uint3
                                gpu_ms);
 }
 
-lucent_status lucent_scene_rebuild_env_sampler(lucent_ctx* ctx, lucent_scene* scene, double* total_out) {
+lucent_status lucent_scene_rebuild_env_sampler(lucent_ctx* ctx, lucent_scene* scene, uint32_t on_device,
+                                               double* total_out) {
   clear_error();
   if (!ctx || !scene) return fail(LUCENT_E_INVALID, "null ctx/scene");
   PtScene& pt = scene->pt;
   if (!scene->lighting_set || !pt.env_map || !pt.env_table) return fail(LUCENT_E_INVALID, "scene has no lighting");
+  const uint32_t w = uint32_t(pt.env_w), h = uint32_t(pt.env_h);
   uint32_t gw = 0, gh = 0;
   double total = 0;
-  // Writes the scene's own table in place (same grid: the map keeps its size);
-  // on failure the table is left as it was.
-  Status s = build_env_sampler_gpu(ctx, pt.env_map, uint32_t(pt.env_w), uint32_t(pt.env_h), 1024,
-                                   const_cast<float*>(pt.env_table), &gw, &gh, &total, ctx->stream, nullptr);
-  if (s != LUCENT_OK) return s;
+  Status s;
+  if (on_device) {
+    // Writes the scene's own table in place (same grid: the map keeps its
+    // size); on failure the table is left as it was.
+    s = build_env_sampler_gpu(ctx, pt.env_map, w, h, 1024, const_cast<float*>(pt.env_table), &gw, &gh, &total,
+                              ctx->stream, nullptr);
+    if (s != LUCENT_OK) return s;
+  } else {
+    // The map comes back to the host, the host build runs, the table goes up.
+    std::vector<float> map4(size_t(w) * h * 4), rgb(size_t(w) * h * 3);
+    if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+    if ((s = dev_download(map4.data(), pt.env_map, map4.size() * sizeof(float), ctx->stream)) != LUCENT_OK) return s;
+    if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+    for (size_t i = 0; i < size_t(w) * h; ++i)
+      for (int c = 0; c < 3; ++c) rgb[i * 3 + c] = map4[i * 4 + c];
+    EnvSampler es;
+    if ((s = build_env_sampler(rgb.data(), w, h, 1024, &es)) != LUCENT_OK) return s;
+    gw = es.grid_w;
+    gh = es.grid_h;
+    total = es.total;
+    if (int32_t(gw) == pt.grid_w && int32_t(gh) == pt.grid_h) {
+      if ((s = dev_upload(const_cast<float*>(pt.env_table), es.table.data(), es.table.size() * sizeof(float),
+                          ctx->stream)) != LUCENT_OK)
+        return s;
+      if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+    }
+  }
   if (int32_t(gw) != pt.grid_w || int32_t(gh) != pt.grid_h) return fail(LUCENT_E_INVALID, "sampler grid changed");
   pt.env_sampling = total > 0 ? 1 : 0;
   if (total_out) *total_out = total;

@@ -58,15 +58,16 @@ class DiffRenderer:
         self.materials = list(materials)
         self._version += 1
 
-    def set_env_texels(self, rgb, rebuild_sampler: bool = False) -> None:
+    def set_env_texels(self, rgb, rebuild_sampler: bool = False, on_device: bool = False) -> None:
         """New environment texels (same size).  The importance sampler stays
-        the one built with the lighting unless `rebuild_sampler` (rebuilt on
-        the device from the new texels)."""
+        the one built with the lighting unless `rebuild_sampler`: rebuilt
+        from the new texels by the host build or, with `on_device`, by the
+        FP64 device build (the same table either way)."""
         arr = np.ascontiguousarray(np.asarray(rgb, np.float32).reshape(-1))
         _native.check(_native.lib().lucent_scene_update_env_texels(self.ctx.handle, self.renderer.device_scene.handle,
                                                                    arr.ctypes.data, *self.env_size))
         if rebuild_sampler:
-            self.renderer.device_scene.rebuild_env_sampler()
+            self.renderer.device_scene.rebuild_env_sampler(on_device)
         self.env = arr.copy()
         self._version += 1
 

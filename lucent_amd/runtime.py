@@ -171,12 +171,15 @@ class Scene:
         _native.check(_native.lib().lucent_scene_stats_get(self.handle, C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_}
 
-    def rebuild_env_sampler(self) -> float:
-        """lucent_scene_rebuild_env_sampler: the importance sampler rebuilt on
-        the device from the scene's current environment texels; returns the
-        luminance-weighted total (0 switches light sampling off)."""
+    def rebuild_env_sampler(self, on_device: bool = False) -> float:
+        """lucent_scene_rebuild_env_sampler: the importance sampler rebuilt
+        from the scene's current environment texels, by the host build (the
+        map copied back) or, with on_device, by the FP64 device build (same
+        table bits); returns the luminance-weighted total (0 switches light
+        sampling off)."""
         total = C.c_double()
-        _native.check(_native.lib().lucent_scene_rebuild_env_sampler(self.ctx.handle, self.handle, C.byref(total)))
+        _native.check(_native.lib().lucent_scene_rebuild_env_sampler(self.ctx.handle, self.handle, int(on_device),
+                                                                     C.byref(total)))
         return total.value
 
     def destroy(self) -> None:

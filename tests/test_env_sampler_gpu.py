@@ -4,8 +4,9 @@ oracle's buildEnvSampler restatement): grid, total and every table texel
 bit-identical on maps that exercise box averaging, uneven boxes, tall maps,
 a single texel, black maps (uniform fallback), one dominant texel (a long
 Vose chain) and zero rows; the host's error on a bad texel with the output
-left untouched; and a scene whose sampler is rebuilt on the device after a
-texel update renders bit-identically to one set up with the new map."""
+left untouched; and a scene whose sampler is rebuilt (by the host build or
+on the device) after a texel update renders bit-identically to one set up
+with the new map."""
 import ctypes as C
 from dataclasses import replace
 
@@ -101,7 +102,8 @@ def test_bad_texel_fails_like_the_host_and_leaves_the_table(ctx):
     assert bool((table == 7.0).all())
 
 
-def test_scene_rebuild_after_texel_update_matches_a_fresh_scene(ctx):
+@pytest.mark.parametrize("on_device", [False, True])
+def test_scene_rebuild_after_texel_update_matches_a_fresh_scene(ctx, on_device):
     sc = workloads.spheres_scene()
     ew, eh, env = sc.env.map
     env = np.asarray(env, np.float32).reshape(eh, ew, 3)
@@ -113,7 +115,7 @@ def test_scene_rebuild_after_texel_update_matches_a_fresh_scene(ctx):
     stale_before = d.forward(spp, 0, record=False)
     d.set_env_texels(new, rebuild_sampler=False)
     stale = d.forward(spp, 0, record=False)
-    d.set_env_texels(new, rebuild_sampler=True)
+    d.set_env_texels(new, rebuild_sampler=True, on_device=on_device)
     rebuilt = d.forward(spp, 0, record=False)
     fresh_scene = replace(sc, env=replace(sc.env, map=(ew, eh, new.reshape(-1))))
     f = DiffRenderer(ctx, fresh_scene, w, h, depth)
