@@ -38,6 +38,10 @@ constexpr int kTile = 8;
 constexpr int kNumParams = 5;        // albedo r, g, b, roughness, metallic
 constexpr int kLdsMaterials = 64;    // materials whose gradients are pre-reduced in LDS
 constexpr int kEnvReplicas = 8;
+#ifndef LUCENT_ADJ_STACK
+#define LUCENT_ADJ_STACK 8
+#endif
+constexpr int kAdjStack = LUCENT_ADJ_STACK;  // LDS entries of the replay's traversal stack ring
 
 // Forward-mode dual number with the five material-parameter tangents.
 struct D5 {
@@ -202,6 +206,23 @@ __device__ __forceinline__ ParamScale param_scale(const PtScene& sc, const Hit& 
 
 // Gradient sums are FP64 throughout (LDS and global double atomics): a
 // persistent launch sums every sample of the frame into them.
+//
+// Environment texels go through a per-block cache in LDS first.  The
+// environment sampler draws most light samples from a few bright texels,
+// and every addition to global memory is a memory-side atomic (the MI355X
+// executes device atomics beyond the XCD's L2), so many lanes queued on the
+// same few addresses.  The block sums additions to up to kEnvSlots texels
+// in LDS (direct-mapped by a hash of the texel index; a slot is claimed by
+// the first texel that reaches it) and adds each claimed slot to its global
+// replica once, after the block's last path; a texel whose slot another
+// texel holds goes to global memory directly.
+#ifndef LUCENT_ADJ_ENV_SLOTS_LOG2
+#define LUCENT_ADJ_ENV_SLOTS_LOG2 7
+#endif
+constexpr int kEnvSlots = 1 << LUCENT_ADJ_ENV_SLOTS_LOG2;
+constexpr uint32_t kFreeSlot = 0xFFFFFFFFu;
+__device__ inline uint32_t env_slot(uint32_t texel) { return (texel * 2654435761u) >> (32 - LUCENT_ADJ_ENV_SLOTS_LOG2); }
+
 struct GradSink {
   double* lds;  // kLdsMaterials * 5 (block-shared)
   double* env;  // kEnvReplicas replicas of env_w * env_h * 3
@@ -209,6 +230,8 @@ struct GradSink {
   uint32_t replica;
   int material_count;
   double* global_mat;  // materials beyond the LDS table
+  uint32_t* env_tag;   // kEnvSlots texel indices (block-shared LDS)
+  double* env_val;     // kEnvSlots * 3 sums (block-shared LDS)
 };
 
 __device__ __forceinline__ void add_material(GradSink& g, int mat, const float v[kNumParams]) {
@@ -232,9 +255,29 @@ __device__ __forceinline__ void add_env(const PtScene& sc, GradSink& g, float u,
   for (int k = 0; k < 4; ++k) {
     if (w[k] == 0.0f) continue;
     const float s = w[k] * sc.env_intensity;
-    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 0], double(weight.x * s));
-    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 1], double(weight.y * s));
-    unsafeAtomicAdd(&base[size_t(idx[k]) * 3 + 2], double(weight.z * s));
+    const double v[3] = {double(weight.x * s), double(weight.y * s), double(weight.z * s)};
+    const uint32_t texel = uint32_t(idx[k]), slot = env_slot(texel);
+    uint32_t owner = g.env_tag[slot];
+    if (owner == kFreeSlot) {
+      owner = atomicCAS(&g.env_tag[slot], kFreeSlot, texel);
+      if (owner == kFreeSlot) owner = texel;
+    }
+    if (owner == texel) {
+      for (int c = 0; c < 3; ++c) atomicAdd(&g.env_val[slot * 3 + c], v[c]);
+    } else {
+      for (int c = 0; c < 3; ++c) unsafeAtomicAdd(&base[size_t(texel) * 3 + c], v[c]);
+    }
+  }
+}
+
+// Adds the block's cached texel sums to its replica (after the block's
+// last path, the block synchronised).
+__device__ inline void flush_env(GradSink& g, int lane, int lanes) {
+  double* base = g.env + g.replica * g.env_stride;
+  for (int slot = lane; slot < kEnvSlots; slot += lanes) {
+    const uint32_t texel = g.env_tag[slot];
+    if (texel == kFreeSlot) continue;
+    for (int c = 0; c < 3; ++c) unsafeAtomicAdd(&base[size_t(texel) * 3 + c], g.env_val[slot * 3 + c]);
   }
 }
 
@@ -566,12 +609,24 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWave
     uint32_t tile_w, uint32_t tile_h) {
   const PtScene& sc = view->sc;
   const PtFrame& f = view->f;
-  __shared__ uint32_t lds[kPtStackWords * kAdjBlock];
+  // LDS per block: the stack ring (kAdjStack entries, 2 words each), the
+  // material table and the texel cache: 10,240 B, so that 16 blocks of one
+  // wave fit a CU's 160 KiB at 4 waves per SIMD.  (Measured: 128 slots and
+  // an 8-entry ring beat 64 slots and 10 entries, and 32 and 10.)
+  __shared__ uint32_t lds[2 * kAdjStack * kAdjBlock];
   __shared__ double lds_grad[kLdsMaterials * kNumParams];
+  __shared__ uint32_t env_tag[kEnvSlots];
+  __shared__ double env_val[kEnvSlots * 3];
   for (int i = threadIdx.x; i < kLdsMaterials * kNumParams; i += kAdjBlock) lds_grad[i] = 0.0;
+  for (int i = threadIdx.x; i < kEnvSlots; i += kAdjBlock) {
+    env_tag[i] = kFreeSlot;
+    env_val[i * 3] = env_val[i * 3 + 1] = env_val[i * 3 + 2] = 0.0;
+  }
   __syncthreads();
-  PtStack<kAdjBlock> st{as_lds(lds + threadIdx.x), spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth, 0, 0};
-  GradSink g{lds_grad, env_acc, env_stride, blockIdx.x % kEnvReplicas, int(sc.material_count), mat_overflow};
+  Stack4<kAdjStack, kAdjBlock> st{as_lds(lds + threadIdx.x), spill + size_t(blockIdx.x * kAdjBlock + threadIdx.x) * spill_depth,
+                                  0, 0};
+  GradSink g{lds_grad, env_acc, env_stride, blockIdx.x % kEnvReplicas, int(sc.material_count), mat_overflow,
+             env_tag, env_val};
   const uint32_t lane = threadIdx.x;
   const size_t frame_px = size_t(f.width) * f.height;
   auto take = [&]() {
@@ -651,6 +706,7 @@ __global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(kWave
   const int nm = min(int(sc.material_count), kLdsMaterials);
   for (int i = threadIdx.x; i < nm * kNumParams; i += kAdjBlock)
     mat_partials[size_t(blockIdx.x) * kLdsMaterials * kNumParams + i] = lds_grad[i];
+  flush_env(g, int(threadIdx.x), kAdjBlock);
 }
 
 // FP64 reductions: per-block material partials, then the env replicas.
