@@ -516,10 +516,10 @@ def bench_batch(args) -> dict:
 
     settings = RenderSettings(max_bounces=workloads.BATCH_DEPTH, firefly_clamp=0.0)
     from lucent_amd.renderer import autotune
-    tune = autotune(ctx, jobs, settings, trial_samples=1)
-    log(f"autotuned {tune} (1 spp of every scene)")
+    tune = autotune(ctx, jobs, settings, trial_samples=8)
+    log(f"autotuned {tune} (up to 8 spp of every scene, tiles sized for the real spp)")
 
-    def run(mode):
+    def run(mode, plain_settings=None):
         reset_all()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t = time.perf_counter()
@@ -528,8 +528,7 @@ def bench_batch(args) -> dict:
             trace_batch(ctx, jobs, settings, stream=sp)
         else:
             for r, n in jobs:
-                r.settings.per_sample_launch = mode == "per_sample"
-                r.settings.occupancy = settings.occupancy
+                r.settings = plain_settings
                 r.trace_samples(n, stream=sp)
         e1.record(stream)
         torch.cuda.synchronize()
@@ -552,15 +551,34 @@ def bench_batch(args) -> dict:
         return {"metric": "samples/s, 64-scene batch, one persistent launch (plain paths not run)",
                 "value": round(paths / tb_wall, 1), "unit": "samples/s", "batched_s": round(tb_wall, 3),
                 "batched_gpu_s": round(tb_gpu, 3), "autotune": tune, "paths": paths, "setup_s": round(setup_s, 1)}
+    def plain(**kw):
+        return RenderSettings(max_bounces=workloads.BATCH_DEPTH, firefly_clamp=0.0, **kw)
+
     log("plain: one launch per sample per scene")
-    ps_wall, ps_gpu = run("per_sample")
+    ps_wall, ps_gpu = run("per_sample", plain(per_sample_launch=True, occupancy=settings.occupancy))
     ref_digest = films_digest()
     log(f"per-sample launches {ps_wall:.2f} s; plain: one lucent_trace_samples launch per scene")
-    run("per_scene")
-    pl = [run("per_scene") for _ in range(2)]
-    same_plain = films_digest() == ref_digest
-    pl_wall, pl_gpu = min(pl)
-    log(f"one launch per scene {pl_wall:.2f} s; batched warm-up")
+    # One launch per scene under several launch shapes: the batch's autotuned
+    # shape, the library default (tiles sized for 256 paths per job) and
+    # uniform tiles; the plain path is credited with its fastest.
+    plain_shapes = {
+        "batch_autotuned": plain(occupancy=settings.occupancy, tile_w=settings.tile_w, tile_h=settings.tile_h,
+                                 job_paths=settings.job_paths, shade_batch=settings.shade_batch,
+                                 trace_turns=settings.trace_turns),
+        "default": plain(),
+        "tiles_2x2": plain(tile_w=2, tile_h=2),
+        "tiles_8x8": plain(tile_w=8, tile_h=8),
+    }
+    plain_s, same_plain = {}, True
+    for name, ps in plain_shapes.items():
+        run("per_scene", ps)
+        pl = [run("per_scene", ps) for _ in range(2)]
+        same_plain = same_plain and films_digest() == ref_digest
+        plain_s[name] = min(pl)[0]
+        log(f"one launch per scene, {name}: {plain_s[name]:.3f} s")
+    best_plain = min(plain_s, key=plain_s.get)
+    pl_wall = plain_s[best_plain]
+    log("batched warm-up")
     run("batched")
     log("batched: timing")
     bt = [run("batched") for _ in range(max(1, min(args.steps, 3)))]
@@ -590,6 +608,9 @@ def bench_batch(args) -> dict:
             "value": round(paths / tb_wall, 1), "unit": "samples/s",
             "plain_one_launch_per_scene_samples_per_s": round(paths / pl_wall, 1),
             "speedup_vs_one_launch_per_scene": round(pl_wall / tb_wall, 3),
+            "plain_fastest_shape": best_plain,
+            "one_launch_per_scene_s_by_shape": {k: round(v, 3) for k, v in plain_s.items()},
+            "speedup_vs_one_launch_per_scene_by_shape": {k: round(v / tb_wall, 3) for k, v in plain_s.items()},
             "plain_per_sample_launches_samples_per_s": round(paths / ps_wall, 1),
             "speedup_vs_per_sample_launches": round(ps_wall / tb_wall, 3),
             "batched_s": round(tb_wall, 3), "batched_gpu_s": round(tb_gpu, 3),

@@ -344,8 +344,10 @@ typedef struct lucent_camera {
 } lucent_camera;
 
 /* Launch structure of lucent_trace_samples / lucent_trace_batch.  Both give
- * bit-identical films.  Default: persistent waves run all samples of an 8x8
- * tile, starting a pixel's next sample as soon as its path ends.
+ * bit-identical films.  Default: persistent waves take tile jobs (all
+ * samples of a tile of pixels, sized by the settings' launch shape) and hand
+ * their pixel samples to their lanes, starting a new path as soon as one
+ * ends.
  * LUCENT_TRACE_PER_SAMPLE: one launch over the film per sample, as the
  * reference's traceSamples draws one full-screen pass per sample. */
 #define LUCENT_TRACE_PER_SAMPLE 0x1u
@@ -373,10 +375,15 @@ typedef struct lucent_render_settings {
    * a job is a tile_w x tile_h tile of pixels (at most 64) with all its
    * samples; a wave shades once shade_batch of its 64 lanes wait and runs up
    * to trace_turns traversal steps between its checks for shading and new
-   * work.  0 = 8, 8, 32, 8.  Results do not depend on them
-   * (lucent_autotune_launch picks them with the occupancy).  lucent_backward
-   * hands out its replays by tiles of tile_w x tile_h too. */
+   * work.  0 = 32, 8 for the last two.  tile_w = tile_h = 0 sizes each
+   * film's tile to its sample count: the largest of 8x8, 8x4, 4x4, 4x2,
+   * 2x2, 2x1, 1x1 whose pixels x samples stay within job_paths (0 = 256),
+   * at least one pixel; a 0 beside a non-zero side is 8.  Results do not
+   * depend on them (lucent_autotune_launch picks them with the occupancy).
+   * lucent_backward hands out its replays by tiles of tile_w x tile_h
+   * (0 = 8) and ignores job_paths. */
   uint32_t tile_w, tile_h, shade_batch, trace_turns;
+  uint32_t job_paths;
 } lucent_render_settings;
 
 lucent_status lucent_trace_samples(lucent_ctx *ctx, lucent_scene *scene, lucent_film *film,
@@ -403,8 +410,9 @@ lucent_status lucent_trace_reprojected(lucent_ctx *ctx, lucent_scene *scene, luc
 /* ------------------------------------------------------------------------
  * Batched multi-scene tracing: for every item, `samples` samples of every
  * pixel of its film (seeds continue from the film's seed, history advances by
- * `samples`), all items in ONE persistent launch whose waves pull 8x8 tile
- * jobs from a shared queue, longest jobs first.  The result for each film is
+ * `samples`), all items in ONE persistent launch whose waves pull tile
+ * jobs (each film's tile sized by the launch shape) from a shared queue,
+ * longest jobs first.  The result for each film is
  * bit-identical to lucent_trace_samples(scene, film, camera, settings,
  * samples).  A film may appear once per batch.  No counterpart in the
  * reference, which renders one scene per Renderer with one draw per sample
@@ -434,14 +442,16 @@ lucent_status lucent_autotune_tiles(lucent_ctx *ctx, const lucent_batch_item *it
                                     uint32_t *occupancy_out, float *ms_out);
 
 /* The launch autotuner: searches the persistent launch's register variant
- * (occupancy 2..6), then its job tile (8x8, 8x4, 4x4, 4x2, 2x2), shade
+ * (occupancy 2..6), then its job tile (sized per film for 128, 256, 512
+ * or 1024 paths per job, or one of 8x8, 8x4, 4x4, 4x2, 2x2 for all), shade
  * batch (16, 24, 32, 40, 48 lanes) and traversal turns per check (4, 8,
  * 12, 16), one parameter at a time with the others at their best so far,
  * each candidate timed as lucent_autotune_tiles times a variant (best of two
  * runs of min(trial_samples, samples) samples of every item after an
- * untimed round, in scratch films).  A candidate replaces the current choice
- * only when more than 1 % faster.  Writes the choice into *settings
- * (occupancy, tile_w, tile_h, shade_batch, trace_turns; its other fields
+ * untimed round, in scratch films; sized tiles are chosen for `samples`).
+ * A candidate replaces the current choice only when more than 1 % faster.
+ * Writes the choice into *settings (occupancy, tile_w, tile_h,
+ * shade_batch, trace_turns, job_paths; its other fields
  * are read: flags, depth, clamp, waves_per_cu) and, if trials is not NULL,
  * up to trials_cap timed candidates in order; *n_trials receives their
  * number.  Every choice gives identical films.  No counterpart in the
@@ -449,6 +459,7 @@ lucent_status lucent_autotune_tiles(lucent_ctx *ctx, const lucent_batch_item *it
 typedef struct lucent_autotune_trial {
   uint32_t occupancy, tile_w, tile_h, shade_batch, trace_turns;
   float ms;
+  uint32_t job_paths;
 } lucent_autotune_trial;
 lucent_status lucent_autotune_launch(lucent_ctx *ctx, const lucent_batch_item *items, uint32_t count,
                                      lucent_render_settings *settings, uint32_t trial_samples,

@@ -100,12 +100,14 @@ TRACE_INLINE_SHADOWS = 0x4
 class RenderSettingsStruct(C.Structure):
     _fields_ = [("max_bounces", C.c_uint32), ("firefly_clamp", C.c_float), ("flags", C.c_uint32),
                 ("waves_per_cu", C.c_uint32), ("occupancy", C.c_uint32), ("tile_w", C.c_uint32),
-                ("tile_h", C.c_uint32), ("shade_batch", C.c_uint32), ("trace_turns", C.c_uint32)]
+                ("tile_h", C.c_uint32), ("shade_batch", C.c_uint32), ("trace_turns", C.c_uint32),
+                ("job_paths", C.c_uint32)]
 
 
 class AutotuneTrialStruct(C.Structure):
     _fields_ = [("occupancy", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32),
-                ("shade_batch", C.c_uint32), ("trace_turns", C.c_uint32), ("ms", C.c_float)]
+                ("shade_batch", C.c_uint32), ("trace_turns", C.c_uint32), ("ms", C.c_float),
+                ("job_paths", C.c_uint32)]
 
 
 class BatchItemStruct(C.Structure):

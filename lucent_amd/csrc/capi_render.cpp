@@ -407,6 +407,7 @@ lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_
   cfg.tile_h = settings->tile_h;
   cfg.shade_batch = settings->shade_batch;
   cfg.trace_turns = settings->trace_turns;
+  cfg.job_paths = settings->job_paths;
   if ((s = launch_trace_tiles(ctx, &it, 1, cfg, st)) != LUCENT_OK) return s;
   film->seed += count;
   film->samples += count;
@@ -439,12 +440,24 @@ lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items
   }
   bool fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS);
   for (uint32_t i = 0; i < count; ++i) fused = fused && !items[i].scene->blend_alpha;
-  // Longest jobs first: a tile job costs about samples x (log of the scene size).
+  TileLaunch cfg;
+  cfg.fused = fused;
+  cfg.waves_per_cu = settings->waves_per_cu;
+  cfg.occupancy = settings->occupancy;
+  cfg.tile_w = settings->tile_w;
+  cfg.tile_h = settings->tile_h;
+  cfg.shade_batch = settings->shade_batch;
+  cfg.trace_turns = settings->trace_turns;
+  cfg.job_paths = settings->job_paths;
+  // Longest jobs first: a job costs about its paths (tile pixels x samples)
+  // x (log of the scene size).
   std::vector<uint32_t> order(count);
   for (uint32_t i = 0; i < count; ++i) order[i] = i;
   auto job_cost = [&](uint32_t i) {
     const double tris = double(items[i].scene->bvh.tri_count) + 2.0;
-    return double(items[i].samples) * std::log2(tris);
+    uint32_t tw = 0, th = 0;
+    launch_job_tile(cfg, items[i].samples, &tw, &th);
+    return double(tw) * th * items[i].samples * std::log2(tris);
   };
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return job_cost(a) > job_cost(b); });
   for (uint32_t i : order) {
@@ -458,14 +471,6 @@ lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items
     it.spp = b.samples;
     tiles.push_back(it);
   }
-  TileLaunch cfg;
-  cfg.fused = fused;
-  cfg.waves_per_cu = settings->waves_per_cu;
-  cfg.occupancy = settings->occupancy;
-  cfg.tile_w = settings->tile_w;
-  cfg.tile_h = settings->tile_h;
-  cfg.shade_batch = settings->shade_batch;
-  cfg.trace_turns = settings->trace_turns;
   Status s = launch_trace_tiles(ctx, tiles.data(), uint32_t(tiles.size()), cfg, st);
   if (s != LUCENT_OK) return s;
   for (uint32_t i = 0; i < count; ++i) {
@@ -503,6 +508,7 @@ lucent_status lucent_autotune_tiles(lucent_ctx* ctx, const lucent_batch_item* it
     it.frame.seed = items[i].film->seed;
     it.film = films[i]->f;
     it.spp = std::max(1u, std::min(trial_samples, items[i].samples ? items[i].samples : trial_samples));
+    it.job_spp = items[i].samples;  // sized tiles as for the real sample count
     tiles.push_back(it);
   }
   if (s == LUCENT_OK) {
@@ -584,6 +590,7 @@ lucent_status lucent_autotune_launch(lucent_ctx* ctx, const lucent_batch_item* i
     it.frame.seed = items[i].film->seed;
     it.film = films[i]->f;
     it.spp = std::max(1u, std::min(trial_samples, items[i].samples ? items[i].samples : trial_samples));
+    it.job_spp = items[i].samples;  // sized tiles as for the real sample count
     tiles.push_back(it);
   }
   if (s == LUCENT_OK) {
@@ -610,7 +617,8 @@ lucent_status lucent_autotune_launch(lucent_ctx* ctx, const lucent_batch_item* i
   best.fused = fused;
   best.waves_per_cu = settings->waves_per_cu;
   best.occupancy = 4;
-  best.tile_w = best.tile_h = 8;
+  best.tile_w = best.tile_h = 0;  // sized per film
+  best.job_paths = kJobPaths;
   best.shade_batch = 32;
   best.trace_turns = 8;
   float best_ms = 0.0f;
@@ -624,8 +632,8 @@ lucent_status lucent_autotune_launch(lucent_ctx* ctx, const lucent_batch_item* i
     if ((r = run(cfg, &b)) != LUCENT_OK) return r;
     *out = std::min(a, b);
     if (trials && logged < trials_cap)
-      trials[logged] = lucent_autotune_trial{cfg.occupancy, cfg.tile_w, cfg.tile_h, cfg.shade_batch, cfg.trace_turns,
-                                             *out};
+      trials[logged] = lucent_autotune_trial{cfg.occupancy, cfg.tile_w, cfg.tile_h, cfg.shade_batch,
+                                             cfg.trace_turns, *out, cfg.job_paths};
     ++logged;
     return LUCENT_OK;
   };
@@ -642,8 +650,11 @@ lucent_status lucent_autotune_launch(lucent_ctx* ctx, const lucent_batch_item* i
       best_ms = ms;
     }
   }
-  // Stages 1-3: the hand-out shape, one parameter at a time.
-  const uint32_t tiles_wh[][2] = {{8, 8}, {8, 4}, {4, 4}, {4, 2}, {2, 2}};
+  // Stages 1-3: the hand-out shape, one parameter at a time.  Tiles: one
+  // shape for every film, or (0 x 0) each film's tile sized to its sample
+  // count for a number of paths per job.
+  const uint32_t tiles_wh[][3] = {{0, 0, 128}, {0, 0, 256}, {0, 0, 512}, {0, 0, 1024}, {8, 8, 0},
+                                  {8, 4, 0},   {4, 4, 0},   {4, 2, 0},   {2, 2, 0}};
   const uint32_t batches[] = {16, 24, 32, 40, 48};
   const uint32_t turns[] = {4, 8, 12, 16};
   auto consider = [&](const TileLaunch& c) -> Status {
@@ -655,13 +666,14 @@ lucent_status lucent_autotune_launch(lucent_ctx* ctx, const lucent_batch_item* i
     }
     return r;
   };
-  const uint32_t tile0[2] = {best.tile_w, best.tile_h};
+  const uint32_t tile0[3] = {best.tile_w, best.tile_h, best.job_paths};
   for (const auto& wh : tiles_wh) {
     if (s != LUCENT_OK) break;
-    if (wh[0] == tile0[0] && wh[1] == tile0[1]) continue;  // timed already
+    if (wh[0] == tile0[0] && wh[1] == tile0[1] && wh[2] == tile0[2]) continue;  // timed already
     TileLaunch c = best;
     c.tile_w = wh[0];
     c.tile_h = wh[1];
+    c.job_paths = wh[2];
     s = consider(c);
   }
   const uint32_t batch0 = best.shade_batch;
@@ -689,6 +701,7 @@ lucent_status lucent_autotune_launch(lucent_ctx* ctx, const lucent_batch_item* i
   settings->tile_h = best.tile_h;
   settings->shade_batch = best.shade_batch;
   settings->trace_turns = best.trace_turns;
+  settings->job_paths = best.job_paths;
   if (n_trials) *n_trials = std::min(logged, trials ? trials_cap : 0u);
   return LUCENT_OK;
 }
@@ -740,6 +753,7 @@ lucent_status lucent_forward(lucent_ctx* ctx, lucent_scene* scene, lucent_film* 
   cfg.tile_h = settings->tile_h;
   cfg.shade_batch = settings->shade_batch;
   cfg.trace_turns = settings->trace_turns;
+  cfg.job_paths = settings->job_paths;
   if ((s = launch_trace_tiles(ctx, &it, 1, cfg, stream ? stream : ctx->stream)) != LUCENT_OK) return s;
   film->seed += spp;
   film->samples += spp;
@@ -1071,6 +1085,7 @@ lucent_status lucent_frame_graph_create(lucent_ctx* ctx, lucent_scene* scene, lu
   cfg.tile_h = settings->tile_h;
   cfg.shade_batch = settings->shade_batch;
   cfg.trace_turns = settings->trace_turns;
+  cfg.job_paths = settings->job_paths;
   DenoiseArgs da;
   if (denoise && (s = denoise_args(film, cam, denoise, film->samples + samples, &da)) != LUCENT_OK) return s;
   DisplayArgs disp;

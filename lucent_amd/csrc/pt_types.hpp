@@ -107,6 +107,10 @@ struct TileItem {
   // films are bit-identical for every choice (the autotuner's search space).
   uint32_t tile_w = 8, tile_h = 8;
   uint32_t shade_batch = 32, trace_turns = 8;
+  // Host only: the sample count a sized job tile is chosen for (0: spp).
+  // A timing trial that traces fewer samples than the real launch sets it
+  // to the real count, so that both use the same tiles.
+  uint32_t job_spp = 0;
   // Optional: each path's radiance as tracePath returns it, float4 per pixel
   // per sample, sample-major (sample k of the launch at k * width * height).
   float* sample_radiance = nullptr;

@@ -44,8 +44,11 @@ struct TileLaunch {
   uint32_t waves_per_cu = 0;  // persistent waves per CU (0: 4 x occupancy)
   uint32_t occupancy = 0;     // register-allocation target in waves per SIMD: 2..6 (0: 4)
   // Per-sample hand-out shape (TileItem): tile of a job, shading batch and
-  // traversal turns per check; 0 = 8 x 8, 32, 8.
+  // traversal turns per check; 0 = 32, 8 for the last two.  A tile of 0 x 0
+  // is sized per film to its sample count (launch_job_tile): about
+  // job_paths paths per job (0 = kJobPaths).
   uint32_t tile_w = 0, tile_h = 0, shade_batch = 0, trace_turns = 0;
+  uint32_t job_paths = 0;
   // Optional caller-owned job counter, film descriptor slot and spill buffer
   // (one film only); otherwise the context's buffers are used.
   uint32_t* counter = nullptr;
@@ -66,6 +69,13 @@ size_t tile_spill_bytes(const Context* ctx, const TileItem& item, const TileLaun
 // *p += v on the device (a graph node).
 Status launch_add_u32(uint32_t* p, uint32_t v, void* stream);
 uint32_t tile_jobs(uint32_t width, uint32_t height);
+// Paths per job a sized job tile aims at when the launch names none.
+constexpr uint32_t kJobPaths = 256;
+// Job tile of a film traced with `spp` samples under `cfg`: its explicit
+// tile_w x tile_h (a 0 beside a non-zero side is 8), or for 0 x 0 the
+// largest tile of the sequence 8x8, 8x4, 4x4, 4x2, 2x2, 2x1, 1x1 whose
+// pixels times spp stay within job_paths (at least one pixel).
+void launch_job_tile(const TileLaunch& cfg, uint32_t spp, uint32_t* tile_w, uint32_t* tile_h);
 // Checks a launch shape (tile of at most 64 pixels, shade batch 1..64,
 // trace turns 1..64; 0 = default) and writes it into `item`.
 Status apply_launch_shape(const TileLaunch& cfg, TileItem* item);

@@ -38,12 +38,14 @@ class RenderSettings:
     occupancy: int = 0
     wavefront: bool = False
     inline_shadows: bool = False
-    # Shape of the persistent per-sample hand-out (0 = 8 x 8 tiles, shade
-    # batch 32, 8 traversal turns per check); autotune() picks them.
+    # Shape of the persistent per-sample hand-out (0 = each film's tile sized
+    # for job_paths paths per job (0 = 256), shade batch 32, 8 traversal
+    # turns per check); autotune() picks them.
     tile_w: int = 0
     tile_h: int = 0
     shade_batch: int = 0
     trace_turns: int = 0
+    job_paths: int = 0
     # Presentation part (renderer.ts:42-84 defaults).
     temporal: bool = True
     denoise: bool = True
@@ -62,7 +64,7 @@ class RenderSettings:
                 (_native.TRACE_INLINE_SHADOWS if self.inline_shadows else 0)
         return _native.RenderSettingsStruct(self.max_bounces, self.firefly_clamp, flags, self.waves_per_cu,
                                             self.occupancy, self.tile_w, self.tile_h, self.shade_batch,
-                                            self.trace_turns)
+                                            self.trace_turns, self.job_paths)
 
 
 def lighting_struct(scene: Scene, keep: list) -> _native.LightingStruct:
@@ -457,7 +459,9 @@ def autotune(ctx: Context, jobs, settings: RenderSettings, trial_samples: int = 
     """lucent_autotune_launch: times the persistent launch's occupancy
     variants, then its job tile, shade batch and traversal turns, on `jobs`
     (a list of (Renderer, samples)), and sets them in `settings`.  Returns
-    {"occupancy", "tile", "shade_batch", "trace_turns", "trials": [...]}."""
+    {"occupancy", "tile", "job_paths", "shade_batch", "trace_turns",
+    "trials": [...]}; a tile of [0, 0] is sized per film for job_paths paths
+    per job."""
     st = settings.struct()
     trials = (_native.AutotuneTrialStruct * 64)()
     n = C.c_uint32()
@@ -465,10 +469,12 @@ def autotune(ctx: Context, jobs, settings: RenderSettings, trial_samples: int = 
                                                        trial_samples, trials, 64, C.byref(n)))
     settings.occupancy, settings.tile_w, settings.tile_h = st.occupancy, st.tile_w, st.tile_h
     settings.shade_batch, settings.trace_turns = st.shade_batch, st.trace_turns
-    return {"occupancy": st.occupancy, "tile": [st.tile_w, st.tile_h], "shade_batch": st.shade_batch,
-            "trace_turns": st.trace_turns,
-            "trials": [{"occupancy": t.occupancy, "tile": [t.tile_w, t.tile_h], "shade_batch": t.shade_batch,
-                        "trace_turns": t.trace_turns, "ms": round(t.ms, 3)} for t in trials[:n.value]]}
+    settings.job_paths = st.job_paths
+    return {"occupancy": st.occupancy, "tile": [st.tile_w, st.tile_h], "job_paths": st.job_paths,
+            "shade_batch": st.shade_batch, "trace_turns": st.trace_turns,
+            "trials": [{"occupancy": t.occupancy, "tile": [t.tile_w, t.tile_h], "job_paths": t.job_paths,
+                        "shade_batch": t.shade_batch, "trace_turns": t.trace_turns, "ms": round(t.ms, 3)}
+                       for t in trials[:n.value]]}
 
 
 def autotune_occupancy(ctx: Context, jobs, settings: RenderSettings, trial_samples: int = 2) -> dict:

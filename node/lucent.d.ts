@@ -39,7 +39,9 @@ export declare class LucentRenderer {
   settings: { maxBounces: number; fireflyClamp: number; temporal: boolean; denoise: boolean; exposure: number;
     tonemap: 'aces' | 'reinhard' | 'none';
     /* launch structure (every one gives the same film): */
-    perSampleLaunch?: boolean; wavefront?: boolean; inlineShadows?: boolean; occupancy?: number; wavesPerCU?: number };
+    perSampleLaunch?: boolean; wavefront?: boolean; inlineShadows?: boolean; occupancy?: number; wavesPerCU?: number;
+    /* per-sample hand-out shape (0 = default; tile 0 x 0 sizes each film's jobs for jobPaths paths): */
+    tileWidth?: number; tileHeight?: number; shadeBatch?: number; traceTurns?: number; jobPaths?: number };
   loadScene(packed: object, scene: object): void;
   uploadLighting(scene: object): void;
   uploadCamera(cam: Camera, moving?: boolean): void;

@@ -638,6 +638,7 @@ static void read_settings(napi_env env, napi_value o, lucent_render_settings *rs
   rs->tile_h = (uint32_t)num(env, prop(env, o, "tileHeight"), 0);
   rs->shade_batch = (uint32_t)num(env, prop(env, o, "shadeBatch"), 0);
   rs->trace_turns = (uint32_t)num(env, prop(env, o, "traceTurns"), 0);
+  rs->job_paths = (uint32_t)num(env, prop(env, o, "jobPaths"), 0);
 }
 
 /* traceReprojected(ctx, scene, film, camera, prevFilm, prevCamera, historyCap, settings):

@@ -228,12 +228,18 @@ def test_launch_autotuner_searches_the_shape_and_leaves_films_alone(ctx):
     st = RenderSettings(8, 0.0)
     res = autotune(ctx, [(r, 8)], st, trial_samples=2)
     trials = res["trials"]
-    assert len(trials) == 5 + 4 + 4 + 3
+    # occupancy 2..6; tiles: sized for 128/512/1024 paths (256 is the start)
+    # and five uniform shapes; shade batches; traversal turns.
+    assert len(trials) == 5 + 8 + 4 + 3
     assert [t["occupancy"] for t in trials[:5]] == [2, 3, 4, 5, 6]
+    assert all(t["tile"] == [0, 0] and t["job_paths"] == 256 for t in trials[:5])
+    assert [(*t["tile"], t["job_paths"]) for t in trials[5:13]] == [
+        (0, 0, 128), (0, 0, 512), (0, 0, 1024), (8, 8, 0), (8, 4, 0), (4, 4, 0), (4, 2, 0), (2, 2, 0)]
     assert all(t["ms"] > 0 for t in trials)
-    chosen = (st.occupancy, st.tile_w, st.tile_h, st.shade_batch, st.trace_turns)
-    assert chosen in [(t["occupancy"], *t["tile"], t["shade_batch"], t["trace_turns"]) for t in trials]
-    print("autotuned", res["occupancy"], res["tile"], res["shade_batch"], res["trace_turns"])
+    chosen = (st.occupancy, st.tile_w, st.tile_h, st.job_paths, st.shade_batch, st.trace_turns)
+    assert chosen in [(t["occupancy"], *t["tile"], t["job_paths"], t["shade_batch"], t["trace_turns"])
+                      for t in trials]
+    print("autotuned", res["occupancy"], res["tile"], res["job_paths"], res["shade_batch"], res["trace_turns"])
     after = r.film.read()
     assert r.film.info() == info
     for k in before:
@@ -241,8 +247,13 @@ def test_launch_autotuner_searches_the_shape_and_leaves_films_alone(ctx):
     r.close()
 
 
+# The last four size each film's tile to its sample count (5 and 3 samples
+# in the batch below: job_paths 64 gives 4x2 and 4x4 tiles, 1 single
+# pixels, 4096 8x8 tiles, the default 256 paths 8x4 and 8x8 tiles).
 SHAPES = [dict(tile_w=4, tile_h=4), dict(tile_w=2, tile_h=2), dict(tile_w=8, tile_h=2, shade_batch=16),
-          dict(shade_batch=48, trace_turns=3), dict(tile_w=1, tile_h=1, shade_batch=1, trace_turns=1)]
+          dict(shade_batch=48, trace_turns=3), dict(tile_w=1, tile_h=1, shade_batch=1, trace_turns=1),
+          dict(tile_w=0, tile_h=4), dict(job_paths=64), dict(job_paths=1, shade_batch=8), dict(job_paths=4096),
+          dict()]
 
 
 @pytest.mark.parametrize("shape", range(len(SHAPES)))
