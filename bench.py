@@ -671,7 +671,10 @@ def bench_env_sampler(args) -> dict:
     the host build of the same tables: the path-trace HDRI (1024x512) and
     the differentiable-rendering map.  Device time between events around the
     seven kernels (median of 9 builds); host time of lucent_env_sampler from
-    the same texels; tables compared bit for bit."""
+    the same texels; tables compared bit for bit.  Beside them the parallel
+    device build (FP64 tree sum and prefix-sum alias pairing: the same
+    distribution, other pairs), its table checked by the probability with
+    which it selects each cell."""
     import ctypes as C
 
     import torch
@@ -703,10 +706,21 @@ def bench_env_sampler(args) -> dict:
             host_ms.append((time.perf_counter() - t0) * 1e3)
         same = bool(np.array_equal(table.cpu().numpy().reshape(-1).view(np.uint32), host_table.view(np.uint32))
                     and total == htotal.value)
+        par_ms = []
+        for _ in range(10):
+            ptable, _, _, _, ms = ctx.env_sampler_gpu(d_map, parallel=True)
+            par_ms.append(ms)
+        pt = ptable.cpu().numpy().astype(np.float64).reshape(-1, 4)
+        implied = pt[:, 0].copy()
+        np.add.at(implied, pt[:, 1].astype(np.int64), 1.0 - pt[:, 0])
+        target = host_table.reshape(-1, 4)[:, 2].astype(np.float64)
         res[name] = {"grid": [gw, gh], "device_ms": round(float(np.median(dev_ms[1:])), 4),
-                     "host_ms": round(float(np.median(host_ms)), 4), "bit_identical": same}
+                     "host_ms": round(float(np.median(host_ms)), 4), "bit_identical": same,
+                     "parallel_device_ms": round(float(np.median(par_ms[1:])), 4),
+                     "parallel_max_rel_error_of_selection_probability":
+                         float(np.max(np.abs(implied - target) / np.maximum(target, 1e-30)))}
         log(f"env sampler {name}: device {res[name]['device_ms']} ms, host {res[name]['host_ms']} ms, "
-            f"identical {same}")
+            f"identical {same}; parallel device build {res[name]['parallel_device_ms']} ms")
     ctx.close()
     return res
 

@@ -293,6 +293,24 @@ lucent_status lucent_env_sampler_gpu(lucent_ctx *ctx, const float *d_rgba, uint3
                                      uint32_t max_width, float *d_table_out, uint32_t *grid_w, uint32_t *grid_h,
                                      double *total, float *gpu_ms, void *stream);
 
+/* lucent_env_sampler_gpu with both sequential chains replaced by parallel
+ * FP64 work: the weights' total is a tree sum, and the alias pairs come from
+ * inclusive FP64 prefix sums of the small cells' deficits (1 - pmf n) and
+ * the large cells' excesses (pmf n - 1), each cell's pair found by binary
+ * search (a small cell takes the first large cell whose excess prefix
+ * reaches its deficit prefix; a large cell keeps 1 minus its overdraw and
+ * aliases the next large cell).  Same grid and the same distribution as
+ * buildEnvSampler (each cell's implied probability is its pmf up to the
+ * float32 rounding of the stored probabilities), other alias pairs, so a
+ * random number may select another cell than the reference's table does;
+ * the total may differ from the sequential sum in its last bits.  For
+ * callers that rebuild the sampler often (a map under optimisation).  Same
+ * arguments, errors and synchronisation as lucent_env_sampler_gpu.  No
+ * counterpart in the reference. */
+lucent_status lucent_env_sampler_gpu_parallel(lucent_ctx *ctx, const float *d_rgba, uint32_t width, uint32_t height,
+                                              uint32_t max_width, float *d_table_out, uint32_t *grid_w,
+                                              uint32_t *grid_h, double *total, float *gpu_ms, void *stream);
+
 /* buildAliasTable (src/math/alias.ts:17-64), host-only. */
 lucent_status lucent_alias_table(const double *weights, uint32_t n, float *prob_out, uint32_t *alias_out,
                                  double *pmf_out, double *total_out);
@@ -675,7 +693,8 @@ lucent_status lucent_scene_update_env_texels(lucent_ctx *ctx, lucent_scene *scen
  * uploads the table: 7 ms for a 1024x512 map.  on_device = 1 runs
  * lucent_env_sampler_gpu on the map in HBM, same table bits, no host
  * involvement, but bound by its sequential FP64 chains (112 ms at
- * 1024x512).  Synchronous on the context's stream; traces on other streams
+ * 1024x512).  on_device = 2 runs lucent_env_sampler_gpu_parallel: the same
+ * distribution with other alias pairs, in parallel.  Synchronous on the context's stream; traces on other streams
  * must not overlap it.  total_out may be NULL.  The reference rebuilds on
  * the CPU when a map is uploaded (renderer.ts:389-406). */
 lucent_status lucent_scene_rebuild_env_sampler(lucent_ctx *ctx, lucent_scene *scene, uint32_t on_device,

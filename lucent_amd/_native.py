@@ -155,6 +155,8 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_alias_table": (_I32, [_P, _U32, _P, _P, _P, C.POINTER(_D)]),
     "lucent_env_sampler_gpu": (_I32, [_P, _P, _U32, _U32, _U32, _P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_D),
                                       C.POINTER(C.c_float), _P]),
+    "lucent_env_sampler_gpu_parallel": (_I32, [_P, _P, _U32, _U32, _U32, _P, C.POINTER(_U32), C.POINTER(_U32),
+                                               C.POINTER(_D), C.POINTER(C.c_float), _P]),
     "lucent_scene_rebuild_env_sampler": (_I32, [_P, _P, _U32, C.POINTER(_D)]),
     "lucent_sheen_lut": (_I32, [_U32, _U32, _P]),
     "lucent_film_create": (_I32, [_P, _U32, _U32, C.POINTER(_P)]),

@@ -866,6 +866,15 @@ lucent_status lucent_env_sampler_gpu(lucent_ctx* ctx, const float* d_rgba, uint3
                                gpu_ms);
 }
 
+lucent_status lucent_env_sampler_gpu_parallel(lucent_ctx* ctx, const float* d_rgba, uint32_t width, uint32_t height,
+                                              uint32_t max_width, float* d_table_out, uint32_t* grid_w,
+                                              uint32_t* grid_h, double* total, float* gpu_ms, void* stream) {
+  clear_error();
+  if (!ctx) return fail(LUCENT_E_INVALID, "null ctx");
+  return build_env_sampler_gpu(ctx, d_rgba, width, height, max_width, d_table_out, grid_w, grid_h, total, stream,
+                               gpu_ms, true);
+}
+
 lucent_status lucent_scene_rebuild_env_sampler(lucent_ctx* ctx, lucent_scene* scene, uint32_t on_device,
                                                double* total_out) {
   clear_error();
@@ -876,11 +885,12 @@ lucent_status lucent_scene_rebuild_env_sampler(lucent_ctx* ctx, lucent_scene* sc
   uint32_t gw = 0, gh = 0;
   double total = 0;
   Status s;
+  if (on_device > 2) return fail(LUCENT_E_INVALID, "on_device must be 0, 1 or 2");
   if (on_device) {
     // Writes the scene's own table in place (same grid: the map keeps its
     // size); on failure the table is left as it was.
     s = build_env_sampler_gpu(ctx, pt.env_map, w, h, 1024, const_cast<float*>(pt.env_table), &gw, &gh, &total,
-                              ctx->stream, nullptr);
+                              ctx->stream, nullptr, on_device == 2);
     if (s != LUCENT_OK) return s;
   } else {
     // The map comes back to the host, the host build runs, the table goes up.

@@ -25,11 +25,21 @@
 // gives a valid but different table, which would pick a different cell than
 // the reference's sampler for the same random numbers.
 //
+// The parallel build (for callers that rebuild the sampler often, e.g. after
+// every gradient step on the map) replaces both chains: the total is a tree
+// sum, and the pairing is decided per cell by binary searches over FP64
+// prefix sums of the small cells' deficits and the large cells' excesses
+// (k_env_gather, k_env_pair).  Its table samples the same distribution
+// (every cell's implied probability is its pmf up to the float32 rounding
+// of the stored probabilities) with other alias pairs.
+//
 // Every kernel after k_env_cells returns at once when a weight was negative
 // or not finite, so a failed build leaves the output table untouched.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <hipcub/hipcub.hpp>
 #include <vector>
 
 #include "render_host.hpp"
@@ -308,6 +318,96 @@ __global__ __launch_bounds__(64) void k_env_sweep(const uint32_t* __restrict__ s
   }
 }
 
+// ---------------------------------------------------------------------------
+// The parallel pairing.  Let the small cells s_1 < s_2 < ... (scaled < 1)
+// have deficits d_k = 1 - scaled and the large cells l_1 < l_2 < ... have
+// excesses e_j = scaled - 1, with inclusive prefix sums D_k and E_j.  A
+// sweep that fills the small cells in order from the large cells in order,
+// a large cell turning small (filled by the next large cell) once its
+// remaining weight drops below 1, makes these choices, which prefix sums
+// decide independently of one another:
+//   small s_k:  prob = scaled, alias = l_j for the first j with E_j >= D_{k-1}
+//               (the large cell still holding weight >= 1 when s_k's turn
+//               comes);
+//   large l_j:  prob = 1 - (D_k - E_j) for the first k with D_k > E_j (the
+//               small cell whose fill takes l_j below 1), alias = l_{j+1};
+//               prob 1 if no such k or no next large cell.
+// Every cell's own share plus the shares aliased to it then add up to its
+// scaled weight (exactly, in exact arithmetic), so the table samples the
+// same distribution as Vose's, with other pairs.
+
+// Deficit (small cells) or excess (large cells) of a cell, 0 otherwise.
+struct Deficit {
+  __host__ __device__ double operator()(double sc) const { return sc < 1 ? 1.0 - sc : 0.0; }
+};
+struct Excess {
+  __host__ __device__ double operator()(double sc) const { return sc < 1 ? 0.0 : sc - 1.0; }
+};
+
+// dl[k] = D_k over the small list, el[j] = E_j over the large list, gathered
+// from the per-cell inclusive scans.
+__global__ __launch_bounds__(kBlock) void k_env_gather(const uint32_t* __restrict__ small,
+                                                       const uint32_t* __restrict__ large,
+                                                       const double* __restrict__ dcell, const double* __restrict__ ecell,
+                                                       const uint32_t* __restrict__ info, double* __restrict__ dl,
+                                                       double* __restrict__ el) {
+  if (build_failed(info)) return;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < info[1]) dl[i] = dcell[small[i]];
+  if (i < info[2]) el[i] = ecell[large[i]];
+}
+
+// First position p in [0, n) with a[p] >= x (kStrict: a[p] > x); n if none.
+template <bool kStrict>
+__device__ inline uint32_t first_reaching(const double* __restrict__ a, uint32_t n, double x) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    const bool reach = kStrict ? a[mid] > x : a[mid] >= x;
+    if (reach)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// Thread i pairs small cell i and large cell i (where they exist).
+__global__ __launch_bounds__(kBlock) void k_env_pair(const uint32_t* __restrict__ small,
+                                                     const uint32_t* __restrict__ large,
+                                                     const double* __restrict__ small_val,
+                                                     const double* __restrict__ dl, const double* __restrict__ el,
+                                                     const uint32_t* __restrict__ info, float* __restrict__ prob,
+                                                     uint32_t* __restrict__ alias) {
+  if (build_failed(info)) return;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t a = info[1], b = info[2];
+  if (i < a) {
+    const uint32_t s = small[i];
+    const double before = i ? dl[i - 1] : 0.0;
+    const uint32_t j = first_reaching<false>(el, b, before);
+    if (j < b) {
+      prob[s] = float(small_val[i]);
+      alias[s] = large[j];
+    } else {  // beyond the large cells' excess (rounding): its own bucket
+      prob[s] = 1.0f;
+      alias[s] = s;
+    }
+  }
+  if (i < b) {
+    const uint32_t l = large[i];
+    const uint32_t k = first_reaching<true>(dl, a, el[i]);
+    if (k < a && i + 1 < b) {
+      const double keep = 1.0 - (dl[k] - el[i]);
+      prob[l] = float(fmin(1.0, fmax(0.0, keep)));
+      alias[l] = large[i + 1];
+    } else {
+      prob[l] = 1.0f;
+      alias[l] = l;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_env_pack(const float* __restrict__ prob, const uint32_t* __restrict__ alias,
                                                      const double* __restrict__ pmf, uint32_t n,
                                                      const uint32_t* __restrict__ info, float4* __restrict__ table) {
@@ -324,7 +424,7 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 Status build_env_sampler_gpu(Context* ctx, const float* d_rgba, uint32_t width, uint32_t height, uint32_t max_width,
                              float* d_table, uint32_t* grid_w, uint32_t* grid_h, double* total_out, void* stream,
-                             float* ms_out) {
+                             float* ms_out, bool parallel) {
   if (!ctx || !d_rgba || !d_table || width == 0 || height == 0) return fail(LUCENT_E_INVALID, "empty environment map");
   if (reinterpret_cast<uintptr_t>(d_rgba) % 16 || reinterpret_cast<uintptr_t>(d_table) % 16)
     return fail(LUCENT_E_INVALID, "environment map and table must be 16-byte aligned device arrays");
@@ -346,6 +446,25 @@ Status build_env_sampler_gpu(Context* ctx, const float* d_rgba, uint32_t width, 
   const size_t o_small_val = take(8 * n), o_large_val = take(8 * n);
   const size_t o_alias = take(4 * n), o_prob = take(4 * n), o_counts = take(8 * size_t(nb));
   const size_t o_info = take(16), o_total = take(8), o_sin = take(8 * size_t(gh));
+  // The parallel pairing's prefix sums: per cell, then per list; and the
+  // library's temporary storage for the sum and the scans.
+  size_t o_dcell = 0, o_ecell = 0, o_dl = 0, o_el = 0, o_temp = 0, temp_bytes = 0;
+  if (parallel) {
+    o_dcell = take(8 * n);
+    o_ecell = take(8 * n);
+    o_dl = take(8 * n);
+    o_el = take(8 * n);
+    size_t a = 0, b = 0;
+    hipError_t qe = hipcub::DeviceReduce::Sum(nullptr, a, static_cast<const double*>(nullptr),
+                                              static_cast<double*>(nullptr), int(n));
+    if (qe == hipSuccess)
+      qe = hipcub::DeviceScan::InclusiveSum(
+          nullptr, b, hipcub::TransformInputIterator<double, Deficit, const double*>(nullptr, Deficit{}),
+          static_cast<double*>(nullptr), int(n));
+    if (qe != hipSuccess) return fail_hip(qe, "environment sampler scan set-up");
+    temp_bytes = std::max(a, b);
+    o_temp = take(temp_bytes);
+  }
   Status s = ensure_scratch(ctx, off);
   if (s != LUCENT_OK) return s;
   char* base = static_cast<char*>(ctx->scratch);
@@ -379,13 +498,41 @@ Status build_env_sampler_gpu(Context* ctx, const float* d_rgba, uint32_t width, 
   if (e == hipSuccess) {
     k_env_cells<<<nb, kBlock, 0, st>>>(reinterpret_cast<const float4*>(d_rgba), width, height, gw, gh, row_sin, w,
                                         info);
-    k_env_total<<<1, 64, 0, st>>>(w, un, info, total);
-    k_env_split<<<nb, kBlock, 0, st>>>(w, scaled, total, un, info, counts);
-    k_env_offsets<<<1, 1024, 0, st>>>(counts, nb, info);
-    k_env_scatter<<<nb, kBlock, 0, st>>>(scaled, un, counts, info, small, large, small_val, large_val);
-    k_env_sweep<<<1, 64, 0, st>>>(small, large, small_val, large_val, info, prob, alias);
-    k_env_pack<<<nb, kBlock, 0, st>>>(prob, alias, w, un, info, reinterpret_cast<float4*>(d_table));
-    e = hipGetLastError();
+    void* temp = base + o_temp;
+    if (parallel)
+      e = hipcub::DeviceReduce::Sum(temp, temp_bytes, static_cast<const double*>(w), total, int(n), st);
+    else
+      k_env_total<<<1, 64, 0, st>>>(w, un, info, total);
+    if (e == hipSuccess) {
+      k_env_split<<<nb, kBlock, 0, st>>>(w, scaled, total, un, info, counts);
+      k_env_offsets<<<1, 1024, 0, st>>>(counts, nb, info);
+      k_env_scatter<<<nb, kBlock, 0, st>>>(scaled, un, counts, info, small, large, small_val, large_val);
+    }
+    if (e == hipSuccess && parallel) {
+      double* dcell = reinterpret_cast<double*>(base + o_dcell);
+      double* ecell = reinterpret_cast<double*>(base + o_ecell);
+      double* dl = reinterpret_cast<double*>(base + o_dl);
+      double* el = reinterpret_cast<double*>(base + o_el);
+      size_t tb = temp_bytes;
+      e = hipcub::DeviceScan::InclusiveSum(
+          temp, tb, hipcub::TransformInputIterator<double, Deficit, const double*>(scaled, Deficit{}), dcell, int(n),
+          st);
+      tb = temp_bytes;
+      if (e == hipSuccess)
+        e = hipcub::DeviceScan::InclusiveSum(
+            temp, tb, hipcub::TransformInputIterator<double, Excess, const double*>(scaled, Excess{}), ecell, int(n),
+            st);
+      if (e == hipSuccess) {
+        k_env_gather<<<nb, kBlock, 0, st>>>(small, large, dcell, ecell, info, dl, el);
+        k_env_pair<<<nb, kBlock, 0, st>>>(small, large, small_val, dl, el, info, prob, alias);
+      }
+    } else if (e == hipSuccess) {
+      k_env_sweep<<<1, 64, 0, st>>>(small, large, small_val, large_val, info, prob, alias);
+    }
+    if (e == hipSuccess) {
+      k_env_pack<<<nb, kBlock, 0, st>>>(prob, alias, w, un, info, reinterpret_cast<float4*>(d_table));
+      e = hipGetLastError();
+    }
   }
   if (ms_out && e == hipSuccess) e = hipEventRecord(ev[1], st);
   uint32_t info_out[4] = {0, 0, 0, 0};

@@ -24,10 +24,12 @@ void env_grid(uint32_t width, uint32_t height, uint32_t max_width, uint32_t* gw,
 double env_row_sine(uint32_t cy, uint32_t gh);
 // buildEnvSampler on the device from an RGBA float map already in device
 // memory; the table (4 floats per cell) is written to d_table.  Bit-identical
-// to build_env_sampler on the same texels.  Synchronous on `stream`.
+// to build_env_sampler on the same texels, or with `parallel` the alias
+// pairs from FP64 prefix sums (the same distribution, other pairs).
+// Synchronous on `stream`.
 Status build_env_sampler_gpu(Context* ctx, const float* d_rgba, uint32_t width, uint32_t height, uint32_t max_width,
                              float* d_table, uint32_t* grid_w, uint32_t* grid_h, double* total, void* stream,
-                             float* ms_out);
+                             float* ms_out, bool parallel = false);
 void bake_sky(const lucent_sky& sky, const lucent_sun& sun, uint32_t width, std::vector<float>* out);
 double sheen_albedo(double cos_v, double roughness, uint32_t steps);
 void sheen_lut(uint32_t size, uint32_t steps, float* out);

@@ -58,11 +58,12 @@ class DiffRenderer:
         self.materials = list(materials)
         self._version += 1
 
-    def set_env_texels(self, rgb, rebuild_sampler: bool = False, on_device: bool = False) -> None:
+    def set_env_texels(self, rgb, rebuild_sampler: bool = False, on_device: int = 0) -> None:
         """New environment texels (same size).  The importance sampler stays
         the one built with the lighting unless `rebuild_sampler`: rebuilt
-        from the new texels by the host build or, with `on_device`, by the
-        FP64 device build (the same table either way)."""
+        from the new texels by the host build (on_device 0) or the FP64
+        device build (1; the same table either way), or by the parallel FP64
+        device build (2; the same distribution, other alias pairs)."""
         arr = np.ascontiguousarray(np.asarray(rgb, np.float32).reshape(-1))
         _native.check(_native.lib().lucent_scene_update_env_texels(self.ctx.handle, self.renderer.device_scene.handle,
                                                                    arr.ctypes.data, *self.env_size))

@@ -62,11 +62,12 @@ class Context:
         launch may use (0: the default, 4.5 GiB)."""
         _native.check(_native.lib().lucent_set_record_budget(self.handle, nbytes))
 
-    def env_sampler_gpu(self, rgba, max_width: int = 1024):
+    def env_sampler_gpu(self, rgba, max_width: int = 1024, parallel: bool = False):
         """lucent_env_sampler_gpu: buildEnvSampler in float64 on the device
         from `rgba`, a (height, width, 4) float32 torch tensor on this
-        context's device.  Returns (table (cells, 4) float32 tensor, grid_w,
-        grid_h, total, device milliseconds)."""
+        context's device (with `parallel`, lucent_env_sampler_gpu_parallel:
+        the alias pairs from FP64 prefix sums).  Returns (table (cells, 4)
+        float32 tensor, grid_w, grid_h, total, device milliseconds)."""
         import torch
         if rgba.dtype != torch.float32 or rgba.dim() != 3 or rgba.shape[2] != 4 or not rgba.is_contiguous():
             raise ValueError("rgba must be a contiguous (height, width, 4) float32 tensor")
@@ -74,9 +75,9 @@ class Context:
         gw, gh, total, ms = C.c_uint32(), C.c_uint32(), C.c_double(), C.c_float()
         cells = min(w, max_width) * h  # upper bound on the grid
         table = torch.empty((max(cells, 1), 4), dtype=torch.float32, device=rgba.device)
-        _native.check(_native.lib().lucent_env_sampler_gpu(
-            self.handle, C.c_void_p(rgba.data_ptr()), w, h, max_width, C.c_void_p(table.data_ptr()), C.byref(gw),
-            C.byref(gh), C.byref(total), C.byref(ms), None))
+        fn = _native.lib().lucent_env_sampler_gpu_parallel if parallel else _native.lib().lucent_env_sampler_gpu
+        _native.check(fn(self.handle, C.c_void_p(rgba.data_ptr()), w, h, max_width, C.c_void_p(table.data_ptr()),
+                         C.byref(gw), C.byref(gh), C.byref(total), C.byref(ms), None))
         return table[:gw.value * gh.value], gw.value, gh.value, total.value, ms.value
 
     def close(self) -> None:
@@ -171,12 +172,13 @@ class Scene:
         _native.check(_native.lib().lucent_scene_stats_get(self.handle, C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_}
 
-    def rebuild_env_sampler(self, on_device: bool = False) -> float:
+    def rebuild_env_sampler(self, on_device: int = 0) -> float:
         """lucent_scene_rebuild_env_sampler: the importance sampler rebuilt
         from the scene's current environment texels, by the host build (the
-        map copied back) or, with on_device, by the FP64 device build (same
-        table bits); returns the luminance-weighted total (0 switches light
-        sampling off)."""
+        map copied back; on_device 0), the FP64 device build (same table
+        bits; 1) or the parallel FP64 device build (same distribution, other
+        alias pairs; 2); returns the luminance-weighted total (0 switches
+        light sampling off)."""
         total = C.c_double()
         _native.check(_native.lib().lucent_scene_rebuild_env_sampler(self.ctx.handle, self.handle, int(on_device),
                                                                      C.byref(total)))
