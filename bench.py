@@ -365,7 +365,7 @@ def bench_adjoint(args) -> dict:
     sc = workloads.spheres_scene()
     d = DiffRenderer(ctx, sc, W, H, workloads.DIFF_DEPTH)
     from lucent_amd.renderer import autotune
-    tune = autotune(ctx, [(d.renderer, SPP)], d.renderer.settings, trial_samples=2)
+    tune = autotune(ctx, [(d.renderer, SPP)], d.renderer.settings, trial_samples=SPP)
     log(f"forward occupancy {tune['occupancy']}")
     base_mats, base_env = list(d.materials), d.env.copy()
     pm, penv = workloads.perturb_params(base_mats, base_env, seed=5)
