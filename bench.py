@@ -52,10 +52,10 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline_closest_hit(bvh, positions, origins, dirs, sample: int) -> dict:
+def cpu_baseline_closest_hit(bvh, positions, origins, dirs, sample: int, workers: int = 0) -> dict:
     """Times the JavaScript port of intersectBVH (oracle/js) with Node worker_threads on a
     contiguous sample of the benchmark rays; checks its hit ids against the GPU's."""
-    workers = int(os.environ.get("LUCENT_CPU_WORKERS", "0")) or min(16, os.cpu_count() or 1)
+    workers = workers or int(os.environ.get("LUCENT_CPU_WORKERS", "0")) or min(16, os.cpu_count() or 1)
     with tempfile.TemporaryDirectory() as d:
         bvh.nodes.tofile(f"{d}/nodes.f32")
         bvh.tri_indices.tofile(f"{d}/tri_indices.u32")
@@ -147,6 +147,25 @@ def bench_closest_hit(args) -> dict:
                           f"(Node {res['node']}, worker_threads={res['workers']}, host cpus={res['cpus']}), "
                           f"median of 5 after 1 warm-up; hit ids agree with GPU on {agree:.6f} of rays"),
                "median_s": round(res["median_s"], 4)}
+        # The same port on one worker, over a smaller contiguous sample.
+        one_n = min(sample, 1 << 16)
+        one = cpu_baseline_closest_hit(bvh, pos, org, dirs, one_n, workers=1)
+        cpu["one_worker"] = {"value": round(one["mrays_per_s"], 4), "unit": "Mrays/s", "cores": 1,
+                             "sample": f"first {one_n} rays, median of 5 after 1 warm-up",
+                             "ids_agree_with_gpu": float((one.pop("ids") == ids[:one_n]).mean())}
+        # Secondary figure: the FP64 C oracle (intersectBVH restated in C) on host threads.
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        O.intersect_bvh_batch(bvh, pos, org[:4096], dirs[:4096], threads=threads)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, otri, _, _ = O.intersect_bvh_batch(bvh, pos, org[:sample], dirs[:sample], threads=threads)
+            times.append(time.perf_counter() - t0)
+        cpu["c_oracle"] = {"value": round(sample / float(np.median(times)) / 1e6, 4), "unit": "Mrays/s",
+                           "cores": threads, "kind": "port",
+                           "sample": f"first {sample} rays, FP64 C oracle on {threads} threads, median of 3",
+                           "ids_agree_with_gpu": float((otri == ids[:sample]).mean())}
     ctx_info = ctx.device_info()
     scene.destroy()
     ctx.close()
