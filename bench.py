@@ -799,6 +799,42 @@ def bench_frame_graph(args) -> dict:
         out["speedup"] = round(out["eager"]["ms_per_frame"] / out["graph"]["ms_per_frame"], 3)
         out["film_bit_identical"] = same
         res[f"{W}x{H}"] = out
+    # A batch of 16 small films (four scenes, 160x120, 1 spp each) per
+    # iteration: eager lucent_trace_batch (job table uploaded, host
+    # synchronised per call) against a replayed batch graph.
+    from lucent_amd.renderer import BatchGraph, trace_batch
+    scenes = [workloads.knot_scene(), workloads.spheres_scene()]
+    settings = RenderSettings(8, 0.0)
+    out = {}
+    for mode in ("eager", "graph"):
+        rs = []
+        for i in range(16):
+            r = Renderer(ctx, 160, 120, settings)
+            r.load_scene(scenes[i % len(scenes)])
+            r.film.set_seed(i)
+            r.reset()
+            rs.append(r)
+        jobs = [(r, 1) for r in rs]
+        g = BatchGraph(ctx, jobs, settings) if mode == "graph" else None
+        step = g.launch if g is not None else (lambda: trace_batch(ctx, jobs, settings))
+        for _ in range(5):
+            step()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(N):
+            step()
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / N
+        out[mode] = {"ms_per_iteration": round(dt * 1e3, 4), "iterations_per_s": round(1 / dt, 1)}
+        out[mode + "_films"] = [r.film.read()["color"] for r in rs]
+        if g is not None:
+            g.close()
+        for r in rs:
+            r.close()
+    same = all(np.array_equal(a, b) for a, b in zip(out.pop("eager_films"), out.pop("graph_films")))
+    out["speedup"] = round(out["eager"]["ms_per_iteration"] / out["graph"]["ms_per_iteration"], 3)
+    out["films_bit_identical"] = same
+    res["batch_16x160x120_1spp"] = out
     ctx.close()
     return {"metric": "interactive frames/s (1 spp + denoise + display), graph replay", "value":
             res["128x96"]["graph"]["frames_per_s"], "unit": "frames/s @128x96", **res}

@@ -628,6 +628,31 @@ lucent_status lucent_frame_graph_launch(lucent_frame_graph *graph, void *stream)
 void lucent_frame_graph_destroy(lucent_frame_graph *graph);
 
 /* ------------------------------------------------------------------------
+ * Batch graphs: one lucent_trace_batch over `items` (each film `samples`
+ * more samples, all films in one persistent launch) captured once into a
+ * HIP graph, with its own job table, record rings and spill area in device
+ * memory, and replayed by lucent_batch_graph_launch with one launch call:
+ * no table upload and no host synchronisation per iteration.  Each film's
+ * seed continues on the device (the graph advances a device copy per film),
+ * so n replays leave every film bit-identical to n lucent_trace_batch calls
+ * (and to lucent_trace_samples per film).  Cameras and settings are fixed at
+ * capture; the graph holds the scenes' and films' device addresses:
+ * re-create it after a camera move, a change of lighting, textures or
+ * materials, or when any of them is destroyed.  A replay fails if a film
+ * was advanced by other calls since the last replay.  Settings with
+ * LUCENT_TRACE_PER_SAMPLE or LUCENT_TRACE_WAVEFRONT are refused.  No
+ * counterpart in the reference (one scene per Renderer, one draw per
+ * sample, renderer.ts:279, 702-737).
+ * ---------------------------------------------------------------------- */
+typedef struct lucent_batch_graph lucent_batch_graph;
+
+lucent_status lucent_batch_graph_create(lucent_ctx *ctx, const lucent_batch_item *items, uint32_t count,
+                                        const lucent_render_settings *settings, lucent_batch_graph **out);
+/* Enqueues one replay on `stream` (NULL: the context's stream). */
+lucent_status lucent_batch_graph_launch(lucent_batch_graph *graph, void *stream);
+void lucent_batch_graph_destroy(lucent_batch_graph *graph);
+
+/* ------------------------------------------------------------------------
  * Differentiable rendering.  The forward pass is lucent_trace_samples; the
  * backward pass replays each of the `spp` paths per pixel (seeds seed0 ..
  * seed0+spp-1, the seeds the forward used) from its RNG seed and returns the

@@ -181,6 +181,9 @@ SIGNATURES: dict[str, tuple] = {
                                          C.POINTER(DenoiseParamsStruct), C.POINTER(DisplayParamsStruct), _U32, _U32,
                                          _P, C.POINTER(_P)]),
     "lucent_frame_graph_launch": (_I32, [_P, _P]),
+    "lucent_batch_graph_create": (_I32, [_P, _P, _U32, C.POINTER(RenderSettingsStruct), C.POINTER(_P)]),
+    "lucent_batch_graph_launch": (_I32, [_P, _P]),
+    "lucent_batch_graph_destroy": (None, [_P]),
     "lucent_decode_hdr": (_I32, [_P, C.c_size_t, C.POINTER(_U32), C.POINTER(_U32), _P, C.c_size_t]),
     "lucent_gltf_parse": (_I32, [_P, C.c_size_t, C.POINTER(_P)]),
     "lucent_gltf_resource_count": (_U32, [_P]),

@@ -414,13 +414,15 @@ lucent_status lucent_trace_samples(lucent_ctx* ctx, lucent_scene* scene, lucent_
   return LUCENT_OK;
 }
 
-lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items, uint32_t count,
-                                 const lucent_render_settings* settings, void* stream) {
-  clear_error();
-  if (!ctx || (!items && count) || !settings) return fail(LUCENT_E_INVALID, "null argument");
-  std::vector<TileItem> tiles;
-  std::vector<uint64_t> cost;
-  tiles.reserve(count);
+}  // extern "C"
+
+// Checks a batch (scenes traceable, no film twice) and turns it into launch
+// items, longest jobs first (a job costs about its paths, tile pixels x
+// samples, x the log of the scene size), with the launch shape of
+// `settings`.  films[k] is the film of tiles[k]; items without samples are
+// left out.
+static Status batch_tiles(const lucent_batch_item* items, uint32_t count, const lucent_render_settings* settings,
+                          TileLaunch* cfg, std::vector<TileItem>* tiles, std::vector<lucent_film*>* films) {
   for (uint32_t i = 0; i < count; ++i) {
     const lucent_batch_item& b = items[i];
     if (!b.scene || !b.film) return fail(LUCENT_E_INVALID, "batch item without scene or film");
@@ -429,37 +431,27 @@ lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items
     for (uint32_t j = 0; j < i; ++j)
       if (items[j].film == b.film) return fail(LUCENT_E_INVALID, "a film appears twice in one batch");
   }
-  void* st = stream ? stream : ctx->stream;
-  if (settings->flags & LUCENT_TRACE_PER_SAMPLE) {
-    for (uint32_t i = 0; i < count; ++i) {
-      lucent_status s = lucent_trace_samples(ctx, items[i].scene, items[i].film, &items[i].camera, settings,
-                                             items[i].samples, st);
-      if (s != LUCENT_OK) return s;
-    }
-    return LUCENT_OK;
-  }
   bool fused = !(settings->flags & LUCENT_TRACE_INLINE_SHADOWS);
   for (uint32_t i = 0; i < count; ++i) fused = fused && !items[i].scene->blend_alpha;
-  TileLaunch cfg;
-  cfg.fused = fused;
-  cfg.waves_per_cu = settings->waves_per_cu;
-  cfg.occupancy = settings->occupancy;
-  cfg.tile_w = settings->tile_w;
-  cfg.tile_h = settings->tile_h;
-  cfg.shade_batch = settings->shade_batch;
-  cfg.trace_turns = settings->trace_turns;
-  cfg.job_paths = settings->job_paths;
-  // Longest jobs first: a job costs about its paths (tile pixels x samples)
-  // x (log of the scene size).
+  cfg->fused = fused;
+  cfg->waves_per_cu = settings->waves_per_cu;
+  cfg->occupancy = settings->occupancy;
+  cfg->tile_w = settings->tile_w;
+  cfg->tile_h = settings->tile_h;
+  cfg->shade_batch = settings->shade_batch;
+  cfg->trace_turns = settings->trace_turns;
+  cfg->job_paths = settings->job_paths;
   std::vector<uint32_t> order(count);
   for (uint32_t i = 0; i < count; ++i) order[i] = i;
   auto job_cost = [&](uint32_t i) {
     const double tris = double(items[i].scene->bvh.tri_count) + 2.0;
     uint32_t tw = 0, th = 0;
-    launch_job_tile(cfg, items[i].samples, &tw, &th);
+    launch_job_tile(*cfg, items[i].samples, &tw, &th);
     return double(tw) * th * items[i].samples * std::log2(tris);
   };
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return job_cost(a) > job_cost(b); });
+  tiles->clear();
+  films->clear();
   for (uint32_t i : order) {
     const lucent_batch_item& b = items[i];
     if (b.samples == 0) continue;
@@ -469,10 +461,33 @@ lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items
     it.frame.seed = b.film->seed;
     it.film = b.film->f;
     it.spp = b.samples;
-    tiles.push_back(it);
+    tiles->push_back(it);
+    films->push_back(b.film);
   }
-  Status s = launch_trace_tiles(ctx, tiles.data(), uint32_t(tiles.size()), cfg, st);
+  return LUCENT_OK;
+}
+
+extern "C" {
+
+lucent_status lucent_trace_batch(lucent_ctx* ctx, const lucent_batch_item* items, uint32_t count,
+                                 const lucent_render_settings* settings, void* stream) {
+  clear_error();
+  if (!ctx || (!items && count) || !settings) return fail(LUCENT_E_INVALID, "null argument");
+  void* st = stream ? stream : ctx->stream;
+  TileLaunch cfg;
+  std::vector<TileItem> tiles;
+  std::vector<lucent_film*> films;
+  Status s = batch_tiles(items, count, settings, &cfg, &tiles, &films);
   if (s != LUCENT_OK) return s;
+  if (settings->flags & LUCENT_TRACE_PER_SAMPLE) {
+    for (uint32_t i = 0; i < count; ++i) {
+      lucent_status r = lucent_trace_samples(ctx, items[i].scene, items[i].film, &items[i].camera, settings,
+                                             items[i].samples, st);
+      if (r != LUCENT_OK) return r;
+    }
+    return LUCENT_OK;
+  }
+  if ((s = launch_trace_tiles(ctx, tiles.data(), uint32_t(tiles.size()), cfg, st)) != LUCENT_OK) return s;
   for (uint32_t i = 0; i < count; ++i) {
     items[i].film->seed += items[i].samples;
     items[i].film->samples += items[i].samples;
@@ -1142,6 +1157,117 @@ lucent_status lucent_frame_graph_create(lucent_ctx* ctx, lucent_scene* scene, lu
   if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess)
     return bail(fail_hip(e, "hipGraphInstantiate"));
   *out = g;
+  return LUCENT_OK;
+}
+
+struct lucent_batch_graph {
+  lucent_ctx* ctx = nullptr;
+  std::vector<lucent_film*> films;      // in launch order
+  std::vector<uint32_t> samples;        // per film
+  std::vector<uint32_t> expected_seed;  // per film, the host seed a replay expects
+  void* buf = nullptr;  // [job table | seed words | arrival counters | path records | spill]
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture = nullptr;
+};
+
+void lucent_batch_graph_destroy(lucent_batch_graph* g) {
+  if (!g) return;
+  if (g->ctx) hipStreamSynchronize(static_cast<hipStream_t>(g->ctx->stream));
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  if (g->capture) hipStreamDestroy(g->capture);
+  if (g->buf) {
+    hipDeviceSynchronize();
+    hipFree(g->buf);
+  }
+  delete g;
+}
+
+lucent_status lucent_batch_graph_create(lucent_ctx* ctx, const lucent_batch_item* items, uint32_t count,
+                                        const lucent_render_settings* settings, lucent_batch_graph** out) {
+  clear_error();
+  if (!ctx || !items || count == 0 || !settings || !out) return fail(LUCENT_E_INVALID, "null argument");
+  *out = nullptr;
+  if (settings->flags & (LUCENT_TRACE_PER_SAMPLE | LUCENT_TRACE_WAVEFRONT))
+    return fail(LUCENT_E_UNSUPPORTED, "batch graphs capture the persistent batch launch only");
+  TileLaunch cfg;
+  std::vector<TileItem> tiles;
+  std::vector<lucent_film*> films;
+  Status s = batch_tiles(items, count, settings, &cfg, &tiles, &films);
+  if (s != LUCENT_OK) return s;
+  if (tiles.empty()) return fail(LUCENT_E_INVALID, "a batch graph traces at least one sample");
+  // Finish queued work on the films before their seeds are read.
+  if ((s = stream_sync(ctx->stream)) != LUCENT_OK) return s;
+  for (size_t k = 0; k < tiles.size(); ++k) tiles[k].frame.seed = films[k]->seed;
+  BatchPlan plan;
+  const uint32_t n = uint32_t(tiles.size());
+  if ((s = plan_batch(ctx, tiles.data(), n, cfg, &plan)) != LUCENT_OK) return s;
+  auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+  size_t begin_off = 0, items_off = 0;
+  const size_t table = align(batch_table_bytes(n, &begin_off, &items_off));
+  const size_t seeds_off = table, arrived_off = seeds_off + align(size_t(n) * sizeof(uint32_t));
+  const size_t records_off = arrived_off + align(plan.arrived_bytes);
+  const size_t spill_off = records_off + align(plan.records_bytes);
+  const size_t bytes = spill_off + align(plan.spill_bytes);
+  auto* g = new lucent_batch_graph();
+  g->ctx = ctx;
+  auto bail = [&](Status st) {
+    lucent_batch_graph_destroy(g);
+    return st;
+  };
+  if ((s = dev_alloc(&g->buf, bytes)) != LUCENT_OK) return bail(s);
+  char* base = static_cast<char*>(g->buf);
+  uint32_t* seeds = reinterpret_cast<uint32_t*>(base + seeds_off);
+  std::vector<uint32_t> seed0(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    plan.items[k].seed_src = seeds + k;
+    seed0[k] = films[k]->seed;
+  }
+  std::vector<char> host(table - begin_off, 0);
+  std::memcpy(host.data(), plan.job_begin.data(), plan.job_begin.size() * sizeof(uint32_t));
+  std::memcpy(host.data() + (items_off - begin_off), plan.items.data(), size_t(n) * sizeof(TileItem));
+  hipError_t e = hipMemcpy(base + begin_off, host.data(), host.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(seeds, seed0.data(), size_t(n) * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return bail(fail_hip(e, "batch graph capture start"));
+  // Trace every film, then advance the device seeds.
+  const TileItem* d_items = reinterpret_cast<const TileItem*>(base + items_off);
+  Status cs = launch_batch_planned(plan, reinterpret_cast<uint32_t*>(base),
+                                   reinterpret_cast<const uint32_t*>(base + begin_off), d_items,
+                                   reinterpret_cast<uint32_t*>(base + spill_off), base + records_off,
+                                   reinterpret_cast<uint32_t*>(base + arrived_off), g->capture);
+  if (cs == LUCENT_OK) cs = launch_advance_seeds(seeds, d_items, n, g->capture);
+  e = hipStreamEndCapture(g->capture, &g->graph);
+  if (cs != LUCENT_OK) return bail(cs);
+  if (e != hipSuccess) return bail(fail_hip(e, "batch graph capture"));
+  if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess)
+    return bail(fail_hip(e, "hipGraphInstantiate"));
+  g->films = films;
+  g->samples.resize(n);
+  g->expected_seed.resize(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    g->samples[k] = tiles[k].spp;
+    g->expected_seed[k] = films[k]->seed;
+  }
+  *out = g;
+  return LUCENT_OK;
+}
+
+lucent_status lucent_batch_graph_launch(lucent_batch_graph* g, void* stream) {
+  clear_error();
+  if (!g) return fail(LUCENT_E_INVALID, "null graph");
+  for (size_t k = 0; k < g->films.size(); ++k)
+    if (g->films[k]->seed != g->expected_seed[k])
+      return fail(LUCENT_E_INVALID, "a film was advanced outside this graph; create a new graph");
+  const hipError_t e = hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream ? stream : g->ctx->stream));
+  if (e != hipSuccess) return fail_hip(e, "hipGraphLaunch");
+  for (size_t k = 0; k < g->films.size(); ++k) {
+    g->films[k]->seed += g->samples[k];
+    g->films[k]->samples += g->samples[k];
+    g->expected_seed[k] = g->films[k]->seed;
+  }
   return LUCENT_OK;
 }
 

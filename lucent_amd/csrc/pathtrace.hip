@@ -842,6 +842,11 @@ __global__ void k_begin_units(TileItem* dst, TileItem v, uint32_t* counter) {
   *counter = 0u;
 }
 __global__ void k_add_u32(uint32_t* p, uint32_t v) { *p += v; }
+// seeds[i] += items[i].spp: the device seed words of a replayed batch.
+__global__ void k_advance_seeds(uint32_t* seeds, const TileItem* items, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) seeds[i] += items[i].spp;
+}
 
 }  // namespace
 
@@ -1000,6 +1005,99 @@ Status launch_trace_units(Context* ctx, TileItem item, const TileLaunch& cfg, hi
   return LUCENT_OK;
 }
 
+Status plan_batch(const Context* ctx, const TileItem* items, uint32_t n_items, const TileLaunch& cfg, BatchPlan* p) {
+  p->items.assign(items, items + n_items);
+  p->job_begin.assign(n_items + 1, 0);
+  uint32_t depth = 1;
+  for (uint32_t i = 0; i < n_items; ++i) depth = std::max(depth, pt::pt_spill_words(items[i].scene.bvh));
+  p->waves = tile_waves(cfg.occupancy);
+  p->fused = cfg.fused;
+  const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * p->waves;
+  const uint32_t slots = uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu;
+  p->lane_words = tile_lane_words(depth);
+  p->ring = false;
+  p->s_max = 0;
+  p->records_bytes = p->arrived_bytes = 0;
+  if (cfg.fused && cfg.units) {
+    // Per-sample hand-out with a record ring of kRingJobs jobs per wave,
+    // when the records fit the context's budget (else the pixel hand-out
+    // below).  Its jobs are tiles of the launch shape.
+    uint32_t s_max = 0;
+    for (uint32_t i = 0; i < n_items; ++i) s_max = std::max(s_max, items[i].spp);
+    std::vector<TileItem> shaped(items, items + n_items);
+    std::vector<uint32_t> ubegin(n_items + 1, 0);
+    for (uint32_t i = 0; i < n_items; ++i) {
+      Status s = apply_launch_shape(cfg, &shaped[i]);
+      if (s != LUCENT_OK) return s;
+      ubegin[i + 1] = ubegin[i] + (shaped[i].spp ? unit_jobs(shaped[i]) : 0);
+    }
+    const uint32_t u_jobs = ubegin[n_items];
+    const uint32_t u_grid = std::min<uint32_t>(u_jobs, slots);
+    const size_t ring_records = size_t(u_grid) * kRingJobs * kPtBlock * s_max * kRecordFloat4 * sizeof(float4);
+    if (s_max > 0 && ring_records <= ctx->record_budget) {
+      p->ring = true;
+      p->items.swap(shaped);
+      p->job_begin.swap(ubegin);
+      p->n_jobs = u_jobs;
+      p->grid = u_grid;
+      p->s_max = s_max;
+      p->records_bytes = ring_records;
+      p->arrived_bytes = size_t(u_grid) * kRingJobs * kPtBlock * sizeof(uint32_t);
+    }
+  }
+  if (!p->ring) {
+    // Whole pixels of 8x8 tiles.
+    for (uint32_t i = 0; i < n_items; ++i) {
+      TileItem& it = p->items[i];
+      it.tiles_x = (it.frame.width + kTile - 1) / kTile;
+      p->job_begin[i + 1] = p->job_begin[i] + (it.spp ? tile_jobs(it.frame.width, it.frame.height) : 0);
+    }
+    p->n_jobs = p->job_begin[n_items];
+    p->grid = std::min<uint32_t>(p->n_jobs, slots);
+  }
+  p->spill_bytes = size_t(p->grid) * kPtBlock * p->lane_words * sizeof(uint32_t);
+  return LUCENT_OK;
+}
+
+Status launch_batch_planned(const BatchPlan& p, uint32_t* counter, const uint32_t* d_begin, const TileItem* d_items,
+                            uint32_t* spill, void* records_, uint32_t* arrived, void* stream_) {
+  if (p.n_jobs == 0) return LUCENT_OK;
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  float4* records = static_cast<float4*>(records_);
+  const uint32_t n = uint32_t(p.items.size());
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
+  if (p.ring) {
+    auto batch_kernel = p.waves == 3   ? k_trace_batch_units<3>
+                        : p.waves == 4 ? k_trace_batch_units<4>
+                        : p.waves == 5 ? k_trace_batch_units<5>
+                        : p.waves == 6 ? k_trace_batch_units<6>
+                                       : k_trace_batch_units<2>;
+    hipLaunchKernelGGL(batch_kernel, dim3(p.grid), dim3(kPtBlock), 0, stream, d_items, d_begin, n, p.n_jobs, counter,
+                       spill, p.lane_words, records, arrived, p.s_max);
+  } else {
+    auto pick = [&](auto fused) {
+      constexpr bool kF = decltype(fused)::value;
+      return p.waves == 3   ? k_trace_tiles<false, 3, kF>
+             : p.waves == 4 ? k_trace_tiles<false, 4, kF>
+             : p.waves == 5 ? k_trace_tiles<false, 5, kF>
+             : p.waves == 6 ? k_trace_tiles<false, 6, kF>
+                            : k_trace_tiles<false, 2, kF>;
+    };
+    auto kernel = p.fused ? pick(std::true_type{}) : pick(std::false_type{});
+    hipLaunchKernelGGL(kernel, dim3(p.grid), dim3(kPtBlock), 0, stream, nullptr, d_items, d_begin, n, p.n_jobs, counter,
+                       spill, p.lane_words);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(e, p.ring ? "k_trace_batch_units launch" : "k_trace_tiles launch");
+  return LUCENT_OK;
+}
+
+size_t batch_table_bytes(uint32_t n_items, size_t* begin_off, size_t* items_off) {
+  *begin_off = 256;
+  *items_off = (*begin_off + size_t(n_items + 1) * sizeof(uint32_t) + 255) / 256 * 256;
+  return *items_off + size_t(n_items) * sizeof(TileItem);
+}
+
 Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const TileLaunch& cfg, void* stream_) {
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   if (n_items == 0) return LUCENT_OK;
@@ -1007,16 +1105,46 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
     if (items[0].spp == 0) return LUCENT_OK;
     return launch_trace_units(ctx, items[0], cfg, stream);
   }
-  std::vector<uint32_t> job_begin(n_items + 1, 0);
-  uint32_t depth = 1;
-  for (uint32_t i = 0; i < n_items; ++i) {
-    items[i].tiles_x = (items[i].frame.width + kTile - 1) / kTile;
-    const uint32_t jobs = items[i].spp ? tile_jobs(items[i].frame.width, items[i].frame.height) : 0;
-    job_begin[i + 1] = job_begin[i] + jobs;
-    depth = std::max(depth, pt::pt_spill_words(items[i].scene.bvh));
+  if (n_items > 1) {
+    if (cfg.counter) return fail(LUCENT_E_INVALID, "caller-owned tile buffers take one film");
+    BatchPlan plan;
+    Status s = plan_batch(ctx, items, n_items, cfg, &plan);
+    if (s != LUCENT_OK) return s;
+    for (uint32_t i = 0; i < n_items; ++i) items[i].tiles_x = plan.items[i].tiles_x;
+    if (plan.n_jobs == 0) return LUCENT_OK;
+    if ((s = ensure_spill(ctx, plan.spill_bytes)) != LUCENT_OK) return s;
+    // Device table: [counter | job_begin (n+1) | items], 256-byte aligned parts.
+    size_t begin_off = 0, items_off = 0;
+    const size_t bytes = batch_table_bytes(n_items, &begin_off, &items_off);
+    if ((s = ensure_tile_table(ctx, bytes)) != LUCENT_OK) return s;
+    char* table = static_cast<char*>(ctx->tile_table);
+    std::vector<char> host(bytes - begin_off);
+    std::memcpy(host.data(), plan.job_begin.data(), plan.job_begin.size() * sizeof(uint32_t));
+    std::memcpy(host.data() + (items_off - begin_off), plan.items.data(), size_t(n_items) * sizeof(TileItem));
+    hipError_t e = hipMemcpyAsync(table + begin_off, host.data(), host.size(), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipEventRecord(static_cast<hipEvent_t>(ctx->tile_event), stream);
+    if (e == hipSuccess) e = hipEventSynchronize(static_cast<hipEvent_t>(ctx->tile_event));  // `host` must outlive the copy
+    if (e != hipSuccess) return fail_hip(e, "tile table upload");
+    float4* records = nullptr;
+    uint32_t* arrived = nullptr;
+    if (plan.ring) {
+      const size_t counters_off = (plan.records_bytes + 255) / 256 * 256;
+      if ((s = ensure_units(ctx, counters_off + plan.arrived_bytes)) != LUCENT_OK) return s;
+      records = static_cast<float4*>(ctx->units);
+      arrived = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->units) + counters_off);
+    }
+    return launch_batch_planned(plan, reinterpret_cast<uint32_t*>(table),
+                                reinterpret_cast<const uint32_t*>(table + begin_off),
+                                reinterpret_cast<const TileItem*>(table + items_off), static_cast<uint32_t*>(ctx->spill),
+                                records, arrived, stream);
   }
-  const uint32_t n_jobs = job_begin[n_items];
+  // One film, whole pixels: the context's buffers or the caller's (a
+  // captured graph).
+  TileItem& it = items[0];
+  it.tiles_x = (it.frame.width + kTile - 1) / kTile;
+  const uint32_t n_jobs = it.spp ? tile_jobs(it.frame.width, it.frame.height) : 0;
   if (n_jobs == 0) return LUCENT_OK;
+  const uint32_t depth = std::max(1u, pt::pt_spill_words(it.scene.bvh));
   const uint32_t waves = tile_waves(cfg.occupancy);
   const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu : 4 * waves;
   const uint32_t grid = std::min<uint32_t>(n_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
@@ -1024,116 +1152,31 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
   const size_t spill_bytes = size_t(grid) * kPtBlock * lane_words * sizeof(uint32_t);
   uint32_t* spill = cfg.spill;
   uint32_t* counter = cfg.counter;
+  TileItem* d_single = cfg.item;
   Status s;
   if (cfg.counter) {
-    // Caller-owned buffers (a captured graph): one film, nothing to upload.
-    if (n_items != 1) return fail(LUCENT_E_INVALID, "caller-owned tile buffers take one film");
     if (cfg.spill_bytes < spill_bytes) return fail(LUCENT_E_INVALID, "caller-owned spill buffer too small");
     if (!cfg.item) return fail(LUCENT_E_INVALID, "caller-owned tile buffers need a film descriptor slot");
   } else {
     if ((s = ensure_spill(ctx, spill_bytes)) != LUCENT_OK) return s;
     spill = static_cast<uint32_t*>(ctx->spill);
+    if ((s = ensure_tile_table(ctx, 256 + sizeof(TileItem))) != LUCENT_OK) return s;
+    counter = static_cast<uint32_t*>(ctx->tile_table);
+    d_single = reinterpret_cast<TileItem*>(static_cast<char*>(ctx->tile_table) + 256);
   }
-  // Device table: [counter | job_begin (n+1) | items], 256-byte aligned parts.
-  const size_t begin_off = 256;
-  const size_t items_off = (begin_off + job_begin.size() * sizeof(uint32_t) + 255) / 256 * 256;
-  const size_t bytes = items_off + size_t(n_items) * sizeof(TileItem);
-  char* table = nullptr;
-  if (!cfg.counter) {
-    if ((s = ensure_tile_table(ctx, bytes)) != LUCENT_OK) return s;
-    table = static_cast<char*>(ctx->tile_table);
-    counter = reinterpret_cast<uint32_t*>(table);
-  }
-  if (n_items > 1) {
-    std::vector<char> host(bytes - begin_off);
-    std::memcpy(host.data(), job_begin.data(), job_begin.size() * sizeof(uint32_t));
-    std::memcpy(host.data() + (items_off - begin_off), items, size_t(n_items) * sizeof(TileItem));
-    hipError_t e = hipMemcpyAsync(table + begin_off, host.data(), host.size(), hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipEventRecord(static_cast<hipEvent_t>(ctx->tile_event), stream);
-    if (e == hipSuccess) e = hipEventSynchronize(static_cast<hipEvent_t>(ctx->tile_event));  // `host` must outlive the copy
-    if (e != hipSuccess) return fail_hip(e, "tile table upload");
-  }
-  // One film: its descriptor goes to device memory with the counter reset
-  // (a kernel, so that a captured graph holds it), and the kernel reads it
+  // The film's descriptor goes to device memory with the counter reset (a
+  // kernel, so that a captured graph holds it), and the kernel reads it
   // through a pointer (fewer registers than a by-value argument).
-  TileItem* d_single = nullptr;
-  if (n_items == 1) {
-    d_single = cfg.counter ? cfg.item : reinterpret_cast<TileItem*>(table + items_off);
-    hipLaunchKernelGGL(k_begin_units, dim3(1), dim3(1), 0, stream, d_single, items[0], counter);
-  } else {
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, counter, 0u);
-  }
-  const TileItem* d_items = n_items > 1 ? reinterpret_cast<const TileItem*>(table + items_off) : nullptr;
-  const uint32_t* d_begin = n_items > 1 ? reinterpret_cast<const uint32_t*>(table + begin_off) : nullptr;
-  if (n_items > 1 && cfg.fused && cfg.units) {
-    // Per-sample hand-out with a record ring of kRingJobs jobs per wave,
-    // when the records fit the context's budget (else the pixel hand-out
-    // below).  Its jobs are tiles of the launch shape: the job table and the
-    // film descriptors are rewritten for it.
-    uint32_t s_max = 0;
-    for (uint32_t i = 0; i < n_items; ++i) s_max = std::max(s_max, items[i].spp);
-    std::vector<TileItem> shaped(items, items + n_items);
-    std::vector<uint32_t> ubegin(n_items + 1, 0);
-    for (uint32_t i = 0; i < n_items; ++i) {
-      if ((s = apply_launch_shape(cfg, &shaped[i])) != LUCENT_OK) return s;
-      ubegin[i + 1] = ubegin[i] + (shaped[i].spp ? unit_jobs(shaped[i]) : 0);
-    }
-    const uint32_t u_jobs = ubegin[n_items];
-    const uint32_t u_grid = std::min<uint32_t>(u_jobs, uint32_t(std::max(ctx->compute_units, 1)) * waves_per_cu);
-    const size_t ring_records = size_t(u_grid) * kRingJobs * kPtBlock * s_max * kRecordFloat4 * sizeof(float4);
-    const size_t counters_off = (ring_records + 255) / 256 * 256;
-    if (s_max > 0 && ring_records <= ctx->record_budget) {
-      if ((s = ensure_spill(ctx, size_t(u_grid) * kPtBlock * lane_words * sizeof(uint32_t))) != LUCENT_OK) return s;
-      spill = static_cast<uint32_t*>(ctx->spill);
-      std::vector<char> host(bytes - begin_off);
-      std::memcpy(host.data(), ubegin.data(), ubegin.size() * sizeof(uint32_t));
-      std::memcpy(host.data() + (items_off - begin_off), shaped.data(), size_t(n_items) * sizeof(TileItem));
-      hipError_t ue = hipMemcpyAsync(table + begin_off, host.data(), host.size(), hipMemcpyHostToDevice, stream);
-      if (ue == hipSuccess) ue = hipEventRecord(static_cast<hipEvent_t>(ctx->tile_event), stream);
-      if (ue == hipSuccess) ue = hipEventSynchronize(static_cast<hipEvent_t>(ctx->tile_event));
-      if (ue != hipSuccess) return fail_hip(ue, "tile table upload");
-      if ((s = ensure_units(ctx, counters_off + size_t(u_grid) * kRingJobs * kPtBlock * sizeof(uint32_t))) != LUCENT_OK)
-        return s;
-      float4* records = static_cast<float4*>(ctx->units);
-      uint32_t* arrived = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->units) + counters_off);
-      auto batch_kernel = waves == 3 ? k_trace_batch_units<3>
-                          : waves == 4 ? k_trace_batch_units<4>
-                          : waves == 5 ? k_trace_batch_units<5>
-                          : waves == 6 ? k_trace_batch_units<6>
-                                       : k_trace_batch_units<2>;
-      hipLaunchKernelGGL(batch_kernel, dim3(u_grid), dim3(kPtBlock), 0, stream, d_items, d_begin, n_items, u_jobs,
-                         counter, spill, lane_words, records, arrived, s_max);
-      const hipError_t e = hipGetLastError();
-      if (e != hipSuccess) return fail_hip(e, "k_trace_batch_units launch");
-      return LUCENT_OK;
-    }
-  }
-  auto launch = [&](auto single_kernel, auto multi_kernel) {
-    if (n_items == 1)
-      hipLaunchKernelGGL(single_kernel, dim3(grid), dim3(kPtBlock), 0, stream, d_single, nullptr, nullptr, 1u, n_jobs,
-                         counter, spill, lane_words);
-    else
-      hipLaunchKernelGGL(multi_kernel, dim3(grid), dim3(kPtBlock), 0, stream, nullptr, d_items, d_begin, n_items,
-                         n_jobs, counter, spill, lane_words);
-  };
+  hipLaunchKernelGGL(k_begin_units, dim3(1), dim3(1), 0, stream, d_single, it, counter);
   auto launch_waves = [&](auto fused) {
     constexpr bool kF = decltype(fused)::value;
-    switch (waves) {
-      case 3:
-        launch(k_trace_tiles<true, 3, kF>, k_trace_tiles<false, 3, kF>);
-        break;
-      case 4:
-        launch(k_trace_tiles<true, 4, kF>, k_trace_tiles<false, 4, kF>);
-        break;
-      case 5:
-        launch(k_trace_tiles<true, 5, kF>, k_trace_tiles<false, 5, kF>);
-        break;
-      case 6:
-        launch(k_trace_tiles<true, 6, kF>, k_trace_tiles<false, 6, kF>);
-        break;
-      default:
-        launch(k_trace_tiles<true, 2, kF>, k_trace_tiles<false, 2, kF>);
-    }
+    auto kernel = waves == 3   ? k_trace_tiles<true, 3, kF>
+                  : waves == 4 ? k_trace_tiles<true, 4, kF>
+                  : waves == 5 ? k_trace_tiles<true, 5, kF>
+                  : waves == 6 ? k_trace_tiles<true, 6, kF>
+                               : k_trace_tiles<true, 2, kF>;
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kPtBlock), 0, stream, d_single, nullptr, nullptr, 1u, n_jobs, counter,
+                       spill, lane_words);
   };
   if (cfg.fused)
     launch_waves(std::true_type{});
@@ -1141,6 +1184,15 @@ Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const
     launch_waves(std::false_type{});
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "k_trace_tiles launch");
+  return LUCENT_OK;
+}
+
+Status launch_advance_seeds(uint32_t* seeds, const TileItem* d_items, uint32_t n_items, void* stream) {
+  if (n_items == 0) return LUCENT_OK;
+  hipLaunchKernelGGL(k_advance_seeds, dim3((n_items + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream), seeds,
+                     d_items, n_items);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(e, "k_advance_seeds launch");
   return LUCENT_OK;
 }
 

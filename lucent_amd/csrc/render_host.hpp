@@ -85,4 +85,28 @@ Status apply_launch_shape(const TileLaunch& cfg, TileItem* item);
 // items[i].frame.seed; fills items[i].tiles_x.
 Status launch_trace_tiles(Context* ctx, TileItem* items, uint32_t n_items, const TileLaunch& cfg, void* stream);
 
+// A launch over many films (a batch) as data: the film descriptors as the
+// kernel reads them, the job table (film i owns jobs [job_begin[i],
+// job_begin[i+1])), the grid, and the sizes of the buffers it needs.
+// lucent_trace_batch runs it on the context's buffers, a batch graph on
+// its own.
+struct BatchPlan {
+  std::vector<TileItem> items;
+  std::vector<uint32_t> job_begin;
+  uint32_t n_jobs = 0, grid = 0, waves = 4, lane_words = 0, s_max = 0;
+  bool fused = true;
+  bool ring = false;  // per-sample hand-out with per-wave record rings, else whole pixels
+  size_t spill_bytes = 0, records_bytes = 0, arrived_bytes = 0;
+};
+Status plan_batch(const Context* ctx, const TileItem* items, uint32_t n_items, const TileLaunch& cfg, BatchPlan* p);
+// The device table of a batch: [job counter | job_begin | items], 256-byte
+// aligned parts; returns its size.
+size_t batch_table_bytes(uint32_t n_items, size_t* begin_off, size_t* items_off);
+// Resets the job counter and launches a planned batch whose job table and
+// descriptors are in device memory (kernels only: capturable in a graph).
+Status launch_batch_planned(const BatchPlan& p, uint32_t* counter, const uint32_t* d_begin, const TileItem* d_items,
+                            uint32_t* spill, void* records, uint32_t* arrived, void* stream);
+// seeds[i] += d_items[i].spp on the device.
+Status launch_advance_seeds(uint32_t* seeds, const TileItem* d_items, uint32_t n_items, void* stream);
+
 }  // namespace lucent

@@ -496,3 +496,34 @@ def trace_batch(ctx: Context, jobs, settings: RenderSettings, stream: Optional[i
     of `settings`."""
     st = settings.struct()
     _native.check(_native.lib().lucent_trace_batch(ctx.handle, _batch_items(ctx, jobs), len(jobs), C.byref(st), stream))
+
+
+class BatchGraph:
+    """lucent_batch_graph_*: one trace_batch over `jobs` (a list of
+    (Renderer, samples)) captured into a HIP graph with its own device
+    buffers; launch() replays it, each replay adding every job's samples to
+    its film, bit-identical to the same number of trace_batch calls.
+    Cameras and settings are fixed at capture."""
+
+    def __init__(self, ctx: Context, jobs, settings: RenderSettings):
+        st = settings.struct()
+        h = C.c_void_p()
+        _native.check(_native.lib().lucent_batch_graph_create(ctx.handle, _batch_items(ctx, jobs), len(jobs),
+                                                              C.byref(st), C.byref(h)))
+        self._h = h
+        self.jobs = list(jobs)
+        ctx.adopt(self)
+
+    def launch(self, stream: Optional[int] = None) -> None:
+        _native.check(_native.lib().lucent_batch_graph_launch(self._h, stream))
+
+    def close(self) -> None:
+        if self._h is not None:
+            _native.lib().lucent_batch_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -76,3 +76,70 @@ def test_graph_refuses_a_film_advanced_elsewhere(ctx):
     assert st != 0 and b"outside this graph" in _native.lib().lucent_last_error(ctx.handle)
     graph.close()
     r.close()
+
+
+def batch_set(ctx, seed):
+    """Three films of two scenes at different sizes (one with an alpha-tested
+    surface), seeded alike, for a batch graph and its eager twin."""
+    from lucent_amd import workloads
+    from test_pathtrace_gpu import alpha_scene
+    scenes = [feature_scene(), workloads.spheres_scene(), alpha_scene()]
+    sizes = [(64, 40), (33, 17), (40, 24)]
+    rs = []
+    for sc, (w, h) in zip(scenes, sizes):
+        r = Renderer(ctx, w, h, RenderSettings(max_bounces=8, firefly_clamp=0.0))
+        r.load_scene(sc)
+        r.film.set_seed(seed)
+        r.reset()
+        rs.append(r)
+    return rs
+
+
+@pytest.mark.parametrize("budget", [0, 1])
+def test_batch_graph_replays_equal_eager_batches(ctx, budget):
+    """lucent_batch_graph_*: a batch captured once and replayed n times
+    leaves every film bit-identical to n eager lucent_trace_batch calls
+    (default record budget: the per-sample hand-out with record rings; a
+    budget of 1 byte: whole pixels), with eager work and synchronisations
+    between the replays; the films' sample counts and seeds advance."""
+    import torch
+    from lucent_amd.renderer import BatchGraph, trace_batch
+
+    ctx.set_record_budget(budget)
+    try:
+        g_rs, e_rs = batch_set(ctx, 9), batch_set(ctx, 9)
+        spps = [3, 1, 2]
+        settings = RenderSettings(max_bounces=8, firefly_clamp=0.0)
+        graph = BatchGraph(ctx, list(zip(g_rs, spps)), settings)
+        for rep in range(3):
+            print(f"batch graph replay {rep}, record budget {budget}", flush=True)
+            graph.launch()
+            trace_batch(ctx, list(zip(e_rs, spps)), settings)
+            ctx.synchronize()
+            torch.cuda.synchronize()
+            for g, e, n in zip(g_rs, e_rs, spps):
+                a, b = g.film.read(), e.film.read()
+                for k in a:
+                    assert np.array_equal(a[k], b[k]), (rep, k)
+                assert g.film.info() == e.film.info()
+                assert g.film.info()["samples"] == n * (rep + 1)
+        graph.close()
+        for r in g_rs + e_rs:
+            r.close()
+    finally:
+        ctx.set_record_budget(0)
+
+
+def test_batch_graph_refuses_a_film_advanced_elsewhere(ctx):
+    from lucent_amd.renderer import BatchGraph
+    rs = batch_set(ctx, 4)
+    graph = BatchGraph(ctx, [(r, 1) for r in rs], RenderSettings(max_bounces=8, firefly_clamp=0.0))
+    graph.launch()
+    rs[1].trace_samples(1)
+    st = _native.lib().lucent_batch_graph_launch(graph._h, None)
+    assert st != 0 and b"outside this graph" in _native.lib().lucent_last_error(ctx.handle)
+    with pytest.raises(_native.LucentError, match="persistent batch launch"):
+        BatchGraph(ctx, [(r, 1) for r in rs], RenderSettings(max_bounces=8, firefly_clamp=0.0, per_sample_launch=True))
+    graph.close()
+    for r in rs:
+        r.close()
