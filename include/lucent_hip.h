@@ -94,6 +94,20 @@ lucent_status lucent_build_bvh_gpu(lucent_ctx *ctx, const float *positions, uint
                                    float *nodes_out, uint32_t *node_count_out, uint32_t *tri_indices_out,
                                    double *bounds6_out);
 
+/* GPU builder with the outputs and layout of lucent_build_bvh_gpu whose tree
+ * is built by surface area: after the Morton sort, clusters (the triangles at
+ * first) are merged in rounds on the device, each cluster pairing with the
+ * neighbour within `radius` positions of the Morton order whose union box has
+ * the least surface area, mutual pairs merging, until one cluster is left
+ * (parallel locally-ordered clustering, Meister and Bittner 2018).  Subtrees
+ * of at most max_leaf triangles become leaves.  radius 0 means 16, at most
+ * 64; max_leaf 0 means 4.  Its trees cost about what the SAH builder's do to
+ * traverse.  No counterpart in the reference. */
+lucent_status lucent_build_bvh_gpu_clustered(lucent_ctx *ctx, const float *positions, uint32_t tri_count,
+                                             uint32_t max_leaf, uint32_t radius, float *nodes_out,
+                                             uint32_t *node_count_out, uint32_t *tri_indices_out,
+                                             double *bounds6_out);
+
 /* ------------------------------------------------------------------------
  * Scene upload — PackedScene (src/scene/scene.ts:106-134) as uploaded by
  * Renderer.loadScene (src/renderer.ts:279-312).  Arrays use the reference's
@@ -138,9 +152,9 @@ lucent_status lucent_scene_upload(lucent_ctx *ctx, const lucent_packed_scene *pa
  * the caller's tree.  max_leaf = 0 keeps the caller's tree as the device
  * tree.  The SAH rebuild runs its independent subtrees on the host's
  * threads (lucent_build_bvh_parallel); bins = 0 builds the device tree on
- * the GPU instead (lucent_build_bvh_gpu's linear BVH with leaves of at most
- * max_leaf).  No counterpart in the reference (its shaders walk the tree
- * they are given). */
+ * the GPU instead (lucent_build_bvh_gpu_clustered with radius 16 and leaves
+ * of at most max_leaf).  No counterpart in the reference (its shaders walk
+ * the tree they are given). */
 lucent_status lucent_set_tree_build(lucent_ctx *ctx, uint32_t max_leaf, uint32_t bins);
 
 /* Device memory a tile launch may use for path records (48 B per pixel

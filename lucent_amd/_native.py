@@ -141,6 +141,7 @@ SIGNATURES: dict[str, tuple] = {
     "lucent_device_info": (_I32, [_P, C.POINTER(_I32), C.POINTER(C.c_int64)]),
     "lucent_build_bvh": (_I32, [_P, _U32, _U32, _U32, _P, C.POINTER(_U32), _P, _P]),
     "lucent_build_bvh_gpu": (_I32, [_P, _P, _U32, _U32, _P, C.POINTER(_U32), _P, _P]),
+    "lucent_build_bvh_gpu_clustered": (_I32, [_P, _P, _U32, _U32, _U32, _P, C.POINTER(_U32), _P, _P]),
     "lucent_build_bvh_parallel": (_I32, [_P, _U32, _U32, _U32, _U32, _P, C.POINTER(_U32), _P, _P]),
     "lucent_scene_upload": (_I32, [_P, C.POINTER(PackedSceneStruct), C.POINTER(_P)]),
     "lucent_scene_from_bvh": (_I32, [_P, _P, _U32, _P, _U32, _P, C.POINTER(_P)]),

@@ -64,10 +64,12 @@ def build_bvh(positions, max_leaf_size: int = 4, bins: int = 12) -> BVH:
     return BVH(nodes[: n * NODE_STRIDE].copy(), n, tri, (tuple(bounds[:3]), tuple(bounds[3:])))
 
 
-def build_bvh_gpu(ctx, positions, max_leaf_size: int = 4) -> BVH:
+def build_bvh_gpu(ctx, positions, max_leaf_size: int = 4, radius: int = 0) -> BVH:
     """lucent_build_bvh_gpu: a linear BVH built on the context's GPU (Morton
     order, radix tree), in buildBVH's layout (bvh.ts:6-16).  Same consumers,
-    higher traversal cost, far faster build for large meshes."""
+    higher traversal cost, far faster build for large meshes.  radius > 0:
+    lucent_build_bvh_gpu_clustered, clusters in Morton order merged by
+    surface area within that many neighbours (about the SAH tree's cost)."""
     pos = _as_f32(positions).reshape(-1)
     if pos.size % 9 != 0:
         raise ValueError(f"positions length {pos.size} is not a multiple of 9")
@@ -76,12 +78,17 @@ def build_bvh_gpu(ctx, positions, max_leaf_size: int = 4) -> BVH:
     tri = np.zeros(tri_count, dtype=np.uint32)
     bounds = np.zeros(6, dtype=np.float64)
     count = C.c_uint32(0)
-    _native.check(
-        _native.lib().lucent_build_bvh_gpu(
-            ctx.handle, pos.ctypes.data, tri_count, max_leaf_size, nodes.ctypes.data, C.byref(count),
-            tri.ctypes.data, bounds.ctypes.data,
+    if radius > 0:
+        _native.check(_native.lib().lucent_build_bvh_gpu_clustered(
+            ctx.handle, pos.ctypes.data, tri_count, max_leaf_size, radius, nodes.ctypes.data, C.byref(count),
+            tri.ctypes.data, bounds.ctypes.data))
+    else:
+        _native.check(
+            _native.lib().lucent_build_bvh_gpu(
+                ctx.handle, pos.ctypes.data, tri_count, max_leaf_size, nodes.ctypes.data, C.byref(count),
+                tri.ctypes.data, bounds.ctypes.data,
+            )
         )
-    )
     n = count.value
     return BVH(nodes[: n * NODE_STRIDE].copy(), n, tri, (tuple(bounds[:3]), tuple(bounds[3:])))
 

@@ -151,10 +151,12 @@ Status finish_scene(Context* ctx, Scene* sc, const float* nodes, uint32_t node_c
     double bounds[6];
     // The host cores build independent subtrees in parallel (any valid tree
     // serves: exact queries break ties by the caller's slots).
-    // With tree_bins = 0 the device builds a linear BVH instead.
+    // With tree_bins = 0 the device builds the tree (Morton order, clusters
+    // merged by surface area) instead.
     const uint32_t threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     if (ctx->tree_bins == 0)
-      s = build_bvh_gpu(ctx, pos.data(), n, ctx->tree_leaf, own_nodes.data(), &dev_node_count, ref.data(), bounds);
+      s = build_bvh_gpu(ctx, pos.data(), n, ctx->tree_leaf, own_nodes.data(), &dev_node_count, ref.data(), bounds,
+                        kClusterRadius);
     else
       s = build_bvh_parallel(pos.data(), n, ctx->tree_leaf, ctx->tree_bins, own_nodes.data(), &dev_node_count,
                              ref.data(), bounds, threads);
@@ -329,6 +331,19 @@ lucent_status lucent_build_bvh_gpu(lucent_ctx* ctx, const float* positions, uint
     return fail(LUCENT_E_INVALID, "null argument");
   if (tri_count >= 0x7FFFFFFFu) return fail(LUCENT_E_INVALID, "too many triangles");
   return build_bvh_gpu(ctx, positions, tri_count, max_leaf, nodes_out, node_count_out, tri_indices_out, bounds6_out);
+}
+
+lucent_status lucent_build_bvh_gpu_clustered(lucent_ctx* ctx, const float* positions, uint32_t tri_count,
+                                             uint32_t max_leaf, uint32_t radius, float* nodes_out,
+                                             uint32_t* node_count_out, uint32_t* tri_indices_out,
+                                             double* bounds6_out) {
+  clear_error();
+  if (!ctx || !nodes_out || !node_count_out || (tri_count > 0 && (!positions || !tri_indices_out)))
+    return fail(LUCENT_E_INVALID, "null argument");
+  if (tri_count >= 0x7FFFFFFFu) return fail(LUCENT_E_INVALID, "too many triangles");
+  if (radius > 64) return fail(LUCENT_E_INVALID, "search radius above 64");
+  return build_bvh_gpu(ctx, positions, tri_count, max_leaf, nodes_out, node_count_out, tri_indices_out, bounds6_out,
+                       radius ? radius : kClusterRadius);
 }
 
 lucent_status lucent_scene_from_bvh(lucent_ctx* ctx, const float* positions, uint32_t tri_count,

@@ -26,8 +26,14 @@ Status build_bvh_parallel(const float* positions, uint32_t tri_count, uint32_t m
                           float* nodes_out, uint32_t* node_count, uint32_t* tri_indices_out, double bounds[6],
                           uint32_t threads);
 struct Context;
+// Search radius of the clustered GPU build (neighbours each side in Morton
+// order) unless the caller names one.
+constexpr uint32_t kClusterRadius = 16;
+// radius 0: the linear BVH (radix tree of the sorted Morton codes); > 0
+// (at most 64): clusters merged by surface area within that many positions
+// of the Morton order.
 Status build_bvh_gpu(Context* ctx, const float* positions, uint32_t tri_count, uint32_t max_leaf, float* nodes_out,
-                     uint32_t* node_count_out, uint32_t* tri_indices_out, double bounds_out[6]);
+                     uint32_t* node_count_out, uint32_t* tri_indices_out, double bounds_out[6], uint32_t radius = 0);
 
 // ---------------------------------------------------------------------------
 // Device scene layout (all device pointers; plain types so that host C++

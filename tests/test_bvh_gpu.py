@@ -1,4 +1,5 @@
-"""GPU linear-BVH builder (lucent_build_bvh_gpu): the reference's node-layout
+"""GPU BVH builders (lucent_build_bvh_gpu, the linear BVH, and
+lucent_build_bvh_gpu_clustered, clusters merged by surface area): the reference's node-layout
 invariants (bvh.test.ts:31-86: every triangle in exactly one leaf, children
 adjacent and nested, boxes containing their triangles); with the LBVH kept as
 the device tree (lucent_set_tree_build(0)), exact closest hits through it
@@ -50,22 +51,30 @@ def check_layout(bvh, pos, max_leaf):
     assert np.all(seen == 1) and reached == bvh.node_count
 
 
+RADII = [0, 16]  # the linear BVH; the clustered build
+
+
+@pytest.mark.parametrize("radius", RADII)
 @pytest.mark.parametrize("which", ["soup", "knot"])
-def test_layout_invariants(ctx, which):
+def test_layout_invariants(ctx, which, radius):
     pos = soup(20000, 3) if which == "soup" else S.torus_knot(2, 3, 1.0, 0.3, 400, 24).positions
     for max_leaf in (1, 4, 8):
-        check_layout(build_bvh_gpu(ctx, pos, max_leaf), np.asarray(pos, np.float32), max_leaf)
+        check_layout(build_bvh_gpu(ctx, pos, max_leaf, radius), np.asarray(pos, np.float32), max_leaf)
 
 
-def test_tiny_inputs(ctx):
-    for n in (1, 2, 3, 5):
+@pytest.mark.parametrize("radius", RADII + [1, 64])
+def test_tiny_inputs(ctx, radius):
+    for n in (1, 2, 3, 5, 300):
         pos = soup(n, n)
-        b = build_bvh_gpu(ctx, pos, 4)
+        b = build_bvh_gpu(ctx, pos, 4, radius)
         check_layout(b, pos, 4)
-    b = build_bvh_gpu(ctx, np.zeros(0, np.float32))
+    # Coincident triangles: equal boxes everywhere, ties at every step.
+    same = np.tile(soup(1, 7), 257)
+    check_layout(build_bvh_gpu(ctx, same, 2, radius), same, 2)
+    b = build_bvh_gpu(ctx, np.zeros(0, np.float32), 4, radius)
     assert b.node_count == 1
     with pytest.raises(ValueError):
-        build_bvh_gpu(ctx, np.zeros(10, np.float32))
+        build_bvh_gpu(ctx, np.zeros(10, np.float32), 4, radius)
 
 
 class caller_tree:
@@ -82,7 +91,8 @@ class caller_tree:
         self.ctx.set_tree_build(2, 32)
 
 
-def test_lbvh_traversal_matches_the_oracle(ctx):
+@pytest.mark.parametrize("radius", RADII)
+def test_lbvh_traversal_matches_the_oracle(ctx, radius):
     """Exact closest hits through the GPU-built tree equal the FP64 oracle's
     (intersectBVH over the reference builder's tree), ties included: the
     exact mode breaks ties by the lowest slot of the caller's tree, and the
@@ -90,7 +100,7 @@ def test_lbvh_traversal_matches_the_oracle(ctx):
     import oracle as O
     cfg = workloads.config1(1 << 16)
     pos = cfg["positions"]
-    lb = build_bvh_gpu(ctx, pos)
+    lb = build_bvh_gpu(ctx, pos, 4, radius)
     with caller_tree(ctx):
         sc = ctx.scene_from_bvh(pos, lb)
         st = sc.stats()
@@ -130,7 +140,8 @@ def test_device_built_upload_tree(ctx):
     sc.destroy()
 
 
-def test_path_traced_film_through_the_lbvh(ctx):
+@pytest.mark.parametrize("radius", RADII)
+def test_path_traced_film_through_the_lbvh(ctx, radius):
     """The knot scene's film traced through the LBVH itself (no rebuild)
     equals the one traced through the SAH tree up to equal-distance ties,
     and agrees with the oracle."""
@@ -143,7 +154,7 @@ def test_path_traced_film_through_the_lbvh(ctx):
         pos = S.merge_meshes([m.mesh for m in sc.meshes]).positions
         if builder == "gpu":
             with caller_tree(ctx):
-                lb = build_bvh_gpu(ctx, pos)
+                lb = build_bvh_gpu(ctx, pos, 4, radius)
                 r.load_scene(sc, lb)
                 assert r.device_scene.stats()["node_count"] == lb.node_count
         else:
@@ -188,3 +199,28 @@ def test_build_time_and_cost_on_1_5m_triangles(ctx):
     print(f"{pos.size // 9} tris: GPU LBVH {t_gpu * 1e3:.0f} ms (incl. upload and layout), CPU SAH {t_cpu * 1e3:.0f} ms; "
           f"SAH cost ratio LBVH/SAH {ratio:.3f}")
     assert g.node_count > 1 and ratio < 2.0
+
+
+def test_clustered_build_cost_on_1_5m_triangles(ctx):
+    """The clustered GPU build on the 1.5M-triangle sphere: its tree's SAH
+    cost against the host SAH builder's and the linear BVH's, and its time."""
+    mesh = S.uv_sphere(866, 866, 1.0, displace=lambda p, n: 0.05 * np.sin(7 * p[:, 0]) * np.cos(5 * p[:, 2]))
+    pos = mesh.positions
+    build_bvh_gpu(ctx, pos[: 9 * 1000], 4, 16)
+    t0 = time.perf_counter()
+    g = build_bvh_gpu(ctx, pos, 4, 16)
+    t_gpu = time.perf_counter() - t0
+    lb = build_bvh_gpu(ctx, pos)
+    c = build_bvh(pos)
+    check_layout(g, np.asarray(pos, np.float32), 4)
+    ratio, ratio_lbvh = sah_cost(g) / sah_cost(c), sah_cost(lb) / sah_cost(c)
+    print(f"{pos.size // 9} tris: clustered GPU build {t_gpu * 1e3:.0f} ms (incl. upload and layout); SAH cost ratio "
+          f"clustered/SAH {ratio:.3f}, LBVH/SAH {ratio_lbvh:.3f}")
+    assert ratio < ratio_lbvh and ratio < 1.2
+    # The path-tracing benchmark's knots: the clustered tree beats the
+    # reference builder's (12 bins) by the SAH measure.
+    knots = S.merge_meshes([m.mesh for m in workloads.knot_scene().meshes]).positions
+    kc = sah_cost(build_bvh(knots))
+    k_ratio, k_lbvh = sah_cost(build_bvh_gpu(ctx, knots, 4, 16)) / kc, sah_cost(build_bvh_gpu(ctx, knots)) / kc
+    print(f"knots: SAH cost ratio clustered/SAH {k_ratio:.3f}, LBVH/SAH {k_lbvh:.3f}")
+    assert k_ratio < 1.0 < k_lbvh
