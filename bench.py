@@ -132,9 +132,15 @@ def bench_closest_hit(args) -> dict:
     b_ray = 24 + hit_bytes + 32 + 64 * P + 48 * T
     achieved = b_ray * n / avg_launch_s / 1e9
     traffic = profiled_traffic("closest_hit_icosphere81920_2e20rays")
+    # Compulsory bytes: each ray in and its hit out (40 B) plus the scene's
+    # nodes and triangles read once (the reference layout: 32 B per node,
+    # 48 B per triangle).
+    compulsory = 40 * n + 32 * stats["node_count"] + 48 * (pos.size // 9)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_ray": round(b_ray, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+                "bytes_per_ray": round(b_ray, 1), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+                "compulsory_bytes_per_launch": int(compulsory),
+                "compulsory_gbs": round(compulsory / avg_launch_s / 1e9, 1)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -343,20 +349,32 @@ def bench_denoise(args) -> dict:
     b_px = 44 + 5 * 48  # variance pass + 5 iterations, compulsory bytes per pixel
     achieved = b_px * px / (ms / 1e3) / 1e9
     cpu = None
+    error = None
     if not args.no_cpu_baseline:
         import oracle as O
         film = r.film.read()
         cam = O.camera13(r.scene.camera)
         t0 = time.perf_counter()
-        O.denoise_window(film["color"], film["albedo"], film["normal"], film["history"], cam, 4.0, 128.0, 1.0, 5,
-                         720, 405, 480, 270)
+        ref, _ = O.denoise_window(film["color"], film["albedo"], film["normal"], film["history"], cam, 4.0, 128.0,
+                                  1.0, 5, 720, 405, 480, 270)
         dt = time.perf_counter() - t0
         cpu = {"value": round(dt * 1e3 * 16, 1), "unit": "ms per 1920x1080 denoise (extrapolated)", "cores": 1,
                "kind": "port", "sample": f"FP64 oracle of the same filter on a 480x270 window (1/16 of the frame): "
                                          f"{dt * 1e3:.1f} ms, x16"}
+        # Error of the GPU result (FP16 guides, FP32 taps) against the FP64
+        # filter on the window's pixels beyond the filter's reach (62 px)
+        # from its edges: relative to |ref| + 1e-3 of the mean.
+        m = 62
+        g = r.read_denoised()["illum_var"][405 + m:405 + 270 - m, 720 + m:720 + 480 - m, :3].astype(np.float64)
+        rr = ref[m:270 - m, m:480 - m, :3]
+        e = np.abs(g - rr) / (np.abs(rr) + 1e-3 * np.abs(rr).mean() + 1e-9)
+        error = {"p99_9_rel": float(np.quantile(e, 0.999)), "rms_rel": float(np.sqrt((e ** 2).mean())),
+                 "max_rel": float(e.max()), "pixels": int(e.size // 3),
+                 "against": "FP64 oracle of the same filter on the same stored film (480x270 window interior)"}
     r.close()
     ctx.close()
     return {"metric": "denoise ms @1920x1080, 8 spp, 5 a-trous iterations", "value": round(ms, 4), "unit": "ms",
+            "error_vs_fp64_oracle": error,
             "higher_is_better": False,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_pixel": b_px,
@@ -896,6 +914,9 @@ def main() -> int:
             extra["adjoint"] = bench_adjoint(args)
             extra["frame_graph"] = bench_frame_graph(args)
             extra["env_sampler"] = bench_env_sampler(args)
+            cp = bench_copy(argparse.Namespace(steps=10, warmup=2))
+            extra["streaming_copy"] = {"value": cp["value"], "unit": cp["unit"], "bytes_per_copy": 1 << 30,
+                                       "note": "measured device-to-device copy bandwidth, beside the 8 TB/s peak"}
     line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"), "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
             "higher_is_better": res.pop("higher_is_better", True), "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
