@@ -366,14 +366,15 @@ __device__ __forceinline__ void tile_store(const TileRegs<S>& r, float4* sc, flo
   }
 }
 
-// Persistent: each block filters a run of tiles, loading the next tile's
-// pixels into registers while it filters the current one from LDS, so the
-// global-memory latency of the staging hides behind the taps (a block that
-// loaded, synchronised and then filtered exposed that latency once per
-// tile).  Tiles are dealt in the XCD-aware order: workgroups go to the 8
-// XCDs round robin (b mod 8); XCD k's blocks take interleaved tiles of a
-// contiguous run of the order, so the halos that neighbouring tiles share
-// stay in its own L2.
+// Persistent: each block filters a run of tiles, staging each (global loads
+// into registers, then LDS) before it filters it.  The staging latency
+// hides behind other blocks' taps: without a register copy of the next
+// tile held across the taps the kernels need 74-95 VGPRs, so 5 waves per
+// SIMD fit where LDS allows (a prefetching version needed 110-139 and ran
+// 3-4 waves, 3.5 % slower).  Tiles are dealt in the XCD-aware order:
+// workgroups go to the 8 XCDs round robin (b mod 8); XCD k's blocks take
+// interleaved tiles of a contiguous run of the order, so the halos that
+// neighbouring tiles share stay in its own L2.
 #ifndef LUCENT_ATROUS_WAVES_PER_EU
 #define LUCENT_ATROUS_WAVES_PER_EU 1
 #endif
@@ -404,19 +405,17 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(LUC
   const uint32_t blocks_in_xcd = mine;
   uint32_t t = first + b / kXcds;
   if (t >= end) return;
-  TileRegs<S> regs;
   int x0, y0;
-  tile_origin<S>(t, w, h, x0, y0);
-  tile_load<S>(regs, in, guides, var_in, w, h, x0, y0);
   for (;;) {
-    tile_store<S>(regs, sc, sg, sl, sv);
+    tile_origin<S>(t, w, h, x0, y0);
+    {
+      TileRegs<S> regs;
+      tile_load<S>(regs, in, guides, var_in, w, h, x0, y0);
+      tile_store<S>(regs, sc, sg, sl, sv);
+    }
     __syncthreads();
     const int cx0 = x0, cy0 = y0;
     const uint32_t next = t + blocks_in_xcd;
-    if (next < end) {
-      tile_origin<S>(next, w, h, x0, y0);
-      tile_load<S>(regs, in, guides, var_in, w, h, x0, y0);
-    }
     const int x = cx0 + lane;
     for (int r = wave; r < T::R; r += kWaves) {
       const int y = cy0 + r * S;
