@@ -185,8 +185,9 @@ __device__ inline void centre_begin(Centre& c, float4 cp, float4 gp, float var_b
   c.row[3] = a0 + ay;
   c.row[4] = a0 + 2.0f * ay;
   c.d = gp.w * a0;
-  // footprint = 2 tan(fov/2) depth / H with depth = tau_p |v_p|.
-  const float depth = gp.w * sqrtf(vx * vx + vy * vy + vz * vz);
+  // footprint = 2 tan(fov/2) depth / H with depth = tau_p |v_p|.  The
+  // square roots here are the hardware's (1 ulp, as GLSL's sqrt allows).
+  const float depth = gp.w * __builtin_amdgcn_sqrtf(vx * vx + vy * vy + vz * vz);
   const float footprint = 2.0f * cam.tan_half_fov * depth * inv_h;
   const float base = sigma_p * footprint * float(step);
   c.irl[0] = kLog2e * rcp(base * 1.0f + 1e-6f);
@@ -194,7 +195,7 @@ __device__ inline void centre_begin(Centre& c, float4 cp, float4 gp, float var_b
   c.irl[2] = kLog2e * rcp(base * 2.0f + 1e-6f);
   c.irl[3] = kLog2e * rcp(base * 2.23606798f + 1e-6f);
   c.irl[4] = kLog2e * rcp(base * 2.82842712f + 1e-6f);
-  c.il = kLog2e * rcp(sigma_l * sqrtf(fmaxf(var_blur, 0.0f)) + 1e-6f);
+  c.il = kLog2e * rcp(sigma_l * __builtin_amdgcn_sqrtf(fmaxf(var_blur, 0.0f)) + 1e-6f);
   c.lp = lum3(cp.x, cp.y, cp.z);
   constexpr float h0 = 0.375f;
   c.sr = cp.x * (h0 * h0);
@@ -271,7 +272,7 @@ __device__ inline float blur_variance(const float v[3][3], int x, int y, int w, 
   for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) var += v[dy][dx] * (kx[dx] * ky[dy]);
-  return var / ((kx[0] + kx[1] + kx[2]) * (ky[0] + ky[1] + ky[2]));
+  return var * rcp((kx[0] + kx[1] + kx[2]) * (ky[0] + ky[1] + ky[2]));
 }
 
 // Tile of the LDS kernel: 64 consecutive columns times R rows spaced by the
