@@ -357,6 +357,12 @@ static Status cluster_and_emit(hipStream_t st, const float* d_pos, const uint32_
     if ((s = stream_sync(st)) != LUCENT_OK) break;
     const uint64_t sum = tail[0] + tail[1];
     const uint32_t next = uint32_t(sum & 0xFFFFFFFFu), made = uint32_t(sum >> 32);
+    if (made == 0 && !forced) {
+      // No mutual pair (possible only when areas do not order strictly,
+      // e.g. NaN coordinates): the list is unchanged; pair neighbours.
+      forced = true;
+      continue;
+    }
     if (made == 0 || next >= c) {
       s = fail(LUCENT_E_OVERFLOW, "cluster round merged nothing");
       break;
