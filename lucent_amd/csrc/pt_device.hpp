@@ -342,6 +342,17 @@ __device__ inline uint32_t link_count(uint32_t l) { return ((l >> 26) & 31u) + 1
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 __device__ inline lds_u32* as_lds(uint32_t* p) { return (lds_u32*)p; }
 
+// A float4 of device memory read as a global load: the scene's arrays reach
+// the kernels through pointers read from memory, which the compiler would
+// otherwise access with flat instructions (counted on the LDS counter too,
+// and addressed by 64-bit per-lane sums).
+typedef float gf32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const gf32x4 global_f4;
+__device__ __forceinline__ float4 ld_global(const float4* p) {
+  const gf32x4 v = *(global_f4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 template <int kLds, int kBlock>
 struct Stack4 {
   lds_u32* lds;     // this lane's word of plane 0; plane 1 starts kLds * kBlock words later
@@ -424,7 +435,8 @@ __device__ inline void node4_hits(const float* nodes, uint32_t n, v3 o, v3 inv, 
 __device__ inline void node4_hits_f32(const float* nodes, uint32_t n, v3 o, v3 inv, float tmax, float tn[4],
                                       uint32_t link[4]) {
   const float4* r = reinterpret_cast<const float4*>(nodes) + size_t(n) * 8;
-  const float4 A = r[0], lx = r[1], hx = r[2], ly = r[3], hy = r[4], lz = r[5], hz = r[6], L = r[7];
+  const float4 A = ld_global(r), lx = ld_global(r + 1), hx = ld_global(r + 2), ly = ld_global(r + 3),
+               hy = ld_global(r + 4), lz = ld_global(r + 5), hz = ld_global(r + 6), L = ld_global(r + 7);
   link[0] = __float_as_uint(L.x);
   link[1] = __float_as_uint(L.y);
   link[2] = __float_as_uint(L.z);
@@ -477,7 +489,8 @@ __device__ __forceinline__ bool test_next_triangle(const PtScene& s, Trav4& t, R
   const uint32_t k = t.la++;
   --t.na;
   const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris);
-  const float4 p0 = tris[size_t(k) * 3], p1 = tris[size_t(k) * 3 + 1], p2 = tris[size_t(k) * 3 + 2];
+  const float4* tp = tris + size_t(k) * 3;
+  const float4 p0 = ld_global(tp), p1 = ld_global(tp + 1), p2 = ld_global(tp + 2);
   float th, u, v;
   if (hit_triangle(t.ro, t.rd, mk(p0), mk(p1), mk(p2), t.tmax, th, u, v) &&
       (__float_as_int(p2.w) == 0 || alpha_opaque(s, k, u, v, __float_as_int(p0.w), rng))) {
