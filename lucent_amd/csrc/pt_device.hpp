@@ -20,6 +20,26 @@ constexpr float EPS = 1e-6f;
 constexpr float FAR = 1e4f;
 constexpr float MIN_SHEEN_ROUGHNESS = 0.07f;
 
+// Scene data read as global loads: the scene's arrays reach the kernels
+// through pointers read from memory, which the compiler would otherwise
+// access with flat instructions (counted on the LDS counter too, so that
+// LDS waits also wait for them).  Global<T> indexes such an array.
+typedef float gf32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const gf32x4 global_f4;
+typedef __attribute__((address_space(1))) const float global_f32;
+typedef __attribute__((address_space(1))) const uint32_t global_u32;
+__device__ __forceinline__ float4 ld_global(const float4* p) {
+  const gf32x4 v = *(global_f4*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float ld_global(const float* p) { return *(global_f32*)p; }
+__device__ __forceinline__ uint32_t ld_global(const uint32_t* p) { return *(global_u32*)p; }
+template <class T>
+struct Global {
+  const T* p;
+  __device__ __forceinline__ T operator[](size_t i) const { return ld_global(p + i); }
+};
+
 // ---------------------------------------------------------------------------
 // Small vector algebra with GLSL meaning (normalize = v * inversesqrt(v.v)).
 // ---------------------------------------------------------------------------
@@ -147,7 +167,7 @@ __device__ inline float srgb_to_linear(float c) {
 __device__ inline float4 texel_rgba8(const uint32_t* maps, int size, int layer, int x, int y, bool srgb) {
   x = ((x % size) + size) % size;
   y = ((y % size) + size) % size;
-  const uint32_t p = maps[(size_t(layer) * size + y) * size + x];
+  const uint32_t p = ld_global(maps + (size_t(layer) * size + y) * size + x);
   float4 c = make_float4(float(p & 255u) / 255.0f, float((p >> 8) & 255u) / 255.0f, float((p >> 16) & 255u) / 255.0f,
                          float(p >> 24) / 255.0f);
   if (srgb) {
@@ -193,7 +213,7 @@ struct Material {
 };
 
 __device__ inline Material fetch_material(const PtScene& s, int index) {
-  const float4* m = reinterpret_cast<const float4*>(s.materials) + size_t(index) * kMaterialTexels;
+  const Global<float4> m{reinterpret_cast<const float4*>(s.materials) + size_t(index) * kMaterialTexels};
   const float4 t0 = m[0], t1 = m[1], t2 = m[2], t3 = m[3], t4 = m[4], t5 = m[5], t6 = m[6], t7 = m[7], t8 = m[8],
                t9 = m[9];
   Material r;
@@ -284,7 +304,7 @@ __device__ inline float hit_ground(const PtScene& s, v3 ro, v3 rd, float tmax) {
 // Interpolated uv of a leaf slot (triangleUV, scene.glsl:271-281).
 __device__ inline void triangle_uvs(const PtScene& s, uint32_t slot, float& u0, float& v0, float& u1, float& v1,
                                     float& u2, float& v2) {
-  const float4* a = reinterpret_cast<const float4*>(s.bvh.attrs) + size_t(slot) * 4;
+  const Global<float4> a{reinterpret_cast<const float4*>(s.bvh.attrs) + size_t(slot) * 4};
   const float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
   u0 = a0.w;
   v0 = a3.x;
@@ -300,7 +320,7 @@ __device__ inline bool alpha_opaque(const PtScene& s, uint32_t slot, float bu, f
   if (s.bvh.attrs) triangle_uvs(s, slot, u0, v0, u1, v1, u2, v2);
   const float w = 1.0f - bu - bv;
   const float uu = u0 * w + u1 * bu + u2 * bv, vv = v0 * w + v1 * bu + v2 * bv;
-  const float4* m = reinterpret_cast<const float4*>(s.materials) + size_t(mat) * kMaterialTexels;
+  const Global<float4> m{reinterpret_cast<const float4*>(s.materials) + size_t(mat) * kMaterialTexels};
   float alpha = m[2].w;
   const int mode = int(m[3].w);
   const int map = int(m[4].x);
@@ -341,17 +361,6 @@ __device__ inline uint32_t link_count(uint32_t l) { return ((l >> 26) & 31u) + 1
 // the LDS and the vector-memory counters.)
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 __device__ inline lds_u32* as_lds(uint32_t* p) { return (lds_u32*)p; }
-
-// A float4 of device memory read as a global load: the scene's arrays reach
-// the kernels through pointers read from memory, which the compiler would
-// otherwise access with flat instructions (counted on the LDS counter too,
-// and addressed by 64-bit per-lane sums).
-typedef float gf32x4 __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) const gf32x4 global_f4;
-__device__ __forceinline__ float4 ld_global(const float4* p) {
-  const gf32x4 v = *(global_f4*)p;
-  return make_float4(v.x, v.y, v.z, v.w);
-}
 
 template <int kLds, int kBlock>
 struct Stack4 {
@@ -708,7 +717,7 @@ __device__ inline bool build_hit(const PtScene& s, v3 ro, v3 rd, const HitRecord
     return true;
   }
   const float bu = r.u, bv = r.v;
-  const float4* tris = reinterpret_cast<const float4*>(s.bvh.tris) + size_t(r.prim) * 3;
+  const Global<float4> tris{reinterpret_cast<const float4*>(s.bvh.tris) + size_t(r.prim) * 3};
   const float4 p0 = tris[0], p1 = tris[1], p2 = tris[2];
   v3 n0 = mk(0.f, 0.f, 0.f), n1 = n0, n2 = n0;
   float u0 = 0, v0 = 0, u1 = 0, v1 = 0, u2 = 0, v2 = 0;
@@ -717,7 +726,7 @@ __device__ inline bool build_hit(const PtScene& s, v3 ro, v3 rd, const HitRecord
   v3 ns;
   const float w = 1.0f - bu - bv;
   if (s.bvh.attrs) {
-    const float4* a = reinterpret_cast<const float4*>(s.bvh.attrs) + size_t(r.prim) * 4;
+    const Global<float4> a{reinterpret_cast<const float4*>(s.bvh.attrs) + size_t(r.prim) * 4};
     const float4 a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
     n0 = mk(a0);
     n1 = mk(a1);
@@ -820,7 +829,7 @@ __device__ inline v3 env_lookup(const PtScene& s, float u, float v) {
   if (x0 < 0) x0 += s.env_w;
   const int x1 = x0 + 1 == s.env_w ? 0 : x0 + 1;
   const int y0 = min(max(int(fy), 0), s.env_h - 1), y1 = min(max(int(fy) + 1, 0), s.env_h - 1);
-  const float4* m = reinterpret_cast<const float4*>(s.env_map);
+  const Global<float4> m{reinterpret_cast<const float4*>(s.env_map)};
   const float4 a = m[size_t(y0) * s.env_w + x0], b = m[size_t(y0) * s.env_w + x1];
   const float4 c = m[size_t(y1) * s.env_w + x0], d = m[size_t(y1) * s.env_w + x1];
   const v3 top = mk(a) * (1.0f - tx) + mk(b) * tx;
@@ -840,13 +849,14 @@ __device__ inline float env_pdf(const PtScene& s, v3 d) {
   equirect_uv(l, u, v);
   const int cx = min(int(u * float(s.grid_w)), s.grid_w - 1);
   const int cy = min(int(v * float(s.grid_h)), s.grid_h - 1);
-  return reinterpret_cast<const float4*>(s.env_table)[size_t(cy) * s.grid_w + cx].z / (2.0f * PI * PI * st);
+  return ld_global(reinterpret_cast<const float4*>(s.env_table) + size_t(cy) * s.grid_w + cx).z /
+         (2.0f * PI * PI * st);
 }
 __device__ inline LightSample sample_environment(const PtScene& s, Rng& rng) {
   LightSample ls;
   const int n = s.grid_w * s.grid_h;
   const int i = min(int(rng.rand() * float(n)), n - 1);
-  const float4 e = reinterpret_cast<const float4*>(s.env_table)[i];
+  const float4 e = ld_global(reinterpret_cast<const float4*>(s.env_table) + i);
   int cell = i;
   float pdf_uv = e.z;
   if (rng.rand() >= e.x) {
@@ -1004,7 +1014,7 @@ __device__ inline float sheen_albedo(const PtScene& s, float ndv, float r) {
   const float tx = x - flx, ty = y - fly;
   const int x0 = min(max(int(flx), 0), kSheenLutSize - 1), x1 = min(max(int(flx) + 1, 0), kSheenLutSize - 1);
   const int y0 = min(max(int(fly), 0), kSheenLutSize - 1), y1 = min(max(int(fly) + 1, 0), kSheenLutSize - 1);
-  const float* L = s.sheen_lut;
+  const Global<float> L{s.sheen_lut};
   const float top = L[y0 * kSheenLutSize + x0] * (1.0f - tx) + L[y0 * kSheenLutSize + x1] * tx;
   const float bot = L[y1 * kSheenLutSize + x0] * (1.0f - tx) + L[y1 * kSheenLutSize + x1] * tx;
   return top * (1.0f - ty) + bot * ty;
