@@ -223,6 +223,22 @@ constexpr int kEnvSlots = 1 << LUCENT_ADJ_ENV_SLOTS_LOG2;
 constexpr uint32_t kFreeSlot = 0xFFFFFFFFu;
 __device__ inline uint32_t env_slot(uint32_t texel) { return (texel * 2654435761u) >> (32 - LUCENT_ADJ_ENV_SLOTS_LOG2); }
 
+// FP64 atomic additions with the memory named: the sink's pointers are read
+// from memory, so left generic they would compile to flat atomics (counted
+// on the LDS counter as well).  Both are the hardware's FP64 add.
+__device__ __forceinline__ void add_global_f64(double* p, double v) {
+  [[clang::atomic(ignore_denormal_mode)]] {
+    __hip_atomic_fetch_add((__attribute__((address_space(1))) double*)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+__device__ __forceinline__ void add_lds_f64(double* p, double v) {
+  [[clang::atomic(ignore_denormal_mode)]] {
+    __hip_atomic_fetch_add((__attribute__((address_space(3))) double*)p, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
 struct GradSink {
   double* lds;  // kLdsMaterials * 5 (block-shared)
   double* env;  // kEnvReplicas replicas of env_w * env_h * 3
@@ -238,11 +254,11 @@ __device__ __forceinline__ void add_material(GradSink& g, int mat, const float v
   if (mat < kLdsMaterials) {
   #pragma unroll
   for (int i = 0; i < kNumParams; ++i)
-      if (v[i] != 0.0f) atomicAdd(&g.lds[mat * kNumParams + i], double(v[i]));
+      if (v[i] != 0.0f) add_lds_f64(&g.lds[mat * kNumParams + i], double(v[i]));
   } else {
   #pragma unroll
   for (int i = 0; i < kNumParams; ++i)
-      if (v[i] != 0.0f) unsafeAtomicAdd(&g.global_mat[mat * kNumParams + i], double(v[i]));
+      if (v[i] != 0.0f) add_global_f64(&g.global_mat[mat * kNumParams + i], double(v[i]));
   }
 }
 
@@ -263,9 +279,9 @@ __device__ __forceinline__ void add_env(const PtScene& sc, GradSink& g, float u,
       if (owner == kFreeSlot) owner = texel;
     }
     if (owner == texel) {
-      for (int c = 0; c < 3; ++c) atomicAdd(&g.env_val[slot * 3 + c], v[c]);
+      for (int c = 0; c < 3; ++c) add_lds_f64(&g.env_val[slot * 3 + c], v[c]);
     } else {
-      for (int c = 0; c < 3; ++c) unsafeAtomicAdd(&base[size_t(texel) * 3 + c], v[c]);
+      for (int c = 0; c < 3; ++c) add_global_f64(&base[size_t(texel) * 3 + c], v[c]);
     }
   }
 }
@@ -277,7 +293,7 @@ __device__ inline void flush_env(GradSink& g, int lane, int lanes) {
   for (int slot = lane; slot < kEnvSlots; slot += lanes) {
     const uint32_t texel = g.env_tag[slot];
     if (texel == kFreeSlot) continue;
-    for (int c = 0; c < 3; ++c) unsafeAtomicAdd(&base[size_t(texel) * 3 + c], g.env_val[slot * 3 + c]);
+    for (int c = 0; c < 3; ++c) add_global_f64(&base[size_t(texel) * 3 + c], g.env_val[slot * 3 + c]);
   }
 }
 
