@@ -233,8 +233,8 @@ __device__ __forceinline__ uint32_t run_item(const TileItem& it, uint32_t job, u
     }
     if (active && !pt::path_step(it.scene, f, p, rng, st)) {
       if (it.sample_radiance)
-        reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
-            make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
+        pt::st_global(reinterpret_cast<float4*>(it.sample_radiance) + size_t(k) * f.width * f.height + idx,
+            make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f));
       pt::accumulate(it.film, idx, p.radiance, p.guide);
       if (++k >= it.spp)
         active = false;
@@ -386,7 +386,7 @@ struct FusedPath {
       const pt::v3 ro = p.ro, rd = pt::mk(d);
       bool blocked = false;
       for (uint32_t i = 0; i < sc.sphere_count && !blocked; ++i)
-        blocked = pt::hit_sphere(ro, rd, reinterpret_cast<const float4*>(sc.spheres)[i], d.w) < d.w;
+        blocked = pt::hit_sphere(ro, rd, pt::ld_global(reinterpret_cast<const float4*>(sc.spheres) + i), d.w) < d.w;
       if (!blocked) blocked = pt::hit_ground(sc, ro, rd, d.w) < d.w;
       if (!blocked) {
         pt::trav4_begin(sc, t, ro, rd, d.w, st);
@@ -504,8 +504,8 @@ __device__ __forceinline__ uint32_t run_item_fused(const TileItem& it, uint32_t 
       if (lp.state == kLaneShade) lp.shade(f);
       if (lp.state == kLaneFinish) {
         if (it.sample_radiance)
-          reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
-              make_float4(lp.p.radiance.x, lp.p.radiance.y, lp.p.radiance.z, 0.0f);
+          pt::st_global(reinterpret_cast<float4*>(it.sample_radiance) + size_t(k) * f.width * f.height + idx,
+              make_float4(lp.p.radiance.x, lp.p.radiance.y, lp.p.radiance.z, 0.0f));
         pt::accumulate(it.film, idx, lp.p.radiance, lp.p.guide);
         if (++k >= it.spp)
           lp.state = kLaneIdle;
@@ -589,8 +589,8 @@ __device__ __forceinline__ void run_units_fused(const TileItem& it, uint32_t job
       if (lp.state == kLaneFinish) {
         const pt::PathState& p = lp.p;
         if (it.sample_radiance)
-          reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
-              make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
+          pt::st_global(reinterpret_cast<float4*>(it.sample_radiance) + size_t(k) * f.width * f.height + idx,
+              make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f));
         float4* r = records + (size_t(rslot) * spp + k) * kRecordFloat4;
         r[0] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, p.guide.depth);
         r[1] = make_float4(p.guide.albedo.x, p.guide.albedo.y, p.guide.albedo.z, p.guide.coverage);
@@ -701,8 +701,8 @@ __device__ __forceinline__ uint32_t run_units_ring(const TileItem& it, uint32_t 
       if (lp.state == kLaneFinish) {
         const pt::PathState& p = lp.p;
         if (it.sample_radiance)
-          reinterpret_cast<float4*>(it.sample_radiance)[size_t(k) * f.width * f.height + idx] =
-              make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f);
+          pt::st_global(reinterpret_cast<float4*>(it.sample_radiance) + size_t(k) * f.width * f.height + idx,
+              make_float4(p.radiance.x, p.radiance.y, p.radiance.z, 0.0f));
         float4* r = records + (size_t(rslot) * s_max + k) * kRecordFloat4;
         r[0] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, p.guide.depth);
         r[1] = make_float4(p.guide.albedo.x, p.guide.albedo.y, p.guide.albedo.z, p.guide.coverage);

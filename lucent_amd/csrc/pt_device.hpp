@@ -34,6 +34,17 @@ __device__ __forceinline__ float4 ld_global(const float4* p) {
 }
 __device__ __forceinline__ float ld_global(const float* p) { return *(global_f32*)p; }
 __device__ __forceinline__ uint32_t ld_global(const uint32_t* p) { return *(global_u32*)p; }
+__device__ __forceinline__ int32_t ld_global(const int32_t* p) {
+  return *(__attribute__((address_space(1))) const int32_t*)p;
+}
+__device__ __forceinline__ void st_global(float4* p, float4 v) {
+  gf32x4 w;
+  w.x = v.x;
+  w.y = v.y;
+  w.z = v.z;
+  w.w = v.w;
+  *(__attribute__((address_space(1))) gf32x4*)p = w;
+}
 template <class T>
 struct Global {
   const T* p;
@@ -648,7 +659,7 @@ __device__ inline int trace_bvh(const PtScene& s, v3 ro, v3 rd, float& closest, 
 template <class StackT>
 __device__ inline bool occluded(const PtScene& s, v3 ro, v3 rd, float tmax, Rng& rng, StackT& st) {
   for (uint32_t i = 0; i < s.sphere_count; ++i) {
-    const float4 sp = reinterpret_cast<const float4*>(s.spheres)[i];
+    const float4 sp = ld_global(reinterpret_cast<const float4*>(s.spheres) + i);
     if (hit_sphere(ro, rd, sp, tmax) < tmax) return true;
   }
   if (hit_ground(s, ro, rd, tmax) < tmax) return true;
@@ -674,7 +685,7 @@ __device__ inline void search_analytic(const PtScene& s, v3 ro, v3 rd, HitRecord
   r.prim = 0;
   r.u = r.v = 0.0f;
   for (uint32_t i = 0; i < s.sphere_count; ++i) {
-    const float t = hit_sphere(ro, rd, reinterpret_cast<const float4*>(s.spheres)[i], r.t);
+    const float t = hit_sphere(ro, rd, ld_global(reinterpret_cast<const float4*>(s.spheres) + i), r.t);
     if (t < r.t) {
       r.t = t;
       r.kind = kHitSphere;
@@ -707,13 +718,13 @@ __device__ inline bool build_hit(const PtScene& s, v3 ro, v3 rd, const HitRecord
     return true;
   }
   if (r.kind == kHitSphere) {
-    const float4 sp = reinterpret_cast<const float4*>(s.spheres)[r.prim];
+    const float4 sp = ld_global(reinterpret_cast<const float4*>(s.spheres) + r.prim);
     const v3 n = (hit.position - mk(sp)) / sp.w;
     hit.front_face = dot(rd, n) < 0.0f;
     hit.normal = hit.front_face ? n : -n;
     hit.geom_normal = hit.normal;
-    hit.material = s.sphere_material[r.prim];
-    hit.light = s.sphere_light[r.prim];
+    hit.material = ld_global(s.sphere_material + r.prim);
+    hit.light = ld_global(s.sphere_light + r.prim);
     return true;
   }
   const float bu = r.u, bv = r.v;
@@ -1459,15 +1470,21 @@ struct PixelAccum {
   float n;
 };
 
+// The film's targets, read and written as global memory (as the scene's
+// arrays): four halves travel as two 32-bit words, low half first.
+typedef uint32_t gu32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) gu32x2 global_u32x2;
+typedef __attribute__((address_space(1))) float global_f32_rw;
+
 __device__ inline PixelAccum load_accum(const PtFilm& film, uint32_t idx) {
-  auto h2f = [](unsigned short h) { return __half2float(__ushort_as_half(h)); };
+  auto h2f = [](uint32_t h) { return __half2float(__ushort_as_half((unsigned short)h)); };
   PixelAccum a;
-  a.color = reinterpret_cast<const float4*>(film.color)[idx];
-  const ushort4 pa = reinterpret_cast<const ushort4*>(film.albedo)[idx];
-  const ushort4 pn = reinterpret_cast<const ushort4*>(film.normal)[idx];
-  a.albedo = make_float4(h2f(pa.x), h2f(pa.y), h2f(pa.z), h2f(pa.w));
-  a.normal = make_float4(h2f(pn.x), h2f(pn.y), h2f(pn.z), h2f(pn.w));
-  a.n = film.history[idx];
+  a.color = ld_global(reinterpret_cast<const float4*>(film.color) + idx);
+  const gu32x2 pa = *(const global_u32x2*)(reinterpret_cast<const uint2*>(film.albedo) + idx);
+  const gu32x2 pn = *(const global_u32x2*)(reinterpret_cast<const uint2*>(film.normal) + idx);
+  a.albedo = make_float4(h2f(pa.x & 0xFFFFu), h2f(pa.x >> 16), h2f(pa.y & 0xFFFFu), h2f(pa.y >> 16));
+  a.normal = make_float4(h2f(pn.x & 0xFFFFu), h2f(pn.x >> 16), h2f(pn.y & 0xFFFFu), h2f(pn.y >> 16));
+  a.n = ld_global(film.history + idx);
   return a;
 }
 
@@ -1499,13 +1516,16 @@ __device__ inline void fold_sample(PixelAccum& a, v3 color, const Guide& g) {
 }
 
 __device__ inline void store_accum(const PtFilm& film, uint32_t idx, const PixelAccum& a) {
-  auto f2h = [](float x) { return __half_as_ushort(__float2half_rn(x)); };
-  reinterpret_cast<float4*>(film.color)[idx] = a.color;
-  reinterpret_cast<ushort4*>(film.albedo)[idx] =
-      make_ushort4(f2h(a.albedo.x), f2h(a.albedo.y), f2h(a.albedo.z), f2h(a.albedo.w));
-  reinterpret_cast<ushort4*>(film.normal)[idx] =
-      make_ushort4(f2h(a.normal.x), f2h(a.normal.y), f2h(a.normal.z), f2h(a.normal.w));
-  film.history[idx] = a.n;
+  auto f2h = [](float x) { return uint32_t(__half_as_ushort(__float2half_rn(x))); };
+  st_global(reinterpret_cast<float4*>(film.color) + idx, a.color);
+  gu32x2 pa, pn;
+  pa.x = f2h(a.albedo.x) | (f2h(a.albedo.y) << 16);
+  pa.y = f2h(a.albedo.z) | (f2h(a.albedo.w) << 16);
+  pn.x = f2h(a.normal.x) | (f2h(a.normal.y) << 16);
+  pn.y = f2h(a.normal.z) | (f2h(a.normal.w) << 16);
+  *(global_u32x2*)(reinterpret_cast<uint2*>(film.albedo) + idx) = pa;
+  *(global_u32x2*)(reinterpret_cast<uint2*>(film.normal) + idx) = pn;
+  *(global_f32_rw*)(film.history + idx) = a.n;
 }
 
 __device__ inline void accumulate(const PtFilm& film, uint32_t idx, v3 color, const Guide& g) {
