@@ -379,23 +379,46 @@ struct Stack4 {
   uint32_t* spill;  // 2 words per entry
   uint32_t sp, base;
   uint32_t top, bot;  // sp % kLds, base % kLds
+  static constexpr bool kPow2 = (kLds & (kLds - 1)) == 0;
+  __device__ static inline uint32_t next(uint32_t i) {
+    if constexpr (kPow2)
+      return (i + 1u) & uint32_t(kLds - 1);
+    else
+      return i + 1u == uint32_t(kLds) ? 0u : i + 1u;
+  }
+  __device__ static inline uint32_t prev(uint32_t i) {
+    if constexpr (kPow2)
+      return (i - 1u) & uint32_t(kLds - 1);
+    else
+      return i == 0u ? uint32_t(kLds - 1) : i - 1u;
+  }
   __device__ inline void reset() { sp = base = top = bot = 0; }
   __device__ inline bool empty() const { return sp == 0; }
-  __device__ inline void push(uint32_t link, float dist) {
-    if (sp - base == kLds) {
+  // Room in the ring for n more entries: the oldest entries move to the
+  // spill area (the same entries, in the same order, that n pushes that
+  // each spill when the ring is full would move; n <= kLds).
+  __device__ inline void reserve(uint32_t n) {
+    while (sp - base + n > uint32_t(kLds)) {
       spill[2 * base] = lds[bot * kBlock];
       spill[2 * base + 1] = lds[(kLds + bot) * kBlock];
       ++base;
-      bot = bot + 1 == kLds ? 0 : bot + 1;
+      bot = next(bot);
     }
+  }
+  // A push into a ring with room for it.
+  __device__ inline void push_reserved(uint32_t link, float dist) {
     lds[top * kBlock] = link;
     lds[(kLds + top) * kBlock] = __float_as_uint(dist);
     ++sp;
-    top = top + 1 == kLds ? 0 : top + 1;
+    top = next(top);
+  }
+  __device__ inline void push(uint32_t link, float dist) {
+    reserve(1);
+    push_reserved(link, dist);
   }
   __device__ inline void pop(uint32_t& link, float& dist) {
     --sp;
-    top = top == 0 ? kLds - 1 : top - 1;
+    top = prev(top);
     if (sp >= base) {
       link = lds[top * kBlock];
       dist = __uint_as_float(lds[(kLds + top) * kBlock]);
@@ -523,6 +546,38 @@ __device__ __forceinline__ bool test_next_triangle(const PtScene& s, Trav4& t, R
   return false;
 }
 
+// Pushes the children whose keys are not kMissKey in descending key order
+// (the key's two low bits are the child index, so keys are distinct): a
+// sorting network of compare-exchanges that carry each key's link, then one
+// check for ring room for all of them.
+template <class StackT>
+__device__ __forceinline__ void push_sorted(StackT& st, uint32_t k[4], uint32_t l[4]) {
+  auto cx = [&](int a, int b) {
+    const bool swap = k[a] < k[b];
+    const uint32_t ka = k[a], kb = k[b], la = l[a], lb = l[b];
+    k[a] = swap ? kb : ka;
+    k[b] = swap ? ka : kb;
+    l[a] = swap ? lb : la;
+    l[b] = swap ? la : lb;
+  };
+  cx(0, 1);
+  cx(2, 3);
+  cx(0, 2);
+  cx(1, 3);
+  cx(1, 2);
+  bool hit[4];
+  uint32_t n = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    hit[i] = (k[i] & ~3u) != kMissKey;
+    n += uint32_t(hit[i]);
+  }
+  st.reserve(n);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (hit[i]) st.push_reserved(l[i], __uint_as_float(k[i] & ~3u));
+}
+
 // One step: with no leaf pending, pop an entry and visit it (a node's
 // children are pushed; a leaf's first triangle is tested in the same step),
 // else test the pending leaf's next triangle.  For shadow rays (kAny) true
@@ -543,38 +598,23 @@ __device__ inline bool trav4_step(const PtScene& s, Trav4& t, StackT& st, Rng& r
       node4_hits_f32(s.bvh.nodes4f, link, t.ro, t.inv, t.tmax, tn, l);
 #endif
       if (kAny) {
+        st.reserve(uint32_t(tn[0] != INF) + uint32_t(tn[1] != INF) + uint32_t(tn[2] != INF) + uint32_t(tn[3] != INF));
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          if (tn[i] != INF) st.push(l[i], tn[i]);
+          if (tn[i] != INF) st.push_reserved(l[i], tn[i]);
         return false;
       }
       // Farthest first, so that the nearest child ends on top.  Entry
       // distances are >= 0, so their bit patterns order like the floats: the
       // keys carry the child index in the two low bits (the distance
-      // truncated by up to 3 ulps, which only weakens culling) and sort
-      // descending with a branch-free network of integer min/max; missed
-      // children (kMissKey) sort first and are not pushed.
+      // truncated by up to 3 ulps, which only weakens culling; distinct
+      // keys, so the order is fixed) and sort descending with a branch-free
+      // network that moves each child's link with its key; missed children
+      // (kMissKey) sort first and are not pushed.
       uint32_t k[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) k[i] = (__float_as_uint(tn[i]) & ~3u) | uint32_t(i);
-      auto cx = [&](int a, int b) {
-        const uint32_t lo = min(k[a], k[b]), hi = max(k[a], k[b]);
-        k[a] = hi;
-        k[b] = lo;
-      };
-      cx(0, 1);
-      cx(2, 3);
-      cx(0, 2);
-      cx(1, 3);
-      cx(1, 2);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if ((k[i] & ~3u) != kMissKey) {
-          const uint32_t j = k[i] & 3u;
-          const uint32_t lk = (j & 1u) ? ((j & 2u) ? l[3] : l[1]) : ((j & 2u) ? l[2] : l[0]);
-          st.push(lk, __uint_as_float(k[i] & ~3u));
-        }
-      }
+      push_sorted(st, k, l);
       return false;
     }
     t.la = link_first(link);
@@ -608,29 +648,16 @@ __device__ inline bool trav4_step_mixed(const PtScene& s, Trav4& t, StackT& st, 
       // two low bits (as trav4_step).  Shadow rays: (3 - i) above the index,
       // so the descending sort leaves them in index order; missed children
       // sort first with kMissKey either way and are not pushed.
+      // (A missed child's closest-ray key is kMissKey | i already: its
+      // distance is INF.)
       uint32_t k[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        k[i] = any ? (tn[i] != INF ? ((3u - uint32_t(i)) << 2) | uint32_t(i) : kMissKey | uint32_t(i))
-                   : (__float_as_uint(tn[i]) & ~3u) | uint32_t(i);
-      auto cx = [&](int a, int b) {
-        const uint32_t lo = min(k[a], k[b]), hi = max(k[a], k[b]);
-        k[a] = hi;
-        k[b] = lo;
-      };
-      cx(0, 1);
-      cx(2, 3);
-      cx(0, 2);
-      cx(1, 3);
-      cx(1, 2);
-#pragma unroll
       for (int i = 0; i < 4; ++i) {
-        if ((k[i] & ~3u) != kMissKey) {
-          const uint32_t j = k[i] & 3u;
-          const uint32_t lk = (j & 1u) ? ((j & 2u) ? l[3] : l[1]) : ((j & 2u) ? l[2] : l[0]);
-          st.push(lk, __uint_as_float(k[i] & ~3u));
-        }
+        const uint32_t closest = (__float_as_uint(tn[i]) & ~3u) | uint32_t(i);
+        const uint32_t shadow = ((3u - uint32_t(i)) << 2) | uint32_t(i);
+        k[i] = any && tn[i] != INF ? shadow : closest;
       }
+      push_sorted(st, k, l);
       return false;
     }
     t.la = link_first(link);
