@@ -375,6 +375,7 @@ __device__ inline lds_u32* as_lds(uint32_t* p) { return (lds_u32*)p; }
 
 template <int kLds, int kBlock>
 struct Stack4 {
+  static_assert(kLds >= 4, "a node visit reserves ring room for its four children at once");
   lds_u32* lds;     // this lane's word of plane 0; plane 1 starts kLds * kBlock words later
   uint32_t* spill;  // 2 words per entry
   uint32_t sp, base;

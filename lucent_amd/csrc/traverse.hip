@@ -56,28 +56,41 @@ struct LaneStack {
   lds_word* lds;    // this lane's column of plane 0: entry k at lds[k * kBlock]; plane 1 kLds * kBlock later
   uint32_t* spill;  // this lane's global area
   uint32_t sp = 0, base = 0;
+  uint32_t top = 0, bot = 0;  // sp % kLds and base % kLds, kept incrementally (no division)
 
-  __device__ void push(uint32_t link, float dist) {
-    if (sp - base == kLds) {
-      const uint32_t slot = (base % kLds) * kBlock;
-      spill[2 * base] = lds[slot];
-      spill[2 * base + 1] = lds[kLds * kBlock + slot];
+  __device__ static uint32_t next(uint32_t i) { return i + 1u == uint32_t(kLds) ? 0u : i + 1u; }
+  __device__ static uint32_t prev(uint32_t i) { return i == 0u ? uint32_t(kLds - 1) : i - 1u; }
+  // Room in the ring for n more entries: the oldest move to the spill area
+  // (the entries, in the order, that n pushes spilling one at a time when
+  // the ring is full would move; n <= kLds).
+  __device__ void reserve(uint32_t n) {
+    while (sp - base + n > uint32_t(kLds)) {
+      spill[2 * base] = lds[bot * kBlock];
+      spill[2 * base + 1] = lds[(kLds + bot) * kBlock];
       ++base;
+      bot = next(bot);
     }
-    const uint32_t slot = (sp % kLds) * kBlock;
-    lds[slot] = link;
-    lds[kLds * kBlock + slot] = __float_as_uint(dist);
+  }
+  __device__ void push_reserved(uint32_t link, float dist) {
+    lds[top * kBlock] = link;
+    lds[(kLds + top) * kBlock] = __float_as_uint(dist);
     ++sp;
+    top = next(top);
+  }
+  __device__ void push(uint32_t link, float dist) {
+    reserve(1);
+    push_reserved(link, dist);
   }
   __device__ void pop(uint32_t& link, float& dist) {
     --sp;
+    top = prev(top);
     if (sp >= base) {
-      const uint32_t slot = (sp % kLds) * kBlock;
-      link = lds[slot];
-      dist = __uint_as_float(lds[kLds * kBlock + slot]);
+      link = lds[top * kBlock];
+      dist = __uint_as_float(lds[(kLds + top) * kBlock]);
       return;
     }
     --base;
+    bot = top;
     link = spill[2 * sp];
     dist = __uint_as_float(spill[2 * sp + 1]);
   }
@@ -331,9 +344,17 @@ __global__ __launch_bounds__(kBlock) void k_intersect(DevBvh bvh, const void* __
         cs(1, 3);
         cs(1, 2);
       }
+      if constexpr (kLds >= 4) {
+        st.reserve(uint32_t(tn[0] != INFINITY) + uint32_t(tn[1] != INFINITY) + uint32_t(tn[2] != INFINITY) +
+                   uint32_t(tn[3] != INFINITY));
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (tn[i] != INFINITY) st.push(l[i], tn[i]);
+        for (int i = 0; i < 4; ++i)
+          if (tn[i] != INFINITY) st.push_reserved(l[i], tn[i]);
+      } else {  // a ring smaller than a node's children: room checked per push
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (tn[i] != INFINITY) st.push(l[i], tn[i]);
+      }
     }
     if (best.slot != 0xFFFFFFFFu) {
       t_out[r] = best.t;
