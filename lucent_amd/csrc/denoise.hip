@@ -310,60 +310,90 @@ __device__ inline void tile_origin(uint32_t t, int w, int h, int& x0, int& y0) {
 }
 
 // A thread's share of one tile's staged pixels, held in registers between
-// the global loads and the LDS stores.
+// the global loads and the LDS stores.  Each wave stages whole rows (staged
+// rows wave, wave+4, ...; variance rows likewise), a lane one column of each
+// in runs of 64: the row's index, bounds and base address are the wave's
+// (scalar), and a lane adds only its column.
 template <int S>
 struct TileRegs {
   using T = AtrousTile<S>;
-  static constexpr int kPix = (T::rows * T::cols + kWaves * 64 - 1) / (kWaves * 64);
-  static constexpr int kVar = (T::R * 3 * T::vcols + kWaves * 64 - 1) / (kWaves * 64);
-  float4 c[kPix], g[kPix];
-  float v[kVar];
+  static constexpr int kRows = (T::rows + kWaves - 1) / kWaves;       // staged rows per wave
+  static constexpr int kRuns = (T::cols + 63) / 64;                   // 64-column runs per row
+  static constexpr int kVRows = (T::R * 3 + kWaves - 1) / kWaves;     // variance rows per wave
+  static constexpr int kVRuns = (T::vcols + 63) / 64;
+  float4 c[kRows][kRuns], g[kRows][kRuns];
+  float v[kVRows][kVRuns];
 };
 
 template <int S>
 __device__ __forceinline__ void tile_load(TileRegs<S>& r, const float4* __restrict__ in,
                                           const float4* __restrict__ guides, const float* __restrict__ var_in, int w,
-                                          int h, int x0, int y0) {
+                                          int h, int x0, int y0, int wave, int lane) {
   using T = AtrousTile<S>;
+  using TR = TileRegs<S>;
 #pragma unroll
-  for (int k = 0; k < TileRegs<S>::kPix; ++k) {
-    const int e = int(threadIdx.x) + k * kWaves * 64;
-    const int j = e / T::cols, cc = e % T::cols;
-    const int gy = y0 + (j - 2) * S, gx = x0 - 2 * S + cc;
-    float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g = c;
-    if (e < T::rows * T::cols && gx >= 0 && gx < w && gy >= 0 && gy < h) {
-      const size_t q = size_t(gy) * w + gx;
-      c = in[q];
-      g = guides[q];
+  for (int k = 0; k < TR::kRows; ++k) {
+    const int j = wave + k * kWaves;
+    const int gy = y0 + (j - 2) * S;
+    const bool row_ok = j < T::rows && gy >= 0 && gy < h;
+    const size_t row = size_t(row_ok ? gy : 0) * size_t(w);
+    const float4* crow = in + row;
+    const float4* grow = guides + row;
+#pragma unroll
+    for (int m = 0; m < TR::kRuns; ++m) {
+      const int cc = lane + 64 * m;
+      const int gx = x0 - 2 * S + cc;
+      float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g = c;
+      if (row_ok && cc < T::cols && uint32_t(gx) < uint32_t(w)) {
+        c = crow[gx];
+        g = grow[gx];
+      }
+      r.c[k][m] = c;
+      r.g[k][m] = g;
     }
-    r.c[k] = c;
-    r.g[k] = g;
   }
 #pragma unroll
-  for (int k = 0; k < TileRegs<S>::kVar; ++k) {
-    const int e = int(threadIdx.x) + k * kWaves * 64;
-    const int j = e / T::vcols, cc = e % T::vcols;
-    const int gy = y0 + (j / 3) * S + (j % 3) - 1, gx = x0 - 1 + cc;
-    r.v[k] = e < T::R * 3 * T::vcols && gx >= 0 && gx < w && gy >= 0 && gy < h ? var_in[size_t(gy) * w + gx] : 0.0f;
+  for (int k = 0; k < TR::kVRows; ++k) {
+    const int j = wave + k * kWaves;
+    const int gy = y0 + (j / 3) * S + (j % 3) - 1;
+    const bool row_ok = j < T::R * 3 && gy >= 0 && gy < h;
+    const float* vrow = var_in + size_t(row_ok ? gy : 0) * size_t(w);
+#pragma unroll
+    for (int m = 0; m < TR::kVRuns; ++m) {
+      const int cc = lane + 64 * m;
+      const int gx = x0 - 1 + cc;
+      r.v[k][m] = row_ok && cc < T::vcols && uint32_t(gx) < uint32_t(w) ? vrow[gx] : 0.0f;
+    }
   }
 }
 
 template <int S>
-__device__ __forceinline__ void tile_store(const TileRegs<S>& r, float4* sc, float4* sg, float* sl, float* sv) {
+__device__ __forceinline__ void tile_store(const TileRegs<S>& r, float4* sc, float4* sg, float* sl, float* sv,
+                                           int wave, int lane) {
   using T = AtrousTile<S>;
+  using TR = TileRegs<S>;
 #pragma unroll
-  for (int k = 0; k < TileRegs<S>::kPix; ++k) {
-    const int e = int(threadIdx.x) + k * kWaves * 64;
-    if (e < T::rows * T::cols) {
-      sc[e] = r.c[k];
-      sg[e] = r.g[k];
-      sl[e] = lum3(r.c[k].x, r.c[k].y, r.c[k].z);
+  for (int k = 0; k < TR::kRows; ++k) {
+    const int j = wave + k * kWaves;
+#pragma unroll
+    for (int m = 0; m < TR::kRuns; ++m) {
+      const int cc = lane + 64 * m;
+      if (j < T::rows && cc < T::cols) {
+        const int e = j * T::cols + cc;
+        sc[e] = r.c[k][m];
+        sg[e] = r.g[k][m];
+        sl[e] = lum3(r.c[k][m].x, r.c[k][m].y, r.c[k][m].z);
+      }
     }
   }
 #pragma unroll
-  for (int k = 0; k < TileRegs<S>::kVar; ++k) {
-    const int e = int(threadIdx.x) + k * kWaves * 64;
-    if (e < T::R * 3 * T::vcols) sv[e] = r.v[k];
+  for (int k = 0; k < TR::kVRows; ++k) {
+    const int j = wave + k * kWaves;
+#pragma unroll
+    for (int m = 0; m < TR::kVRuns; ++m) {
+      const int cc = lane + 64 * m;
+      if (j < T::R * 3 && cc < T::vcols) sv[j * T::vcols + cc] = r.v[k][m];
+    }
   }
 }
 
@@ -393,7 +423,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(LUC
   __shared__ float4 sg[T::rows * T::cols];
   __shared__ float sl[T::rows * T::cols];  // luminance of the staged illumination
   __shared__ float sv[T::R * 3 * T::vcols];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // This block's tiles: XCD k (the blocks b = k mod 8, `mine` of them)
   // owns a contiguous run [first, end) of the order, in proportion to its
   // blocks, dealt to its blocks round robin.
@@ -411,8 +441,8 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(LUC
     tile_origin<S>(t, w, h, x0, y0);
     {
       TileRegs<S> regs;
-      tile_load<S>(regs, in, guides, var_in, w, h, x0, y0);
-      tile_store<S>(regs, sc, sg, sl, sv);
+      tile_load<S>(regs, in, guides, var_in, w, h, x0, y0, wave, lane);
+      tile_store<S>(regs, sc, sg, sl, sv, wave, lane);
     }
     __syncthreads();
     const int cx0 = x0, cy0 = y0;
