@@ -217,7 +217,10 @@ __device__ __forceinline__ void tap(Centre& c, float4 cq, float4 gq, float lq, f
   const float dot = c.nx * gq.x + c.ny * gq.y + c.nz * gq.z;
   float e;
   if constexpr (kNormalStop) {
-    e = sigma_n * __builtin_amdgcn_logf(fmaxf(0.0f, dot)) + log2k;
+    // max(0, n_p·n_q) is not taken: a negative dot gives a NaN exponent,
+    // and the clamp of the weight to [0, 1] (the exp's output modifier) maps
+    // NaN to 0, the weight that max(0, .)^sigma_n gives.
+    e = sigma_n * __builtin_amdgcn_logf(dot) + log2k;
   } else {
     const bool covered = gq.x != 0.0f || gq.y != 0.0f || gq.z != 0.0f;
     e = covered ? log2k : -INFINITY;
@@ -230,7 +233,9 @@ __device__ __forceinline__ void tap(Centre& c, float4 cq, float4 gq, float lq, f
   const float dl = fabsf(lq - c.lp);
   e -= plane * c.irl[reach_slot(OX, OY)];
   e -= dl * c.il;
-  const float wgt = __builtin_amdgcn_exp2f(e);
+  // A tap's weight is at most its kernel weight (<= 1/4), so the clamp
+  // changes no finite weight.
+  const float wgt = __builtin_amdgcn_fmed3f(__builtin_amdgcn_exp2f(e), 0.0f, 1.0f);
   c.sr += cq.x * wgt;
   c.sg += cq.y * wgt;
   c.sb += cq.z * wgt;
@@ -265,6 +270,17 @@ __device__ inline void passthrough(float4 cp, size_t i, float4* out, float* var_
 // (atrous.frag:43-54); v[dy+1][dx+1] are the neighbours' variances (zero
 // outside the image).
 __device__ inline float blur_variance(const float v[3][3], int x, int y, int w, int h) {
+  if (x > 0 && y > 0 && x + 1 < w && y + 1 < h) {
+    // Inside the image the weights sum to 1: the same sum as below, whose
+    // normalisation is then a multiplication by rcp(1) = 1.
+    constexpr float k[3] = {0.25f, 0.5f, 0.25f};
+    float var = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) var += v[dy][dx] * (k[dx] * k[dy]);
+    return var;
+  }
   const float kx[3] = {x > 0 ? 0.25f : 0.0f, 0.5f, x + 1 < w ? 0.25f : 0.0f};
   const float ky[3] = {y > 0 ? 0.25f : 0.0f, 0.5f, y + 1 < h ? 0.25f : 0.0f};
   float var = 0.0f;
